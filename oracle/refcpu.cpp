@@ -1,0 +1,1528 @@
+// refcpu.cpp — CPU oracle for Petrichor's rendering hot path.
+//
+// TEST INFRASTRUCTURE ONLY: a clean-room restatement of the reference renderer's
+// per-ray and per-pixel arithmetic, used by tests/ as the parity checker and by
+// bench.py as the timed CPU baseline. The product (petrichor_amd) never links it.
+//
+// Every function cites the reference file:line it restates (paths relative to
+// the reference's LibPetrichor/). Floating-point rules followed throughout:
+//  * float arithmetic in the reference's evaluation order, no contraction
+//    (built with -ffp-contract=off, no -ffast-math);
+//  * the reference's unqualified sin/cos/acos/atan2/sqrt calls on floats bind to
+//    the double C functions, so those are evaluated in double (pt_libm.h, a
+//    portable implementation checked against glibc) and rounded where the
+//    reference assigns to a float;
+//  * `sqrt` of a float that is immediately stored to a float equals sqrtf.
+#include "refcpu.h"
+
+#include "../petrichor_amd/csrc/pt_libm.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <stack>
+#include <thread>
+#include <vector>
+
+namespace oracle {
+
+// ---- constants: Core/Constants.h:8-16 ----------------------------------------
+static const float kEps = 1.0e-3f;
+static const float kInfinity = 0.9f * FLT_MAX;
+static const float kPi = 3.14159265358979f;
+static const float kInvPi = 1.0f / 3.14159265358979f;
+
+// ---- Vector3f: Math/Vector3f.h:128-357 ---------------------------------------
+struct V {
+    float x, y, z;
+};
+static inline V mk(float x, float y, float z) { V v = {x, y, z}; return v; }
+static inline V add(V a, V b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V sub(V a, V b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V neg(V a) { return mk(-a.x, -a.y, -a.z); }
+static inline V mul(V a, V b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+// Vector * float and float * Vector (Vector3f.h:279-282,325-328) are both v.c * c.
+static inline V scl(V a, float c) { return mk(a.x * c, a.y * c, a.z * c); }
+static inline V divs(V a, float c) { return mk(a.x / c, a.y / c, a.z / c); }
+static inline V divv(V a, V b) { return mk(a.x / b.x, a.y / b.y, a.z / b.z); }
+// Dot = (x*x' + y*y') + z*z' (Vector3f.h:128-132); Cross (Vector3f.h:134-140).
+static inline float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V cross(V a, V b)
+{
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline float sqlen(V a) { return dot(a, a); }
+// Length = sqrt(Dot) stored as float (Vector3f.h:170-173): correctly rounded sqrtf.
+static inline float length(V a) { return sqrtf(dot(a, a)); }
+static inline V normalized(V a) { return divs(a, length(a)); }
+// std::min(a, b) = (b < a) ? b : a ; std::max(a, b) = (a < b) ? b : a.
+static inline float smin(float a, float b) { return (b < a) ? b : a; }
+static inline float smax(float a, float b) { return (a < b) ? b : a; }
+static inline float min_elem(V a) { return smin(a.x, smin(a.y, a.z)); } // Vector3f.h:197-201
+static inline float at(V a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+// Reflected: this - 2*n*Dot(this, n) (Vector3f.h:203-208).
+static inline V reflected(V d, V n) { return sub(d, scl(scl(n, 2.0f), dot(d, n))); }
+// Refracted (Vector3f.h:210-234): discriminant = (dn^2 - 1) + eta^2; on success
+// (1/eta) * (d - n * (float)(-dn + sqrt(disc))) with the inner sum in double.
+static inline bool refracted(V d, V n, float eta, V* out)
+{
+    const float dn = dot(d, n);
+    const float disc = dn * dn - 1 + eta * eta;
+    if (disc < 0) return false;
+    const float k = (float)((double)(-dn) + pt_sqrt_d((double)disc));
+    *out = scl(sub(d, scl(n, k)), 1.0f / eta);
+    return true;
+}
+static inline float pow2(float x) { return x * x; }
+static inline float pow5(float x) { return x * (pow2(x) * pow2(x)); } // MathUtils.h:38-56
+static inline float fmodp(float x, float m) { return x - m * std::floor(x / m); } // MathUtils.h:30-35
+static inline float lerpf(float a, float b, float t) { return (1.0f - t) * a + t * b; } // MathUtils.h:24-29
+static inline bool approx_eq(float a, float b, float eps) { return std::fabs(a - b) < eps; } // MathUtils.h:12-16
+
+// (float)(double(a) * cos_d(b)) etc.: a float times a double libm result, rounded once.
+static inline float sin_f(float x) { return (float)pt_sin((double)x); }
+static inline float cos_f(float x) { return (float)pt_cos((double)x); }
+
+// ---- XorShift128 / samplers: Random/XorShift.h:17-58, Core/Sampler -------------
+struct XorShift {
+    uint32_t x = 123456789u, y = 362436069u, z = 521288629u, w = 486213789u;
+    explicit XorShift(uint32_t seed) : w(seed) {}
+    uint32_t random()
+    {
+        uint32_t t = x ^ (x << 11);
+        x = y;
+        y = z;
+        z = w;
+        return w = (w ^ (w >> 19)) ^ (t ^ (t >> 8));
+    }
+    // random() / float(UINT_MAX): UINT_MAX converts to 2^32 exactly.
+    float next() { return (float)random() / 4294967296.0f; }
+};
+struct Sampler1D { // RandomSampler1D.h:11-25
+    XorShift s;
+    explicit Sampler1D(uint32_t seed) : s(seed) {}
+    float Next() { return s.next(); }
+};
+struct Sampler2D { // RandomSampler2D.cpp:8-21 (x stream first, then y stream)
+    XorShift s0, s1;
+    int64_t draws = 0;
+    Sampler2D(uint32_t a, uint32_t b) : s0(a), s1(b) {}
+    void Next(float* a, float* b)
+    {
+        *a = s0.next();
+        *b = s1.next();
+        ++draws;
+    }
+};
+
+// Per-pixel seeding defined by this build: the reference seeds one sampler pair per
+// tile and never advances the tile index (Core/Petrichor.cpp:72-77), so any
+// pixel-parallel renderer needs its own seeds. MurmurHash3's 32-bit finaliser
+// (fmix32) spreads (job seed, pixel, stream) into XorShift128's w word.
+static inline uint32_t fmix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+static inline uint32_t pixel_seed(uint32_t job_seed, uint32_t pixel, uint32_t stream)
+{
+    return fmix32(fmix32(fmix32(job_seed) ^ pixel) + 0x9e3779b9u * (stream + 1u));
+}
+
+// ---- Ray: Core/Ray.h:20-49 ---------------------------------------------------
+struct Ray {
+    V o, dir;
+    V throughput = {1.0f, 1.0f, 1.0f};
+    int bounce = 0;
+    float prob = 1.0f;
+    float ior = 1.0f;
+};
+
+// ---- AABB: Core/Accel/AABB.h:11-81, AABB.cpp:11-52 ----------------------------
+struct AABB {
+    V lower = {FLT_MAX, FLT_MAX, FLT_MAX};
+    V upper = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    void merge(const AABB& o)
+    {
+        lower.x = smin(lower.x, o.lower.x);
+        lower.y = smin(lower.y, o.lower.y);
+        lower.z = smin(lower.z, o.lower.z);
+        upper.x = smax(upper.x, o.upper.x);
+        upper.y = smax(upper.y, o.upper.y);
+        upper.z = smax(upper.z, o.upper.z);
+    }
+    void merge(V p)
+    {
+        lower.x = smin(lower.x, p.x);
+        lower.y = smin(lower.y, p.y);
+        lower.z = smin(lower.z, p.z);
+        upper.x = smax(upper.x, p.x);
+        upper.y = smax(upper.y, p.y);
+        upper.z = smax(upper.z, p.z);
+    }
+    bool contains(V p) const
+    {
+        return lower.x < p.x && p.x < upper.x && lower.y < p.y && p.y < upper.y && lower.z < p.z &&
+               p.z < upper.z;
+    }
+    V centroid() const { return scl(add(lower, upper), 0.5f); }
+    float area() const
+    {
+        const V d = sub(upper, lower);
+        return 2.0f * (d.x * d.y + d.y * d.z + d.z * d.x);
+    }
+    int widest_axis() const
+    {
+        float widest = 0.0f;
+        int axis = 0;
+        for (int a = 0; a < 3; ++a) {
+            const float w = at(upper, a) - at(lower, a);
+            if (w > widest) {
+                widest = w;
+                axis = a;
+            }
+        }
+        return axis;
+    }
+    const V& operator[](int i) const { return i ? upper : lower; }
+};
+
+// ---- Materials: Core/Material/*.cpp --------------------------------------------
+enum MatType { kLambert = 0, kGlass = 1, kEmission = 2, kGlossy = 3, kMix = 4 };
+struct Material {
+    int type = kLambert;
+    V color = {1.0f, 1.0f, 1.0f}; // Lambert kd, Glass colour, Emission Le, GGX reflectance (f0)
+    float alpha = 0.0f;           // GGX: roughness^2 (GGX.cpp:16-20)
+    float ior = 1.333f;           // Glass (Glass.h:16,59)
+    float mix = 0.0f;             // Mix weight
+    int mat0 = -1, mat1 = -1;     // Mix sub-materials
+    bool is_enabled = true;       // MaterialBase::m_isImportanceSamplingEnabled
+};
+
+static Material make_material(int type, const float* p)
+{
+    Material m;
+    m.type = type;
+    m.color = mk(p[0], p[1], p[2]);
+    if (type == kGlossy) m.alpha = p[3] * p[3];
+    if (type == kGlass) m.ior = p[3];
+    if (type == kMix) {
+        m.mix = p[3];
+        m.mat0 = (int)p[4];
+        m.mat1 = (int)p[5];
+    }
+    m.is_enabled = p[6] != 0.0f;
+    return m;
+}
+
+struct Shading { // Core/HitInfo.h:20-27 (tangent and uv are not read by untextured materials)
+    V normal, pos;
+    int material = -1;
+};
+
+// GGX::GetF0 (GGX.h:126-139) with metalness 0, base colour 1, no reflectance map:
+// 0.16*r^2*(1-0) + 1*0, evaluated as written.
+static inline V ggx_f0(const Material& m)
+{
+    const V r2 = mul(m.color, m.color);
+    const V a = scl(scl(r2, 0.16f), 1.0f - 0.0f);
+    return add(a, scl(mk(1.0f, 1.0f, 1.0f), 0.0f));
+}
+
+// GGX::Lambda (GGX.cpp:141-153): -0.5f + 0.5f*sqrt(1 + a2*tan2) evaluated in double.
+static inline float ggx_lambda(V dir, V half, float alpha)
+{
+    const float alpha2 = alpha * alpha;
+    const float c = std::fabs(dot(dir, half));
+    const float tan2 = c > 0 ? 1.0f / (c * c) - 1.0f : kInfinity;
+    const float arg = 1.0f + alpha2 * tan2;
+    return (float)((double)-0.5f + (double)0.5f * pt_sqrt_d((double)arg));
+}
+
+// Material::BxDF (Lambert.cpp:27-33, GGX.cpp:22-57, Glass.cpp:11-19).
+static V bxdf(const Material& m, V din, V dout, const Shading& si)
+{
+    switch (m.type) {
+    case kLambert:
+        return scl(mul(m.color, mk(1.0f, 1.0f, 1.0f)), kInvPi);
+    case kGlossy: {
+        const V normal = si.normal;
+        const V half = normalized(add(neg(din), dout));
+        const float hn = std::fabs(dot(half, normal));
+        const float alpha = m.alpha;
+        const float alpha2 = alpha * alpha;
+        const float hn2 = hn * hn;
+        const float k = (alpha2 - 1.0f) * hn2 + 1.0f;
+        const float d_term = alpha2 / (kPi * k * k);
+        const float li = ggx_lambda(dout, half, alpha);
+        const float lo = ggx_lambda(neg(din), half, alpha);
+        const float g_term = 1.0f / (1.0f + li + lo);
+        const float hl = std::fabs(dot(half, dout));
+        const V f0 = ggx_f0(m);
+        const V f_term = add(f0, scl(sub(mk(1.0f, 1.0f, 1.0f), f0), pow5(1.0f - hl)));
+        const float ln = std::fabs(dot(dout, normal));
+        const float vn = std::fabs(dot(neg(din), normal));
+        return divs(scl(f_term, d_term * g_term), 4.0f * ln * vn);
+    }
+    default:
+        return mk(0.0f, 0.0f, 0.0f);
+    }
+}
+
+// Material::PDF (Lambert.cpp:35-49, GGX.cpp:59-76, Glass.cpp:21-27).
+static float pdf(const Material& m, V din, V dout, const Shading& si)
+{
+    switch (m.type) {
+    case kLambert:
+        if (m.is_enabled) {
+            const float c = std::fabs(dot(dout, si.normal));
+            return c * kInvPi;
+        }
+        return 0.5f * kInvPi;
+    case kGlossy: {
+        const V half = normalized(add(neg(din), dout));
+        const float hn = std::fabs(dot(half, si.normal));
+        const float alpha = m.alpha;
+        const float alpha2 = alpha * alpha;
+        const float hn2 = hn * hn;
+        const float tmp = 1.0f - (1.0f - alpha2) * hn2;
+        return tmp > 0 ? kInvPi * alpha2 / (tmp * tmp) : kInfinity;
+    }
+    default:
+        return 0.0f;
+    }
+}
+
+// GGX::SampleGGXVNDF (GGX.cpp:155-208), Heitz 2017 visible-normal sampling.
+static V ggx_sample_vndf(const Material& m, V view, const Shading& si, Sampler2D& s2)
+{
+    const V n0 = si.normal;
+    const float hit_sign = dot(view, n0);
+    const V normal = scl(n0, std::copysign(1.0f, hit_sign));
+    const V t = (std::fabs(normal.z) < 0.9999f) ? cross(normal, mk(0.0f, 0.0f, 1.0f))
+                                                 : mk(1.0f, 0.0f, 0.0f);
+    // OrthonormalBasis::Build(w, u): w, u normalised, v = w x u (OrthonormalBasis.cpp:29-35)
+    const V bw = normalized(normal);
+    const V bu = normalized(t);
+    const V bv = cross(bw, bu);
+    // WorldToLocal (OrthonormalBasis.cpp:45-51)
+    V v = mk(dot(view, bu), dot(view, bv), dot(view, bw));
+    const float alpha = m.alpha;
+    v = mul(v, mk(alpha, alpha, 1.0f));
+    v = divs(v, length(v));
+    const V t1 = (v.z < 0.9999f) ? normalized(cross(v, mk(0.0f, 0.0f, 1.0f))) : mk(1.0f, 0.0f, 0.0f);
+    const V t2 = cross(t1, v);
+    float u0, u1;
+    s2.Next(&u0, &u1);
+    const float a = 1.0f / (1.0f + v.z);
+    const float r = sqrtf(u0);
+    const float phi = (u1 < a) ? u1 / a * kPi : kPi + (u1 - a) / (1.0f - a) * kPi;
+    double sp, cp;
+    pt_sincos((double)phi, &sp, &cp);
+    const float p1 = (float)((double)r * cp);
+    const float p2 = (float)((double)r * sp * (double)((u1 < a) ? 1.0f : v.z));
+    const float s = sqrtf(smax(0.0f, 1.0f - p1 * p1 - p2 * p2));
+    V n = add(add(scl(t1, p1), scl(t2, p2)), scl(v, s));
+    n = mk(alpha * n.x, alpha * n.y, smax(0.0f, n.z));
+    // LocalToWorld (OrthonormalBasis.cpp:53-60): (u*x + v*y) + w*z
+    n = add(add(scl(bu, n.x), scl(bv, n.y)), scl(bw, n.z));
+    n = divs(n, length(n));
+    return n;
+}
+
+// Material::CreateNextRay (Lambert.cpp:51-102, GGX.cpp:78-139, Glass.cpp:29-95).
+static Ray next_ray(const Material& m, const Ray& in, const Shading& si, Sampler2D& s2)
+{
+    Ray out;
+    switch (m.type) {
+    case kLambert: {
+        const float hit_sign = -dot(in.dir, si.normal);
+        const V normal = scl(si.normal, std::copysign(1.0f, hit_sign));
+        // OrthonormalBasis::Build(w) (OrthonormalBasis.cpp:10-27), Duff et al. 2017.
+        const V w = normalized(normal);
+        const float sign = std::copysign(1.0f, w.z);
+        const float a = -1.0f / (sign + w.z);
+        const float b = w.x * w.y * a;
+        const V u = mk(1.0f + sign * w.x * w.x * a, sign * b, -sign * w.x);
+        const V v = mk(b, sign + w.y * w.y * a, -w.y);
+        float r0, r1;
+        s2.Next(&r0, &r1);
+        V dir;
+        float cos_theta = 0.0f;
+        if (m.is_enabled) {
+            const float phi = 2.0f * kPi * r1;
+            const float sin_theta = sqrtf(r0);
+            double sp, cp;
+            pt_sincos((double)phi, &sp, &cp);
+            const float cu = (float)((double)sin_theta * cp);
+            const float cv = (float)((double)sin_theta * sp);
+            const float cw = sqrtf(1.0f - sin_theta * sin_theta);
+            dir = add(add(scl(u, cu), scl(v, cv)), scl(w, cw));
+        } else {
+            cos_theta = r0;
+            const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+            const float phi = 2.0f * kPi * r1;
+            double sp, cp;
+            pt_sincos((double)phi, &sp, &cp);
+            const float cu = (float)((double)sin_theta * cp);
+            const float cv = (float)((double)sin_theta * sp);
+            dir = add(add(scl(u, cu), scl(v, cv)), scl(w, cos_theta));
+        }
+        out.o = add(si.pos, scl(normal, kEps));
+        out.dir = dir;
+        out.bounce = in.bounce + 1;
+        if (m.is_enabled) {
+            // throughput * GetAlbedo(): 1 for an untextured Lambert (Lambert.cpp:117-126)
+            out.throughput = mul(in.throughput, mk(1.0f, 1.0f, 1.0f));
+        } else {
+            out.throughput = in.throughput;
+            const V f = bxdf(m, in.dir, out.dir, si);
+            out.throughput = mul(out.throughput, divs(scl(f, cos_theta), pdf(m, in.dir, out.dir, si)));
+        }
+        return out;
+    }
+    case kGlossy: {
+        const V n0 = si.normal;
+        const float hit_sign = -dot(in.dir, n0);
+        const V normal = scl(n0, std::copysign(1.0f, hit_sign));
+        const V h = ggx_sample_vndf(m, neg(in.dir), si, s2);
+        const V dir = reflected(in.dir, h);
+        const float hl = std::fabs(dot(h, dir));
+        const V f0 = ggx_f0(m);
+        const V f_term = add(f0, scl(sub(mk(1.0f, 1.0f, 1.0f), f0), pow5(1.0f - hl)));
+        out.o = si.pos;
+        out.dir = dir;
+        out.throughput = in.throughput;
+        out.bounce = in.bounce + 1;
+        const float alpha = m.alpha;
+        const float li = ggx_lambda(dir, h, alpha);
+        const float lo = ggx_lambda(neg(in.dir), h, alpha);
+        const float g2 = 1.0f / (1.0f + li + lo);
+        const float g1 = 1.0f / (1.0f + li);
+        out.throughput = mul(out.throughput, divs(scl(f_term, g2), g1));
+        out.o = add(out.o, scl(normal, kEps));
+        return out;
+    }
+    case kGlass: {
+        const float hit_sign = -dot(in.dir, si.normal);
+        const V normal = scl(si.normal, std::copysign(1.0f, hit_sign));
+        const float eta = m.ior / in.ior;
+        V rd;
+        if (!refracted(in.dir, si.normal, eta, &rd)) {
+            // total internal reflection: zero throughput, bounce restarts at 1
+            out.o = add(si.pos, scl(normal, kEps));
+            out.dir = normalized(reflected(in.dir, normal));
+            out.throughput = mk(0.0f, 0.0f, 0.0f);
+            out.bounce = 0 + 1;
+            out.prob = in.prob;
+            out.ior = in.ior;
+            return out;
+        }
+        out.dir = normalized(rd);
+        const float ndo = dot(si.normal, out.dir);
+        const float f0 = pow2((1.0f - eta) / (1.0f + eta));
+        const float refl = f0 + (1.0f - f0) * pow5(1.0f - std::fabs(ndo));
+        float rx, ry;
+        s2.Next(&rx, &ry);
+        if (rx <= refl) {
+            // "reflection" keeps the refracted direction (Glass.cpp:70-80)
+            out.o = add(si.pos, scl(normal, kEps));
+            out.throughput = in.throughput;
+            out.bounce = in.bounce + 1;
+            out.prob = in.prob;
+            out.ior = in.ior;
+            return out;
+        }
+        const bool enter = (ndo < 0);
+        out.o = sub(si.pos, scl(normal, kEps));
+        out.throughput = in.throughput;
+        out.bounce = in.bounce + 1;
+        out.prob = in.prob;
+        out.ior = enter ? 1.0f * eta : 1.0f / eta;
+        return out;
+    }
+    default:
+        return in;
+    }
+}
+
+// ---- Texture2D fetch: Core/Texture2D.cpp:260-335 --------------------------------
+struct Texture {
+    int w = 0, h = 0;
+    bool loaded = false;
+    std::vector<V> px;
+    bool valid() const { return loaded && w > 0 && h > 0; }
+    V get(int i, int j) const
+    {
+        if (!valid()) return mk(1.0f, 0.0f, 1.0f);
+        return px[(size_t)w * j + i];
+    }
+    static int clampi(int v, int lo, int hi) { return v < lo ? lo : (hi < v ? hi : v); }
+    V bilinear(float x, float y) const
+    {
+        int ix0 = (int)x;
+        int ix1 = (int)(x + 1.0f);
+        int iy0 = (int)y;
+        int iy1 = (int)(y + 1.0f);
+        const float wx0 = x - ix0;
+        const float wy0 = y - iy0;
+        const float wx1 = ix1 - x;
+        const float wy1 = iy1 - y;
+        ix0 = clampi(ix0, 0, w - 1);
+        ix1 = clampi(ix1, 0, w - 1);
+        iy0 = clampi(iy0, 0, h - 1);
+        iy1 = clampi(iy1, 0, h - 1);
+        const V c00 = get(ix0, iy0), c10 = get(ix1, iy0), c01 = get(ix0, iy1), c11 = get(ix1, iy1);
+        const V cm0 = add(scl(c00, wx1), scl(c10, wx0));
+        const V cm1 = add(scl(c01, wx1), scl(c11, wx0));
+        return add(scl(cm0, wy1), scl(cm1, wy0));
+    }
+    V by_uv(float u, float v) const
+    {
+        u = fmodp(u, 1.0f);
+        v = fmodp(v, 1.0f);
+        return bilinear(u * w, v * h);
+    }
+};
+
+// ---- Environment: Core/Environment.cpp ----------------------------------------
+struct Environment {
+    Texture tex;          // m_texEnv
+    Texture pdf2d, cdf2d; // built with Texture2D(w, h): never "loaded" (Texture2D.cpp:100-105)
+    std::vector<float> pdf1d, cdf1d;
+    V base = {0.0f, 0.0f, 0.0f};
+    float rot = 0.0f;
+    bool use_is = true;
+
+    bool use_env_is() const { return tex.valid() && use_is; } // Environment.h:45-49
+
+    // Environment::GetColor (Environment.cpp:29-48)
+    V color(V dir) const
+    {
+        if (!tex.valid()) return base;
+        const float theta = (float)pt_acos((double)dir.z);
+        const float phi = (float)pt_atan2((double)dir.y, (double)dir.x);
+        float u = 1.0f - (phi - rot) * 0.5f * kInvPi;
+        float v = theta * kInvPi;
+        u = fmodp(u, 1.0f);
+        v = fmodp(v, 1.0f);
+        return mul(base, tex.by_uv(u, v));
+    }
+
+    // Environment::PreCalcCumulativeDistTex (Environment.cpp:56-128). The 2D tables are
+    // invalid textures, so every read returns (1,0,1); the 1D tables follow from that.
+    void precalc()
+    {
+        if (!use_env_is()) return;
+        pdf2d = Texture();
+        pdf2d.w = tex.w;
+        pdf2d.h = tex.h;
+        cdf2d = pdf2d;
+        pdf1d.clear();
+        cdf1d.clear();
+        float sum = 0.0f;
+        const int imax = cdf2d.w - 1;
+        for (int j = 0; j < cdf2d.h; j++) {
+            const float lum = cdf2d.get(imax, j).x;
+            pdf1d.push_back(lum);
+            sum += lum;
+            cdf1d.push_back(sum);
+        }
+        for (int j = 0; j < cdf2d.h; j++) {
+            pdf1d[j] *= ((float)cdf2d.h / sum);
+            cdf1d[j] /= sum;
+        }
+    }
+
+    // Out-of-range reads of the 1D tables happen in the reference only for dir.z == -1
+    // exactly (Environment.cpp:237-240, reading index H); every entry of pdf1d equals
+    // the last one, so that value stands in.
+    float pdf1(int i) const { return pdf1d[std::min(i, (int)pdf1d.size() - 1)]; }
+
+    // Environment::ImportanceSampling (Environment.cpp:130-220)
+    V sample(Sampler2D& s2, float* pdf_xy) const
+    {
+        if (!use_env_is()) return mk(1.0f, 0.0f, 0.0f);
+        const float texel_w = 1.0f / pdf2d.w;
+        const float texel_h = 1.0f / pdf2d.h;
+        float r0, r1;
+        s2.Next(&r0, &r1);
+        float v0 = 0.0f, v1 = 1.0f;
+        for (;;) {
+            const float v = 0.5f * (v0 + v1);
+            const float y = v * cdf2d.h;
+            const int fy = (int)y;
+            const int cy = std::min(fy + 1, cdf2d.h - 1);
+            const float c = lerpf(cdf1d[fy], cdf1d[cy], y - fy);
+            const float diff = c - r0;
+            if (v1 - v0 <= texel_h) {
+                v0 = v1 = v;
+                break;
+            }
+            if (diff > 0) v1 = v;
+            else if (diff < 0) v0 = v;
+            else {
+                v0 = v1 = v;
+                break;
+            }
+        }
+        float u0 = 0.0f, u1 = 1.0f;
+        for (;;) {
+            const float u = 0.5f * (u0 + u1);
+            const float c = cdf2d.by_uv(u, v0).x;
+            const float diff = c - r1;
+            if (u1 - u0 <= texel_w) {
+                u0 = u1 = u;
+                break;
+            }
+            if (diff >= 0) u1 = u;
+            else u0 = u;
+        }
+        if (pdf_xy) {
+            const float y = v0 * pdf2d.h;
+            const int y0 = (int)y;
+            const int y1 = std::min(y0 + 1, pdf2d.h - 1);
+            const float py = lerpf(pdf1(y0), pdf1(y1), y - y0);
+            const float px = pdf2d.by_uv(u0, v0).x;
+            *pdf_xy = py * px;
+        }
+        const float theta = v0 * kPi;
+        const float phi = 2.0f * kPi * (1.0f - u0) + rot;
+        // OrthonormalBasis::Build(UnitZ, UnitX) then GetDir (OrthonormalBasis.cpp:29-43)
+        const V bw = normalized(mk(0.0f, 0.0f, 1.0f));
+        const V bu = normalized(mk(1.0f, 0.0f, 0.0f));
+        const V bv = cross(bw, bu);
+        double st, ct, sp, cp;
+        pt_sincos((double)theta, &st, &ct);
+        pt_sincos((double)phi, &sp, &cp);
+        const float cx = (float)(st * cp);
+        const float cy = (float)(st * sp);
+        const float cz = (float)ct;
+        return add(add(scl(bu, cx), scl(bv, cy)), scl(bw, cz));
+    }
+
+    // Environment::GetImportanceSamplingPDF (Environment.cpp:222-244)
+    float pdf(V dir) const
+    {
+        if (!use_env_is()) return 1.0f;
+        const float theta = (float)pt_acos((double)dir.z);
+        const float phi = (float)pt_atan2((double)dir.y, (double)dir.x);
+        const float u0 = 1.0f - (phi - rot) * 0.5f * kInvPi;
+        const float v0 = theta * kInvPi;
+        const float y = v0 * pdf2d.h;
+        const int y0 = (int)y;
+        const int y1 = std::min(y0 + 1, pdf2d.h - 1);
+        const float py = lerpf(pdf1(y0), pdf1(y1), y - y0);
+        const float px = pdf2d.by_uv(u0, v0).x;
+        return py * px;
+    }
+};
+
+// ---- Camera: Core/Camera.cpp:52-91 --------------------------------------------
+struct Camera {
+    V pos, right, up, forward;
+    float focus = 1.0f, h_perf = 24e-3f / 55e-3f, aperture = 25e-3f;
+};
+
+static Ray generate_ray(const Camera& c, int i, int j, int w, int h, Sampler2D& s2)
+{
+    float r0, r1;
+    s2.Next(&r0, &r1);
+    const float aspect = (float)w / h;
+    const float u = (i + r0) / w - 0.5f;
+    const float v = (j + r1) / h - 0.5f;
+    const V a = scl(scl(scl(scl(c.right, u), c.h_perf), c.focus), aspect);
+    const V b = scl(scl(scl(neg(c.up), v), c.h_perf), c.focus);
+    const V sensor = add(add(add(c.pos, a), b), scl(c.forward, c.focus));
+    s2.Next(&r0, &r1);
+    const float r = sqrtf(r0);
+    const float theta = 2.0f * kPi * r1;
+    double st, ct;
+    pt_sincos((double)theta, &st, &ct);
+    const V la = scl(scl(scl(scl(c.right, 0.5f), c.aperture), r), (float)ct);
+    const V lb = scl(scl(scl(scl(neg(c.up), 0.5f), c.aperture), r), (float)st);
+    const V lens = add(add(c.pos, la), lb);
+    Ray ray;
+    ray.o = lens;
+    ray.dir = normalized(sub(sensor, lens));
+    const float d = dot(ray.dir, c.forward);
+    const V t = divs(scl(scl(scl(scl(mk(1.0f, 1.0f, 1.0f), d), d), c.focus), c.focus),
+                     sqlen(sub(sensor, lens)));
+    ray.throughput = t;
+    return ray;
+}
+
+// ---- Scene + triangles: Core/Geometry/Triangle.cpp -----------------------------
+struct Tri {
+    V p0, p1, p2;
+    V n0, n1, n2;
+    bool smooth = false;
+    int material = 0;
+};
+
+struct Scene {
+    std::vector<Tri> tris;
+    std::vector<Material> mats;
+    std::vector<int> lights;
+    Camera cam;
+    Environment env;
+
+    // GeometryBase::GetMaterial (GeometryBase.h:59-69), MixMaterial::GetSingleMaterial
+    // (MixMaterial.h:45-49)
+    int material_of(int prim, float rand) const
+    {
+        const int m = tris[prim].material;
+        if (mats[m].type == kMix) return mats[m].mix <= rand ? mats[m].mat0 : mats[m].mat1;
+        return m;
+    }
+};
+
+// Triangle::Intersect (Triangle.cpp:51-88)
+static inline bool tri_intersect(const Tri& tr, const Ray& ray, float* dist_out)
+{
+    const V e1 = sub(tr.p1, tr.p0);
+    const V e2 = sub(tr.p2, tr.p0);
+    const V ce = cross(e1, e2);
+    const float inv_det = 1.0f / dot(neg(ray.dir), ce);
+    const V vec = sub(ray.o, tr.p0);
+    const float w1 = dot(vec, cross(ray.dir, e2)) * inv_det;
+    const float w2 = dot(vec, cross(e1, ray.dir)) * inv_det;
+    if (w1 < 0.0f || w1 >= 1.0f) return false;
+    if (w2 < 0.0f || w2 >= 1.0f) return false;
+    if (w1 + w2 >= 1.0f) return false;
+    const float dist = dot(vec, ce) * inv_det;
+    if (dist < 0.0f) return false;
+    *dist_out = dist;
+    return true;
+}
+
+// Triangle::Interpolate (Triangle.cpp:90-153): position, shading normal, material.
+static Shading interpolate(const Scene& sc, int prim, const Ray& ray, float t)
+{
+    const Tri& tr = sc.tris[prim];
+    Shading s;
+    s.pos = add(ray.o, scl(ray.dir, t));
+    const V e1 = sub(tr.p1, tr.p0);
+    const V e2 = sub(tr.p2, tr.p0);
+    const V ce = cross(e1, e2);
+    if (!tr.smooth) {
+        s.normal = normalized(ce);
+    } else {
+        const float inv_det = 1.0f / dot(neg(ray.dir), ce);
+        const V vec = sub(ray.o, tr.p0);
+        const float w1 = dot(vec, cross(ray.dir, e2)) * inv_det;
+        const float w2 = dot(vec, cross(e1, ray.dir)) * inv_det;
+        V n = add(add(scl(sub(tr.n1, tr.n0), w1), scl(sub(tr.n2, tr.n0), w2)), tr.n0);
+        s.normal = divs(n, length(n));
+    }
+    s.material = sc.material_of(prim, 0.0f);
+    return s;
+}
+
+// Triangle::SampleSurface (Triangle.cpp:155-182)
+static void tri_sample(const Tri& tr, Sampler2D& s2, V* pos, V* normal, float* pdf_area)
+{
+    const V e0 = sub(tr.p1, tr.p0);
+    const V e1 = sub(tr.p2, tr.p0);
+    float u0, u1;
+    s2.Next(&u0, &u1);
+    const float su = sqrtf(u0);
+    *pos = add(add(tr.p0, scl(e0, 1.0f - su)), scl(e1, su * u1));
+    *normal = normalized(cross(e0, e1));
+    *pdf_area = 1.0f / (0.5f * length(cross(e0, e1)));
+}
+
+// ---- BinnedSAHBVH: Core/Accel/BinnedSAHBVH.cpp ---------------------------------
+struct Node {
+    AABB box;
+    int child[2] = {0, 0};
+    int begin = 0, end = 0;
+    bool leaf = false;
+};
+
+struct BVH {
+    std::vector<Node> nodes;
+    std::vector<int> ids;
+    int max_depth = 0;
+};
+
+// BinnedSAHBVH::Build (BinnedSAHBVH.cpp:15-225) with GetSAHCost (:227-270). The
+// per-k cost pass of the reference is restated with per-bin boxes: min/max merges are
+// exact and order-free, so the left/right boxes and counts are identical.
+static void build_bvh(const Scene& sc, int num_bins, BVH* bvh)
+{
+    const int n = (int)sc.tris.size();
+    std::vector<AABB> pbox(n);
+    std::vector<V> cen(n);
+    std::vector<int> bin(n, -1);
+    for (int i = 0; i < n; ++i) {
+        const Tri& t = sc.tris[i];
+        AABB b;
+        b.merge(t.p0);
+        b.merge(t.p1);
+        b.merge(t.p2);
+        pbox[i] = b;
+        cen[i] = divs(add(add(t.p0, t.p1), t.p2), 3.0f); // Triangle.h:45-50
+    }
+    bvh->nodes.clear();
+    bvh->nodes.reserve(2 * (size_t)n + 1);
+    AABB root;
+    for (int i = 0; i < n; ++i) root.merge(pbox[i]);
+    Node r;
+    r.box = root;
+    r.begin = 0;
+    r.end = n;
+    bvh->nodes.push_back(r);
+    std::vector<int> depth_of(1, 1);
+    bvh->ids.resize(n);
+    std::iota(bvh->ids.begin(), bvh->ids.end(), 0);
+    std::stack<uint32_t> st;
+    st.push(0);
+    int max_depth = 1;
+    std::vector<AABB> bbox(num_bins);
+    std::vector<int> bcnt(num_bins);
+    while (!st.empty()) {
+        const uint32_t ni = st.top();
+        st.pop();
+        const int begin = bvh->nodes[ni].begin;
+        const int end = bvh->nodes[ni].end;
+        const int cnt = end - begin;
+        max_depth = std::max(max_depth, depth_of[ni]);
+        if (cnt <= 4) {
+            bvh->nodes[ni].leaf = true;
+            continue;
+        }
+        int* ib = bvh->ids.data() + begin;
+        int* ie = bvh->ids.data() + end;
+        AABB cb;
+        for (int* it = ib; it != ie; ++it) cb.merge(cen[*it]);
+        const int axis = cb.widest_axis();
+        std::sort(ib, ie, [&](int a, int b) { return at(cen[a], axis) < at(cen[b], axis); });
+        const float len = at(cb.upper, axis) - at(cb.lower, axis);
+        for (int k = 0; k < num_bins; ++k) {
+            bbox[k] = AABB();
+            bcnt[k] = 0;
+        }
+        bool any_below = false; // bins < 0 only arise from a NaN bin position (len == 0)
+        for (int* it = ib; it != ie; ++it) {
+            const float l = at(cen[*it], axis) - at(cb.lower, axis);
+            const float x = std::nextafter((float)num_bins * l / len, 0.0f);
+            // (int)NaN: x86-64 cvttss2si yields INT_MIN, the reference's platform.
+            const int b = (x != x) ? INT_MIN : (int)x;
+            bin[*it] = b;
+            if (b < 0) {
+                any_below = true;
+                continue;
+            }
+            bbox[b].merge(pbox[*it]);
+            bcnt[b]++;
+        }
+        int best_k = 0, best_left = 0;
+        if (!any_below) {
+            float min_cost = FLT_MAX;
+            // prefix boxes: left = bins [0, k), right = bins [k, num_bins)
+            std::vector<AABB> rbox(num_bins + 1);
+            std::vector<int> rcnt(num_bins + 1, 0);
+            for (int k = num_bins - 1; k >= 0; --k) {
+                rbox[k] = rbox[k + 1];
+                rbox[k].merge(bbox[k]);
+                rcnt[k] = rcnt[k + 1] + bcnt[k];
+            }
+            AABB lbox;
+            int lcnt = 0;
+            for (int k = 0; k < num_bins; ++k) {
+                if (k > 0) {
+                    lbox.merge(bbox[k - 1]);
+                    lcnt += bcnt[k - 1];
+                }
+                const int rc = rcnt[k];
+                const float sal = lcnt ? lbox.area() : 0;
+                const float sar = rc ? rbox[k].area() : 0;
+                const float cost = lcnt * sal + rc * sar;
+                if (cost < min_cost) {
+                    min_cost = cost;
+                    best_k = k;
+                    best_left = lcnt;
+                }
+            }
+        }
+        // With all bins INT_MIN every k puts everything left at equal cost: k = 0 wins.
+        if (best_k == 0) {
+            bvh->nodes[ni].leaf = true;
+            continue;
+        }
+        AABB lb, rb;
+        for (int* it = ib; it != ie; ++it) {
+            if (bin[*it] < best_k) lb.merge(pbox[*it]);
+            else rb.merge(pbox[*it]);
+        }
+        Node left, right;
+        left.box = lb;
+        left.child[0] = left.child[1] = -1;
+        left.begin = begin;
+        left.end = begin + best_left;
+        right.box = rb;
+        right.child[0] = right.child[1] = -1;
+        right.begin = begin + best_left;
+        right.end = end;
+        const int d = depth_of[ni] + 1;
+        bvh->nodes.push_back(left);
+        depth_of.push_back(d);
+        const int li = (int)bvh->nodes.size() - 1;
+        st.push(li);
+        bvh->nodes.push_back(right);
+        depth_of.push_back(d);
+        const int ri = (int)bvh->nodes.size() - 1;
+        st.push(ri);
+        bvh->nodes[ni].child[0] = li;
+        bvh->nodes[ni].child[1] = ri;
+    }
+    bvh->max_depth = max_depth;
+}
+
+// Slab test with the reference's distance rule (BinnedSAHBVH.h:95-170).
+static inline bool slab(const Ray& ray, const AABB& b, V inv, const int sign[3], float* dist)
+{
+    float tmin = (b[sign[0]].x - ray.o.x) * inv.x;
+    float tmax = (b[1 - sign[0]].x - ray.o.x) * inv.x;
+    const float tymin = (b[sign[1]].y - ray.o.y) * inv.y;
+    const float tymax = (b[1 - sign[1]].y - ray.o.y) * inv.y;
+    if (tymax < tmin || tmax < tymin) return false;
+    if (tmin < tymin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    const float tzmin = (b[sign[2]].z - ray.o.z) * inv.z;
+    const float tzmax = (b[1 - sign[2]].z - ray.o.z) * inv.z;
+    if (tzmax < tmin || tmax < tzmin) return false;
+    if (tmin < tzmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    float d = FLT_MAX;
+    const bool near_ok = (0.0f <= tmin && tmin < d);
+    if (near_ok) d = tmin;
+    const bool far_ok = (0.0f <= tmax && tmax < d);
+    if (far_ok) d = tmax;
+    if (!(near_ok || far_ok)) return false;
+    *dist = d;
+    return true;
+}
+
+// BinnedSAHBVH::Intersect (BinnedSAHBVH.cpp:272-374). Returns prim or -1.
+static int bvh_intersect(const Scene& sc, const BVH& bvh, const Ray& ray, float dmin, float dmax,
+                         float* t_out, int64_t* nodes_visited, int64_t* tris_tested)
+{
+    const V inv = divv(mk(1.0f, 1.0f, 1.0f), ray.dir);
+    const int sign[3] = {inv.x < 0.0f, inv.y < 0.0f, inv.z < 0.0f};
+    int stack[1024];
+    int sp = 0;
+    stack[sp++] = 0;
+    int best = -1;
+    float best_t = 0.0f;
+    int64_t nv = 0, nt = 0;
+    while (sp > 0) {
+        const int ni = stack[--sp];
+        const Node& node = bvh.nodes[ni];
+        ++nv;
+        float nd;
+        if (!slab(ray, node.box, inv, sign, &nd)) continue;
+        if (best >= 0 && node.box.contains(ray.o) == false) {
+            if (best_t < nd) continue;
+        }
+        if (node.leaf) {
+            for (int k = node.begin; k < node.end; ++k) {
+                const int prim = bvh.ids[k];
+                ++nt;
+                float t;
+                if (tri_intersect(sc.tris[prim], ray, &t)) {
+                    if (t < dmin || t > dmax) continue;
+                    if (best < 0 || t < best_t) {
+                        best = prim;
+                        best_t = t;
+                    }
+                }
+            }
+        } else {
+            const int l = node.child[0], r = node.child[1];
+            const float dl = sqlen(sub(bvh.nodes[l].box.centroid(), ray.o));
+            const float dr = sqlen(sub(bvh.nodes[r].box.centroid(), ray.o));
+            if (sp + 2 > 1024) abort();
+            if (dl < dr) {
+                stack[sp++] = r;
+                stack[sp++] = l;
+            } else {
+                stack[sp++] = l;
+                stack[sp++] = r;
+            }
+        }
+    }
+    if (nodes_visited) *nodes_visited += nv;
+    if (tris_tested) *tris_tested += nt;
+    *t_out = best_t;
+    return best;
+}
+
+// BruteForce::Intersect (BruteForce.cpp:21-50)
+static int brute_intersect(const Scene& sc, const Ray& ray, float dmin, float dmax, float* t_out)
+{
+    int best = -1;
+    float best_t = 0.0f;
+    for (int i = 0; i < (int)sc.tris.size(); ++i) {
+        float t;
+        if (tri_intersect(sc.tris[i], ray, &t)) {
+            if (t < dmin || t > dmax) continue;
+            if (best < 0 || t < best_t) {
+                best = i;
+                best_t = t;
+            }
+        }
+    }
+    *t_out = best_t;
+    return best;
+}
+
+// ---- integrators ---------------------------------------------------------------
+struct Ctx {
+    const Scene* sc;
+    const BVH* bvh;
+    int64_t queries = 0;
+    int closest(const Ray& r, float dmin, float dmax, float* t)
+    {
+        ++queries;
+        return bvh_intersect(*sc, *bvh, r, dmin, dmax, t, nullptr, nullptr);
+    }
+};
+
+// PathTracing::SampleLight (PathTracing.cpp:329-386). Returns false for the env slot.
+static bool sample_light(Ctx& c, V origin, float rnd, Sampler2D& s2, float* pdf_area, bool want_env,
+                         V* pl_pos, V* pl_nrm)
+{
+    const Scene& sc = *c.sc;
+    const size_t L = sc.lights.size();
+    int index = 0;
+    if (!want_env) {
+        index = (int)(rnd * L);
+        index = std::min(index, (int)(L - 1));
+    } else {
+        index = (int)(rnd * (L + 1));
+        index = std::min(index, (int)L);
+        if ((size_t)index >= L) return false;
+    }
+    (void)origin;
+    float sum_area = 0.0f;
+    *pl_pos = mk(0.0f, 0.0f, 0.0f);
+    *pl_nrm = mk(0.0f, 0.0f, 0.0f);
+    for (size_t i = 0; i < L; i++) {
+        float pa = 0.0f;
+        V p, n;
+        tri_sample(sc.tris[sc.lights[i]], s2, &p, &n, &pa);
+        sum_area += 1.0f / pa;
+        if ((int64_t)i == (int64_t)index) {
+            *pl_pos = p;
+            *pl_nrm = n;
+        }
+    }
+    if (pdf_area) *pdf_area = 1.0f / sum_area;
+    return true;
+}
+
+// PathTracing::CalcLightContribution (PathTracing.cpp:202-327)
+static V light_contribution(Ctx& c, const Shading& si, Sampler1D& s1, Sampler2D& s2, const Ray& ray)
+{
+    const Scene& sc = *c.sc;
+    V contrib = mk(0.0f, 0.0f, 0.0f);
+    const float d = -dot(ray.dir, si.normal);
+    const V p = add(si.pos, scl(si.normal, kEps * std::copysign(1.0f, d)));
+    float pdf_area = 1.0f;
+    V pl_pos, pl_nrm;
+    const bool is_area = sample_light(c, p, s1.Next(), s2, &pdf_area, true, &pl_pos, &pl_nrm);
+    const Material& mat = sc.mats[si.material];
+    if (is_area) {
+        if (sc.lights.empty()) return mk(0.0f, 0.0f, 0.0f);
+        Ray to_light;
+        to_light.o = p;
+        to_light.dir = normalized(sub(pl_pos, p));
+        float t;
+        const int hit = c.closest(to_light, 0.0f, FLT_MAX, &t);
+        if (hit >= 0 && sc.mats[sc.material_of(hit, s1.Next())].type == kEmission) {
+            const Shading sl = interpolate(sc, hit, to_light, t);
+            const float dist = sqlen(sub(pl_pos, sl.pos));
+            if (approx_eq(dist, 0.0f, kEps)) {
+                const float l2 = sqlen(sub(sl.pos, si.pos));
+                const float cos_p = std::fabs(dot(to_light.dir, sl.normal));
+                float mis = 0.0f;
+                if (l2 > 0.0f) {
+                    const float pl = pdf_area;
+                    const float pb = pdf(mat, ray.dir, to_light.dir, si) * cos_p / l2;
+                    mis = pl * pl / (pl * pl + pb * pb);
+                }
+                const V li = sc.mats[sl.material].color;
+                const V f = bxdf(mat, ray.dir, to_light.dir, si);
+                const float cs = std::fabs(dot(to_light.dir, si.normal));
+                const V k = divs(scl(scl(mul(li, f), cs), cos_p), pdf_area * l2);
+                contrib = add(contrib, mul(scl(ray.throughput, mis), k));
+            }
+        }
+    } else if (sc.env.use_env_is()) {
+        float pdf_uv = 0.0f;
+        const V dir = sc.env.sample(s2, &pdf_uv);
+        Ray to_env;
+        to_env.o = add(si.pos, scl(si.normal, kEps * std::copysign(1.0f, d)));
+        to_env.dir = dir;
+        float t;
+        const int hit = c.closest(to_env, 0.0f, FLT_MAX, &t);
+        if (hit < 0) {
+            const float cs = std::fabs(dot(to_env.dir, si.normal));
+            const float sn = sqrtf(smax(0.0f, 1.0f - cs * cs));
+            const float pb = 2.0f * kPi * kPi * sn * pdf(mat, ray.dir, to_env.dir, si);
+            const float mis = pdf_uv * pdf_uv / (pdf_uv * pdf_uv + pb * pb);
+            const V li = sc.env.color(to_env.dir);
+            const V f = bxdf(mat, ray.dir, to_env.dir, si);
+            // misWeight * T * (li * f * sin * cos * 2 * pi * pi) / pdfuv: the division
+            // applies to the whole product (PathTracing.cpp:316-318).
+            const V k = scl(scl(scl(scl(scl(mul(li, f), sn), cs), 2.0f), kPi), kPi);
+            contrib = add(contrib, divs(mul(scl(ray.throughput, mis), k), pdf_uv));
+        }
+    }
+    return contrib;
+}
+
+// PathTracing::Render (PathTracing.cpp:20-200) for one pixel.
+static V render_mis(Ctx& c, int px, int py, int w, int h, int spp, int max_bounces, Sampler1D& s1,
+                    Sampler2D& s2)
+{
+    const Scene& sc = *c.sc;
+    V sum = mk(0.0f, 0.0f, 0.0f);
+    for (int s = 0; s < spp; s++) {
+        V color = mk(0.0f, 0.0f, 0.0f);
+        Ray ray = generate_ray(sc.cam, px, py, w, h, s2);
+        Ray prev = ray;
+        Shading si;
+        {
+            float t;
+            const int hit = c.closest(ray, 0.0f, FLT_MAX, &t);
+            if (hit < 0) {
+                sum = add(sum, mul(ray.throughput, sc.env.color(ray.dir)));
+                continue;
+            }
+            si = interpolate(sc, hit, ray, t);
+        }
+        if (sc.mats[si.material].type == kEmission) {
+            sum = add(sum, mul(ray.throughput, sc.mats[si.material].color));
+            continue;
+        }
+        for (int b = 0; b < max_bounces; b++) {
+            color = add(color, light_contribution(c, si, s1, s2, ray));
+            prev = ray;
+            const Material& mat = sc.mats[si.material];
+            ray = next_ray(mat, ray, si, s2);
+            if (approx_eq(sqlen(ray.throughput), 0.0f, kEps)) break;
+            float t;
+            const int hit = c.closest(ray, 0.0f, FLT_MAX, &t);
+            if (hit >= 0) {
+                const Shading sn = interpolate(sc, hit, ray, t);
+                if (sc.mats[sn.material].type == kEmission) {
+                    float pdf_area = 0.0f;
+                    const float l2 = sqlen(sub(sn.pos, ray.o));
+                    const float cos_p = std::fabs(dot(ray.dir, sn.normal));
+                    V dp, dn;
+                    sample_light(c, ray.o, s1.Next(), s2, &pdf_area, false, &dp, &dn);
+                    const float pb = pdf(mat, prev.dir, ray.dir, si);
+                    const float pl = l2 / cos_p * pdf_area;
+                    const float mis = pb * pb / (pl * pl + pb * pb);
+                    color = add(color, mul(scl(ray.throughput, mis), sc.mats[sn.material].color));
+                    break;
+                }
+                si = sn;
+            } else {
+                if (sc.env.use_env_is()) {
+                    const float cs = std::fabs(dot(si.normal, ray.dir));
+                    const float sn = sqrtf(smax(0.0f, 1.0f - cs * cs));
+                    const float pb = pdf(mat, prev.dir, ray.dir, si);
+                    float pe = 0.0f;
+                    float mis = 0.0;
+                    if (ray.dir.x != 0 && ray.dir.y != 0 && sn > 0) {
+                        pe = sc.env.pdf(ray.dir) / (2.0f * kPi * kPi * sn);
+                        mis = pb * pb / (pe * pe + pb * pb);
+                    } else {
+                        mis = 1.0f;
+                    }
+                    color = add(color, mul(scl(ray.throughput, mis), sc.env.color(ray.dir)));
+                } else {
+                    color = add(color, mul(ray.throughput, sc.env.color(ray.dir)));
+                }
+                break;
+            }
+        }
+        if (min_elem(color) >= 0.0f) sum = add(sum, color);
+    }
+    return divs(sum, (float)spp);
+}
+
+// SimplePathTracing::Render (SimplePathTracing.cpp:16-101) for one pixel.
+static V render_simple(Ctx& c, int px, int py, int w, int h, int spp, int max_bounces, Sampler1D& s1,
+                       Sampler2D& s2)
+{
+    const Scene& sc = *c.sc;
+    V sum = mk(0.0f, 0.0f, 0.0f);
+    for (int s = 0; s < spp; s++) {
+        Ray ray = generate_ray(sc.cam, px, py, w, h, s2);
+        V contrib = mk(0.0f, 0.0f, 0.0f);
+        for (;;) {
+            float t;
+            const int hit = c.closest(ray, kEps, kInfinity, &t);
+            if (hit < 0) {
+                contrib = add(contrib, mul(ray.throughput, sc.env.color(ray.dir)));
+                break;
+            }
+            const int m = sc.material_of(hit, s1.Next());
+            if (sc.mats[m].type == kEmission) {
+                contrib = add(contrib, mul(ray.throughput, sc.mats[m].color));
+                break;
+            }
+            const Shading si = interpolate(sc, hit, ray, t);
+            ray = next_ray(sc.mats[m], ray, si, s2);
+            if (ray.bounce > max_bounces) {
+                ray.prob *= 0.9f;
+                ray.prob = smax(0.1f, ray.prob);
+                if (s1.Next() < ray.prob) {
+                    ray.throughput = divs(ray.throughput, ray.prob);
+                } else {
+                    break;
+                }
+            }
+        }
+        sum = add(sum, contrib);
+    }
+    return divs(sum, (float)spp);
+}
+
+} // namespace oracle
+
+using namespace oracle;
+
+extern "C" {
+
+void* oracle_scene_create(int num_tris, const float* tri_pos, const float* tri_nrm,
+                          const int32_t* tri_mat, const int32_t* tri_smooth, int num_mats,
+                          const int32_t* mat_type, const float* mat_param, int num_lights,
+                          const int32_t* light_tris, const float* cam, int env_w, int env_h,
+                          const float* env_rgb, const float* env_param)
+{
+    Scene* sc = new Scene();
+    sc->tris.resize(num_tris);
+    for (int i = 0; i < num_tris; ++i) {
+        Tri& t = sc->tris[i];
+        const float* p = tri_pos + 9 * (size_t)i;
+        t.p0 = mk(p[0], p[1], p[2]);
+        t.p1 = mk(p[3], p[4], p[5]);
+        t.p2 = mk(p[6], p[7], p[8]);
+        if (tri_nrm) {
+            const float* q = tri_nrm + 9 * (size_t)i;
+            t.n0 = mk(q[0], q[1], q[2]);
+            t.n1 = mk(q[3], q[4], q[5]);
+            t.n2 = mk(q[6], q[7], q[8]);
+        } else {
+            t.n0 = t.n1 = t.n2 = mk(0.0f, 0.0f, 1.0f); // Vertex.h:21 default normal
+        }
+        t.smooth = tri_smooth ? tri_smooth[i] != 0 : false;
+        t.material = tri_mat[i];
+    }
+    for (int m = 0; m < num_mats; ++m) sc->mats.push_back(make_material(mat_type[m], mat_param + 8 * m));
+    sc->lights.assign(light_tris, light_tris + num_lights);
+    Camera& c = sc->cam;
+    c.pos = mk(cam[0], cam[1], cam[2]);
+    c.right = mk(cam[3], cam[4], cam[5]);
+    c.up = mk(cam[6], cam[7], cam[8]);
+    c.forward = mk(cam[9], cam[10], cam[11]);
+    c.focus = cam[12];
+    c.h_perf = cam[13];
+    c.aperture = cam[14];
+    Environment& e = sc->env;
+    e.base = mk(env_param[0], env_param[1], env_param[2]);
+    e.rot = env_param[3];
+    e.use_is = env_param[4] != 0.0f;
+    if (env_rgb && env_w > 0 && env_h > 0) {
+        e.tex.w = env_w;
+        e.tex.h = env_h;
+        e.tex.loaded = true;
+        e.tex.px.resize((size_t)env_w * env_h);
+        for (size_t i = 0; i < e.tex.px.size(); ++i)
+            e.tex.px[i] = mk(env_rgb[3 * i], env_rgb[3 * i + 1], env_rgb[3 * i + 2]);
+        e.precalc();
+    }
+    return sc;
+}
+
+void oracle_scene_destroy(void* s) { delete (Scene*)s; }
+
+void* oracle_bvh_build(void* s, int num_bins)
+{
+    BVH* b = new BVH();
+    build_bvh(*(Scene*)s, num_bins, b);
+    return b;
+}
+void oracle_bvh_destroy(void* b) { delete (BVH*)b; }
+int oracle_bvh_num_nodes(void* b) { return (int)((BVH*)b)->nodes.size(); }
+int oracle_bvh_max_depth(void* b) { return ((BVH*)b)->max_depth; }
+
+void oracle_bvh_export(void* bp, float* boxes, int32_t* child, int32_t* range, int32_t* leaf,
+                       int32_t* prim_ids)
+{
+    const BVH& b = *(BVH*)bp;
+    for (size_t i = 0; i < b.nodes.size(); ++i) {
+        const Node& n = b.nodes[i];
+        if (boxes) {
+            float* o = boxes + 6 * i;
+            o[0] = n.box.lower.x; o[1] = n.box.lower.y; o[2] = n.box.lower.z;
+            o[3] = n.box.upper.x; o[4] = n.box.upper.y; o[5] = n.box.upper.z;
+        }
+        if (child) { child[2 * i] = n.child[0]; child[2 * i + 1] = n.child[1]; }
+        if (range) { range[2 * i] = n.begin; range[2 * i + 1] = n.end; }
+        if (leaf) leaf[i] = n.leaf ? 1 : 0;
+    }
+    if (prim_ids) std::memcpy(prim_ids, b.ids.data(), b.ids.size() * sizeof(int32_t));
+}
+
+void oracle_trace_closest(void* s, void* b, int64_t n, const float* rays, float* t_out,
+                          int32_t* prim_out, int64_t* stats)
+{
+    const Scene& sc = *(Scene*)s;
+    const BVH& bvh = *(BVH*)b;
+    int64_t nv = 0, nt = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        Ray ray;
+        ray.o = mk(r[0], r[1], r[2]);
+        ray.dir = mk(r[3], r[4], r[5]);
+        float t;
+        const int p = bvh_intersect(sc, bvh, ray, r[6], r[7], &t, &nv, &nt);
+        prim_out[i] = p;
+        t_out[i] = p >= 0 ? t : INFINITY;
+    }
+    if (stats) {
+        stats[0] = nv;
+        stats[1] = nt;
+    }
+}
+
+void oracle_trace_brute(void* s, int64_t n, const float* rays, float* t_out, int32_t* prim_out)
+{
+    const Scene& sc = *(Scene*)s;
+    for (int64_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        Ray ray;
+        ray.o = mk(r[0], r[1], r[2]);
+        ray.dir = mk(r[3], r[4], r[5]);
+        float t;
+        const int p = brute_intersect(sc, ray, r[6], r[7], &t);
+        prim_out[i] = p;
+        t_out[i] = p >= 0 ? t : INFINITY;
+    }
+}
+
+int64_t oracle_render(void* s, void* b, int width, int height, int spp, int max_bounces,
+                      int integrator, uint32_t seed, int num_threads, const int32_t* pixels,
+                      int64_t num_pixels, float* rgb_out)
+{
+    const Scene& sc = *(Scene*)s;
+    const BVH& bvh = *(BVH*)b;
+    // Core/TileManager.cpp:8-43 (16x16 tiles) and Core/Thread/ThreadPool.cpp:17-26
+    // (hardware_concurrency workers); tiles are taken from an atomic counter.
+    const int tw = 16, th = 16;
+    int64_t total = pixels ? num_pixels : (int64_t)width * height;
+    const int64_t tile_px = (int64_t)tw * th;
+    const int64_t ntiles = pixels ? (total + tile_px - 1) / tile_px
+                                  : (int64_t)((width + tw - 1) / tw) * ((height + th - 1) / th);
+    const int tiles_x = (width + tw - 1) / tw;
+    std::atomic<int64_t> next_tile(0);
+    std::atomic<int64_t> queries(0);
+    unsigned nt = num_threads > 0 ? (unsigned)num_threads : std::thread::hardware_concurrency();
+    if (nt == 0) nt = 1;
+    auto worker = [&]() {
+        Ctx c;
+        c.sc = &sc;
+        c.bvh = &bvh;
+        for (;;) {
+            const int64_t tile = next_tile.fetch_add(1);
+            if (tile >= ntiles) break;
+            auto do_pixel = [&](int64_t slot, int x, int y) {
+                const uint32_t pix = (uint32_t)((int64_t)y * width + x);
+                Sampler1D s1(pixel_seed(seed, pix, 0));
+                Sampler2D s2(pixel_seed(seed, pix, 1), pixel_seed(seed, pix, 2));
+                const V r = integrator == 0 ? render_mis(c, x, y, width, height, spp, max_bounces, s1, s2)
+                                            : render_simple(c, x, y, width, height, spp, max_bounces, s1, s2);
+                rgb_out[3 * slot] = r.x;
+                rgb_out[3 * slot + 1] = r.y;
+                rgb_out[3 * slot + 2] = r.z;
+            };
+            if (pixels) {
+                const int64_t e = std::min(total, (tile + 1) * tile_px);
+                for (int64_t k = tile * tile_px; k < e; ++k)
+                    do_pixel(k, pixels[k] % width, pixels[k] / width);
+            } else {
+                const int x0 = (int)(tile % tiles_x) * tw, y0 = (int)(tile / tiles_x) * th;
+                for (int y = y0; y < std::min(y0 + th, height); ++y)
+                    for (int x = x0; x < std::min(x0 + tw, width); ++x)
+                        do_pixel((int64_t)y * width + x, x, y);
+            }
+        }
+        queries += c.queries;
+    };
+    std::vector<std::thread> pool;
+    for (unsigned i = 1; i < nt; ++i) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+    return queries.load();
+}
+
+uint32_t oracle_pixel_seed(uint32_t job_seed, uint32_t pixel, uint32_t stream)
+{
+    return pixel_seed(job_seed, pixel, stream);
+}
+
+void oracle_xorshift(uint32_t seed, int n, uint32_t* u32_out, float* f_out)
+{
+    XorShift a(seed), b(seed);
+    for (int i = 0; i < n; ++i) {
+        if (u32_out) u32_out[i] = a.random();
+        if (f_out) f_out[i] = b.next();
+    }
+}
+
+static Camera cam_from(const float* cam)
+{
+    Camera c;
+    c.pos = mk(cam[0], cam[1], cam[2]);
+    c.right = mk(cam[3], cam[4], cam[5]);
+    c.up = mk(cam[6], cam[7], cam[8]);
+    c.forward = mk(cam[9], cam[10], cam[11]);
+    c.focus = cam[12];
+    c.h_perf = cam[13];
+    c.aperture = cam[14];
+    return c;
+}
+
+void oracle_camera_ray(const float* cam, int i, int j, int w, int h, uint32_t seed0, uint32_t seed1,
+                       float* out)
+{
+    Sampler2D s2(seed0, seed1);
+    const Ray r = generate_ray(cam_from(cam), i, j, w, h, s2);
+    const float v[10] = {r.o.x, r.o.y, r.o.z, r.dir.x, r.dir.y, r.dir.z,
+                         r.throughput.x, r.throughput.y, r.throughput.z, (float)s2.draws};
+    std::memcpy(out, v, sizeof v);
+}
+
+void oracle_next_ray(int mat_type, const float* mat_param, const float* shading, const float* ray_in,
+                     uint32_t seed0, uint32_t seed1, float* out)
+{
+    const Material m = make_material(mat_type, mat_param);
+    Shading si;
+    si.pos = mk(shading[0], shading[1], shading[2]);
+    si.normal = mk(shading[3], shading[4], shading[5]);
+    si.material = 0;
+    Ray in;
+    in.o = mk(ray_in[0], ray_in[1], ray_in[2]);
+    in.dir = mk(ray_in[3], ray_in[4], ray_in[5]);
+    in.throughput = mk(ray_in[6], ray_in[7], ray_in[8]);
+    in.bounce = (int)ray_in[9];
+    in.prob = ray_in[10];
+    in.ior = ray_in[11];
+    Sampler2D s2(seed0, seed1);
+    const Ray r = next_ray(m, in, si, s2);
+    const float v[12] = {r.o.x, r.o.y, r.o.z, r.dir.x, r.dir.y, r.dir.z, r.throughput.x,
+                         r.throughput.y, r.throughput.z, (float)r.bounce, r.prob, r.ior};
+    std::memcpy(out, v, sizeof v);
+}
+
+void oracle_bsdf_eval(int mat_type, const float* mat_param, const float* normal, const float* dir_in,
+                      const float* dir_out, float* out4)
+{
+    const Material m = make_material(mat_type, mat_param);
+    Shading si;
+    si.normal = mk(normal[0], normal[1], normal[2]);
+    si.pos = mk(0.0f, 0.0f, 0.0f);
+    const V di = mk(dir_in[0], dir_in[1], dir_in[2]);
+    const V dout = mk(dir_out[0], dir_out[1], dir_out[2]);
+    const V f = bxdf(m, di, dout, si);
+    out4[0] = f.x;
+    out4[1] = f.y;
+    out4[2] = f.z;
+    out4[3] = pdf(m, di, dout, si);
+}
+
+int oracle_triangle_intersect(const float* tri9, const float* ray6, float* t)
+{
+    Tri tr;
+    tr.p0 = mk(tri9[0], tri9[1], tri9[2]);
+    tr.p1 = mk(tri9[3], tri9[4], tri9[5]);
+    tr.p2 = mk(tri9[6], tri9[7], tri9[8]);
+    Ray r;
+    r.o = mk(ray6[0], ray6[1], ray6[2]);
+    r.dir = mk(ray6[3], ray6[4], ray6[5]);
+    return tri_intersect(tr, r, t) ? 1 : 0;
+}
+
+void oracle_triangle_sample(const float* tri9, uint32_t seed0, uint32_t seed1, float* out7)
+{
+    Tri tr;
+    tr.p0 = mk(tri9[0], tri9[1], tri9[2]);
+    tr.p1 = mk(tri9[3], tri9[4], tri9[5]);
+    tr.p2 = mk(tri9[6], tri9[7], tri9[8]);
+    Sampler2D s2(seed0, seed1);
+    V p, n;
+    float pa;
+    tri_sample(tr, s2, &p, &n, &pa);
+    const float v[7] = {p.x, p.y, p.z, n.x, n.y, n.z, pa};
+    std::memcpy(out7, v, sizeof v);
+}
+
+void oracle_env_color(void* s, const float* dir, float* rgb)
+{
+    const V c = ((Scene*)s)->env.color(mk(dir[0], dir[1], dir[2]));
+    rgb[0] = c.x;
+    rgb[1] = c.y;
+    rgb[2] = c.z;
+}
+
+void oracle_env_sample(void* s, uint32_t seed0, uint32_t seed1, float* out4)
+{
+    Sampler2D s2(seed0, seed1);
+    float p = 0.0f;
+    const V d = ((Scene*)s)->env.sample(s2, &p);
+    out4[0] = d.x;
+    out4[1] = d.y;
+    out4[2] = d.z;
+    out4[3] = p;
+}
+
+float oracle_env_pdf(void* s, const float* dir)
+{
+    return ((Scene*)s)->env.pdf(mk(dir[0], dir[1], dir[2]));
+}
+
+void oracle_onb(const float* w_in, float theta, float phi, float* out12)
+{
+    // OrthonormalBasis::Build(w) then GetDir(theta, phi) (OrthonormalBasis.cpp:10-43)
+    const V w = normalized(mk(w_in[0], w_in[1], w_in[2]));
+    const float sign = std::copysign(1.0f, w.z);
+    const float a = -1.0f / (sign + w.z);
+    const float b = w.x * w.y * a;
+    const V u = mk(1.0f + sign * w.x * w.x * a, sign * b, -sign * w.x);
+    const V v = mk(b, sign + w.y * w.y * a, -w.y);
+    double st, ct, sp, cp;
+    pt_sincos((double)theta, &st, &ct);
+    pt_sincos((double)phi, &sp, &cp);
+    const float cx = (float)(st * cp), cy = (float)(st * sp), cz = (float)ct;
+    const V d = add(add(scl(u, cx), scl(v, cy)), scl(w, cz));
+    const float o[12] = {u.x, u.y, u.z, v.x, v.y, v.z, w.x, w.y, w.z, d.x, d.y, d.z};
+    std::memcpy(out12, o, sizeof o);
+}
+
+} // extern "C"

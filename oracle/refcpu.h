@@ -1,0 +1,95 @@
+/* refcpu.h — C API of the CPU oracle for Petrichor's rendering hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. The oracle is a clean-room CPU restatement of the
+ * reference renderer (LibPetrichor/Core, see refcpu.cpp for per-function
+ * citations). Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg may load it, and only as the checker or as the timed CPU baseline; the
+ * product library (petrichor_amd) never calls it.
+ *
+ * Scene layout shared with the product's C-ABI (include/petrichor_hip.h):
+ *   tri_pos      9 floats per triangle (p0, p1, p2)
+ *   tri_nrm      9 floats per triangle (vertex normals) or NULL
+ *   tri_mat      material index per triangle
+ *   tri_smooth   0 = flat, 1 = smooth shading per triangle, or NULL (all flat)
+ *   mat_type     0 Lambert, 1 Glass, 2 Emission, 3 GGX ("Glossy"), 4 Mix
+ *                (the order of Core/Material/MaterialBase.h:15-22)
+ *   mat_param    8 floats per material: color[3] (kd / ior-colour / Le / f0),
+ *                p3 (GGX roughness, Glass ior, Mix weight), p4, p5 (Mix
+ *                sub-material indices), p6 (importance sampling enabled), p7 unused
+ *   light_tris   triangle indices registered with Scene::AppendLight, in order
+ *   cam          16 floats: pos[3] right[3] up[3] forward[3] focus_dist h_perf aperture 0
+ *   env_rgb      env_w*env_h*3 floats (row-major, RGB) or NULL (no texture)
+ *   env_param    5 floats: base_color[3], z_rotation, use_importance_sampling
+ */
+#ifndef PETRICHOR_ORACLE_REFCPU_H
+#define PETRICHOR_ORACLE_REFCPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void* oracle_scene_create(int num_tris, const float* tri_pos, const float* tri_nrm,
+                          const int32_t* tri_mat, const int32_t* tri_smooth,
+                          int num_mats, const int32_t* mat_type, const float* mat_param,
+                          int num_lights, const int32_t* light_tris, const float* cam,
+                          int env_w, int env_h, const float* env_rgb, const float* env_param);
+void oracle_scene_destroy(void* scene);
+
+/* Binary binned-SAH BVH, Core/Accel/BinnedSAHBVH.cpp:15-225 (num_bins = 16 there). */
+void* oracle_bvh_build(void* scene, int num_bins);
+void oracle_bvh_destroy(void* bvh);
+int oracle_bvh_num_nodes(void* bvh);
+int oracle_bvh_max_depth(void* bvh);
+/* Node table in the reference's order: box[6] (lower, upper), child[2], range[2], leaf. */
+void oracle_bvh_export(void* bvh, float* boxes, int32_t* child, int32_t* range, int32_t* leaf,
+                       int32_t* prim_ids);
+
+/* Closest-hit queries. rays: 8 floats each (o[3], d[3], dist_min, dist_max).
+ * Outputs t (+inf on a miss) and prim (-1 on a miss). stats (optional, 2 int64):
+ * node visits (slab tests) and triangle tests summed over the rays. */
+void oracle_trace_closest(void* scene, void* bvh, int64_t n, const float* rays, float* t_out,
+                          int32_t* prim_out, int64_t* stats);
+/* Core/Accel/BruteForce.cpp:21-50: layout-independent closest hit. */
+void oracle_trace_brute(void* scene, int64_t n, const float* rays, float* t_out, int32_t* prim_out);
+
+/* Render with the MIS integrator (integrator 0, Core/Integrator/PathTracing.cpp) or the
+ * simple path tracer (integrator 1, Core/Integrator/SimplePathTracing.cpp), per-pixel
+ * seeded samplers, 16x16 tiles on num_threads workers (0 = all hardware threads).
+ * If pixels != NULL only those num_pixels pixel indices are rendered (rgb_out then holds
+ * 3 floats per listed pixel). Returns the number of BVH queries issued. */
+int64_t oracle_render(void* scene, void* bvh, int width, int height, int spp, int max_bounces,
+                      int integrator, uint32_t seed, int num_threads, const int32_t* pixels,
+                      int64_t num_pixels, float* rgb_out);
+
+/* Building blocks, exposed for unit tests against the reference translation units. */
+uint32_t oracle_pixel_seed(uint32_t job_seed, uint32_t pixel, uint32_t stream);
+void oracle_xorshift(uint32_t seed, int n, uint32_t* u32_out, float* f_out);
+/* Camera::GenerateRay with a RandomSampler2D(seed0, seed1): out 10 floats
+ * (o[3], dir[3], throughput[3], samples consumed). */
+void oracle_camera_ray(const float* cam, int i, int j, int w, int h, uint32_t seed0, uint32_t seed1,
+                       float* out);
+/* Material::CreateNextRay for material `mat` (mat_type/mat_param as above) on a shading
+ * point {pos[3], normal[3]} for an incoming ray {o[3], dir[3], throughput[3], bounce, prob,
+ * ior}; RandomSampler2D(seed0, seed1). out: o[3] dir[3] throughput[3] bounce prob ior. */
+void oracle_next_ray(int mat_type, const float* mat_param, const float* shading, const float* ray_in,
+                     uint32_t seed0, uint32_t seed1, float* out);
+/* BxDF (3 floats) and PDF (1 float) for an incoming/outgoing direction pair. */
+void oracle_bsdf_eval(int mat_type, const float* mat_param, const float* normal, const float* dir_in,
+                      const float* dir_out, float* out4);
+/* Triangle::Intersect on one triangle: returns 1 on a hit and writes t. */
+int oracle_triangle_intersect(const float* tri9, const float* ray6, float* t);
+/* Triangle::SampleSurface: out pos[3] normal[3] pdf_area. */
+void oracle_triangle_sample(const float* tri9, uint32_t seed0, uint32_t seed1, float* out7);
+/* Environment: GetColor (3), ImportanceSampling (dir[3], pdf), GetImportanceSamplingPDF. */
+void oracle_env_color(void* scene, const float* dir, float* rgb);
+void oracle_env_sample(void* scene, uint32_t seed0, uint32_t seed1, float* out4);
+float oracle_env_pdf(void* scene, const float* dir);
+/* OrthonormalBasis::Build(w) + GetDir(theta, phi): out u[3] v[3] w[3] dir[3]. */
+void oracle_onb(const float* w, float theta, float phi, float* out12);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
