@@ -1,0 +1,148 @@
+/* petrichor_hip.h — C-ABI of the MI355X (gfx950) rendering hot path.
+ *
+ * Plain pointers, sizes and status codes; no C++ or torch types cross this
+ * boundary. Each entry point names the reference interface it replaces
+ * (paths relative to the reference's LibPetrichor/).
+ *
+ * Status codes: 0 = ok, < 0 = error (see PT_ERR_*); pt_last_error() returns a
+ * description of the last failure on a context. No entry point aborts the process.
+ * Calls on one context are externally serialised (one HIP stream per context).
+ */
+#ifndef PETRICHOR_HIP_H
+#define PETRICHOR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_OK 0
+#define PT_ERR_ARG (-1)         /* bad argument or descriptor */
+#define PT_ERR_OOM (-2)         /* device allocation failed */
+#define PT_ERR_BUDGET (-3)      /* request exceeds the memory budget (serving) */
+#define PT_ERR_UNSUPPORTED (-4) /* feature outside the supported scene model */
+#define PT_ERR_HIP (-5)         /* HIP runtime error */
+#define PT_ERR_STATE (-6)       /* wrong object state (e.g. no BVH built) */
+
+/* Material types in the order of Core/Material/MaterialBase.h:15-22. */
+#define PT_MAT_LAMBERT 0
+#define PT_MAT_GLASS 1
+#define PT_MAT_EMISSION 2
+#define PT_MAT_GLOSSY 3 /* GGX */
+#define PT_MAT_MIX 4
+
+#define PT_INTEGRATOR_MIS 0    /* Core/Integrator/PathTracing.cpp */
+#define PT_INTEGRATOR_SIMPLE 1 /* Core/Integrator/SimplePathTracing.cpp */
+
+typedef struct pt_context pt_context;
+typedef struct pt_scene pt_scene;
+typedef struct pt_bvh pt_bvh;
+
+/* Flattened Core::Scene (Core/Scene.h:60-218). Triangles are given as three
+ * positions each (the reference's Triangle holds three Vertex pointers,
+ * Core/Geometry/Triangle.h:59); a hit identifies a triangle by its index in this
+ * array instead of a GeometryBase pointer. */
+typedef struct pt_scene_desc {
+    int32_t num_triangles;
+    const float* positions;      /* 9 floats per triangle: p0, p1, p2 */
+    const float* normals;        /* 9 floats per triangle (vertex normals) or NULL */
+    const int32_t* material_ids; /* one per triangle */
+    const int32_t* smooth;       /* 1 = ShadingTypes::Smooth per triangle, or NULL (all flat) */
+    int32_t num_materials;
+    const int32_t* material_types; /* PT_MAT_* */
+    /* 8 floats per material: color[3] (Lambert kd, Glass colour, Emission Le, GGX f0),
+     * p3 (GGX roughness, Glass ior, Mix weight), p4, p5 (Mix sub-material indices),
+     * p6 (importance sampling enabled: 1 or 0), p7 unused */
+    const float* material_params;
+    int32_t num_lights;
+    const int32_t* light_triangles; /* Scene::AppendLight order (Core/Scene.h:86-92) */
+    /* Camera state of Core/Camera.h:121-130: pos[3] right[3] up[3] forward[3]
+     * focus_dist h_perf aperture, then one unused float. */
+    float camera[16];
+    int32_t env_width, env_height;
+    const float* env_rgb; /* env_width*env_height RGB floats, row-major, or NULL */
+    float env_base_color[3];
+    float env_z_rotation;
+    int32_t env_importance_sampling; /* Environment::useEnvImportanceSampling */
+} pt_scene_desc;
+
+typedef struct pt_bvh_desc {
+    int32_t num_bins; /* 16 in the reference (Core/Accel/BinnedSAHBVH.h:174); 32 for the microbenchmark */
+    int32_t leaf_max; /* 4 in the reference (BinnedSAHBVH.cpp:79) */
+    int32_t width;    /* traversal node width: 2 (reference order) */
+} pt_bvh_desc;
+
+typedef struct pt_render_desc {
+    int32_t width, height; /* Core/RenderSetting.h:12-13 */
+    int32_t spp;           /* numSamplesPerPixel */
+    int32_t max_bounces;   /* numMaxBounces */
+    int32_t integrator;    /* PT_INTEGRATOR_* */
+    uint32_t seed;         /* job seed of the per-pixel sampler streams */
+} pt_render_desc;
+
+typedef struct pt_render_stats {
+    int64_t queries;        /* BVH queries issued (extension + shadow) */
+    int64_t iterations;     /* wavefront iterations */
+    double elapsed_ms;      /* device time of the render loop */
+} pt_render_stats;
+
+/* Context = one device + one HIP stream. */
+int pt_ctx_create(int device, pt_context** out);
+int pt_ctx_destroy(pt_context* ctx);
+const char* pt_last_error(pt_context* ctx);
+
+/* Uploads a scene (replaces building Core::Scene + Environment::Load/PreCalc,
+ * Core/Scene.h:60-218, Core/Environment.cpp:14-128). Host pointers. */
+int pt_scene_create(pt_context* ctx, const pt_scene_desc* desc, pt_scene** out);
+int pt_scene_destroy(pt_scene* scene);
+
+/* Binned-SAH BVH over the scene (replaces BinnedSAHBVH::Build,
+ * Core/Accel/BinnedSAHBVH.cpp:15-225). */
+int pt_bvh_build(pt_scene* scene, const pt_bvh_desc* desc, pt_bvh** out);
+int pt_bvh_destroy(pt_bvh* bvh);
+/* Leaf ordering (the reference's m_primitiveIDs, BinnedSAHBVH.h:178): prim_order[k]
+ * is the triangle at leaf slot k. Host pointer of num_triangles ints. */
+int pt_bvh_export_order(pt_bvh* bvh, int32_t* prim_order);
+/* Node table of the binary tree in the reference's node order (BinnedSAHBVH::m_nodes,
+ * BinnedSAHBVH.h:22-52): per node box[6] (lower, upper), child[2] (-1 for leaves),
+ * range[2] (first and end leaf slot) and leaf flag. Host pointers sized by pt_bvh_info's
+ * num_nodes; any may be NULL. */
+int pt_bvh_export_nodes(pt_bvh* bvh, float* boxes, int32_t* child, int32_t* range, int32_t* leaf);
+/* Node and depth counts of the binary tree. */
+int pt_bvh_info(pt_bvh* bvh, int32_t* num_nodes, int32_t* max_depth, double* build_ms);
+
+/* Builds the binary tree on the host only, without a device (for validation and for
+ * callers that build offline): tri_pos 9 floats per triangle. Outputs, each sized by the
+ * caller: prim_order[num_triangles], and per node (at most 2*num_triangles - 1, or 1)
+ * boxes[6], child[2], range[2], leaf[1]. Returns the node count (>= 1) or a PT_ERR_*. */
+int pt_bvh_build_host(const float* tri_pos, int32_t num_triangles, const pt_bvh_desc* desc,
+                      int32_t* prim_order, float* boxes, int32_t* child, int32_t* range, int32_t* leaf,
+                      int32_t* max_depth);
+
+/* Closest hit for n rays (replaces BinnedSAHBVH::Intersect, BinnedSAHBVH.cpp:272-374).
+ * Device pointers. rays: 8 floats per ray (ox oy oz dist_min dx dy dz dist_max);
+ * hits: t_out (+inf on a miss) and prim_out (triangle index, -1 on a miss).
+ * stream: a hipStream_t or NULL for the context's stream. */
+int pt_trace_closest(pt_bvh* bvh, int64_t n, const float* rays, float* t_out, int32_t* prim_out,
+                     void* stream);
+/* Any hit (occlusion) for n rays: hit_out[i] = 1 if some triangle is hit within
+ * [dist_min, dist_max], else 0. No reference counterpart (the reference's shadow
+ * rays call the closest-hit query, Core/Integrator/PathTracing.cpp:293). */
+int pt_trace_any(pt_bvh* bvh, int64_t n, const float* rays, int32_t* hit_out, void* stream);
+
+/* Closest hit as pt_trace_closest, synchronous, also returning the traversal work:
+ * host_stats[0] = node slab tests, host_stats[1] = triangle tests, summed over rays. */
+int pt_trace_stats(pt_bvh* bvh, int64_t n, const float* rays, float* t_out, int32_t* prim_out,
+                   int64_t* host_stats);
+
+/* Renders one image (replaces Core::Petrichor::Render, Core/Petrichor.cpp:34-102, with
+ * per-pixel sampler seeds). rgb_out: width*height*3 floats, host pointer when
+ * out_on_device == 0, device pointer otherwise. stats may be NULL. */
+int pt_render(pt_scene* scene, pt_bvh* bvh, const pt_render_desc* desc, float* rgb_out,
+              int32_t out_on_device, pt_render_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1268,6 +1268,28 @@ void* oracle_bvh_build(void* s, int num_bins)
     build_bvh(*(Scene*)s, num_bins, b);
     return b;
 }
+void* oracle_bvh_from_tables(void* s, int num_nodes, const float* boxes, const int32_t* child,
+                             const int32_t* range, const int32_t* leaf, const int32_t* prim_ids)
+{
+    (void)s;
+    BVH* b = new BVH();
+    b->nodes.resize(num_nodes);
+    for (int i = 0; i < num_nodes; ++i) {
+        Node& n = b->nodes[i];
+        const float* o = boxes + 6 * i;
+        n.box.lower = mk(o[0], o[1], o[2]);
+        n.box.upper = mk(o[3], o[4], o[5]);
+        n.child[0] = child[2 * i];
+        n.child[1] = child[2 * i + 1];
+        n.begin = range[2 * i];
+        n.end = range[2 * i + 1];
+        n.leaf = leaf[i] != 0;
+    }
+    const int nt = (int)((Scene*)s)->tris.size();
+    b->ids.assign(prim_ids, prim_ids + nt);
+    return b;
+}
+
 void oracle_bvh_destroy(void* b) { delete (BVH*)b; }
 int oracle_bvh_num_nodes(void* b) { return (int)((BVH*)b)->nodes.size(); }
 int oracle_bvh_max_depth(void* b) { return ((BVH*)b)->max_depth; }
@@ -1300,9 +1322,9 @@ void oracle_trace_closest(void* s, void* b, int64_t n, const float* rays, float*
         const float* r = rays + 8 * i;
         Ray ray;
         ray.o = mk(r[0], r[1], r[2]);
-        ray.dir = mk(r[3], r[4], r[5]);
+        ray.dir = mk(r[4], r[5], r[6]);
         float t;
-        const int p = bvh_intersect(sc, bvh, ray, r[6], r[7], &t, &nv, &nt);
+        const int p = bvh_intersect(sc, bvh, ray, r[3], r[7], &t, &nv, &nt);
         prim_out[i] = p;
         t_out[i] = p >= 0 ? t : INFINITY;
     }
@@ -1319,9 +1341,9 @@ void oracle_trace_brute(void* s, int64_t n, const float* rays, float* t_out, int
         const float* r = rays + 8 * i;
         Ray ray;
         ray.o = mk(r[0], r[1], r[2]);
-        ray.dir = mk(r[3], r[4], r[5]);
+        ray.dir = mk(r[4], r[5], r[6]);
         float t;
-        const int p = brute_intersect(sc, ray, r[6], r[7], &t);
+        const int p = brute_intersect(sc, ray, r[3], r[7], &t);
         prim_out[i] = p;
         t_out[i] = p >= 0 ? t : INFINITY;
     }

@@ -40,13 +40,18 @@ void oracle_scene_destroy(void* scene);
 /* Binary binned-SAH BVH, Core/Accel/BinnedSAHBVH.cpp:15-225 (num_bins = 16 there). */
 void* oracle_bvh_build(void* scene, int num_bins);
 void oracle_bvh_destroy(void* bvh);
+/* A tree given as node tables (as oracle_bvh_export writes them), e.g. one built by the
+ * product, so that traversal results can be compared on an identical leaf order. */
+void* oracle_bvh_from_tables(void* scene, int num_nodes, const float* boxes, const int32_t* child,
+                             const int32_t* range, const int32_t* leaf, const int32_t* prim_ids);
 int oracle_bvh_num_nodes(void* bvh);
 int oracle_bvh_max_depth(void* bvh);
 /* Node table in the reference's order: box[6] (lower, upper), child[2], range[2], leaf. */
 void oracle_bvh_export(void* bvh, float* boxes, int32_t* child, int32_t* range, int32_t* leaf,
                        int32_t* prim_ids);
 
-/* Closest-hit queries. rays: 8 floats each (o[3], d[3], dist_min, dist_max).
+/* Closest-hit queries. rays: 8 floats each (o[3], dist_min, d[3], dist_max), the layout
+ * of the product's trace entry points.
  * Outputs t (+inf on a miss) and prim (-1 on a miss). stats (optional, 2 int64):
  * node visits (slab tests) and triangle tests summed over the rays. */
 void oracle_trace_closest(void* scene, void* bvh, int64_t n, const float* rays, float* t_out,

@@ -1,0 +1,312 @@
+"""petrichor_amd — MI355X-native rendering hot path of the Petrichor path tracer.
+
+Python host layer over the C-ABI of include/petrichor_hip.h (libpetrichor_hip.so,
+built in-tree by petrichor_amd.build). It mirrors LibPetrichor Core's objects:
+a Scene (Core/Scene.h) is uploaded once, a BVH is built over it
+(Core/Accel/BinnedSAHBVH.cpp), and render() replaces Core::Petrichor::Render
+(Core/Petrichor.cpp:34-102). Device buffers for the trace entry points are torch
+tensors on the GPU (torch is plumbing here: memory and streams).
+
+The library fails loudly: there is no CPU fallback. If the shared library is
+missing, importing works but every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpetrichor_hip.so")
+
+PT_MAT_LAMBERT, PT_MAT_GLASS, PT_MAT_EMISSION, PT_MAT_GLOSSY, PT_MAT_MIX = 0, 1, 2, 3, 4
+INTEGRATOR_MIS, INTEGRATOR_SIMPLE = 0, 1
+
+_f = ctypes.POINTER(ctypes.c_float)
+_i = ctypes.POINTER(ctypes.c_int32)
+_l = ctypes.POINTER(ctypes.c_int64)
+
+
+class PtError(RuntimeError):
+    pass
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_triangles", ctypes.c_int32), ("positions", _f), ("normals", _f), ("material_ids", _i),
+        ("smooth", _i), ("num_materials", ctypes.c_int32), ("material_types", _i), ("material_params", _f),
+        ("num_lights", ctypes.c_int32), ("light_triangles", _i), ("camera", ctypes.c_float * 16),
+        ("env_width", ctypes.c_int32), ("env_height", ctypes.c_int32), ("env_rgb", _f),
+        ("env_base_color", ctypes.c_float * 3), ("env_z_rotation", ctypes.c_float),
+        ("env_importance_sampling", ctypes.c_int32),
+    ]
+
+
+class BvhDesc(ctypes.Structure):
+    _fields_ = [("num_bins", ctypes.c_int32), ("leaf_max", ctypes.c_int32), ("width", ctypes.c_int32)]
+
+
+class RenderDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("spp", ctypes.c_int32),
+                ("max_bounces", ctypes.c_int32), ("integrator", ctypes.c_int32), ("seed", ctypes.c_uint32)]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [("queries", ctypes.c_int64), ("iterations", ctypes.c_int64), ("elapsed_ms", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libpetrichor_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PtError("libpetrichor_hip.so is not built (run python -m petrichor_amd.build)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    pvp = ctypes.POINTER(ctypes.c_void_p)
+    L.pt_ctx_create.argtypes = [ctypes.c_int, pvp]
+    L.pt_ctx_destroy.argtypes = [vp]
+    L.pt_last_error.restype = ctypes.c_char_p
+    L.pt_last_error.argtypes = [vp]
+    L.pt_scene_create.argtypes = [vp, ctypes.POINTER(SceneDesc), pvp]
+    L.pt_scene_destroy.argtypes = [vp]
+    L.pt_bvh_build.argtypes = [vp, ctypes.POINTER(BvhDesc), pvp]
+    L.pt_bvh_destroy.argtypes = [vp]
+    L.pt_bvh_export_order.argtypes = [vp, _i]
+    L.pt_bvh_info.argtypes = [vp, _i, _i, ctypes.POINTER(ctypes.c_double)]
+    L.pt_bvh_export_nodes.argtypes = [vp, _f, _i, _i, _i]
+    L.pt_bvh_build_host.argtypes = [_f, ctypes.c_int32, ctypes.POINTER(BvhDesc), _i, _f, _i, _i, _i, _i]
+    L.pt_trace_closest.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp]
+    L.pt_trace_any.argtypes = [vp, ctypes.c_int64, vp, vp, vp]
+    L.pt_trace_stats.argtypes = [vp, ctypes.c_int64, vp, vp, vp, _l]
+    L.pt_render.argtypes = [vp, vp, ctypes.POINTER(RenderDesc), vp, ctypes.c_int32, ctypes.POINTER(RenderStats)]
+    _lib = L
+    return L
+
+
+# every symbol include/petrichor_hip.h declares
+EXPORTED = ["pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_scene_create", "pt_scene_destroy",
+            "pt_bvh_build", "pt_bvh_destroy", "pt_bvh_export_order", "pt_bvh_export_nodes", "pt_bvh_build_host", "pt_bvh_info", "pt_trace_closest",
+            "pt_trace_any", "pt_trace_stats", "pt_render"]
+
+
+def _ptr(a, t):
+    return ctypes.cast(None, t) if a is None else a.ctypes.data_as(t)
+
+
+def build_bvh_host(tri_pos, bins=16, leaf_max=4):
+    """Binary BVH built on the host only (pt_bvh_build_host): the reference's node table
+    and leaf order, without a device."""
+    tri_pos = np.ascontiguousarray(tri_pos, np.float32).reshape(-1, 9)
+    n = tri_pos.shape[0]
+    cap = max(2 * n - 1, 1)
+    order = np.zeros(n, np.int32)
+    boxes = np.zeros((cap, 6), np.float32)
+    child = np.zeros((cap, 2), np.int32)
+    rng = np.zeros((cap, 2), np.int32)
+    leaf = np.zeros(cap, np.int32)
+    depth = ctypes.c_int32()
+    d = BvhDesc(bins, leaf_max, 2)
+    nn = lib().pt_bvh_build_host(_ptr(tri_pos, _f), n, ctypes.byref(d), _ptr(order, _i), _ptr(boxes, _f),
+                                 _ptr(child, _i), _ptr(rng, _i), _ptr(leaf, _i), ctypes.byref(depth))
+    if nn < 0:
+        raise PtError("pt_bvh_build_host failed (%d)" % nn)
+    return {"prim_ids": order, "boxes": boxes[:nn], "child": child[:nn], "range": rng[:nn], "leaf": leaf[:nn],
+            "max_depth": depth.value}
+
+
+class Context:
+    """One device + one HIP stream (pt_ctx_create)."""
+
+    def __init__(self, device=0):
+        self._lib = lib()
+        h = ctypes.c_void_p()
+        rc = self._lib.pt_ctx_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise PtError("pt_ctx_create failed (%d)" % rc)
+        self.handle = h
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = self._lib.pt_last_error(self.handle)
+            raise PtError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def scene(self, scene):
+        return Scene(self, scene)
+
+    def close(self):
+        if self.handle:
+            self._lib.pt_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """A scene uploaded to the device (pt_scene_create). `scene` is a dict as produced
+    by petrichor_amd.scenes."""
+
+    def __init__(self, ctx, scene):
+        self.ctx = ctx
+        self.src = scene
+        L = ctx._lib
+        keep = self._keep = []
+
+        def k(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a
+
+        d = SceneDesc()
+        tri_pos = k(scene["tri_pos"], np.float32).reshape(-1, 9)
+        d.num_triangles = tri_pos.shape[0]
+        d.positions = _ptr(tri_pos, _f)
+        d.normals = _ptr(k(scene["tri_nrm"], np.float32), _f) if scene.get("tri_nrm") is not None else None
+        d.material_ids = _ptr(k(scene["tri_mat"], np.int32), _i)
+        d.smooth = _ptr(k(scene["tri_smooth"], np.int32), _i) if scene.get("tri_smooth") is not None else None
+        mt = k(scene["mat_type"], np.int32)
+        d.num_materials = len(mt)
+        d.material_types = _ptr(mt, _i)
+        d.material_params = _ptr(k(scene["mat_param"], np.float32), _f)
+        lt = k(scene.get("light_tris", np.zeros(0, np.int32)), np.int32)
+        d.num_lights = len(lt)
+        d.light_triangles = _ptr(lt, _i)
+        cam = np.asarray(scene["camera"], np.float32)
+        for i in range(16):
+            d.camera[i] = float(cam[i])
+        env = scene.get("env_rgb")
+        if env is not None:
+            env = k(env, np.float32)
+            d.env_height, d.env_width = env.shape[0], env.shape[1]
+            d.env_rgb = _ptr(env, _f)
+        else:
+            d.env_width = d.env_height = 0
+            d.env_rgb = None
+        ep = np.asarray(scene["env_param"], np.float32)
+        for i in range(3):
+            d.env_base_color[i] = float(ep[i])
+        d.env_z_rotation = float(ep[3])
+        d.env_importance_sampling = int(ep[4] != 0)
+        h = ctypes.c_void_p()
+        ctx.check(L.pt_scene_create(ctx.handle, ctypes.byref(d), ctypes.byref(h)), "pt_scene_create")
+        self.handle = h
+        self.num_triangles = d.num_triangles
+        self._keep = []  # the library copied everything
+
+    def build_bvh(self, bins=16, leaf_max=4, width=2):
+        return Bvh(self, bins, leaf_max, width)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx._lib.pt_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Bvh:
+    """BVH over a scene (pt_bvh_build) with the trace and render entry points."""
+
+    def __init__(self, scene, bins=16, leaf_max=4, width=2):
+        self.scene = scene
+        self.ctx = scene.ctx
+        d = BvhDesc(bins, leaf_max, width)
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx._lib.pt_bvh_build(scene.handle, ctypes.byref(d), ctypes.byref(h)), "pt_bvh_build")
+        self.handle = h
+
+    def info(self):
+        nn = ctypes.c_int32()
+        md = ctypes.c_int32()
+        ms = ctypes.c_double()
+        self.ctx.check(self.ctx._lib.pt_bvh_info(self.handle, ctypes.byref(nn), ctypes.byref(md), ctypes.byref(ms)),
+                       "pt_bvh_info")
+        return {"num_nodes": nn.value, "max_depth": md.value, "build_ms": ms.value}
+
+    def export_order(self):
+        out = np.zeros(self.scene.num_triangles, np.int32)
+        self.ctx.check(self.ctx._lib.pt_bvh_export_order(self.handle, _ptr(out, _i)), "pt_bvh_export_order")
+        return out
+
+    def export_nodes(self):
+        n = self.info()["num_nodes"]
+        boxes = np.zeros((n, 6), np.float32)
+        child = np.zeros((n, 2), np.int32)
+        rng = np.zeros((n, 2), np.int32)
+        leaf = np.zeros(n, np.int32)
+        self.ctx.check(self.ctx._lib.pt_bvh_export_nodes(self.handle, _ptr(boxes, _f), _ptr(child, _i),
+                                                         _ptr(rng, _i), _ptr(leaf, _i)), "pt_bvh_export_nodes")
+        return {"boxes": boxes, "child": child, "range": rng, "leaf": leaf}
+
+    def trace_closest(self, rays, t_out=None, prim_out=None, stream=None):
+        """rays: torch float32 CUDA tensor [n, 8]. Returns (t, prim) tensors."""
+        import torch
+        n = rays.shape[0]
+        assert rays.is_cuda and rays.dtype == torch.float32 and rays.is_contiguous() and rays.shape[1] == 8
+        if t_out is None:
+            t_out = torch.empty(n, dtype=torch.float32, device=rays.device)
+        if prim_out is None:
+            prim_out = torch.empty(n, dtype=torch.int32, device=rays.device)
+        st = ctypes.c_void_p(stream) if stream else None
+        self.ctx.check(self.ctx._lib.pt_trace_closest(self.handle, n, rays.data_ptr(), t_out.data_ptr(),
+                                                      prim_out.data_ptr(), st), "pt_trace_closest")
+        return t_out, prim_out
+
+    def trace_any(self, rays, hit_out=None, stream=None):
+        import torch
+        n = rays.shape[0]
+        assert rays.is_cuda and rays.dtype == torch.float32 and rays.is_contiguous() and rays.shape[1] == 8
+        if hit_out is None:
+            hit_out = torch.empty(n, dtype=torch.int32, device=rays.device)
+        st = ctypes.c_void_p(stream) if stream else None
+        self.ctx.check(self.ctx._lib.pt_trace_any(self.handle, n, rays.data_ptr(), hit_out.data_ptr(), st),
+                       "pt_trace_any")
+        return hit_out
+
+    def trace_stats(self, rays):
+        import torch
+        n = rays.shape[0]
+        t_out = torch.empty(n, dtype=torch.float32, device=rays.device)
+        prim_out = torch.empty(n, dtype=torch.int32, device=rays.device)
+        st = np.zeros(2, np.int64)
+        self.ctx.check(self.ctx._lib.pt_trace_stats(self.handle, n, rays.data_ptr(), t_out.data_ptr(),
+                                                    prim_out.data_ptr(), _ptr(st, _l)), "pt_trace_stats")
+        return t_out, prim_out, st
+
+    def render(self, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1, out=None):
+        """Renders one image; returns (rgb [h, w, 3] float32 numpy array or the given device
+        tensor, stats dict)."""
+        d = RenderDesc(width, height, spp, max_bounces, integrator, seed)
+        st = RenderStats()
+        if out is None:
+            host = np.zeros((height, width, 3), np.float32)
+            self.ctx.check(self.ctx._lib.pt_render(self.scene.handle, self.handle, ctypes.byref(d),
+                                                   host.ctypes.data, 0, ctypes.byref(st)), "pt_render")
+            res = host
+        else:
+            self.ctx.check(self.ctx._lib.pt_render(self.scene.handle, self.handle, ctypes.byref(d),
+                                                   out.data_ptr(), 1, ctypes.byref(st)), "pt_render")
+            res = out
+        return res, {"queries": st.queries, "iterations": st.iterations, "elapsed_ms": st.elapsed_ms}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx._lib.pt_bvh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,29 @@
+// pt_bvh_host.h — host construction of the binary BVH (reference topology).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "pt_device.h"
+
+namespace pt {
+
+struct HostNode {
+    V3 lo, hi;
+    int child[2] = {-1, -1};
+    int begin = 0, end = 0;
+    bool leaf = false;
+};
+
+struct HostBvh {
+    std::vector<HostNode> nodes; // the reference's node order (BinnedSAHBVH::m_nodes)
+    std::vector<int> order;      // leaf slot -> triangle (BinnedSAHBVH::m_primitiveIDs)
+    int max_depth = 0;
+};
+
+bool build_bvh_host(const float* tri_pos, int num_tris, int num_bins, int leaf_max, HostBvh* out);
+void pack_bvh2(const HostBvh& h, std::vector<float4>& nodes, V3* root_lo, V3* root_hi, int* root_ref,
+               int* root_cnt);
+
+} // namespace pt

@@ -1,0 +1,575 @@
+// pt_capi.cpp — the C-ABI of include/petrichor_hip.h: contexts, scene upload, BVH
+// build, trace entry points and the plain per-frame wavefront render loop.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/petrichor_hip.h"
+#include "pt_bvh_host.h"
+#include "pt_kernels.h"
+
+using namespace pt;
+
+struct pt_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+struct pt_scene {
+    pt_context* ctx = nullptr;
+    int num_tris = 0;
+    std::vector<float> tri_pos, tri_nrm;
+    std::vector<int32_t> tri_mat, tri_smooth;
+    std::vector<MaterialDev> mats;
+    MaterialDev* mats_d = nullptr;
+    float4* lights_d = nullptr;
+    int num_lights = 0;
+    CameraDev cam;
+    EnvDev env;
+    float* env_rgb_d = nullptr;
+    float* pdf1d_d = nullptr;
+    float* cdf1d_d = nullptr;
+};
+
+struct pt_bvh {
+    pt_scene* scene = nullptr;
+    HostBvh host;
+    float4* nodes_d = nullptr;
+    float4* tris_d = nullptr;
+    int32_t* prim_ids_d = nullptr;
+    int32_t* tri_mat_d = nullptr;
+    int32_t* tri_mat_shading_d = nullptr;
+    int32_t* tri_smooth_d = nullptr;
+    float* tri_nrm_d = nullptr;
+    BvhView view;
+    double build_ms = 0.0;
+};
+
+namespace {
+
+int fail(pt_context* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define PT_HIP(ctx, call)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,                    \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                            \
+    } while (0)
+
+template <typename T>
+hipError_t upload(T** dst, const T* src, size_t count, hipStream_t st)
+{
+    *dst = nullptr;
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipMalloc((void**)dst, count * sizeof(T));
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, st);
+}
+
+void free_all(std::initializer_list<void*> ptrs)
+{
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+}
+
+MaterialDev material_from(int type, const float* p)
+{
+    MaterialDev m;
+    m.type = type;
+    m.color = v3(p[0], p[1], p[2]);
+    m.alpha = type == kMatGlossy ? p[3] * p[3] : 0.0f; // GGX::GGX (GGX.cpp:16-20)
+    m.ior = type == kMatGlass ? p[3] : 1.333f;
+    m.mix = type == kMatMix ? p[3] : 0.0f;
+    m.mat0 = type == kMatMix ? (int)p[4] : -1;
+    m.mat1 = type == kMatMix ? (int)p[5] : -1;
+    m.is_enabled = p[6] != 0.0f ? 1 : 0;
+    return m;
+}
+
+} // namespace
+
+extern "C" {
+
+int pt_ctx_create(int device, pt_context** out)
+{
+    if (!out) return PT_ERR_ARG;
+    *out = nullptr;
+    pt_context* c = new pt_context();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) {
+        delete c;
+        return PT_ERR_HIP;
+    }
+    *out = c;
+    return PT_OK;
+}
+
+int pt_ctx_destroy(pt_context* ctx)
+{
+    if (!ctx) return PT_ERR_ARG;
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PT_OK;
+}
+
+const char* pt_last_error(pt_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
+{
+    if (!ctx || !d || !out) return fail(ctx, PT_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (d->num_triangles < 0 || (d->num_triangles > 0 && (!d->positions || !d->material_ids)))
+        return fail(ctx, PT_ERR_ARG, "bad triangle arrays");
+    if (d->num_materials <= 0 || !d->material_types || !d->material_params)
+        return fail(ctx, PT_ERR_ARG, "scene needs at least one material");
+    if (d->num_lights < 0 || (d->num_lights > 0 && !d->light_triangles))
+        return fail(ctx, PT_ERR_ARG, "bad light list");
+    for (int i = 0; i < d->num_triangles; ++i)
+        if (d->material_ids[i] < 0 || d->material_ids[i] >= d->num_materials)
+            return fail(ctx, PT_ERR_ARG, "material id out of range at triangle " + std::to_string(i));
+    for (int i = 0; i < d->num_lights; ++i)
+        if (d->light_triangles[i] < 0 || d->light_triangles[i] >= d->num_triangles)
+            return fail(ctx, PT_ERR_ARG, "light triangle out of range");
+    for (int m = 0; m < d->num_materials; ++m) {
+        const int t = d->material_types[m];
+        if (t < 0 || t > PT_MAT_MIX) return fail(ctx, PT_ERR_UNSUPPORTED, "unknown material type");
+        if (t == PT_MAT_MIX) {
+            const int a = (int)d->material_params[8 * m + 4], b = (int)d->material_params[8 * m + 5];
+            if (a < 0 || a >= d->num_materials || b < 0 || b >= d->num_materials ||
+                d->material_types[a] == PT_MAT_MIX || d->material_types[b] == PT_MAT_MIX)
+                return fail(ctx, PT_ERR_UNSUPPORTED, "mix material must reference two non-mix materials");
+        }
+    }
+    if (d->env_rgb && (d->env_width <= 0 || d->env_height <= 0)) return fail(ctx, PT_ERR_ARG, "bad env size");
+    PT_HIP(ctx, hipSetDevice(ctx->device));
+
+    pt_scene* s = new pt_scene();
+    s->ctx = ctx;
+    s->num_tris = d->num_triangles;
+    const size_t nt = (size_t)d->num_triangles;
+    s->tri_pos.assign(d->positions, d->positions + 9 * nt);
+    if (d->normals) s->tri_nrm.assign(d->normals, d->normals + 9 * nt);
+    s->tri_mat.assign(d->material_ids, d->material_ids + nt);
+    if (d->smooth) s->tri_smooth.assign(d->smooth, d->smooth + nt);
+    for (int m = 0; m < d->num_materials; ++m)
+        s->mats.push_back(material_from(d->material_types[m], d->material_params + 8 * m));
+    std::vector<float4> lights;
+    for (int i = 0; i < d->num_lights; ++i) {
+        const float* p = d->positions + 9 * (size_t)d->light_triangles[i];
+        const TriDev t = tri_from_points(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), v3(p[6], p[7], p[8]));
+        lights.push_back(make_float4(t.p0.x, t.p0.y, t.p0.z, t.e1.x));
+        lights.push_back(make_float4(t.e1.y, t.e1.z, t.e2.x, t.e2.y));
+        lights.push_back(make_float4(t.e2.z, t.n.x, t.n.y, t.n.z));
+    }
+    s->num_lights = d->num_lights;
+    const float* c = d->camera;
+    s->cam.pos = v3(c[0], c[1], c[2]);
+    s->cam.right = v3(c[3], c[4], c[5]);
+    s->cam.up = v3(c[6], c[7], c[8]);
+    s->cam.forward = v3(c[9], c[10], c[11]);
+    s->cam.focus = c[12];
+    s->cam.h_perf = c[13];
+    s->cam.aperture = c[14];
+
+    EnvDev& e = s->env;
+    e.rgb = nullptr;
+    e.w = e.h = 0;
+    e.base = v3(d->env_base_color[0], d->env_base_color[1], d->env_base_color[2]);
+    e.rot = d->env_z_rotation;
+    e.use_is = d->env_importance_sampling;
+    e.pdf1d = e.cdf1d = nullptr;
+    hipError_t err = upload(&s->mats_d, s->mats.data(), s->mats.size(), ctx->stream);
+    if (err == hipSuccess) err = upload(&s->lights_d, lights.data(), lights.size(), ctx->stream);
+    if (err == hipSuccess && d->env_rgb) {
+        e.w = d->env_width;
+        e.h = d->env_height;
+        err = upload(&s->env_rgb_d, d->env_rgb, 3 * (size_t)e.w * e.h, ctx->stream);
+        e.rgb = s->env_rgb_d;
+        if (err == hipSuccess && e.use_is) {
+            // Environment::PreCalcCumulativeDistTex (Environment.cpp:56-128): the 2D tables
+            // are never-loaded textures reading (1,0,1), so each row contributes 1.
+            std::vector<float> pdf1(e.h), cdf1(e.h);
+            float sum = 0.0f;
+            for (int j = 0; j < e.h; ++j) {
+                const float lum = 1.0f;
+                pdf1[j] = lum;
+                sum += lum;
+                cdf1[j] = sum;
+            }
+            for (int j = 0; j < e.h; ++j) {
+                pdf1[j] *= ((float)e.h / sum);
+                cdf1[j] /= sum;
+            }
+            err = upload(&s->pdf1d_d, pdf1.data(), pdf1.size(), ctx->stream);
+            if (err == hipSuccess) err = upload(&s->cdf1d_d, cdf1.data(), cdf1.size(), ctx->stream);
+            e.pdf1d = s->pdf1d_d;
+            e.cdf1d = s->cdf1d_d;
+        }
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
+    if (err != hipSuccess) {
+        free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->pdf1d_d, s->cdf1d_d});
+        delete s;
+        return fail(ctx, err == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
+                    std::string("scene upload: ") + hipGetErrorString(err));
+    }
+    *out = s;
+    return PT_OK;
+}
+
+int pt_scene_destroy(pt_scene* s)
+{
+    if (!s) return PT_ERR_ARG;
+    free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->pdf1d_d, s->cdf1d_d});
+    delete s;
+    return PT_OK;
+}
+
+int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
+{
+    if (!s || !out) return PT_ERR_ARG;
+    pt_context* ctx = s->ctx;
+    *out = nullptr;
+    const int bins = desc ? desc->num_bins : 16;
+    const int leaf_max = desc ? desc->leaf_max : 4;
+    const int width = desc ? desc->width : 2;
+    if (width != 2) return fail(ctx, PT_ERR_UNSUPPORTED, "only width 2 is implemented");
+    pt_bvh* b = new pt_bvh();
+    b->scene = s;
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!build_bvh_host(s->tri_pos.data(), s->num_tris, bins, leaf_max, &b->host)) {
+        delete b;
+        return fail(ctx, PT_ERR_ARG, "bad BVH parameters");
+    }
+    if (b->host.max_depth > PT_MAX_STACK - 2) {
+        delete b;
+        return fail(ctx, PT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
+    }
+    std::vector<float4> nodes;
+    V3 rlo, rhi;
+    int rref, rcnt;
+    pack_bvh2(b->host, nodes, &rlo, &rhi, &rref, &rcnt);
+    const size_t nt = (size_t)s->num_tris;
+    std::vector<float4> tris(3 * nt);
+    std::vector<int32_t> mat(nt), mat_sh(nt), smooth;
+    std::vector<float> nrm;
+    if (!s->tri_smooth.empty()) smooth.resize(nt);
+    if (!s->tri_nrm.empty()) nrm.resize(9 * nt);
+    for (size_t k = 0; k < nt; ++k) {
+        const int id = b->host.order[k];
+        const float* p = &s->tri_pos[9 * (size_t)id];
+        const TriDev t = tri_from_points(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), v3(p[6], p[7], p[8]));
+        tris[3 * k] = make_float4(t.p0.x, t.p0.y, t.p0.z, t.e1.x);
+        tris[3 * k + 1] = make_float4(t.e1.y, t.e1.z, t.e2.x, t.e2.y);
+        tris[3 * k + 2] = make_float4(t.e2.z, t.n.x, t.n.y, t.n.z);
+        const int m = s->tri_mat[id];
+        mat[k] = m;
+        const MaterialDev& md = s->mats[m];
+        mat_sh[k] = md.type == kMatMix ? (md.mix <= 0.0f ? md.mat0 : md.mat1) : m; // GetMaterial(0.0f)
+        if (!smooth.empty()) smooth[k] = s->tri_smooth[id];
+        if (!nrm.empty()) std::memcpy(&nrm[9 * k], &s->tri_nrm[9 * (size_t)id], 9 * sizeof(float));
+    }
+    hipError_t err = hipSetDevice(ctx->device);
+    if (err == hipSuccess) err = upload(&b->nodes_d, nodes.data(), nodes.size(), ctx->stream);
+    if (err == hipSuccess) err = upload(&b->tris_d, tris.data(), tris.size(), ctx->stream);
+    if (err == hipSuccess) err = upload(&b->prim_ids_d, b->host.order.data(), nt, ctx->stream);
+    if (err == hipSuccess) err = upload(&b->tri_mat_d, mat.data(), nt, ctx->stream);
+    if (err == hipSuccess) err = upload(&b->tri_mat_shading_d, mat_sh.data(), nt, ctx->stream);
+    if (err == hipSuccess) err = upload(&b->tri_smooth_d, smooth.data(), smooth.size(), ctx->stream);
+    if (err == hipSuccess) err = upload(&b->tri_nrm_d, nrm.data(), nrm.size(), ctx->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
+    if (err != hipSuccess) {
+        free_all({b->nodes_d, b->tris_d, b->prim_ids_d, b->tri_mat_d, b->tri_mat_shading_d, b->tri_smooth_d,
+                  b->tri_nrm_d});
+        delete b;
+        return fail(ctx, err == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
+                    std::string("bvh upload: ") + hipGetErrorString(err));
+    }
+    b->view.nodes = b->nodes_d;
+    b->view.tris = b->tris_d;
+    b->view.prim_ids = b->prim_ids_d;
+    b->view.root_lo = rlo;
+    b->view.root_hi = rhi;
+    b->view.root_ref = rref;
+    b->view.root_cnt = rcnt;
+    b->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = b;
+    return PT_OK;
+}
+
+int pt_bvh_destroy(pt_bvh* b)
+{
+    if (!b) return PT_ERR_ARG;
+    free_all({b->nodes_d, b->tris_d, b->prim_ids_d, b->tri_mat_d, b->tri_mat_shading_d, b->tri_smooth_d,
+              b->tri_nrm_d});
+    delete b;
+    return PT_OK;
+}
+
+int pt_bvh_export_order(pt_bvh* b, int32_t* prim_order)
+{
+    if (!b || !prim_order) return PT_ERR_ARG;
+    std::memcpy(prim_order, b->host.order.data(), b->host.order.size() * sizeof(int32_t));
+    return PT_OK;
+}
+
+int pt_bvh_export_nodes(pt_bvh* b, float* boxes, int32_t* child, int32_t* range, int32_t* leaf)
+{
+    if (!b) return PT_ERR_ARG;
+    for (size_t i = 0; i < b->host.nodes.size(); ++i) {
+        const HostNode& n = b->host.nodes[i];
+        if (boxes) {
+            float* o = boxes + 6 * i;
+            o[0] = n.lo.x; o[1] = n.lo.y; o[2] = n.lo.z;
+            o[3] = n.hi.x; o[4] = n.hi.y; o[5] = n.hi.z;
+        }
+        if (child) { child[2 * i] = n.child[0]; child[2 * i + 1] = n.child[1]; }
+        if (range) { range[2 * i] = n.begin; range[2 * i + 1] = n.end; }
+        if (leaf) leaf[i] = n.leaf ? 1 : 0;
+    }
+    return PT_OK;
+}
+
+int pt_bvh_build_host(const float* tri_pos, int32_t num_triangles, const pt_bvh_desc* desc, int32_t* prim_order,
+                      float* boxes, int32_t* child, int32_t* range, int32_t* leaf, int32_t* max_depth)
+{
+    if (num_triangles < 0 || (num_triangles > 0 && !tri_pos)) return PT_ERR_ARG;
+    HostBvh h;
+    if (!build_bvh_host(tri_pos, num_triangles, desc ? desc->num_bins : 16, desc ? desc->leaf_max : 4, &h))
+        return PT_ERR_ARG;
+    if (prim_order) std::memcpy(prim_order, h.order.data(), h.order.size() * sizeof(int32_t));
+    for (size_t i = 0; i < h.nodes.size(); ++i) {
+        const HostNode& n = h.nodes[i];
+        if (boxes) {
+            float* o = boxes + 6 * i;
+            o[0] = n.lo.x; o[1] = n.lo.y; o[2] = n.lo.z;
+            o[3] = n.hi.x; o[4] = n.hi.y; o[5] = n.hi.z;
+        }
+        if (child) { child[2 * i] = n.child[0]; child[2 * i + 1] = n.child[1]; }
+        if (range) { range[2 * i] = n.begin; range[2 * i + 1] = n.end; }
+        if (leaf) leaf[i] = n.leaf ? 1 : 0;
+    }
+    if (max_depth) *max_depth = h.max_depth;
+    return (int)h.nodes.size();
+}
+
+int pt_bvh_info(pt_bvh* b, int32_t* num_nodes, int32_t* max_depth, double* build_ms)
+{
+    if (!b) return PT_ERR_ARG;
+    if (num_nodes) *num_nodes = (int32_t)b->host.nodes.size();
+    if (max_depth) *max_depth = b->host.max_depth;
+    if (build_ms) *build_ms = b->build_ms;
+    return PT_OK;
+}
+
+int pt_trace_closest(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_t* prim_out, void* stream)
+{
+    if (!b) return PT_ERR_ARG;
+    pt_context* ctx = b->scene->ctx;
+    if (n < 0 || (n > 0 && (!rays || !t_out || !prim_out))) return fail(ctx, PT_ERR_ARG, "bad trace arguments");
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    PT_HIP(ctx, launch_trace_closest(b->view, n, rays, t_out, prim_out, nullptr, st));
+    return PT_OK;
+}
+
+int pt_trace_any(pt_bvh* b, int64_t n, const float* rays, int32_t* hit_out, void* stream)
+{
+    if (!b) return PT_ERR_ARG;
+    pt_context* ctx = b->scene->ctx;
+    if (n < 0 || (n > 0 && (!rays || !hit_out))) return fail(ctx, PT_ERR_ARG, "bad trace arguments");
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    PT_HIP(ctx, launch_trace_any(b->view, n, rays, hit_out, st));
+    return PT_OK;
+}
+
+int pt_trace_stats(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_t* prim_out, int64_t* host_stats)
+{
+    if (!b || !host_stats) return PT_ERR_ARG;
+    pt_context* ctx = b->scene->ctx;
+    unsigned long long* st_d = nullptr;
+    PT_HIP(ctx, hipMalloc((void**)&st_d, 2 * sizeof(unsigned long long)));
+    unsigned long long zero[2] = {0, 0};
+    hipError_t e = hipMemcpyAsync(st_d, zero, sizeof zero, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = launch_trace_closest(b->view, n, rays, t_out, prim_out, st_d, ctx->stream);
+    unsigned long long h[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(h, st_d, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(st_d);
+    if (e != hipSuccess) return fail(ctx, PT_ERR_HIP, hipGetErrorString(e));
+    host_stats[0] = (int64_t)h[0];
+    host_stats[1] = (int64_t)h[1];
+    return PT_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------------
+// render
+// ---------------------------------------------------------------------------------
+
+namespace {
+
+struct Buffers {
+    std::vector<void*> ptrs;
+    template <typename T>
+    hipError_t alloc(T** p, size_t count)
+    {
+        *p = nullptr;
+        hipError_t e = hipMalloc((void**)p, (count ? count : 1) * sizeof(T));
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+    ~Buffers()
+    {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+} // namespace
+
+extern "C" int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, int32_t out_on_device,
+                         pt_render_stats* stats)
+{
+    if (!s || !b || !d || !rgb_out) return PT_ERR_ARG;
+    pt_context* ctx = s->ctx;
+    if (b->scene != s) return fail(ctx, PT_ERR_ARG, "BVH belongs to another scene");
+    if (d->width <= 0 || d->height <= 0 || d->spp <= 0 || d->max_bounces < 0 ||
+        (d->integrator != PT_INTEGRATOR_MIS && d->integrator != PT_INTEGRATOR_SIMPLE))
+        return fail(ctx, PT_ERR_ARG, "bad render settings");
+    const int64_t P64 = (int64_t)d->width * d->height;
+    if (P64 > (1 << 30)) return fail(ctx, PT_ERR_ARG, "image too large");
+    const int P = (int)P64;
+    hipStream_t st = ctx->stream;
+    PT_HIP(ctx, hipSetDevice(ctx->device));
+
+    Buffers buf;
+    PathView ps;
+    Queue ext, any, clo;
+    float *hit_t, *sh_t, *out_d;
+    int *hit_k, *sh_k, *act0, *act1, *counters;
+    float4 *ext_rays, *any_rays, *clo_rays;
+    int32_t *ext_path, *any_path, *clo_path;
+#define ALLOC(p, n) PT_HIP(ctx, buf.alloc(&(p), (n)))
+    ALLOC(ps.rng, 3 * (size_t)P);
+    ALLOC(ps.sum, P);
+    ALLOC(ps.color, P);
+    ALLOC(ps.ray_o, P);
+    ALLOC(ps.ray_d, P);
+    ALLOC(ps.thr, P);
+    ALLOC(ps.meta, P);
+    ALLOC(ps.pixel, P);
+    ALLOC(ps.pend_a, P);
+    ALLOC(ps.pend_b, P);
+    ALLOC(ps.pend_c, P);
+    ALLOC(ps.pend_d, P);
+    ALLOC(ps.pend_o, P);
+    ALLOC(ps.pend_dir, P);
+    ALLOC(hit_t, P);
+    ALLOC(hit_k, P);
+    ALLOC(sh_t, P);
+    ALLOC(sh_k, P);
+    ALLOC(act0, P);
+    ALLOC(act1, P);
+    ALLOC(counters, 8);
+    ALLOC(ext_rays, 2 * (size_t)P);
+    ALLOC(any_rays, 2 * (size_t)P);
+    ALLOC(clo_rays, 2 * (size_t)P);
+    ALLOC(ext_path, P);
+    ALLOC(any_path, P);
+    ALLOC(clo_path, P);
+    if (out_on_device) out_d = rgb_out;
+    else ALLOC(out_d, 3 * (size_t)P);
+#undef ALLOC
+
+    SceneView sc;
+    sc.bvh = b->view;
+    sc.tri_mat = b->tri_mat_d;
+    sc.tri_mat_shading = b->tri_mat_shading_d;
+    sc.tri_smooth = b->tri_smooth_d;
+    sc.tri_nrm = b->tri_nrm_d;
+    sc.mats = s->mats_d;
+    sc.lights = s->lights_d;
+    sc.num_lights = s->num_lights;
+    sc.cam = s->cam;
+    sc.env = s->env;
+    JobView job;
+    job.width = d->width;
+    job.height = d->height;
+    job.spp = d->spp;
+    job.max_bounces = d->max_bounces;
+    job.integrator = d->integrator;
+    job.seed = d->seed;
+    job.out_offset = 0;
+
+    // counter sets: [0..3] and [4..7] = ext, any-hit shadow, closest-hit shadow, live paths
+    auto queues_for = [&](int set, Queue* e, Queue* a, Queue* c) {
+        e->rays = ext_rays;
+        e->path = ext_path;
+        e->count = counters + 4 * set + 0;
+        a->rays = any_rays;
+        a->path = any_path;
+        a->count = counters + 4 * set + 1;
+        c->rays = clo_rays;
+        c->path = clo_path;
+        c->count = counters + 4 * set + 2;
+    };
+    hipEvent_t ev0, ev1;
+    PT_HIP(ctx, hipEventCreate(&ev0));
+    PT_HIP(ctx, hipEventCreate(&ev1));
+    PT_HIP(ctx, launch_zero(counters, 8, st));
+    queues_for(1, &ext, &any, &clo);
+    PT_HIP(ctx, hipEventRecord(ev0, st));
+    PT_HIP(ctx, launch_paths_init(sc, job, ps, ext, 0, P, act0, st));
+    const int p_host = P;
+    PT_HIP(ctx, hipMemcpyAsync(counters + 7, &p_host, sizeof(int), hipMemcpyHostToDevice, st));
+    int h[4] = {P, 0, 0, P};
+    int64_t queries = 0, iters = 0;
+    int* act_in = act0;
+    int* act_out = act1;
+    int prev = 1;
+    while (h[3] > 0) {
+        Queue pe, pa, pc;
+        queues_for(prev, &pe, &pa, &pc);
+        PT_HIP(ctx, launch_trace_queue(b->view, pe, h[0], false, hit_t, hit_k, st));
+        PT_HIP(ctx, launch_trace_queue(b->view, pa, h[1], true, sh_t, sh_k, st));
+        PT_HIP(ctx, launch_trace_queue(b->view, pc, h[2], false, sh_t, sh_k, st));
+        queries += (int64_t)h[0] + h[1] + h[2];
+        const int cur = 1 - prev;
+        PT_HIP(ctx, launch_zero(counters + 4 * cur, 4, st));
+        Queue ne, na, nc;
+        queues_for(cur, &ne, &na, &nc);
+        PT_HIP(ctx, launch_shade(d->integrator, sc, job, ps, act_in, counters + 4 * prev + 3, h[3], act_out,
+                                 counters + 4 * cur + 3, ne, na, nc, out_d, hit_t, hit_k, sh_t, sh_k, st));
+        PT_HIP(ctx, hipMemcpyAsync(h, counters + 4 * cur, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+        PT_HIP(ctx, hipStreamSynchronize(st));
+        std::swap(act_in, act_out);
+        prev = cur;
+        ++iters;
+    }
+    PT_HIP(ctx, hipEventRecord(ev1, st));
+    PT_HIP(ctx, hipEventSynchronize(ev1));
+    float ms = 0.0f;
+    PT_HIP(ctx, hipEventElapsedTime(&ms, ev0, ev1));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (!out_on_device)
+        PT_HIP(ctx, hipMemcpy(rgb_out, out_d, 3 * sizeof(float) * (size_t)P, hipMemcpyDeviceToHost));
+    if (stats) {
+        stats->queries = queries;
+        stats->iterations = iters;
+        stats->elapsed_ms = ms;
+    }
+    return PT_OK;
+}

@@ -1,0 +1,621 @@
+// pt_device.h — per-ray arithmetic of the render path, for gfx950 kernels and the
+// host builder.
+//
+// Every function restates a reference function (cited file:line, paths relative to
+// the reference's LibPetrichor/) with the same float evaluation order; translation
+// units including this are built with -ffp-contract=off and correctly rounded
+// f32 division/sqrt, so results are bit-identical to the x86-64 reference build.
+// Calls the reference makes to the double C library (sin/cos/acos/atan2 and a few
+// sqrt whose result is not stored straight to a float) go through pt_libm.h.
+#pragma once
+
+#include "pt_libm.h"
+
+#include <float.h>
+#include <stdint.h>
+
+namespace pt {
+
+#define PT_EPS 1.0e-3f                  /* kEps, Core/Constants.h:8 */
+#define PT_INFINITY (0.9f * FLT_MAX)    /* kInfinity, Core/Constants.h:10 */
+#define PT_PI 3.14159265358979f         /* Math::kPi, Core/Constants.h:14 */
+#define PT_INV_PI (1.0f / 3.14159265358979f)
+
+struct V3 {
+    float x, y, z;
+};
+
+PT_HD V3 v3(float x, float y, float z)
+{
+    V3 r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    return r;
+}
+// Math/Vector3f.h:236-357 operators, element by element.
+PT_HD V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+PT_HD V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+PT_HD V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+PT_HD V3 operator*(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+PT_HD V3 operator*(V3 a, float c) { return v3(a.x * c, a.y * c, a.z * c); }
+PT_HD V3 operator*(float c, V3 a) { return v3(a.x * c, a.y * c, a.z * c); }
+PT_HD V3 operator/(V3 a, float c) { return v3(a.x / c, a.y / c, a.z / c); }
+PT_HD V3 operator/(V3 a, V3 b) { return v3(a.x / b.x, a.y / b.y, a.z / b.z); }
+PT_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; } // Vector3f.h:128-132
+PT_HD V3 cross(V3 a, V3 b) // Vector3f.h:134-140
+{
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+PT_HD float sq_length(V3 a) { return dot(a, a); }
+PT_HD float length(V3 a) { return __builtin_sqrtf(dot(a, a)); }
+PT_HD V3 normalized(V3 a) { return a / length(a); }
+PT_HD float smin(float a, float b) { return (b < a) ? b : a; } // std::min
+PT_HD float smax(float a, float b) { return (a < b) ? b : a; } // std::max
+PT_HD float min_elem(V3 a) { return smin(a.x, smin(a.y, a.z)); }
+PT_HD float fabsf_(float x) { return __builtin_fabsf(x); }
+PT_HD float copysignf_(float m, float s) { return __builtin_copysignf(m, s); }
+PT_HD float floorf_(float x) { return __builtin_floorf(x); }
+PT_HD float comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+PT_HD V3 reflected(V3 d, V3 n) { return d - n * 2.0f * dot(d, n); } // Vector3f.h:203-208
+PT_HD float pow2(float x) { return x * x; }
+PT_HD float pow5(float x) { return x * (pow2(x) * pow2(x)); }          // MathUtils.h:38-56
+PT_HD float fmodp(float x, float m) { return x - m * floorf_(x / m); } // MathUtils.h:30-35
+PT_HD float lerpf(float a, float b, float t) { return (1.0f - t) * a + t * b; }
+
+// ---- XorShift128 + samplers (Random/XorShift.h:36-52, Core/Sampler) ----------
+struct XorShift {
+    uint32_t x, y, z, w;
+};
+PT_HD XorShift xorshift_seeded(uint32_t seed)
+{
+    XorShift s;
+    s.x = 123456789u;
+    s.y = 362436069u;
+    s.z = 521288629u;
+    s.w = seed;
+    return s;
+}
+PT_HD uint32_t xs_random(XorShift& s)
+{
+    const uint32_t t = s.x ^ (s.x << 11);
+    s.x = s.y;
+    s.y = s.z;
+    s.z = s.w;
+    s.w = (s.w ^ (s.w >> 19)) ^ (t ^ (t >> 8));
+    return s.w;
+}
+// random() / float(UINT_MAX) == (float)u32 / 2^32
+PT_HD float xs_next(XorShift& s) { return (float)xs_random(s) / 4294967296.0f; }
+
+// Sampler state of one path: the 1D stream and the two streams of the 2D sampler.
+struct Samplers {
+    XorShift s1, s2x, s2y;
+};
+PT_HD float next1(Samplers& s) { return xs_next(s.s1); }
+PT_HD void next2(Samplers& s, float* a, float* b)
+{
+    *a = xs_next(s.s2x);
+    *b = xs_next(s.s2y);
+}
+
+// Per-pixel seeds (this build's definition; the reference seeds per tile):
+// MurmurHash3 fmix32 over (job seed, pixel index, stream).
+PT_HD uint32_t fmix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+PT_HD uint32_t pixel_seed(uint32_t job_seed, uint32_t pixel, uint32_t stream)
+{
+    return fmix32(fmix32(fmix32(job_seed) ^ pixel) + 0x9e3779b9u * (stream + 1u));
+}
+PT_HD Samplers samplers_for_pixel(uint32_t job_seed, uint32_t pixel)
+{
+    Samplers s;
+    s.s1 = xorshift_seeded(pixel_seed(job_seed, pixel, 0));
+    s.s2x = xorshift_seeded(pixel_seed(job_seed, pixel, 1));
+    s.s2y = xorshift_seeded(pixel_seed(job_seed, pixel, 2));
+    return s;
+}
+
+// ---- rays ----------------------------------------------------------------------
+struct Ray { // Core/Ray.h:20-49
+    V3 o, dir, throughput;
+    int bounce;
+    float prob, ior;
+};
+PT_HD Ray make_ray(V3 o, V3 d)
+{
+    Ray r;
+    r.o = o;
+    r.dir = d;
+    r.throughput = v3(1.0f, 1.0f, 1.0f);
+    r.bounce = 0;
+    r.prob = 1.0f;
+    r.ior = 1.0f;
+    return r;
+}
+
+// ---- camera (Core/Camera.cpp:52-91) --------------------------------------------
+struct CameraDev {
+    V3 pos, right, up, forward;
+    float focus, h_perf, aperture;
+};
+
+PT_HD Ray generate_ray(const CameraDev& c, int i, int j, int w, int h, Samplers& s)
+{
+    float r0, r1;
+    next2(s, &r0, &r1);
+    const float aspect = (float)w / h;
+    const float u = (i + r0) / w - 0.5f;
+    const float v = (j + r1) / h - 0.5f;
+    const V3 sensor = c.pos + c.right * u * c.h_perf * c.focus * aspect + -c.up * v * c.h_perf * c.focus +
+                      c.forward * c.focus;
+    next2(s, &r0, &r1);
+    const float r = __builtin_sqrtf(r0);
+    const float theta = 2.0f * PT_PI * r1;
+    double st, ct;
+    pt_sincos((double)theta, &st, &ct);
+    const V3 lens = c.pos + c.right * 0.5f * c.aperture * r * (float)ct + -c.up * 0.5f * c.aperture * r * (float)st;
+    Ray ray = make_ray(lens, normalized(sensor - lens));
+    const float d = dot(ray.dir, c.forward);
+    ray.throughput = v3(1.0f, 1.0f, 1.0f) * d * d * c.focus * c.focus / sq_length(sensor - lens);
+    return ray;
+}
+
+// ---- triangles (Core/Geometry/Triangle.cpp) -------------------------------------
+// Triangle record: p0, e1 = p1 - p0, e2 = p2 - p0 and n = cross(e1, e2), the values
+// Triangle::Intersect forms first (Triangle.cpp:57-60), stored precomputed.
+struct TriDev {
+    V3 p0, e1, e2, n;
+};
+
+PT_HD TriDev tri_from_points(V3 p0, V3 p1, V3 p2)
+{
+    TriDev t;
+    t.p0 = p0;
+    t.e1 = p1 - p0;
+    t.e2 = p2 - p0;
+    t.n = cross(t.e1, t.e2);
+    return t;
+}
+
+// Triangle::Intersect (Triangle.cpp:51-88): returns true and the distance on a hit.
+PT_HD bool tri_intersect(const TriDev& tr, V3 o, V3 d, float* dist_out)
+{
+    const float inv_det = 1.0f / dot(-d, tr.n);
+    const V3 vec = o - tr.p0;
+    const float w1 = dot(vec, cross(d, tr.e2)) * inv_det;
+    const float w2 = dot(vec, cross(tr.e1, d)) * inv_det;
+    if (w1 < 0.0f || w1 >= 1.0f) return false;
+    if (w2 < 0.0f || w2 >= 1.0f) return false;
+    if (w1 + w2 >= 1.0f) return false;
+    const float dist = dot(vec, tr.n) * inv_det;
+    if (dist < 0.0f) return false;
+    *dist_out = dist;
+    return true;
+}
+
+// Triangle::Interpolate (Triangle.cpp:90-153): position and shading normal.
+PT_HD void tri_shading(const TriDev& tr, bool smooth, V3 n0, V3 n1, V3 n2, V3 o, V3 d, float t, V3* pos,
+                       V3* normal)
+{
+    *pos = o + d * t;
+    if (!smooth) {
+        *normal = normalized(tr.n);
+    } else {
+        const float inv_det = 1.0f / dot(-d, tr.n);
+        const V3 vec = o - tr.p0;
+        const float w1 = dot(vec, cross(d, tr.e2)) * inv_det;
+        const float w2 = dot(vec, cross(tr.e1, d)) * inv_det;
+        const V3 n = w1 * (n1 - n0) + w2 * (n2 - n0) + n0;
+        *normal = n / length(n);
+    }
+}
+
+// Triangle::SampleSurface (Triangle.cpp:155-182) with e0 = e1, e1 = e2 of the record.
+PT_HD void tri_sample(const TriDev& tr, Samplers& s, V3* pos, V3* normal, float* pdf_area)
+{
+    float u0, u1;
+    next2(s, &u0, &u1);
+    const float su = __builtin_sqrtf(u0);
+    *pos = tr.p0 + (1.0f - su) * tr.e1 + su * u1 * tr.e2;
+    *normal = normalized(tr.n);
+    *pdf_area = 1.0f / (0.5f * length(tr.n));
+}
+
+// ---- materials (Core/Material) ----------------------------------------------------
+enum { kMatLambert = 0, kMatGlass = 1, kMatEmission = 2, kMatGlossy = 3, kMatMix = 4 };
+
+struct MaterialDev {
+    int type;
+    V3 color;     // Lambert kd, Glass colour, Emission Le, GGX reflectance
+    float alpha;  // GGX roughness^2 (GGX.cpp:16-20)
+    float ior;    // Glass
+    float mix;    // Mix weight
+    int mat0, mat1;
+    int is_enabled;
+};
+
+// GGX::GetF0 (GGX.h:126-139), metalness 0, base colour 1, no map.
+PT_HD V3 ggx_f0(const MaterialDev& m)
+{
+    return 0.16f * (m.color * m.color) * (1.0f - 0.0f) + v3(1.0f, 1.0f, 1.0f) * 0.0f;
+}
+
+// GGX::Lambda (GGX.cpp:141-153); the final expression is evaluated in double.
+PT_HD float ggx_lambda(V3 dir, V3 half, float alpha)
+{
+    const float alpha2 = alpha * alpha;
+    const float c = fabsf_(dot(dir, half));
+    const float tan2 = c > 0 ? 1.0f / (c * c) - 1.0f : PT_INFINITY;
+    const float arg = 1.0f + alpha2 * tan2;
+    return (float)((double)-0.5f + (double)0.5f * pt_sqrt_d((double)arg));
+}
+
+// Material::BxDF (Lambert.cpp:27-33, GGX.cpp:22-57, Glass.cpp:11-19).
+PT_HD V3 bxdf(const MaterialDev& m, V3 din, V3 dout, V3 normal)
+{
+    if (m.type == kMatLambert) return m.color * v3(1.0f, 1.0f, 1.0f) * PT_INV_PI;
+    if (m.type == kMatGlossy) {
+        const V3 half = normalized(-din + dout);
+        const float hn = fabsf_(dot(half, normal));
+        const float alpha = m.alpha;
+        const float alpha2 = alpha * alpha;
+        const float hn2 = hn * hn;
+        const float k = (alpha2 - 1.0f) * hn2 + 1.0f;
+        const float d_term = alpha2 / (PT_PI * k * k);
+        const float li = ggx_lambda(dout, half, alpha);
+        const float lo = ggx_lambda(-din, half, alpha);
+        const float g_term = 1.0f / (1.0f + li + lo);
+        const float hl = fabsf_(dot(half, dout));
+        const V3 f0 = ggx_f0(m);
+        const V3 f_term = f0 + (v3(1.0f, 1.0f, 1.0f) - f0) * pow5(1.0f - hl);
+        const float ln = fabsf_(dot(dout, normal));
+        const float vn = fabsf_(dot(-din, normal));
+        return d_term * g_term * f_term / (4.0f * ln * vn);
+    }
+    return v3(0.0f, 0.0f, 0.0f);
+}
+
+// Material::PDF (Lambert.cpp:35-49, GGX.cpp:59-76, Glass.cpp:21-27).
+PT_HD float bsdf_pdf(const MaterialDev& m, V3 din, V3 dout, V3 normal)
+{
+    if (m.type == kMatLambert) {
+        if (m.is_enabled) return fabsf_(dot(dout, normal)) * PT_INV_PI;
+        return 0.5f * PT_INV_PI;
+    }
+    if (m.type == kMatGlossy) {
+        const V3 half = normalized(-din + dout);
+        const float hn = fabsf_(dot(half, normal));
+        const float alpha2 = m.alpha * m.alpha;
+        const float hn2 = hn * hn;
+        const float tmp = 1.0f - (1.0f - alpha2) * hn2;
+        return tmp > 0 ? PT_INV_PI * alpha2 / (tmp * tmp) : PT_INFINITY;
+    }
+    return 0.0f;
+}
+
+// GGX::SampleGGXVNDF (GGX.cpp:155-208), Heitz 2017.
+PT_HD V3 ggx_sample_vndf(const MaterialDev& m, V3 view, V3 n0, Samplers& s)
+{
+    const V3 normal = n0 * copysignf_(1.0f, dot(view, n0));
+    const V3 t = (fabsf_(normal.z) < 0.9999f) ? cross(normal, v3(0.0f, 0.0f, 1.0f)) : v3(1.0f, 0.0f, 0.0f);
+    const V3 bw = normalized(normal);
+    const V3 bu = normalized(t);
+    const V3 bv = cross(bw, bu);
+    V3 v = v3(dot(view, bu), dot(view, bv), dot(view, bw));
+    const float alpha = m.alpha;
+    v = v * v3(alpha, alpha, 1.0f);
+    v = v / length(v);
+    const V3 t1 = (v.z < 0.9999f) ? normalized(cross(v, v3(0.0f, 0.0f, 1.0f))) : v3(1.0f, 0.0f, 0.0f);
+    const V3 t2 = cross(t1, v);
+    float u0, u1;
+    next2(s, &u0, &u1);
+    const float a = 1.0f / (1.0f + v.z);
+    const float r = __builtin_sqrtf(u0);
+    const float phi = (u1 < a) ? u1 / a * PT_PI : PT_PI + (u1 - a) / (1.0f - a) * PT_PI;
+    double sp, cp;
+    pt_sincos((double)phi, &sp, &cp);
+    const float p1 = (float)((double)r * cp);
+    const float p2 = (float)((double)r * sp * (double)((u1 < a) ? 1.0f : v.z));
+    V3 n = p1 * t1 + p2 * t2 + __builtin_sqrtf(smax(0.0f, 1.0f - p1 * p1 - p2 * p2)) * v;
+    n = v3(alpha * n.x, alpha * n.y, smax(0.0f, n.z));
+    n = bu * n.x + bv * n.y + bw * n.z;
+    return n / length(n);
+}
+
+// Material::CreateNextRay (Lambert.cpp:51-102, GGX.cpp:78-139, Glass.cpp:29-95).
+PT_HD Ray next_ray(const MaterialDev& m, const Ray& in, V3 pos, V3 sn, Samplers& s)
+{
+    Ray out = make_ray(v3(0.0f, 0.0f, 0.0f), v3(0.0f, 0.0f, 0.0f));
+    if (m.type == kMatLambert) {
+        const V3 normal = sn * copysignf_(1.0f, -dot(in.dir, sn));
+        // OrthonormalBasis::Build(w) (OrthonormalBasis.cpp:10-27)
+        const V3 w = normalized(normal);
+        const float sign = copysignf_(1.0f, w.z);
+        const float a = -1.0f / (sign + w.z);
+        const float b = w.x * w.y * a;
+        const V3 u = v3(1.0f + sign * w.x * w.x * a, sign * b, -sign * w.x);
+        const V3 vv = v3(b, sign + w.y * w.y * a, -w.y);
+        float r0, r1;
+        next2(s, &r0, &r1);
+        out.o = pos + PT_EPS * normal;
+        out.bounce = in.bounce + 1;
+        if (m.is_enabled) {
+            const float phi = 2.0f * PT_PI * r1;
+            const float sin_theta = __builtin_sqrtf(r0);
+            double sp, cp;
+            pt_sincos((double)phi, &sp, &cp);
+            const float cu = (float)((double)sin_theta * cp);
+            const float cv = (float)((double)sin_theta * sp);
+            const float cw = __builtin_sqrtf(1.0f - sin_theta * sin_theta);
+            out.dir = cu * u + cv * vv + cw * w;
+            out.throughput = in.throughput * v3(1.0f, 1.0f, 1.0f);
+        } else {
+            const float cos_theta = r0;
+            const float sin_theta = __builtin_sqrtf(1.0f - cos_theta * cos_theta);
+            const float phi = 2.0f * PT_PI * r1;
+            double sp, cp;
+            pt_sincos((double)phi, &sp, &cp);
+            const float cu = (float)((double)sin_theta * cp);
+            const float cv = (float)((double)sin_theta * sp);
+            out.dir = cu * u + cv * vv + cos_theta * w;
+            out.throughput = in.throughput;
+            const V3 f = bxdf(m, in.dir, out.dir, sn);
+            out.throughput = out.throughput * (f * cos_theta / bsdf_pdf(m, in.dir, out.dir, sn));
+        }
+        return out;
+    }
+    if (m.type == kMatGlossy) {
+        const V3 normal = sn * copysignf_(1.0f, -dot(in.dir, sn));
+        const V3 h = ggx_sample_vndf(m, -in.dir, sn, s);
+        const V3 dir = reflected(in.dir, h);
+        const float hl = fabsf_(dot(h, dir));
+        const V3 f0 = ggx_f0(m);
+        const V3 f_term = f0 + (v3(1.0f, 1.0f, 1.0f) - f0) * pow5(1.0f - hl);
+        out.o = pos;
+        out.dir = dir;
+        out.throughput = in.throughput;
+        out.bounce = in.bounce + 1;
+        const float li = ggx_lambda(dir, h, m.alpha);
+        const float lo = ggx_lambda(-in.dir, h, m.alpha);
+        const float g2 = 1.0f / (1.0f + li + lo);
+        const float g1 = 1.0f / (1.0f + li);
+        out.throughput = out.throughput * (f_term * g2 / g1);
+        out.o = out.o + PT_EPS * normal;
+        return out;
+    }
+    if (m.type == kMatGlass) {
+        const V3 normal = sn * copysignf_(1.0f, -dot(in.dir, sn));
+        const float eta = m.ior / in.ior;
+        // Vector3f::Refracted (Vector3f.h:210-234) with the un-flipped normal
+        const float dn = dot(in.dir, sn);
+        const float disc = pow2(dn) - 1 + pow2(eta);
+        if (disc < 0) {
+            out.o = pos + PT_EPS * normal;
+            out.dir = normalized(reflected(in.dir, normal));
+            out.throughput = v3(0.0f, 0.0f, 0.0f);
+            out.bounce = 0 + 1;
+            out.prob = in.prob;
+            out.ior = in.ior;
+            return out;
+        }
+        const float k = (float)((double)(-dn) + pt_sqrt_d((double)disc));
+        const V3 rd = (1.0f / eta) * (in.dir - sn * k);
+        out.dir = normalized(rd);
+        const float ndo = dot(sn, out.dir);
+        const float f0 = pow2((1.0f - eta) / (1.0f + eta));
+        const float refl = f0 + (1.0f - f0) * pow5(1.0f - fabsf_(ndo));
+        float rx, ry;
+        next2(s, &rx, &ry);
+        out.throughput = in.throughput;
+        out.bounce = in.bounce + 1;
+        out.prob = in.prob;
+        if (rx <= refl) {
+            out.o = pos + PT_EPS * normal; // keeps the refracted direction (Glass.cpp:70-80)
+            out.ior = in.ior;
+            return out;
+        }
+        out.o = pos - PT_EPS * normal;
+        out.ior = (ndo < 0) ? 1.0f * eta : 1.0f / eta;
+        return out;
+    }
+    return in;
+}
+
+// ---- environment (Core/Environment.cpp) -------------------------------------------
+struct EnvDev {
+    const float* rgb; // w*h RGB texels or null
+    int w, h;
+    V3 base;
+    float rot;
+    int use_is;
+    const float* pdf1d; // h entries
+    const float* cdf1d; // h entries
+};
+
+PT_HD bool env_valid(const EnvDev& e) { return e.rgb != nullptr && e.w > 0 && e.h > 0; }
+PT_HD bool env_use_is(const EnvDev& e) { return env_valid(e) && e.use_is; }
+
+PT_HD int clampi(int v, int lo, int hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+// Texture2D::GetPixel(float, float, Bilinear) (Texture2D.cpp:287-317)
+PT_HD V3 tex_texel(const float* rgb, int w, int i, int j)
+{
+    const float* p = rgb + 3 * ((int64_t)w * j + i);
+    return v3(p[0], p[1], p[2]);
+}
+PT_HD V3 tex_bilinear(const float* rgb, int w, int h, float x, float y)
+{
+    int ix0 = (int)x;
+    int ix1 = (int)(x + 1.0f);
+    int iy0 = (int)y;
+    int iy1 = (int)(y + 1.0f);
+    const float wx0 = x - ix0;
+    const float wy0 = y - iy0;
+    const float wx1 = ix1 - x;
+    const float wy1 = iy1 - y;
+    ix0 = clampi(ix0, 0, w - 1);
+    ix1 = clampi(ix1, 0, w - 1);
+    iy0 = clampi(iy0, 0, h - 1);
+    iy1 = clampi(iy1, 0, h - 1);
+    const V3 cm0 = wx1 * tex_texel(rgb, w, ix0, iy0) + wx0 * tex_texel(rgb, w, ix1, iy0);
+    const V3 cm1 = wx1 * tex_texel(rgb, w, ix0, iy1) + wx0 * tex_texel(rgb, w, ix1, iy1);
+    return wy1 * cm0 + wy0 * cm1;
+}
+// .x channel of the bilinear fetch of a never-loaded texture: every texel reads as
+// (1, 0, 1) (Texture2D.cpp:263-266), so only the weights' arithmetic remains.
+PT_HD float invalid_tex_bilinear_x(float x, float y)
+{
+    const int ix0 = (int)x;
+    const int ix1 = (int)(x + 1.0f);
+    const int iy0 = (int)y;
+    const int iy1 = (int)(y + 1.0f);
+    const float wx0 = x - ix0;
+    const float wy0 = y - iy0;
+    const float wx1 = ix1 - x;
+    const float wy1 = iy1 - y;
+    const float cm0 = wx1 * 1.0f + wx0 * 1.0f;
+    const float cm1 = wx1 * 1.0f + wx0 * 1.0f;
+    return wy1 * cm0 + wy0 * cm1;
+}
+
+// Environment::GetColor (Environment.cpp:29-48)
+PT_HD V3 env_color(const EnvDev& e, V3 dir)
+{
+    if (!env_valid(e)) return e.base;
+    const float theta = (float)pt_acos((double)dir.z);
+    const float phi = (float)pt_atan2((double)dir.y, (double)dir.x);
+    float u = 1.0f - (phi - e.rot) * 0.5f * PT_INV_PI;
+    float v = theta * PT_INV_PI;
+    u = fmodp(u, 1.0f);
+    v = fmodp(v, 1.0f);
+    u = fmodp(u, 1.0f); // Texture2D::GetPixelByUV wraps again (Texture2D.cpp:327-335)
+    v = fmodp(v, 1.0f);
+    return e.base * tex_bilinear(e.rgb, e.w, e.h, u * e.w, v * e.h);
+}
+
+PT_HD float env_pdf1(const EnvDev& e, int i) { return e.pdf1d[i < e.h - 1 ? i : e.h - 1]; }
+
+// Environment::ImportanceSampling (Environment.cpp:130-220). m_cdf2D / m_pdf2D are
+// never-loaded textures, so their fetches go through invalid_tex_bilinear_x.
+PT_HD V3 env_sample(const EnvDev& e, Samplers& s, float* pdf_xy)
+{
+    if (!env_use_is(e)) return v3(1.0f, 0.0f, 0.0f);
+    const float texel_w = 1.0f / e.w;
+    const float texel_h = 1.0f / e.h;
+    float r0, r1;
+    next2(s, &r0, &r1);
+    float v0 = 0.0f, v1 = 1.0f;
+    for (;;) {
+        const float v = 0.5f * (v0 + v1);
+        const float y = v * e.h;
+        const int fy = (int)y;
+        const int cy = fy + 1 < e.h - 1 ? fy + 1 : e.h - 1;
+        const float c = lerpf(e.cdf1d[fy], e.cdf1d[cy], y - fy);
+        const float diff = c - r0;
+        if (v1 - v0 <= texel_h) {
+            v0 = v1 = v;
+            break;
+        }
+        if (diff > 0) v1 = v;
+        else if (diff < 0) v0 = v;
+        else {
+            v0 = v1 = v;
+            break;
+        }
+    }
+    float u0 = 0.0f, u1 = 1.0f;
+    for (;;) {
+        const float u = 0.5f * (u0 + u1);
+        const float uu = fmodp(u, 1.0f), vv = fmodp(v0, 1.0f);
+        const float c = invalid_tex_bilinear_x(uu * e.w, vv * e.h);
+        const float diff = c - r1;
+        if (u1 - u0 <= texel_w) {
+            u0 = u1 = u;
+            break;
+        }
+        if (diff >= 0) u1 = u;
+        else u0 = u;
+    }
+    {
+        const float y = v0 * e.h;
+        const int y0 = (int)y;
+        const int y1 = y0 + 1 < e.h - 1 ? y0 + 1 : e.h - 1;
+        const float py = lerpf(env_pdf1(e, y0), env_pdf1(e, y1), y - y0);
+        const float uu = fmodp(u0, 1.0f), vv = fmodp(v0, 1.0f);
+        const float px = invalid_tex_bilinear_x(uu * e.w, vv * e.h);
+        *pdf_xy = py * px;
+    }
+    const float theta = v0 * PT_PI;
+    const float phi = 2.0f * PT_PI * (1.0f - u0) + e.rot;
+    // OrthonormalBasis::Build(UnitZ, UnitX) + GetDir (OrthonormalBasis.cpp:29-43)
+    const V3 bw = normalized(v3(0.0f, 0.0f, 1.0f));
+    const V3 bu = normalized(v3(1.0f, 0.0f, 0.0f));
+    const V3 bv = cross(bw, bu);
+    double st, ct, sp, cp;
+    pt_sincos((double)theta, &st, &ct);
+    pt_sincos((double)phi, &sp, &cp);
+    const float cx = (float)(st * cp);
+    const float cy = (float)(st * sp);
+    const float cz = (float)ct;
+    return cx * bu + cy * bv + cz * bw;
+}
+
+// Environment::GetImportanceSamplingPDF (Environment.cpp:222-244)
+PT_HD float env_pdf(const EnvDev& e, V3 dir)
+{
+    if (!env_use_is(e)) return 1.0f;
+    const float theta = (float)pt_acos((double)dir.z);
+    const float phi = (float)pt_atan2((double)dir.y, (double)dir.x);
+    const float u0 = 1.0f - (phi - e.rot) * 0.5f * PT_INV_PI;
+    const float v0 = theta * PT_INV_PI;
+    const float y = v0 * e.h;
+    const int y0 = (int)y;
+    const int y1 = y0 + 1 < e.h - 1 ? y0 + 1 : e.h - 1;
+    const float py = lerpf(env_pdf1(e, y0), env_pdf1(e, y1), y - y0);
+    const float uu = fmodp(u0, 1.0f), vv = fmodp(v0, 1.0f);
+    const float px = invalid_tex_bilinear_x(uu * e.w, vv * e.h);
+    return py * px;
+}
+
+// ---- BVH slab test (Core/Accel/BinnedSAHBVH.h:95-170) -------------------------------
+// Box given as lower/upper; inv = 1/dir, sgn[i] = inv[i] < 0. Returns the node distance
+// (tMin if 0 <= tMin, else tMax if 0 <= tMax) or false on a miss.
+PT_HD bool slab(V3 o, V3 inv, int sx, int sy, int sz, V3 lo, V3 hi, float* dist)
+{
+    float tmin = ((sx ? hi.x : lo.x) - o.x) * inv.x;
+    float tmax = ((sx ? lo.x : hi.x) - o.x) * inv.x;
+    const float tymin = ((sy ? hi.y : lo.y) - o.y) * inv.y;
+    const float tymax = ((sy ? lo.y : hi.y) - o.y) * inv.y;
+    if (tymax < tmin || tmax < tymin) return false;
+    if (tmin < tymin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    const float tzmin = ((sz ? hi.z : lo.z) - o.z) * inv.z;
+    const float tzmax = ((sz ? lo.z : hi.z) - o.z) * inv.z;
+    if (tzmax < tmin || tmax < tzmin) return false;
+    if (tmin < tzmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    float d = FLT_MAX;
+    const bool near_ok = (0.0f <= tmin && tmin < d);
+    if (near_ok) d = tmin;
+    const bool far_ok = (0.0f <= tmax && tmax < d);
+    if (far_ok) d = tmax;
+    if (!(near_ok || far_ok)) return false;
+    *dist = d;
+    return true;
+}
+
+// AABB::Contains (AABB.h:58-68): strict inequalities.
+PT_HD bool box_contains(V3 lo, V3 hi, V3 p)
+{
+    return lo.x < p.x && p.x < hi.x && lo.y < p.y && p.y < hi.y && lo.z < p.z && p.z < hi.z;
+}
+
+} // namespace pt

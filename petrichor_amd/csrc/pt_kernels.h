@@ -1,0 +1,82 @@
+// pt_kernels.h — device data views and kernel launchers of the render path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pt_device.h"
+
+// Traversal stack entries per ray. A walk holds at most one pending sibling per
+// tree level plus the current node, so trees up to PT_MAX_STACK - 2 levels deep
+// are accepted (checked when the BVH is built).
+#define PT_MAX_STACK 64
+
+namespace pt {
+
+// Binary BVH in device memory: inner nodes as 4 float4 (left box, right box, refs),
+// triangles in leaf order as 3 float4 (p0, e1, e2, n).
+struct BvhView {
+    const float4* nodes;
+    const float4* tris;
+    const int32_t* prim_ids; // leaf slot -> triangle index
+    V3 root_lo, root_hi;
+    int root_ref, root_cnt;  // root as a stack entry: inner node 0, or a leaf
+};
+
+struct SceneView {
+    BvhView bvh;
+    const int32_t* tri_mat;         // material per leaf slot (may be a Mix)
+    const int32_t* tri_mat_shading; // GetMaterial(0.0f) per leaf slot (Triangle.cpp:151)
+    const int32_t* tri_smooth;      // per leaf slot, or null
+    const float* tri_nrm;           // 9 floats per leaf slot, or null
+    const MaterialDev* mats;
+    const float4* lights;           // light triangles, AppendLight order, 3 float4 each
+    int num_lights;
+    CameraDev cam;
+    EnvDev env;
+};
+
+struct JobView {
+    int width, height, spp, max_bounces, integrator;
+    uint32_t seed;
+    int64_t out_offset; // first pixel of this job in the output buffer
+};
+
+// Per-path state, structure of arrays indexed by path slot.
+struct PathView {
+    uint4* rng;    // 3 per path: 1D stream, 2D x stream, 2D y stream
+    float4* sum;   // pixel sum
+    float4* color; // sample colour, w = sin of the last BSDF direction to the normal
+    float4* ray_o; // origin, w = prob
+    float4* ray_d; // direction, w = ior
+    float4* thr;   // throughput, w = BSDF pdf of the last sampled direction
+    int4* meta;    // sample index, ray bounce, phase | pending << 8, loop bounce
+    int32_t* pixel;
+    float4* pend_a; // pending shadow data (see k_shade)
+    float4* pend_b;
+    float4* pend_c;
+    float4* pend_d;
+    float4* pend_o;
+    float4* pend_dir;
+};
+
+struct Queue {
+    float4* rays; // 2 per entry: (o, dist_min), (d, dist_max)
+    int32_t* path;
+    int* count;
+};
+
+hipError_t launch_trace_closest(const BvhView& b, int64_t n, const float* rays, float* t_out, int32_t* prim_out,
+                                unsigned long long* stats, hipStream_t st);
+hipError_t launch_trace_any(const BvhView& b, int64_t n, const float* rays, int32_t* hit_out, hipStream_t st);
+hipError_t launch_zero(int* counters, int n, hipStream_t st);
+hipError_t launch_paths_init(const SceneView& sc, const JobView& job, const PathView& ps, const Queue& ext,
+                             int path_base, int num_paths, int* active, hipStream_t st);
+hipError_t launch_trace_queue(const BvhView& b, const Queue& q, int max_n, bool any, float* hit_t, int* hit_k,
+                              hipStream_t st);
+hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job, const PathView& ps,
+                        const int* active_in, const int* active_count, int max_active, int* active_out,
+                        int* active_out_count, const Queue& ext, const Queue& sh_any, const Queue& sh_closest,
+                        float* out_rgb, const float* hit_t, const int* hit_k, const float* sh_t, const int* sh_k,
+                        hipStream_t st);
+
+} // namespace pt

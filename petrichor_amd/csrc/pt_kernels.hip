@@ -1,0 +1,694 @@
+// pt_kernels.hip — gfx950 kernels of the render path: BVH traversal (closest / any
+// hit) and the wavefront path tracer (ray generation, shading with NEE + MIS,
+// queue compaction, accumulation).
+#include <hip/hip_runtime.h>
+
+#include "pt_device.h"
+#include "pt_kernels.h"
+
+namespace pt {
+
+// ---------------------------------------------------------------------------------
+// BVH traversal
+//
+// Binary tree in the reference's topology (BinnedSAHBVH.cpp:15-225) stored with both
+// child boxes in the parent (4 x 16 B per inner node). The walk keeps the reference's
+// order and culling (BinnedSAHBVH.cpp:272-374): children are pushed far-first by
+// squared distance from the origin to the child-box centroid; a popped child is
+// skipped when a hit exists, the origin is not strictly inside the child box and the
+// hit is nearer than the child's slab distance. Missing children are not pushed (the
+// reference pushes and then rejects them), which changes nothing but the work.
+// ---------------------------------------------------------------------------------
+
+struct StackEntry {
+    int ref;   // inner node index, or first leaf slot
+    int cnt;   // 0 = inner node, else leaf size; bit 30 set = origin inside the box
+    float d;   // slab distance of the child box
+};
+
+#define PT_INSIDE_BIT (1 << 30)
+
+template <bool ANY, bool STATS>
+__device__ __forceinline__ int traverse(const BvhView& b, V3 o, V3 d, float dmin, float dmax, float* t_out,
+                                        uint32_t* nv_out, uint32_t* nt_out)
+{
+    const V3 inv = v3(1.0f, 1.0f, 1.0f) / d;
+    const int sx = inv.x < 0.0f, sy = inv.y < 0.0f, sz = inv.z < 0.0f;
+    int best = -1;
+    float best_t = 0.0f;
+    uint32_t nv = 0, nt = 0;
+    StackEntry stack[PT_MAX_STACK];
+    int sp = 0;
+    float rd;
+    if (b.root_cnt < 0) return -1; // empty scene
+    if (STATS) ++nv;
+    if (!slab(o, inv, sx, sy, sz, b.root_lo, b.root_hi, &rd)) {
+        if (STATS) { *nv_out = nv; *nt_out = nt; }
+        return -1;
+    }
+    stack[sp].ref = b.root_ref;
+    stack[sp].cnt = b.root_cnt;
+    stack[sp].d = rd;
+    ++sp;
+    while (sp > 0) {
+        --sp;
+        const StackEntry e = stack[sp];
+        const int cnt = e.cnt & ~PT_INSIDE_BIT;
+        if (best >= 0 && !(e.cnt & PT_INSIDE_BIT)) {
+            if (best_t < e.d) continue;
+        }
+        if (cnt > 0) {
+            for (int k = e.ref; k < e.ref + cnt; ++k) {
+                if (STATS) ++nt;
+                const float4 r0 = b.tris[3 * k];
+                const float4 r1 = b.tris[3 * k + 1];
+                const float4 r2 = b.tris[3 * k + 2];
+                TriDev tr;
+                tr.p0 = v3(r0.x, r0.y, r0.z);
+                tr.e1 = v3(r0.w, r1.x, r1.y);
+                tr.e2 = v3(r1.z, r1.w, r2.x);
+                tr.n = v3(r2.y, r2.z, r2.w);
+                float t;
+                if (tri_intersect(tr, o, d, &t)) {
+                    if (t < dmin || t > dmax) continue;
+                    if (best < 0 || t < best_t) {
+                        best = k;
+                        best_t = t;
+                        if (ANY) {
+                            if (STATS) { *nv_out = nv; *nt_out = nt; }
+                            *t_out = best_t;
+                            return best;
+                        }
+                    }
+                }
+            }
+            continue;
+        }
+        // inner node: both child boxes and their references
+        const float4* nd = b.nodes + 4 * (size_t)e.ref;
+        const float4 a = nd[0], bb = nd[1], c = nd[2];
+        const int4 meta = *reinterpret_cast<const int4*>(nd + 3);
+        const V3 llo = v3(a.x, a.y, a.z), lhi = v3(a.w, bb.x, bb.y);
+        const V3 rlo = v3(bb.z, bb.w, c.x), rhi = v3(c.y, c.z, c.w);
+        if (STATS) nv += 2;
+        // AABB::CalcCentroid = 0.5 * (lower + upper) (AABB.h:70-74)
+        const float dl = sq_length(0.5f * (llo + lhi) - o);
+        const float dr = sq_length(0.5f * (rlo + rhi) - o);
+        float tl, tr_;
+        const bool hl = slab(o, inv, sx, sy, sz, llo, lhi, &tl);
+        const bool hr = slab(o, inv, sx, sy, sz, rlo, rhi, &tr_);
+        StackEntry el, er;
+        el.ref = meta.x;
+        el.cnt = meta.y | (box_contains(llo, lhi, o) ? PT_INSIDE_BIT : 0);
+        el.d = tl;
+        er.ref = meta.z;
+        er.cnt = meta.w | (box_contains(rlo, rhi, o) ? PT_INSIDE_BIT : 0);
+        er.d = tr_;
+        if (dl < dr) {
+            if (hr) stack[sp++] = er;
+            if (hl) stack[sp++] = el;
+        } else {
+            if (hl) stack[sp++] = el;
+            if (hr) stack[sp++] = er;
+        }
+    }
+    if (STATS) { *nv_out = nv; *nt_out = nt; }
+    *t_out = best_t;
+    return best;
+}
+
+// Public trace API: rays are 8 floats (o.xyz, dist_min, d.xyz, dist_max).
+template <bool STATS>
+__global__ void __launch_bounds__(256) k_trace_closest(BvhView b, int64_t n, const float4* __restrict__ rays,
+                                                       float* __restrict__ t_out, int32_t* __restrict__ prim_out,
+                                                       unsigned long long* stats)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+    float t;
+    uint32_t nv = 0, nt = 0;
+    const int k = traverse<false, STATS>(b, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w, r1.w, &t, &nv, &nt);
+    t_out[i] = k >= 0 ? t : __builtin_inff();
+    prim_out[i] = k >= 0 ? b.prim_ids[k] : -1;
+    if (STATS) {
+        atomicAdd(&stats[0], (unsigned long long)nv);
+        atomicAdd(&stats[1], (unsigned long long)nt);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_trace_any(BvhView b, int64_t n, const float4* __restrict__ rays,
+                                                   int32_t* __restrict__ hit_out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+    float t;
+    const int k = traverse<true, false>(b, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w, r1.w, &t, nullptr,
+                                        nullptr);
+    hit_out[i] = k >= 0 ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------------
+// Wavefront path tracer
+// ---------------------------------------------------------------------------------
+
+enum Phase { kPhCamera = 0, kPhBounce = 1, kPhWait = 2, kPhDone = 3 };
+enum Pending { kPendNone = 0, kPendEnv = 1, kPendArea = 2 };
+
+__device__ __forceinline__ V3 ld3(const float4 v) { return v3(v.x, v.y, v.z); }
+__device__ __forceinline__ float4 st3(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+__device__ __forceinline__ uint4 pack_xs(const XorShift& s) { return make_uint4(s.x, s.y, s.z, s.w); }
+__device__ __forceinline__ XorShift unpack_xs(uint4 v)
+{
+    XorShift s;
+    s.x = v.x;
+    s.y = v.y;
+    s.z = v.z;
+    s.w = v.w;
+    return s;
+}
+
+__device__ __forceinline__ TriDev load_tri(const float4* tris, int k)
+{
+    const float4 r0 = tris[3 * k], r1 = tris[3 * k + 1], r2 = tris[3 * k + 2];
+    TriDev t;
+    t.p0 = v3(r0.x, r0.y, r0.z);
+    t.e1 = v3(r0.w, r1.x, r1.y);
+    t.e2 = v3(r1.z, r1.w, r2.x);
+    t.n = v3(r2.y, r2.z, r2.w);
+    return t;
+}
+
+__device__ __forceinline__ MaterialDev load_mat(const SceneView& s, int m) { return s.mats[m]; }
+
+// GeometryBase::GetMaterial (GeometryBase.h:59-69): Mix picks one of two.
+__device__ __forceinline__ int material_of(const SceneView& s, int slot, float rnd)
+{
+    const int m = s.tri_mat[slot];
+    const MaterialDev& md = s.mats[m];
+    if (md.type == kMatMix) return md.mix <= rnd ? md.mat0 : md.mat1;
+    return m;
+}
+
+// Shading point of a hit (Triangle::Interpolate, Triangle.cpp:90-153) at leaf slot k.
+__device__ __forceinline__ void shade_point(const SceneView& s, int k, V3 o, V3 d, float t, V3* pos, V3* nrm)
+{
+    const TriDev tr = load_tri(s.bvh.tris, k);
+    const bool smooth = s.tri_smooth != nullptr && s.tri_smooth[k] != 0;
+    V3 n0 = v3(0.0f, 0.0f, 1.0f), n1 = n0, n2 = n0;
+    if (smooth) {
+        const float* q = s.tri_nrm + 9 * (size_t)k;
+        n0 = v3(q[0], q[1], q[2]);
+        n1 = v3(q[3], q[4], q[5]);
+        n2 = v3(q[6], q[7], q[8]);
+    }
+    tri_shading(tr, smooth, n0, n1, n2, o, d, t, pos, nrm);
+}
+
+// PathTracing::SampleLight (PathTracing.cpp:329-386). Returns false for the env slot.
+__device__ __forceinline__ bool sample_light(const SceneView& s, float rnd, Samplers& smp, bool want_env,
+                                             float* pdf_area, V3* pl_pos)
+{
+    const int L = s.num_lights;
+    int index;
+    if (!want_env) {
+        index = (int)(rnd * (float)(uint64_t)L);
+        index = min(index, (int)((uint64_t)L - 1));
+    } else {
+        index = (int)(rnd * (float)((uint64_t)L + 1));
+        index = min(index, L);
+        if (index >= L) return false;
+    }
+    float sum_area = 0.0f;
+    *pl_pos = v3(0.0f, 0.0f, 0.0f);
+    for (int i = 0; i < L; ++i) {
+        const TriDev tr = load_tri(s.lights, i);
+        V3 p, n;
+        float pa;
+        tri_sample(tr, smp, &p, &n, &pa);
+        sum_area += 1.0f / pa;
+        if (i == index) *pl_pos = p;
+    }
+    *pdf_area = 1.0f / sum_area;
+    return true;
+}
+
+// Wave-aggregated queue append: one atomic per wavefront, lanes get consecutive slots.
+__device__ __forceinline__ int queue_slot(int* counter, bool want)
+{
+    const unsigned long long mask = __ballot(want);
+    if (!want) return -1;
+    const int leader = __ffsll((long long)mask) - 1;
+    const int rank = __popcll(mask & ((1ull << (threadIdx.x & 63)) - 1ull));
+    int base = 0;
+    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader, 64);
+    return base + rank;
+}
+
+__device__ __forceinline__ void push_ray(const Queue& q, int slot, int path, V3 o, float dmin, V3 d, float dmax)
+{
+    q.rays[2 * slot] = make_float4(o.x, o.y, o.z, dmin);
+    q.rays[2 * slot + 1] = make_float4(d.x, d.y, d.z, dmax);
+    q.path[slot] = path;
+}
+
+// Start sample `s` of a path: Camera::GenerateRay (Camera.cpp:52-91).
+__device__ __forceinline__ Ray camera_ray(const SceneView& sc, const JobView& job, int pixel, Samplers& smp)
+{
+    const int x = pixel % job.width, y = pixel / job.width;
+    return generate_ray(sc.cam, x, y, job.width, job.height, smp);
+}
+
+// One-time set-up of every path of a job: sampler seeds, sums and the first camera ray.
+__global__ void __launch_bounds__(256) k_paths_init(SceneView sc, JobView job, PathView ps, Queue ext, int path_base,
+                                                    int num_paths, int* active)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < num_paths;
+    const int p = path_base + i;
+    Samplers smp;
+    Ray r;
+    if (live) {
+        smp = samplers_for_pixel(job.seed, (uint32_t)i);
+        r = camera_ray(sc, job, i, smp);
+        ps.sum[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ps.color[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ps.ray_o[p] = st3(r.o, r.prob);
+        ps.ray_d[p] = st3(r.dir, r.ior);
+        ps.thr[p] = st3(r.throughput, 0.0f);
+        ps.meta[p] = make_int4(0, r.bounce, kPhCamera, 0);
+        ps.rng[3 * p] = pack_xs(smp.s1);
+        ps.rng[3 * p + 1] = pack_xs(smp.s2x);
+        ps.rng[3 * p + 2] = pack_xs(smp.s2y);
+        ps.pixel[p] = i;
+        active[p] = p;
+    }
+    const int slot = queue_slot(ext.count, live);
+    if (live) push_ray(ext, slot, p, r.o, job.integrator == 0 ? 0.0f : PT_EPS, r.dir,
+                       job.integrator == 0 ? FLT_MAX : PT_INFINITY);
+}
+
+// Extension rays: closest hit, result stored per path.
+__global__ void __launch_bounds__(256) k_trace_queue_closest(BvhView b, Queue q, const int* count, float* hit_t,
+                                                             int* hit_k)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    const float4 r0 = q.rays[2 * i], r1 = q.rays[2 * i + 1];
+    float t;
+    const int k = traverse<false, false>(b, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w, r1.w, &t, nullptr,
+                                         nullptr);
+    const int p = q.path[i];
+    hit_t[p] = t;
+    hit_k[p] = k;
+}
+
+__global__ void __launch_bounds__(256) k_trace_queue_any(BvhView b, Queue q, const int* count, int* hit_k)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    const float4 r0 = q.rays[2 * i], r1 = q.rays[2 * i + 1];
+    float t;
+    const int k = traverse<true, false>(b, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w, r1.w, &t, nullptr,
+                                        nullptr);
+    hit_k[q.path[i]] = k;
+}
+
+// The shading stage of the MIS integrator (PathTracing::Render, PathTracing.cpp:20-200,
+// CalcLightContribution :202-327) and of the simple integrator
+// (SimplePathTracing::Render, SimplePathTracing.cpp:16-101). One thread per live path:
+//   1. resolve the previous bounce's shadow ray,
+//   2. consume the extension hit,
+//   3. at a surface, sample a light (issue a shadow ray) and the BSDF (issue the next
+//      extension ray); at the end of a sample, accumulate and start the next sample.
+template <int INTEGRATOR>
+__global__ void __launch_bounds__(256) k_shade(SceneView sc, JobView job, PathView ps, const int* active_in,
+                                               const int* active_count, int* active_out, int* active_out_count,
+                                               Queue ext, Queue sh_any, Queue sh_closest, float* out_rgb,
+                                               const float* hit_t, const int* hit_k, const float* sh_t,
+                                               const int* sh_k)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in_range = i < *active_count;
+    const int p = in_range ? active_in[i] : 0;
+
+    bool keep = false, want_ext = false, want_any = false, want_closest = false;
+    V3 ext_o, ext_d, sh_o, sh_d;
+    float ext_dmin = 0.0f, ext_dmax = FLT_MAX;
+
+    if (in_range) {
+        Samplers smp;
+        smp.s1 = unpack_xs(ps.rng[3 * p]);
+        smp.s2x = unpack_xs(ps.rng[3 * p + 1]);
+        smp.s2y = unpack_xs(ps.rng[3 * p + 2]);
+        const float4 o4 = ps.ray_o[p], d4 = ps.ray_d[p], t4 = ps.thr[p];
+        Ray ray;
+        ray.o = ld3(o4);
+        ray.prob = o4.w;
+        ray.dir = ld3(d4);
+        ray.ior = d4.w;
+        ray.throughput = ld3(t4);
+        float pdf_bsdf = t4.w;
+        int4 meta = ps.meta[p]; // x sample, y ray.bounce, z phase | pending << 8, w loop bounce
+        ray.bounce = meta.y;
+        int phase = meta.z & 0xff;
+        const int pend = (meta.z >> 8) & 0xff;
+        V3 color = ld3(ps.color[p]);
+        float sin_n = ps.color[p].w;
+        V3 sum = ld3(ps.sum[p]);
+        const int pixel = ps.pixel[p];
+        const int spp = job.spp;
+        bool sample_done = false, add_color = true;
+
+        if (INTEGRATOR == 0) {
+            // 1. shadow ray of the previous bounce
+            if (pend == kPendEnv) {
+                if (sh_k[p] < 0) color = color + ld3(ps.pend_a[p]);
+            } else if (pend == kPendArea) {
+                const int k = sh_k[p];
+                if (k >= 0 && sc.mats[material_of(sc, k, next1(smp))].type == kMatEmission) {
+                    const float4 pa = ps.pend_a[p]; // pl.xyz, pdf_area
+                    const float4 pb = ps.pend_b[p]; // si.pos.xyz, cs
+                    const float4 pc = ps.pend_c[p]; // T_in.xyz, pdf_mat
+                    const float4 pd = ps.pend_d[p]; // f.xyz
+                    const float4 so = ps.pend_o[p], sd = ps.pend_dir[p];
+                    V3 lpos, lnrm;
+                    shade_point(sc, k, ld3(so), ld3(sd), sh_t[p], &lpos, &lnrm);
+                    const float dist = sq_length(ld3(pa) - lpos);
+                    if (fabsf_(dist - 0.0f) < PT_EPS) {
+                        const float l2 = sq_length(lpos - ld3(pb));
+                        const float cos_p = fabsf_(dot(ld3(sd), lnrm));
+                        float mis = 0.0f;
+                        const float pdf_area = pa.w;
+                        if (l2 > 0.0f) {
+                            const float pl = pdf_area;
+                            const float pbs = pc.w * cos_p / l2;
+                            mis = pl * pl / (pl * pl + pbs * pbs);
+                        }
+                        const V3 li = sc.mats[sc.tri_mat_shading[k]].color;
+                        const V3 kk = li * ld3(pd) * pb.w * cos_p / (pdf_area * l2);
+                        color = color + mis * ld3(pc) * kk;
+                    }
+                }
+            }
+        }
+
+        // 2. the extension hit
+        bool at_surface = false;
+        V3 si_pos = v3(0.0f, 0.0f, 0.0f), si_n = v3(0.0f, 0.0f, 1.0f);
+        int si_mat = 0;
+        int loop_b = meta.w;
+        if (phase == kPhWait) {
+            sample_done = true;
+        } else if (INTEGRATOR == 0) {
+            const int k = hit_k[p];
+            const float t = hit_t[p];
+            if (phase == kPhCamera) {
+                if (k < 0) {
+                    sum = sum + ray.throughput * env_color(sc.env, ray.dir);
+                    sample_done = true;
+                    add_color = false;
+                } else {
+                    shade_point(sc, k, ray.o, ray.dir, t, &si_pos, &si_n);
+                    si_mat = sc.tri_mat_shading[k];
+                    if (sc.mats[si_mat].type == kMatEmission) {
+                        sum = sum + ray.throughput * sc.mats[si_mat].color;
+                        sample_done = true;
+                        add_color = false;
+                    } else {
+                        at_surface = true;
+                        loop_b = 0;
+                    }
+                }
+            } else { // kPhBounce
+                if (k >= 0) {
+                    V3 npos, nn;
+                    shade_point(sc, k, ray.o, ray.dir, t, &npos, &nn);
+                    const int nmat = sc.tri_mat_shading[k];
+                    if (sc.mats[nmat].type == kMatEmission) {
+                        const float l2 = sq_length(npos - ray.o);
+                        const float cos_p = fabsf_(dot(ray.dir, nn));
+                        float pdf_area = 0.0f;
+                        V3 dummy;
+                        sample_light(sc, next1(smp), smp, false, &pdf_area, &dummy);
+                        const float pb = pdf_bsdf;
+                        const float pl = l2 / cos_p * pdf_area;
+                        const float mis = pb * pb / (pl * pl + pb * pb);
+                        color = color + mis * ray.throughput * sc.mats[nmat].color;
+                        sample_done = true;
+                    } else {
+                        loop_b += 1;
+                        if (loop_b < job.max_bounces) {
+                            at_surface = true;
+                            si_pos = npos;
+                            si_n = nn;
+                            si_mat = nmat;
+                        } else {
+                            sample_done = true;
+                        }
+                    }
+                } else {
+                    if (env_use_is(sc.env)) {
+                        float mis = 0.0;
+                        if (ray.dir.x != 0 && ray.dir.y != 0 && sin_n > 0) {
+                            const float pe = env_pdf(sc.env, ray.dir) / (2.0f * PT_PI * PT_PI * sin_n);
+                            const float pb = pdf_bsdf;
+                            mis = pb * pb / (pe * pe + pb * pb);
+                        } else {
+                            mis = 1.0f;
+                        }
+                        color = color + mis * ray.throughput * env_color(sc.env, ray.dir);
+                    } else {
+                        color = color + ray.throughput * env_color(sc.env, ray.dir);
+                    }
+                    sample_done = true;
+                }
+            }
+        } else {
+            // simple integrator: one hit, material chosen with a 1D sample
+            const int k = hit_k[p];
+            const float t = hit_t[p];
+            if (k < 0) {
+                color = color + ray.throughput * env_color(sc.env, ray.dir);
+                sample_done = true;
+            } else {
+                const int m = material_of(sc, k, next1(smp));
+                const MaterialDev mat = load_mat(sc, m);
+                if (mat.type == kMatEmission) {
+                    color = color + ray.throughput * mat.color;
+                    sample_done = true;
+                } else {
+                    shade_point(sc, k, ray.o, ray.dir, t, &si_pos, &si_n);
+                    ray = next_ray(mat, ray, si_pos, si_n, smp);
+                    bool cont = true;
+                    if (ray.bounce > job.max_bounces) {
+                        ray.prob *= 0.9f;
+                        ray.prob = smax(0.1f, ray.prob);
+                        if (next1(smp) < ray.prob) ray.throughput = ray.throughput / ray.prob;
+                        else cont = false;
+                    }
+                    if (cont) {
+                        want_ext = true;
+                        ext_o = ray.o;
+                        ext_d = ray.dir;
+                        ext_dmin = PT_EPS;
+                        ext_dmax = PT_INFINITY;
+                        phase = kPhBounce;
+                    } else {
+                        sample_done = true;
+                    }
+                }
+            }
+        }
+
+        // 3. a surface of the MIS loop: NEE then BSDF sampling
+        int new_pend = kPendNone;
+        if (INTEGRATOR == 0 && at_surface) {
+            const MaterialDev mat = load_mat(sc, si_mat);
+            const float dd = -dot(ray.dir, si_n);
+            const V3 pnee = si_pos + PT_EPS * copysignf_(1.0f, dd) * si_n;
+            float pdf_area = 1.0f;
+            V3 pl;
+            const bool is_area = sample_light(sc, next1(smp), smp, true, &pdf_area, &pl);
+            if (is_area) {
+                if (sc.num_lights > 0) {
+                    const V3 ldir = normalized(pl - pnee);
+                    want_closest = true;
+                    sh_o = pnee;
+                    sh_d = ldir;
+                    new_pend = kPendArea;
+                    const V3 f = bxdf(mat, ray.dir, ldir, si_n);
+                    ps.pend_a[p] = st3(pl, pdf_area);
+                    ps.pend_b[p] = st3(si_pos, fabsf_(dot(ldir, si_n)));
+                    ps.pend_c[p] = st3(ray.throughput, bsdf_pdf(mat, ray.dir, ldir, si_n));
+                    ps.pend_d[p] = st3(f, 0.0f);
+                    ps.pend_o[p] = st3(pnee, 0.0f);
+                    ps.pend_dir[p] = st3(ldir, 0.0f);
+                }
+            } else if (env_use_is(sc.env)) {
+                float pdf_uv = 0.0f;
+                const V3 edir = env_sample(sc.env, smp, &pdf_uv);
+                const V3 eo = si_pos + PT_EPS * copysignf_(1.0f, dd) * si_n;
+                want_any = true;
+                sh_o = eo;
+                sh_d = edir;
+                new_pend = kPendEnv;
+                const float cs = fabsf_(dot(edir, si_n));
+                const float sn = __builtin_sqrtf(smax(0.0f, 1.0f - cs * cs));
+                const float pb = 2.0f * PT_PI * PT_PI * sn * bsdf_pdf(mat, ray.dir, edir, si_n);
+                const float mis = pdf_uv * pdf_uv / (pdf_uv * pdf_uv + pb * pb);
+                const V3 li = env_color(sc.env, edir);
+                const V3 f = bxdf(mat, ray.dir, edir, si_n);
+                const V3 c = mis * ray.throughput * (li * f * sn * cs * 2.0f * PT_PI * PT_PI) / pdf_uv;
+                ps.pend_a[p] = st3(c, 0.0f);
+            }
+            const V3 prev_dir = ray.dir;
+            ray = next_ray(mat, ray, si_pos, si_n, smp);
+            if (fabsf_(sq_length(ray.throughput) - 0.0f) < PT_EPS) {
+                if (new_pend != kPendNone) phase = kPhWait;
+                else sample_done = true;
+            } else {
+                pdf_bsdf = bsdf_pdf(mat, prev_dir, ray.dir, si_n);
+                const float cs = fabsf_(dot(si_n, ray.dir));
+                sin_n = __builtin_sqrtf(smax(0.0f, 1.0f - cs * cs));
+                want_ext = true;
+                ext_o = ray.o;
+                ext_d = ray.dir;
+                ext_dmin = 0.0f;
+                ext_dmax = FLT_MAX;
+                phase = kPhBounce;
+            }
+        }
+
+        // end of a sample: accumulate, then start the next one or finish the pixel
+        int sample = meta.x;
+        if (sample_done) {
+            if (INTEGRATOR == 0) {
+                if (add_color && min_elem(color) >= 0.0f) sum = sum + color;
+            } else {
+                sum = sum + color;
+            }
+            color = v3(0.0f, 0.0f, 0.0f);
+            sample += 1;
+            if (sample < spp) {
+                ray = camera_ray(sc, job, pixel, smp);
+                want_ext = true;
+                ext_o = ray.o;
+                ext_d = ray.dir;
+                ext_dmin = INTEGRATOR == 0 ? 0.0f : PT_EPS;
+                ext_dmax = INTEGRATOR == 0 ? FLT_MAX : PT_INFINITY;
+                phase = kPhCamera;
+                loop_b = 0;
+            } else {
+                const V3 avg = sum / (float)spp;
+                float* o = out_rgb + 3 * ((size_t)job.out_offset + pixel);
+                o[0] = avg.x;
+                o[1] = avg.y;
+                o[2] = avg.z;
+                phase = kPhDone;
+            }
+        }
+        keep = phase != kPhDone;
+        if (keep) {
+            ps.rng[3 * p] = pack_xs(smp.s1);
+            ps.rng[3 * p + 1] = pack_xs(smp.s2x);
+            ps.rng[3 * p + 2] = pack_xs(smp.s2y);
+            ps.ray_o[p] = st3(ray.o, ray.prob);
+            ps.ray_d[p] = st3(ray.dir, ray.ior);
+            ps.thr[p] = st3(ray.throughput, pdf_bsdf);
+            ps.meta[p] = make_int4(sample, ray.bounce, phase | (new_pend << 8), loop_b);
+            ps.color[p] = st3(color, sin_n);
+            ps.sum[p] = st3(sum, 0.0f);
+        }
+    }
+    const int e = queue_slot(ext.count, want_ext);
+    if (want_ext) push_ray(ext, e, p, ext_o, ext_dmin, ext_d, ext_dmax);
+    const int a = queue_slot(sh_any.count, want_any);
+    if (want_any) push_ray(sh_any, a, p, sh_o, 0.0f, sh_d, FLT_MAX);
+    const int c = queue_slot(sh_closest.count, want_closest);
+    if (want_closest) push_ray(sh_closest, c, p, sh_o, 0.0f, sh_d, FLT_MAX);
+    const int s = queue_slot(active_out_count, keep);
+    if (keep) active_out[s] = p;
+}
+
+__global__ void k_zero_counters(int* c, int n)
+{
+    const int i = threadIdx.x;
+    if (i < n) c[i] = 0;
+}
+
+// ---------------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------------
+
+static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+hipError_t launch_trace_closest(const BvhView& b, int64_t n, const float* rays, float* t_out, int32_t* prim_out,
+                                unsigned long long* stats, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (stats)
+        hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid_for(n, 256)), dim3(256), 0, st, b, n,
+                           (const float4*)rays, t_out, prim_out, stats);
+    else
+        hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid_for(n, 256)), dim3(256), 0, st, b, n,
+                           (const float4*)rays, t_out, prim_out, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_any(const BvhView& b, int64_t n, const float* rays, int32_t* hit_out, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_trace_any, dim3(grid_for(n, 256)), dim3(256), 0, st, b, n, (const float4*)rays, hit_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(int* counters, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, counters, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_paths_init(const SceneView& sc, const JobView& job, const PathView& ps, const Queue& ext,
+                             int path_base, int num_paths, int* active, hipStream_t st)
+{
+    if (num_paths <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_paths_init, dim3(grid_for(num_paths, 256)), dim3(256), 0, st, sc, job, ps, ext, path_base,
+                       num_paths, active);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_queue(const BvhView& b, const Queue& q, int max_n, bool any, float* hit_t, int* hit_k,
+                              hipStream_t st)
+{
+    if (max_n <= 0) return hipSuccess;
+    if (any)
+        hipLaunchKernelGGL(k_trace_queue_any, dim3(grid_for(max_n, 256)), dim3(256), 0, st, b, q, q.count, hit_k);
+    else
+        hipLaunchKernelGGL(k_trace_queue_closest, dim3(grid_for(max_n, 256)), dim3(256), 0, st, b, q, q.count,
+                           hit_t, hit_k);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job, const PathView& ps,
+                        const int* active_in, const int* active_count, int max_active, int* active_out,
+                        int* active_out_count, const Queue& ext, const Queue& sh_any, const Queue& sh_closest,
+                        float* out_rgb, const float* hit_t, const int* hit_k, const float* sh_t, const int* sh_k,
+                        hipStream_t st)
+{
+    if (max_active <= 0) return hipSuccess;
+    if (integrator == 0)
+        hipLaunchKernelGGL(k_shade<0>, dim3(grid_for(max_active, 256)), dim3(256), 0, st, sc, job, ps, active_in,
+                           active_count, active_out, active_out_count, ext, sh_any, sh_closest, out_rgb, hit_t, hit_k,
+                           sh_t, sh_k);
+    else
+        hipLaunchKernelGGL(k_shade<1>, dim3(grid_for(max_active, 256)), dim3(256), 0, st, sc, job, ps, active_in,
+                           active_count, active_out, active_out_count, ext, sh_any, sh_closest, out_rgb, hit_t, hit_k,
+                           sh_t, sh_k);
+    return hipGetLastError();
+}
+
+} // namespace pt

@@ -40,6 +40,8 @@ def lib():
     L.oracle_bvh_build.restype = _v
     L.oracle_bvh_build.argtypes = [_v, ctypes.c_int]
     L.oracle_bvh_destroy.argtypes = [_v]
+    L.oracle_bvh_from_tables.restype = _v
+    L.oracle_bvh_from_tables.argtypes = [_v, ctypes.c_int, _f, _i, _i, _i, _i]
     L.oracle_bvh_num_nodes.argtypes = [_v]
     L.oracle_bvh_max_depth.argtypes = [_v]
     L.oracle_bvh_export.argtypes = [_v, _f, _i, _i, _i, _i]
@@ -103,11 +105,23 @@ class OracleScene:
             _ptr(tri_smooth, _i), len(mat_type), _ptr(mat_type, _i), _ptr(mat_param, _f),
             len(lights), _ptr(lights, _i), _ptr(cam, _f), ew, eh, _ptr(env, _f), _ptr(env_param, _f))
         self.bvh = None
+        self.num_tris = len(tri_mat)
 
     def build_bvh(self, bins=16):
         if self.bvh is not None:
             lib().oracle_bvh_destroy(self.bvh)
         self.bvh = lib().oracle_bvh_build(self.ptr, bins)
+        return self
+
+    def use_tree(self, t):
+        """Adopts a tree given as node tables (dict with boxes/child/range/leaf/prim_ids)."""
+        if self.bvh is not None:
+            lib().oracle_bvh_destroy(self.bvh)
+        self._tree = {k: np.ascontiguousarray(t[k]) for k in ("boxes", "child", "range", "leaf", "prim_ids")}
+        T = self._tree
+        self.bvh = lib().oracle_bvh_from_tables(self.ptr, len(T["leaf"]), _ptr(f32(T["boxes"]), _f),
+                                                _ptr(i32(T["child"]), _i), _ptr(i32(T["range"]), _i),
+                                                _ptr(i32(T["leaf"]), _i), _ptr(i32(T["prim_ids"]), _i))
         return self
 
     def bvh_export(self):
@@ -117,8 +131,7 @@ class OracleScene:
         child = np.zeros((n, 2), np.int32)
         rng = np.zeros((n, 2), np.int32)
         leaf = np.zeros(n, np.int32)
-        ntri = len(self._keep[2])
-        ids = np.zeros(ntri, np.int32)
+        ids = np.zeros(self.num_tris, np.int32)
         L.oracle_bvh_export(self.bvh, _ptr(boxes, _f), _ptr(child, _i), _ptr(rng, _i), _ptr(leaf, _i),
                             _ptr(ids, _i))
         return {"boxes": boxes, "child": child, "range": rng, "leaf": leaf, "prim_ids": ids,
