@@ -79,12 +79,21 @@ typedef struct pt_render_desc {
     int32_t max_bounces;   /* numMaxBounces */
     int32_t integrator;    /* PT_INTEGRATOR_* */
     uint32_t seed;         /* job seed of the per-pixel sampler streams */
+    int32_t flags;         /* PT_RENDER_TIME_KERNELS: time every launch with HIP events */
 } pt_render_desc;
 
+#define PT_RENDER_TIME_KERNELS 1
+
+/* Per-kernel figures: index 0 extension (closest hit), 1 environment shadow rays (any
+ * hit), 2 area-light shadow rays (closest hit), 3 shading. Times are filled only with
+ * PT_RENDER_TIME_KERNELS. */
 typedef struct pt_render_stats {
-    int64_t queries;        /* BVH queries issued (extension + shadow) */
-    int64_t iterations;     /* wavefront iterations */
-    double elapsed_ms;      /* device time of the render loop */
+    int64_t queries;         /* BVH queries issued (extension + shadow) */
+    int64_t iterations;      /* wavefront iterations */
+    double elapsed_ms;       /* device time of the render loop */
+    int64_t kernel_items[4]; /* rays traced / paths shaded per kernel kind */
+    int64_t kernel_launches[4];
+    double kernel_ms[4];     /* summed launch durations */
 } pt_render_stats;
 
 /* Context = one device + one HIP stream. */

@@ -988,14 +988,18 @@ static int brute_intersect(const Scene& sc, const Ray& ray, float dmin, float dm
 }
 
 // ---- integrators ---------------------------------------------------------------
+// Query kinds counted separately: 0 extension (camera and bounce rays), 1 area-light
+// shadow rays, 2 environment shadow rays.
 struct Ctx {
     const Scene* sc;
     const BVH* bvh;
     int64_t queries = 0;
-    int closest(const Ray& r, float dmin, float dmax, float* t)
+    int64_t work[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}; // per kind: queries, node visits, triangle tests
+    int closest(const Ray& r, float dmin, float dmax, float* t, int kind = 0)
     {
         ++queries;
-        return bvh_intersect(*sc, *bvh, r, dmin, dmax, t, nullptr, nullptr);
+        ++work[3 * kind];
+        return bvh_intersect(*sc, *bvh, r, dmin, dmax, t, &work[3 * kind + 1], &work[3 * kind + 2]);
     }
 };
 
@@ -1049,7 +1053,7 @@ static V light_contribution(Ctx& c, const Shading& si, Sampler1D& s1, Sampler2D&
         to_light.o = p;
         to_light.dir = normalized(sub(pl_pos, p));
         float t;
-        const int hit = c.closest(to_light, 0.0f, FLT_MAX, &t);
+        const int hit = c.closest(to_light, 0.0f, FLT_MAX, &t, 1);
         if (hit >= 0 && sc.mats[sc.material_of(hit, s1.Next())].type == kEmission) {
             const Shading sl = interpolate(sc, hit, to_light, t);
             const float dist = sqlen(sub(pl_pos, sl.pos));
@@ -1076,7 +1080,7 @@ static V light_contribution(Ctx& c, const Shading& si, Sampler1D& s1, Sampler2D&
         to_env.o = add(si.pos, scl(si.normal, kEps * std::copysign(1.0f, d)));
         to_env.dir = dir;
         float t;
-        const int hit = c.closest(to_env, 0.0f, FLT_MAX, &t);
+        const int hit = c.closest(to_env, 0.0f, FLT_MAX, &t, 2);
         if (hit < 0) {
             const float cs = std::fabs(dot(to_env.dir, si.normal));
             const float sn = sqrtf(smax(0.0f, 1.0f - cs * cs));
@@ -1351,7 +1355,7 @@ void oracle_trace_brute(void* s, int64_t n, const float* rays, float* t_out, int
 
 int64_t oracle_render(void* s, void* b, int width, int height, int spp, int max_bounces,
                       int integrator, uint32_t seed, int num_threads, const int32_t* pixels,
-                      int64_t num_pixels, float* rgb_out)
+                      int64_t num_pixels, float* rgb_out, int64_t* work)
 {
     const Scene& sc = *(Scene*)s;
     const BVH& bvh = *(BVH*)b;
@@ -1359,12 +1363,15 @@ int64_t oracle_render(void* s, void* b, int width, int height, int spp, int max_
     // (hardware_concurrency workers); tiles are taken from an atomic counter.
     const int tw = 16, th = 16;
     int64_t total = pixels ? num_pixels : (int64_t)width * height;
-    const int64_t tile_px = (int64_t)tw * th;
+    // a pixel list is handed out in chunks of 16 so that sparse samples still fill the pool
+    const int64_t tile_px = pixels ? 16 : (int64_t)tw * th;
     const int64_t ntiles = pixels ? (total + tile_px - 1) / tile_px
                                   : (int64_t)((width + tw - 1) / tw) * ((height + th - 1) / th);
     const int tiles_x = (width + tw - 1) / tw;
     std::atomic<int64_t> next_tile(0);
     std::atomic<int64_t> queries(0);
+    std::vector<std::atomic<int64_t>> wk(9);
+    for (auto& x : wk) x = 0;
     unsigned nt = num_threads > 0 ? (unsigned)num_threads : std::thread::hardware_concurrency();
     if (nt == 0) nt = 1;
     auto worker = [&]() {
@@ -1396,11 +1403,14 @@ int64_t oracle_render(void* s, void* b, int width, int height, int spp, int max_
             }
         }
         queries += c.queries;
+        for (int k = 0; k < 9; ++k) wk[k] += c.work[k];
     };
     std::vector<std::thread> pool;
     for (unsigned i = 1; i < nt; ++i) pool.emplace_back(worker);
     worker();
     for (auto& t : pool) t.join();
+    if (work)
+        for (int k = 0; k < 9; ++k) work[k] = wk[k].load();
     return queries.load();
 }
 

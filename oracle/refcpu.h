@@ -63,10 +63,12 @@ void oracle_trace_brute(void* scene, int64_t n, const float* rays, float* t_out,
  * simple path tracer (integrator 1, Core/Integrator/SimplePathTracing.cpp), per-pixel
  * seeded samplers, 16x16 tiles on num_threads workers (0 = all hardware threads).
  * If pixels != NULL only those num_pixels pixel indices are rendered (rgb_out then holds
- * 3 floats per listed pixel). Returns the number of BVH queries issued. */
+ * 3 floats per listed pixel). Returns the number of BVH queries issued. work (optional,
+ * 9 int64): per query kind (extension, area-light shadow, environment shadow) the
+ * number of queries, node visits and triangle tests. */
 int64_t oracle_render(void* scene, void* bvh, int width, int height, int spp, int max_bounces,
                       int integrator, uint32_t seed, int num_threads, const int32_t* pixels,
-                      int64_t num_pixels, float* rgb_out);
+                      int64_t num_pixels, float* rgb_out, int64_t* work);
 
 /* Building blocks, exposed for unit tests against the reference translation units. */
 uint32_t oracle_pixel_seed(uint32_t job_seed, uint32_t pixel, uint32_t stream);

@@ -47,11 +47,14 @@ class BvhDesc(ctypes.Structure):
 
 class RenderDesc(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("spp", ctypes.c_int32),
-                ("max_bounces", ctypes.c_int32), ("integrator", ctypes.c_int32), ("seed", ctypes.c_uint32)]
+                ("max_bounces", ctypes.c_int32), ("integrator", ctypes.c_int32), ("seed", ctypes.c_uint32),
+                ("flags", ctypes.c_int32)]
 
 
 class RenderStats(ctypes.Structure):
-    _fields_ = [("queries", ctypes.c_int64), ("iterations", ctypes.c_int64), ("elapsed_ms", ctypes.c_double)]
+    _fields_ = [("queries", ctypes.c_int64), ("iterations", ctypes.c_int64), ("elapsed_ms", ctypes.c_double),
+                ("kernel_items", ctypes.c_int64 * 4), ("kernel_launches", ctypes.c_int64 * 4),
+                ("kernel_ms", ctypes.c_double * 4)]
 
 
 _lib = None
@@ -284,10 +287,11 @@ class Bvh:
                                                     prim_out.data_ptr(), _ptr(st, _l)), "pt_trace_stats")
         return t_out, prim_out, st
 
-    def render(self, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1, out=None):
+    def render(self, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1, out=None,
+               time_kernels=False):
         """Renders one image; returns (rgb [h, w, 3] float32 numpy array or the given device
         tensor, stats dict)."""
-        d = RenderDesc(width, height, spp, max_bounces, integrator, seed)
+        d = RenderDesc(width, height, spp, max_bounces, integrator, seed, 1 if time_kernels else 0)
         st = RenderStats()
         if out is None:
             host = np.zeros((height, width, 3), np.float32)
@@ -298,7 +302,10 @@ class Bvh:
             self.ctx.check(self.ctx._lib.pt_render(self.scene.handle, self.handle, ctypes.byref(d),
                                                    out.data_ptr(), 1, ctypes.byref(st)), "pt_render")
             res = out
-        return res, {"queries": st.queries, "iterations": st.iterations, "elapsed_ms": st.elapsed_ms}
+        kinds = ("extension", "shadow_any", "shadow_closest", "shade")
+        return res, {"queries": st.queries, "iterations": st.iterations, "elapsed_ms": st.elapsed_ms,
+                     "kernels": {k: {"items": st.kernel_items[i], "launches": st.kernel_launches[i],
+                                     "ms": st.kernel_ms[i]} for i, k in enumerate(kinds)}}
 
     def close(self):
         if getattr(self, "handle", None):

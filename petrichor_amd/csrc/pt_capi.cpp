@@ -528,6 +528,11 @@ extern "C" int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float*
     hipEvent_t ev0, ev1;
     PT_HIP(ctx, hipEventCreate(&ev0));
     PT_HIP(ctx, hipEventCreate(&ev1));
+    const bool timing = (d->flags & PT_RENDER_TIME_KERNELS) != 0;
+    hipEvent_t kev[8];
+    for (int k = 0; k < 8; ++k) PT_HIP(ctx, hipEventCreate(&kev[k]));
+    int64_t k_items[4] = {0, 0, 0, 0}, k_launch[4] = {0, 0, 0, 0};
+    double k_ms[4] = {0, 0, 0, 0};
     PT_HIP(ctx, launch_zero(counters, 8, st));
     queues_for(1, &ext, &any, &clo);
     PT_HIP(ctx, hipEventRecord(ev0, st));
@@ -542,18 +547,37 @@ extern "C" int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float*
     while (h[3] > 0) {
         Queue pe, pa, pc;
         queues_for(prev, &pe, &pa, &pc);
+        const int n_items[4] = {h[0], h[1], h[2], h[3]};
+        if (timing) PT_HIP(ctx, hipEventRecord(kev[0], st));
         PT_HIP(ctx, launch_trace_queue(b->view, pe, h[0], false, hit_t, hit_k, st));
+        if (timing) PT_HIP(ctx, hipEventRecord(kev[1], st));
+        if (timing) PT_HIP(ctx, hipEventRecord(kev[2], st));
         PT_HIP(ctx, launch_trace_queue(b->view, pa, h[1], true, sh_t, sh_k, st));
+        if (timing) PT_HIP(ctx, hipEventRecord(kev[3], st));
+        if (timing) PT_HIP(ctx, hipEventRecord(kev[4], st));
         PT_HIP(ctx, launch_trace_queue(b->view, pc, h[2], false, sh_t, sh_k, st));
+        if (timing) PT_HIP(ctx, hipEventRecord(kev[5], st));
         queries += (int64_t)h[0] + h[1] + h[2];
         const int cur = 1 - prev;
         PT_HIP(ctx, launch_zero(counters + 4 * cur, 4, st));
         Queue ne, na, nc;
         queues_for(cur, &ne, &na, &nc);
+        if (timing) PT_HIP(ctx, hipEventRecord(kev[6], st));
         PT_HIP(ctx, launch_shade(d->integrator, sc, job, ps, act_in, counters + 4 * prev + 3, h[3], act_out,
                                  counters + 4 * cur + 3, ne, na, nc, out_d, hit_t, hit_k, sh_t, sh_k, st));
+        if (timing) PT_HIP(ctx, hipEventRecord(kev[7], st));
         PT_HIP(ctx, hipMemcpyAsync(h, counters + 4 * cur, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
         PT_HIP(ctx, hipStreamSynchronize(st));
+        for (int k = 0; k < 4; ++k) {
+            if (n_items[k] <= 0) continue;
+            k_items[k] += n_items[k];
+            k_launch[k] += 1;
+            if (timing) {
+                float ms = 0.0f;
+                PT_HIP(ctx, hipEventElapsedTime(&ms, kev[2 * k], kev[2 * k + 1]));
+                k_ms[k] += ms;
+            }
+        }
         std::swap(act_in, act_out);
         prev = cur;
         ++iters;
@@ -564,12 +588,18 @@ extern "C" int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float*
     PT_HIP(ctx, hipEventElapsedTime(&ms, ev0, ev1));
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
+    for (int k = 0; k < 8; ++k) (void)hipEventDestroy(kev[k]);
     if (!out_on_device)
         PT_HIP(ctx, hipMemcpy(rgb_out, out_d, 3 * sizeof(float) * (size_t)P, hipMemcpyDeviceToHost));
     if (stats) {
         stats->queries = queries;
         stats->iterations = iters;
         stats->elapsed_ms = ms;
+        for (int k = 0; k < 4; ++k) {
+            stats->kernel_items[k] = k_items[k];
+            stats->kernel_launches[k] = k_launch[k];
+            stats->kernel_ms[k] = k_ms[k];
+        }
     }
     return PT_OK;
 }

@@ -136,31 +136,35 @@ def cornell_box(emission=15.0):
 
 
 # ---- Config 2: spheres over a terrain, equirect sky -----------------------------------------
+_SPHERE_CACHE = {}
+
+
+def _unit_sphere(seg, bands):
+    key = (seg, bands)
+    if key not in _SPHERE_CACHE:
+        th = np.linspace(0.0, np.pi, bands + 1)
+        ph = np.linspace(0.0, 2 * np.pi, seg + 1)[:-1]
+        ring = lambda k: np.stack([np.sin(th[k]) * np.cos(ph), np.sin(th[k]) * np.sin(ph),
+                                   np.full_like(ph, np.cos(th[k]))], -1)
+        i = np.arange(seg)
+        j = (i + 1) % seg
+        top = np.broadcast_to(np.array([0, 0, 1.0]), (seg, 3))
+        bot = np.broadcast_to(np.array([0, 0, -1.0]), (seg, 3))
+        out = [np.concatenate([top, ring(1)[i], ring(1)[j]], 1)]
+        for k in range(1, bands - 1):
+            a, b = ring(k), ring(k + 1)
+            q = np.stack([np.concatenate([a[i], b[i], b[j]], 1), np.concatenate([a[i], b[j], a[j]], 1)], 1)
+            out.append(q.reshape(-1, 9))
+        rl = ring(bands - 1)
+        out.append(np.concatenate([bot, rl[j], rl[i]], 1))
+        _SPHERE_CACHE[key] = np.concatenate(out)
+    return _SPHERE_CACHE[key]
+
+
 def uv_sphere(center, radius, seg=64, bands=33):
     """seg x bands UV sphere with triangle fans at the poles: 2*seg + 2*seg*(bands-2)
     triangles (4096 for 64 x 33), none degenerate."""
-    th = np.linspace(0.0, np.pi, bands + 1)
-    ph = np.linspace(0.0, 2 * np.pi, seg + 1)[:-1]
-    st, ct = np.sin(th), np.cos(th)
-    ring = lambda k: np.stack([st[k] * np.cos(ph), st[k] * np.sin(ph), np.full_like(ph, ct[k])], -1)
-    c = np.asarray(center, np.float64)
-    out = []
-    top, bot = np.array([0, 0, 1.0]), np.array([0, 0, -1.0])
-    r1 = ring(1)
-    for i in range(seg):
-        j = (i + 1) % seg
-        out.append(np.concatenate([top, r1[i], r1[j]]))
-    for k in range(1, bands - 1):
-        a, b = ring(k), ring(k + 1)
-        for i in range(seg):
-            j = (i + 1) % seg
-            out.append(np.concatenate([a[i], b[i], b[j]]))
-            out.append(np.concatenate([a[i], b[j], a[j]]))
-    rl = ring(bands - 1)
-    for i in range(seg):
-        j = (i + 1) % seg
-        out.append(np.concatenate([bot, rl[j], rl[i]]))
-    t = np.asarray(out).reshape(-1, 3, 3) * radius + c
+    t = _unit_sphere(seg, bands).reshape(-1, 3, 3) * radius + np.asarray(center, np.float64)
     return t.reshape(-1, 9).astype(np.float32)
 
 

@@ -49,7 +49,7 @@ def lib():
     L.oracle_trace_brute.argtypes = [_v, ctypes.c_int64, _f, _f, _i]
     L.oracle_render.restype = ctypes.c_int64
     L.oracle_render.argtypes = [_v, _v, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                ctypes.c_int, ctypes.c_uint32, ctypes.c_int, _i, ctypes.c_int64, _f]
+                                ctypes.c_int, ctypes.c_uint32, ctypes.c_int, _i, ctypes.c_int64, _f, _l]
     L.oracle_pixel_seed.restype = ctypes.c_uint32
     L.oracle_pixel_seed.argtypes = [ctypes.c_uint32] * 3
     L.oracle_xorshift.argtypes = [ctypes.c_uint32, ctypes.c_int, _u, _f]
@@ -155,7 +155,10 @@ class OracleScene:
         lib().oracle_trace_brute(self.ptr, n, _ptr(rays, _f), _ptr(t, _f), _ptr(p, _i))
         return t, p
 
-    def render(self, width, height, spp, max_bounces, integrator=0, seed=1, threads=0, pixels=None):
+    def render(self, width, height, spp, max_bounces, integrator=0, seed=1, threads=0, pixels=None, work=None):
+        """Returns (image or per-pixel rows, queries). If `work` is an int64[9] array it
+        receives per query kind (extension, area shadow, env shadow): queries, node
+        visits, triangle tests."""
         if pixels is not None:
             pixels = i32(pixels)
             out = np.zeros((len(pixels), 3), np.float32)
@@ -164,7 +167,7 @@ class OracleScene:
             out = np.zeros((height, width, 3), np.float32)
             npx = 0
         q = lib().oracle_render(self.ptr, self.bvh, width, height, spp, max_bounces, integrator, seed,
-                                threads, _ptr(pixels, _i), npx, _ptr(out, _f))
+                                threads, _ptr(pixels, _i), npx, _ptr(out, _f), _ptr(work, _l))
         return out, q
 
     def env_color(self, d):
