@@ -78,6 +78,9 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--depth", type=int, default=12)
+    ap.add_argument("--mode", default="frame", choices=["frame", "copy"],
+                    help="copy: a plain 1 GiB read + 1 GiB write elementwise kernel, the reference "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -89,6 +92,14 @@ def main():
     from petrichor_amd import scenes
 
     assert torch.cuda.is_available(), "no GPU"
+    if args.mode == "copy":
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        for _ in range(4):
+            x.add_(1.0)  # reads 1 GiB and writes 1 GiB per launch
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "copy", "launches": 4, "read_bytes_per_launch": 1 << 30,
+                          "write_bytes_per_launch": 1 << 30}), flush=True)
+        return
     log("generating scene")
     t0 = time.perf_counter()
     sc = scenes.spheres_scene(seed=2)

@@ -218,27 +218,38 @@ bool build_bvh_host(const float* tri_pos, int num_tris, int num_bins, int leaf_m
     return true;
 }
 
-// Inner nodes in depth-first order -> 4 float4 each: (lo_l, hi_l.x), (hi_l.yz, lo_r.xy),
-// (lo_r.z, hi_r), (ref_l, cnt_l, ref_r, cnt_r) with cnt = 0 for an inner child.
-void pack_bvh2(const HostBvh& h, std::vector<float4>& nodes, V3* root_lo, V3* root_hi, int* root_ref,
-               int* root_cnt)
+// Packs the tree for the traversal kernel. Inner nodes get indices in depth-first
+// preorder (a node, then its left subtree, then its right subtree), so the first child
+// usually follows its parent in memory. Each inner node is 4 float4: (lo_l, hi_l.x),
+// (hi_l.yz, lo_r.xy), (lo_r.z, hi_r), (ref_l, ref_r, 0, 0) with child references
+// encoded as in PT_REF_* (pt_device.h); leaves of more than PT_REF_MAX_SMALL_LEAF
+// triangles go through the big-leaf table (first slot, count).
+void pack_bvh2(const HostBvh& h, std::vector<float4>& nodes, std::vector<int2>& big_leaves, V3* root_lo,
+               V3* root_hi, uint32_t* root_ref)
 {
     nodes.clear();
+    big_leaves.clear();
     const int nn = (int)h.nodes.size();
     std::vector<int> gidx(nn, -1);
     int inner = 0;
-    for (int i = 0; i < nn; ++i)
-        if (!h.nodes[i].leaf) gidx[i] = inner++;
+    std::vector<int> st;
+    st.push_back(0);
+    while (!st.empty()) { // preorder, left first
+        const int i = st.back();
+        st.pop_back();
+        if (h.nodes[i].leaf) continue;
+        gidx[i] = inner++;
+        st.push_back(h.nodes[i].child[1]);
+        st.push_back(h.nodes[i].child[0]);
+    }
     nodes.resize(4 * (size_t)inner);
-    auto ref_of = [&](int c, int* ref, int* cnt) {
+    auto ref_of = [&](int c) -> uint32_t {
         const HostNode& x = h.nodes[c];
-        if (x.leaf) {
-            *ref = x.begin;
-            *cnt = x.end - x.begin;
-        } else {
-            *ref = gidx[c];
-            *cnt = 0;
-        }
+        if (!x.leaf) return pt_ref_inner((uint32_t)gidx[c]);
+        const int cnt = x.end - x.begin;
+        if (cnt <= PT_REF_MAX_SMALL_LEAF) return pt_ref_leaf((uint32_t)x.begin, (uint32_t)cnt);
+        big_leaves.push_back(make_int2(x.begin, cnt));
+        return pt_ref_big_leaf((uint32_t)(big_leaves.size() - 1));
     };
     for (int i = 0; i < nn; ++i) {
         const HostNode& x = h.nodes[i];
@@ -249,20 +260,12 @@ void pack_bvh2(const HostBvh& h, std::vector<float4>& nodes, V3* root_lo, V3* ro
         o[0] = make_float4(l.lo.x, l.lo.y, l.lo.z, l.hi.x);
         o[1] = make_float4(l.hi.y, l.hi.z, r.lo.x, r.lo.y);
         o[2] = make_float4(r.lo.z, r.hi.x, r.hi.y, r.hi.z);
-        int rl, cl, rr, cr;
-        ref_of(x.child[0], &rl, &cl);
-        ref_of(x.child[1], &rr, &cr);
-        int4 m = make_int4(rl, cl, rr, cr);
+        uint4 m = make_uint4(ref_of(x.child[0]), ref_of(x.child[1]), 0u, 0u);
         o[3] = *reinterpret_cast<float4*>(&m);
     }
     *root_lo = h.nodes[0].lo;
     *root_hi = h.nodes[0].hi;
-    if (h.nodes[0].end - h.nodes[0].begin == 0) {
-        *root_ref = 0;
-        *root_cnt = -1; // empty scene: every query misses
-    } else {
-        ref_of(0, root_ref, root_cnt);
-    }
+    *root_ref = (h.nodes[0].end - h.nodes[0].begin == 0) ? PT_REF_EMPTY : ref_of(0);
 }
 
 } // namespace pt

@@ -40,6 +40,8 @@ struct pt_bvh {
     pt_scene* scene = nullptr;
     HostBvh host;
     float4* nodes_d = nullptr;
+    int2* big_leaves_d = nullptr;
+    int* cursor_d = nullptr; // trace work cursor of the API entry points
     float4* tris_d = nullptr;
     int32_t* prim_ids_d = nullptr;
     int32_t* tri_mat_d = nullptr;
@@ -258,9 +260,10 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
         return fail(ctx, PT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     }
     std::vector<float4> nodes;
+    std::vector<int2> big;
     V3 rlo, rhi;
-    int rref, rcnt;
-    pack_bvh2(b->host, nodes, &rlo, &rhi, &rref, &rcnt);
+    uint32_t rref;
+    pack_bvh2(b->host, nodes, big, &rlo, &rhi, &rref);
     const size_t nt = (size_t)s->num_tris;
     std::vector<float4> tris(3 * nt);
     std::vector<int32_t> mat(nt), mat_sh(nt), smooth;
@@ -283,6 +286,8 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     }
     hipError_t err = hipSetDevice(ctx->device);
     if (err == hipSuccess) err = upload(&b->nodes_d, nodes.data(), nodes.size(), ctx->stream);
+    if (err == hipSuccess) err = upload(&b->big_leaves_d, big.data(), big.size(), ctx->stream);
+    if (err == hipSuccess) err = hipMalloc((void**)&b->cursor_d, 4 * sizeof(int));
     if (err == hipSuccess) err = upload(&b->tris_d, tris.data(), tris.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&b->prim_ids_d, b->host.order.data(), nt, ctx->stream);
     if (err == hipSuccess) err = upload(&b->tri_mat_d, mat.data(), nt, ctx->stream);
@@ -291,8 +296,8 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     if (err == hipSuccess) err = upload(&b->tri_nrm_d, nrm.data(), nrm.size(), ctx->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
     if (err != hipSuccess) {
-        free_all({b->nodes_d, b->tris_d, b->prim_ids_d, b->tri_mat_d, b->tri_mat_shading_d, b->tri_smooth_d,
-                  b->tri_nrm_d});
+        free_all({b->nodes_d, b->big_leaves_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
+                  b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d});
         delete b;
         return fail(ctx, err == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
                     std::string("bvh upload: ") + hipGetErrorString(err));
@@ -303,7 +308,7 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     b->view.root_lo = rlo;
     b->view.root_hi = rhi;
     b->view.root_ref = rref;
-    b->view.root_cnt = rcnt;
+    b->view.big_leaves = b->big_leaves_d;
     b->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = b;
     return PT_OK;
@@ -312,8 +317,8 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
 int pt_bvh_destroy(pt_bvh* b)
 {
     if (!b) return PT_ERR_ARG;
-    free_all({b->nodes_d, b->tris_d, b->prim_ids_d, b->tri_mat_d, b->tri_mat_shading_d, b->tri_smooth_d,
-              b->tri_nrm_d});
+    free_all({b->nodes_d, b->big_leaves_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
+              b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d});
     delete b;
     return PT_OK;
 }
@@ -374,39 +379,76 @@ int pt_bvh_info(pt_bvh* b, int32_t* num_nodes, int32_t* max_depth, double* build
     return PT_OK;
 }
 
+} // extern "C"
+
+namespace {
+
+TraceSegment api_segment(int64_t n, const float* rays, bool any, float* t_out, int32_t* k_out)
+{
+    TraceSegment g;
+    g.rays = (const float4*)rays;
+    g.path = nullptr;
+    g.n = (int)n;
+    g.any = any ? 1 : 0;
+    g.map_prim = any ? 0 : 1;
+    g.hit_flag = any ? 1 : 0;
+    g.out_t = t_out;
+    g.out_k = k_out;
+    return g;
+}
+
+int api_trace(pt_bvh* b, const TraceSegment& seg, unsigned long long* stats, hipStream_t st)
+{
+    pt_context* ctx = b->scene->ctx;
+    TraceJob job;
+    job.seg[0] = seg;
+    job.num_seg = 1;
+    PT_HIP(ctx, launch_zero(b->cursor_d, 1, st));
+    PT_HIP(ctx, launch_trace_job(b->view, job, b->cursor_d, stats, st));
+    return PT_OK;
+}
+
+} // namespace
+
+extern "C" {
+
 int pt_trace_closest(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_t* prim_out, void* stream)
 {
     if (!b) return PT_ERR_ARG;
     pt_context* ctx = b->scene->ctx;
-    if (n < 0 || (n > 0 && (!rays || !t_out || !prim_out))) return fail(ctx, PT_ERR_ARG, "bad trace arguments");
+    if (n < 0 || n > 0x7fffff00 || (n > 0 && (!rays || !t_out || !prim_out)))
+        return fail(ctx, PT_ERR_ARG, "bad trace arguments");
+    if (n == 0) return PT_OK;
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-    PT_HIP(ctx, launch_trace_closest(b->view, n, rays, t_out, prim_out, nullptr, st));
-    return PT_OK;
+    return api_trace(b, api_segment(n, rays, false, t_out, prim_out), nullptr, st);
 }
 
 int pt_trace_any(pt_bvh* b, int64_t n, const float* rays, int32_t* hit_out, void* stream)
 {
     if (!b) return PT_ERR_ARG;
     pt_context* ctx = b->scene->ctx;
-    if (n < 0 || (n > 0 && (!rays || !hit_out))) return fail(ctx, PT_ERR_ARG, "bad trace arguments");
+    if (n < 0 || n > 0x7fffff00 || (n > 0 && (!rays || !hit_out))) return fail(ctx, PT_ERR_ARG, "bad trace arguments");
+    if (n == 0) return PT_OK;
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-    PT_HIP(ctx, launch_trace_any(b->view, n, rays, hit_out, st));
-    return PT_OK;
+    return api_trace(b, api_segment(n, rays, true, nullptr, hit_out), nullptr, st);
 }
 
 int pt_trace_stats(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_t* prim_out, int64_t* host_stats)
 {
     if (!b || !host_stats) return PT_ERR_ARG;
     pt_context* ctx = b->scene->ctx;
+    if (n < 0 || n > 0x7fffff00 || (n > 0 && (!rays || !t_out || !prim_out)))
+        return fail(ctx, PT_ERR_ARG, "bad trace arguments");
     unsigned long long* st_d = nullptr;
     PT_HIP(ctx, hipMalloc((void**)&st_d, 2 * sizeof(unsigned long long)));
-    unsigned long long zero[2] = {0, 0};
-    hipError_t e = hipMemcpyAsync(st_d, zero, sizeof zero, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = launch_trace_closest(b->view, n, rays, t_out, prim_out, st_d, ctx->stream);
     unsigned long long h[2] = {0, 0};
-    if (e == hipSuccess) e = hipMemcpyAsync(h, st_d, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipError_t e = hipMemcpyAsync(st_d, h, sizeof h, hipMemcpyHostToDevice, ctx->stream);
+    int rc = PT_OK;
+    if (e == hipSuccess && n > 0) rc = api_trace(b, api_segment(n, rays, false, t_out, prim_out), st_d, ctx->stream);
+    if (e == hipSuccess && rc == PT_OK) e = hipMemcpyAsync(h, st_d, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && rc == PT_OK) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(st_d);
+    if (rc != PT_OK) return rc;
     if (e != hipSuccess) return fail(ctx, PT_ERR_HIP, hipGetErrorString(e));
     host_stats[0] = (int64_t)h[0];
     host_stats[1] = (int64_t)h[1];
@@ -482,7 +524,7 @@ extern "C" int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float*
     ALLOC(sh_k, P);
     ALLOC(act0, P);
     ALLOC(act1, P);
-    ALLOC(counters, 8);
+    ALLOC(counters, 12); // two sets of queue counters, then 3 trace cursors
     ALLOC(ext_rays, 2 * (size_t)P);
     ALLOC(any_rays, 2 * (size_t)P);
     ALLOC(clo_rays, 2 * (size_t)P);
@@ -533,7 +575,20 @@ extern "C" int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float*
     for (int k = 0; k < 8; ++k) PT_HIP(ctx, hipEventCreate(&kev[k]));
     int64_t k_items[4] = {0, 0, 0, 0}, k_launch[4] = {0, 0, 0, 0};
     double k_ms[4] = {0, 0, 0, 0};
-    PT_HIP(ctx, launch_zero(counters, 8, st));
+    PT_HIP(ctx, launch_zero(counters, 12, st));
+    int* cursors = counters + 8;
+    auto segment = [&](const Queue& q, int n, bool any, float* out_t, int* out_k) {
+        TraceSegment g;
+        g.rays = q.rays;
+        g.path = q.path;
+        g.n = n;
+        g.any = any ? 1 : 0;
+        g.map_prim = 0;
+        g.hit_flag = 0;
+        g.out_t = out_t;
+        g.out_k = out_k;
+        return g;
+    };
     queues_for(1, &ext, &any, &clo);
     PT_HIP(ctx, hipEventRecord(ev0, st));
     PT_HIP(ctx, launch_paths_init(sc, job, ps, ext, 0, P, act0, st));
@@ -548,15 +603,26 @@ extern "C" int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float*
         Queue pe, pa, pc;
         queues_for(prev, &pe, &pa, &pc);
         const int n_items[4] = {h[0], h[1], h[2], h[3]};
-        if (timing) PT_HIP(ctx, hipEventRecord(kev[0], st));
-        PT_HIP(ctx, launch_trace_queue(b->view, pe, h[0], false, hit_t, hit_k, st));
-        if (timing) PT_HIP(ctx, hipEventRecord(kev[1], st));
-        if (timing) PT_HIP(ctx, hipEventRecord(kev[2], st));
-        PT_HIP(ctx, launch_trace_queue(b->view, pa, h[1], true, sh_t, sh_k, st));
-        if (timing) PT_HIP(ctx, hipEventRecord(kev[3], st));
-        if (timing) PT_HIP(ctx, hipEventRecord(kev[4], st));
-        PT_HIP(ctx, launch_trace_queue(b->view, pc, h[2], false, sh_t, sh_k, st));
-        if (timing) PT_HIP(ctx, hipEventRecord(kev[5], st));
+        const TraceSegment segs[3] = {segment(pe, h[0], false, hit_t, hit_k), segment(pa, h[1], true, nullptr, sh_k),
+                                      segment(pc, h[2], false, sh_t, sh_k)};
+        PT_HIP(ctx, launch_zero(cursors, 3, st));
+        if (timing) {
+            // one launch per queue so that each kind is timed on its own
+            for (int k = 0; k < 3; ++k) {
+                TraceJob job;
+                job.seg[0] = segs[k];
+                job.num_seg = 1;
+                PT_HIP(ctx, hipEventRecord(kev[2 * k], st));
+                PT_HIP(ctx, launch_trace_job(b->view, job, cursors + k, nullptr, st));
+                PT_HIP(ctx, hipEventRecord(kev[2 * k + 1], st));
+            }
+        } else {
+            TraceJob job;
+            job.num_seg = 0;
+            for (int k = 0; k < 3; ++k)
+                if (segs[k].n > 0) job.seg[job.num_seg++] = segs[k];
+            PT_HIP(ctx, launch_trace_job(b->view, job, cursors, nullptr, st));
+        }
         queries += (int64_t)h[0] + h[1] + h[2];
         const int cur = 1 - prev;
         PT_HIP(ctx, launch_zero(counters + 4 * cur, 4, st));

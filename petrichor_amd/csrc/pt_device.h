@@ -612,6 +612,20 @@ PT_HD bool slab(V3 o, V3 inv, int sx, int sy, int sz, V3 lo, V3 hi, float* dist)
     return true;
 }
 
+// Child / stack references of the traversal: bits 0-26 node index or first leaf slot,
+// bits 27-30 leaf size (0 = inner node, 15 = big leaf: bits 0-26 index the big-leaf
+// table), bit 31 = the ray origin is strictly inside the box (stack entries only).
+#define PT_REF_INDEX_MASK 0x07FFFFFFu
+#define PT_REF_MAX_SMALL_LEAF 14
+#define PT_REF_BIG 15u
+#define PT_REF_INSIDE 0x80000000u
+#define PT_REF_EMPTY 0xFFFFFFFFu
+PT_HD uint32_t pt_ref_inner(uint32_t idx) { return idx; }
+PT_HD uint32_t pt_ref_leaf(uint32_t first, uint32_t cnt) { return first | (cnt << 27); }
+PT_HD uint32_t pt_ref_big_leaf(uint32_t i) { return i | (PT_REF_BIG << 27); }
+PT_HD uint32_t pt_ref_cnt(uint32_t r) { return (r >> 27) & 0xFu; }
+PT_HD uint32_t pt_ref_index(uint32_t r) { return r & PT_REF_INDEX_MASK; }
+
 // AABB::Contains (AABB.h:58-68): strict inequalities.
 PT_HD bool box_contains(V3 lo, V3 hi, V3 p)
 {

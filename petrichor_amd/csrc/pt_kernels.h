@@ -17,9 +17,10 @@ namespace pt {
 struct BvhView {
     const float4* nodes;
     const float4* tris;
-    const int32_t* prim_ids; // leaf slot -> triangle index
+    const int32_t* prim_ids;  // leaf slot -> triangle index
+    const int2* big_leaves;   // (first slot, count) of leaves over PT_REF_MAX_SMALL_LEAF triangles
     V3 root_lo, root_hi;
-    int root_ref, root_cnt;  // root as a stack entry: inner node 0, or a leaf
+    uint32_t root_ref;        // PT_REF_* encoding, PT_REF_EMPTY for an empty scene
 };
 
 struct SceneView {
@@ -65,14 +66,31 @@ struct Queue {
     int* count;
 };
 
-hipError_t launch_trace_closest(const BvhView& b, int64_t n, const float* rays, float* t_out, int32_t* prim_out,
-                                unsigned long long* stats, hipStream_t st);
-hipError_t launch_trace_any(const BvhView& b, int64_t n, const float* rays, int32_t* hit_out, hipStream_t st);
+// One segment of a trace launch: `n` rays (8 floats each) traced closest-hit or any-hit.
+// Results go to out_t / out_k indexed by path[i] (or by i when path is null); with
+// map_prim the leaf slot is replaced by the triangle index, with hit_flag out_k gets 1/0.
+struct TraceSegment {
+    const float4* rays;
+    const int32_t* path;
+    int n;
+    int any;
+    int map_prim;
+    int hit_flag;
+    float* out_t;
+    int32_t* out_k;
+};
+struct TraceJob {
+    TraceSegment seg[3];
+    int num_seg;
+};
+// Persistent trace launch over every segment of a job (rays handed out to waves in
+// chunks from *cursor, which must be zero). stats (optional, 2 counters) accumulates
+// node visits and triangle tests.
+hipError_t launch_trace_job(const BvhView& b, const TraceJob& job, int* cursor, unsigned long long* stats,
+                            hipStream_t st);
 hipError_t launch_zero(int* counters, int n, hipStream_t st);
 hipError_t launch_paths_init(const SceneView& sc, const JobView& job, const PathView& ps, const Queue& ext,
                              int path_base, int num_paths, int* active, hipStream_t st);
-hipError_t launch_trace_queue(const BvhView& b, const Queue& q, int max_n, bool any, float* hit_t, int* hit_k,
-                              hipStream_t st);
 hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job, const PathView& ps,
                         const int* active_in, const int* active_count, int max_active, int* active_out,
                         int* active_out_count, const Queue& ext, const Queue& sh_any, const Queue& sh_closest,

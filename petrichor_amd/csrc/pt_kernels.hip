@@ -1,153 +1,12 @@
-// pt_kernels.hip — gfx950 kernels of the render path: BVH traversal (closest / any
-// hit) and the wavefront path tracer (ray generation, shading with NEE + MIS,
-// queue compaction, accumulation).
+// pt_kernels.hip — gfx950 kernels of the wavefront path tracer: path set-up and
+// camera rays, the shading stage (NEE + MIS, BSDF sampling, accumulation) and queue
+// compaction. BVH traversal lives in pt_trace.hip.
 #include <hip/hip_runtime.h>
 
 #include "pt_device.h"
 #include "pt_kernels.h"
 
 namespace pt {
-
-// ---------------------------------------------------------------------------------
-// BVH traversal
-//
-// Binary tree in the reference's topology (BinnedSAHBVH.cpp:15-225) stored with both
-// child boxes in the parent (4 x 16 B per inner node). The walk keeps the reference's
-// order and culling (BinnedSAHBVH.cpp:272-374): children are pushed far-first by
-// squared distance from the origin to the child-box centroid; a popped child is
-// skipped when a hit exists, the origin is not strictly inside the child box and the
-// hit is nearer than the child's slab distance. Missing children are not pushed (the
-// reference pushes and then rejects them), which changes nothing but the work.
-// ---------------------------------------------------------------------------------
-
-struct StackEntry {
-    int ref;   // inner node index, or first leaf slot
-    int cnt;   // 0 = inner node, else leaf size; bit 30 set = origin inside the box
-    float d;   // slab distance of the child box
-};
-
-#define PT_INSIDE_BIT (1 << 30)
-
-template <bool ANY, bool STATS>
-__device__ __forceinline__ int traverse(const BvhView& b, V3 o, V3 d, float dmin, float dmax, float* t_out,
-                                        uint32_t* nv_out, uint32_t* nt_out)
-{
-    const V3 inv = v3(1.0f, 1.0f, 1.0f) / d;
-    const int sx = inv.x < 0.0f, sy = inv.y < 0.0f, sz = inv.z < 0.0f;
-    int best = -1;
-    float best_t = 0.0f;
-    uint32_t nv = 0, nt = 0;
-    StackEntry stack[PT_MAX_STACK];
-    int sp = 0;
-    float rd;
-    if (b.root_cnt < 0) return -1; // empty scene
-    if (STATS) ++nv;
-    if (!slab(o, inv, sx, sy, sz, b.root_lo, b.root_hi, &rd)) {
-        if (STATS) { *nv_out = nv; *nt_out = nt; }
-        return -1;
-    }
-    stack[sp].ref = b.root_ref;
-    stack[sp].cnt = b.root_cnt;
-    stack[sp].d = rd;
-    ++sp;
-    while (sp > 0) {
-        --sp;
-        const StackEntry e = stack[sp];
-        const int cnt = e.cnt & ~PT_INSIDE_BIT;
-        if (best >= 0 && !(e.cnt & PT_INSIDE_BIT)) {
-            if (best_t < e.d) continue;
-        }
-        if (cnt > 0) {
-            for (int k = e.ref; k < e.ref + cnt; ++k) {
-                if (STATS) ++nt;
-                const float4 r0 = b.tris[3 * k];
-                const float4 r1 = b.tris[3 * k + 1];
-                const float4 r2 = b.tris[3 * k + 2];
-                TriDev tr;
-                tr.p0 = v3(r0.x, r0.y, r0.z);
-                tr.e1 = v3(r0.w, r1.x, r1.y);
-                tr.e2 = v3(r1.z, r1.w, r2.x);
-                tr.n = v3(r2.y, r2.z, r2.w);
-                float t;
-                if (tri_intersect(tr, o, d, &t)) {
-                    if (t < dmin || t > dmax) continue;
-                    if (best < 0 || t < best_t) {
-                        best = k;
-                        best_t = t;
-                        if (ANY) {
-                            if (STATS) { *nv_out = nv; *nt_out = nt; }
-                            *t_out = best_t;
-                            return best;
-                        }
-                    }
-                }
-            }
-            continue;
-        }
-        // inner node: both child boxes and their references
-        const float4* nd = b.nodes + 4 * (size_t)e.ref;
-        const float4 a = nd[0], bb = nd[1], c = nd[2];
-        const int4 meta = *reinterpret_cast<const int4*>(nd + 3);
-        const V3 llo = v3(a.x, a.y, a.z), lhi = v3(a.w, bb.x, bb.y);
-        const V3 rlo = v3(bb.z, bb.w, c.x), rhi = v3(c.y, c.z, c.w);
-        if (STATS) nv += 2;
-        // AABB::CalcCentroid = 0.5 * (lower + upper) (AABB.h:70-74)
-        const float dl = sq_length(0.5f * (llo + lhi) - o);
-        const float dr = sq_length(0.5f * (rlo + rhi) - o);
-        float tl, tr_;
-        const bool hl = slab(o, inv, sx, sy, sz, llo, lhi, &tl);
-        const bool hr = slab(o, inv, sx, sy, sz, rlo, rhi, &tr_);
-        StackEntry el, er;
-        el.ref = meta.x;
-        el.cnt = meta.y | (box_contains(llo, lhi, o) ? PT_INSIDE_BIT : 0);
-        el.d = tl;
-        er.ref = meta.z;
-        er.cnt = meta.w | (box_contains(rlo, rhi, o) ? PT_INSIDE_BIT : 0);
-        er.d = tr_;
-        if (dl < dr) {
-            if (hr) stack[sp++] = er;
-            if (hl) stack[sp++] = el;
-        } else {
-            if (hl) stack[sp++] = el;
-            if (hr) stack[sp++] = er;
-        }
-    }
-    if (STATS) { *nv_out = nv; *nt_out = nt; }
-    *t_out = best_t;
-    return best;
-}
-
-// Public trace API: rays are 8 floats (o.xyz, dist_min, d.xyz, dist_max).
-template <bool STATS>
-__global__ void __launch_bounds__(256) k_trace_closest(BvhView b, int64_t n, const float4* __restrict__ rays,
-                                                       float* __restrict__ t_out, int32_t* __restrict__ prim_out,
-                                                       unsigned long long* stats)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
-    float t;
-    uint32_t nv = 0, nt = 0;
-    const int k = traverse<false, STATS>(b, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w, r1.w, &t, &nv, &nt);
-    t_out[i] = k >= 0 ? t : __builtin_inff();
-    prim_out[i] = k >= 0 ? b.prim_ids[k] : -1;
-    if (STATS) {
-        atomicAdd(&stats[0], (unsigned long long)nv);
-        atomicAdd(&stats[1], (unsigned long long)nt);
-    }
-}
-
-__global__ void __launch_bounds__(256) k_trace_any(BvhView b, int64_t n, const float4* __restrict__ rays,
-                                                   int32_t* __restrict__ hit_out)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
-    float t;
-    const int k = traverse<true, false>(b, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w, r1.w, &t, nullptr,
-                                        nullptr);
-    hit_out[i] = k >= 0 ? 1 : 0;
-}
 
 // ---------------------------------------------------------------------------------
 // Wavefront path tracer
@@ -289,32 +148,6 @@ __global__ void __launch_bounds__(256) k_paths_init(SceneView sc, JobView job, P
     const int slot = queue_slot(ext.count, live);
     if (live) push_ray(ext, slot, p, r.o, job.integrator == 0 ? 0.0f : PT_EPS, r.dir,
                        job.integrator == 0 ? FLT_MAX : PT_INFINITY);
-}
-
-// Extension rays: closest hit, result stored per path.
-__global__ void __launch_bounds__(256) k_trace_queue_closest(BvhView b, Queue q, const int* count, float* hit_t,
-                                                             int* hit_k)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *count) return;
-    const float4 r0 = q.rays[2 * i], r1 = q.rays[2 * i + 1];
-    float t;
-    const int k = traverse<false, false>(b, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w, r1.w, &t, nullptr,
-                                         nullptr);
-    const int p = q.path[i];
-    hit_t[p] = t;
-    hit_k[p] = k;
-}
-
-__global__ void __launch_bounds__(256) k_trace_queue_any(BvhView b, Queue q, const int* count, int* hit_k)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *count) return;
-    const float4 r0 = q.rays[2 * i], r1 = q.rays[2 * i + 1];
-    float t;
-    const int k = traverse<true, false>(b, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w, r1.w, &t, nullptr,
-                                        nullptr);
-    hit_k[q.path[i]] = k;
 }
 
 // The shading stage of the MIS integrator (PathTracing::Render, PathTracing.cpp:20-200,
@@ -626,26 +459,6 @@ __global__ void k_zero_counters(int* c, int n)
 
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-hipError_t launch_trace_closest(const BvhView& b, int64_t n, const float* rays, float* t_out, int32_t* prim_out,
-                                unsigned long long* stats, hipStream_t st)
-{
-    if (n <= 0) return hipSuccess;
-    if (stats)
-        hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid_for(n, 256)), dim3(256), 0, st, b, n,
-                           (const float4*)rays, t_out, prim_out, stats);
-    else
-        hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid_for(n, 256)), dim3(256), 0, st, b, n,
-                           (const float4*)rays, t_out, prim_out, stats);
-    return hipGetLastError();
-}
-
-hipError_t launch_trace_any(const BvhView& b, int64_t n, const float* rays, int32_t* hit_out, hipStream_t st)
-{
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_trace_any, dim3(grid_for(n, 256)), dim3(256), 0, st, b, n, (const float4*)rays, hit_out);
-    return hipGetLastError();
-}
-
 hipError_t launch_zero(int* counters, int n, hipStream_t st)
 {
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, counters, n);
@@ -658,18 +471,6 @@ hipError_t launch_paths_init(const SceneView& sc, const JobView& job, const Path
     if (num_paths <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_paths_init, dim3(grid_for(num_paths, 256)), dim3(256), 0, st, sc, job, ps, ext, path_base,
                        num_paths, active);
-    return hipGetLastError();
-}
-
-hipError_t launch_trace_queue(const BvhView& b, const Queue& q, int max_n, bool any, float* hit_t, int* hit_k,
-                              hipStream_t st)
-{
-    if (max_n <= 0) return hipSuccess;
-    if (any)
-        hipLaunchKernelGGL(k_trace_queue_any, dim3(grid_for(max_n, 256)), dim3(256), 0, st, b, q, q.count, hit_k);
-    else
-        hipLaunchKernelGGL(k_trace_queue_closest, dim3(grid_for(max_n, 256)), dim3(256), 0, st, b, q, q.count,
-                           hit_t, hit_k);
     return hipGetLastError();
 }
 

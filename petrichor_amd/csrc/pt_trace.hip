@@ -1,0 +1,293 @@
+// pt_trace.hip — BVH traversal for gfx950: closest hit and any hit over the binary
+// tree in the reference's topology.
+//
+// Walk order and culling are the reference's (BinnedSAHBVH::Intersect,
+// Core/Accel/BinnedSAHBVH.cpp:272-374): from an inner node the child whose box centroid
+// is nearer the ray origin (squared distance, AABB::CalcCentroid, AABB.h:70-74) is
+// visited first, the other is pushed with its slab distance; a popped child is skipped
+// when a hit exists, the origin is not strictly inside the child's box and the hit is
+// nearer than the child's slab distance; leaf triangles are tested in leaf order and a
+// hit replaces the best one only when strictly nearer. The reference pushes children
+// whose box the ray misses and rejects them when popped; they are not pushed here.
+//
+// Execution: persistent waves. A wave takes rays in chunks of 64 from a global cursor
+// and refills each lane as soon as its ray finishes, so lanes do not wait for the
+// longest ray of the wave. Each lane keeps its traversal stack in LDS (the first
+// PT_LDS_STACK entries) with deeper entries in private memory.
+#include <hip/hip_runtime.h>
+
+#include "pt_device.h"
+#include "pt_kernels.h"
+
+namespace pt {
+
+#define PT_LDS_STACK 16
+#define PT_TRACE_BLOCK 256
+#define PT_TRACE_CHUNK 64
+
+struct StackEntry {
+    uint32_t ref; // PT_REF_* word with the inside bit
+    float d;      // slab distance of that child's box
+};
+
+struct TraceRay {
+    V3 o, d, inv;
+    float dmin, dmax;
+    int sx, sy, sz;
+    bool any;
+};
+
+struct TraceState {
+    int best;     // leaf slot of the nearest hit so far, -1 = none
+    float best_t;
+    uint32_t cur; // entry being processed
+    float cur_d;
+    int sp;
+    uint32_t nv, nt; // node visits (slab tests) and triangle tests
+};
+
+__device__ __forceinline__ void stack_push(uint2* lds, StackEntry* spill, int sp, uint32_t ref, float d)
+{
+    if (sp < PT_LDS_STACK) {
+        lds[sp * PT_TRACE_BLOCK] = make_uint2(ref, __float_as_uint(d));
+    } else {
+        spill[sp - PT_LDS_STACK].ref = ref;
+        spill[sp - PT_LDS_STACK].d = d;
+    }
+}
+
+__device__ __forceinline__ void stack_pop(const uint2* lds, const StackEntry* spill, int sp, uint32_t* ref,
+                                          float* d)
+{
+    if (sp < PT_LDS_STACK) {
+        const uint2 e = lds[sp * PT_TRACE_BLOCK];
+        *ref = e.x;
+        *d = __uint_as_float(e.y);
+    } else {
+        *ref = spill[sp - PT_LDS_STACK].ref;
+        *d = spill[sp - PT_LDS_STACK].d;
+    }
+}
+
+// Starts a ray at the root: returns false when the root box is missed (or the scene is
+// empty), which finishes the ray with no hit.
+__device__ __forceinline__ bool trace_begin(const BvhView& b, const float4 r0, const float4 r1, bool any,
+                                            TraceRay& r, TraceState& s)
+{
+    r.o = v3(r0.x, r0.y, r0.z);
+    r.d = v3(r1.x, r1.y, r1.z);
+    r.dmin = r0.w;
+    r.dmax = r1.w;
+    r.any = any;
+    r.inv = v3(1.0f, 1.0f, 1.0f) / r.d; // Vector3f::One() / ray.dir (BinnedSAHBVH.cpp:280)
+    r.sx = r.inv.x < 0.0f;
+    r.sy = r.inv.y < 0.0f;
+    r.sz = r.inv.z < 0.0f;
+    s.best = -1;
+    s.best_t = 0.0f;
+    s.sp = 0;
+    s.nv = 0;
+    s.nt = 0;
+    if (b.root_ref == PT_REF_EMPTY) return false;
+    s.nv = 1;
+    s.cur = b.root_ref;
+    return slab(r.o, r.inv, r.sx, r.sy, r.sz, b.root_lo, b.root_hi, &s.cur_d);
+}
+
+// One traversal step: the current entry (an inner node or a whole leaf), then the next
+// entry from the stack. Returns true when the ray is finished.
+template <bool STATS>
+__device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
+                                           StackEntry* spill)
+{
+    bool next = false;
+    if (s.best >= 0 && !(s.cur & PT_REF_INSIDE) && s.best_t < s.cur_d) {
+        // culled: a nearer hit exists and the origin is outside this box
+    } else if (pt_ref_cnt(s.cur) != 0) {
+        uint32_t first = pt_ref_index(s.cur), cnt = pt_ref_cnt(s.cur);
+        if (cnt == PT_REF_BIG) {
+            const int2 bl = b.big_leaves[first];
+            first = (uint32_t)bl.x;
+            cnt = (uint32_t)bl.y;
+        }
+        for (uint32_t k = first; k < first + cnt; ++k) {
+            if (STATS) ++s.nt;
+            const float4 t0 = b.tris[3 * (size_t)k];
+            const float4 t1 = b.tris[3 * (size_t)k + 1];
+            const float4 t2 = b.tris[3 * (size_t)k + 2];
+            TriDev tr;
+            tr.p0 = v3(t0.x, t0.y, t0.z);
+            tr.e1 = v3(t0.w, t1.x, t1.y);
+            tr.e2 = v3(t1.z, t1.w, t2.x);
+            tr.n = v3(t2.y, t2.z, t2.w);
+            float t;
+            if (tri_intersect(tr, r.o, r.d, &t)) {
+                if (t < r.dmin || t > r.dmax) continue;
+                if (s.best < 0 || t < s.best_t) {
+                    s.best = (int)k;
+                    s.best_t = t;
+                    if (r.any) return true;
+                }
+            }
+        }
+    } else {
+        const float4* nd = b.nodes + 4 * (size_t)pt_ref_index(s.cur);
+        const float4 a = nd[0], bb = nd[1], c = nd[2], mf = nd[3];
+        const uint32_t ml = __float_as_uint(mf.x), mr = __float_as_uint(mf.y);
+        const V3 llo = v3(a.x, a.y, a.z), lhi = v3(a.w, bb.x, bb.y);
+        const V3 rlo = v3(bb.z, bb.w, c.x), rhi = v3(c.y, c.z, c.w);
+        if (STATS) s.nv += 2;
+        const float dl = sq_length(0.5f * (llo + lhi) - r.o);
+        const float dr = sq_length(0.5f * (rlo + rhi) - r.o);
+        float tl, tr_;
+        const bool hl = slab(r.o, r.inv, r.sx, r.sy, r.sz, llo, lhi, &tl);
+        const bool hr = slab(r.o, r.inv, r.sx, r.sy, r.sz, rlo, rhi, &tr_);
+        const uint32_t el = ml | (box_contains(llo, lhi, r.o) ? PT_REF_INSIDE : 0u);
+        const uint32_t er = mr | (box_contains(rlo, rhi, r.o) ? PT_REF_INSIDE : 0u);
+        // the child the reference pops first: left if dl < dr, else right
+        const bool left_first = dl < dr;
+        const bool h1 = left_first ? hl : hr, h2 = left_first ? hr : hl;
+        const uint32_t e1 = left_first ? el : er, e2 = left_first ? er : el;
+        const float d1 = left_first ? tl : tr_, d2 = left_first ? tr_ : tl;
+        if (h1) {
+            if (h2) {
+                stack_push(lds, spill, s.sp, e2, d2);
+                ++s.sp;
+            }
+            s.cur = e1;
+            s.cur_d = d1;
+            next = true;
+        } else if (h2) {
+            s.cur = e2;
+            s.cur_d = d2;
+            next = true;
+        }
+    }
+    if (!next) {
+        if (s.sp == 0) return true;
+        --s.sp;
+        stack_pop(lds, spill, s.sp, &s.cur, &s.cur_d);
+    }
+    return false;
+}
+
+__device__ __forceinline__ void trace_finish(const BvhView& b, const TraceSegment& seg, int i, const TraceState& s)
+{
+    const int p = seg.path ? seg.path[i] : i;
+    if (seg.hit_flag) {
+        seg.out_k[p] = s.best >= 0 ? 1 : 0;
+        return;
+    }
+    int k = s.best;
+    if (seg.map_prim && k >= 0) k = b.prim_ids[k];
+    if (seg.out_t) seg.out_t[p] = s.best >= 0 ? s.best_t : (seg.map_prim ? __builtin_inff() : s.best_t);
+    seg.out_k[p] = k;
+}
+
+template <bool STATS>
+__global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b, TraceJob job, int total, int* cursor,
+                                                                     unsigned long long* stats)
+{
+    __shared__ uint2 lds_stack[PT_LDS_STACK * PT_TRACE_BLOCK];
+    uint2* lds = lds_stack + threadIdx.x;
+    StackEntry spill[PT_MAX_STACK - PT_LDS_STACK];
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lane_lt = (1ull << lane) - 1ull;
+    int pool_next = 0, pool_end = 0; // wave-uniform: this wave's share of rays
+    bool exhausted = false;          // wave-uniform
+    int item = -1, seg_id = 0, seg_i = 0;
+    TraceRay r;
+    TraceState s;
+    uint64_t nv = 0, nt = 0;
+    for (;;) {
+        // refill lanes without a ray: first from the wave's pool, then from a new chunk
+        const bool need = item < 0;
+        const unsigned long long need_mask = __ballot(need);
+        if (need_mask != 0ull && !exhausted) {
+            const int n_need = __popcll(need_mask);
+            const int rank = __popcll(need_mask & lane_lt);
+            int avail = pool_end - pool_next;
+            int base2 = 0;
+            if (avail < n_need) {
+                int got = 0;
+                if (lane == __ffsll((long long)need_mask) - 1) got = atomicAdd(cursor, PT_TRACE_CHUNK);
+                base2 = __shfl(got, __ffsll((long long)need_mask) - 1, 64);
+                if (base2 >= total) exhausted = true;
+            }
+            if (need) {
+                int g = -1;
+                if (rank < avail) g = pool_next + rank;
+                else if (base2 < total) g = base2 + (rank - avail);
+                item = (g >= 0 && g < total) ? g : -1;
+            }
+            if (avail >= n_need) {
+                pool_next += n_need;
+            } else {
+                pool_next = base2 + (n_need - avail);
+                pool_end = base2 + PT_TRACE_CHUNK;
+            }
+            if (pool_end > total) pool_end = total;
+            if (pool_next > pool_end) pool_next = pool_end;
+            if (need && item >= 0) {
+                int g = item;
+                seg_id = 0;
+                while (seg_id < job.num_seg - 1 && g >= job.seg[seg_id].n) {
+                    g -= job.seg[seg_id].n;
+                    ++seg_id;
+                }
+                seg_i = g;
+                const TraceSegment& sg = job.seg[seg_id];
+                const float4 r0 = sg.rays[2 * (size_t)g], r1 = sg.rays[2 * (size_t)g + 1];
+                if (!trace_begin(b, r0, r1, sg.any != 0, r, s)) {
+                    if (STATS) nv += s.nv;
+                    trace_finish(b, sg, g, s);
+                    item = -1;
+                }
+            }
+        }
+        if (__ballot(item >= 0) == 0ull) {
+            if (exhausted) break; // no ray left anywhere for this wave
+            continue;
+        }
+        if (item >= 0) {
+            if (trace_step<STATS>(b, r, s, lds, spill)) {
+                if (STATS) {
+                    nv += s.nv;
+                    nt += s.nt;
+                }
+                trace_finish(b, job.seg[seg_id], seg_i, s);
+                item = -1;
+            }
+        }
+    }
+    if (STATS) {
+        atomicAdd(&stats[0], (unsigned long long)nv);
+        atomicAdd(&stats[1], (unsigned long long)nt);
+    }
+}
+
+hipError_t launch_trace_job(const BvhView& b, const TraceJob& job, int* cursor, unsigned long long* stats,
+                            hipStream_t st)
+{
+    int64_t total = 0;
+    for (int k = 0; k < job.num_seg; ++k) total += job.seg[k].n;
+    if (total <= 0) return hipSuccess;
+    if (total > 0x7fffff00) return hipErrorInvalidValue;
+    // persistent grid: enough waves to fill the device, fewer for small launches
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int64_t waves_needed = (total + PT_TRACE_CHUNK - 1) / PT_TRACE_CHUNK;
+    int64_t blocks = (int64_t)cus * 5; // five 32 KB LDS stacks fit a CU
+    const int64_t blocks_needed = (waves_needed + 3) / 4; // 4 waves per block
+    if (blocks_needed < blocks) blocks = blocks_needed;
+    if (stats)
+        hipLaunchKernelGGL(k_trace_persistent<true>, dim3((unsigned)blocks), dim3(PT_TRACE_BLOCK), 0, st, b, job,
+                           (int)total, cursor, stats);
+    else
+        hipLaunchKernelGGL(k_trace_persistent<false>, dim3((unsigned)blocks), dim3(PT_TRACE_BLOCK), 0, st, b, job,
+                           (int)total, cursor, stats);
+    return hipGetLastError();
+}
+
+} // namespace pt
