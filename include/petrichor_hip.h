@@ -151,6 +151,41 @@ int pt_trace_stats(pt_bvh* bvh, int64_t n, const float* rays, float* t_out, int3
 int pt_render(pt_scene* scene, pt_bvh* bvh, const pt_render_desc* desc, float* rgb_out,
               int32_t out_on_device, pt_render_stats* stats);
 
+/* ---- batched rendering ---------------------------------------------------------
+ * An engine holds a pool of path slots (one per pixel of the jobs in flight) in device
+ * memory. Jobs of different scenes, sizes and settings share every launch: one
+ * persistent trace launch and one shading launch per wavefront iteration, replayed as
+ * a captured hipGraph. Jobs are admitted as slots free up and retired when done. No
+ * reference counterpart: the reference renders one scene per Petrichor::Render call
+ * (Core/Petrichor.cpp:34-102). */
+typedef struct pt_engine pt_engine;
+
+typedef struct pt_job_desc {
+    pt_scene* scene;
+    pt_bvh* bvh;
+    pt_render_desc render;
+    float* rgb_out;        /* width*height*3 floats */
+    int32_t out_on_device; /* 1: rgb_out is a device pointer */
+} pt_job_desc;
+
+typedef struct pt_batch_stats {
+    int64_t queries;    /* BVH queries of every job */
+    int64_t iterations; /* wavefront iterations (graph replays) */
+    double elapsed_ms;  /* device time from the first admission to the last retirement */
+    double wall_ms;     /* host time of the same span */
+    int64_t syncs;      /* host synchronisations */
+} pt_batch_stats;
+
+/* max_paths: path slots (pixels in flight); max_jobs: jobs in flight; multi_job = 1
+ * for engines that mix jobs (pt_render_batch), 0 for one job at a time. */
+int pt_engine_create(pt_context* ctx, int64_t max_paths, int32_t max_jobs, int32_t multi_job,
+                     pt_engine** out);
+int pt_engine_destroy(pt_engine* engine);
+/* Renders n jobs to completion, admitting them in order as slots free up;
+ * iterations_per_sync graph replays run between host synchronisations (0 = 8). */
+int pt_render_batch(pt_engine* engine, int32_t n, const pt_job_desc* jobs, int32_t iterations_per_sync,
+                    pt_batch_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

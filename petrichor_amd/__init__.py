@@ -51,6 +51,16 @@ class RenderDesc(ctypes.Structure):
                 ("flags", ctypes.c_int32)]
 
 
+class JobDesc(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("bvh", ctypes.c_void_p), ("render", RenderDesc),
+                ("rgb_out", ctypes.c_void_p), ("out_on_device", ctypes.c_int32)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("queries", ctypes.c_int64), ("iterations", ctypes.c_int64), ("elapsed_ms", ctypes.c_double),
+                ("wall_ms", ctypes.c_double), ("syncs", ctypes.c_int64)]
+
+
 class RenderStats(ctypes.Structure):
     _fields_ = [("queries", ctypes.c_int64), ("iterations", ctypes.c_int64), ("elapsed_ms", ctypes.c_double),
                 ("kernel_items", ctypes.c_int64 * 4), ("kernel_launches", ctypes.c_int64 * 4),
@@ -86,6 +96,10 @@ def lib():
     L.pt_trace_any.argtypes = [vp, ctypes.c_int64, vp, vp, vp]
     L.pt_trace_stats.argtypes = [vp, ctypes.c_int64, vp, vp, vp, _l]
     L.pt_render.argtypes = [vp, vp, ctypes.POINTER(RenderDesc), vp, ctypes.c_int32, ctypes.POINTER(RenderStats)]
+    L.pt_engine_create.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, pvp]
+    L.pt_engine_destroy.argtypes = [vp]
+    L.pt_render_batch.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(JobDesc), ctypes.c_int32,
+                                  ctypes.POINTER(BatchStats)]
     _lib = L
     return L
 
@@ -93,7 +107,8 @@ def lib():
 # every symbol include/petrichor_hip.h declares
 EXPORTED = ["pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_scene_create", "pt_scene_destroy",
             "pt_bvh_build", "pt_bvh_destroy", "pt_bvh_export_order", "pt_bvh_export_nodes", "pt_bvh_build_host", "pt_bvh_info", "pt_trace_closest",
-            "pt_trace_any", "pt_trace_stats", "pt_render"]
+            "pt_trace_any", "pt_trace_stats", "pt_render", "pt_engine_create", "pt_engine_destroy",
+            "pt_render_batch"]
 
 
 def _ptr(a, t):
@@ -310,6 +325,63 @@ class Bvh:
     def close(self):
         if getattr(self, "handle", None):
             self.ctx._lib.pt_bvh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Job:
+    """One render job of a batch: a BVH (and its scene) plus render settings."""
+
+    def __init__(self, bvh, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1, out=None):
+        self.bvh, self.width, self.height = bvh, width, height
+        self.spp, self.max_bounces, self.integrator, self.seed = spp, max_bounces, integrator, seed
+        self.out = out  # device tensor [h, w, 3] or None (host numpy result)
+        self.result = None
+
+
+class Engine:
+    """A multi-job engine (pt_engine_create): renders batches of jobs that share every
+    launch (pt_render_batch)."""
+
+    def __init__(self, ctx, max_paths, max_jobs=256, multi=True):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(ctx._lib.pt_engine_create(ctx.handle, int(max_paths), int(max_jobs), 1 if multi else 0,
+                                            ctypes.byref(h)), "pt_engine_create")
+        self.handle = h
+
+    def render_batch(self, jobs, iterations_per_sync=8):
+        arr = (JobDesc * len(jobs))()
+        keep = []
+        for i, j in enumerate(jobs):
+            d = arr[i]
+            d.scene = j.bvh.scene.handle
+            d.bvh = j.bvh.handle
+            d.render = RenderDesc(j.width, j.height, j.spp, j.max_bounces, j.integrator, j.seed, 0)
+            if j.out is None:
+                host = np.zeros((j.height, j.width, 3), np.float32)
+                keep.append(host)
+                j.result = host
+                d.rgb_out = host.ctypes.data
+                d.out_on_device = 0
+            else:
+                j.result = j.out
+                d.rgb_out = j.out.data_ptr()
+                d.out_on_device = 1
+        st = BatchStats()
+        self.ctx.check(self.ctx._lib.pt_render_batch(self.handle, len(jobs), arr, iterations_per_sync,
+                                                     ctypes.byref(st)), "pt_render_batch")
+        return {"queries": st.queries, "iterations": st.iterations, "elapsed_ms": st.elapsed_ms,
+                "wall_ms": st.wall_ms, "syncs": st.syncs}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx._lib.pt_engine_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -9,64 +9,19 @@
 #include <vector>
 
 #include "../../include/petrichor_hip.h"
-#include "pt_bvh_host.h"
-#include "pt_kernels.h"
+#include "pt_internal.h"
 
 using namespace pt;
 
-struct pt_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-};
-
-struct pt_scene {
-    pt_context* ctx = nullptr;
-    int num_tris = 0;
-    std::vector<float> tri_pos, tri_nrm;
-    std::vector<int32_t> tri_mat, tri_smooth;
-    std::vector<MaterialDev> mats;
-    MaterialDev* mats_d = nullptr;
-    float4* lights_d = nullptr;
-    int num_lights = 0;
-    CameraDev cam;
-    EnvDev env;
-    float* env_rgb_d = nullptr;
-    float* pdf1d_d = nullptr;
-    float* cdf1d_d = nullptr;
-};
-
-struct pt_bvh {
-    pt_scene* scene = nullptr;
-    HostBvh host;
-    float4* nodes_d = nullptr;
-    int2* big_leaves_d = nullptr;
-    int* cursor_d = nullptr; // trace work cursor of the API entry points
-    float4* tris_d = nullptr;
-    int32_t* prim_ids_d = nullptr;
-    int32_t* tri_mat_d = nullptr;
-    int32_t* tri_mat_shading_d = nullptr;
-    int32_t* tri_smooth_d = nullptr;
-    float* tri_nrm_d = nullptr;
-    BvhView view;
-    double build_ms = 0.0;
-};
-
-namespace {
-
-int fail(pt_context* ctx, int code, const std::string& msg)
+int pt_fail(pt_context* ctx, int code, const std::string& msg)
 {
     if (ctx) ctx->err = msg;
     return code;
 }
 
-#define PT_HIP(ctx, call)                                                                              \
-    do {                                                                                               \
-        hipError_t e_ = (call);                                                                        \
-        if (e_ != hipSuccess)                                                                          \
-            return fail((ctx), e_ == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,                    \
-                        std::string(#call) + ": " + hipGetErrorString(e_));                            \
-    } while (0)
+namespace {
+
+int fail(pt_context* ctx, int code, const std::string& msg) { return pt_fail(ctx, code, msg); }
 
 template <typename T>
 hipError_t upload(T** dst, const T* src, size_t count, hipStream_t st)
@@ -121,6 +76,7 @@ int pt_ctx_create(int device, pt_context** out)
 int pt_ctx_destroy(pt_context* ctx)
 {
     if (!ctx) return PT_ERR_ARG;
+    if (ctx->engine) pt_engine_destroy(ctx->engine);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PT_OK;
@@ -221,6 +177,8 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
         }
     }
     if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
+    s->device_bytes = (int64_t)(s->mats.size() * sizeof(MaterialDev) + lights.size() * sizeof(float4)) +
+                      (d->env_rgb ? 3 * (int64_t)e.w * e.h * 4 + 8 * (int64_t)e.h : 0);
     if (err != hipSuccess) {
         free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->pdf1d_d, s->cdf1d_d});
         delete s;
@@ -310,6 +268,8 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     b->view.root_ref = rref;
     b->view.big_leaves = b->big_leaves_d;
     b->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    b->device_bytes = (int64_t)(nodes.size() * sizeof(float4) + big.size() * sizeof(int2) + tris.size() * sizeof(float4)) +
+                      (int64_t)nt * 12 + (int64_t)(smooth.size() * 4 + nrm.size() * 4);
     *out = b;
     return PT_OK;
 }
@@ -383,12 +343,12 @@ int pt_bvh_info(pt_bvh* b, int32_t* num_nodes, int32_t* max_depth, double* build
 
 namespace {
 
-TraceSegment api_segment(int64_t n, const float* rays, bool any, float* t_out, int32_t* k_out)
+TraceSegment api_segment(const int* count_d, const float* rays, bool any, float* t_out, int32_t* k_out)
 {
     TraceSegment g;
     g.rays = (const float4*)rays;
     g.path = nullptr;
-    g.n = (int)n;
+    g.count = count_d;
     g.any = any ? 1 : 0;
     g.map_prim = any ? 0 : 1;
     g.hit_flag = any ? 1 : 0;
@@ -397,13 +357,20 @@ TraceSegment api_segment(int64_t n, const float* rays, bool any, float* t_out, i
     return g;
 }
 
-int api_trace(pt_bvh* b, const TraceSegment& seg, unsigned long long* stats, hipStream_t st)
+// cursor_d[0] is the work cursor, cursor_d[1] the ray count of the launch.
+int api_trace(pt_bvh* b, int64_t n, const float* rays, bool any, float* t_out, int32_t* k_out,
+              unsigned long long* stats, hipStream_t st)
 {
     pt_context* ctx = b->scene->ctx;
     TraceJob job;
-    job.seg[0] = seg;
+    job.seg[0] = api_segment(b->cursor_d + 1, rays, any, t_out, k_out);
     job.num_seg = 1;
+    job.views = nullptr;
+    job.job_of_path = nullptr;
+    job.queries = nullptr;
+    const int n32 = (int)n;
     PT_HIP(ctx, launch_zero(b->cursor_d, 1, st));
+    PT_HIP(ctx, hipMemcpyAsync(b->cursor_d + 1, &n32, sizeof(int), hipMemcpyHostToDevice, st));
     PT_HIP(ctx, launch_trace_job(b->view, job, b->cursor_d, stats, st));
     return PT_OK;
 }
@@ -420,7 +387,7 @@ int pt_trace_closest(pt_bvh* b, int64_t n, const float* rays, float* t_out, int3
         return fail(ctx, PT_ERR_ARG, "bad trace arguments");
     if (n == 0) return PT_OK;
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-    return api_trace(b, api_segment(n, rays, false, t_out, prim_out), nullptr, st);
+    return api_trace(b, n, rays, false, t_out, prim_out, nullptr, st);
 }
 
 int pt_trace_any(pt_bvh* b, int64_t n, const float* rays, int32_t* hit_out, void* stream)
@@ -430,7 +397,7 @@ int pt_trace_any(pt_bvh* b, int64_t n, const float* rays, int32_t* hit_out, void
     if (n < 0 || n > 0x7fffff00 || (n > 0 && (!rays || !hit_out))) return fail(ctx, PT_ERR_ARG, "bad trace arguments");
     if (n == 0) return PT_OK;
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-    return api_trace(b, api_segment(n, rays, true, nullptr, hit_out), nullptr, st);
+    return api_trace(b, n, rays, true, nullptr, hit_out, nullptr, st);
 }
 
 int pt_trace_stats(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_t* prim_out, int64_t* host_stats)
@@ -444,7 +411,7 @@ int pt_trace_stats(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_
     unsigned long long h[2] = {0, 0};
     hipError_t e = hipMemcpyAsync(st_d, h, sizeof h, hipMemcpyHostToDevice, ctx->stream);
     int rc = PT_OK;
-    if (e == hipSuccess && n > 0) rc = api_trace(b, api_segment(n, rays, false, t_out, prim_out), st_d, ctx->stream);
+    if (e == hipSuccess && n > 0) rc = api_trace(b, n, rays, false, t_out, prim_out, st_d, ctx->stream);
     if (e == hipSuccess && rc == PT_OK) e = hipMemcpyAsync(h, st_d, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess && rc == PT_OK) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(st_d);
@@ -457,215 +424,3 @@ int pt_trace_stats(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_
 
 } // extern "C"
 
-// ---------------------------------------------------------------------------------
-// render
-// ---------------------------------------------------------------------------------
-
-namespace {
-
-struct Buffers {
-    std::vector<void*> ptrs;
-    template <typename T>
-    hipError_t alloc(T** p, size_t count)
-    {
-        *p = nullptr;
-        hipError_t e = hipMalloc((void**)p, (count ? count : 1) * sizeof(T));
-        if (e == hipSuccess) ptrs.push_back(*p);
-        return e;
-    }
-    ~Buffers()
-    {
-        for (void* p : ptrs) (void)hipFree(p);
-    }
-};
-
-} // namespace
-
-extern "C" int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, int32_t out_on_device,
-                         pt_render_stats* stats)
-{
-    if (!s || !b || !d || !rgb_out) return PT_ERR_ARG;
-    pt_context* ctx = s->ctx;
-    if (b->scene != s) return fail(ctx, PT_ERR_ARG, "BVH belongs to another scene");
-    if (d->width <= 0 || d->height <= 0 || d->spp <= 0 || d->max_bounces < 0 ||
-        (d->integrator != PT_INTEGRATOR_MIS && d->integrator != PT_INTEGRATOR_SIMPLE))
-        return fail(ctx, PT_ERR_ARG, "bad render settings");
-    const int64_t P64 = (int64_t)d->width * d->height;
-    if (P64 > (1 << 30)) return fail(ctx, PT_ERR_ARG, "image too large");
-    const int P = (int)P64;
-    hipStream_t st = ctx->stream;
-    PT_HIP(ctx, hipSetDevice(ctx->device));
-
-    Buffers buf;
-    PathView ps;
-    Queue ext, any, clo;
-    float *hit_t, *sh_t, *out_d;
-    int *hit_k, *sh_k, *act0, *act1, *counters;
-    float4 *ext_rays, *any_rays, *clo_rays;
-    int32_t *ext_path, *any_path, *clo_path;
-#define ALLOC(p, n) PT_HIP(ctx, buf.alloc(&(p), (n)))
-    ALLOC(ps.rng, 3 * (size_t)P);
-    ALLOC(ps.sum, P);
-    ALLOC(ps.color, P);
-    ALLOC(ps.ray_o, P);
-    ALLOC(ps.ray_d, P);
-    ALLOC(ps.thr, P);
-    ALLOC(ps.meta, P);
-    ALLOC(ps.pixel, P);
-    ALLOC(ps.pend_a, P);
-    ALLOC(ps.pend_b, P);
-    ALLOC(ps.pend_c, P);
-    ALLOC(ps.pend_d, P);
-    ALLOC(ps.pend_o, P);
-    ALLOC(ps.pend_dir, P);
-    ALLOC(hit_t, P);
-    ALLOC(hit_k, P);
-    ALLOC(sh_t, P);
-    ALLOC(sh_k, P);
-    ALLOC(act0, P);
-    ALLOC(act1, P);
-    ALLOC(counters, 12); // two sets of queue counters, then 3 trace cursors
-    ALLOC(ext_rays, 2 * (size_t)P);
-    ALLOC(any_rays, 2 * (size_t)P);
-    ALLOC(clo_rays, 2 * (size_t)P);
-    ALLOC(ext_path, P);
-    ALLOC(any_path, P);
-    ALLOC(clo_path, P);
-    if (out_on_device) out_d = rgb_out;
-    else ALLOC(out_d, 3 * (size_t)P);
-#undef ALLOC
-
-    SceneView sc;
-    sc.bvh = b->view;
-    sc.tri_mat = b->tri_mat_d;
-    sc.tri_mat_shading = b->tri_mat_shading_d;
-    sc.tri_smooth = b->tri_smooth_d;
-    sc.tri_nrm = b->tri_nrm_d;
-    sc.mats = s->mats_d;
-    sc.lights = s->lights_d;
-    sc.num_lights = s->num_lights;
-    sc.cam = s->cam;
-    sc.env = s->env;
-    JobView job;
-    job.width = d->width;
-    job.height = d->height;
-    job.spp = d->spp;
-    job.max_bounces = d->max_bounces;
-    job.integrator = d->integrator;
-    job.seed = d->seed;
-    job.out_offset = 0;
-
-    // counter sets: [0..3] and [4..7] = ext, any-hit shadow, closest-hit shadow, live paths
-    auto queues_for = [&](int set, Queue* e, Queue* a, Queue* c) {
-        e->rays = ext_rays;
-        e->path = ext_path;
-        e->count = counters + 4 * set + 0;
-        a->rays = any_rays;
-        a->path = any_path;
-        a->count = counters + 4 * set + 1;
-        c->rays = clo_rays;
-        c->path = clo_path;
-        c->count = counters + 4 * set + 2;
-    };
-    hipEvent_t ev0, ev1;
-    PT_HIP(ctx, hipEventCreate(&ev0));
-    PT_HIP(ctx, hipEventCreate(&ev1));
-    const bool timing = (d->flags & PT_RENDER_TIME_KERNELS) != 0;
-    hipEvent_t kev[8];
-    for (int k = 0; k < 8; ++k) PT_HIP(ctx, hipEventCreate(&kev[k]));
-    int64_t k_items[4] = {0, 0, 0, 0}, k_launch[4] = {0, 0, 0, 0};
-    double k_ms[4] = {0, 0, 0, 0};
-    PT_HIP(ctx, launch_zero(counters, 12, st));
-    int* cursors = counters + 8;
-    auto segment = [&](const Queue& q, int n, bool any, float* out_t, int* out_k) {
-        TraceSegment g;
-        g.rays = q.rays;
-        g.path = q.path;
-        g.n = n;
-        g.any = any ? 1 : 0;
-        g.map_prim = 0;
-        g.hit_flag = 0;
-        g.out_t = out_t;
-        g.out_k = out_k;
-        return g;
-    };
-    queues_for(1, &ext, &any, &clo);
-    PT_HIP(ctx, hipEventRecord(ev0, st));
-    PT_HIP(ctx, launch_paths_init(sc, job, ps, ext, 0, P, act0, st));
-    const int p_host = P;
-    PT_HIP(ctx, hipMemcpyAsync(counters + 7, &p_host, sizeof(int), hipMemcpyHostToDevice, st));
-    int h[4] = {P, 0, 0, P};
-    int64_t queries = 0, iters = 0;
-    int* act_in = act0;
-    int* act_out = act1;
-    int prev = 1;
-    while (h[3] > 0) {
-        Queue pe, pa, pc;
-        queues_for(prev, &pe, &pa, &pc);
-        const int n_items[4] = {h[0], h[1], h[2], h[3]};
-        const TraceSegment segs[3] = {segment(pe, h[0], false, hit_t, hit_k), segment(pa, h[1], true, nullptr, sh_k),
-                                      segment(pc, h[2], false, sh_t, sh_k)};
-        PT_HIP(ctx, launch_zero(cursors, 3, st));
-        if (timing) {
-            // one launch per queue so that each kind is timed on its own
-            for (int k = 0; k < 3; ++k) {
-                TraceJob job;
-                job.seg[0] = segs[k];
-                job.num_seg = 1;
-                PT_HIP(ctx, hipEventRecord(kev[2 * k], st));
-                PT_HIP(ctx, launch_trace_job(b->view, job, cursors + k, nullptr, st));
-                PT_HIP(ctx, hipEventRecord(kev[2 * k + 1], st));
-            }
-        } else {
-            TraceJob job;
-            job.num_seg = 0;
-            for (int k = 0; k < 3; ++k)
-                if (segs[k].n > 0) job.seg[job.num_seg++] = segs[k];
-            PT_HIP(ctx, launch_trace_job(b->view, job, cursors, nullptr, st));
-        }
-        queries += (int64_t)h[0] + h[1] + h[2];
-        const int cur = 1 - prev;
-        PT_HIP(ctx, launch_zero(counters + 4 * cur, 4, st));
-        Queue ne, na, nc;
-        queues_for(cur, &ne, &na, &nc);
-        if (timing) PT_HIP(ctx, hipEventRecord(kev[6], st));
-        PT_HIP(ctx, launch_shade(d->integrator, sc, job, ps, act_in, counters + 4 * prev + 3, h[3], act_out,
-                                 counters + 4 * cur + 3, ne, na, nc, out_d, hit_t, hit_k, sh_t, sh_k, st));
-        if (timing) PT_HIP(ctx, hipEventRecord(kev[7], st));
-        PT_HIP(ctx, hipMemcpyAsync(h, counters + 4 * cur, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
-        PT_HIP(ctx, hipStreamSynchronize(st));
-        for (int k = 0; k < 4; ++k) {
-            if (n_items[k] <= 0) continue;
-            k_items[k] += n_items[k];
-            k_launch[k] += 1;
-            if (timing) {
-                float ms = 0.0f;
-                PT_HIP(ctx, hipEventElapsedTime(&ms, kev[2 * k], kev[2 * k + 1]));
-                k_ms[k] += ms;
-            }
-        }
-        std::swap(act_in, act_out);
-        prev = cur;
-        ++iters;
-    }
-    PT_HIP(ctx, hipEventRecord(ev1, st));
-    PT_HIP(ctx, hipEventSynchronize(ev1));
-    float ms = 0.0f;
-    PT_HIP(ctx, hipEventElapsedTime(&ms, ev0, ev1));
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
-    for (int k = 0; k < 8; ++k) (void)hipEventDestroy(kev[k]);
-    if (!out_on_device)
-        PT_HIP(ctx, hipMemcpy(rgb_out, out_d, 3 * sizeof(float) * (size_t)P, hipMemcpyDeviceToHost));
-    if (stats) {
-        stats->queries = queries;
-        stats->iterations = iters;
-        stats->elapsed_ms = ms;
-        for (int k = 0; k < 4; ++k) {
-            stats->kernel_items[k] = k_items[k];
-            stats->kernel_launches[k] = k_launch[k];
-            stats->kernel_ms[k] = k_ms[k];
-        }
-    }
-    return PT_OK;
-}

@@ -1,0 +1,64 @@
+// pt_internal.h — library-internal object definitions shared by the C-ABI files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/petrichor_hip.h"
+#include "pt_bvh_host.h"
+#include "pt_kernels.h"
+
+struct pt_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    pt_engine* engine = nullptr; // single-job engine of pt_render, grown on demand
+};
+
+struct pt_scene {
+    pt_context* ctx = nullptr;
+    int num_tris = 0;
+    std::vector<float> tri_pos, tri_nrm;
+    std::vector<int32_t> tri_mat, tri_smooth;
+    std::vector<pt::MaterialDev> mats;
+    pt::MaterialDev* mats_d = nullptr;
+    float4* lights_d = nullptr;
+    int num_lights = 0;
+    pt::CameraDev cam;
+    pt::EnvDev env;
+    float* env_rgb_d = nullptr;
+    float* pdf1d_d = nullptr;
+    float* cdf1d_d = nullptr;
+    int64_t device_bytes = 0; // materials, lights, environment
+};
+
+struct pt_bvh {
+    pt_scene* scene = nullptr;
+    pt::HostBvh host;
+    float4* nodes_d = nullptr;
+    int2* big_leaves_d = nullptr;
+    int* cursor_d = nullptr; // trace work cursor of the API entry points
+    float4* tris_d = nullptr;
+    int32_t* prim_ids_d = nullptr;
+    int32_t* tri_mat_d = nullptr;
+    int32_t* tri_mat_shading_d = nullptr;
+    int32_t* tri_smooth_d = nullptr;
+    float* tri_nrm_d = nullptr;
+    pt::BvhView view;
+    double build_ms = 0.0;
+    int64_t device_bytes = 0; // nodes, leaf-ordered triangles and per-triangle data
+};
+
+
+// Sets the context's error text and returns the code.
+int pt_fail(pt_context* ctx, int code, const std::string& msg);
+
+#define PT_HIP(ctx, call)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return pt_fail((ctx), e_ == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,                 \
+                           std::string(#call) + ": " + hipGetErrorString(e_));                         \
+    } while (0)

@@ -39,7 +39,16 @@ struct SceneView {
 struct JobView {
     int width, height, spp, max_bounces, integrator;
     uint32_t seed;
-    int64_t out_offset; // first pixel of this job in the output buffer
+    float* out;      // width*height*3 floats of this job's image
+    int* done;       // pixels finished so far
+    int path_base;   // first path slot of this job
+    int job_index;   // index in the engine's job table
+};
+
+// A job as the kernels see it in the engine's job table.
+struct JobDev {
+    SceneView sc;
+    JobView job;
 };
 
 // Per-path state, structure of arrays indexed by path slot.
@@ -52,6 +61,7 @@ struct PathView {
     float4* thr;   // throughput, w = BSDF pdf of the last sampled direction
     int4* meta;    // sample index, ray bounce, phase | pending << 8, loop bounce
     int32_t* pixel;
+    int32_t* job;   // engine job index of each path slot
     float4* pend_a; // pending shadow data (see k_shade)
     float4* pend_b;
     float4* pend_c;
@@ -72,7 +82,7 @@ struct Queue {
 struct TraceSegment {
     const float4* rays;
     const int32_t* path;
-    int n;
+    const int* count; // device-side ray count
     int any;
     int map_prim;
     int hit_flag;
@@ -82,19 +92,40 @@ struct TraceSegment {
 struct TraceJob {
     TraceSegment seg[3];
     int num_seg;
+    // Multi-job launches: the BVH of a ray is views[job_of_path[path]]; otherwise `bvh`.
+    const BvhView* views;
+    const int32_t* job_of_path;
+    unsigned long long* queries; // optional: += rays traced by this launch
 };
 // Persistent trace launch over every segment of a job (rays handed out to waves in
-// chunks from *cursor, which must be zero). stats (optional, 2 counters) accumulates
-// node visits and triangle tests.
+// chunks from *cursor, which must be zero when the launch starts). stats (optional,
+// 2 counters) accumulates node visits and triangle tests. The grid does not depend
+// on the ray counts, so the launch can be captured in a graph.
 hipError_t launch_trace_job(const BvhView& b, const TraceJob& job, int* cursor, unsigned long long* stats,
                             hipStream_t st);
 hipError_t launch_zero(int* counters, int n, hipStream_t st);
+// Compute units of the current device (queried once, then cached; called before any
+// graph capture so that no device query happens while capturing).
+int cu_count();
+// Sets up every path of a job (seeds, sums, first camera ray) and appends the paths to
+// the live list and their camera rays to the extension queue.
 hipError_t launch_paths_init(const SceneView& sc, const JobView& job, const PathView& ps, const Queue& ext,
-                             int path_base, int num_paths, int* active, hipStream_t st);
-hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job, const PathView& ps,
-                        const int* active_in, const int* active_count, int max_active, int* active_out,
-                        int* active_out_count, const Queue& ext, const Queue& sh_any, const Queue& sh_closest,
-                        float* out_rgb, const float* hit_t, const int* hit_k, const float* sh_t, const int* sh_k,
-                        hipStream_t st);
+                             int* active, int* active_count, hipStream_t st);
+// Shading of every live path (count read on the device). With `jobs` non-null each
+// path takes its scene and settings from jobs[ps.job[p]], else from sc / job.
+struct ShadeArgs {
+    PathView ps;
+    const int* active_in;
+    const int* active_count;
+    int* active_out;
+    int* active_out_count;
+    Queue ext, sh_any, sh_closest;
+    const float* hit_t;
+    const int* hit_k;
+    const float* sh_t;
+    const int* sh_k;
+};
+hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job, const JobDev* jobs,
+                        const ShadeArgs& a, hipStream_t st);
 
 } // namespace pt

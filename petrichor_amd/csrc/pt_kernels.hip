@@ -121,13 +121,15 @@ __device__ __forceinline__ Ray camera_ray(const SceneView& sc, const JobView& jo
     return generate_ray(sc.cam, x, y, job.width, job.height, smp);
 }
 
-// One-time set-up of every path of a job: sampler seeds, sums and the first camera ray.
-__global__ void __launch_bounds__(256) k_paths_init(SceneView sc, JobView job, PathView ps, Queue ext, int path_base,
-                                                    int num_paths, int* active)
+// One-time set-up of every path of a job: sampler seeds, sums and the first camera ray;
+// the path joins the live list and its camera ray the extension queue.
+__global__ void __launch_bounds__(256) k_paths_init(SceneView sc, JobView job, PathView ps, Queue ext, int* active,
+                                                    int* active_count)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int num_paths = job.width * job.height;
     const bool live = i < num_paths;
-    const int p = path_base + i;
+    const int p = job.path_base + i;
     Samplers smp;
     Ray r;
     if (live) {
@@ -143,8 +145,10 @@ __global__ void __launch_bounds__(256) k_paths_init(SceneView sc, JobView job, P
         ps.rng[3 * p + 1] = pack_xs(smp.s2x);
         ps.rng[3 * p + 2] = pack_xs(smp.s2y);
         ps.pixel[p] = i;
-        active[p] = p;
+        ps.job[p] = job.job_index;
     }
+    const int a = queue_slot(active_count, live);
+    if (live) active[a] = p;
     const int slot = queue_slot(ext.count, live);
     if (live) push_ray(ext, slot, p, r.o, job.integrator == 0 ? 0.0f : PT_EPS, r.dir,
                        job.integrator == 0 ? FLT_MAX : PT_INFINITY);
@@ -157,22 +161,26 @@ __global__ void __launch_bounds__(256) k_paths_init(SceneView sc, JobView job, P
 //   2. consume the extension hit,
 //   3. at a surface, sample a light (issue a shadow ray) and the BSDF (issue the next
 //      extension ray); at the end of a sample, accumulate and start the next sample.
-template <int INTEGRATOR>
-__global__ void __launch_bounds__(256) k_shade(SceneView sc, JobView job, PathView ps, const int* active_in,
-                                               const int* active_count, int* active_out, int* active_out_count,
-                                               Queue ext, Queue sh_any, Queue sh_closest, float* out_rgb,
-                                               const float* hit_t, const int* hit_k, const float* sh_t,
-                                               const int* sh_k)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in_range = i < *active_count;
-    const int p = in_range ? active_in[i] : 0;
-
-    bool keep = false, want_ext = false, want_any = false, want_closest = false;
+// What shading one path asks of the queues.
+struct ShadeOut {
+    bool keep, want_ext, want_any, want_closest;
     V3 ext_o, ext_d, sh_o, sh_d;
-    float ext_dmin = 0.0f, ext_dmax = FLT_MAX;
+    float ext_dmin, ext_dmax;
+};
 
-    if (in_range) {
+template <int INTEGRATOR>
+__device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& job, const ShadeArgs& A, int p,
+                                          ShadeOut& out)
+{
+    const PathView& ps = A.ps;
+    const float* hit_t = A.hit_t;
+    const int* hit_k = A.hit_k;
+    const float* sh_t = A.sh_t;
+    const int* sh_k = A.sh_k;
+    bool keep = false, want_ext = false, want_any = false, want_closest = false;
+    V3 ext_o = v3(0.0f, 0.0f, 0.0f), ext_d = ext_o, sh_o = ext_o, sh_d = ext_o;
+    float ext_dmin = 0.0f, ext_dmax = FLT_MAX;
+    {
         Samplers smp;
         smp.s1 = unpack_xs(ps.rng[3 * p]);
         smp.s2x = unpack_xs(ps.rng[3 * p + 1]);
@@ -417,10 +425,11 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc, JobView job, PathVi
                 loop_b = 0;
             } else {
                 const V3 avg = sum / (float)spp;
-                float* o = out_rgb + 3 * ((size_t)job.out_offset + pixel);
+                float* o = job.out + 3 * (size_t)pixel;
                 o[0] = avg.x;
                 o[1] = avg.y;
                 o[2] = avg.z;
+                atomicAdd(job.done, 1);
                 phase = kPhDone;
             }
         }
@@ -437,14 +446,51 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc, JobView job, PathVi
             ps.sum[p] = st3(sum, 0.0f);
         }
     }
-    const int e = queue_slot(ext.count, want_ext);
-    if (want_ext) push_ray(ext, e, p, ext_o, ext_dmin, ext_d, ext_dmax);
-    const int a = queue_slot(sh_any.count, want_any);
-    if (want_any) push_ray(sh_any, a, p, sh_o, 0.0f, sh_d, FLT_MAX);
-    const int c = queue_slot(sh_closest.count, want_closest);
-    if (want_closest) push_ray(sh_closest, c, p, sh_o, 0.0f, sh_d, FLT_MAX);
-    const int s = queue_slot(active_out_count, keep);
-    if (keep) active_out[s] = p;
+    out.keep = keep;
+    out.want_ext = want_ext;
+    out.want_any = want_any;
+    out.want_closest = want_closest;
+    out.ext_o = ext_o;
+    out.ext_d = ext_d;
+    out.sh_o = sh_o;
+    out.sh_d = sh_d;
+    out.ext_dmin = ext_dmin;
+    out.ext_dmax = ext_dmax;
+}
+
+// Grid-stride over the live paths (count read on the device, so the grid is fixed).
+// Per-wave ballots compact the new rays into the queues and the surviving paths into
+// the next live list.
+template <int INTEGRATOR, bool MULTI>
+__global__ void __launch_bounds__(256) k_shade(SceneView sc1, JobView job1, const JobDev* jobs, ShadeArgs A)
+{
+    const int n = *A.active_count;
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int i = base + threadIdx.x;
+        const bool in_range = i < n;
+        const int p = in_range ? A.active_in[i] : 0;
+        ShadeOut o;
+        o.keep = o.want_ext = o.want_any = o.want_closest = false;
+        if (in_range) {
+            if (MULTI) {
+                // the integrator is a per-job setting here
+                const JobDev* J = jobs + A.ps.job[p];
+                if (J->job.integrator == 0) shade_one<0>(J->sc, J->job, A, p, o);
+                else shade_one<1>(J->sc, J->job, A, p, o);
+            } else {
+                shade_one<INTEGRATOR>(sc1, job1, A, p, o);
+            }
+        }
+        const int e = queue_slot(A.ext.count, o.want_ext);
+        if (o.want_ext) push_ray(A.ext, e, p, o.ext_o, o.ext_dmin, o.ext_d, o.ext_dmax);
+        const int a = queue_slot(A.sh_any.count, o.want_any);
+        if (o.want_any) push_ray(A.sh_any, a, p, o.sh_o, 0.0f, o.sh_d, FLT_MAX);
+        const int c = queue_slot(A.sh_closest.count, o.want_closest);
+        if (o.want_closest) push_ray(A.sh_closest, c, p, o.sh_o, 0.0f, o.sh_d, FLT_MAX);
+        const int s = queue_slot(A.active_out_count, o.keep);
+        if (o.keep) A.active_out[s] = p;
+    }
 }
 
 __global__ void k_zero_counters(int* c, int n)
@@ -466,29 +512,37 @@ hipError_t launch_zero(int* counters, int n, hipStream_t st)
 }
 
 hipError_t launch_paths_init(const SceneView& sc, const JobView& job, const PathView& ps, const Queue& ext,
-                             int path_base, int num_paths, int* active, hipStream_t st)
+                             int* active, int* active_count, hipStream_t st)
 {
-    if (num_paths <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_paths_init, dim3(grid_for(num_paths, 256)), dim3(256), 0, st, sc, job, ps, ext, path_base,
-                       num_paths, active);
+    const int64_t n = (int64_t)job.width * job.height;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_paths_init, dim3(grid_for(n, 256)), dim3(256), 0, st, sc, job, ps, ext, active,
+                       active_count);
     return hipGetLastError();
 }
 
-hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job, const PathView& ps,
-                        const int* active_in, const int* active_count, int max_active, int* active_out,
-                        int* active_out_count, const Queue& ext, const Queue& sh_any, const Queue& sh_closest,
-                        float* out_rgb, const float* hit_t, const int* hit_k, const float* sh_t, const int* sh_k,
-                        hipStream_t st)
+int cu_count()
 {
-    if (max_active <= 0) return hipSuccess;
-    if (integrator == 0)
-        hipLaunchKernelGGL(k_shade<0>, dim3(grid_for(max_active, 256)), dim3(256), 0, st, sc, job, ps, active_in,
-                           active_count, active_out, active_out_count, ext, sh_any, sh_closest, out_rgb, hit_t, hit_k,
-                           sh_t, sh_k);
-    else
-        hipLaunchKernelGGL(k_shade<1>, dim3(grid_for(max_active, 256)), dim3(256), 0, st, sc, job, ps, active_in,
-                           active_count, active_out, active_out_count, ext, sh_any, sh_closest, out_rgb, hit_t, hit_k,
-                           sh_t, sh_k);
+    static int cached = 0;
+    if (cached == 0) {
+        int dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cached = cus;
+    }
+    return cached;
+}
+
+static int shade_blocks() { return cu_count() * 8; }
+
+hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job, const JobDev* jobs,
+                        const ShadeArgs& a, hipStream_t st)
+{
+    const dim3 grid(shade_blocks()), block(256);
+    if (jobs) hipLaunchKernelGGL((k_shade<0, true>), grid, block, 0, st, sc, job, jobs, a);
+    else if (integrator == 0) hipLaunchKernelGGL((k_shade<0, false>), grid, block, 0, st, sc, job, jobs, a);
+    else hipLaunchKernelGGL((k_shade<1, false>), grid, block, 0, st, sc, job, jobs, a);
     return hipGetLastError();
 }
 

@@ -184,10 +184,19 @@ __device__ __forceinline__ void trace_finish(const BvhView& b, const TraceSegmen
     seg.out_k[p] = k;
 }
 
-template <bool STATS>
-__global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b, TraceJob job, int total, int* cursor,
+template <bool STATS, bool MULTI>
+__global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1, TraceJob job, int* cursor,
                                                                      unsigned long long* stats)
 {
+    // segment sizes come from device counters (written by the previous stage)
+    int seg_n[3] = {0, 0, 0};
+    int total = 0;
+    for (int k = 0; k < job.num_seg; ++k) {
+        seg_n[k] = *job.seg[k].count;
+        total += seg_n[k];
+    }
+    if (job.queries && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(job.queries, (unsigned long long)total);
+    if (total == 0) return;
     __shared__ uint2 lds_stack[PT_LDS_STACK * PT_TRACE_BLOCK];
     uint2* lds = lds_stack + threadIdx.x;
     StackEntry spill[PT_MAX_STACK - PT_LDS_STACK];
@@ -198,6 +207,7 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b, 
     int item = -1, seg_id = 0, seg_i = 0;
     TraceRay r;
     TraceState s;
+    BvhView b = b1; // per lane in multi-job launches
     uint64_t nv = 0, nt = 0;
     for (;;) {
         // refill lanes without a ray: first from the wave's pool, then from a new chunk
@@ -231,12 +241,13 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b, 
             if (need && item >= 0) {
                 int g = item;
                 seg_id = 0;
-                while (seg_id < job.num_seg - 1 && g >= job.seg[seg_id].n) {
-                    g -= job.seg[seg_id].n;
+                while (seg_id < job.num_seg - 1 && g >= seg_n[seg_id]) {
+                    g -= seg_n[seg_id];
                     ++seg_id;
                 }
                 seg_i = g;
                 const TraceSegment& sg = job.seg[seg_id];
+                if (MULTI) b = job.views[job.job_of_path[sg.path[g]]];
                 const float4 r0 = sg.rays[2 * (size_t)g], r1 = sg.rays[2 * (size_t)g + 1];
                 if (!trace_begin(b, r0, r1, sg.any != 0, r, s)) {
                     if (STATS) nv += s.nv;
@@ -269,24 +280,17 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b, 
 hipError_t launch_trace_job(const BvhView& b, const TraceJob& job, int* cursor, unsigned long long* stats,
                             hipStream_t st)
 {
-    int64_t total = 0;
-    for (int k = 0; k < job.num_seg; ++k) total += job.seg[k].n;
-    if (total <= 0) return hipSuccess;
-    if (total > 0x7fffff00) return hipErrorInvalidValue;
-    // persistent grid: enough waves to fill the device, fewer for small launches
-    int dev = 0, cus = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int64_t waves_needed = (total + PT_TRACE_CHUNK - 1) / PT_TRACE_CHUNK;
-    int64_t blocks = (int64_t)cus * 5; // five 32 KB LDS stacks fit a CU
-    const int64_t blocks_needed = (waves_needed + 3) / 4; // 4 waves per block
-    if (blocks_needed < blocks) blocks = blocks_needed;
-    if (stats)
-        hipLaunchKernelGGL(k_trace_persistent<true>, dim3((unsigned)blocks), dim3(PT_TRACE_BLOCK), 0, st, b, job,
-                           (int)total, cursor, stats);
-    else
-        hipLaunchKernelGGL(k_trace_persistent<false>, dim3((unsigned)blocks), dim3(PT_TRACE_BLOCK), 0, st, b, job,
-                           (int)total, cursor, stats);
+    if (job.num_seg <= 0) return hipSuccess;
+    // persistent grid: five 32 KB LDS stacks (5 blocks of 4 waves) per CU
+    const dim3 grid((unsigned)(cu_count() * 5)), block(PT_TRACE_BLOCK);
+    const bool multi = job.views != nullptr;
+    if (stats) {
+        if (multi) hipLaunchKernelGGL((k_trace_persistent<true, true>), grid, block, 0, st, b, job, cursor, stats);
+        else hipLaunchKernelGGL((k_trace_persistent<true, false>), grid, block, 0, st, b, job, cursor, stats);
+    } else {
+        if (multi) hipLaunchKernelGGL((k_trace_persistent<false, true>), grid, block, 0, st, b, job, cursor, stats);
+        else hipLaunchKernelGGL((k_trace_persistent<false, false>), grid, block, 0, st, b, job, cursor, stats);
+    }
     return hipGetLastError();
 }
 

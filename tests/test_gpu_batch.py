@@ -1,0 +1,44 @@
+"""Batched rendering: jobs of different scenes and settings in one engine give, per job,
+the same image as the plain per-job render (same kernels, same seeds) and as the oracle."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import petrichor_amd as P
+from gpu_util import bits_equal, ctx, log, rel_err
+from petrichor_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_matches_plain_and_oracle(gpu):
+    c = ctx()
+    cb = c.scene(scenes.cornell_box()).build_bvh(16)
+    sp_sc = scenes.spheres_scene(n_spheres=8, seg=16, bands=9, terrain_n=32, env_w=256, env_h=128)
+    sp = c.scene(sp_sc).build_bvh(16)
+    specs = [(cb, 48, 40, 6, 8, 0, 11), (sp, 40, 32, 5, 12, 0, 12), (cb, 32, 32, 3, 8, 1, 13),
+             (sp, 24, 24, 7, 10, 1, 14), (cb, 64, 16, 2, 8, 0, 15)]
+    jobs = [P.Job(b, w, h, spp, d, integ, seed) for (b, w, h, spp, d, integ, seed) in specs]
+    eng = P.Engine(c, max_paths=4096, max_jobs=8)  # smaller than the total: admission waits for slots
+    st = eng.render_batch(jobs, iterations_per_sync=4)
+    log("batch", st)
+    total_q = 0
+    for j, (b, w, h, spp, d, integ, seed) in zip(jobs, specs):
+        plain, pst = b.render(w, h, spp, d, integ, seed)
+        total_q += pst["queries"]
+        assert bits_equal(j.result, plain), (w, h, spp, integ, seed)
+        o = O.OracleScene(b.scene.src).use_tree(b.export_nodes() | {"prim_ids": b.export_order()})
+        ref, q = o.render(w, h, spp, d, integ, seed, 0)
+        assert rel_err(j.result, ref).max() <= 1e-4
+    assert st["queries"] == total_q
+
+
+def test_timed_render_matches_graph_render(gpu):
+    c = ctx()
+    b = c.scene(scenes.cornell_box()).build_bvh(16)
+    a, sa = b.render(40, 30, 4, 8, 0, 3)
+    t, stt = b.render(40, 30, 4, 8, 0, 3, time_kernels=True)
+    assert bits_equal(a, t)
+    assert sa["queries"] == stt["queries"]
+    k = stt["kernels"]
+    assert k["extension"]["items"] + k["shadow_any"]["items"] + k["shadow_closest"]["items"] == stt["queries"]
