@@ -71,7 +71,12 @@ typedef struct pt_bvh_desc {
     int32_t num_bins; /* 16 in the reference (Core/Accel/BinnedSAHBVH.h:174); 32 for the microbenchmark */
     int32_t leaf_max; /* 4 in the reference (BinnedSAHBVH.cpp:79) */
     int32_t width;    /* traversal node width: 2 (reference order) */
+    int32_t builder;  /* PT_BUILD_DEVICE (default) or PT_BUILD_HOST; both give the same tree */
 } pt_bvh_desc;
+
+/* pt_bvh_desc.builder. The device builder takes 2..64 bins, the host builder 2..256. */
+#define PT_BUILD_DEVICE 0
+#define PT_BUILD_HOST 1
 
 typedef struct pt_render_desc {
     int32_t width, height; /* Core/RenderSetting.h:12-13 */

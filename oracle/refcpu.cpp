@@ -817,7 +817,10 @@ static void build_bvh(const Scene& sc, int num_bins, BVH* bvh)
             const float l = at(cen[*it], axis) - at(cb.lower, axis);
             const float x = std::nextafter((float)num_bins * l / len, 0.0f);
             // (int)NaN: x86-64 cvttss2si yields INT_MIN, the reference's platform.
-            const int b = (x != x) ? INT_MIN : (int)x;
+            int b = (x != x) ? INT_MIN : (int)x;
+            // bins * l / len can round up to bins when bins is not a power of two; the
+            // reference then indexes one past its bin array. Clamped to the last bin.
+            if (b >= num_bins) b = num_bins - 1;
             bin[*it] = b;
             if (b < 0) {
                 any_below = true;

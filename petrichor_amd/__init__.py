@@ -41,8 +41,13 @@ class SceneDesc(ctypes.Structure):
     ]
 
 
+BUILD_DEVICE = 0
+BUILD_HOST = 1
+
+
 class BvhDesc(ctypes.Structure):
-    _fields_ = [("num_bins", ctypes.c_int32), ("leaf_max", ctypes.c_int32), ("width", ctypes.c_int32)]
+    _fields_ = [("num_bins", ctypes.c_int32), ("leaf_max", ctypes.c_int32), ("width", ctypes.c_int32),
+                ("builder", ctypes.c_int32)]
 
 
 class RenderDesc(ctypes.Structure):
@@ -127,7 +132,7 @@ def build_bvh_host(tri_pos, bins=16, leaf_max=4):
     rng = np.zeros((cap, 2), np.int32)
     leaf = np.zeros(cap, np.int32)
     depth = ctypes.c_int32()
-    d = BvhDesc(bins, leaf_max, 2)
+    d = BvhDesc(bins, leaf_max, 2, BUILD_HOST)
     nn = lib().pt_bvh_build_host(_ptr(tri_pos, _f), n, ctypes.byref(d), _ptr(order, _i), _ptr(boxes, _f),
                                  _ptr(child, _i), _ptr(rng, _i), _ptr(leaf, _i), ctypes.byref(depth))
     if nn < 0:
@@ -218,8 +223,8 @@ class Scene:
         self.num_triangles = d.num_triangles
         self._keep = []  # the library copied everything
 
-    def build_bvh(self, bins=16, leaf_max=4, width=2):
-        return Bvh(self, bins, leaf_max, width)
+    def build_bvh(self, bins=16, leaf_max=4, width=2, builder=BUILD_DEVICE):
+        return Bvh(self, bins, leaf_max, width, builder)
 
     def close(self):
         if getattr(self, "handle", None):
@@ -236,10 +241,10 @@ class Scene:
 class Bvh:
     """BVH over a scene (pt_bvh_build) with the trace and render entry points."""
 
-    def __init__(self, scene, bins=16, leaf_max=4, width=2):
+    def __init__(self, scene, bins=16, leaf_max=4, width=2, builder=BUILD_DEVICE):
         self.scene = scene
         self.ctx = scene.ctx
-        d = BvhDesc(bins, leaf_max, width)
+        d = BvhDesc(bins, leaf_max, width, builder)
         h = ctypes.c_void_p()
         self.ctx.check(self.ctx._lib.pt_bvh_build(scene.handle, ctypes.byref(d), ctypes.byref(h)), "pt_bvh_build")
         self.handle = h

@@ -5,6 +5,7 @@ python -m petrichor_amd.build   (also called by __graft_entry__.build())
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -19,7 +20,7 @@ FP_FLAGS = ["-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt", "-f
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"] + FP_FLAGS
 
-SOURCES = ["pt_kernels.hip", "pt_trace.hip", "pt_bvh_host.cpp", "pt_capi.cpp", "pt_engine.cpp"]
+SOURCES = ["pt_kernels.hip", "pt_trace.hip", "pt_build.hip", "pt_bvh_host.cpp", "pt_capi.cpp", "pt_engine.cpp"]
 
 
 def _run(cmd):
@@ -33,16 +34,19 @@ def _run(cmd):
 
 
 def build(verbose=False):
-    objs = []
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
-    for src in SOURCES:
-        obj = os.path.join(bdir, src + ".o")
-        cmd = [HIPCC] + COMMON + DEVICE_FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
-        out = _run(cmd)
-        if verbose and out.strip():
-            print(out)
-        objs.append(obj)
+    objs = [os.path.join(bdir, src + ".o") for src in SOURCES]
+
+    def compile_one(i):
+        cmd = [HIPCC] + COMMON + DEVICE_FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, SOURCES[i]), "-o", objs[i]]
+        return _run(cmd)
+
+    workers = max(1, min(len(SOURCES), (os.cpu_count() or 1) // 2, 8))
+    with ThreadPoolExecutor(workers) as pool:
+        for out in pool.map(compile_one, range(len(SOURCES))):
+            if verbose and out.strip():
+                print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)

@@ -141,7 +141,9 @@ bool build_bvh_host(const float* tri_pos, int num_tris, int num_bins, int leaf_m
                 nan_bins = true;
                 break;
             }
-            const int b = (int)x;
+            // bins * l / len can round up to bins when bins is not a power of two (the
+            // reference then indexes one past its bin array): clamped to the last bin
+            const int b = std::min((int)x, num_bins - 1);
             bin[id] = b;
             bbox[b].merge(pbox[id]);
             bcnt[b]++;
@@ -243,13 +245,18 @@ void pack_bvh2(const HostBvh& h, std::vector<float4>& nodes, std::vector<int2>& 
         st.push_back(h.nodes[i].child[0]);
     }
     nodes.resize(4 * (size_t)inner);
+    // big leaves are numbered in leaf-slot order
+    for (const HostNode& x : h.nodes)
+        if (x.leaf && x.end - x.begin > PT_REF_MAX_SMALL_LEAF) big_leaves.push_back(make_int2(x.begin, x.end - x.begin));
+    std::sort(big_leaves.begin(), big_leaves.end(), [](int2 a, int2 b) { return a.x < b.x; });
     auto ref_of = [&](int c) -> uint32_t {
         const HostNode& x = h.nodes[c];
         if (!x.leaf) return pt_ref_inner((uint32_t)gidx[c]);
         const int cnt = x.end - x.begin;
         if (cnt <= PT_REF_MAX_SMALL_LEAF) return pt_ref_leaf((uint32_t)x.begin, (uint32_t)cnt);
-        big_leaves.push_back(make_int2(x.begin, cnt));
-        return pt_ref_big_leaf((uint32_t)(big_leaves.size() - 1));
+        const auto it = std::lower_bound(big_leaves.begin(), big_leaves.end(), x.begin,
+                                         [](int2 a, int b) { return a.x < b; });
+        return pt_ref_big_leaf((uint32_t)(it - big_leaves.begin()));
     };
     for (int i = 0; i < nn; ++i) {
         const HostNode& x = h.nodes[i];

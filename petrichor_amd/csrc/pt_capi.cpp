@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/petrichor_hip.h"
+#include "pt_build.h"
 #include "pt_internal.h"
 
 using namespace pt;
@@ -51,6 +52,147 @@ MaterialDev material_from(int type, const float* p)
     m.mat1 = type == kMatMix ? (int)p[5] : -1;
     m.is_enabled = p[6] != 0.0f ? 1 : 0;
     return m;
+}
+
+} // namespace
+
+// ---- device-built trees ---------------------------------------------------------------
+
+namespace {
+
+int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
+{
+    pt_context* ctx = s->ctx;
+    PT_HIP(ctx, hipSetDevice(ctx->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    GpuBvhResult r;
+    hipError_t e = gpu_build_bvh(s->tri_pos_d, s->num_tris, bins, leaf_max, ctx->stream, &r);
+    if (e != hipSuccess) {
+        gpu_bvh_free(&r);
+        return fail(ctx, e == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
+                    std::string("device BVH build: ") + hipGetErrorString(e));
+    }
+    if (r.max_depth > PT_MAX_STACK - 2) {
+        gpu_bvh_free(&r);
+        return fail(ctx, PT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
+    }
+    GpuPacked pk;
+    e = gpu_pack_bvh(r, s->num_tris, s->tri_pos_d, s->tri_mat_d, s->mat_shading_of_d, s->tri_smooth_d, s->tri_nrm_d,
+                     ctx->stream, &pk);
+    const int num_nodes = r.num_nodes, max_depth = r.max_depth;
+    gpu_bvh_free(&r);
+    pt_bvh* b = new pt_bvh();
+    b->scene = s;
+    if (e == hipSuccess) e = hipMalloc((void**)&b->cursor_d, 4 * sizeof(int));
+    if (e != hipSuccess) {
+        gpu_packed_free(&pk);
+        if (b->cursor_d) (void)hipFree(b->cursor_d);
+        delete b;
+        return fail(ctx, e == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
+                    std::string("device BVH pack: ") + hipGetErrorString(e));
+    }
+    b->nodes_d = pk.nodes_d;
+    b->big_leaves_d = pk.big_leaves_d;
+    b->tris_d = pk.tris_d;
+    b->prim_ids_d = pk.prim_ids_d;
+    b->tri_mat_d = pk.mat_d;
+    b->tri_mat_shading_d = pk.mat_sh_d;
+    b->tri_smooth_d = pk.smooth_d;
+    b->tri_nrm_d = pk.nrm_d;
+    b->view.nodes = b->nodes_d;
+    b->view.tris = b->tris_d;
+    b->view.prim_ids = b->prim_ids_d;
+    b->view.big_leaves = b->big_leaves_d;
+    b->view.root_lo = pk.root_lo;
+    b->view.root_hi = pk.root_hi;
+    b->view.root_ref = pk.root_ref;
+    b->num_nodes = num_nodes;
+    b->max_depth = max_depth;
+    b->num_inner = pk.num_inner;
+    b->num_big = pk.num_big;
+    b->host_ready = false;
+    b->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    const int64_t nt = s->num_tris;
+    b->device_bytes = (int64_t)pk.num_inner * 64 + (int64_t)pk.num_big * 8 + nt * 60 +
+                      (pk.smooth_d ? nt * 4 : 0) + (pk.nrm_d ? nt * 36 : 0);
+    *out = b;
+    return PT_OK;
+}
+
+// Rebuilds the node table in the reference's numbering from the packed tree: boxes come
+// from the parent's record, leaf ranges from the child references, inner ranges from the
+// leftmost and rightmost leaves. Node ids follow BinnedSAHBVH::Build: the two children
+// of a split get the next two ids when the parent leaves the stack, the right child is
+// taken from the stack first.
+int ensure_host(pt_bvh* b)
+{
+    if (b->host_ready) return PT_OK;
+    pt_context* ctx = b->scene->ctx;
+    const int nt = b->scene->num_tris;
+    std::vector<float4> nodes(4 * (size_t)b->num_inner);
+    std::vector<int2> big((size_t)b->num_big);
+    HostBvh& h = b->host;
+    h.order.resize(nt);
+    PT_HIP(ctx, hipMemcpy(nodes.data(), b->nodes_d, nodes.size() * sizeof(float4), hipMemcpyDeviceToHost));
+    if (!big.empty()) PT_HIP(ctx, hipMemcpy(big.data(), b->big_leaves_d, big.size() * sizeof(int2), hipMemcpyDeviceToHost));
+    PT_HIP(ctx, hipMemcpy(h.order.data(), b->prim_ids_d, (size_t)nt * sizeof(int32_t), hipMemcpyDeviceToHost));
+    auto ref_of = [&](int i, int side) {
+        uint32_t r;
+        std::memcpy(&r, (const char*)&nodes[4 * (size_t)i + 3] + 4 * side, 4);
+        return r;
+    };
+    std::vector<int> beg(b->num_inner), end(b->num_inner);
+    auto leaf_range = [&](uint32_t r, int* lo, int* hi) {
+        const uint32_t c = pt_ref_cnt(r), idx = pt_ref_index(r);
+        if (c == PT_REF_BIG) {
+            *lo = big[idx].x;
+            *hi = big[idx].x + big[idx].y;
+        } else {
+            *lo = (int)idx;
+            *hi = (int)(idx + c);
+        }
+    };
+    for (int i = b->num_inner - 1; i >= 0; --i) { // children follow their parent in preorder
+        int lo, hi;
+        const uint32_t rl = ref_of(i, 0), rr = ref_of(i, 1);
+        if (pt_ref_cnt(rl) == 0) beg[i] = beg[pt_ref_index(rl)];
+        else leaf_range(rl, &beg[i], &hi);
+        if (pt_ref_cnt(rr) == 0) end[i] = end[pt_ref_index(rr)];
+        else leaf_range(rr, &lo, &end[i]);
+    }
+    struct Item {
+        int id;
+        uint32_t ref;
+        V3 lo, hi;
+    };
+    h.nodes.assign(b->num_nodes, HostNode());
+    int next = 1;
+    std::vector<Item> st;
+    st.push_back({0, b->view.root_ref, b->view.root_lo, b->view.root_hi});
+    while (!st.empty()) {
+        const Item it = st.back();
+        st.pop_back();
+        HostNode& x = h.nodes[it.id];
+        x.lo = it.lo;
+        x.hi = it.hi;
+        if (pt_ref_cnt(it.ref) != 0) {
+            x.leaf = true;
+            leaf_range(it.ref, &x.begin, &x.end);
+            continue;
+        }
+        const int i = (int)pt_ref_index(it.ref);
+        const float4* o = &nodes[4 * (size_t)i];
+        x.begin = beg[i];
+        x.end = end[i];
+        x.child[0] = next;
+        x.child[1] = next + 1;
+        next += 2;
+        st.push_back({x.child[0], ref_of(i, 0), v3(o[0].x, o[0].y, o[0].z), v3(o[0].w, o[1].x, o[1].y)});
+        st.push_back({x.child[1], ref_of(i, 1), v3(o[1].z, o[1].w, o[2].x), v3(o[2].y, o[2].z, o[2].w)});
+    }
+    h.max_depth = b->max_depth;
+    b->host_ready = true;
+    return PT_OK;
 }
 
 } // namespace
@@ -176,11 +318,23 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
             e.cdf1d = s->cdf1d_d;
         }
     }
+    std::vector<int32_t> mat_sh(s->mats.size());
+    for (size_t m = 0; m < s->mats.size(); ++m) { // MixMaterial::GetMaterial(0.0f)
+        const MaterialDev& md = s->mats[m];
+        mat_sh[m] = md.type == kMatMix ? (md.mix <= 0.0f ? md.mat0 : md.mat1) : (int32_t)m;
+    }
+    if (err == hipSuccess) err = upload(&s->mat_shading_of_d, mat_sh.data(), mat_sh.size(), ctx->stream);
+    if (err == hipSuccess) err = upload(&s->tri_pos_d, s->tri_pos.data(), s->tri_pos.size(), ctx->stream);
+    if (err == hipSuccess) err = upload(&s->tri_mat_d, s->tri_mat.data(), s->tri_mat.size(), ctx->stream);
+    if (err == hipSuccess) err = upload(&s->tri_smooth_d, s->tri_smooth.data(), s->tri_smooth.size(), ctx->stream);
+    if (err == hipSuccess) err = upload(&s->tri_nrm_d, s->tri_nrm.data(), s->tri_nrm.size(), ctx->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
-    s->device_bytes = (int64_t)(s->mats.size() * sizeof(MaterialDev) + lights.size() * sizeof(float4)) +
-                      (d->env_rgb ? 3 * (int64_t)e.w * e.h * 4 + 8 * (int64_t)e.h : 0);
+    s->device_bytes = (int64_t)(s->mats.size() * (sizeof(MaterialDev) + 4) + lights.size() * sizeof(float4)) +
+                      (d->env_rgb ? 3 * (int64_t)e.w * e.h * 4 + 8 * (int64_t)e.h : 0) +
+                      (int64_t)nt * 40 + (int64_t)(s->tri_smooth.size() * 4 + s->tri_nrm.size() * 4);
     if (err != hipSuccess) {
-        free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->pdf1d_d, s->cdf1d_d});
+        free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->pdf1d_d, s->cdf1d_d, s->mat_shading_of_d, s->tri_pos_d,
+                  s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d});
         delete s;
         return fail(ctx, err == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
                     std::string("scene upload: ") + hipGetErrorString(err));
@@ -192,7 +346,8 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
 int pt_scene_destroy(pt_scene* s)
 {
     if (!s) return PT_ERR_ARG;
-    free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->pdf1d_d, s->cdf1d_d});
+    free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->pdf1d_d, s->cdf1d_d, s->mat_shading_of_d, s->tri_pos_d,
+              s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d});
     delete s;
     return PT_OK;
 }
@@ -205,7 +360,13 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     const int bins = desc ? desc->num_bins : 16;
     const int leaf_max = desc ? desc->leaf_max : 4;
     const int width = desc ? desc->width : 2;
+    const int builder = desc ? desc->builder : PT_BUILD_DEVICE;
     if (width != 2) return fail(ctx, PT_ERR_UNSUPPORTED, "only width 2 is implemented");
+    if (bins < 2 || bins > 256 || leaf_max < 1) return fail(ctx, PT_ERR_ARG, "bad BVH parameters");
+    if (builder != PT_BUILD_DEVICE && builder != PT_BUILD_HOST) return fail(ctx, PT_ERR_ARG, "unknown builder");
+    if (builder == PT_BUILD_DEVICE && bins > 64)
+        return fail(ctx, PT_ERR_UNSUPPORTED, "the device builder takes at most 64 bins");
+    if (builder == PT_BUILD_DEVICE && s->num_tris > 0) return build_device(s, bins, leaf_max, out);
     pt_bvh* b = new pt_bvh();
     b->scene = s;
     const auto t0 = std::chrono::steady_clock::now();
@@ -217,6 +378,9 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
         delete b;
         return fail(ctx, PT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     }
+    b->host_ready = true;
+    b->num_nodes = (int)b->host.nodes.size();
+    b->max_depth = b->host.max_depth;
     std::vector<float4> nodes;
     std::vector<int2> big;
     V3 rlo, rhi;
@@ -286,6 +450,7 @@ int pt_bvh_destroy(pt_bvh* b)
 int pt_bvh_export_order(pt_bvh* b, int32_t* prim_order)
 {
     if (!b || !prim_order) return PT_ERR_ARG;
+    if (int rc = ensure_host(b)) return rc;
     std::memcpy(prim_order, b->host.order.data(), b->host.order.size() * sizeof(int32_t));
     return PT_OK;
 }
@@ -293,6 +458,7 @@ int pt_bvh_export_order(pt_bvh* b, int32_t* prim_order)
 int pt_bvh_export_nodes(pt_bvh* b, float* boxes, int32_t* child, int32_t* range, int32_t* leaf)
 {
     if (!b) return PT_ERR_ARG;
+    if (int rc = ensure_host(b)) return rc;
     for (size_t i = 0; i < b->host.nodes.size(); ++i) {
         const HostNode& n = b->host.nodes[i];
         if (boxes) {
@@ -333,8 +499,8 @@ int pt_bvh_build_host(const float* tri_pos, int32_t num_triangles, const pt_bvh_
 int pt_bvh_info(pt_bvh* b, int32_t* num_nodes, int32_t* max_depth, double* build_ms)
 {
     if (!b) return PT_ERR_ARG;
-    if (num_nodes) *num_nodes = (int32_t)b->host.nodes.size();
-    if (max_depth) *max_depth = b->host.max_depth;
+    if (num_nodes) *num_nodes = b->num_nodes;
+    if (max_depth) *max_depth = b->max_depth;
     if (build_ms) *build_ms = b->build_ms;
     return PT_OK;
 }

@@ -31,7 +31,13 @@ struct pt_scene {
     float* env_rgb_d = nullptr;
     float* pdf1d_d = nullptr;
     float* cdf1d_d = nullptr;
-    int64_t device_bytes = 0; // materials, lights, environment
+    // triangle data resident for the device BVH builder
+    float* tri_pos_d = nullptr;        // 9 floats per triangle
+    int32_t* tri_mat_d = nullptr;
+    int32_t* tri_smooth_d = nullptr;   // null without per-vertex normals
+    float* tri_nrm_d = nullptr;
+    int32_t* mat_shading_of_d = nullptr; // material -> material that shades it (mix resolved)
+    int64_t device_bytes = 0; // materials, lights, environment, triangles
 };
 
 struct pt_bvh {
@@ -47,6 +53,11 @@ struct pt_bvh {
     int32_t* tri_smooth_d = nullptr;
     float* tri_nrm_d = nullptr;
     pt::BvhView view;
+    int num_nodes = 0;
+    int max_depth = 0;
+    int num_inner = 0;
+    int num_big = 0;
+    bool host_ready = false; // `host` holds the node table (built lazily after a device build)
     double build_ms = 0.0;
     int64_t device_bytes = 0; // nodes, leaf-ordered triangles and per-triangle data
 };

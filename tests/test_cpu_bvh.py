@@ -151,6 +151,12 @@ def test_host_builder_32_bins_and_edge_cases():
     h = P.build_bvh_host(sc["tri_pos"], 32)
     for k in ("boxes", "child", "range", "leaf", "prim_ids"):
         assert np.array_equal(e[k], h[k]), k
+    # 12 bins: bins * l / len rounds up to 12 at some nodes (the last bin takes it)
+    sc12 = scenes.random_triangles(4000, seed=24, mu=-4.0)
+    e12 = O.OracleScene(sc12).build_bvh(12).bvh_export()
+    h12 = P.build_bvh_host(sc12["tri_pos"], 12)
+    for k in ("boxes", "child", "range", "leaf", "prim_ids"):
+        assert np.array_equal(e12[k], h12[k]), k
     # a single triangle, four triangles (one leaf), identical triangles (zero-width bins)
     one = scenes.random_triangles(1, seed=1)["tri_pos"]
     assert len(P.build_bvh_host(one)["leaf"]) == 1
