@@ -67,6 +67,177 @@ def cpu_baseline(sc, tree, width, height, spp, depth, seconds):
     }, work
 
 
+def torch_primary_rays(cam, width, height, gen):
+    """Pinhole rays through jittered pixel positions, built on the device (8 floats per ray)."""
+    import torch
+    dev = "cuda"
+    pos, right, up, fwd = (torch.tensor(cam[a:a + 3], dtype=torch.float64, device=dev) for a in (0, 3, 6, 9))
+    focus, hp = float(cam[12]), float(cam[13])
+    n = width * height
+    idx = torch.arange(n, device=dev)
+    i, j = (idx % width).double(), (idx // width).double()
+    u = (i + torch.rand(n, generator=gen, device=dev, dtype=torch.float64)) / width - 0.5
+    v = (j + torch.rand(n, generator=gen, device=dev, dtype=torch.float64)) / height - 0.5
+    d = (right[None] * (u * hp * focus * width / height)[:, None] - up[None] * (v * hp * focus)[:, None]
+         + fwd[None] * focus)
+    d = d / d.norm(dim=1, keepdim=True)
+    r = torch.zeros((n, 8), dtype=torch.float32, device=dev)
+    r[:, 0:3] = pos.float()
+    r[:, 4:7] = d.float()
+    r[:, 7] = float(np.finfo(np.float32).max)
+    return r
+
+
+def torch_surface_rays(tri_pos_dev, n, gen, dist_min):
+    """Rays from uniform points on random triangles with uniform directions on the sphere."""
+    import torch
+    dev = tri_pos_dev.device
+    k = torch.randint(0, tri_pos_dev.shape[0], (n,), generator=gen, device=dev)
+    t = tri_pos_dev[k].view(n, 3, 3).double()
+    u = torch.rand(n, generator=gen, device=dev, dtype=torch.float64)
+    v = torch.rand(n, generator=gen, device=dev, dtype=torch.float64)
+    su = u.sqrt()
+    p = t[:, 0] * (1 - su)[:, None] + t[:, 1] * (su * (1 - v))[:, None] + t[:, 2] * (su * v)[:, None]
+    d = torch.randn((n, 3), generator=gen, device=dev, dtype=torch.float64)
+    d = d / d.norm(dim=1, keepdim=True)
+    r = torch.zeros((n, 8), dtype=torch.float32, device=dev)
+    r[:, 0:3] = p.float()
+    r[:, 3] = dist_min
+    r[:, 4:7] = d.float()
+    r[:, 7] = float(np.finfo(np.float32).max)
+    return r
+
+
+def cpu_trace_baseline(sc, tree, rays_host, seconds):
+    """The CPU oracle's closest hit (restatement of BinnedSAHBVH::Intersect) on a sample of
+    the same rays through the same tree. ctypes releases the GIL during the call, so the
+    chunks run in parallel, one worker thread per core."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    threads = min(os.cpu_count() or 1, 16)
+    o = O.OracleScene(sc).use_tree(tree)
+    n = 1 << 14
+    done, total_t = 0, 0.0
+    pool = ThreadPoolExecutor(threads)
+    while True:
+        batch = rays_host[done:done + n]
+        chunks = np.array_split(batch, threads)
+        t0 = time.perf_counter()
+        list(pool.map(o.trace_closest, chunks))
+        dt = time.perf_counter() - t0
+        done += len(batch)
+        total_t += dt
+        if total_t >= seconds or done >= len(rays_host):
+            break
+        n = int(min(max(n * 2, n * (seconds - total_t) / max(dt, 1e-3)), len(rays_host) - done))
+    pool.shutdown()
+    return {"value": done / total_t / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "%d rays (a random half-primary, half-surface sample of the 2^26), closest hit, %.1f s"
+                      % (done, total_t)}
+
+
+def trace_bench(args):
+    """BVH build and traversal microbenchmark: random triangles with centroids uniform in the
+    unit cube and lognormal(-6, 1) edge lengths, binned SAH with 32 bins and leaves of at
+    most 4, half pinhole primary rays and half surface rays with uniform directions;
+    closest hit and any hit timed separately."""
+    import torch
+    import petrichor_amd as P
+    from petrichor_amd import scenes
+    N = args.triangles
+    log("generating %d triangles" % N)
+    sc = scenes.random_triangles(N, seed=3, mu=-6.0, sigma=1.0)
+    ctx = P.Context(0)
+    s = ctx.scene(sc)
+    builds = []
+    bvh = None
+    for i in range(args.warmup + args.steps):
+        if bvh is not None:
+            bvh.close()
+        bvh = s.build_bvh(32, 4, builder=P.BUILD_DEVICE)
+        info = bvh.info()
+        if i >= args.warmup:
+            builds.append(info["build_ms"])
+        log("device build %d: %.1f ms, %d nodes, depth %d" % (i, info["build_ms"], info["num_nodes"],
+                                                              info["max_depth"]))
+    build_ms = float(np.median(builds))
+    nr = 1 << args.log2_rays
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    w = 1 << ((args.log2_rays - 1) // 2 + 1)
+    h = (nr // 2) // w
+    rays = torch.empty((nr, 8), dtype=torch.float32, device="cuda")
+    rays[: nr // 2] = torch_primary_rays(sc["camera"], w, h, gen)
+    tri_dev = torch.from_numpy(sc["tri_pos"]).cuda()
+    rays[nr // 2:] = torch_surface_rays(tri_dev, nr - nr // 2, gen, 1e-5)
+    del tri_dev
+    t_out = torch.empty(nr, dtype=torch.float32, device="cuda")
+    p_out = torch.empty(nr, dtype=torch.int32, device="cuda")
+    hit_out = torch.empty(nr, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        evs = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            evs.append((a, b))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall, [a.elapsed_time(b) for a, b in evs]
+
+    wall_c, ms_c = timed(lambda: bvh.trace_closest(rays, t_out, p_out, stream=stream.cuda_stream))
+    wall_a, ms_a = timed(lambda: bvh.trace_any(rays, hit_out, stream=stream.cuda_stream))
+    hits = int((p_out >= 0).sum().item())
+    assert torch.equal(hit_out, (p_out >= 0).int()), "any hit must agree with closest hit"
+    log("closest: %.2f ms/launch, any: %.2f ms/launch, %d hits" % (np.mean(ms_c), np.mean(ms_a), hits))
+    _, _, st = bvh.trace_stats(rays)
+    n_node, n_tri = st[0] / nr, st[1] / nr
+    b_ray = RAY_IN_B + HIT_OUT_B + n_node * NODE_B + n_tri * TRI_B
+    avg_ms = float(np.mean(ms_c))
+    achieved = nr * b_ray / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_trace_persistent (closest hit, one launch over all rays)",
+                "bytes_per_query": round(b_ray, 1), "n_node": round(n_node, 2), "n_tri": round(n_tri, 3),
+                "s_node": NODE_B, "s_tri": TRI_B, "avg_launch_ms": round(avg_ms, 3), "queries_per_launch": nr}
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        tree = bvh.export_nodes() | {"prim_ids": bvh.export_order()}
+        sel = torch.randperm(nr, generator=torch.Generator().manual_seed(11))[: 1 << 21]
+        sample = rays[sel.cuda()].cpu().numpy()
+        cpu = cpu_trace_baseline(sc, tree, sample, args.cpu_seconds)
+        log("cpu:", cpu)
+    value = nr * args.steps / wall_c / 1e6
+    res = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall_c / args.steps * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "BVH microbenchmark: %d random triangles (centroids U[0,1]^3, edges lognormal(-6,1)), "
+                               "2^%d rays (half pinhole primary, half surface rays with uniform directions), "
+                               "closest hit" % (N, args.log2_rays),
+                   "triangles": N, "rays": nr, "bvh": "binned SAH, 32 bins, leaf<=4, binary (reference topology), "
+                                                   "device builder"},
+        "closest_mrays_s": round(value, 2),
+        "any_mrays_s": round(nr * args.steps / wall_a / 1e6, 2),
+        "closest_ms_per_launch": round(avg_ms, 3), "any_ms_per_launch": round(float(np.mean(ms_a)), 3),
+        "hit_fraction": round(hits / nr, 4),
+        "bvh_build_ms": round(build_ms, 1), "bvh_build_mprims_s": round(N / build_ms / 1e3, 1),
+        "bvh_nodes": info["num_nodes"], "bvh_depth": info["max_depth"],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,9 +249,13 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--depth", type=int, default=12)
-    ap.add_argument("--mode", default="frame", choices=["frame", "copy"],
-                    help="copy: a plain 1 GiB read + 1 GiB write elementwise kernel, the reference "
-                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--mode", default="frame", choices=["frame", "trace", "copy"],
+                    help="frame: the single large scene (default); trace: the BVH build and "
+                         "traversal microbenchmark (10M triangles, 2^26 rays); copy: a plain 1 GiB "
+                         "read + 1 GiB write elementwise kernel, the reference for scaling rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--triangles", type=int, default=10_000_000, help="trace mode: triangle count")
+    ap.add_argument("--log2-rays", type=int, default=26, help="trace mode: log2 of the ray count")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -100,6 +275,8 @@ def main():
         print(json.dumps({"mode": "copy", "launches": 4, "read_bytes_per_launch": 1 << 30,
                           "write_bytes_per_launch": 1 << 30}), flush=True)
         return
+    if args.mode == "trace":
+        return trace_bench(args)
     log("generating scene")
     t0 = time.perf_counter()
     sc = scenes.spheres_scene(seed=2)
