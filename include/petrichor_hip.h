@@ -191,6 +191,75 @@ int pt_engine_destroy(pt_engine* engine);
 int pt_render_batch(pt_engine* engine, int32_t n, const pt_job_desc* jobs, int32_t iterations_per_sync,
                     pt_batch_stats* stats);
 
+/* ---- Serving ----------------------------------------------------------------------------
+ * A render server keeps scenes and their BVHs resident on the device and runs requests
+ * with continuous batching: a host thread admits queued requests into the shared path
+ * pool whenever path slots and budget allow, so requests that arrive while others run
+ * join the same graph-replayed iterations. No reference counterpart: the reference
+ * renders one scene per Petrichor::Render call (Core/Petrichor.cpp:34-102), with a
+ * progress callback (Petrichor.h:33-38) that pt_server_preview stands in for.
+ * Every function is thread-safe. */
+typedef struct pt_server pt_server;
+
+typedef struct pt_server_desc {
+    int32_t device;
+    int32_t max_jobs;            /* requests running at once (0 = 64) */
+    int64_t hbm_budget_bytes;    /* device memory for scenes, BVHs, the path pool and request images */
+    int64_t max_paths;           /* path slots shared by running requests (0 = 8M) */
+    int32_t iterations_per_sync; /* graph replays between host synchronisations (0 = 8) */
+} pt_server_desc;
+
+#define PT_REQ_QUEUED 0
+#define PT_REQ_RUNNING 1
+#define PT_REQ_DONE 2
+#define PT_REQ_FAILED 3
+
+typedef struct pt_request_info {
+    int32_t state;     /* PT_REQ_* */
+    float progress;    /* completed samples / (pixels * spp) as of the last preview; 1 when done */
+    double queue_ms;   /* submission to admission (running or done) */
+    double latency_ms; /* submission to the final image on the host (done) */
+} pt_request_info;
+
+typedef struct pt_server_stats {
+    int64_t submitted, completed, running, queued;
+    int64_t resident_scenes, evicted_scenes;
+    int64_t bytes_in_use, budget_bytes;
+    int64_t queries;    /* BVH queries of every request so far */
+    int64_t iterations; /* wavefront iterations */
+    int64_t syncs;      /* host synchronisations of the request loop */
+    double busy_ms;     /* loop time with requests on the device */
+} pt_server_stats;
+
+int pt_server_create(const pt_server_desc* desc, pt_server** out);
+/* Stops the request loop (running requests are abandoned) and frees everything. */
+int pt_server_destroy(pt_server* server);
+/* Error text of the calling thread's last failed pt_server_* call. */
+const char* pt_server_last_error(void);
+/* Uploads a scene and builds its BVH (desc NULL = 16 bins, leaf <= 4, device builder).
+ * When the budget is short, idle scenes (no queued or running request) are evicted,
+ * least recently used first; PT_ERR_BUDGET if that is not enough. */
+int pt_server_add_scene(pt_server* server, const pt_scene_desc* scene, const pt_bvh_desc* bvh, int64_t* scene_id);
+/* PT_ERR_STATE if the scene was evicted or removed. A scene with requests is removed
+ * when they finish. */
+int pt_server_remove_scene(pt_server* server, int64_t scene_id);
+int pt_server_scene_resident(pt_server* server, int64_t scene_id);
+/* Queues a request; PT_ERR_STATE for an unknown or evicted scene, PT_ERR_BUDGET for a
+ * request that can never fit the path pool or the budget. */
+int pt_server_submit(pt_server* server, int64_t scene_id, const pt_render_desc* render, int64_t* request_id);
+int pt_server_poll(pt_server* server, int64_t request_id, pt_request_info* info);
+/* Waits until the request is done or failed, at most timeout_ms (< 0: no limit);
+ * returns PT_OK with info filled, whichever state it is in. */
+int pt_server_wait(pt_server* server, int64_t request_id, int32_t timeout_ms, pt_request_info* info);
+/* Progressive result: the current estimate (finished pixels final, others the mean of
+ * their completed samples, black before the first) into a host buffer of W*H*3 floats.
+ * Blocks until the loop's next synchronisation while the request runs. */
+int pt_server_preview(pt_server* server, int64_t request_id, float* rgb_out, float* progress);
+/* Final image of a done request into a host buffer of W*H*3 floats; the request is
+ * then released. PT_ERR_STATE before it is done. */
+int pt_server_fetch(pt_server* server, int64_t request_id, float* rgb_out);
+int pt_server_get_stats(pt_server* server, pt_server_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

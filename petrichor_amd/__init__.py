@@ -66,6 +66,25 @@ class BatchStats(ctypes.Structure):
                 ("wall_ms", ctypes.c_double), ("syncs", ctypes.c_int64)]
 
 
+class ServerDesc(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("max_jobs", ctypes.c_int32), ("hbm_budget_bytes", ctypes.c_int64),
+                ("max_paths", ctypes.c_int64), ("iterations_per_sync", ctypes.c_int32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("progress", ctypes.c_float), ("queue_ms", ctypes.c_double),
+                ("latency_ms", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "completed", "running", "queued", "resident_scenes",
+                                             "evicted_scenes", "bytes_in_use", "budget_bytes", "queries",
+                                             "iterations", "syncs")] + [("busy_ms", ctypes.c_double)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
 class RenderStats(ctypes.Structure):
     _fields_ = [("queries", ctypes.c_int64), ("iterations", ctypes.c_int64), ("elapsed_ms", ctypes.c_double),
                 ("kernel_items", ctypes.c_int64 * 4), ("kernel_launches", ctypes.c_int64 * 4),
@@ -105,6 +124,20 @@ def lib():
     L.pt_engine_destroy.argtypes = [vp]
     L.pt_render_batch.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(JobDesc), ctypes.c_int32,
                                   ctypes.POINTER(BatchStats)]
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    L.pt_server_create.argtypes = [ctypes.POINTER(ServerDesc), pvp]
+    L.pt_server_destroy.argtypes = [vp]
+    L.pt_server_last_error.restype = ctypes.c_char_p
+    L.pt_server_last_error.argtypes = []
+    L.pt_server_add_scene.argtypes = [vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(BvhDesc), i64p]
+    L.pt_server_remove_scene.argtypes = [vp, ctypes.c_int64]
+    L.pt_server_scene_resident.argtypes = [vp, ctypes.c_int64]
+    L.pt_server_submit.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(RenderDesc), i64p]
+    L.pt_server_poll.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(RequestInfo)]
+    L.pt_server_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(RequestInfo)]
+    L.pt_server_preview.argtypes = [vp, ctypes.c_int64, _f, ctypes.POINTER(ctypes.c_float)]
+    L.pt_server_fetch.argtypes = [vp, ctypes.c_int64, _f]
+    L.pt_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
     _lib = L
     return L
 
@@ -113,7 +146,9 @@ def lib():
 EXPORTED = ["pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_scene_create", "pt_scene_destroy",
             "pt_bvh_build", "pt_bvh_destroy", "pt_bvh_export_order", "pt_bvh_export_nodes", "pt_bvh_build_host", "pt_bvh_info", "pt_trace_closest",
             "pt_trace_any", "pt_trace_stats", "pt_render", "pt_engine_create", "pt_engine_destroy",
-            "pt_render_batch"]
+            "pt_render_batch", "pt_server_create", "pt_server_destroy", "pt_server_last_error",
+            "pt_server_add_scene", "pt_server_remove_scene", "pt_server_scene_resident", "pt_server_submit",
+            "pt_server_poll", "pt_server_wait", "pt_server_preview", "pt_server_fetch", "pt_server_get_stats"]
 
 
 def _ptr(a, t):
@@ -180,43 +215,7 @@ class Scene:
         self.ctx = ctx
         self.src = scene
         L = ctx._lib
-        keep = self._keep = []
-
-        def k(a, dt):
-            a = np.ascontiguousarray(a, dtype=dt)
-            keep.append(a)
-            return a
-
-        d = SceneDesc()
-        tri_pos = k(scene["tri_pos"], np.float32).reshape(-1, 9)
-        d.num_triangles = tri_pos.shape[0]
-        d.positions = _ptr(tri_pos, _f)
-        d.normals = _ptr(k(scene["tri_nrm"], np.float32), _f) if scene.get("tri_nrm") is not None else None
-        d.material_ids = _ptr(k(scene["tri_mat"], np.int32), _i)
-        d.smooth = _ptr(k(scene["tri_smooth"], np.int32), _i) if scene.get("tri_smooth") is not None else None
-        mt = k(scene["mat_type"], np.int32)
-        d.num_materials = len(mt)
-        d.material_types = _ptr(mt, _i)
-        d.material_params = _ptr(k(scene["mat_param"], np.float32), _f)
-        lt = k(scene.get("light_tris", np.zeros(0, np.int32)), np.int32)
-        d.num_lights = len(lt)
-        d.light_triangles = _ptr(lt, _i)
-        cam = np.asarray(scene["camera"], np.float32)
-        for i in range(16):
-            d.camera[i] = float(cam[i])
-        env = scene.get("env_rgb")
-        if env is not None:
-            env = k(env, np.float32)
-            d.env_height, d.env_width = env.shape[0], env.shape[1]
-            d.env_rgb = _ptr(env, _f)
-        else:
-            d.env_width = d.env_height = 0
-            d.env_rgb = None
-        ep = np.asarray(scene["env_param"], np.float32)
-        for i in range(3):
-            d.env_base_color[i] = float(ep[i])
-        d.env_z_rotation = float(ep[3])
-        d.env_importance_sampling = int(ep[4] != 0)
+        d, self._keep = scene_desc(scene)
         h = ctypes.c_void_p()
         ctx.check(L.pt_scene_create(ctx.handle, ctypes.byref(d), ctypes.byref(h)), "pt_scene_create")
         self.handle = h
@@ -236,6 +235,49 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+
+def scene_desc(scene):
+    """pt_scene_desc of a scene dict (petrichor_amd.scenes); returns (desc, arrays that
+    must stay alive while the descriptor is used)."""
+    keep = []
+
+    def k(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a
+
+    d = SceneDesc()
+    tri_pos = k(scene["tri_pos"], np.float32).reshape(-1, 9)
+    d.num_triangles = tri_pos.shape[0]
+    d.positions = _ptr(tri_pos, _f)
+    d.normals = _ptr(k(scene["tri_nrm"], np.float32), _f) if scene.get("tri_nrm") is not None else None
+    d.material_ids = _ptr(k(scene["tri_mat"], np.int32), _i)
+    d.smooth = _ptr(k(scene["tri_smooth"], np.int32), _i) if scene.get("tri_smooth") is not None else None
+    mt = k(scene["mat_type"], np.int32)
+    d.num_materials = len(mt)
+    d.material_types = _ptr(mt, _i)
+    d.material_params = _ptr(k(scene["mat_param"], np.float32), _f)
+    lt = k(scene.get("light_tris", np.zeros(0, np.int32)), np.int32)
+    d.num_lights = len(lt)
+    d.light_triangles = _ptr(lt, _i)
+    cam = np.asarray(scene["camera"], np.float32)
+    for i in range(16):
+        d.camera[i] = float(cam[i])
+    env = scene.get("env_rgb")
+    if env is not None:
+        env = k(env, np.float32)
+        d.env_height, d.env_width = env.shape[0], env.shape[1]
+        d.env_rgb = _ptr(env, _f)
+    else:
+        d.env_width = d.env_height = 0
+        d.env_rgb = None
+    ep = np.asarray(scene["env_param"], np.float32)
+    for i in range(3):
+        d.env_base_color[i] = float(ep[i])
+    d.env_z_rotation = float(ep[3])
+    d.env_importance_sampling = int(ep[4] != 0)
+    return d, keep
 
 
 class Bvh:
@@ -387,6 +429,90 @@ class Engine:
     def close(self):
         if getattr(self, "handle", None):
             self.ctx._lib.pt_engine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Render server (pt_server_*): resident scenes, queued requests and a request loop
+    with continuous batching."""
+
+    def __init__(self, budget_bytes, device=0, max_paths=0, max_jobs=0, iterations_per_sync=0):
+        self._lib = lib()
+        d = ServerDesc(device, max_jobs, int(budget_bytes), int(max_paths), iterations_per_sync)
+        h = ctypes.c_void_p()
+        self._check(self._lib.pt_server_create(ctypes.byref(d), ctypes.byref(h)), "pt_server_create")
+        self.handle = h
+        self._shapes = {}
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._lib.pt_server_last_error()
+            raise PtError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def add_scene(self, scene, bins=16, leaf_max=4, builder=BUILD_DEVICE):
+        d, keep = scene_desc(scene)
+        bd = BvhDesc(bins, leaf_max, 2, builder)
+        sid = ctypes.c_int64()
+        self._check(self._lib.pt_server_add_scene(self.handle, ctypes.byref(d), ctypes.byref(bd), ctypes.byref(sid)),
+                    "pt_server_add_scene")
+        del keep
+        return sid.value
+
+    def remove_scene(self, scene_id):
+        self._check(self._lib.pt_server_remove_scene(self.handle, scene_id), "pt_server_remove_scene")
+
+    def resident(self, scene_id):
+        return bool(self._lib.pt_server_scene_resident(self.handle, scene_id))
+
+    def submit(self, scene_id, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1):
+        rd = RenderDesc(width, height, spp, max_bounces, integrator, seed, 0)
+        rid = ctypes.c_int64()
+        self._check(self._lib.pt_server_submit(self.handle, scene_id, ctypes.byref(rd), ctypes.byref(rid)),
+                    "pt_server_submit")
+        self._shapes[rid.value] = (height, width, 3)
+        return rid.value
+
+    @staticmethod
+    def _info(i):
+        return {"state": i.state, "progress": i.progress, "queue_ms": i.queue_ms, "latency_ms": i.latency_ms}
+
+    def poll(self, rid):
+        i = RequestInfo()
+        self._check(self._lib.pt_server_poll(self.handle, rid, ctypes.byref(i)), "pt_server_poll")
+        return self._info(i)
+
+    def wait(self, rid, timeout_ms=-1):
+        i = RequestInfo()
+        self._check(self._lib.pt_server_wait(self.handle, rid, int(timeout_ms), ctypes.byref(i)), "pt_server_wait")
+        return self._info(i)
+
+    def preview(self, rid):
+        img = np.zeros(self._shapes[rid], np.float32)
+        pr = ctypes.c_float()
+        self._check(self._lib.pt_server_preview(self.handle, rid, _ptr(img, _f), ctypes.byref(pr)),
+                    "pt_server_preview")
+        return img, pr.value
+
+    def fetch(self, rid):
+        img = np.zeros(self._shapes[rid], np.float32)
+        self._check(self._lib.pt_server_fetch(self.handle, rid, _ptr(img, _f)), "pt_server_fetch")
+        del self._shapes[rid]
+        return img
+
+    def stats(self):
+        st = ServerStats()
+        self._check(self._lib.pt_server_get_stats(self.handle, ctypes.byref(st)), "pt_server_get_stats")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.pt_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

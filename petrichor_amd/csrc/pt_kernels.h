@@ -107,6 +107,11 @@ hipError_t launch_zero(int* counters, int n, hipStream_t st);
 // Compute units of the current device (queried once, then cached; called before any
 // graph capture so that no device query happens while capturing).
 int cu_count();
+
+// Progressive image of a running job into out (3 floats per pixel); adds the job's
+// completed samples to *samples.
+hipError_t launch_preview(const JobView& job, const PathView& ps, float* out, unsigned long long* samples,
+                          hipStream_t st);
 // Sets up every path of a job (seeds, sums, first camera ray) and appends the paths to
 // the live list and their camera rays to the extension queue.
 hipError_t launch_paths_init(const SceneView& sc, const JobView& job, const PathView& ps, const Queue& ext,

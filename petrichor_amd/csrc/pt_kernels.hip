@@ -430,6 +430,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
                 o[1] = avg.y;
                 o[2] = avg.z;
                 atomicAdd(job.done, 1);
+                ps.meta[p] = make_int4(spp, 0, kPhDone, 0); // read by the progressive preview
                 phase = kPhDone;
             }
         }
@@ -518,6 +519,47 @@ hipError_t launch_paths_init(const SceneView& sc, const JobView& job, const Path
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_paths_init, dim3(grid_for(n, 256)), dim3(256), 0, st, sc, job, ps, ext, active,
                        active_count);
+    return hipGetLastError();
+}
+
+// Progressive image of a running job: finished pixels from the output, the others as the
+// mean of their completed samples (black before the first); adds the completed samples
+// to *samples.
+__global__ void __launch_bounds__(256) k_preview(JobView job, PathView ps, float* out,
+                                                 unsigned long long* samples)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = job.width * job.height;
+    unsigned long long c = 0;
+    if (i < n) {
+        const int p = job.path_base + i;
+        const int s = ps.meta[p].x;
+        float r = 0.0f, g = 0.0f, b = 0.0f;
+        if (s >= job.spp) {
+            r = job.out[3 * (size_t)i];
+            g = job.out[3 * (size_t)i + 1];
+            b = job.out[3 * (size_t)i + 2];
+        } else if (s > 0) {
+            const float4 sum = ps.sum[p];
+            r = sum.x / (float)s;
+            g = sum.y / (float)s;
+            b = sum.z / (float)s;
+        }
+        out[3 * (size_t)i] = r;
+        out[3 * (size_t)i + 1] = g;
+        out[3 * (size_t)i + 2] = b;
+        c = (unsigned long long)min(s, job.spp);
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(samples, c);
+}
+
+hipError_t launch_preview(const JobView& job, const PathView& ps, float* out, unsigned long long* samples,
+                          hipStream_t st)
+{
+    const int64_t n = (int64_t)job.width * job.height;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_preview, dim3(grid_for(n, 256)), dim3(256), 0, st, job, ps, out, samples);
     return hipGetLastError();
 }
 
