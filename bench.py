@@ -238,6 +238,234 @@ def trace_bench(args):
     print(json.dumps(res), flush=True)
 
 
+RESOLUTIONS = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+
+
+def job_specs(n, rng, tris_max, spp_max):
+    """Render jobs of the batched and served workloads: triangles log-uniform in
+    [1e3, tris_max], resolution from RESOLUTIONS, spp log-uniform in [4, spp_max],
+    max depth uniform in 8..12."""
+    out = []
+    for _ in range(n):
+        tris = int(round(np.exp(rng.uniform(np.log(1e3), np.log(tris_max)))))
+        w, h = RESOLUTIONS[rng.integers(0, len(RESOLUTIONS))]
+        spp = int(round(np.exp(rng.uniform(np.log(4), np.log(spp_max)))))
+        depth = int(rng.integers(8, 13))
+        out.append({"tris": tris, "width": w, "height": h, "spp": spp, "depth": depth})
+    return out
+
+
+def batch_bench(args):
+    """Batched form: jobs of uneven scene size, resolution and spp in one graph-replayed
+    multi-job engine (autotuned), against the same jobs rendered one by one with plain
+    launches (one synchronisation per iteration) and with the single-job graph path."""
+    import torch
+    import petrichor_amd as P
+    from petrichor_amd import autotune, scenes
+    rng = np.random.default_rng(2024)
+    specs = job_specs(args.jobs, rng, args.tris_max, args.spp_max)
+    env = scenes.sky_env()
+    ctx = P.Context(0)
+    t0 = time.perf_counter()
+    bvhs = []
+    for i, sp in enumerate(specs):
+        sc = scenes.scaled_scene(sp["tris"], seed=1000 + i, env=env)
+        sp["tris"] = int(len(sc["tri_mat"]))
+        bvhs.append(ctx.scene(sc).build_bvh(16))
+    log("%d scenes (%d triangles) generated, uploaded and built in %.1f s"
+        % (len(specs), sum(sp["tris"] for sp in specs), time.perf_counter() - t0))
+
+    def outs():
+        return [torch.empty((sp["height"], sp["width"], 3), dtype=torch.float32, device="cuda") for sp in specs]
+
+    def jobs_with(o):
+        return [P.Job(b, sp["width"], sp["height"], sp["spp"], sp["depth"], P.INTEGRATOR_MIS, 7 + i, out=o[i])
+                for i, (b, sp) in enumerate(zip(bvhs, specs))]
+
+    out_b = outs()
+    jobs = jobs_with(out_b)
+    best = autotune.tune(ctx, jobs, log=log)
+    eng = P.Engine(ctx, best["max_paths"], max_jobs=len(jobs))
+    eng.set_tuning(**best["tuning"])
+    eng.render_batch(autotune.calibration_jobs(jobs))  # warm-up at 1 spp
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = eng.render_batch(jobs)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    log("batched: %.2f s, %d queries, %d iterations, %d syncs" % (t_batch, st["queries"], st["iterations"],
+                                                                   st["syncs"]))
+    out_p = outs()
+    t0 = time.perf_counter()
+    q_plain = 0
+    for i, (b, sp) in enumerate(zip(bvhs, specs)):
+        _, s1 = b.render(sp["width"], sp["height"], sp["spp"], sp["depth"], P.INTEGRATOR_MIS, 7 + i, out=out_p[i],
+                         plain=True)
+        q_plain += s1["queries"]
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    log("plain launches, one job at a time: %.2f s" % t_plain)
+    out_g = outs()
+    t0 = time.perf_counter()
+    for i, (b, sp) in enumerate(zip(bvhs, specs)):
+        b.render(sp["width"], sp["height"], sp["spp"], sp["depth"], P.INTEGRATOR_MIS, 7 + i, out=out_g[i])
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t0
+    log("single-job graph path, one job at a time: %.2f s" % t_graph)
+    same_plain = all(torch.equal(a, b) for a, b in zip(out_b, out_p))
+    same_graph = all(torch.equal(a, b) for a, b in zip(out_b, out_g))
+    assert same_plain and same_graph, "batched images must equal the per-job images bit for bit"
+    assert q_plain == st["queries"]
+    n = len(specs)
+    res = {
+        "metric": METRIC, "value": round(n / t_batch, 3), "unit": "jobs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(t_batch * 1e3, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched form: %d jobs, scenes of 1e3-%.0e triangles (scaled single-large-scene "
+                               "generator, 2048x1024 sky), resolutions 256^2/512^2/1280x720/1920x1080, spp "
+                               "log-uniform 4-%d, depth 8-12, MIS" % (n, args.tris_max, args.spp_max),
+                   "jobs": n, "triangles_total": int(sum(sp["tris"] for sp in specs)),
+                   "samples_total": int(sum(sp["width"] * sp["height"] * sp["spp"] for sp in specs))},
+        "batched_jobs_s": round(n / t_batch, 3),
+        "plain_launch_jobs_s": round(n / t_plain, 3),
+        "single_graph_jobs_s": round(n / t_graph, 3),
+        "speedup_vs_plain_launches": round(t_plain / t_batch, 2),
+        "speedup_vs_single_graph": round(t_graph / t_batch, 2),
+        "batched_mrays_s": round(st["queries"] / t_batch / 1e6, 1),
+        "queries": int(st["queries"]),
+        "images_bit_identical": bool(same_plain and same_graph),
+        "autotune": {"max_paths": best["max_paths"], "tuning": best["tuning"], "trials": best["trials"],
+                     "calibration_ms": round(best["seconds"] * 1e3, 1)},
+        "jobs_detail": specs,
+    }
+    print(json.dumps(res), flush=True)
+
+
+def serve_bench(args):
+    """Served form: Poisson arrivals of render requests (batched-form job distribution, a
+    quarter reusing a resident scene) against a render server with an HBM budget; latency
+    is arrival to final image on the host."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    import petrichor_amd as P
+    from petrichor_amd import scenes
+    rng = np.random.default_rng(99)
+    n_req = int(rng.poisson(args.rate * args.duration))
+    specs = job_specs(n_req, rng, args.tris_max, args.spp_max)
+    arrivals = np.sort(rng.uniform(0, args.duration, n_req))
+    reuse = rng.uniform(0, 1, n_req) < 0.25
+    env = scenes.sky_env()
+    # geometry of the new scenes is generated before the stream starts (it stands for
+    # scene files a client would upload); upload and BVH build happen per request
+    t0 = time.perf_counter()
+    pool_n = min(n_req, 48)
+    geo = [scenes.scaled_scene(specs[i]["tris"], seed=5000 + i, env=env) for i in range(pool_n)]
+    log("%d requests over %.0f s; %d scene geometries generated in %.1f s" % (n_req, args.duration, pool_n,
+                                                                            time.perf_counter() - t0))
+    srv = P.Server(int(args.budget_gb * (1 << 30)), max_paths=1 << 23, max_jobs=128, iterations_per_sync=8)
+    lock = threading.Lock()
+    resident = []          # scene ids added so far (some may be evicted later)
+    outstanding = {}       # request id -> (index, arrival, submission), seconds from the start
+    lat, queue_ms, failed, evicted_retry = [], [], [0], [0]
+
+    def submit(i, t_arr):
+        sp = specs[i]
+        sid = None
+        with lock:
+            live = [x for x in resident if srv.resident(x)]
+            if reuse[i] and live:
+                sid = live[int(rng.integers(0, len(live)))]
+        if sid is None:
+            sid = srv.add_scene(geo[i % pool_n])
+            with lock:
+                resident.append(sid)
+        args_ = (sp["width"], sp["height"], sp["spp"], sp["depth"], P.INTEGRATOR_MIS, 11 + i)
+        try:
+            rid = srv.submit(sid, *args_)
+        except P.PtError:
+            # the scene was evicted between the check and the submission: upload it again
+            with lock:
+                evicted_retry[0] += 1
+            sid = srv.add_scene(geo[i % pool_n])
+            rid = srv.submit(sid, *args_)
+        with lock:
+            outstanding[rid] = (i, t_arr, time.perf_counter() - t_start)
+
+    def collect(deadline):
+        while True:
+            with lock:
+                items = list(outstanding.items())
+            if not items and arrivals_done[0]:
+                return
+            if time.perf_counter() - t_start > deadline:
+                with lock:
+                    failed[0] += len(outstanding)
+                return
+            for rid, (i, t_arr, t_sub) in items:
+                info = srv.poll(rid)
+                if info["state"] == P.REQ_DONE:
+                    srv.fetch(rid)
+                    with lock:
+                        del outstanding[rid]
+                        lat.append(info["latency_ms"] + (t_sub - t_arr) * 1e3)
+                        queue_ms.append(info["queue_ms"])
+                elif info["state"] == P.REQ_FAILED:
+                    with lock:
+                        del outstanding[rid]
+                        failed[0] += 1
+            time.sleep(0.002)
+
+    arrivals_done = [False]
+    pool = ThreadPoolExecutor(16)
+    t_start = time.perf_counter()
+    collector = threading.Thread(target=collect, args=(args.duration + args.drain,))
+    collector.start()
+    futs = []
+    next_log = 10.0
+    for i in range(n_req):
+        dt = arrivals[i] - (time.perf_counter() - t_start)
+        if dt > 0:
+            time.sleep(dt)
+        futs.append(pool.submit(submit, i, arrivals[i]))
+        if arrivals[i] > next_log:
+            st = srv.stats()
+            log("t=%.0f s: %d submitted, %d done, %d running, %d queued" % (arrivals[i], st["submitted"],
+                                                                             st["completed"], st["running"],
+                                                                             st["queued"]))
+            next_log += 10.0
+    for f in futs:
+        f.result()
+    arrivals_done[0] = True
+    while collector.is_alive():
+        collector.join(10.0)
+        st = srv.stats()
+        log("draining: %d done, %d running, %d queued" % (st["completed"], st["running"], st["queued"]))
+    wall = time.perf_counter() - t_start
+    pool.shutdown()
+    st = srv.stats()
+    srv.close()
+    lat_a = np.array(lat)
+    res = {
+        "metric": METRIC, "value": round(len(lat) / wall, 3), "unit": "jobs/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": round(wall * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "served form: Poisson %.1f req/s for %.0f s, batched-form job distribution "
+                               "(spp <= %d, scenes <= %.0e triangles), 25%% of requests on a resident scene, "
+                               "%.0f GB budget" % (args.rate, args.duration, args.spp_max, args.tris_max,
+                                                   args.budget_gb),
+                   "requests": n_req},
+        "completed": len(lat), "failed": failed[0],
+        "p50_latency_ms": round(float(np.percentile(lat_a, 50)), 1) if len(lat_a) else None,
+        "p99_latency_ms": round(float(np.percentile(lat_a, 99)), 1) if len(lat_a) else None,
+        "p50_queue_ms": round(float(np.percentile(queue_ms, 50)), 1) if queue_ms else None,
+        "mrays_s": round(st["queries"] / wall / 1e6, 1),
+        "busy_fraction": round(st["busy_ms"] / (wall * 1e3), 3),
+        "evicted_scenes": st["evicted_scenes"], "resubmitted_after_eviction": evicted_retry[0],
+        "peak_note": "latency counts arrival to final image on the host, scene upload and BVH build included",
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -249,12 +477,19 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--depth", type=int, default=12)
-    ap.add_argument("--mode", default="frame", choices=["frame", "trace", "copy"],
+    ap.add_argument("--mode", default="frame", choices=["frame", "trace", "batch", "serve", "copy"],
                     help="frame: the single large scene (default); trace: the BVH build and "
                          "traversal microbenchmark (10M triangles, 2^26 rays); copy: a plain 1 GiB "
                          "read + 1 GiB write elementwise kernel, the reference for scaling rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--triangles", type=int, default=10_000_000, help="trace mode: triangle count")
+    ap.add_argument("--jobs", type=int, default=64, help="batch mode: number of jobs")
+    ap.add_argument("--tris-max", type=float, default=2e6, help="batch/serve modes: largest scene")
+    ap.add_argument("--spp-max", type=int, default=256, help="batch/serve modes: largest spp")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve mode: Poisson arrival rate (requests/s)")
+    ap.add_argument("--duration", type=float, default=120.0, help="serve mode: seconds of arrivals")
+    ap.add_argument("--drain", type=float, default=600.0, help="serve mode: longest wait for the backlog")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve mode: HBM budget")
     ap.add_argument("--log2-rays", type=int, default=26, help="trace mode: log2 of the ray count")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -277,6 +512,10 @@ def main():
         return
     if args.mode == "trace":
         return trace_bench(args)
+    if args.mode == "batch":
+        return batch_bench(args)
+    if args.mode == "serve":
+        return serve_bench(args)
     log("generating scene")
     t0 = time.perf_counter()
     sc = scenes.spheres_scene(seed=2)

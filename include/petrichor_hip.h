@@ -84,10 +84,14 @@ typedef struct pt_render_desc {
     int32_t max_bounces;   /* numMaxBounces */
     int32_t integrator;    /* PT_INTEGRATOR_* */
     uint32_t seed;         /* job seed of the per-pixel sampler streams */
-    int32_t flags;         /* PT_RENDER_TIME_KERNELS: time every launch with HIP events */
+    int32_t flags;         /* PT_RENDER_* */
 } pt_render_desc;
 
+/* Every launch on its own, timed with HIP events (same image). */
 #define PT_RENDER_TIME_KERNELS 1
+/* Plain launches: one host synchronisation per wavefront iteration, no graph (same
+ * image; the per-job baseline of the batched and served forms). */
+#define PT_RENDER_PLAIN_LAUNCHES 2
 
 /* Per-kernel figures: index 0 extension (closest hit), 1 environment shadow rays (any
  * hit), 2 area-light shadow rays (closest hit), 3 shading. Times are filled only with
@@ -186,8 +190,20 @@ typedef struct pt_batch_stats {
 int pt_engine_create(pt_context* ctx, int64_t max_paths, int32_t max_jobs, int32_t multi_job,
                      pt_engine** out);
 int pt_engine_destroy(pt_engine* engine);
+
+/* Launch shapes of an engine, chosen by petrichor_amd/autotune.py. No reference
+ * counterpart (the reference's ThreadPool takes a thread count, ThreadPool.cpp:19-26). */
+typedef struct pt_tuning {
+    int32_t iterations_per_sync; /* graph replays between host synchronisations, 1..256 */
+    int32_t trace_blocks_per_cu; /* persistent trace blocks of 256 threads per CU, 1..5 */
+    int32_t trace_chunk;         /* rays a wave takes from the shared cursor at a time, 64..4096 */
+    int32_t shade_blocks_per_cu; /* shading blocks of 256 threads per CU, 1..32 */
+} pt_tuning;
+int pt_engine_set_tuning(pt_engine* engine, const pt_tuning* tuning);
+int pt_engine_get_tuning(pt_engine* engine, pt_tuning* tuning);
 /* Renders n jobs to completion, admitting them in order as slots free up;
- * iterations_per_sync graph replays run between host synchronisations (0 = 8). */
+ * iterations_per_sync graph replays run between host synchronisations (0 = the
+ * engine's tuning). */
 int pt_render_batch(pt_engine* engine, int32_t n, const pt_job_desc* jobs, int32_t iterations_per_sync,
                     pt_batch_stats* stats);
 

@@ -66,6 +66,15 @@ class BatchStats(ctypes.Structure):
                 ("wall_ms", ctypes.c_double), ("syncs", ctypes.c_int64)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("iterations_per_sync", ctypes.c_int32), ("trace_blocks_per_cu", ctypes.c_int32),
+                ("trace_chunk", ctypes.c_int32), ("shade_blocks_per_cu", ctypes.c_int32)]
+
+
+RENDER_TIME_KERNELS = 1
+RENDER_PLAIN_LAUNCHES = 2
+
+
 class ServerDesc(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("max_jobs", ctypes.c_int32), ("hbm_budget_bytes", ctypes.c_int64),
                 ("max_paths", ctypes.c_int64), ("iterations_per_sync", ctypes.c_int32)]
@@ -124,6 +133,8 @@ def lib():
     L.pt_engine_destroy.argtypes = [vp]
     L.pt_render_batch.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(JobDesc), ctypes.c_int32,
                                   ctypes.POINTER(BatchStats)]
+    L.pt_engine_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.pt_engine_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     i64p = ctypes.POINTER(ctypes.c_int64)
     L.pt_server_create.argtypes = [ctypes.POINTER(ServerDesc), pvp]
     L.pt_server_destroy.argtypes = [vp]
@@ -146,7 +157,7 @@ def lib():
 EXPORTED = ["pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_scene_create", "pt_scene_destroy",
             "pt_bvh_build", "pt_bvh_destroy", "pt_bvh_export_order", "pt_bvh_export_nodes", "pt_bvh_build_host", "pt_bvh_info", "pt_trace_closest",
             "pt_trace_any", "pt_trace_stats", "pt_render", "pt_engine_create", "pt_engine_destroy",
-            "pt_render_batch", "pt_server_create", "pt_server_destroy", "pt_server_last_error",
+            "pt_render_batch", "pt_engine_set_tuning", "pt_engine_get_tuning", "pt_server_create", "pt_server_destroy", "pt_server_last_error",
             "pt_server_add_scene", "pt_server_remove_scene", "pt_server_scene_resident", "pt_server_submit",
             "pt_server_poll", "pt_server_wait", "pt_server_preview", "pt_server_fetch", "pt_server_get_stats"]
 
@@ -350,10 +361,12 @@ class Bvh:
         return t_out, prim_out, st
 
     def render(self, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1, out=None,
-               time_kernels=False):
+               time_kernels=False, plain=False):
         """Renders one image; returns (rgb [h, w, 3] float32 numpy array or the given device
-        tensor, stats dict)."""
-        d = RenderDesc(width, height, spp, max_bounces, integrator, seed, 1 if time_kernels else 0)
+        tensor, stats dict). time_kernels: every launch timed on its own; plain: plain
+        launches with one synchronisation per iteration (both give the same image)."""
+        flags = (RENDER_TIME_KERNELS if time_kernels else 0) | (RENDER_PLAIN_LAUNCHES if plain else 0)
+        d = RenderDesc(width, height, spp, max_bounces, integrator, seed, flags)
         st = RenderStats()
         if out is None:
             host = np.zeros((height, width, 3), np.float32)
@@ -402,7 +415,16 @@ class Engine:
                                             ctypes.byref(h)), "pt_engine_create")
         self.handle = h
 
-    def render_batch(self, jobs, iterations_per_sync=8):
+    def set_tuning(self, iterations_per_sync=8, trace_blocks_per_cu=5, trace_chunk=64, shade_blocks_per_cu=8):
+        t = Tuning(iterations_per_sync, trace_blocks_per_cu, trace_chunk, shade_blocks_per_cu)
+        self.ctx.check(self.ctx._lib.pt_engine_set_tuning(self.handle, ctypes.byref(t)), "pt_engine_set_tuning")
+
+    def get_tuning(self):
+        t = Tuning()
+        self.ctx.check(self.ctx._lib.pt_engine_get_tuning(self.handle, ctypes.byref(t)), "pt_engine_get_tuning")
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def render_batch(self, jobs, iterations_per_sync=0):
         arr = (JobDesc * len(jobs))()
         keep = []
         for i, j in enumerate(jobs):

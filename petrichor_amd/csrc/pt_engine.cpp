@@ -177,11 +177,35 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
         ctx->engine = nullptr;
         rc = pt_engine_create(ctx, npix, 1, 0, &ctx->engine);
         if (rc != PT_OK) return rc;
+        ctx->engine->iters_per_sync = 16;
     }
     pt_engine& E = *ctx->engine;
     E.iterations = 0;
     PT_HIP(ctx, hipMemsetAsync(E.queries_d, 0, sizeof(unsigned long long), E.st));
     std::vector<EngineJob> jobs(1, make_job(jd, 0));
+    if (d->flags & PT_RENDER_PLAIN_LAUNCHES) {
+        EngineJob& j = jobs[0];
+        rc = prepare_output(ctx, j);
+        if (rc == PT_OK) rc = E.start(j);
+        if (rc != PT_OK) return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        bool live = true;
+        while (live) {
+            rc = E.run_plain(&live);
+            if (rc != PT_OK) return rc;
+        }
+        const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        rc = finish_output(ctx, j, E.st);
+        E.retire(j);
+        if (rc != PT_OK) return rc;
+        if (stats) {
+            std::memset(stats, 0, sizeof *stats);
+            stats->queries = E.queries();
+            stats->iterations = E.iterations;
+            stats->elapsed_ms = wall;
+        }
+        return PT_OK;
+    }
     if (d->flags & PT_RENDER_TIME_KERNELS) {
         EngineJob& j = jobs[0];
         rc = prepare_output(ctx, j);
@@ -212,7 +236,7 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
         return PT_OK;
     }
     pt_batch_stats bs;
-    rc = run_jobs(E, jobs, 16, &bs);
+    rc = run_jobs(E, jobs, E.iters_per_sync, &bs);
     if (rc != PT_OK) return rc;
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
@@ -239,7 +263,32 @@ int pt_render_batch(pt_engine* E, int32_t n, const pt_job_desc* jobs, int32_t it
     }
     E->iterations = 0;
     PT_HIP(E->ctx, hipMemsetAsync(E->queries_d, 0, sizeof(unsigned long long), E->st));
-    return run_jobs(*E, js, iterations_per_sync > 0 ? iterations_per_sync : 8, stats);
+    return run_jobs(*E, js, iterations_per_sync > 0 ? iterations_per_sync : E->iters_per_sync, stats);
+}
+
+int pt_engine_set_tuning(pt_engine* E, const pt_tuning* t)
+{
+    if (!E || !t) return PT_ERR_ARG;
+    if (t->iterations_per_sync < 1 || t->iterations_per_sync > 256 || t->trace_blocks_per_cu < 1 ||
+        t->trace_blocks_per_cu > 5 || t->trace_chunk < 64 || t->trace_chunk > 4096 || t->shade_blocks_per_cu < 1 ||
+        t->shade_blocks_per_cu > 32)
+        return pt_fail(E->ctx, PT_ERR_ARG, "tuning out of range");
+    E->iters_per_sync = t->iterations_per_sync;
+    E->tune.trace_blocks_per_cu = t->trace_blocks_per_cu;
+    E->tune.trace_chunk = t->trace_chunk;
+    E->tune.shade_blocks_per_cu = t->shade_blocks_per_cu;
+    E->drop_graphs(); // the launch shapes are captured in the graphs
+    return PT_OK;
+}
+
+int pt_engine_get_tuning(pt_engine* E, pt_tuning* t)
+{
+    if (!E || !t) return PT_ERR_ARG;
+    t->iterations_per_sync = E->iters_per_sync;
+    t->trace_blocks_per_cu = E->tune.trace_blocks_per_cu;
+    t->trace_chunk = E->tune.trace_chunk;
+    t->shade_blocks_per_cu = E->tune.shade_blocks_per_cu;
+    return PT_OK;
 }
 
 } // extern "C"

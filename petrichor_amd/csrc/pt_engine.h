@@ -81,6 +81,8 @@ struct pt_engine {
     int integrator1 = 0;
     hipGraphExec_t graphs[2] = {nullptr, nullptr};
     int64_t iterations = 0;
+    pt::LaunchTuning tune;     // launch shapes (captured in the graphs)
+    int iters_per_sync = 8;    // default graph replays between host synchronisations
 
     pt::Queue queue(int s, int kind) const
     {
@@ -289,7 +291,7 @@ struct pt_engine {
         tj.job_of_path = multi ? ps.job : nullptr;
         tj.queries = queries_d;
         PT_HIP(ctx, pt::launch_zero(counters + 8, 1, st));
-        PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + 8, nullptr, st));
+        PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + 8, nullptr, st, tune));
         PT_HIP(ctx, pt::launch_zero(counters + 4 * w, 4, st));
         pt::ShadeArgs a;
         a.ps = ps;
@@ -304,7 +306,7 @@ struct pt_engine {
         a.hit_k = hit_k;
         a.sh_t = sh_t;
         a.sh_k = sh_k;
-        PT_HIP(ctx, pt::launch_shade(integrator1, sc1, job1, multi ? jobs_d : nullptr, a, st));
+        PT_HIP(ctx, pt::launch_shade(integrator1, sc1, job1, multi ? jobs_d : nullptr, a, st, tune));
         return PT_OK;
     }
 
@@ -377,7 +379,7 @@ struct pt_engine {
             tj.queries = queries_d;
             PT_HIP(ctx, pt::launch_zero(counters + 8, 1, st));
             PT_HIP(ctx, hipEventRecord(ev[2 * k], st));
-            PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + 8, nullptr, st));
+            PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + 8, nullptr, st, tune));
             PT_HIP(ctx, hipEventRecord(ev[2 * k + 1], st));
         }
         PT_HIP(ctx, pt::launch_zero(counters + 4 * w, 4, st));
@@ -395,7 +397,7 @@ struct pt_engine {
         a.sh_t = sh_t;
         a.sh_k = sh_k;
         PT_HIP(ctx, hipEventRecord(ev[6], st));
-        PT_HIP(ctx, pt::launch_shade(integrator1, sc1, job1, multi ? jobs_d : nullptr, a, st));
+        PT_HIP(ctx, pt::launch_shade(integrator1, sc1, job1, multi ? jobs_d : nullptr, a, st, tune));
         PT_HIP(ctx, hipEventRecord(ev[7], st));
         PT_HIP(ctx, hipMemcpyAsync(done_h.data(), done_d, sizeof(int) * (size_t)max_jobs, hipMemcpyDeviceToHost, st));
         PT_HIP(ctx, hipStreamSynchronize(st));
@@ -408,6 +410,61 @@ struct pt_engine {
             ms[k] += t;
         }
         for (auto& e : ev) (void)hipEventDestroy(e);
+        set = w;
+        ++iterations;
+        return PT_OK;
+    }
+
+    // One iteration as plain launches: the host reads the queue counts, launches a trace
+    // for each non-empty queue and the shading, and waits (no graph, one synchronisation
+    // per iteration; the baseline of the batched and served forms).
+    int run_plain(bool* any_live)
+    {
+        PT_HIP(ctx, hipSetDevice(ctx->device));
+        int h[4];
+        PT_HIP(ctx, hipMemcpyAsync(h, counters + 4 * set, sizeof h, hipMemcpyDeviceToHost, st));
+        PT_HIP(ctx, hipStreamSynchronize(st));
+        *any_live = h[3] > 0;
+        if (!*any_live) return PT_OK;
+        const int r = set, w = 1 - r;
+        float* outs_t[3] = {hit_t, nullptr, sh_t};
+        int* outs_k[3] = {hit_k, sh_k, sh_k};
+        for (int k = 0; k < 3; ++k) {
+            if (h[k] <= 0) continue;
+            pt::TraceJob tj;
+            tj.num_seg = 1;
+            pt::TraceSegment& g = tj.seg[0];
+            g.rays = q_rays[k];
+            g.path = q_path[k];
+            g.count = counters + 4 * r + k;
+            g.any = k == 1;
+            g.map_prim = 0;
+            g.hit_flag = 0;
+            g.out_t = outs_t[k];
+            g.out_k = outs_k[k];
+            tj.views = multi ? views_d : nullptr;
+            tj.job_of_path = multi ? ps.job : nullptr;
+            tj.queries = queries_d;
+            PT_HIP(ctx, pt::launch_zero(counters + 8, 1, st));
+            PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + 8, nullptr, st, tune));
+        }
+        PT_HIP(ctx, pt::launch_zero(counters + 4 * w, 4, st));
+        pt::ShadeArgs a;
+        a.ps = ps;
+        a.active_in = act[r];
+        a.active_count = counters + 4 * r + 3;
+        a.active_out = act[w];
+        a.active_out_count = counters + 4 * w + 3;
+        a.ext = queue(w, 0);
+        a.sh_any = queue(w, 1);
+        a.sh_closest = queue(w, 2);
+        a.hit_t = hit_t;
+        a.hit_k = hit_k;
+        a.sh_t = sh_t;
+        a.sh_k = sh_k;
+        PT_HIP(ctx, pt::launch_shade(integrator1, sc1, job1, multi ? jobs_d : nullptr, a, st, tune));
+        PT_HIP(ctx, hipMemcpyAsync(done_h.data(), done_d, sizeof(int) * (size_t)max_jobs, hipMemcpyDeviceToHost, st));
+        PT_HIP(ctx, hipStreamSynchronize(st));
         set = w;
         ++iterations;
         return PT_OK;

@@ -96,13 +96,21 @@ struct TraceJob {
     const BvhView* views;
     const int32_t* job_of_path;
     unsigned long long* queries; // optional: += rays traced by this launch
+    int chunk;                   // rays a wave takes from the cursor at a time (>= 64; set by the launcher)
+};
+
+// Launch shapes chosen by the autotuner (defaults fit every workload reasonably).
+struct LaunchTuning {
+    int trace_blocks_per_cu = 5; // 256-thread blocks with 32 KB of LDS stacks: at most 5 per CU
+    int trace_chunk = 64;        // >= 64, the wave width
+    int shade_blocks_per_cu = 8;
 };
 // Persistent trace launch over every segment of a job (rays handed out to waves in
 // chunks from *cursor, which must be zero when the launch starts). stats (optional,
 // 2 counters) accumulates node visits and triangle tests. The grid does not depend
 // on the ray counts, so the launch can be captured in a graph.
 hipError_t launch_trace_job(const BvhView& b, const TraceJob& job, int* cursor, unsigned long long* stats,
-                            hipStream_t st);
+                            hipStream_t st, const LaunchTuning& tune = LaunchTuning());
 hipError_t launch_zero(int* counters, int n, hipStream_t st);
 // Compute units of the current device (queried once, then cached; called before any
 // graph capture so that no device query happens while capturing).
@@ -131,6 +139,6 @@ struct ShadeArgs {
     const int* sh_k;
 };
 hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job, const JobDev* jobs,
-                        const ShadeArgs& a, hipStream_t st);
+                        const ShadeArgs& a, hipStream_t st, const LaunchTuning& tune = LaunchTuning());
 
 } // namespace pt

@@ -576,12 +576,11 @@ int cu_count()
     return cached;
 }
 
-static int shade_blocks() { return cu_count() * 8; }
-
 hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job, const JobDev* jobs,
-                        const ShadeArgs& a, hipStream_t st)
+                        const ShadeArgs& a, hipStream_t st, const LaunchTuning& tune)
 {
-    const dim3 grid(shade_blocks()), block(256);
+    const int bpc = tune.shade_blocks_per_cu < 1 ? 1 : (tune.shade_blocks_per_cu > 32 ? 32 : tune.shade_blocks_per_cu);
+    const dim3 grid((unsigned)(cu_count() * bpc)), block(256);
     if (jobs) hipLaunchKernelGGL((k_shade<0, true>), grid, block, 0, st, sc, job, jobs, a);
     else if (integrator == 0) hipLaunchKernelGGL((k_shade<0, false>), grid, block, 0, st, sc, job, jobs, a);
     else hipLaunchKernelGGL((k_shade<1, false>), grid, block, 0, st, sc, job, jobs, a);

@@ -23,7 +23,6 @@ namespace pt {
 
 #define PT_LDS_STACK 16
 #define PT_TRACE_BLOCK 256
-#define PT_TRACE_CHUNK 64
 
 struct StackEntry {
     uint32_t ref; // PT_REF_* word with the inside bit
@@ -220,7 +219,7 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1,
             int base2 = 0;
             if (avail < n_need) {
                 int got = 0;
-                if (lane == __ffsll((long long)need_mask) - 1) got = atomicAdd(cursor, PT_TRACE_CHUNK);
+                if (lane == __ffsll((long long)need_mask) - 1) got = atomicAdd(cursor, job.chunk);
                 base2 = __shfl(got, __ffsll((long long)need_mask) - 1, 64);
                 if (base2 >= total) exhausted = true;
             }
@@ -234,7 +233,7 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1,
                 pool_next += n_need;
             } else {
                 pool_next = base2 + (n_need - avail);
-                pool_end = base2 + PT_TRACE_CHUNK;
+                pool_end = base2 + job.chunk;
             }
             if (pool_end > total) pool_end = total;
             if (pool_next > pool_end) pool_next = pool_end;
@@ -277,12 +276,16 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1,
     }
 }
 
-hipError_t launch_trace_job(const BvhView& b, const TraceJob& job, int* cursor, unsigned long long* stats,
-                            hipStream_t st)
+hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* cursor, unsigned long long* stats,
+                            hipStream_t st, const LaunchTuning& tune)
 {
-    if (job.num_seg <= 0) return hipSuccess;
-    // persistent grid: five 32 KB LDS stacks (5 blocks of 4 waves) per CU
-    const dim3 grid((unsigned)(cu_count() * 5)), block(PT_TRACE_BLOCK);
+    if (job_in.num_seg <= 0) return hipSuccess;
+    TraceJob job = job_in;
+    // a wave refills at most 64 lanes from one chunk, so chunks hold at least 64 rays
+    job.chunk = tune.trace_chunk < 64 ? 64 : tune.trace_chunk;
+    // persistent grid: blocks of 4 waves, each with a 32 KB LDS stack (at most 5 per CU)
+    const int bpc = tune.trace_blocks_per_cu < 1 ? 1 : (tune.trace_blocks_per_cu > 5 ? 5 : tune.trace_blocks_per_cu);
+    const dim3 grid((unsigned)(cu_count() * bpc)), block(PT_TRACE_BLOCK);
     const bool multi = job.views != nullptr;
     if (stats) {
         if (multi) hipLaunchKernelGGL((k_trace_persistent<true, true>), grid, block, 0, st, b, job, cursor, stats);

@@ -214,7 +214,7 @@ def sky_env(width=2048, height=1024, sun_elev_deg=35.0, sun_azim_deg=60.0, sun_s
     return (sky + sun[..., None] * np.array([1.0, 0.95, 0.85])).astype(np.float32)
 
 
-def spheres_scene(seed=2, n_spheres=256, seg=64, bands=33, terrain_n=1024, env_w=2048, env_h=1024):
+def spheres_scene(seed=2, n_spheres=256, seg=64, bands=33, terrain_n=1024, env_w=2048, env_h=1024, env=None):
     """Config 2 at full size: 256 spheres x 4096 triangles = 1,048,576 over a 1024 x 1024
     quad terrain (2,097,152 triangles), 3,145,728 triangles in all; radii
     lognormal(-2, 0.5), centres uniform in [-10,10]^2 x [0,4]; 40% Lambert, 40% GGX
@@ -239,8 +239,22 @@ def spheres_scene(seed=2, n_spheres=256, seg=64, bands=33, terrain_n=1024, env_w
         tri_mat.append(np.full(len(t), len(mats) - 1, np.int32))
     cam = camera(pos=(0.0, -14.0, 3.5), look_at=(0.0, 0.0, 0.8), focal_length=28e-3,
                  focus_target=(0.0, 0.0, 0.8), f_number=8.0)
-    env = sky_env(env_w, env_h)
+    if env is None:
+        env = sky_env(env_w, env_h)
     return _finish(np.concatenate(parts), mats, np.concatenate(tri_mat), [], cam, env, (1, 1, 1, 0, 1))
+
+
+def scaled_scene(n_tris, seed, env=None):
+    """The single-large-scene generator scaled to about n_tris triangles: two thirds in
+    the terrain grid, one third in 1 to 256 UV spheres (the batched and served workloads).
+    env: a shared equirect map (default: the 2048 x 1024 sky)."""
+    terrain_n = max(2, int(round(np.sqrt(n_tris / 3.0))))  # 2 n^2 triangles
+    sphere_tris = max(n_tris - 2 * terrain_n ** 2, 64)
+    n_spheres = int(np.clip(sphere_tris // 1024, 1, 256))
+    per = sphere_tris / n_spheres  # a UV sphere has 2 * seg * (bands - 1) triangles
+    seg = max(8, int(round(np.sqrt(per))))
+    bands = max(3, int(round(per / (2 * seg))) + 1)
+    return spheres_scene(seed=seed, n_spheres=n_spheres, seg=seg, bands=bands, terrain_n=terrain_n, env=env)
 
 
 # ---- Config 3: random triangle soup + rays ---------------------------------------------------
