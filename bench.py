@@ -2,11 +2,16 @@
 
 Default workload: the single large scene — 3,145,728 triangles (256 UV spheres of 4096
 triangles over a 1024 x 1024-quad fBm terrain), 2048 x 1024 equirect sky with a sun lobe,
-1920 x 1080, 64 spp, max depth 12, MIS integrator, per-pixel seeded samplers. One step is
-one full frame. `value` is BVH queries (camera, bounce and shadow rays) per second over
-the timed steps, in Mrays/s.
+1920 x 1080, 64 spp, max depth 12, MIS integrator, per-pixel seeded samplers, BVH built
+on the device and collapsed to 8-wide nodes. One step is one full frame. `value` is BVH
+queries (camera, bounce and shadow rays) per second over the timed steps, in Mrays/s.
+The oracle renders a random sample of the frame's pixels on the host cores: that is the
+CPU baseline, and the same pixels check the GPU frame (per-pixel relative error).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S] [--bvh-width 2|8]
+  python bench.py --mode trace    # BVH build and traversal microbenchmark (10M triangles)
+  python bench.py --mode batch    # 64 uneven jobs: batched vs one by one
+  python bench.py --mode serve    # Poisson request stream against the render server
 
 Prints one JSON line. Only one GPU is supported; --gpus N > 1 exits with a message.
 """
@@ -29,6 +34,8 @@ HBM_PEAK_GBS = 8000.0
 # holds the boxes of both children, and each child box is one visit); 48 B per triangle
 # test (p0, e1, e2, n).
 RAY_IN_B, PATH_B, HIT_OUT_B, NODE_B, TRI_B = 32, 4, 8, 32, 48
+# 8-wide layout: one 128 B record per node visit (8 quantised child boxes and references)
+NODE8_B = 128
 
 
 def log(*a):
@@ -46,12 +53,15 @@ def cpu_baseline(sc, tree, width, height, spp, depth, seconds):
     n = 2048
     total_q, total_t, total_px = 0, 0.0, 0
     work = np.zeros(9, np.int64)
+    pixels, rgbs = [], []
     while True:
         pix = np.sort(rng.choice(width * height, n, replace=False)).astype(np.int32)
         w = np.zeros(9, np.int64)
         t0 = time.perf_counter()
-        _, q = o.render(width, height, spp, depth, 0, 1, threads, pixels=pix, work=w)
+        rgb, q = o.render(width, height, spp, depth, 0, 1, threads, pixels=pix, work=w)
         dt = time.perf_counter() - t0
+        pixels.append(pix)
+        rgbs.append(rgb)
         total_q += q
         total_t += dt
         total_px += n
@@ -64,7 +74,7 @@ def cpu_baseline(sc, tree, width, height, spp, depth, seconds):
         "value": total_q / total_t / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
         "sample": "%d random pixels of the 1920x1080 frame at full 64 spp / depth 12 (%.1f s, %d queries)"
                   % (total_px, total_t, total_q),
-    }, work
+    }, work, np.concatenate(pixels), np.concatenate(rgbs)
 
 
 def torch_primary_rays(cam, width, height, gen):
@@ -120,11 +130,14 @@ def cpu_trace_baseline(sc, tree, rays_host, seconds):
     n = 1 << 14
     done, total_t = 0, 0.0
     pool = ThreadPoolExecutor(threads)
+    ts, ps = [], []
     while True:
         batch = rays_host[done:done + n]
         chunks = np.array_split(batch, threads)
         t0 = time.perf_counter()
-        list(pool.map(o.trace_closest, chunks))
+        for t_, p_, _ in pool.map(o.trace_closest, chunks):
+            ts.append(t_)
+            ps.append(p_)
         dt = time.perf_counter() - t0
         done += len(batch)
         total_t += dt
@@ -134,7 +147,7 @@ def cpu_trace_baseline(sc, tree, rays_host, seconds):
     pool.shutdown()
     return {"value": done / total_t / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": "%d rays (a random half-primary, half-surface sample of the 2^26), closest hit, %.1f s"
-                      % (done, total_t)}
+                      % (done, total_t)}, np.concatenate(ts), np.concatenate(ps)
 
 
 def trace_bench(args):
@@ -146,6 +159,7 @@ def trace_bench(args):
     import petrichor_amd as P
     from petrichor_amd import scenes
     N = args.triangles
+    width = args.bvh_width or 8
     log("generating %d triangles" % N)
     sc = scenes.random_triangles(N, seed=3, mu=-6.0, sigma=1.0)
     ctx = P.Context(0)
@@ -155,7 +169,7 @@ def trace_bench(args):
     for i in range(args.warmup + args.steps):
         if bvh is not None:
             bvh.close()
-        bvh = s.build_bvh(32, 4, builder=P.BUILD_DEVICE)
+        bvh = s.build_bvh(32, 4, width=width, builder=P.BUILD_DEVICE)
         info = bvh.info()
         if i >= args.warmup:
             builds.append(info["build_ms"])
@@ -200,22 +214,32 @@ def trace_bench(args):
     log("closest: %.2f ms/launch, any: %.2f ms/launch, %d hits" % (np.mean(ms_c), np.mean(ms_a), hits))
     _, _, st = bvh.trace_stats(rays)
     n_node, n_tri = st[0] / nr, st[1] / nr
-    b_ray = RAY_IN_B + HIT_OUT_B + n_node * NODE_B + n_tri * TRI_B
+    node_b = NODE8_B if width == 8 else NODE_B
+    b_ray = RAY_IN_B + HIT_OUT_B + n_node * node_b + n_tri * TRI_B
     avg_ms = float(np.mean(ms_c))
     achieved = nr * b_ray / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_trace_persistent (closest hit, one launch over all rays)",
                 "bytes_per_query": round(b_ray, 1), "n_node": round(n_node, 2), "n_tri": round(n_tri, 3),
-                "s_node": NODE_B, "s_tri": TRI_B, "avg_launch_ms": round(avg_ms, 3), "queries_per_launch": nr}
+                "s_node": node_b, "s_tri": TRI_B, "avg_launch_ms": round(avg_ms, 3), "queries_per_launch": nr}
     cpu = None
+    parity = None
     if not args.no_cpu:
         log("cpu baseline")
         tree = bvh.export_nodes() | {"prim_ids": bvh.export_order()}
-        sel = torch.randperm(nr, generator=torch.Generator().manual_seed(11))[: 1 << 21]
-        sample = rays[sel.cuda()].cpu().numpy()
-        cpu = cpu_trace_baseline(sc, tree, sample, args.cpu_seconds)
+        sel = torch.randperm(nr, generator=torch.Generator().manual_seed(11))[: 1 << 21].cuda()
+        sample = rays[sel].cpu().numpy()
+        cpu, t_ref, p_ref = cpu_trace_baseline(sc, tree, sample, args.cpu_seconds)
         log("cpu:", cpu)
+        # the same rays against the oracle (reference walk order on the same binary tree)
+        k = len(t_ref)
+        t_gpu = t_out[sel[:k]].cpu().numpy()
+        p_gpu = p_out[sel[:k]].cpu().numpy()
+        parity = {"rays_checked": int(k),
+                  "t_bit_identical": int((t_gpu.view(np.uint32) == t_ref.view(np.uint32)).sum()),
+                  "prim_identical": int((p_gpu == p_ref).sum())}
+        log("parity vs oracle:", parity)
     value = nr * args.steps / wall_c / 1e6
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -224,8 +248,9 @@ def trace_bench(args):
         "config": {"workload": "BVH microbenchmark: %d random triangles (centroids U[0,1]^3, edges lognormal(-6,1)), "
                                "2^%d rays (half pinhole primary, half surface rays with uniform directions), "
                                "closest hit" % (N, args.log2_rays),
-                   "triangles": N, "rays": nr, "bvh": "binned SAH, 32 bins, leaf<=4, binary (reference topology), "
-                                                   "device builder"},
+                   "triangles": N, "rays": nr, "bvh": "binned SAH, 32 bins, leaf<=4, device builder, %s"
+                                                   % ("collapsed to 8-wide nodes" if width == 8
+                                                      else "binary (reference topology and walk order)")},
         "closest_mrays_s": round(value, 2),
         "any_mrays_s": round(nr * args.steps / wall_a / 1e6, 2),
         "closest_ms_per_launch": round(avg_ms, 3), "any_ms_per_launch": round(float(np.mean(ms_a)), 3),
@@ -234,6 +259,7 @@ def trace_bench(args):
         "bvh_nodes": info["num_nodes"], "bvh_depth": info["max_depth"],
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity_vs_oracle": parity,
     }
     print(json.dumps(res), flush=True)
 
@@ -483,6 +509,8 @@ def main():
                          "read + 1 GiB write elementwise kernel, the reference for scaling rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--triangles", type=int, default=10_000_000, help="trace mode: triangle count")
+    ap.add_argument("--bvh-width", type=int, default=None, choices=[2, 8],
+                    help="BVH width (default 8; 2 walks the binary tree in the reference's order)")
     ap.add_argument("--jobs", type=int, default=64, help="batch mode: number of jobs")
     ap.add_argument("--tris-max", type=float, default=2e6, help="batch/serve modes: largest scene")
     ap.add_argument("--spp-max", type=int, default=256, help="batch/serve modes: largest spp")
@@ -522,7 +550,8 @@ def main():
     log("scene: %d triangles in %.1f s" % (len(sc["tri_mat"]), time.perf_counter() - t0))
     ctx = P.Context(0)
     s = ctx.scene(sc)
-    bvh = s.build_bvh(16)
+    width = args.bvh_width or 8
+    bvh = s.build_bvh(16, width=width)
     info = bvh.info()
     log("bvh:", info)
     W, H, SPP, D = args.width, args.height, args.spp, args.depth
@@ -556,25 +585,43 @@ def main():
     ext_ms = kst["kernels"]["extension"]["ms"]
     kernels = {k: {"items": v["items"], "launches": v["launches"], "ms": round(v["ms"], 3)}
                for k, v in kst["kernels"].items()}
+    # and one with the traversal counters on (node records and triangle tests per kind)
+    counted = torch.empty_like(out)
+    _, cst = bvh.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=counted, trace_stats=True)
+    torch.cuda.synchronize()
+    assert torch.equal(counted, out), "the counting frame must match the production frames"
+    ext_c = cst["kernels"]["extension"]
+    n_node = ext_c["nodes"] / max(ext_c["items"], 1)
+    n_tri = ext_c["tris"] / max(ext_c["items"], 1)
+    for k, v in cst["kernels"].items():
+        if k != "shade":
+            kernels[k]["node_visits"] = v["nodes"]
+            kernels[k]["triangle_tests"] = v["tris"]
 
     cpu = None
-    n_node = n_tri = None
+    parity = None
     if not args.no_cpu:
         log("cpu baseline")
         tree = bvh.export_nodes() | {"prim_ids": bvh.export_order()}
-        cpu, work = cpu_baseline(sc, tree, W, H, SPP, D, args.cpu_seconds)
-        n_node = work[1] / max(work[0], 1)
-        n_tri = work[2] / max(work[0], 1)
+        cpu, work, pix, rgb_ref = cpu_baseline(sc, tree, W, H, SPP, D, args.cpu_seconds)
         log("cpu:", cpu)
+        # the GPU frame against the oracle on the sampled pixels (per-pixel relative error)
+        got = img.reshape(-1, 3)[pix].astype(np.float64)
+        ref = rgb_ref.astype(np.float64)
+        rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)
+        parity = {"pixels_checked": int(len(pix)), "max_rel_err": float(rel.max()),
+                  "pixels_bit_identical": int(np.all(got == ref, axis=1).sum())}
+        log("parity vs oracle:", parity)
     roofline = None
-    if n_node is not None and ext_ms > 0:
-        b_ray = RAY_IN_B + PATH_B + HIT_OUT_B + n_node * NODE_B + n_tri * TRI_B
+    node_b = NODE8_B if width == 8 else NODE_B
+    if ext_ms > 0:
+        b_ray = RAY_IN_B + PATH_B + HIT_OUT_B + n_node * node_b + n_tri * TRI_B
         achieved = ext_items * b_ray / (ext_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                     "kernel": "k_trace_persistent (extension rays, instrumented frame)",
                     "bytes_per_query": round(b_ray, 1), "n_node": round(n_node, 2), "n_tri": round(n_tri, 3),
-                    "s_node": NODE_B, "s_tri": TRI_B, "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4),
+                    "s_node": node_b, "s_tri": TRI_B, "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4),
                     "queries_per_launch": round(ext_items / max(ext_launches, 1), 1)}
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -583,13 +630,17 @@ def main():
         "config": {"workload": "single large scene: %d triangles (256 spheres x 4096 + 1024^2-quad terrain), "
                                "2048x1024 env, %dx%d, %d spp, depth %d, MIS" % (len(sc["tri_mat"]), W, H, SPP, D),
                    "triangles": int(len(sc["tri_mat"])), "width": W, "height": H, "spp": SPP, "max_depth": D,
-                   "integrator": "MIS", "bvh": "binned SAH, 16 bins, leaf<=4, binary (reference topology)"},
+                   "integrator": "MIS",
+                   "bvh": "binned SAH, 16 bins, leaf<=4, device builder, " +
+                          ("collapsed to 8-wide nodes" if width == 8
+                           else "binary (reference topology and walk order)")},
         "queries_per_step": int(queries / args.steps),
         "iterations_per_step": int(stats[-1]["iterations"]),
         "bvh_build_ms": round(info["build_ms"], 1),
         "kernels_instrumented_frame": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity_vs_oracle": parity,
     }
     print(json.dumps(res), flush=True)
 

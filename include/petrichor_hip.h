@@ -92,6 +92,9 @@ typedef struct pt_render_desc {
 /* Plain launches: one host synchronisation per wavefront iteration, no graph (same
  * image; the per-job baseline of the batched and served forms). */
 #define PT_RENDER_PLAIN_LAUNCHES 2
+/* With PT_RENDER_TIME_KERNELS: count node visits and triangle tests per trace kind
+ * (binary tree: child boxes tested, 32 B each; 8-wide tree: 128 B node records). */
+#define PT_RENDER_TRACE_STATS 4
 
 /* Per-kernel figures: index 0 extension (closest hit), 1 environment shadow rays (any
  * hit), 2 area-light shadow rays (closest hit), 3 shading. Times are filled only with
@@ -103,6 +106,8 @@ typedef struct pt_render_stats {
     int64_t kernel_items[4]; /* rays traced / paths shaded per kernel kind */
     int64_t kernel_launches[4];
     double kernel_ms[4];     /* summed launch durations */
+    int64_t kernel_nodes[4]; /* node records visited by the trace kinds (PT_RENDER_TRACE_STATS) */
+    int64_t kernel_tris[4];  /* triangle tests of the trace kinds (PT_RENDER_TRACE_STATS) */
 } pt_render_stats;
 
 /* Context = one device + one HIP stream. */

@@ -73,6 +73,7 @@ class Tuning(ctypes.Structure):
 
 RENDER_TIME_KERNELS = 1
 RENDER_PLAIN_LAUNCHES = 2
+RENDER_TRACE_STATS = 4
 
 
 class ServerDesc(ctypes.Structure):
@@ -97,7 +98,8 @@ REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 class RenderStats(ctypes.Structure):
     _fields_ = [("queries", ctypes.c_int64), ("iterations", ctypes.c_int64), ("elapsed_ms", ctypes.c_double),
                 ("kernel_items", ctypes.c_int64 * 4), ("kernel_launches", ctypes.c_int64 * 4),
-                ("kernel_ms", ctypes.c_double * 4)]
+                ("kernel_ms", ctypes.c_double * 4), ("kernel_nodes", ctypes.c_int64 * 4),
+                ("kernel_tris", ctypes.c_int64 * 4)]
 
 
 _lib = None
@@ -361,11 +363,12 @@ class Bvh:
         return t_out, prim_out, st
 
     def render(self, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1, out=None,
-               time_kernels=False, plain=False):
+               time_kernels=False, plain=False, trace_stats=False):
         """Renders one image; returns (rgb [h, w, 3] float32 numpy array or the given device
         tensor, stats dict). time_kernels: every launch timed on its own; plain: plain
         launches with one synchronisation per iteration (both give the same image)."""
-        flags = (RENDER_TIME_KERNELS if time_kernels else 0) | (RENDER_PLAIN_LAUNCHES if plain else 0)
+        flags = (RENDER_TIME_KERNELS if time_kernels or trace_stats else 0) | \
+            (RENDER_PLAIN_LAUNCHES if plain else 0) | (RENDER_TRACE_STATS if trace_stats else 0)
         d = RenderDesc(width, height, spp, max_bounces, integrator, seed, flags)
         st = RenderStats()
         if out is None:
@@ -380,7 +383,8 @@ class Bvh:
         kinds = ("extension", "shadow_any", "shadow_closest", "shade")
         return res, {"queries": st.queries, "iterations": st.iterations, "elapsed_ms": st.elapsed_ms,
                      "kernels": {k: {"items": st.kernel_items[i], "launches": st.kernel_launches[i],
-                                     "ms": st.kernel_ms[i]} for i, k in enumerate(kinds)}}
+                                     "ms": st.kernel_ms[i], "nodes": st.kernel_nodes[i], "tris": st.kernel_tris[i]}
+                            for i, k in enumerate(kinds)}}
 
     def close(self):
         if getattr(self, "handle", None):

@@ -11,6 +11,7 @@
 #include "../../include/petrichor_hip.h"
 #include "pt_build.h"
 #include "pt_internal.h"
+#include "pt_wide.h"
 
 using namespace pt;
 
@@ -115,6 +116,39 @@ int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
     const int64_t nt = s->num_tris;
     b->device_bytes = (int64_t)pk.num_inner * 64 + (int64_t)pk.num_big * 8 + nt * 60 +
                       (pk.smooth_d ? nt * 4 : 0) + (pk.nrm_d ? nt * 36 : 0);
+    *out = b;
+    return PT_OK;
+}
+
+// Adds the 8-wide nodes to a built binary tree (bvh width 8). On failure the tree is
+// destroyed and *out cleared.
+int make_wide(pt_bvh* b, pt_bvh** out)
+{
+    pt_context* ctx = b->scene->ctx;
+    const auto t0 = std::chrono::steady_clock::now();
+    Wide8 w;
+    hipError_t e = gpu_collapse_bvh8(b->nodes_d, b->num_inner, b->view.root_ref, ctx->stream, &w);
+    // a wide level leaves at most 7 children pending on the stack
+    if (e == hipSuccess && 7 * w.depth + 1 > 192) {
+        if (w.nodes_d) (void)hipFree(w.nodes_d);
+        pt_bvh_destroy(b);
+        *out = nullptr;
+        return fail(ctx, PT_ERR_UNSUPPORTED, "8-wide tree too deep for the traversal stack");
+    }
+    if (e != hipSuccess) {
+        pt_bvh_destroy(b);
+        *out = nullptr;
+        return fail(ctx, e == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
+                    std::string("8-wide collapse: ") + hipGetErrorString(e));
+    }
+    b->nodes8_d = w.nodes_d;
+    b->num_nodes8 = w.num_nodes;
+    b->depth8 = w.depth;
+    b->width = 8;
+    b->view.nodes8 = w.nodes_d;
+    b->view.root8 = w.root;
+    b->device_bytes += (int64_t)w.num_nodes * 128;
+    b->build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = b;
     return PT_OK;
 }
@@ -361,12 +395,16 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     const int leaf_max = desc ? desc->leaf_max : 4;
     const int width = desc ? desc->width : 2;
     const int builder = desc ? desc->builder : PT_BUILD_DEVICE;
-    if (width != 2) return fail(ctx, PT_ERR_UNSUPPORTED, "only width 2 is implemented");
+    if (width != 2 && width != 8) return fail(ctx, PT_ERR_UNSUPPORTED, "BVH width must be 2 or 8");
     if (bins < 2 || bins > 256 || leaf_max < 1) return fail(ctx, PT_ERR_ARG, "bad BVH parameters");
     if (builder != PT_BUILD_DEVICE && builder != PT_BUILD_HOST) return fail(ctx, PT_ERR_ARG, "unknown builder");
     if (builder == PT_BUILD_DEVICE && bins > 64)
         return fail(ctx, PT_ERR_UNSUPPORTED, "the device builder takes at most 64 bins");
-    if (builder == PT_BUILD_DEVICE && s->num_tris > 0) return build_device(s, bins, leaf_max, out);
+    if (builder == PT_BUILD_DEVICE && s->num_tris > 0) {
+        const int rc = build_device(s, bins, leaf_max, out);
+        if (rc != PT_OK || width == 2) return rc;
+        return make_wide(*out, out);
+    }
     pt_bvh* b = new pt_bvh();
     b->scene = s;
     const auto t0 = std::chrono::steady_clock::now();
@@ -434,14 +472,16 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     b->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     b->device_bytes = (int64_t)(nodes.size() * sizeof(float4) + big.size() * sizeof(int2) + tris.size() * sizeof(float4)) +
                       (int64_t)nt * 12 + (int64_t)(smooth.size() * 4 + nrm.size() * 4);
+    b->num_inner = (int)(nodes.size() / 4);
     *out = b;
+    if (width == 8) return make_wide(b, out);
     return PT_OK;
 }
 
 int pt_bvh_destroy(pt_bvh* b)
 {
     if (!b) return PT_ERR_ARG;
-    free_all({b->nodes_d, b->big_leaves_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
+    free_all({b->nodes8_d, b->nodes_d, b->big_leaves_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
               b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d});
     delete b;
     return PT_OK;

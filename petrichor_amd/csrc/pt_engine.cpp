@@ -91,7 +91,8 @@ int run_jobs(pt_engine& E, std::vector<EngineJob>& jobs, int iters_per_sync, pt_
     const auto t0 = std::chrono::steady_clock::now();
     int64_t replays = 0;
     while (next < jobs.size() || !running.empty()) {
-        while (next < jobs.size() && (int64_t)running.size() < E.max_jobs && E.largest_free() >= jobs[next].npix) {
+        while (next < jobs.size() && (int64_t)running.size() < E.max_jobs && E.largest_free() >= jobs[next].npix &&
+               E.width_ok(pt_engine::job_width(jobs[next]))) {
             EngineJob& j = jobs[next];
             int rc = prepare_output(ctx, j);
             if (rc == PT_OK) rc = E.start(j);
@@ -213,11 +214,25 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
         if (rc != PT_OK) return rc;
         int64_t items[4] = {0, 0, 0, 0}, launches[4] = {0, 0, 0, 0};
         double ms[4] = {0, 0, 0, 0};
+        unsigned long long* stats_d = nullptr;
+        unsigned long long stats_h[6] = {0, 0, 0, 0, 0, 0};
+        if (d->flags & PT_RENDER_TRACE_STATS) {
+            PT_HIP(ctx, hipMalloc((void**)&stats_d, sizeof stats_h));
+            PT_HIP(ctx, hipMemcpyAsync(stats_d, stats_h, sizeof stats_h, hipMemcpyHostToDevice, E.st));
+        }
         const auto t0 = std::chrono::steady_clock::now();
         bool live = true;
         while (live) {
-            rc = E.run_timed(items, launches, ms, &live);
-            if (rc != PT_OK) return rc;
+            rc = E.run_timed(items, launches, ms, &live, stats_d);
+            if (rc != PT_OK) {
+                if (stats_d) (void)hipFree(stats_d);
+                return rc;
+            }
+        }
+        if (stats_d) {
+            const hipError_t e = hipMemcpy(stats_h, stats_d, sizeof stats_h, hipMemcpyDeviceToHost);
+            (void)hipFree(stats_d);
+            PT_HIP(ctx, e);
         }
         const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         rc = finish_output(ctx, j, E.st);
@@ -231,6 +246,8 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
                 stats->kernel_items[k] = items[k];
                 stats->kernel_launches[k] = launches[k];
                 stats->kernel_ms[k] = ms[k];
+                stats->kernel_nodes[k] = k < 3 ? (int64_t)stats_h[2 * k] : 0;
+                stats->kernel_tris[k] = k < 3 ? (int64_t)stats_h[2 * k + 1] : 0;
             }
         }
         return PT_OK;

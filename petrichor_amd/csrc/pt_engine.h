@@ -81,6 +81,7 @@ struct pt_engine {
     int integrator1 = 0;
     hipGraphExec_t graphs[2] = {nullptr, nullptr};
     int64_t iterations = 0;
+    int width = 0;             // BVH width of the jobs a multi-job engine runs (0: none yet)
     pt::LaunchTuning tune;     // launch shapes (captured in the graphs)
     int iters_per_sync = 8;    // default graph replays between host synchronisations
 
@@ -221,8 +222,25 @@ struct pt_engine {
 
     // Starts a job: path slots, job table entry, set-up kernel appending its paths and
     // camera rays to the queues the next iteration reads.
+    static int job_width(const pt::EngineJob& j) { return j.bvh->view.nodes8 ? 8 : 2; }
+
+    // A multi-job engine runs BVHs of one width at a time (one trace kernel per launch).
+    bool width_ok(int w) const
+    {
+        if (!multi) return true;
+        for (int i = 0; i < max_jobs; ++i)
+            if (slot_used[i]) return w == width;
+        return true;
+    }
+
     int start(pt::EngineJob& j)
     {
+        const int jw = job_width(j);
+        if (!width_ok(jw)) return pt_fail(ctx, PT_ERR_UNSUPPORTED, "jobs of another BVH width are running");
+        if (multi && jw != width) {
+            width = jw;
+            drop_graphs(); // the trace kernel is captured in the graphs
+        }
         int slot = -1;
         for (int i = 0; i < max_jobs; ++i)
             if (!slot_used[i]) {
@@ -288,6 +306,7 @@ struct pt_engine {
             g.out_k = outs_k[k];
         }
         tj.views = multi ? views_d : nullptr;
+        tj.wide = multi && width == 8;
         tj.job_of_path = multi ? ps.job : nullptr;
         tj.queries = queries_d;
         PT_HIP(ctx, pt::launch_zero(counters + 8, 1, st));
@@ -349,7 +368,9 @@ struct pt_engine {
 
     // One iteration with every launch timed by HIP events (kernel kinds: extension
     // trace, environment-shadow trace, area-shadow trace, shading).
-    int run_timed(int64_t items[4], int64_t launches[4], double ms[4], bool* any_live)
+    // stats_d (optional): 2 device counters per trace kind (node visits, triangle tests)
+    int run_timed(int64_t items[4], int64_t launches[4], double ms[4], bool* any_live,
+                  unsigned long long* stats_d = nullptr)
     {
         PT_HIP(ctx, hipSetDevice(ctx->device));
         int h[4];
@@ -375,11 +396,12 @@ struct pt_engine {
             g.out_t = outs_t[k];
             g.out_k = outs_k[k];
             tj.views = multi ? views_d : nullptr;
+            tj.wide = multi && width == 8;
             tj.job_of_path = multi ? ps.job : nullptr;
             tj.queries = queries_d;
             PT_HIP(ctx, pt::launch_zero(counters + 8, 1, st));
             PT_HIP(ctx, hipEventRecord(ev[2 * k], st));
-            PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + 8, nullptr, st, tune));
+            PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + 8, stats_d ? stats_d + 2 * k : nullptr, st, tune));
             PT_HIP(ctx, hipEventRecord(ev[2 * k + 1], st));
         }
         PT_HIP(ctx, pt::launch_zero(counters + 4 * w, 4, st));
@@ -443,6 +465,7 @@ struct pt_engine {
             g.out_t = outs_t[k];
             g.out_k = outs_k[k];
             tj.views = multi ? views_d : nullptr;
+            tj.wide = multi && width == 8;
             tj.job_of_path = multi ? ps.job : nullptr;
             tj.queries = queries_d;
             PT_HIP(ctx, pt::launch_zero(counters + 8, 1, st));

@@ -52,6 +52,10 @@ struct pt_bvh {
     int32_t* tri_mat_shading_d = nullptr;
     int32_t* tri_smooth_d = nullptr;
     float* tri_nrm_d = nullptr;
+    uint4* nodes8_d = nullptr; // 8-wide nodes (width 8 only)
+    int width = 2;
+    int num_nodes8 = 0;
+    int depth8 = 0;
     pt::BvhView view;
     int num_nodes = 0;
     int max_depth = 0;

@@ -21,6 +21,8 @@ struct BvhView {
     const int2* big_leaves;   // (first slot, count) of leaves over PT_REF_MAX_SMALL_LEAF triangles
     V3 root_lo, root_hi;
     uint32_t root_ref;        // PT_REF_* encoding, PT_REF_EMPTY for an empty scene
+    const uint4* nodes8 = nullptr;  // 8-wide nodes (pt_wide.hip), null for the binary traversal
+    uint32_t root8 = PT_REF_EMPTY;  // root of the 8-wide tree (wide node 0, or a leaf)
 };
 
 struct SceneView {
@@ -96,7 +98,8 @@ struct TraceJob {
     const BvhView* views;
     const int32_t* job_of_path;
     unsigned long long* queries; // optional: += rays traced by this launch
-    int chunk;                   // rays a wave takes from the cursor at a time (>= 64; set by the launcher)
+    int chunk = 64;              // rays a wave takes from the cursor at a time (>= 64; set by the launcher)
+    int wide = 0;                // 8-wide traversal (every BVH of a multi-job launch has the same width)
 };
 
 // Launch shapes chosen by the autotuner (defaults fit every workload reasonably).

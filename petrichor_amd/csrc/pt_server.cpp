@@ -162,7 +162,7 @@ struct pt_server {
             SRequest* r = *it;
             const int64_t bytes = 12 * r->npix;
             const bool fits = (int64_t)running.size() < E->max_jobs && E->largest_free() >= r->npix &&
-                              make_room(bytes);
+                              E->width_ok(scenes.at(r->scene_id).bvh->view.nodes8 ? 8 : 2) && make_room(bytes);
             if (!fits) {
                 if (running.empty() && it == queue.begin() && !make_room(bytes)) {
                     // nothing will free memory for it: no request runs and no scene is idle

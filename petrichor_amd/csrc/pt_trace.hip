@@ -10,6 +10,14 @@
 // hit replaces the best one only when strictly nearer. The reference pushes children
 // whose box the ray misses and rejects them when popped; they are not pushed here.
 //
+// The 8-wide traversal (BVHs built with width 8, nodes in pt_wide.hip) tests the up to 8
+// child boxes of a node against [max(dist_min, 0), min(nearest hit, dist_max)], visits
+// the nearest hit child next and pushes the others farthest first with their entry
+// distance; a popped entry is skipped when the nearest hit is closer than its entry
+// distance. Leaves, triangle tests and the hit rule are the reference's, so it finds the
+// same nearest hit; only which of several hits at exactly the same distance wins, and
+// rays that graze a box within rounding of the hit, can differ.
+//
 // Execution: persistent waves. A wave takes rays in chunks of 64 from a global cursor
 // and refills each lane as soon as its ray finishes, so lanes do not wait for the
 // longest ray of the wave. Each lane keeps its traversal stack in LDS (the first
@@ -23,6 +31,7 @@ namespace pt {
 
 #define PT_LDS_STACK 16
 #define PT_TRACE_BLOCK 256
+#define PT_STACK8 192 // 8-wide traversal: at most 7 pending children per level
 
 struct StackEntry {
     uint32_t ref; // PT_REF_* word with the inside bit
@@ -32,6 +41,7 @@ struct StackEntry {
 struct TraceRay {
     V3 o, d, inv;
     float dmin, dmax;
+    float tnear; // max(dist_min, 0): where child boxes start to matter (8-wide traversal)
     int sx, sy, sz;
     bool any;
 };
@@ -77,6 +87,7 @@ __device__ __forceinline__ bool trace_begin(const BvhView& b, const float4 r0, c
     r.d = v3(r1.x, r1.y, r1.z);
     r.dmin = r0.w;
     r.dmax = r1.w;
+    r.tnear = fmaxf(r0.w, 0.0f);
     r.any = any;
     r.inv = v3(1.0f, 1.0f, 1.0f) / r.d; // Vector3f::One() / ray.dir (BinnedSAHBVH.cpp:280)
     r.sx = r.inv.x < 0.0f;
@@ -89,8 +100,42 @@ __device__ __forceinline__ bool trace_begin(const BvhView& b, const float4 r0, c
     s.nt = 0;
     if (b.root_ref == PT_REF_EMPTY) return false;
     s.nv = 1;
-    s.cur = b.root_ref;
+    s.cur = b.nodes8 ? b.root8 : b.root_ref;
     return slab(r.o, r.inv, r.sx, r.sy, r.sz, b.root_lo, b.root_hi, &s.cur_d);
+}
+
+// Tests the triangles of the leaf s.cur in leaf order (the reference's hit rule); true
+// when an any-hit ray has found its hit.
+template <bool STATS>
+__device__ __forceinline__ bool leaf_test(const BvhView& b, const TraceRay& r, TraceState& s)
+{
+    uint32_t first = pt_ref_index(s.cur), cnt = pt_ref_cnt(s.cur);
+    if (cnt == PT_REF_BIG) {
+        const int2 bl = b.big_leaves[first];
+        first = (uint32_t)bl.x;
+        cnt = (uint32_t)bl.y;
+    }
+    for (uint32_t k = first; k < first + cnt; ++k) {
+        if (STATS) ++s.nt;
+        const float4 t0 = b.tris[3 * (size_t)k];
+        const float4 t1 = b.tris[3 * (size_t)k + 1];
+        const float4 t2 = b.tris[3 * (size_t)k + 2];
+        TriDev tr;
+        tr.p0 = v3(t0.x, t0.y, t0.z);
+        tr.e1 = v3(t0.w, t1.x, t1.y);
+        tr.e2 = v3(t1.z, t1.w, t2.x);
+        tr.n = v3(t2.y, t2.z, t2.w);
+        float t;
+        if (tri_intersect(tr, r.o, r.d, &t)) {
+            if (t < r.dmin || t > r.dmax) continue;
+            if (s.best < 0 || t < s.best_t) {
+                s.best = (int)k;
+                s.best_t = t;
+                if (r.any) return true;
+            }
+        }
+    }
+    return false;
 }
 
 // One traversal step: the current entry (an inner node or a whole leaf), then the next
@@ -103,32 +148,7 @@ __device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, 
     if (s.best >= 0 && !(s.cur & PT_REF_INSIDE) && s.best_t < s.cur_d) {
         // culled: a nearer hit exists and the origin is outside this box
     } else if (pt_ref_cnt(s.cur) != 0) {
-        uint32_t first = pt_ref_index(s.cur), cnt = pt_ref_cnt(s.cur);
-        if (cnt == PT_REF_BIG) {
-            const int2 bl = b.big_leaves[first];
-            first = (uint32_t)bl.x;
-            cnt = (uint32_t)bl.y;
-        }
-        for (uint32_t k = first; k < first + cnt; ++k) {
-            if (STATS) ++s.nt;
-            const float4 t0 = b.tris[3 * (size_t)k];
-            const float4 t1 = b.tris[3 * (size_t)k + 1];
-            const float4 t2 = b.tris[3 * (size_t)k + 2];
-            TriDev tr;
-            tr.p0 = v3(t0.x, t0.y, t0.z);
-            tr.e1 = v3(t0.w, t1.x, t1.y);
-            tr.e2 = v3(t1.z, t1.w, t2.x);
-            tr.n = v3(t2.y, t2.z, t2.w);
-            float t;
-            if (tri_intersect(tr, r.o, r.d, &t)) {
-                if (t < r.dmin || t > r.dmax) continue;
-                if (s.best < 0 || t < s.best_t) {
-                    s.best = (int)k;
-                    s.best_t = t;
-                    if (r.any) return true;
-                }
-            }
-        }
+        if (leaf_test<STATS>(b, r, s)) return true;
     } else {
         const float4* nd = b.nodes + 4 * (size_t)pt_ref_index(s.cur);
         const float4 a = nd[0], bb = nd[1], c = nd[2], mf = nd[3];
@@ -170,6 +190,109 @@ __device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, 
     return false;
 }
 
+__device__ __forceinline__ void sort_pair(float& da, uint32_t& ra, float& db, uint32_t& rb)
+{
+    const bool sw = db < da;
+    const float lo_d = sw ? db : da, hi_d = sw ? da : db;
+    const uint32_t lo_r = sw ? rb : ra, hi_r = sw ? ra : rb;
+    da = lo_d;
+    ra = lo_r;
+    db = hi_d;
+    rb = hi_r;
+}
+
+__device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); }
+
+// One step of the 8-wide traversal: the current entry (a wide node or a leaf), then the
+// next entry from the stack. Returns true when the ray is finished.
+template <bool STATS>
+__device__ __forceinline__ bool trace8_step(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
+                                            StackEntry* spill)
+{
+    bool next = false;
+    if (s.best >= 0 && s.best_t < s.cur_d) {
+        // culled: the box starts beyond the nearest hit
+    } else if (pt_ref_cnt(s.cur) != 0) {
+        if (leaf_test<STATS>(b, r, s)) return true;
+    } else {
+        const uint4* nd = b.nodes8 + 8 * (size_t)pt_ref_index(s.cur);
+        const uint4 w0 = nd[0], w1 = nd[1], w2 = nd[2], w3 = nd[3], w4 = nd[4], w5 = nd[5];
+        const int cnt = (int)(w0.w >> 24);
+        // the ray in the node's quantised frame: t = q * (inv * step) + (p - o) * inv
+        const float ix = r.inv.x * __uint_as_float((w0.w & 0xffu) << 23);
+        const float iy = r.inv.y * __uint_as_float(((w0.w >> 8) & 0xffu) << 23);
+        const float iz = r.inv.z * __uint_as_float(((w0.w >> 16) & 0xffu) << 23);
+        const float ox = (__uint_as_float(w0.x) - r.o.x) * r.inv.x;
+        const float oy = (__uint_as_float(w0.y) - r.o.y) * r.inv.y;
+        const float oz = (__uint_as_float(w0.z) - r.o.z) * r.inv.z;
+        // entry and exit planes by ray octant (qlo: w1.xy, w1.zw, w2.xy; qhi: w2.zw, w3.xy, w3.zw)
+        const uint32_t nx0 = r.sx ? w2.z : w1.x, nx1 = r.sx ? w2.w : w1.y;
+        const uint32_t fx0 = r.sx ? w1.x : w2.z, fx1 = r.sx ? w1.y : w2.w;
+        const uint32_t ny0 = r.sy ? w3.x : w1.z, ny1 = r.sy ? w3.y : w1.w;
+        const uint32_t fy0 = r.sy ? w1.z : w3.x, fy1 = r.sy ? w1.w : w3.y;
+        const uint32_t nz0 = r.sz ? w3.z : w2.x, nz1 = r.sz ? w3.w : w2.y;
+        const uint32_t fz0 = r.sz ? w2.x : w3.z, fz1 = r.sz ? w2.y : w3.w;
+        const float tfar = s.best >= 0 ? fminf(s.best_t, r.dmax) : r.dmax;
+        float d[8];
+        uint32_t ref[8] = {w4.x, w4.y, w4.z, w4.w, w5.x, w5.y, w5.z, w5.w};
+        int hits = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = i & 3;
+            const float tx0 = fmaf(byte_f(i < 4 ? nx0 : nx1, k), ix, ox);
+            const float ty0 = fmaf(byte_f(i < 4 ? ny0 : ny1, k), iy, oy);
+            const float tz0 = fmaf(byte_f(i < 4 ? nz0 : nz1, k), iz, oz);
+            const float tx1 = fmaf(byte_f(i < 4 ? fx0 : fx1, k), ix, ox);
+            const float ty1 = fmaf(byte_f(i < 4 ? fy0 : fy1, k), iy, oy);
+            const float tz1 = fmaf(byte_f(i < 4 ? fz0 : fz1, k), iz, oz);
+            const float tmin = fmaxf(fmaxf(tx0, ty0), fmaxf(tz0, r.tnear));
+            const float tmax = fminf(fminf(tx1, ty1), fminf(tz1, tfar));
+            const bool hit = i < cnt && tmin <= tmax;
+            d[i] = hit ? tmin : __builtin_inff();
+            hits += hit ? 1 : 0;
+        }
+        if (STATS) s.nv += 1; // one 128-byte node record
+        // nearest first: 19-comparator sorting network for 8 keys
+        sort_pair(d[0], ref[0], d[2], ref[2]);
+        sort_pair(d[1], ref[1], d[3], ref[3]);
+        sort_pair(d[4], ref[4], d[6], ref[6]);
+        sort_pair(d[5], ref[5], d[7], ref[7]);
+        sort_pair(d[0], ref[0], d[4], ref[4]);
+        sort_pair(d[1], ref[1], d[5], ref[5]);
+        sort_pair(d[2], ref[2], d[6], ref[6]);
+        sort_pair(d[3], ref[3], d[7], ref[7]);
+        sort_pair(d[0], ref[0], d[1], ref[1]);
+        sort_pair(d[2], ref[2], d[3], ref[3]);
+        sort_pair(d[4], ref[4], d[5], ref[5]);
+        sort_pair(d[6], ref[6], d[7], ref[7]);
+        sort_pair(d[2], ref[2], d[4], ref[4]);
+        sort_pair(d[3], ref[3], d[5], ref[5]);
+        sort_pair(d[1], ref[1], d[4], ref[4]);
+        sort_pair(d[3], ref[3], d[6], ref[6]);
+        sort_pair(d[1], ref[1], d[2], ref[2]);
+        sort_pair(d[3], ref[3], d[4], ref[4]);
+        sort_pair(d[5], ref[5], d[6], ref[6]);
+#pragma unroll
+        for (int j = 7; j >= 1; --j) {
+            if (j < hits) {
+                stack_push(lds, spill, s.sp, ref[j], d[j]);
+                ++s.sp;
+            }
+        }
+        if (hits > 0) {
+            s.cur = ref[0];
+            s.cur_d = d[0];
+            next = true;
+        }
+    }
+    if (!next) {
+        if (s.sp == 0) return true;
+        --s.sp;
+        stack_pop(lds, spill, s.sp, &s.cur, &s.cur_d);
+    }
+    return false;
+}
+
 __device__ __forceinline__ void trace_finish(const BvhView& b, const TraceSegment& seg, int i, const TraceState& s)
 {
     const int p = seg.path ? seg.path[i] : i;
@@ -183,7 +306,7 @@ __device__ __forceinline__ void trace_finish(const BvhView& b, const TraceSegmen
     seg.out_k[p] = k;
 }
 
-template <bool STATS, bool MULTI>
+template <bool STATS, bool MULTI, bool WIDE>
 __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1, TraceJob job, int* cursor,
                                                                      unsigned long long* stats)
 {
@@ -198,7 +321,7 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1,
     if (total == 0) return;
     __shared__ uint2 lds_stack[PT_LDS_STACK * PT_TRACE_BLOCK];
     uint2* lds = lds_stack + threadIdx.x;
-    StackEntry spill[PT_MAX_STACK - PT_LDS_STACK];
+    StackEntry spill[(WIDE ? PT_STACK8 : PT_MAX_STACK) - PT_LDS_STACK];
     const int lane = threadIdx.x & 63;
     const unsigned long long lane_lt = (1ull << lane) - 1ull;
     int pool_next = 0, pool_end = 0; // wave-uniform: this wave's share of rays
@@ -260,7 +383,7 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1,
             continue;
         }
         if (item >= 0) {
-            if (trace_step<STATS>(b, r, s, lds, spill)) {
+            if (WIDE ? trace8_step<STATS>(b, r, s, lds, spill) : trace_step<STATS>(b, r, s, lds, spill)) {
                 if (STATS) {
                     nv += s.nv;
                     nt += s.nt;
@@ -287,13 +410,27 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
     const int bpc = tune.trace_blocks_per_cu < 1 ? 1 : (tune.trace_blocks_per_cu > 5 ? 5 : tune.trace_blocks_per_cu);
     const dim3 grid((unsigned)(cu_count() * bpc)), block(PT_TRACE_BLOCK);
     const bool multi = job.views != nullptr;
+    const bool wide = multi ? job.wide != 0 : b.nodes8 != nullptr;
+#define PT_LAUNCH_TRACE(S, M, W) \
+    hipLaunchKernelGGL((k_trace_persistent<S, M, W>), grid, block, 0, st, b, job, cursor, stats)
     if (stats) {
-        if (multi) hipLaunchKernelGGL((k_trace_persistent<true, true>), grid, block, 0, st, b, job, cursor, stats);
-        else hipLaunchKernelGGL((k_trace_persistent<true, false>), grid, block, 0, st, b, job, cursor, stats);
+        if (multi) {
+            if (wide) PT_LAUNCH_TRACE(true, true, true);
+            else PT_LAUNCH_TRACE(true, true, false);
+        } else {
+            if (wide) PT_LAUNCH_TRACE(true, false, true);
+            else PT_LAUNCH_TRACE(true, false, false);
+        }
     } else {
-        if (multi) hipLaunchKernelGGL((k_trace_persistent<false, true>), grid, block, 0, st, b, job, cursor, stats);
-        else hipLaunchKernelGGL((k_trace_persistent<false, false>), grid, block, 0, st, b, job, cursor, stats);
+        if (multi) {
+            if (wide) PT_LAUNCH_TRACE(false, true, true);
+            else PT_LAUNCH_TRACE(false, true, false);
+        } else {
+            if (wide) PT_LAUNCH_TRACE(false, false, true);
+            else PT_LAUNCH_TRACE(false, false, false);
+        }
     }
+#undef PT_LAUNCH_TRACE
     return hipGetLastError();
 }
 
