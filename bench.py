@@ -3,7 +3,7 @@
 Default workload: the single large scene — 3,145,728 triangles (256 UV spheres of 4096
 triangles over a 1024 x 1024-quad fBm terrain), 2048 x 1024 equirect sky with a sun lobe,
 1920 x 1080, 64 spp, max depth 12, MIS integrator, per-pixel seeded samplers, BVH built
-on the device and collapsed to 8-wide nodes. One step is one full frame. `value` is BVH
+on the device and walked in the reference's order. One step is one full frame. `value` is BVH
 queries (camera, bounce and shadow rays) per second over the timed steps, in Mrays/s.
 The oracle renders a random sample of the frame's pixels on the host cores: that is the
 CPU baseline, and the same pixels check the GPU frame (per-pixel relative error).
@@ -499,6 +499,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-wide", action="store_true", help="skip the 8-wide comparison frame")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=64)
@@ -510,7 +511,8 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--triangles", type=int, default=10_000_000, help="trace mode: triangle count")
     ap.add_argument("--bvh-width", type=int, default=None, choices=[2, 8],
-                    help="BVH width (default 8; 2 walks the binary tree in the reference's order)")
+                    help="BVH width: 2 walks the binary tree in the reference's order (frame default), "
+                         "8 the 8-wide tree (traversal microbenchmark default)")
     ap.add_argument("--jobs", type=int, default=64, help="batch mode: number of jobs")
     ap.add_argument("--tris-max", type=float, default=2e6, help="batch/serve modes: largest scene")
     ap.add_argument("--spp-max", type=int, default=256, help="batch/serve modes: largest spp")
@@ -550,7 +552,7 @@ def main():
     log("scene: %d triangles in %.1f s" % (len(sc["tri_mat"]), time.perf_counter() - t0))
     ctx = P.Context(0)
     s = ctx.scene(sc)
-    width = args.bvh_width or 8
+    width = args.bvh_width or 2
     bvh = s.build_bvh(16, width=width)
     info = bvh.info()
     log("bvh:", info)
@@ -612,17 +614,61 @@ def main():
         parity = {"pixels_checked": int(len(pix)), "max_rel_err": float(rel.max()),
                   "pixels_bit_identical": int(np.all(got == ref, axis=1).sum())}
         log("parity vs oracle:", parity)
-    roofline = None
+    # Roofline of the traversal kernel over every trace launch of the instrumented frame
+    # (extension, environment-shadow and area-shadow queues): algorithmic bytes from the
+    # GPU's own traversal counters, launch durations from HIP events.
     node_b = NODE8_B if width == 8 else NODE_B
-    if ext_ms > 0:
-        b_ray = RAY_IN_B + PATH_B + HIT_OUT_B + n_node * node_b + n_tri * TRI_B
-        achieved = ext_items * b_ray / (ext_ms * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": "k_trace_persistent (extension rays, instrumented frame)",
-                    "bytes_per_query": round(b_ray, 1), "n_node": round(n_node, 2), "n_tri": round(n_tri, 3),
-                    "s_node": node_b, "s_tri": TRI_B, "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4),
-                    "queries_per_launch": round(ext_items / max(ext_launches, 1), 1)}
+    kinds = ("extension", "shadow_any", "shadow_closest")
+    t_items = sum(kst["kernels"][k]["items"] for k in kinds)
+    t_launches = sum(kst["kernels"][k]["launches"] for k in kinds)
+    t_ms = sum(kst["kernels"][k]["ms"] for k in kinds)
+    t_nodes = sum(cst["kernels"][k]["nodes"] for k in kinds)
+    t_tris = sum(cst["kernels"][k]["tris"] for k in kinds)
+    alg_bytes = t_items * (RAY_IN_B + PATH_B + HIT_OUT_B) + t_nodes * node_b + t_tris * TRI_B
+    achieved = alg_bytes / (t_ms * 1e-3) / 1e9 if t_ms > 0 else 0.0
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "pmc_traffic_frame_w%d.json" % width)
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_trace_persistent, every trace launch of the instrumented frame",
+                "algorithmic_bytes_per_launch": round(alg_bytes / max(t_launches, 1), 1),
+                "bytes_per_query": round(alg_bytes / max(t_items, 1), 1),
+                "n_node": round(t_nodes / max(t_items, 1), 2), "n_tri": round(t_tris / max(t_items, 1), 3),
+                "s_node": node_b, "s_tri": TRI_B, "s_ray": RAY_IN_B + PATH_B + HIT_OUT_B,
+                "avg_launch_ms": round(t_ms / max(t_launches, 1), 4), "launches": t_launches,
+                "queries_per_launch": round(t_items / max(t_launches, 1), 1),
+                "extension_only": {"n_node": round(n_node, 2), "n_tri": round(n_tri, 3),
+                                   "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4)}}
+    # The same frame through the 8-wide tree (a faster walk that differs from the
+    # reference's on rays grazing a box within rounding of their hit), for comparison.
+    other = None
+    if width == 2 and not args.no_wide:
+        b8 = s.build_bvh(16, width=8)
+        out8 = torch.empty_like(out)
+        b8.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=out8)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        q8 = 0
+        for _ in range(2):
+            _, st8 = b8.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=out8)
+            q8 += st8["queries"]
+        torch.cuda.synchronize()
+        w8 = time.perf_counter() - t0
+        img8 = out8.cpu().numpy()
+        other = {"bvh": "collapsed to 8-wide nodes", "mrays_s": round(q8 / w8 / 1e6, 2),
+                 "ms_per_frame": round(w8 / 2 * 1e3, 2), "bvh_build_ms": round(b8.info()["build_ms"], 1),
+                 "pixels_identical_to_exact_frame": int(np.all(img8 == img, axis=2).sum())}
+        if parity is not None:
+            got8 = img8.reshape(-1, 3)[pix].astype(np.float64)
+            rel8 = np.abs(got8 - ref) / np.maximum(np.abs(ref), 1e-30)
+            other["parity_vs_oracle"] = {"pixels_checked": int(len(pix)), "max_rel_err": float(rel8.max()),
+                                         "pixels_bit_identical": int(np.all(got8 == ref, axis=1).sum()),
+                                         "pixels_over_1e-4": int((rel8.max(axis=1) > 1e-4).sum())}
+        log("8-wide:", other)
+        b8.close()
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 2), "higher_is_better": True,
@@ -641,6 +687,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
+        "bvh8_frame": other,
     }
     print(json.dumps(res), flush=True)
 

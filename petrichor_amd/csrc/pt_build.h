@@ -32,6 +32,7 @@ struct GpuBvhResult {
 struct GpuPacked {
     float4* nodes_d = nullptr;    // 4 float4 per inner node, depth-first preorder
     int2* big_leaves_d = nullptr; // (first slot, count) of leaves over PT_REF_MAX_SMALL_LEAF
+    int2* inner_range_d = nullptr; // leaf-slot range [begin, end) of every inner node (preorder)
     float4* tris_d = nullptr;     // 3 float4 per leaf slot (TriDev)
     int32_t* prim_ids_d = nullptr;
     int32_t* mat_d = nullptr;

@@ -576,12 +576,13 @@ __device__ __forceinline__ uint32_t child_ref(const BuildNode& c, int c_pre, con
 }
 
 __global__ void k_pack_nodes(const BuildNode* nodes, int num_nodes, const int* pre, const int* big_index,
-                             float4* out)
+                             float4* out, int2* inner_range)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= num_nodes) return;
     const BuildNode& x = nodes[g];
     if (x.leaf) return;
+    inner_range[pre[g]] = make_int2(x.begin, x.end);
     const BuildNode& l = nodes[x.child];
     const BuildNode& r = nodes[x.child + 1];
     float4* o = out + 4 * (size_t)pre[g];
@@ -908,7 +909,9 @@ hipError_t gpu_pack_bvh(const GpuBvhResult& r, int num_tris, const float* tri_po
     out->num_big = counts[1];
     B_CHECK(hipMalloc((void**)&out->nodes_d, sizeof(float4) * 4 * (size_t)std::max(out->num_inner, 1)));
     B_CHECK(hipMalloc((void**)&out->big_leaves_d, sizeof(int2) * (size_t)std::max(out->num_big, 1)));
-    hipLaunchKernelGGL(k_pack_nodes, dim3(blocks(nn)), dim3(256), 0, st, r.nodes_d, nn, pre, big_index, out->nodes_d);
+    B_CHECK(hipMalloc((void**)&out->inner_range_d, sizeof(int2) * (size_t)std::max(out->num_inner, 1)));
+    hipLaunchKernelGGL(k_pack_nodes, dim3(blocks(nn)), dim3(256), 0, st, r.nodes_d, nn, pre, big_index, out->nodes_d,
+                       out->inner_range_d);
     hipLaunchKernelGGL(k_fill_big, dim3(blocks(nn)), dim3(256), 0, st, r.nodes_d, nn, big_index, out->big_leaves_d);
     B_CHECK(hipMalloc((void**)&out->tris_d, sizeof(float4) * 3 * (size_t)num_tris));
     B_CHECK(hipMalloc((void**)&out->prim_ids_d, sizeof(int) * (size_t)num_tris));
@@ -943,7 +946,7 @@ hipError_t gpu_pack_bvh(const GpuBvhResult& r, int num_tris, const float* tri_po
 
 void gpu_packed_free(GpuPacked* p)
 {
-    for (void* q : {(void*)p->nodes_d, (void*)p->big_leaves_d, (void*)p->tris_d, (void*)p->prim_ids_d,
+    for (void* q : {(void*)p->nodes_d, (void*)p->big_leaves_d, (void*)p->inner_range_d, (void*)p->tris_d, (void*)p->prim_ids_d,
                     (void*)p->mat_d, (void*)p->mat_sh_d, (void*)p->smooth_d, (void*)p->nrm_d})
         if (q) (void)hipFree(q);
     *p = GpuPacked();

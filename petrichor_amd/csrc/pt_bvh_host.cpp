@@ -227,7 +227,7 @@ bool build_bvh_host(const float* tri_pos, int num_tris, int num_bins, int leaf_m
 // encoded as in PT_REF_* (pt_device.h); leaves of more than PT_REF_MAX_SMALL_LEAF
 // triangles go through the big-leaf table (first slot, count).
 void pack_bvh2(const HostBvh& h, std::vector<float4>& nodes, std::vector<int2>& big_leaves, V3* root_lo,
-               V3* root_hi, uint32_t* root_ref)
+               V3* root_hi, uint32_t* root_ref, std::vector<int2>* inner_range)
 {
     nodes.clear();
     big_leaves.clear();
@@ -245,6 +245,11 @@ void pack_bvh2(const HostBvh& h, std::vector<float4>& nodes, std::vector<int2>& 
         st.push_back(h.nodes[i].child[0]);
     }
     nodes.resize(4 * (size_t)inner);
+    if (inner_range) {
+        inner_range->assign((size_t)inner, make_int2(0, 0));
+        for (int i = 0; i < nn; ++i)
+            if (gidx[i] >= 0) (*inner_range)[gidx[i]] = make_int2(h.nodes[i].begin, h.nodes[i].end);
+    }
     // big leaves are numbered in leaf-slot order
     for (const HostNode& x : h.nodes)
         if (x.leaf && x.end - x.begin > PT_REF_MAX_SMALL_LEAF) big_leaves.push_back(make_int2(x.begin, x.end - x.begin));

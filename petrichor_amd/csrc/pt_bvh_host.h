@@ -24,6 +24,6 @@ struct HostBvh {
 
 bool build_bvh_host(const float* tri_pos, int num_tris, int num_bins, int leaf_max, HostBvh* out);
 void pack_bvh2(const HostBvh& h, std::vector<float4>& nodes, std::vector<int2>& big_leaves, V3* root_lo,
-               V3* root_hi, uint32_t* root_ref);
+               V3* root_hi, uint32_t* root_ref, std::vector<int2>* inner_range = nullptr);
 
 } // namespace pt

@@ -94,6 +94,7 @@ int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
     }
     b->nodes_d = pk.nodes_d;
     b->big_leaves_d = pk.big_leaves_d;
+    b->inner_range_d = pk.inner_range_d;
     b->tris_d = pk.tris_d;
     b->prim_ids_d = pk.prim_ids_d;
     b->tri_mat_d = pk.mat_d;
@@ -104,6 +105,7 @@ int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
     b->view.tris = b->tris_d;
     b->view.prim_ids = b->prim_ids_d;
     b->view.big_leaves = b->big_leaves_d;
+    b->view.inner_range = b->inner_range_d;
     b->view.root_lo = pk.root_lo;
     b->view.root_hi = pk.root_hi;
     b->view.root_ref = pk.root_ref;
@@ -114,7 +116,7 @@ int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
     b->host_ready = false;
     b->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     const int64_t nt = s->num_tris;
-    b->device_bytes = (int64_t)pk.num_inner * 64 + (int64_t)pk.num_big * 8 + nt * 60 +
+    b->device_bytes = (int64_t)pk.num_inner * 72 + (int64_t)pk.num_big * 8 + nt * 60 +
                       (pk.smooth_d ? nt * 4 : 0) + (pk.nrm_d ? nt * 36 : 0);
     *out = b;
     return PT_OK;
@@ -420,10 +422,10 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     b->num_nodes = (int)b->host.nodes.size();
     b->max_depth = b->host.max_depth;
     std::vector<float4> nodes;
-    std::vector<int2> big;
+    std::vector<int2> big, ranges;
     V3 rlo, rhi;
     uint32_t rref;
-    pack_bvh2(b->host, nodes, big, &rlo, &rhi, &rref);
+    pack_bvh2(b->host, nodes, big, &rlo, &rhi, &rref, &ranges);
     const size_t nt = (size_t)s->num_tris;
     std::vector<float4> tris(3 * nt);
     std::vector<int32_t> mat(nt), mat_sh(nt), smooth;
@@ -447,6 +449,7 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     hipError_t err = hipSetDevice(ctx->device);
     if (err == hipSuccess) err = upload(&b->nodes_d, nodes.data(), nodes.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&b->big_leaves_d, big.data(), big.size(), ctx->stream);
+    if (err == hipSuccess) err = upload(&b->inner_range_d, ranges.data(), ranges.size(), ctx->stream);
     if (err == hipSuccess) err = hipMalloc((void**)&b->cursor_d, 4 * sizeof(int));
     if (err == hipSuccess) err = upload(&b->tris_d, tris.data(), tris.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&b->prim_ids_d, b->host.order.data(), nt, ctx->stream);
@@ -456,7 +459,7 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     if (err == hipSuccess) err = upload(&b->tri_nrm_d, nrm.data(), nrm.size(), ctx->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
     if (err != hipSuccess) {
-        free_all({b->nodes_d, b->big_leaves_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
+        free_all({b->nodes_d, b->big_leaves_d, b->inner_range_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
                   b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d});
         delete b;
         return fail(ctx, err == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
@@ -469,8 +472,10 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     b->view.root_hi = rhi;
     b->view.root_ref = rref;
     b->view.big_leaves = b->big_leaves_d;
+    b->view.inner_range = b->inner_range_d;
     b->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     b->device_bytes = (int64_t)(nodes.size() * sizeof(float4) + big.size() * sizeof(int2) + tris.size() * sizeof(float4)) +
+                      (int64_t)ranges.size() * 8 +
                       (int64_t)nt * 12 + (int64_t)(smooth.size() * 4 + nrm.size() * 4);
     b->num_inner = (int)(nodes.size() / 4);
     *out = b;
@@ -481,7 +486,7 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
 int pt_bvh_destroy(pt_bvh* b)
 {
     if (!b) return PT_ERR_ARG;
-    free_all({b->nodes8_d, b->nodes_d, b->big_leaves_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
+    free_all({b->nodes8_d, b->nodes_d, b->big_leaves_d, b->inner_range_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
               b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d});
     delete b;
     return PT_OK;

@@ -45,6 +45,7 @@ struct pt_bvh {
     pt::HostBvh host;
     float4* nodes_d = nullptr;
     int2* big_leaves_d = nullptr;
+    int2* inner_range_d = nullptr; // leaf-slot range of every inner node (preorder)
     int* cursor_d = nullptr; // trace work cursor of the API entry points
     float4* tris_d = nullptr;
     int32_t* prim_ids_d = nullptr;

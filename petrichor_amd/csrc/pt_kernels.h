@@ -21,6 +21,7 @@ struct BvhView {
     const int2* big_leaves;   // (first slot, count) of leaves over PT_REF_MAX_SMALL_LEAF triangles
     V3 root_lo, root_hi;
     uint32_t root_ref;        // PT_REF_* encoding, PT_REF_EMPTY for an empty scene
+    const int2* inner_range = nullptr; // leaf-slot range of every binary inner node (ties, 8-wide)
     const uint4* nodes8 = nullptr;  // 8-wide nodes (pt_wide.hip), null for the binary traversal
     uint32_t root8 = PT_REF_EMPTY;  // root of the 8-wide tree (wide node 0, or a leaf)
 };

@@ -15,8 +15,9 @@
 // the nearest hit child next and pushes the others farthest first with their entry
 // distance; a popped entry is skipped when the nearest hit is closer than its entry
 // distance. Leaves, triangle tests and the hit rule are the reference's, so it finds the
-// same nearest hit; only which of several hits at exactly the same distance wins, and
-// rays that graze a box within rounding of the hit, can differ.
+// same nearest hit, and of several hits at exactly the same distance it keeps the one
+// the reference's walk reaches first (reference_first); only rays that graze a box
+// within rounding of their hit can differ.
 //
 // Execution: persistent waves. A wave takes rays in chunks of 64 from a global cursor
 // and refills each lane as soon as its ray finishes, so lanes do not wait for the
@@ -104,9 +105,51 @@ __device__ __forceinline__ bool trace_begin(const BvhView& b, const float4 r0, c
     return slab(r.o, r.inv, r.sx, r.sy, r.sz, b.root_lo, b.root_hi, &s.cur_d);
 }
 
+// Whether the reference's walk reaches leaf slot a before leaf slot c, both hit by the
+// ray: below their lowest common ancestor in the binary tree it descends first into the
+// child whose box centroid is nearer the ray origin (and finishes that subtree before
+// the other); inside one leaf the slots are tested in order. Used by the 8-wide
+// traversal to break exact distance ties the way the reference does.
+__device__ __noinline__ bool reference_first(const BvhView& b, const TraceRay& r, int a, int c)
+{
+    uint32_t ref = b.root_ref;
+    for (int depth = 0; depth < PT_MAX_STACK; ++depth) {
+        if (pt_ref_cnt(ref) != 0) return a < c;
+        const float4* nd = b.nodes + 4 * (size_t)pt_ref_index(ref);
+        const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], m = nd[3];
+        const uint32_t ml = __float_as_uint(m.x), mr = __float_as_uint(m.y);
+        int lb, le;
+        if (pt_ref_cnt(ml) == 0) {
+            const int2 rg = b.inner_range[pt_ref_index(ml)];
+            lb = rg.x;
+            le = rg.y;
+        } else if (pt_ref_cnt(ml) == PT_REF_BIG) {
+            const int2 bl = b.big_leaves[pt_ref_index(ml)];
+            lb = bl.x;
+            le = bl.x + bl.y;
+        } else {
+            lb = (int)pt_ref_index(ml);
+            le = lb + (int)pt_ref_cnt(ml);
+        }
+        const bool a_left = a >= lb && a < le, c_left = c >= lb && c < le;
+        if (a_left == c_left) {
+            ref = a_left ? ml : mr;
+            continue;
+        }
+        const V3 llo = v3(q0.x, q0.y, q0.z), lhi = v3(q0.w, q1.x, q1.y);
+        const V3 rlo = v3(q1.z, q1.w, q2.x), rhi = v3(q2.y, q2.z, q2.w);
+        const float dl = sq_length(0.5f * (llo + lhi) - r.o);
+        const float dr = sq_length(0.5f * (rlo + rhi) - r.o);
+        return a_left == (dl < dr);
+    }
+    return a < c;
+}
+
 // Tests the triangles of the leaf s.cur in leaf order (the reference's hit rule); true
-// when an any-hit ray has found its hit.
-template <bool STATS>
+// when an any-hit ray has found its hit. The 8-wide traversal meets leaves out of the
+// reference's order, so an exactly equal distance goes to the triangle the reference
+// would have reached first.
+template <bool STATS, bool WIDE>
 __device__ __forceinline__ bool leaf_test(const BvhView& b, const TraceRay& r, TraceState& s)
 {
     uint32_t first = pt_ref_index(s.cur), cnt = pt_ref_cnt(s.cur);
@@ -132,6 +175,8 @@ __device__ __forceinline__ bool leaf_test(const BvhView& b, const TraceRay& r, T
                 s.best = (int)k;
                 s.best_t = t;
                 if (r.any) return true;
+            } else if (WIDE && t == s.best_t && reference_first(b, r, (int)k, s.best)) {
+                s.best = (int)k;
             }
         }
     }
@@ -148,7 +193,7 @@ __device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, 
     if (s.best >= 0 && !(s.cur & PT_REF_INSIDE) && s.best_t < s.cur_d) {
         // culled: a nearer hit exists and the origin is outside this box
     } else if (pt_ref_cnt(s.cur) != 0) {
-        if (leaf_test<STATS>(b, r, s)) return true;
+        if (leaf_test<STATS, false>(b, r, s)) return true;
     } else {
         const float4* nd = b.nodes + 4 * (size_t)pt_ref_index(s.cur);
         const float4 a = nd[0], bb = nd[1], c = nd[2], mf = nd[3];
@@ -213,7 +258,7 @@ __device__ __forceinline__ bool trace8_step(const BvhView& b, const TraceRay& r,
     if (s.best >= 0 && s.best_t < s.cur_d) {
         // culled: the box starts beyond the nearest hit
     } else if (pt_ref_cnt(s.cur) != 0) {
-        if (leaf_test<STATS>(b, r, s)) return true;
+        if (leaf_test<STATS, true>(b, r, s)) return true;
     } else {
         const uint4* nd = b.nodes8 + 8 * (size_t)pt_ref_index(s.cur);
         const uint4 w0 = nd[0], w1 = nd[1], w2 = nd[2], w3 = nd[3], w4 = nd[4], w5 = nd[5];
