@@ -200,7 +200,8 @@ int pt_engine_destroy(pt_engine* engine);
  * counterpart (the reference's ThreadPool takes a thread count, ThreadPool.cpp:19-26). */
 typedef struct pt_tuning {
     int32_t iterations_per_sync; /* graph replays between host synchronisations, 1..256 */
-    int32_t trace_blocks_per_cu; /* persistent trace blocks of 256 threads per CU, 1..5 */
+    int32_t trace_blocks_per_cu; /* persistent trace blocks of 256 threads per CU, 1..8 (0 = as many
+                                    as the LDS stacks allow; larger values are clamped to that) */
     int32_t trace_chunk;         /* rays a wave takes from the shared cursor at a time, 64..4096 */
     int32_t shade_blocks_per_cu; /* shading blocks of 256 threads per CU, 1..32 */
 } pt_tuning;

@@ -419,7 +419,7 @@ class Engine:
                                             ctypes.byref(h)), "pt_engine_create")
         self.handle = h
 
-    def set_tuning(self, iterations_per_sync=8, trace_blocks_per_cu=5, trace_chunk=64, shade_blocks_per_cu=8):
+    def set_tuning(self, iterations_per_sync=8, trace_blocks_per_cu=0, trace_chunk=64, shade_blocks_per_cu=8):
         t = Tuning(iterations_per_sync, trace_blocks_per_cu, trace_chunk, shade_blocks_per_cu)
         self.ctx.check(self.ctx._lib.pt_engine_set_tuning(self.handle, ctypes.byref(t)), "pt_engine_set_tuning")
 

@@ -9,11 +9,11 @@ changes launch shapes and scheduling), so the fastest one is taken as is.
 """
 import time
 
-DEFAULT = {"iterations_per_sync": 8, "trace_blocks_per_cu": 5, "trace_chunk": 64, "shade_blocks_per_cu": 8}
+DEFAULT = {"iterations_per_sync": 8, "trace_blocks_per_cu": 0, "trace_chunk": 64, "shade_blocks_per_cu": 8}
 
 SPACE = {
     "iterations_per_sync": [2, 4, 8, 16, 32],
-    "trace_blocks_per_cu": [2, 3, 4, 5],
+    "trace_blocks_per_cu": [2, 3, 4, 6],  # 0 (default): as many as the LDS stacks allow
     "trace_chunk": [64, 128, 256, 512],
     "shade_blocks_per_cu": [4, 8, 16],
 }

@@ -286,8 +286,8 @@ int pt_render_batch(pt_engine* E, int32_t n, const pt_job_desc* jobs, int32_t it
 int pt_engine_set_tuning(pt_engine* E, const pt_tuning* t)
 {
     if (!E || !t) return PT_ERR_ARG;
-    if (t->iterations_per_sync < 1 || t->iterations_per_sync > 256 || t->trace_blocks_per_cu < 1 ||
-        t->trace_blocks_per_cu > 5 || t->trace_chunk < 64 || t->trace_chunk > 4096 || t->shade_blocks_per_cu < 1 ||
+    if (t->iterations_per_sync < 1 || t->iterations_per_sync > 256 || t->trace_blocks_per_cu < 0 ||
+        t->trace_blocks_per_cu > 8 || t->trace_chunk < 64 || t->trace_chunk > 4096 || t->shade_blocks_per_cu < 1 ||
         t->shade_blocks_per_cu > 32)
         return pt_fail(E->ctx, PT_ERR_ARG, "tuning out of range");
     E->iters_per_sync = t->iterations_per_sync;

@@ -105,7 +105,7 @@ struct TraceJob {
 
 // Launch shapes chosen by the autotuner (defaults fit every workload reasonably).
 struct LaunchTuning {
-    int trace_blocks_per_cu = 5; // 256-thread blocks with 32 KB of LDS stacks: at most 5 per CU
+    int trace_blocks_per_cu = 0; // 256-thread blocks per CU (0: as many as their LDS stacks allow)
     int trace_chunk = 64;        // >= 64, the wave width
     int shade_blocks_per_cu = 8;
 };

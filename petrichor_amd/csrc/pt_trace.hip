@@ -21,8 +21,9 @@
 //
 // Execution: persistent waves. A wave takes rays in chunks of 64 from a global cursor
 // and refills each lane as soon as its ray finishes, so lanes do not wait for the
-// longest ray of the wave. Each lane keeps its traversal stack in LDS (the first
-// PT_LDS_STACK entries) with deeper entries in private memory.
+// longest ray of the wave. A step handles one inner node or one triangle. Each lane keeps
+// its traversal stack in LDS (the first PT_LDS_STACK entries) with deeper entries in
+// private memory; the LDS size per block sets the blocks per CU.
 #include <hip/hip_runtime.h>
 
 #include "pt_device.h"
@@ -30,7 +31,7 @@
 
 namespace pt {
 
-#define PT_LDS_STACK 16
+#define PT_LDS_STACK 10 // 20 KB per 256-thread block: 8 blocks (32 waves) per CU
 #define PT_TRACE_BLOCK 256
 #define PT_STACK8 192 // 8-wide traversal: at most 7 pending children per level
 
@@ -54,6 +55,7 @@ struct TraceState {
     float cur_d;
     int sp;
     uint32_t nv, nt; // node visits (slab tests) and triangle tests
+    uint32_t leaf_k, leaf_end; // triangles of the current leaf still to test
 };
 
 __device__ __forceinline__ void stack_push(uint2* lds, StackEntry* spill, int sp, uint32_t ref, float d)
@@ -99,6 +101,7 @@ __device__ __forceinline__ bool trace_begin(const BvhView& b, const float4 r0, c
     s.sp = 0;
     s.nv = 0;
     s.nt = 0;
+    s.leaf_k = s.leaf_end = 0;
     if (b.root_ref == PT_REF_EMPTY) return false;
     s.nv = 1;
     s.cur = b.nodes8 ? b.root8 : b.root_ref;
@@ -145,12 +148,37 @@ __device__ __noinline__ bool reference_first(const BvhView& b, const TraceRay& r
     return a < c;
 }
 
-// Tests the triangles of the leaf s.cur in leaf order (the reference's hit rule); true
-// when an any-hit ray has found its hit. The 8-wide traversal meets leaves out of the
-// reference's order, so an exactly equal distance goes to the triangle the reference
-// would have reached first.
+// Tests triangle k (the reference's hit rule); true when an any-hit ray has found its
+// hit. The 8-wide traversal meets leaves out of the reference's order, so an exactly
+// equal distance goes to the triangle the reference would have reached first.
 template <bool STATS, bool WIDE>
-__device__ __forceinline__ bool leaf_test(const BvhView& b, const TraceRay& r, TraceState& s)
+__device__ __forceinline__ bool tri_test(const BvhView& b, const TraceRay& r, TraceState& s, uint32_t k)
+{
+    if (STATS) ++s.nt;
+    const float4 t0 = b.tris[3 * (size_t)k];
+    const float4 t1 = b.tris[3 * (size_t)k + 1];
+    const float4 t2 = b.tris[3 * (size_t)k + 2];
+    TriDev tr;
+    tr.p0 = v3(t0.x, t0.y, t0.z);
+    tr.e1 = v3(t0.w, t1.x, t1.y);
+    tr.e2 = v3(t1.z, t1.w, t2.x);
+    tr.n = v3(t2.y, t2.z, t2.w);
+    float t;
+    if (tri_intersect(tr, r.o, r.d, &t) && !(t < r.dmin || t > r.dmax)) {
+        if (s.best < 0 || t < s.best_t) {
+            s.best = (int)k;
+            s.best_t = t;
+            if (r.any) return true;
+        } else if (WIDE && t == s.best_t && reference_first(b, r, (int)k, s.best)) {
+            s.best = (int)k;
+        }
+    }
+    return false;
+}
+
+// Enters the leaf s.cur: its triangles are tested in leaf order from the next call of
+// leaf_or_pop on.
+__device__ __forceinline__ void enter_leaf(const BvhView& b, TraceState& s)
 {
     uint32_t first = pt_ref_index(s.cur), cnt = pt_ref_cnt(s.cur);
     if (cnt == PT_REF_BIG) {
@@ -158,42 +186,44 @@ __device__ __forceinline__ bool leaf_test(const BvhView& b, const TraceRay& r, T
         first = (uint32_t)bl.x;
         cnt = (uint32_t)bl.y;
     }
-    for (uint32_t k = first; k < first + cnt; ++k) {
-        if (STATS) ++s.nt;
-        const float4 t0 = b.tris[3 * (size_t)k];
-        const float4 t1 = b.tris[3 * (size_t)k + 1];
-        const float4 t2 = b.tris[3 * (size_t)k + 2];
-        TriDev tr;
-        tr.p0 = v3(t0.x, t0.y, t0.z);
-        tr.e1 = v3(t0.w, t1.x, t1.y);
-        tr.e2 = v3(t1.z, t1.w, t2.x);
-        tr.n = v3(t2.y, t2.z, t2.w);
-        float t;
-        if (tri_intersect(tr, r.o, r.d, &t)) {
-            if (t < r.dmin || t > r.dmax) continue;
-            if (s.best < 0 || t < s.best_t) {
-                s.best = (int)k;
-                s.best_t = t;
-                if (r.any) return true;
-            } else if (WIDE && t == s.best_t && reference_first(b, r, (int)k, s.best)) {
-                s.best = (int)k;
-            }
-        }
+    s.leaf_k = first;
+    s.leaf_end = first + cnt;
+}
+
+// End of a step: one triangle of the current leaf (so that lanes in leaves and lanes in
+// inner nodes do similar work per step), then, when the step did not move to a child
+// (`next`) and no triangle is left, the next entry from the stack. Returns true when the
+// ray is finished.
+template <bool STATS, bool WIDE>
+__device__ __forceinline__ bool leaf_or_pop(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
+                                            StackEntry* spill, bool next)
+{
+    if (s.leaf_k < s.leaf_end) {
+        if (tri_test<STATS, WIDE>(b, r, s, s.leaf_k++)) return true;
+        if (s.leaf_k < s.leaf_end) return false;
+        next = false;
+    }
+    if (!next) {
+        if (s.sp == 0) return true;
+        --s.sp;
+        stack_pop(lds, spill, s.sp, &s.cur, &s.cur_d);
     }
     return false;
 }
 
-// One traversal step: the current entry (an inner node or a whole leaf), then the next
-// entry from the stack. Returns true when the ray is finished.
+// One traversal step: an inner node or triangles of a leaf, then the next entry from the
+// stack when the current one is done. Returns true when the ray is finished.
 template <bool STATS>
 __device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
                                            StackEntry* spill)
 {
     bool next = false;
-    if (s.best >= 0 && !(s.cur & PT_REF_INSIDE) && s.best_t < s.cur_d) {
+    if (s.leaf_k < s.leaf_end) {
+        // in the middle of a leaf
+    } else if (s.best >= 0 && !(s.cur & PT_REF_INSIDE) && s.best_t < s.cur_d) {
         // culled: a nearer hit exists and the origin is outside this box
     } else if (pt_ref_cnt(s.cur) != 0) {
-        if (leaf_test<STATS, false>(b, r, s)) return true;
+        enter_leaf(b, s);
     } else {
         const float4* nd = b.nodes + 4 * (size_t)pt_ref_index(s.cur);
         const float4 a = nd[0], bb = nd[1], c = nd[2], mf = nd[3];
@@ -227,12 +257,7 @@ __device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, 
             next = true;
         }
     }
-    if (!next) {
-        if (s.sp == 0) return true;
-        --s.sp;
-        stack_pop(lds, spill, s.sp, &s.cur, &s.cur_d);
-    }
-    return false;
+    return leaf_or_pop<STATS, false>(b, r, s, lds, spill, next);
 }
 
 __device__ __forceinline__ void sort_pair(float& da, uint32_t& ra, float& db, uint32_t& rb)
@@ -255,10 +280,12 @@ __device__ __forceinline__ bool trace8_step(const BvhView& b, const TraceRay& r,
                                             StackEntry* spill)
 {
     bool next = false;
-    if (s.best >= 0 && s.best_t < s.cur_d) {
+    if (s.leaf_k < s.leaf_end) {
+        // in the middle of a leaf
+    } else if (s.best >= 0 && s.best_t < s.cur_d) {
         // culled: the box starts beyond the nearest hit
     } else if (pt_ref_cnt(s.cur) != 0) {
-        if (leaf_test<STATS, true>(b, r, s)) return true;
+        enter_leaf(b, s);
     } else {
         const uint4* nd = b.nodes8 + 8 * (size_t)pt_ref_index(s.cur);
         const uint4 w0 = nd[0], w1 = nd[1], w2 = nd[2], w3 = nd[3], w4 = nd[4], w5 = nd[5];
@@ -330,12 +357,7 @@ __device__ __forceinline__ bool trace8_step(const BvhView& b, const TraceRay& r,
             next = true;
         }
     }
-    if (!next) {
-        if (s.sp == 0) return true;
-        --s.sp;
-        stack_pop(lds, spill, s.sp, &s.cur, &s.cur_d);
-    }
-    return false;
+    return leaf_or_pop<STATS, true>(b, r, s, lds, spill, next);
 }
 
 __device__ __forceinline__ void trace_finish(const BvhView& b, const TraceSegment& seg, int i, const TraceState& s)
@@ -451,8 +473,12 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
     TraceJob job = job_in;
     // a wave refills at most 64 lanes from one chunk, so chunks hold at least 64 rays
     job.chunk = tune.trace_chunk < 64 ? 64 : tune.trace_chunk;
-    // persistent grid: blocks of 4 waves, each with a 32 KB LDS stack (at most 5 per CU)
-    const int bpc = tune.trace_blocks_per_cu < 1 ? 1 : (tune.trace_blocks_per_cu > 5 ? 5 : tune.trace_blocks_per_cu);
+    // persistent grid: blocks of 4 waves; the LDS stacks of a block bound the blocks per CU
+    // (160 KB of LDS, at most 8 waves per SIMD)
+    constexpr int lds_block = PT_LDS_STACK * PT_TRACE_BLOCK * (int)sizeof(uint2);
+    constexpr int max_bpc = (160 * 1024) / lds_block < 8 ? (160 * 1024) / lds_block : 8;
+    const int bpc = tune.trace_blocks_per_cu < 1 || tune.trace_blocks_per_cu > max_bpc ? max_bpc
+                                                                                     : tune.trace_blocks_per_cu;
     const dim3 grid((unsigned)(cu_count() * bpc)), block(PT_TRACE_BLOCK);
     const bool multi = job.views != nullptr;
     const bool wide = multi ? job.wide != 0 : b.nodes8 != nullptr;
