@@ -1,0 +1,605 @@
+// Petrichor.h — C++ host API of the MI355X render path with the names and behaviour of
+// LibPetrichor Core: Math::Vector3f, Scene, Camera, the materials, Texture2D,
+// Environment, RenderSetting and Petrichor::Render. Host code written against the
+// reference builds against this header; Render flattens the scene and calls the C-ABI
+// of petrichor_hip.h (libpetrichor_hip.so).
+//
+// Only what the GPU path renders is here: triangles (flat or smooth), Lambert, GGX,
+// Glass, Emission and MixMaterial, an equirect environment given as a Texture2D,
+// the MIS and simple integrators. File loading (OBJ models, JSON scenes, images) and the
+// denoising AOVs are not part of this API.
+//
+// Floating point: the camera set-up below must be compiled without FMA contraction
+// (-ffp-contract=off) to give the reference's camera state bit for bit.
+#pragma once
+
+#include <array>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <unordered_map>
+#include <vector>
+
+#include "../petrichor_hip.h"
+
+namespace Petrichor
+{
+namespace Math
+{
+
+// Math/Vector3f.h: float components, operators evaluated left to right.
+struct Vector3f
+{
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+
+    constexpr Vector3f() = default;
+    constexpr Vector3f(float x_, float y_, float z_) : x(x_), y(y_), z(z_) {}
+
+    constexpr Vector3f operator+(const Vector3f& v) const { return {x + v.x, y + v.y, z + v.z}; }
+    constexpr Vector3f operator-(const Vector3f& v) const { return {x - v.x, y - v.y, z - v.z}; }
+    constexpr Vector3f operator-() const { return {-x, -y, -z}; }
+    constexpr Vector3f operator*(float c) const { return {x * c, y * c, z * c}; }
+    constexpr Vector3f operator*(const Vector3f& v) const { return {x * v.x, y * v.y, z * v.z}; }
+    constexpr Vector3f operator/(float c) const { return {x / c, y / c, z / c}; }
+    friend constexpr Vector3f operator*(float c, const Vector3f& v) { return v * c; }
+
+    float SquaredLength() const { return x * x + y * y + z * z; }
+    // sqrt of a float: the correctly rounded root, as the reference's sqrt(Dot(...))
+    float Length() const { return std::sqrt(SquaredLength()); }
+    Vector3f Normalized() const { return *this / Length(); }
+
+    static constexpr Vector3f Zero() { return {0.0f, 0.0f, 0.0f}; }
+    static constexpr Vector3f One() { return {1.0f, 1.0f, 1.0f}; }
+    static constexpr Vector3f UnitX() { return {1.0f, 0.0f, 0.0f}; }
+    static constexpr Vector3f UnitY() { return {0.0f, 1.0f, 0.0f}; }
+    static constexpr Vector3f UnitZ() { return {0.0f, 0.0f, 1.0f}; }
+};
+
+inline float Dot(const Vector3f& a, const Vector3f& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+inline Vector3f Cross(const Vector3f& a, const Vector3f& b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+} // namespace Math
+
+using Color3f = Math::Vector3f;
+
+struct RenderingResult
+{
+    float totalSec;
+};
+
+namespace Core
+{
+
+// Core/Material/MaterialBase.h:15-22 (the order of the C-ABI's PT_MAT_* codes).
+enum class MaterialTypes
+{
+    Lambert,
+    Glass,
+    Emission,
+    Glossy,
+    Mix
+};
+
+class MaterialBase
+{
+public:
+    virtual ~MaterialBase() = default;
+    virtual MaterialTypes GetMaterialType() const = 0;
+    void EnableImportanceSampling(bool isEnabled) { m_isImportanceSamplingEnabled = isEnabled; }
+    bool IsImportanceSamplingEnabled() const { return m_isImportanceSamplingEnabled; }
+
+private:
+    bool m_isImportanceSamplingEnabled = true;
+};
+
+class Lambert : public MaterialBase
+{
+public:
+    explicit Lambert(const Color3f& kd) : m_kd(kd) {}
+    MaterialTypes GetMaterialType() const override { return MaterialTypes::Lambert; }
+    const Color3f& GetKd() const { return m_kd; }
+
+private:
+    Color3f m_kd;
+};
+
+class GGX : public MaterialBase
+{
+public:
+    GGX(const Color3f& f0, float roughness = 1.0f) : m_f0(f0), m_roughness(roughness) {}
+    MaterialTypes GetMaterialType() const override { return MaterialTypes::Glossy; }
+    const Color3f& GetReflectance() const { return m_f0; }
+    float GetRoughness() const { return m_roughness; }
+
+private:
+    Color3f m_f0;
+    float m_roughness;
+};
+
+class Glass : public MaterialBase
+{
+public:
+    Glass() = default;
+    Glass(const Color3f& color, float ior) noexcept : m_color(color), m_ior(ior) {}
+    MaterialTypes GetMaterialType() const override { return MaterialTypes::Glass; }
+    const Color3f& GetColor() const { return m_color; }
+    float GetIor() const { return m_ior; }
+
+private:
+    Color3f m_color = Color3f::One();
+    float m_ior = 1.333f; // IOR::Water
+};
+
+class Emission : public MaterialBase
+{
+public:
+    Emission(const Color3f& color) : m_color(color) {}
+    MaterialTypes GetMaterialType() const override { return MaterialTypes::Emission; }
+    const Color3f& GetLightColor() const { return m_color; }
+
+private:
+    Color3f m_color;
+};
+
+class MixMaterial : public MaterialBase
+{
+public:
+    MixMaterial(const MaterialBase* mat0, const MaterialBase* mat1, float mix) : m_mat0(mat0), m_mat1(mat1), m_mix(mix)
+    {
+    }
+    MaterialTypes GetMaterialType() const override { return MaterialTypes::Mix; }
+    // MixMaterial.h:45-49
+    const MaterialBase* GetSingleMaterial(float rand) const { return m_mix <= rand ? m_mat0 : m_mat1; }
+    const MaterialBase* GetMaterial0() const { return m_mat0; }
+    const MaterialBase* GetMaterial1() const { return m_mat1; }
+    float GetMix() const { return m_mix; }
+
+private:
+    const MaterialBase* m_mat0;
+    const MaterialBase* m_mat1;
+    float m_mix;
+};
+
+// Core/Geometry/Vertex.h
+struct Vertex
+{
+    Vertex() = default;
+    Vertex(float x, float y, float z) : pos(x, y, z) {}
+    Vertex(const Math::Vector3f& p, const Math::Vector3f& n, const Math::Vector3f& texcoord)
+      : pos(p), normal(n), uv(texcoord)
+    {
+    }
+    Math::Vector3f pos;
+    Math::Vector3f normal = Math::Vector3f::UnitZ();
+    Math::Vector3f uv;
+};
+
+enum class ShadingTypes
+{
+    Flat,
+    Smooth
+};
+
+class GeometryBase
+{
+public:
+    virtual ~GeometryBase() = default;
+    void SetMaterial(const MaterialBase* material) { m_material = material; }
+    const MaterialBase* GetMaterial() const { return m_material; }
+
+private:
+    const MaterialBase* m_material = nullptr;
+};
+
+// Core/Geometry/Triangle.h: three shared vertices and a shading type.
+class Triangle : public GeometryBase
+{
+public:
+    explicit Triangle(ShadingTypes shadingType = ShadingTypes::Flat) : m_shadingType(shadingType) {}
+    Triangle(const Vertex* v0, const Vertex* v1, const Vertex* v2, ShadingTypes shadingType = ShadingTypes::Flat)
+      : m_vertices{v0, v1, v2}, m_shadingType(shadingType)
+    {
+    }
+    void SetVertices(const Vertex* v0, const Vertex* v1, const Vertex* v2) { m_vertices = {v0, v1, v2}; }
+    const Vertex* GetVertex(int i) const { return m_vertices[i]; }
+    ShadingTypes GetShadingType() const { return m_shadingType; }
+
+private:
+    std::array<const Vertex*, 3> m_vertices{};
+    ShadingTypes m_shadingType;
+};
+
+// Core/Geometry/Mesh.h, built in memory: vertices and triangles over them.
+class Mesh
+{
+public:
+    Mesh() = default;
+    Mesh(const Mesh&) = delete;
+    Mesh& operator=(const Mesh&) = delete;
+    // Reserves room so that vertex pointers stay valid while vertices are added.
+    explicit Mesh(size_t maxVertices) { m_vertices.reserve(maxVertices); }
+    const Vertex* AddVertex(const Vertex& v)
+    {
+        if (m_vertices.size() == m_vertices.capacity()) throw std::length_error("Mesh: vertex capacity reached");
+        m_vertices.push_back(v);
+        return &m_vertices.back();
+    }
+    void AddTriangle(const Triangle& t) { m_triangles.push_back(t); }
+    const std::vector<Triangle>& GetTriangles() const { return m_triangles; }
+
+private:
+    std::vector<Vertex> m_vertices;
+    std::vector<Triangle> m_triangles;
+};
+
+// Core/Texture2D.h: row-major RGB float pixels.
+class Texture2D
+{
+public:
+    Texture2D() = default;
+    Texture2D(int width, int height) : m_width(width), m_height(height), m_pixels((size_t)width * height) {}
+    void Clear(const Color3f& color = Color3f::Zero()) { std::fill(m_pixels.begin(), m_pixels.end(), color); }
+    Color3f GetPixel(int i, int j) const { return m_pixels[(size_t)j * m_width + i]; }
+    void SetPixel(int i, int j, const Color3f& color) { m_pixels[(size_t)j * m_width + i] = color; }
+    int GetWidth() const { return m_width; }
+    int GetHeight() const { return m_height; }
+    bool IsValid() const { return m_width > 0 && m_height > 0; }
+    const float* GetRawDataPtr() const { return &m_pixels[0].x; }
+    float* GetRawDataPtr() { return &m_pixels[0].x; }
+
+private:
+    int m_width = 0, m_height = 0;
+    std::vector<Color3f> m_pixels;
+};
+
+static_assert(sizeof(Color3f) == 3 * sizeof(float), "Texture2D pixels must be packed RGB floats");
+
+// Core/Environment.h. The map is given as a texture (the reference loads it from a file).
+class Environment
+{
+public:
+    void SetBaseColor(const Color3f& baseColor) { m_baseColor = baseColor; }
+    void SetZAxisRotation(float angle) { m_ZAxisRotation = angle; }
+    void SetTexture(const Texture2D* texture) { m_texEnv = texture; }
+    void EnableImportanceSampling(bool isEnabled) { m_useImportanceSampling = isEnabled; }
+    const Texture2D* GetTexture() const { return m_texEnv; }
+    const Color3f& GetBaseColor() const { return m_baseColor; }
+    float GetZAxisRotation() const { return m_ZAxisRotation; }
+    bool UseEnvImportanceSampling() const { return m_texEnv && m_texEnv->IsValid() && m_useImportanceSampling; }
+
+private:
+    const Texture2D* m_texEnv = nullptr;
+    Color3f m_baseColor;
+    float m_ZAxisRotation = 0.0f;
+    bool m_useImportanceSampling = true;
+};
+
+// Core/Camera.h and Camera.cpp:14-50: the same set-up operations.
+class Camera
+{
+public:
+    Camera() = default;
+    Camera(const Math::Vector3f& pos, const Math::Vector3f& dir) : m_pos(pos) { SetViewDir(dir); }
+
+    void SetPosition(const Math::Vector3f& pos) { m_pos = pos; }
+    void SetFocalLength(float focalLength) { m_hPerf = m_sensorHeight / focalLength; }
+    void FocusTo(const Math::Vector3f& target) { m_focusDist = (target - m_pos).Length(); }
+    // 2 tan(fov / 2) with the double-precision tangent, rounded once
+    void SetFOV(float fov) { m_hPerf = (float)(2.0f * std::tan((double)(0.5f * fov))); }
+    void SetViewDir(const Math::Vector3f& dir)
+    {
+        m_forward = dir.Normalized();
+        m_right = Math::Cross(m_forward, Math::Vector3f::UnitZ());
+        m_up = Math::Cross(m_right, m_forward);
+    }
+    void LookAt(const Math::Vector3f& target) { SetViewDir(target - m_pos); }
+    float GetFocalLength() const { return m_sensorHeight / m_hPerf; }
+    float GetApeture() const { return m_apeture; }
+    void SetApeture(float apeture) { m_apeture = apeture; }
+    void SetFNumber(float fNumber) { m_apeture = GetFocalLength() / fNumber; }
+    Math::Vector3f Right() const { return m_right; }
+    Math::Vector3f Up() const { return m_up; }
+    Math::Vector3f Forward() const { return m_forward; }
+
+    // The 16 floats of pt_scene_desc.camera.
+    void CopyState(float* out) const
+    {
+        const Math::Vector3f v[4] = {m_pos, m_right, m_up, m_forward};
+        for (int i = 0; i < 4; ++i) {
+            out[3 * i] = v[i].x;
+            out[3 * i + 1] = v[i].y;
+            out[3 * i + 2] = v[i].z;
+        }
+        out[12] = m_focusDist;
+        out[13] = m_hPerf;
+        out[14] = m_apeture;
+        out[15] = 0.0f;
+    }
+
+private:
+    Math::Vector3f m_pos, m_right, m_up, m_forward;
+    float m_focusDist = 1.0f;
+    float m_sensorHeight = 24e-3f;
+    float m_hPerf = 24e-3f / 55e-3f;
+    float m_apeture = 25e-3f;
+};
+
+// Core/RenderSetting.h
+struct RenderSetting
+{
+    int outputWidth = 1280;
+    int outputHeight = 720;
+    int numSamplesPerPixel = 128;
+    int numSppForDenoising = 128;
+    int numMaxBounces = 16;
+    int tileWidth = 16;
+    int tileHeight = 16;
+    int numThreads = 0;
+};
+
+// Core/Scene.h: non-owning geometry and light lists, owned materials, camera and
+// environment, non-owning target textures.
+class Scene
+{
+public:
+    struct AOVType
+    {
+        enum Value
+        {
+            Rendered,
+            UV,
+            DenoisingAlbedo,
+            DenoisingNormal,
+            NumAOVTypes
+        };
+    };
+
+    void AppendGeometry(const GeometryBase* geometry) { m_geometries.emplace_back(geometry); }
+    void AppendMesh(const Mesh& mesh)
+    {
+        for (const auto& t : mesh.GetTriangles()) AppendGeometry(&t);
+    }
+    void AppendLightMesh(const Mesh& mesh)
+    {
+        for (const auto& t : mesh.GetTriangles()) {
+            AppendGeometry(&t);
+            AppendLight(&t);
+        }
+    }
+    void AppendLight(const GeometryBase* geometry) { m_lights.emplace_back(geometry); }
+    const std::vector<const GeometryBase*>& GetGeometries() const { return m_geometries; }
+    const std::vector<const GeometryBase*>& GetLights() const { return m_lights; }
+    const Camera* GetMainCamera() const { return m_mainCamera.get(); }
+    void RegisterMaterial(std::string_view materialName, std::unique_ptr<const MaterialBase> material)
+    {
+        m_materials[std::string(materialName)] = std::move(material);
+    }
+    const MaterialBase* GetMaterial(std::string_view materialName)
+    {
+        return m_materials[std::string(materialName)].get();
+    }
+    void SetMainCamera(std::unique_ptr<Camera> camera) { m_mainCamera = std::move(camera); }
+    const Environment& GetEnvironment() const { return m_environment; }
+    void SetEnvironment(const Environment& environment) { m_environment = environment; }
+    void SetTargetTexture(AOVType::Value aovType, Texture2D* targetTex) { m_targetTextures[aovType] = targetTex; }
+    Texture2D* GetTargetTexture(AOVType::Value aovType) const { return m_targetTextures[aovType]; }
+    const RenderSetting& GetRenderSetting() const { return m_renderSetting; }
+    // The reference reads these from a settings file (Scene::LoadRenderSetting).
+    void SetRenderSetting(const RenderSetting& renderSetting) { m_renderSetting = renderSetting; }
+
+private:
+    std::vector<const GeometryBase*> m_geometries;
+    std::vector<const GeometryBase*> m_lights;
+    std::unordered_map<std::string, std::unique_ptr<const MaterialBase>> m_materials;
+    Environment m_environment;
+    std::unique_ptr<Camera> m_mainCamera;
+    std::array<Texture2D*, AOVType::NumAOVTypes> m_targetTextures = {};
+    RenderSetting m_renderSetting;
+};
+
+enum class Integrator
+{
+    PathTracing,      // MIS (Core/Integrator/PathTracing.cpp)
+    SimplePathTracing // the integrator Petrichor::Render uses (Petrichor.cpp:58)
+};
+
+// Core/Petrichor.h: the render entry. Render blocks until the image is in the target
+// texture; the callback then receives the total time.
+class Petrichor
+{
+public:
+    explicit Petrichor(int device = 0) : m_device(device) {}
+    ~Petrichor()
+    {
+        if (m_ctx) pt_ctx_destroy(m_ctx);
+    }
+    Petrichor(const Petrichor&) = delete;
+    Petrichor& operator=(const Petrichor&) = delete;
+
+    void SetRenderCallback(const std::function<void(const RenderingResult&)>& onRenderingFinished)
+    {
+        m_onRenderingFinished = onRenderingFinished;
+    }
+    uint32_t GetNumRenderedTiles() const { return m_numRenderedTiles; }
+    uint32_t GetNumTiles() const { return m_numTiles; }
+
+    // Extensions of the GPU path: integrator choice and the sampler seed.
+    void SetIntegrator(Integrator integrator) { m_integrator = integrator; }
+    void SetSeed(uint32_t seed) { m_seed = seed; }
+    // Error text of the last failed Render (empty after a success).
+    const std::string& GetLastError() const { return m_error; }
+
+    // Returns false (with GetLastError) when the scene has no camera or target texture,
+    // holds a material or primitive the GPU path does not render, or the device fails.
+    bool Render(const Scene& scene)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        m_error.clear();
+        Texture2D* target = scene.GetTargetTexture(Scene::AOVType::Rendered);
+        if (!target || !target->IsValid()) return fail("Target texture has not been set.");
+        if (!scene.GetMainCamera()) return fail("The scene has no camera.");
+        const RenderSetting& rs = scene.GetRenderSetting();
+        // TileManager.cpp:8-43 counts (W / tw + 1) * (H / th + 1) tiles
+        m_numTiles = (uint32_t)((target->GetWidth() / rs.tileWidth + 1) * (target->GetHeight() / rs.tileHeight + 1));
+        m_numRenderedTiles = 0;
+
+        Flat f;
+        if (!flatten(scene, f)) return false;
+        if (!m_ctx && pt_ctx_create(m_device, &m_ctx) != PT_OK) return fail("device context creation failed");
+        pt_scene* s = nullptr;
+        pt_bvh* bvh = nullptr;
+        if (pt_scene_create(m_ctx, &f.desc, &s) != PT_OK) return fail(pt_last_error(m_ctx));
+        pt_bvh_desc bd{16, 4, 2, PT_BUILD_DEVICE}; // BinnedSAHBVH: 16 bins, leaves of up to 4
+        if (pt_bvh_build(s, &bd, &bvh) != PT_OK) {
+            pt_scene_destroy(s);
+            return fail(pt_last_error(m_ctx));
+        }
+        pt_render_desc rd{target->GetWidth(), target->GetHeight(), rs.numSamplesPerPixel, rs.numMaxBounces,
+                          m_integrator == Integrator::PathTracing ? PT_INTEGRATOR_MIS : PT_INTEGRATOR_SIMPLE, m_seed,
+                          0};
+        const int rc = pt_render(s, bvh, &rd, target->GetRawDataPtr(), 0, nullptr);
+        if (rc != PT_OK) fail(pt_last_error(m_ctx));
+        pt_bvh_destroy(bvh);
+        pt_scene_destroy(s);
+        if (rc != PT_OK) return false;
+        m_numRenderedTiles = m_numTiles;
+        if (m_onRenderingFinished) {
+            const float sec = std::chrono::duration<float>(std::chrono::steady_clock::now() - t0).count();
+            m_onRenderingFinished(RenderingResult{sec});
+        }
+        return true;
+    }
+
+private:
+    struct Flat {
+        std::vector<float> pos, nrm, matParam, env;
+        std::vector<int32_t> matId, smooth, matType, lights;
+        pt_scene_desc desc{};
+    };
+
+    bool fail(const std::string& msg)
+    {
+        m_error = msg;
+        return false;
+    }
+
+    static int32_t type_code(MaterialTypes t) { return (int32_t)t; }
+
+    // Material table entry (8 floats: colour, p3, p4, p5, importance sampling, 0).
+    int32_t material_index(const MaterialBase* m, std::unordered_map<const MaterialBase*, int32_t>& index, Flat& f)
+    {
+        auto it = index.find(m);
+        if (it != index.end()) return it->second;
+        float p[8] = {0, 0, 0, 0, 0, 0, m->IsImportanceSamplingEnabled() ? 1.0f : 0.0f, 0};
+        auto rgb = [&](const Color3f& c) {
+            p[0] = c.x;
+            p[1] = c.y;
+            p[2] = c.z;
+        };
+        switch (m->GetMaterialType()) {
+        case MaterialTypes::Lambert: rgb(static_cast<const Lambert*>(m)->GetKd()); break;
+        case MaterialTypes::Glossy:
+            rgb(static_cast<const GGX*>(m)->GetReflectance());
+            p[3] = static_cast<const GGX*>(m)->GetRoughness();
+            break;
+        case MaterialTypes::Glass:
+            rgb(static_cast<const Glass*>(m)->GetColor());
+            p[3] = static_cast<const Glass*>(m)->GetIor();
+            break;
+        case MaterialTypes::Emission: rgb(static_cast<const Emission*>(m)->GetLightColor()); break;
+        case MaterialTypes::Mix: {
+            const auto* mix = static_cast<const MixMaterial*>(m);
+            // sub-materials first, so that their indices are known
+            const int32_t a = material_index(mix->GetMaterial0(), index, f);
+            const int32_t b = material_index(mix->GetMaterial1(), index, f);
+            p[3] = mix->GetMix();
+            p[4] = (float)a;
+            p[5] = (float)b;
+            break;
+        }
+        }
+        const int32_t id = (int32_t)f.matType.size();
+        f.matType.push_back(type_code(m->GetMaterialType()));
+        f.matParam.insert(f.matParam.end(), p, p + 8);
+        index[m] = id;
+        return id;
+    }
+
+    bool flatten(const Scene& scene, Flat& f)
+    {
+        std::unordered_map<const MaterialBase*, int32_t> matIndex;
+        std::unordered_map<const GeometryBase*, int32_t> triIndex;
+        bool anySmooth = false;
+        for (const GeometryBase* g : scene.GetGeometries())
+            if (const auto* t = dynamic_cast<const Triangle*>(g))
+                anySmooth |= t->GetShadingType() == ShadingTypes::Smooth;
+        for (const GeometryBase* g : scene.GetGeometries()) {
+            const auto* t = dynamic_cast<const Triangle*>(g);
+            if (!t) return fail("the GPU path renders triangles only");
+            if (!t->GetMaterial()) return fail("a triangle has no material");
+            triIndex[g] = (int32_t)f.matId.size();
+            for (int k = 0; k < 3; ++k) {
+                const Vertex* v = t->GetVertex(k);
+                f.pos.insert(f.pos.end(), {v->pos.x, v->pos.y, v->pos.z});
+                if (anySmooth) f.nrm.insert(f.nrm.end(), {v->normal.x, v->normal.y, v->normal.z});
+            }
+            if (anySmooth) f.smooth.push_back(t->GetShadingType() == ShadingTypes::Smooth ? 1 : 0);
+            f.matId.push_back(material_index(t->GetMaterial(), matIndex, f));
+        }
+        for (const GeometryBase* l : scene.GetLights()) {
+            auto it = triIndex.find(l);
+            if (it == triIndex.end()) return fail("a light is not one of the scene's triangles");
+            f.lights.push_back(it->second);
+        }
+        if (f.matType.empty()) { // pt_scene_create wants one material even for an empty scene
+            f.matType.push_back(type_code(MaterialTypes::Lambert));
+            f.matParam.assign(8, 0.0f);
+        }
+        pt_scene_desc& d = f.desc;
+        d.num_triangles = (int32_t)f.matId.size();
+        d.positions = f.pos.data();
+        d.normals = anySmooth ? f.nrm.data() : nullptr;
+        d.material_ids = f.matId.data();
+        d.smooth = anySmooth ? f.smooth.data() : nullptr;
+        d.num_materials = (int32_t)f.matType.size();
+        d.material_types = f.matType.data();
+        d.material_params = f.matParam.data();
+        d.num_lights = (int32_t)f.lights.size();
+        d.light_triangles = f.lights.data();
+        scene.GetMainCamera()->CopyState(d.camera);
+        const Environment& env = scene.GetEnvironment();
+        const Texture2D* tex = env.GetTexture();
+        if (tex && tex->IsValid()) {
+            d.env_width = tex->GetWidth();
+            d.env_height = tex->GetHeight();
+            d.env_rgb = tex->GetRawDataPtr();
+        }
+        d.env_base_color[0] = env.GetBaseColor().x;
+        d.env_base_color[1] = env.GetBaseColor().y;
+        d.env_base_color[2] = env.GetBaseColor().z;
+        d.env_z_rotation = env.GetZAxisRotation();
+        d.env_importance_sampling = env.UseEnvImportanceSampling() ? 1 : 0;
+        return true;
+    }
+
+    int m_device;
+    pt_context* m_ctx = nullptr;
+    Integrator m_integrator = Integrator::SimplePathTracing;
+    uint32_t m_seed = 1;
+    std::string m_error;
+    std::atomic<uint32_t> m_numRenderedTiles = 0;
+    uint32_t m_numTiles = 0;
+    std::function<void(const RenderingResult&)> m_onRenderingFinished;
+};
+
+} // namespace Core
+} // namespace Petrichor

@@ -1,0 +1,96 @@
+// render_api.cpp — drives the C++ host API (include/petrichor/Petrichor.h) the way host
+// code written against LibPetrichor Core does: vertices, triangles and materials into a
+// Scene, a Camera set up with the reference's calls, a target Texture2D, then
+// Petrichor::Render. Used by the tests to compare with the Python binding.
+//
+//   render_api IN OUT [W H SPP DEPTH INTEGRATOR SEED]   (no size: camera state only)
+// IN:  int32 n, int32 n_lights, float pos[9 n], int32 material[n] (0 white, 1 red,
+//      2 green, 3 light), int32 lights[n_lights]
+// OUT: float camera[16], then W*H*3 floats of the image
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "petrichor/Petrichor.h"
+
+using namespace Petrichor;
+using namespace Petrichor::Core;
+using Math::Vector3f;
+
+int main(int argc, char** argv)
+{
+    if (argc != 3 && argc != 9) {
+        std::fprintf(stderr, "usage: render_api IN OUT [W H SPP DEPTH INTEGRATOR SEED]\n");
+        return 2;
+    }
+    FILE* in = std::fopen(argv[1], "rb");
+    if (!in) return 2;
+    int32_t n = 0, nl = 0;
+    bool ok = std::fread(&n, 4, 1, in) == 1 && std::fread(&nl, 4, 1, in) == 1;
+    std::vector<float> pos(9 * (size_t)n);
+    std::vector<int32_t> mat(n), lights(nl);
+    ok = ok && std::fread(pos.data(), 4, pos.size(), in) == pos.size();
+    ok = ok && std::fread(mat.data(), 4, mat.size(), in) == mat.size();
+    ok = ok && std::fread(lights.data(), 4, lights.size(), in) == lights.size();
+    std::fclose(in);
+    if (!ok) return 2;
+
+    Scene scene;
+    scene.RegisterMaterial("white", std::make_unique<Lambert>(Color3f(0.73f, 0.73f, 0.73f)));
+    scene.RegisterMaterial("red", std::make_unique<Lambert>(Color3f(0.65f, 0.05f, 0.05f)));
+    scene.RegisterMaterial("green", std::make_unique<Lambert>(Color3f(0.05f, 0.65f, 0.05f)));
+    scene.RegisterMaterial("light", std::make_unique<Emission>(Color3f(15.0f, 15.0f, 15.0f)));
+    const MaterialBase* mats[4] = {scene.GetMaterial("white"), scene.GetMaterial("red"), scene.GetMaterial("green"),
+                                   scene.GetMaterial("light")};
+    Mesh mesh(3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const float* p = &pos[9 * (size_t)i];
+        const Vertex* v0 = mesh.AddVertex(Vertex(p[0], p[1], p[2]));
+        const Vertex* v1 = mesh.AddVertex(Vertex(p[3], p[4], p[5]));
+        const Vertex* v2 = mesh.AddVertex(Vertex(p[6], p[7], p[8]));
+        Triangle t(v0, v1, v2);
+        t.SetMaterial(mats[mat[i]]);
+        mesh.AddTriangle(t);
+    }
+    scene.AppendMesh(mesh);
+    for (int32_t l : lights) scene.AppendLight(&mesh.GetTriangles()[l]);
+
+    // the Cornell box camera: Camera(pos, dir), SetFocalLength, FocusTo, SetFNumber
+    const Vector3f eye(0.5f, -1.35f, 0.5f);
+    auto camera = std::make_unique<Camera>(eye, Vector3f(0.5f, 0.5f, 0.5f) - eye);
+    camera->SetFocalLength(35e-3f);
+    camera->FocusTo(Vector3f(0.5f, 0.5f, 0.3f));
+    camera->SetFNumber(8.0f);
+    float state[16];
+    camera->CopyState(state);
+    scene.SetMainCamera(std::move(camera));
+
+    FILE* out = std::fopen(argv[2], "wb");
+    if (!out) return 2;
+    std::fwrite(state, 4, 16, out);
+    int rc = 0;
+    if (argc == 9) {
+        const int w = std::atoi(argv[3]), h = std::atoi(argv[4]);
+        Texture2D target(w, h);
+        scene.SetTargetTexture(Scene::AOVType::Rendered, &target);
+        RenderSetting rs;
+        rs.numSamplesPerPixel = std::atoi(argv[5]);
+        rs.numMaxBounces = std::atoi(argv[6]);
+        scene.SetRenderSetting(rs);
+        Core::Petrichor renderer;
+        renderer.SetIntegrator(std::atoi(argv[7]) == 0 ? Integrator::PathTracing : Integrator::SimplePathTracing);
+        renderer.SetSeed((uint32_t)std::atoi(argv[8]));
+        float seconds = -1.0f;
+        renderer.SetRenderCallback([&](const RenderingResult& r) { seconds = r.totalSec; });
+        if (!renderer.Render(scene)) {
+            std::fprintf(stderr, "render failed: %s\n", renderer.GetLastError().c_str());
+            rc = 1;
+        } else {
+            std::fwrite(target.GetRawDataPtr(), 4, 3 * (size_t)w * h, out);
+            std::printf("rendered %dx%d in %.3f s, %u of %u tiles\n", w, h, seconds, renderer.GetNumRenderedTiles(),
+                        renderer.GetNumTiles());
+        }
+    }
+    std::fclose(out);
+    return rc;
+}

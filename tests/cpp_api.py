@@ -1,0 +1,29 @@
+"""Runs the C++ host API driver (tests/cpp/render_api.cpp, built next to the library)."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "petrichor_amd", "_build", "render_api")
+
+
+def run(scene, size=None):
+    """Builds the scene through the C++ API; returns (camera state [16], image or None).
+    size: (width, height, spp, depth, integrator, seed) to render."""
+    n = len(scene["tri_mat"])
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([n, len(scene["light_tris"])], np.int32).tofile(f)
+            np.ascontiguousarray(scene["tri_pos"], np.float32).tofile(f)
+            np.ascontiguousarray(scene["tri_mat"], np.int32).tofile(f)
+            np.ascontiguousarray(scene["light_tris"], np.int32).tofile(f)
+        args = [DRIVER, fin, fout] + ([str(v) for v in size] if size else [])
+        r = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout
+        data = np.fromfile(fout, np.float32)
+    cam = data[:16]
+    img = data[16:].reshape(size[1], size[0], 3) if size else None
+    return cam, img
