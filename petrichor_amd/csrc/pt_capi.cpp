@@ -325,7 +325,6 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
     e.base = v3(d->env_base_color[0], d->env_base_color[1], d->env_base_color[2]);
     e.rot = d->env_z_rotation;
     e.use_is = d->env_importance_sampling;
-    e.pdf1d = e.cdf1d = nullptr;
     hipError_t err = upload(&s->mats_d, s->mats.data(), s->mats.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&s->lights_d, lights.data(), lights.size(), ctx->stream);
     if (err == hipSuccess && d->env_rgb) {
@@ -333,26 +332,6 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
         e.h = d->env_height;
         err = upload(&s->env_rgb_d, d->env_rgb, 3 * (size_t)e.w * e.h, ctx->stream);
         e.rgb = s->env_rgb_d;
-        if (err == hipSuccess && e.use_is) {
-            // Environment::PreCalcCumulativeDistTex (Environment.cpp:56-128): the 2D tables
-            // are never-loaded textures reading (1,0,1), so each row contributes 1.
-            std::vector<float> pdf1(e.h), cdf1(e.h);
-            float sum = 0.0f;
-            for (int j = 0; j < e.h; ++j) {
-                const float lum = 1.0f;
-                pdf1[j] = lum;
-                sum += lum;
-                cdf1[j] = sum;
-            }
-            for (int j = 0; j < e.h; ++j) {
-                pdf1[j] *= ((float)e.h / sum);
-                cdf1[j] /= sum;
-            }
-            err = upload(&s->pdf1d_d, pdf1.data(), pdf1.size(), ctx->stream);
-            if (err == hipSuccess) err = upload(&s->cdf1d_d, cdf1.data(), cdf1.size(), ctx->stream);
-            e.pdf1d = s->pdf1d_d;
-            e.cdf1d = s->cdf1d_d;
-        }
     }
     std::vector<int32_t> mat_sh(s->mats.size());
     for (size_t m = 0; m < s->mats.size(); ++m) { // MixMaterial::GetMaterial(0.0f)
@@ -366,10 +345,10 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
     if (err == hipSuccess) err = upload(&s->tri_nrm_d, s->tri_nrm.data(), s->tri_nrm.size(), ctx->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
     s->device_bytes = (int64_t)(s->mats.size() * (sizeof(MaterialDev) + 4) + lights.size() * sizeof(float4)) +
-                      (d->env_rgb ? 3 * (int64_t)e.w * e.h * 4 + 8 * (int64_t)e.h : 0) +
+                      (d->env_rgb ? 3 * (int64_t)e.w * e.h * 4 : 0) +
                       (int64_t)nt * 40 + (int64_t)(s->tri_smooth.size() * 4 + s->tri_nrm.size() * 4);
     if (err != hipSuccess) {
-        free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->pdf1d_d, s->cdf1d_d, s->mat_shading_of_d, s->tri_pos_d,
+        free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->mat_shading_of_d, s->tri_pos_d,
                   s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d});
         delete s;
         return fail(ctx, err == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
@@ -382,7 +361,7 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
 int pt_scene_destroy(pt_scene* s)
 {
     if (!s) return PT_ERR_ARG;
-    free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->pdf1d_d, s->cdf1d_d, s->mat_shading_of_d, s->tri_pos_d,
+    free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->mat_shading_of_d, s->tri_pos_d,
               s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d});
     delete s;
     return PT_OK;

@@ -436,8 +436,6 @@ struct EnvDev {
     V3 base;
     float rot;
     int use_is;
-    const float* pdf1d; // h entries
-    const float* cdf1d; // h entries
 };
 
 PT_HD bool env_valid(const EnvDev& e) { return e.rgb != nullptr && e.w > 0 && e.h > 0; }
@@ -486,12 +484,23 @@ PT_HD float invalid_tex_bilinear_x(float x, float y)
     return wy1 * cm0 + wy0 * cm1;
 }
 
-// Environment::GetColor (Environment.cpp:29-48)
-PT_HD V3 env_color(const EnvDev& e, V3 dir)
+// Polar angles of a direction as Environment::GetColor and GetImportanceSamplingPDF
+// compute them (Environment.cpp:31-32, 225-226): double acos / atan2, rounded to float.
+struct EnvAngles {
+    float theta, phi;
+};
+PT_HD EnvAngles env_angles(V3 dir)
 {
-    if (!env_valid(e)) return e.base;
-    const float theta = (float)pt_acos((double)dir.z);
-    const float phi = (float)pt_atan2((double)dir.y, (double)dir.x);
+    EnvAngles a;
+    a.theta = (float)pt_acos((double)dir.z);
+    a.phi = (float)pt_atan2((double)dir.y, (double)dir.x);
+    return a;
+}
+
+// Environment::GetColor (Environment.cpp:29-48) for a valid map, from the angles
+PT_HD V3 env_color_at(const EnvDev& e, EnvAngles a)
+{
+    const float theta = a.theta, phi = a.phi;
     float u = 1.0f - (phi - e.rot) * 0.5f * PT_INV_PI;
     float v = theta * PT_INV_PI;
     u = fmodp(u, 1.0f);
@@ -501,7 +510,21 @@ PT_HD V3 env_color(const EnvDev& e, V3 dir)
     return e.base * tex_bilinear(e.rgb, e.w, e.h, u * e.w, v * e.h);
 }
 
-PT_HD float env_pdf1(const EnvDev& e, int i) { return e.pdf1d[i < e.h - 1 ? i : e.h - 1]; }
+PT_HD V3 env_color(const EnvDev& e, V3 dir)
+{
+    if (!env_valid(e)) return e.base;
+    return env_color_at(e, env_angles(dir));
+}
+
+// The 1D tables of Environment::PreCalcCumulativeDistTex (Environment.cpp:56-128). The
+// reference builds them from its 2D tables, which are never-loaded textures reading
+// (1, 0, 1), so every row weighs 1 whatever the map: m_pdf1D[j] = H / H = 1 and
+// m_cdf1D[j] = (j + 1) / H (a running float sum of ones, exact below 2^24, then one
+// correctly rounded division). They are evaluated here with the same rounding instead of
+// stored, which keeps the bisection of env_sample free of dependent loads. The oracle
+// builds the tables the reference's way, and the render parity tests compare.
+PT_HD float env_pdf1(const EnvDev&, int) { return 1.0f; }
+PT_HD float env_cdf1(const EnvDev& e, int j) { return (float)(j + 1) / (float)e.h; }
 
 // Environment::ImportanceSampling (Environment.cpp:130-220). m_cdf2D / m_pdf2D are
 // never-loaded textures, so their fetches go through invalid_tex_bilinear_x.
@@ -518,7 +541,7 @@ PT_HD V3 env_sample(const EnvDev& e, Samplers& s, float* pdf_xy)
         const float y = v * e.h;
         const int fy = (int)y;
         const int cy = fy + 1 < e.h - 1 ? fy + 1 : e.h - 1;
-        const float c = lerpf(e.cdf1d[fy], e.cdf1d[cy], y - fy);
+        const float c = lerpf(env_cdf1(e, fy), env_cdf1(e, cy), y - fy);
         const float diff = c - r0;
         if (v1 - v0 <= texel_h) {
             v0 = v1 = v;
@@ -568,12 +591,11 @@ PT_HD V3 env_sample(const EnvDev& e, Samplers& s, float* pdf_xy)
     return cx * bu + cy * bv + cz * bw;
 }
 
-// Environment::GetImportanceSamplingPDF (Environment.cpp:222-244)
-PT_HD float env_pdf(const EnvDev& e, V3 dir)
+// Environment::GetImportanceSamplingPDF (Environment.cpp:222-244) with importance
+// sampling on, from the angles
+PT_HD float env_pdf_at(const EnvDev& e, EnvAngles a)
 {
-    if (!env_use_is(e)) return 1.0f;
-    const float theta = (float)pt_acos((double)dir.z);
-    const float phi = (float)pt_atan2((double)dir.y, (double)dir.x);
+    const float theta = a.theta, phi = a.phi;
     const float u0 = 1.0f - (phi - e.rot) * 0.5f * PT_INV_PI;
     const float v0 = theta * PT_INV_PI;
     const float y = v0 * e.h;
@@ -583,6 +605,12 @@ PT_HD float env_pdf(const EnvDev& e, V3 dir)
     const float uu = fmodp(u0, 1.0f), vv = fmodp(v0, 1.0f);
     const float px = invalid_tex_bilinear_x(uu * e.w, vv * e.h);
     return py * px;
+}
+
+PT_HD float env_pdf(const EnvDev& e, V3 dir)
+{
+    if (!env_use_is(e)) return 1.0f;
+    return env_pdf_at(e, env_angles(dir));
 }
 
 // ---- BVH slab test (Core/Accel/BinnedSAHBVH.h:95-170) -------------------------------

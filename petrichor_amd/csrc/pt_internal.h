@@ -29,8 +29,6 @@ struct pt_scene {
     pt::CameraDev cam;
     pt::EnvDev env;
     float* env_rgb_d = nullptr;
-    float* pdf1d_d = nullptr;
-    float* cdf1d_d = nullptr;
     // triangle data resident for the device BVH builder
     float* tri_pos_d = nullptr;        // 9 floats per triangle
     int32_t* tri_mat_d = nullptr;

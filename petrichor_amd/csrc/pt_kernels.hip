@@ -107,6 +107,25 @@ __device__ __forceinline__ int queue_slot(int* counter, bool want)
     return base + rank;
 }
 
+// Appends to four queues at once. Lanes 0-3 each own one queue's wave-level atomic, so
+// the four atomics leave as one instruction and the wave waits for one round trip to
+// L2 instead of four. Every lane of the wave must call it.
+__device__ __forceinline__ void queue_slots4(int* const counter[4], const bool want[4], int slot[4])
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lane_lt = (1ull << lane) - 1ull;
+    unsigned long long m[4];
+    for (int q = 0; q < 4; ++q) m[q] = __ballot(want[q]);
+    int* const c = lane == 0 ? counter[0] : (lane == 1 ? counter[1] : (lane == 2 ? counter[2] : counter[3]));
+    const unsigned long long mine = lane == 0 ? m[0] : (lane == 1 ? m[1] : (lane == 2 ? m[2] : m[3]));
+    int base = 0;
+    if (lane < 4 && mine != 0ull) base = atomicAdd(c, __popcll(mine));
+    for (int q = 0; q < 4; ++q) {
+        const int b = __shfl(base, q, 64);
+        slot[q] = want[q] ? b + __popcll(m[q] & lane_lt) : -1;
+    }
+}
+
 __device__ __forceinline__ void push_ray(const Queue& q, int slot, int path, V3 o, float dmin, V3 d, float dmax)
 {
     q.rays[2 * slot] = make_float4(o.x, o.y, o.z, dmin);
@@ -292,16 +311,18 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
                         }
                     }
                 } else {
+                    // the environment's colour and pdf share the direction's angles
                     if (env_use_is(sc.env)) {
+                        const EnvAngles ang = env_angles(ray.dir);
                         float mis = 0.0;
                         if (ray.dir.x != 0 && ray.dir.y != 0 && sin_n > 0) {
-                            const float pe = env_pdf(sc.env, ray.dir) / (2.0f * PT_PI * PT_PI * sin_n);
+                            const float pe = env_pdf_at(sc.env, ang) / (2.0f * PT_PI * PT_PI * sin_n);
                             const float pb = pdf_bsdf;
                             mis = pb * pb / (pe * pe + pb * pb);
                         } else {
                             mis = 1.0f;
                         }
-                        color = color + mis * ray.throughput * env_color(sc.env, ray.dir);
+                        color = color + mis * ray.throughput * env_color_at(sc.env, ang);
                     } else {
                         color = color + ray.throughput * env_color(sc.env, ray.dir);
                     }
@@ -483,14 +504,14 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc1, JobView job1, cons
                 shade_one<INTEGRATOR>(sc1, job1, A, p, o);
             }
         }
-        const int e = queue_slot(A.ext.count, o.want_ext);
-        if (o.want_ext) push_ray(A.ext, e, p, o.ext_o, o.ext_dmin, o.ext_d, o.ext_dmax);
-        const int a = queue_slot(A.sh_any.count, o.want_any);
-        if (o.want_any) push_ray(A.sh_any, a, p, o.sh_o, 0.0f, o.sh_d, FLT_MAX);
-        const int c = queue_slot(A.sh_closest.count, o.want_closest);
-        if (o.want_closest) push_ray(A.sh_closest, c, p, o.sh_o, 0.0f, o.sh_d, FLT_MAX);
-        const int s = queue_slot(A.active_out_count, o.keep);
-        if (o.keep) A.active_out[s] = p;
+        int* const counters[4] = {A.ext.count, A.sh_any.count, A.sh_closest.count, A.active_out_count};
+        const bool want[4] = {o.want_ext, o.want_any, o.want_closest, o.keep};
+        int slot[4];
+        queue_slots4(counters, want, slot);
+        if (o.want_ext) push_ray(A.ext, slot[0], p, o.ext_o, o.ext_dmin, o.ext_d, o.ext_dmax);
+        if (o.want_any) push_ray(A.sh_any, slot[1], p, o.sh_o, 0.0f, o.sh_d, FLT_MAX);
+        if (o.want_closest) push_ray(A.sh_closest, slot[2], p, o.sh_o, 0.0f, o.sh_d, FLT_MAX);
+        if (o.keep) A.active_out[slot[3]] = p;
     }
 }
 
