@@ -399,10 +399,12 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1,
     BvhView b = b1; // per lane in multi-job launches
     uint64_t nv = 0, nt = 0;
     for (;;) {
-        // refill lanes without a ray: first from the wave's pool, then from a new chunk
+        // refill lanes without a ray, first from the wave's pool, then from a new chunk,
+        // once a quarter of the wave is idle (the refill's bookkeeping then runs every few
+        // steps instead of whenever one ray ends)
         const bool need = item < 0;
         const unsigned long long need_mask = __ballot(need);
-        if (need_mask != 0ull && !exhausted) {
+        if (__popcll(need_mask) >= 16 && !exhausted) {
             const int n_need = __popcll(need_mask);
             const int rank = __popcll(need_mask & lane_lt);
             int avail = pool_end - pool_next;
