@@ -126,6 +126,21 @@ __device__ __forceinline__ void queue_slots4(int* const counter[4], const bool w
     }
 }
 
+// Adds one per lane with `want` to that lane's counter: one atomic per distinct counter in
+// the wave (the lanes of a wave usually share one job). Every lane of the wave must call it.
+__device__ __forceinline__ void wave_count(int* counter, bool want)
+{
+    const int lane = threadIdx.x & 63;
+    unsigned long long left = __ballot(want);
+    while (left) {
+        const int leader = __ffsll((long long)left) - 1;
+        const unsigned long long lp = (unsigned long long)__shfl((long long)(uintptr_t)counter, leader, 64);
+        const unsigned long long same = __ballot(want && (unsigned long long)(uintptr_t)counter == lp);
+        if (lane == leader) atomicAdd(counter, __popcll(same));
+        left &= ~same;
+    }
+}
+
 __device__ __forceinline__ void push_ray(const Queue& q, int slot, int path, V3 o, float dmin, V3 d, float dmax)
 {
     q.rays[2 * slot] = make_float4(o.x, o.y, o.z, dmin);
@@ -183,6 +198,7 @@ __global__ void __launch_bounds__(256) k_paths_init(SceneView sc, JobView job, P
 // What shading one path asks of the queues.
 struct ShadeOut {
     bool keep, want_ext, want_any, want_closest;
+    int* done; // the job's finished-pixel counter when this path finished its pixel
     V3 ext_o, ext_d, sh_o, sh_d;
     float ext_dmin, ext_dmax;
 };
@@ -197,6 +213,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
     const float* sh_t = A.sh_t;
     const int* sh_k = A.sh_k;
     bool keep = false, want_ext = false, want_any = false, want_closest = false;
+    int* done = nullptr;
     V3 ext_o = v3(0.0f, 0.0f, 0.0f), ext_d = ext_o, sh_o = ext_o, sh_d = ext_o;
     float ext_dmin = 0.0f, ext_dmax = FLT_MAX;
     {
@@ -450,7 +467,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
                 o[0] = avg.x;
                 o[1] = avg.y;
                 o[2] = avg.z;
-                atomicAdd(job.done, 1);
+                done = job.done;
                 ps.meta[p] = make_int4(spp, 0, kPhDone, 0); // read by the progressive preview
                 phase = kPhDone;
             }
@@ -469,6 +486,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
         }
     }
     out.keep = keep;
+    out.done = done;
     out.want_ext = want_ext;
     out.want_any = want_any;
     out.want_closest = want_closest;
@@ -484,7 +502,8 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
 // Per-wave ballots compact the new rays into the queues and the surviving paths into
 // the next live list.
 template <int INTEGRATOR, bool MULTI>
-__global__ void __launch_bounds__(256) k_shade(SceneView sc1, JobView job1, const JobDev* jobs, ShadeArgs A)
+__device__ __forceinline__ void shade_live_paths(const SceneView& sc1, const JobView& job1, const JobDev* jobs,
+                                                 const ShadeArgs& A)
 {
     const int n = *A.active_count;
     const int stride = gridDim.x * blockDim.x;
@@ -494,6 +513,7 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc1, JobView job1, cons
         const int p = in_range ? A.active_in[i] : 0;
         ShadeOut o;
         o.keep = o.want_ext = o.want_any = o.want_closest = false;
+        o.done = nullptr;
         if (in_range) {
             if (MULTI) {
                 // the integrator is a per-job setting here
@@ -512,7 +532,20 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc1, JobView job1, cons
         if (o.want_any) push_ray(A.sh_any, slot[1], p, o.sh_o, 0.0f, o.sh_d, FLT_MAX);
         if (o.want_closest) push_ray(A.sh_closest, slot[2], p, o.sh_o, 0.0f, o.sh_d, FLT_MAX);
         if (o.keep) A.active_out[slot[3]] = p;
+        wave_count(o.done, o.done != nullptr);
     }
+}
+
+template <int INTEGRATOR>
+__global__ void __launch_bounds__(256) k_shade(SceneView sc, JobView job, ShadeArgs A)
+{
+    shade_live_paths<INTEGRATOR, false>(sc, job, nullptr, A);
+}
+
+// The multi-job form: the integrator is a per-job setting.
+__global__ void __launch_bounds__(256) k_shade_multi(const JobDev* jobs, ShadeArgs A)
+{
+    shade_live_paths<0, true>(SceneView{}, JobView{}, jobs, A);
 }
 
 __global__ void k_zero_counters(int* c, int n)
@@ -602,9 +635,9 @@ hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job,
 {
     const int bpc = tune.shade_blocks_per_cu < 1 ? 1 : (tune.shade_blocks_per_cu > 32 ? 32 : tune.shade_blocks_per_cu);
     const dim3 grid((unsigned)(cu_count() * bpc)), block(256);
-    if (jobs) hipLaunchKernelGGL((k_shade<0, true>), grid, block, 0, st, sc, job, jobs, a);
-    else if (integrator == 0) hipLaunchKernelGGL((k_shade<0, false>), grid, block, 0, st, sc, job, jobs, a);
-    else hipLaunchKernelGGL((k_shade<1, false>), grid, block, 0, st, sc, job, jobs, a);
+    if (jobs) hipLaunchKernelGGL(k_shade_multi, grid, block, 0, st, jobs, a);
+    else if (integrator == 0) hipLaunchKernelGGL((k_shade<0>), grid, block, 0, st, sc, job, a);
+    else hipLaunchKernelGGL((k_shade<1>), grid, block, 0, st, sc, job, a);
     return hipGetLastError();
 }
 

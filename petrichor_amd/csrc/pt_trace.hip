@@ -22,7 +22,7 @@
 // Execution: persistent waves. A wave takes rays in chunks of 64 from a global cursor
 // and refills each lane as soon as its ray finishes, so lanes do not wait for the
 // longest ray of the wave. A step handles one inner node or one triangle. Each lane keeps
-// its traversal stack in LDS (the first PT_LDS_STACK entries) with deeper entries in
+// its traversal stack in LDS (the first lds_stack_entries entries) with deeper entries in
 // private memory; the LDS size per block sets the blocks per CU.
 #include <hip/hip_runtime.h>
 
@@ -31,7 +31,14 @@
 
 namespace pt {
 
-#define PT_LDS_STACK 10 // 20 KB per 256-thread block: 8 blocks (32 waves) per CU
+// LDS stack entries per lane. Binary: 10 (20 KB per 256-thread block, 8 blocks or 32
+// waves per CU). 8-wide: 16 (32 KB, 5 blocks per CU, which its registers allow anyway),
+// as its stacks grow by up to 7 entries per node.
+template <bool WIDE>
+constexpr int lds_stack_entries()
+{
+    return WIDE ? 16 : 10;
+}
 #define PT_TRACE_BLOCK 256
 #define PT_STACK8 192 // 8-wide traversal: at most 7 pending children per level
 
@@ -58,26 +65,30 @@ struct TraceState {
     uint32_t leaf_k, leaf_end; // triangles of the current leaf still to test
 };
 
+template <bool WIDE>
 __device__ __forceinline__ void stack_push(uint2* lds, StackEntry* spill, int sp, uint32_t ref, float d)
 {
-    if (sp < PT_LDS_STACK) {
+    constexpr int NL = lds_stack_entries<WIDE>();
+    if (sp < NL) {
         lds[sp * PT_TRACE_BLOCK] = make_uint2(ref, __float_as_uint(d));
     } else {
-        spill[sp - PT_LDS_STACK].ref = ref;
-        spill[sp - PT_LDS_STACK].d = d;
+        spill[sp - NL].ref = ref;
+        spill[sp - NL].d = d;
     }
 }
 
+template <bool WIDE>
 __device__ __forceinline__ void stack_pop(const uint2* lds, const StackEntry* spill, int sp, uint32_t* ref,
                                           float* d)
 {
-    if (sp < PT_LDS_STACK) {
+    constexpr int NL = lds_stack_entries<WIDE>();
+    if (sp < NL) {
         const uint2 e = lds[sp * PT_TRACE_BLOCK];
         *ref = e.x;
         *d = __uint_as_float(e.y);
     } else {
-        *ref = spill[sp - PT_LDS_STACK].ref;
-        *d = spill[sp - PT_LDS_STACK].d;
+        *ref = spill[sp - NL].ref;
+        *d = spill[sp - NL].d;
     }
 }
 
@@ -199,14 +210,19 @@ __device__ __forceinline__ bool leaf_or_pop(const BvhView& b, const TraceRay& r,
                                             StackEntry* spill, bool next)
 {
     if (s.leaf_k < s.leaf_end) {
-        if (tri_test<STATS, WIDE>(b, r, s, s.leaf_k++)) return true;
-        if (s.leaf_k < s.leaf_end) return false;
+        if (WIDE) { // a wide node's work per step is larger: whole leaves keep steps alike
+            for (; s.leaf_k < s.leaf_end; ++s.leaf_k)
+                if (tri_test<STATS, WIDE>(b, r, s, s.leaf_k)) return true;
+        } else {
+            if (tri_test<STATS, WIDE>(b, r, s, s.leaf_k++)) return true;
+            if (s.leaf_k < s.leaf_end) return false;
+        }
         next = false;
     }
     if (!next) {
         if (s.sp == 0) return true;
         --s.sp;
-        stack_pop(lds, spill, s.sp, &s.cur, &s.cur_d);
+        stack_pop<WIDE>(lds, spill, s.sp, &s.cur, &s.cur_d);
     }
     return false;
 }
@@ -245,7 +261,7 @@ __device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, 
         const float d1 = left_first ? tl : tr_, d2 = left_first ? tr_ : tl;
         if (h1) {
             if (h2) {
-                stack_push(lds, spill, s.sp, e2, d2);
+                stack_push<false>(lds, spill, s.sp, e2, d2);
                 ++s.sp;
             }
             s.cur = e1;
@@ -347,7 +363,7 @@ __device__ __forceinline__ bool trace8_step(const BvhView& b, const TraceRay& r,
 #pragma unroll
         for (int j = 7; j >= 1; --j) {
             if (j < hits) {
-                stack_push(lds, spill, s.sp, ref[j], d[j]);
+                stack_push<true>(lds, spill, s.sp, ref[j], d[j]);
                 ++s.sp;
             }
         }
@@ -386,9 +402,9 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1,
     }
     if (job.queries && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(job.queries, (unsigned long long)total);
     if (total == 0) return;
-    __shared__ uint2 lds_stack[PT_LDS_STACK * PT_TRACE_BLOCK];
+    __shared__ uint2 lds_stack[lds_stack_entries<WIDE>() * PT_TRACE_BLOCK];
     uint2* lds = lds_stack + threadIdx.x;
-    StackEntry spill[(WIDE ? PT_STACK8 : PT_MAX_STACK) - PT_LDS_STACK];
+    StackEntry spill[(WIDE ? PT_STACK8 : PT_MAX_STACK) - lds_stack_entries<WIDE>()];
     const int lane = threadIdx.x & 63;
     const unsigned long long lane_lt = (1ull << lane) - 1ull;
     int pool_next = 0, pool_end = 0; // wave-uniform: this wave's share of rays
@@ -475,15 +491,16 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
     TraceJob job = job_in;
     // a wave refills at most 64 lanes from one chunk, so chunks hold at least 64 rays
     job.chunk = tune.trace_chunk < 64 ? 64 : tune.trace_chunk;
+    const bool multi = job.views != nullptr;
+    const bool wide = multi ? job.wide != 0 : b.nodes8 != nullptr;
     // persistent grid: blocks of 4 waves; the LDS stacks of a block bound the blocks per CU
     // (160 KB of LDS, at most 8 waves per SIMD)
-    constexpr int lds_block = PT_LDS_STACK * PT_TRACE_BLOCK * (int)sizeof(uint2);
-    constexpr int max_bpc = (160 * 1024) / lds_block < 8 ? (160 * 1024) / lds_block : 8;
+    const int lds_block = (wide ? lds_stack_entries<true>() : lds_stack_entries<false>()) * PT_TRACE_BLOCK *
+                          (int)sizeof(uint2);
+    const int max_bpc = (160 * 1024) / lds_block < 8 ? (160 * 1024) / lds_block : 8;
     const int bpc = tune.trace_blocks_per_cu < 1 || tune.trace_blocks_per_cu > max_bpc ? max_bpc
                                                                                      : tune.trace_blocks_per_cu;
     const dim3 grid((unsigned)(cu_count() * bpc)), block(PT_TRACE_BLOCK);
-    const bool multi = job.views != nullptr;
-    const bool wide = multi ? job.wide != 0 : b.nodes8 != nullptr;
 #define PT_LAUNCH_TRACE(S, M, W) \
     hipLaunchKernelGGL((k_trace_persistent<S, M, W>), grid, block, 0, st, b, job, cursor, stats)
     if (stats) {
