@@ -84,7 +84,7 @@ int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
     gpu_bvh_free(&r);
     pt_bvh* b = new pt_bvh();
     b->scene = s;
-    if (e == hipSuccess) e = hipMalloc((void**)&b->cursor_d, 4 * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&b->cursor_d, (PT_CURSOR_INTS + 1) * sizeof(int));
     if (e != hipSuccess) {
         gpu_packed_free(&pk);
         if (b->cursor_d) (void)hipFree(b->cursor_d);
@@ -429,7 +429,7 @@ int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     if (err == hipSuccess) err = upload(&b->nodes_d, nodes.data(), nodes.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&b->big_leaves_d, big.data(), big.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&b->inner_range_d, ranges.data(), ranges.size(), ctx->stream);
-    if (err == hipSuccess) err = hipMalloc((void**)&b->cursor_d, 4 * sizeof(int));
+    if (err == hipSuccess) err = hipMalloc((void**)&b->cursor_d, (PT_CURSOR_INTS + 1) * sizeof(int));
     if (err == hipSuccess) err = upload(&b->tris_d, tris.data(), tris.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&b->prim_ids_d, b->host.order.data(), nt, ctx->stream);
     if (err == hipSuccess) err = upload(&b->tri_mat_d, mat.data(), nt, ctx->stream);
@@ -547,20 +547,21 @@ TraceSegment api_segment(const int* count_d, const float* rays, bool any, float*
     return g;
 }
 
-// cursor_d[0] is the work cursor, cursor_d[1] the ray count of the launch.
+// cursor_d holds the work cursors (PT_CURSOR_STRIDE apart), cursor_d[PT_CURSOR_INTS] the ray
+// count of the launch.
 int api_trace(pt_bvh* b, int64_t n, const float* rays, bool any, float* t_out, int32_t* k_out,
               unsigned long long* stats, hipStream_t st)
 {
     pt_context* ctx = b->scene->ctx;
     TraceJob job;
-    job.seg[0] = api_segment(b->cursor_d + 1, rays, any, t_out, k_out);
+    job.seg[0] = api_segment(b->cursor_d + PT_CURSOR_INTS, rays, any, t_out, k_out);
     job.num_seg = 1;
     job.views = nullptr;
     job.job_of_path = nullptr;
     job.queries = nullptr;
     const int n32 = (int)n;
-    PT_HIP(ctx, launch_zero(b->cursor_d, 1, st));
-    PT_HIP(ctx, hipMemcpyAsync(b->cursor_d + 1, &n32, sizeof(int), hipMemcpyHostToDevice, st));
+    PT_HIP(ctx, launch_zero(b->cursor_d, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
+    PT_HIP(ctx, hipMemcpyAsync(b->cursor_d + PT_CURSOR_INTS, &n32, sizeof(int), hipMemcpyHostToDevice, st));
     PT_HIP(ctx, launch_trace_job(b->view, job, b->cursor_d, stats, st));
     return PT_OK;
 }

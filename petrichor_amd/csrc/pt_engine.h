@@ -63,7 +63,9 @@ struct pt_engine {
     float *hit_t = nullptr, *sh_t = nullptr;
     int *hit_k = nullptr, *sh_k = nullptr;
     int* act[2] = {nullptr, nullptr};
-    int* counters = nullptr; // [4*set + kind] kind 0 ext, 1 any, 2 closest, 3 live; [8] trace cursor
+    // [4*set + kind] kind 0 ext, 1 any, 2 closest, 3 live; the trace cursors from
+    // PT_CURSOR_STRIDE on
+    int* counters = nullptr;
     float4* q_rays[3] = {nullptr, nullptr, nullptr};
     int32_t* q_path[3] = {nullptr, nullptr, nullptr};
     pt::JobDev* jobs_d = nullptr;
@@ -141,7 +143,7 @@ struct pt_engine {
         ALLOC(sh_k, P);
         ALLOC(act[0], P);
         ALLOC(act[1], P);
-        ALLOC(counters, 12);
+        ALLOC(counters, PT_CURSOR_STRIDE + PT_CURSOR_INTS);
         for (int k = 0; k < 3; ++k) {
             ALLOC(q_rays[k], 2 * P);
             ALLOC(q_path[k], P);
@@ -156,7 +158,8 @@ struct pt_engine {
         slot_used.assign(jobs, 0);
         free_ranges.clear();
         free_ranges[0] = cap;
-        PT_HIP(ctx, pt::launch_zero(counters, 12, st));
+        PT_HIP(ctx, pt::launch_zero(counters, 8, st));
+        PT_HIP(ctx, pt::launch_zero(counters + PT_CURSOR_STRIDE, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
         PT_HIP(ctx, hipMemsetAsync(queries_d, 0, sizeof(unsigned long long), st));
         PT_HIP(ctx, hipMemsetAsync(done_d, 0, sizeof(int) * (size_t)jobs, st));
         PT_HIP(ctx, hipStreamSynchronize(st));
@@ -309,8 +312,8 @@ struct pt_engine {
         tj.wide = multi && width == 8;
         tj.job_of_path = multi ? ps.job : nullptr;
         tj.queries = queries_d;
-        PT_HIP(ctx, pt::launch_zero(counters + 8, 1, st));
-        PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + 8, nullptr, st, tune));
+        PT_HIP(ctx, pt::launch_zero(counters + PT_CURSOR_STRIDE, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
+        PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + PT_CURSOR_STRIDE, nullptr, st, tune));
         PT_HIP(ctx, pt::launch_zero(counters + 4 * w, 4, st));
         pt::ShadeArgs a;
         a.ps = ps;
@@ -399,9 +402,9 @@ struct pt_engine {
             tj.wide = multi && width == 8;
             tj.job_of_path = multi ? ps.job : nullptr;
             tj.queries = queries_d;
-            PT_HIP(ctx, pt::launch_zero(counters + 8, 1, st));
+            PT_HIP(ctx, pt::launch_zero(counters + PT_CURSOR_STRIDE, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
             PT_HIP(ctx, hipEventRecord(ev[2 * k], st));
-            PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + 8, stats_d ? stats_d + 2 * k : nullptr, st, tune));
+            PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + PT_CURSOR_STRIDE, stats_d ? stats_d + 2 * k : nullptr, st, tune));
             PT_HIP(ctx, hipEventRecord(ev[2 * k + 1], st));
         }
         PT_HIP(ctx, pt::launch_zero(counters + 4 * w, 4, st));
@@ -468,8 +471,8 @@ struct pt_engine {
             tj.wide = multi && width == 8;
             tj.job_of_path = multi ? ps.job : nullptr;
             tj.queries = queries_d;
-            PT_HIP(ctx, pt::launch_zero(counters + 8, 1, st));
-            PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + 8, nullptr, st, tune));
+            PT_HIP(ctx, pt::launch_zero(counters + PT_CURSOR_STRIDE, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
+            PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + PT_CURSOR_STRIDE, nullptr, st, tune));
         }
         PT_HIP(ctx, pt::launch_zero(counters + 4 * w, 4, st));
         pt::ShadeArgs a;

@@ -109,13 +109,26 @@ struct LaunchTuning {
     int trace_chunk = 64;        // >= 64, the wave width
     int shade_blocks_per_cu = 8;
 };
+// The rays of a trace launch are handed out in PT_TRACE_PARTS parts, one per XCD: part x
+// holds the x-th eighth of every segment, and the blocks that the hardware places on XCD
+// x (workgroups go to the XCDs in turn) take their chunks from part x first, then from the
+// other parts. A part's rays are neighbours in the queues (the same job, nearby pixels),
+// so each XCD's L2 holds the BVH nodes of fewer jobs and screen regions, and the chunk
+// atomics, which every wave issues on the same cursor otherwise, are spread over eight.
+#define PT_TRACE_PARTS 8
+// A part's cursor is cursor[x * PT_CURSOR_STRIDE]: each on a 256-byte line of its own, so
+// that the cursors' atomics are spread over memory channels.
+#define PT_CURSOR_STRIDE 64
+#define PT_CURSOR_INTS (PT_TRACE_PARTS * PT_CURSOR_STRIDE)
+
 // Persistent trace launch over every segment of a job (rays handed out to waves in
-// chunks from *cursor, which must be zero when the launch starts). stats (optional,
+// chunks from the PT_TRACE_PARTS cursors, which must be zero when the launch starts). stats (optional,
 // 2 counters) accumulates node visits and triangle tests. The grid does not depend
 // on the ray counts, so the launch can be captured in a graph.
 hipError_t launch_trace_job(const BvhView& b, const TraceJob& job, int* cursor, unsigned long long* stats,
                             hipStream_t st, const LaunchTuning& tune = LaunchTuning());
-hipError_t launch_zero(int* counters, int n, hipStream_t st);
+// Zeroes counters[i * stride] for i < n (n <= 64).
+hipError_t launch_zero(int* counters, int n, hipStream_t st, int stride = 1);
 // Compute units of the current device (queried once, then cached; called before any
 // graph capture so that no device query happens while capturing).
 int cu_count();

@@ -548,10 +548,10 @@ __global__ void __launch_bounds__(256) k_shade_multi(const JobDev* jobs, ShadeAr
     shade_live_paths<0, true>(SceneView{}, JobView{}, jobs, A);
 }
 
-__global__ void k_zero_counters(int* c, int n)
+__global__ void k_zero_counters(int* c, int n, int stride)
 {
     const int i = threadIdx.x;
-    if (i < n) c[i] = 0;
+    if (i < n) c[i * stride] = 0;
 }
 
 // ---------------------------------------------------------------------------------
@@ -560,9 +560,9 @@ __global__ void k_zero_counters(int* c, int n)
 
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-hipError_t launch_zero(int* counters, int n, hipStream_t st)
+hipError_t launch_zero(int* counters, int n, hipStream_t st, int stride)
 {
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, counters, n);
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, counters, n, stride);
     return hipGetLastError();
 }
 

@@ -19,9 +19,10 @@
 // the reference's walk reaches first (reference_first); only rays that graze a box
 // within rounding of their hit can differ.
 //
-// Execution: persistent waves. A wave takes rays in chunks of 64 from a global cursor
-// and refills each lane as soon as its ray finishes, so lanes do not wait for the
-// longest ray of the wave. A step handles one inner node or one triangle. Each lane keeps
+// Execution: persistent waves. A wave takes rays in chunks from the cursor of its XCD's
+// part of the queues (then from the other parts) and refills its lanes as rays finish,
+// so lanes do not wait for the longest ray of the wave. A step handles one inner node or
+// one triangle. Each lane keeps
 // its traversal stack in LDS (the first lds_stack_entries entries) with deeper entries in
 // private memory; the LDS size per block sets the blocks per CU.
 #include <hip/hip_runtime.h>
@@ -389,6 +390,41 @@ __device__ __forceinline__ void trace_finish(const BvhView& b, const TraceSegmen
     seg.out_k[p] = k;
 }
 
+// Part x of a segment of n rays: [min(x q, n), min(x q + q, n)) with q = ceil(n / parts).
+__device__ __forceinline__ void part_range(int n, int x, int* lo, int* hi)
+{
+    const int q = (n + PT_TRACE_PARTS - 1) / PT_TRACE_PARTS;
+    *lo = min(x * q, n);
+    *hi = min(*lo + q, n);
+}
+
+// Rays in part x over all segments.
+__device__ __forceinline__ int part_total(const int* seg_n, int num_seg, int x)
+{
+    int t = 0;
+    for (int k = 0; k < num_seg; ++k) {
+        int lo, hi;
+        part_range(seg_n[k], x, &lo, &hi);
+        t += hi - lo;
+    }
+    return t;
+}
+
+// The g-th ray of part x (g < part_total): its segment and index in the segment.
+__device__ __forceinline__ void part_item(const int* seg_n, int num_seg, int x, int g, int* seg_id, int* seg_i)
+{
+    for (int k = 0; k < num_seg; ++k) {
+        int lo, hi;
+        part_range(seg_n[k], x, &lo, &hi);
+        if (g < hi - lo || k == num_seg - 1) {
+            *seg_id = k;
+            *seg_i = lo + g;
+            return;
+        }
+        g -= hi - lo;
+    }
+}
+
 template <bool STATS, bool MULTI, bool WIDE>
 __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1, TraceJob job, int* cursor,
                                                                      unsigned long long* stats)
@@ -407,8 +443,10 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1,
     StackEntry spill[(WIDE ? PT_STACK8 : PT_MAX_STACK) - lds_stack_entries<WIDE>()];
     const int lane = threadIdx.x & 63;
     const unsigned long long lane_lt = (1ull << lane) - 1ull;
-    int pool_next = 0, pool_end = 0; // wave-uniform: this wave's share of rays
-    bool exhausted = false;          // wave-uniform
+    // wave-uniform: this wave's share of rays [pool_next, pool_end) in part pool_part, and
+    // how many parts it has found empty
+    int pool_next = 0, pool_end = 0, pool_part = (int)(blockIdx.x % PT_TRACE_PARTS), parts_empty = 0;
+    bool exhausted = false; // wave-uniform
     int item = -1, seg_id = 0, seg_i = 0;
     TraceRay r;
     TraceState s;
@@ -423,44 +461,55 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1,
         if (__popcll(need_mask) >= 16 && !exhausted) {
             const int n_need = __popcll(need_mask);
             const int rank = __popcll(need_mask & lane_lt);
-            int avail = pool_end - pool_next;
-            int base2 = 0;
+            const int leader = __ffsll((long long)need_mask) - 1;
+            const int avail = pool_end - pool_next;
+            int part2 = pool_part, base2 = 0, end2 = 0;
             if (avail < n_need) {
-                int got = 0;
-                if (lane == __ffsll((long long)need_mask) - 1) got = atomicAdd(cursor, job.chunk);
-                base2 = __shfl(got, __ffsll((long long)need_mask) - 1, 64);
-                if (base2 >= total) exhausted = true;
+                // a new chunk: from the current part, else from the next parts in turn
+                for (;;) {
+                    const int len = part_total(seg_n, job.num_seg, part2);
+                    int got = len;
+                    if (lane == leader) got = atomicAdd(cursor + part2 * PT_CURSOR_STRIDE, job.chunk);
+                    got = __shfl(got, leader, 64);
+                    if (got < len) {
+                        base2 = got;
+                        end2 = min(got + job.chunk, len);
+                        break;
+                    }
+                    if (++parts_empty == PT_TRACE_PARTS) {
+                        exhausted = true;
+                        break;
+                    }
+                    part2 = (part2 + 1) % PT_TRACE_PARTS;
+                }
             }
             if (need) {
-                int g = -1;
-                if (rank < avail) g = pool_next + rank;
-                else if (base2 < total) g = base2 + (rank - avail);
-                item = (g >= 0 && g < total) ? g : -1;
+                int g = -1, gp = pool_part;
+                if (rank < avail) {
+                    g = pool_next + rank;
+                } else if (base2 + (rank - avail) < end2) {
+                    g = base2 + (rank - avail);
+                    gp = part2;
+                }
+                item = g;
+                if (g >= 0) {
+                    part_item(seg_n, job.num_seg, gp, g, &seg_id, &seg_i);
+                    const TraceSegment& sg = job.seg[seg_id];
+                    if (MULTI) b = job.views[job.job_of_path[sg.path[seg_i]]];
+                    const float4 r0 = sg.rays[2 * (size_t)seg_i], r1 = sg.rays[2 * (size_t)seg_i + 1];
+                    if (!trace_begin(b, r0, r1, sg.any != 0, r, s)) {
+                        if (STATS) nv += s.nv;
+                        trace_finish(b, sg, seg_i, s);
+                        item = -1;
+                    }
+                }
             }
             if (avail >= n_need) {
                 pool_next += n_need;
             } else {
-                pool_next = base2 + (n_need - avail);
-                pool_end = base2 + job.chunk;
-            }
-            if (pool_end > total) pool_end = total;
-            if (pool_next > pool_end) pool_next = pool_end;
-            if (need && item >= 0) {
-                int g = item;
-                seg_id = 0;
-                while (seg_id < job.num_seg - 1 && g >= seg_n[seg_id]) {
-                    g -= seg_n[seg_id];
-                    ++seg_id;
-                }
-                seg_i = g;
-                const TraceSegment& sg = job.seg[seg_id];
-                if (MULTI) b = job.views[job.job_of_path[sg.path[g]]];
-                const float4 r0 = sg.rays[2 * (size_t)g], r1 = sg.rays[2 * (size_t)g + 1];
-                if (!trace_begin(b, r0, r1, sg.any != 0, r, s)) {
-                    if (STATS) nv += s.nv;
-                    trace_finish(b, sg, g, s);
-                    item = -1;
-                }
+                pool_part = part2;
+                pool_next = min(base2 + (n_need - avail), end2);
+                pool_end = end2;
             }
         }
         if (__ballot(item >= 0) == 0ull) {
