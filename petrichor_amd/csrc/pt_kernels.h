@@ -116,6 +116,8 @@ struct LaunchTuning {
 // so each XCD's L2 holds the BVH nodes of fewer jobs and screen regions, and the chunk
 // atomics, which every wave issues on the same cursor otherwise, are spread over eight.
 #define PT_TRACE_PARTS 8
+#define PT_TRACE_STEAL 1             // other parts a wave takes rays from once its own part is empty
+#define PT_TRACE_STEAL_ALL_RAYS (1 << 24) // ... and in launches of more rays, every other part
 // A part's cursor is cursor[x * PT_CURSOR_STRIDE]: each on a 256-byte line of its own, so
 // that the cursors' atomics are spread over memory channels.
 #define PT_CURSOR_STRIDE 64

@@ -447,6 +447,13 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1,
     // how many parts it has found empty
     int pool_next = 0, pool_end = 0, pool_part = (int)(blockIdx.x % PT_TRACE_PARTS), parts_empty = 0;
     bool exhausted = false; // wave-uniform
+    // Parts a wave helps with after its own. Every part has waves of its own (the grid
+    // has at least PT_TRACE_PARTS blocks), which drain it before they stop, so helping is
+    // only for balance: one more part in launches of up to PT_TRACE_STEAL_ALL_RAYS rays,
+    // where each probe of an empty part is an atomic on a cursor that every wave probes
+    // at the launch's end; all parts in larger launches.
+    const int steal_parts =
+        gridDim.x >= PT_TRACE_PARTS && total <= PT_TRACE_STEAL_ALL_RAYS ? PT_TRACE_STEAL : PT_TRACE_PARTS - 1;
     int item = -1, seg_id = 0, seg_i = 0;
     TraceRay r;
     TraceState s;
@@ -476,7 +483,7 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1,
                         end2 = min(got + job.chunk, len);
                         break;
                     }
-                    if (++parts_empty == PT_TRACE_PARTS) {
+                    if (++parts_empty > steal_parts) {
                         exhausted = true;
                         break;
                     }
