@@ -51,10 +51,11 @@ __device__ __forceinline__ int material_of(const SceneView& s, int slot, float r
     return m;
 }
 
-// Shading point of a hit (Triangle::Interpolate, Triangle.cpp:90-153) at leaf slot k.
-__device__ __forceinline__ void shade_point(const SceneView& s, int k, V3 o, V3 d, float t, V3* pos, V3* nrm)
+// Shading point of a hit (Triangle::Interpolate, Triangle.cpp:90-153) at leaf slot k, of
+// the triangle tr loaded from that slot.
+__device__ __forceinline__ void shade_point_tr(const SceneView& s, const TriDev& tr, int k, V3 o, V3 d, float t,
+                                               V3* pos, V3* nrm)
 {
-    const TriDev tr = load_tri(s.bvh.tris, k);
     const bool smooth = s.tri_smooth != nullptr && s.tri_smooth[k] != 0;
     V3 n0 = v3(0.0f, 0.0f, 1.0f), n1 = n0, n2 = n0;
     if (smooth) {
@@ -64,6 +65,10 @@ __device__ __forceinline__ void shade_point(const SceneView& s, int k, V3 o, V3 
         n2 = v3(q[6], q[7], q[8]);
     }
     tri_shading(tr, smooth, n0, n1, n2, o, d, t, pos, nrm);
+}
+__device__ __forceinline__ void shade_point(const SceneView& s, int k, V3 o, V3 d, float t, V3* pos, V3* nrm)
+{
+    shade_point_tr(s, load_tri(s.bvh.tris, k), k, o, d, t, pos, nrm);
 }
 
 // PathTracing::SampleLight (PathTracing.cpp:329-386). Returns false for the env slot.
@@ -283,14 +288,25 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
         } else if (INTEGRATOR == 0) {
             const int k = hit_k[p];
             const float t = hit_t[p];
+            // the hit triangle and its material are read before the branches that use
+            // them (slot 0 on a miss), so that these loads overlap the path-state loads;
+            // an empty scene (no slot 0, no triangle buffers) reads nothing
+            TriDev pre_tr;
+            pre_tr.p0 = pre_tr.e1 = pre_tr.e2 = pre_tr.n = v3(0.0f, 0.0f, 0.0f);
+            int pre_mat = 0;
+            if (sc.bvh.root_ref != PT_REF_EMPTY) {
+                const int kk = k < 0 ? 0 : k;
+                pre_tr = load_tri(sc.bvh.tris, kk);
+                pre_mat = sc.tri_mat_shading[kk];
+            }
             if (phase == kPhCamera) {
                 if (k < 0) {
                     sum = sum + ray.throughput * env_color(sc.env, ray.dir);
                     sample_done = true;
                     add_color = false;
                 } else {
-                    shade_point(sc, k, ray.o, ray.dir, t, &si_pos, &si_n);
-                    si_mat = sc.tri_mat_shading[k];
+                    shade_point_tr(sc, pre_tr, k, ray.o, ray.dir, t, &si_pos, &si_n);
+                    si_mat = pre_mat;
                     if (sc.mats[si_mat].type == kMatEmission) {
                         sum = sum + ray.throughput * sc.mats[si_mat].color;
                         sample_done = true;
@@ -303,8 +319,8 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
             } else { // kPhBounce
                 if (k >= 0) {
                     V3 npos, nn;
-                    shade_point(sc, k, ray.o, ray.dir, t, &npos, &nn);
-                    const int nmat = sc.tri_mat_shading[k];
+                    shade_point_tr(sc, pre_tr, k, ray.o, ray.dir, t, &npos, &nn);
+                    const int nmat = pre_mat;
                     if (sc.mats[nmat].type == kMatEmission) {
                         const float l2 = sq_length(npos - ray.o);
                         const float cos_p = fabsf_(dot(ray.dir, nn));
