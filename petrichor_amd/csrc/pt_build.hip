@@ -52,33 +52,43 @@ __device__ __forceinline__ float next_toward_zero(float x)
 }
 
 
+// Per-primitive centroid and box, and the root box. A wave handles PT_PRIMS_SPAN groups of
+// 64 triangles and adds its running bounds to the root keys once (with one atomic per
+// wave and group, every wave of the grid would queue on the same six keys).
+#define PT_PRIMS_SPAN 16
 __global__ void k_prims(const float* __restrict__ tri_pos, int n, float4* cen, float4* blo, float4* bhi,
                         uint32_t* root_keys)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < n;
-    V3 c = v3(0, 0, 0), lo = c, hi = c;
-    if (live) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int first = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * PT_PRIMS_SPAN;
+    uint32_t k[6] = {kKeyMax, kKeyMax, kKeyMax, kKeyMin, kKeyMin, kKeyMin};
+    for (int it = 0; it < PT_PRIMS_SPAN; ++it) {
+        const int i = first + 64 * it + lane;
+        if (i >= n) break;
         const float* p = tri_pos + 9 * (size_t)i;
         const V3 p0 = v3(p[0], p[1], p[2]), p1 = v3(p[3], p[4], p[5]), p2 = v3(p[6], p[7], p[8]);
-        lo = v3(smin(smin(FLT_MAX, p0.x), p1.x), smin(smin(FLT_MAX, p0.y), p1.y), smin(smin(FLT_MAX, p0.z), p1.z));
+        V3 lo = v3(smin(smin(FLT_MAX, p0.x), p1.x), smin(smin(FLT_MAX, p0.y), p1.y), smin(smin(FLT_MAX, p0.z), p1.z));
         lo = v3(smin(lo.x, p2.x), smin(lo.y, p2.y), smin(lo.z, p2.z));
-        hi = v3(smax(smax(-FLT_MAX, p0.x), p1.x), smax(smax(-FLT_MAX, p0.y), p1.y), smax(smax(-FLT_MAX, p0.z), p1.z));
+        V3 hi = v3(smax(smax(-FLT_MAX, p0.x), p1.x), smax(smax(-FLT_MAX, p0.y), p1.y), smax(smax(-FLT_MAX, p0.z), p1.z));
         hi = v3(smax(hi.x, p2.x), smax(hi.y, p2.y), smax(hi.z, p2.z));
-        c = (p0 + p1 + p2) / 3.0f; // Triangle::GetCentroid (Triangle.h:45-50)
+        const V3 c = (p0 + p1 + p2) / 3.0f; // Triangle::GetCentroid (Triangle.h:45-50)
         cen[i] = make_float4(c.x, c.y, c.z, 0.0f);
         blo[i] = make_float4(lo.x, lo.y, lo.z, 0.0f);
         bhi[i] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+        k[0] = min(k[0], fkey(lo.x));
+        k[1] = min(k[1], fkey(lo.y));
+        k[2] = min(k[2], fkey(lo.z));
+        k[3] = max(k[3], fkey(hi.x));
+        k[4] = max(k[4], fkey(hi.y));
+        k[5] = max(k[5], fkey(hi.z));
     }
     // root box: wave reduction, then one atomic per wave and bound
-    uint32_t k[6] = {live ? fkey(lo.x) : kKeyMax, live ? fkey(lo.y) : kKeyMax, live ? fkey(lo.z) : kKeyMax,
-                     live ? fkey(hi.x) : kKeyMin, live ? fkey(hi.y) : kKeyMin, live ? fkey(hi.z) : kKeyMin};
     for (int off = 32; off > 0; off >>= 1)
         for (int a = 0; a < 6; ++a) {
             const uint32_t o = __shfl_xor(k[a], off, 64);
             k[a] = a < 3 ? min(k[a], o) : max(k[a], o);
         }
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0 && first < n) {
         for (int a = 0; a < 3; ++a) atomicMin(&root_keys[a], k[a]);
         for (int a = 3; a < 6; ++a) atomicMax(&root_keys[a], k[a]);
     }
@@ -110,32 +120,57 @@ __device__ __forceinline__ bool wave_uniform(int v, bool live)
     return __ballot(live && v != first) == 0ull;
 }
 
+// Centroid bounds per node. A wave walks PT_CB_SPAN consecutive groups of 64 positions
+// (a node's positions are contiguous) and keeps a running min/max while its lanes share
+// one node, so it adds to a node's keys once per node met instead of once per group;
+// groups that straddle nodes add per lane.
+#define PT_CB_SPAN 16
+__device__ __forceinline__ void cb_flush(uint32_t* cbk, int node, uint32_t key[6])
+{
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < 6; ++q) {
+            const uint32_t o = __shfl_xor(key[q], off, 64);
+            key[q] = q < 3 ? min(key[q], o) : max(key[q], o);
+        }
+    if ((threadIdx.x & 63) == 0) {
+        for (int q = 0; q < 3; ++q) atomicMin(&cbk[6 * node + q], key[q]);
+        for (int q = 3; q < 6; ++q) atomicMax(&cbk[6 * node + q], key[q]);
+    }
+}
+
 __global__ void k_centroid_bounds(LevelArgs a)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int pn = k < a.n ? a.pos_node[k] : -1;
-    const bool live = pn >= 0;
-    if (__ballot(live) == 0ull) return;
-    float4 c = make_float4(0, 0, 0, 0);
-    if (live) c = a.cen[a.ids[k]];
-    uint32_t key[6] = {live ? fkey(c.x) : kKeyMax, live ? fkey(c.y) : kKeyMax, live ? fkey(c.z) : kKeyMax,
-                       live ? fkey(c.x) : kKeyMin, live ? fkey(c.y) : kKeyMin, live ? fkey(c.z) : kKeyMin};
-    if (wave_uniform(pn, live)) {
-        for (int off = 32; off > 0; off >>= 1)
-            for (int q = 0; q < 6; ++q) {
-                const uint32_t o = __shfl_xor(key[q], off, 64);
-                key[q] = q < 3 ? min(key[q], o) : max(key[q], o);
+    const int lane = (int)(threadIdx.x & 63);
+    const int first = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * PT_CB_SPAN;
+    uint32_t acc[6] = {kKeyMax, kKeyMax, kKeyMax, kKeyMin, kKeyMin, kKeyMin};
+    int acc_node = -1; // wave-uniform: the node acc belongs to
+    for (int it = 0; it < PT_CB_SPAN; ++it) {
+        const int k = first + 64 * it + lane;
+        if (first + 64 * it >= a.n) break;
+        const int pn = k < a.n ? a.pos_node[k] : -1;
+        const bool live = pn >= 0;
+        if (__ballot(live) == 0ull) continue;
+        float4 c = make_float4(0, 0, 0, 0);
+        if (live) c = a.cen[a.ids[k]];
+        const uint32_t key[6] = {live ? fkey(c.x) : kKeyMax, live ? fkey(c.y) : kKeyMax,
+                                 live ? fkey(c.z) : kKeyMax, live ? fkey(c.x) : kKeyMin,
+                                 live ? fkey(c.y) : kKeyMin, live ? fkey(c.z) : kKeyMin};
+        if (wave_uniform(pn, live)) {
+            const int node = __shfl(pn, __ffsll((long long)__ballot(live)) - 1, 64);
+            if (node != acc_node) {
+                if (acc_node >= 0) cb_flush(a.cbk, acc_node, acc);
+                for (int q = 0; q < 3; ++q) acc[q] = kKeyMax;
+                for (int q = 3; q < 6; ++q) acc[q] = kKeyMin;
+                acc_node = node;
             }
-        const int leader = __ffsll((long long)__ballot(live)) - 1;
-        const int node = __shfl(pn, leader, 64);
-        if ((int)(threadIdx.x & 63) == leader) {
-            for (int q = 0; q < 3; ++q) atomicMin(&a.cbk[6 * node + q], key[q]);
-            for (int q = 3; q < 6; ++q) atomicMax(&a.cbk[6 * node + q], key[q]);
+            for (int q = 0; q < 3; ++q) acc[q] = min(acc[q], key[q]);
+            for (int q = 3; q < 6; ++q) acc[q] = max(acc[q], key[q]);
+        } else if (live) {
+            for (int q = 0; q < 3; ++q) atomicMin(&a.cbk[6 * pn + q], key[q]);
+            for (int q = 3; q < 6; ++q) atomicMax(&a.cbk[6 * pn + q], key[q]);
         }
-    } else if (live) {
-        for (int q = 0; q < 3; ++q) atomicMin(&a.cbk[6 * pn + q], key[q]);
-        for (int q = 3; q < 6; ++q) atomicMax(&a.cbk[6 * pn + q], key[q]);
     }
+    if (acc_node >= 0) cb_flush(a.cbk, acc_node, acc);
 }
 
 __global__ void k_axis(LevelArgs a, int num_active)
@@ -160,78 +195,99 @@ __global__ void k_axis(LevelArgs a, int num_active)
     a.bad[i] = 0;
 }
 
-// Bin of every primitive and the per-bin boxes and counts. A block whose positions all
-// belong to one node (the common case near the root) reduces in LDS first and then
-// adds one atomic per bin and value.
+// Bin of every primitive and the per-bin boxes and counts. A block walks PT_BIN_SPAN
+// consecutive groups of 256 positions. While a group's positions all belong to one node
+// (the common case near the root) they are reduced in LDS, across groups of the same
+// node, and the block adds one atomic per bin and value when the node changes or at the
+// end; other groups add per position.
+#define PT_BIN_SPAN 8
+__device__ __forceinline__ void bin_flush(const LevelArgs& a, const uint32_t* lds, int node)
+{
+    const int nb = a.num_bins;
+    for (int i = threadIdx.x; i < 7 * nb; i += blockDim.x) {
+        const int bin = i < 6 * nb ? i / 6 : i - 6 * nb;
+        if (lds[6 * nb + bin] == 0u) continue;
+        const size_t s = (size_t)node * nb + bin;
+        if (i < 6 * nb) {
+            const int q = i % 6;
+            if (q < 3) atomicMin(&a.bin_keys[6 * s + q], lds[i]);
+            else atomicMax(&a.bin_keys[6 * s + q], lds[i]);
+        } else {
+            atomicAdd(&a.bin_cnt[s], (int)lds[i]);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_bin(LevelArgs a)
 {
     extern __shared__ uint32_t lds[]; // 6 keys then 1 count per bin
     const int nb = a.num_bins;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int pn = k < a.n ? a.pos_node[k] : -1;
-    const int pn0 = a.pos_node[blockIdx.x * blockDim.x];
-    const bool uni = __syncthreads_and(pn < 0 || pn == pn0) && pn0 >= 0;
-    int b = -1;
-    float4 blo = make_float4(0, 0, 0, 0), bhi = blo;
-    if (pn >= 0) {
-        const int id = a.ids[k];
-        const float4 c = a.cen[id];
-        const int ax = a.axis[pn];
-        const float len = a.len[pn];
-        const float lo = funkey(a.cbk[6 * pn + ax]);
-        const float cc = ax == 0 ? c.x : (ax == 1 ? c.y : c.z);
-        const float l = cc - lo;
-        const float x = next_toward_zero((float)nb * l / len);
-        if (!(x == x)) {
-            a.bad[pn] = 1;
-            a.pos_bin[k] = 0;
-        } else {
-            // bins * l / len can round up to bins when bins is not a power of two: the
-            // last bin takes it, as in the host builder
-            b = min(max((int)x, 0), nb - 1);
-            a.pos_bin[k] = b;
-            blo = a.blo[id];
-            bhi = a.bhi[id];
+    const int first = (int)blockIdx.x * (int)blockDim.x * PT_BIN_SPAN;
+    int lds_node = -1; // block-uniform: the node whose bins the LDS holds
+    for (int it = 0; it < PT_BIN_SPAN; ++it) {
+        const int base = first + it * (int)blockDim.x;
+        if (base >= a.n) break;
+        const int k = base + threadIdx.x;
+        const int pn = k < a.n ? a.pos_node[k] : -1;
+        const int pn0 = a.pos_node[base];
+        const bool uni = __syncthreads_and(pn < 0 || pn == pn0) && pn0 >= 0;
+        int b = -1;
+        float4 blo = make_float4(0, 0, 0, 0), bhi = blo;
+        if (pn >= 0) {
+            const int id = a.ids[k];
+            const float4 c = a.cen[id];
+            const int ax = a.axis[pn];
+            const float len = a.len[pn];
+            const float lo = funkey(a.cbk[6 * pn + ax]);
+            const float cc = ax == 0 ? c.x : (ax == 1 ? c.y : c.z);
+            const float l = cc - lo;
+            const float x = next_toward_zero((float)nb * l / len);
+            if (!(x == x)) {
+                a.bad[pn] = 1;
+                a.pos_bin[k] = 0;
+            } else {
+                // bins * l / len can round up to bins when bins is not a power of two:
+                // the last bin takes it, as in the host builder
+                b = min(max((int)x, 0), nb - 1);
+                a.pos_bin[k] = b;
+                blo = a.blo[id];
+                bhi = a.bhi[id];
+            }
         }
-    }
-    if (uni) {
-        for (int i = threadIdx.x; i < 7 * nb; i += blockDim.x)
-            lds[i] = i < 6 * nb ? ((i % 6) < 3 ? kKeyMax : kKeyMin) : 0u;
-        __syncthreads();
-        if (b >= 0) {
-            uint32_t* bk = lds + 6 * b;
+        if (uni) {
+            if (pn0 != lds_node) {
+                __syncthreads();
+                if (lds_node >= 0) bin_flush(a, lds, lds_node);
+                __syncthreads();
+                for (int i = threadIdx.x; i < 7 * nb; i += blockDim.x)
+                    lds[i] = i < 6 * nb ? ((i % 6) < 3 ? kKeyMax : kKeyMin) : 0u;
+                __syncthreads();
+                lds_node = pn0;
+            }
+            if (b >= 0) {
+                uint32_t* bk = lds + 6 * b;
+                atomicMin(&bk[0], fkey(blo.x));
+                atomicMin(&bk[1], fkey(blo.y));
+                atomicMin(&bk[2], fkey(blo.z));
+                atomicMax(&bk[3], fkey(bhi.x));
+                atomicMax(&bk[4], fkey(bhi.y));
+                atomicMax(&bk[5], fkey(bhi.z));
+                atomicAdd(&lds[6 * nb + b], 1u);
+            }
+        } else if (b >= 0) {
+            const size_t s = (size_t)pn * nb + b;
+            uint32_t* bk = a.bin_keys + 6 * s;
             atomicMin(&bk[0], fkey(blo.x));
             atomicMin(&bk[1], fkey(blo.y));
             atomicMin(&bk[2], fkey(blo.z));
             atomicMax(&bk[3], fkey(bhi.x));
             atomicMax(&bk[4], fkey(bhi.y));
             atomicMax(&bk[5], fkey(bhi.z));
-            atomicAdd(&lds[6 * nb + b], 1u);
+            atomicAdd(&a.bin_cnt[s], 1);
         }
-        __syncthreads();
-        for (int i = threadIdx.x; i < 7 * nb; i += blockDim.x) {
-            const int bin = i < 6 * nb ? i / 6 : i - 6 * nb;
-            if (lds[6 * nb + bin] == 0u) continue;
-            const size_t s = (size_t)pn0 * nb + bin;
-            if (i < 6 * nb) {
-                const int q = i % 6;
-                if (q < 3) atomicMin(&a.bin_keys[6 * s + q], lds[i]);
-                else atomicMax(&a.bin_keys[6 * s + q], lds[i]);
-            } else {
-                atomicAdd(&a.bin_cnt[s], (int)lds[i]);
-            }
-        }
-    } else if (b >= 0) {
-        const size_t s = (size_t)pn * nb + b;
-        uint32_t* bk = a.bin_keys + 6 * s;
-        atomicMin(&bk[0], fkey(blo.x));
-        atomicMin(&bk[1], fkey(blo.y));
-        atomicMin(&bk[2], fkey(blo.z));
-        atomicMax(&bk[3], fkey(bhi.x));
-        atomicMax(&bk[4], fkey(bhi.y));
-        atomicMax(&bk[5], fkey(bhi.z));
-        atomicAdd(&a.bin_cnt[s], 1);
     }
+    __syncthreads();
+    if (lds_node >= 0) bin_flush(a, lds, lds_node);
 }
 
 __device__ __forceinline__ float key_area(const uint32_t* lo, const uint32_t* hi)
@@ -440,20 +496,31 @@ __global__ void k_partition(PartArgs p)
     int side = -1; // -1: stays, 0 left, 1 right
     int id = inr ? p.ids[k] : 0;
     if (pn >= 0 && p.best_k[pn] != 0) side = p.pos_bin[k] < p.best_k[pn] ? 0 : 1;
-    // wave-aggregated cursor increments when the wave shares one node and side
+    // Cursor increments aggregated per (node, side) key over the wave: one atomic for each
+    // of the first four keys present (the two sides of one node at the top levels, where
+    // every wave would otherwise queue 64 atomics on the same two cursors), then one per
+    // remaining lane.
     const bool moving = side >= 0;
-    int slot = -1;
     const int key = moving ? 2 * pn + side : -1;
-    if (__ballot(moving) != 0ull && wave_uniform(key, moving)) {
-        const unsigned long long m = __ballot(moving);
+    const int lane = (int)(threadIdx.x & 63);
+    int slot = -1;
+    bool pending = moving;
+    for (int it = 0; it < 4; ++it) {
+        const unsigned long long m = __ballot(pending);
+        if (m == 0ull) break;
         const int leader = __ffsll((long long)m) - 1;
+        const int k0 = __shfl(key, leader, 64);
+        const bool mine = pending && key == k0;
+        const unsigned long long same = __ballot(mine);
         int base = 0;
-        if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(&p.cursor[key], __popcll(m));
+        if (lane == leader) base = atomicAdd(&p.cursor[k0], __popcll(same));
         base = __shfl(base, leader, 64);
-        if (moving) slot = base + __popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull));
-    } else if (moving) {
-        slot = atomicAdd(&p.cursor[key], 1);
+        if (mine) {
+            slot = base + __popcll(same & ((1ull << lane) - 1ull));
+            pending = false;
+        }
     }
+    if (pending) slot = atomicAdd(&p.cursor[key], 1);
     if (!inr) return;
     if (!moving) {
         p.ids_out[k] = id;
@@ -723,7 +790,8 @@ hipError_t gpu_build_bvh(const float* tri_pos_d, int n, int num_bins, int leaf_m
 
     // primitives, the root box and the identity order
     hipLaunchKernelGGL(k_fill_keyboxes, dim3(1), dim3(64), 0, st, root_keys, (size_t)1);
-    hipLaunchKernelGGL(k_prims, dim3(blocks(n)), dim3(256), 0, st, tri_pos_d, n, cen, blo, bhi, root_keys);
+    hipLaunchKernelGGL(k_prims, dim3(blocks(((size_t)n + PT_PRIMS_SPAN - 1) / PT_PRIMS_SPAN)), dim3(256), 0, st,
+                       tri_pos_d, n, cen, blo, bhi, root_keys);
     hipLaunchKernelGGL(k_iota, dim3(blocks(n)), dim3(256), 0, st, ids[0], n);
     B_CHECK(hipGetLastError());
     uint32_t rk[6];
@@ -771,9 +839,11 @@ hipError_t gpu_build_bvh(const float* tri_pos_d, int n, int num_bins, int leaf_m
                            (size_t)num_active);
         hipLaunchKernelGGL(k_fill_keyboxes, dim3(blocks(6 * nbins)), dim3(256), 0, st, bin_keys, nbins);
         hipLaunchKernelGGL(k_fill_i32, dim3(blocks(nbins)), dim3(256), 0, st, bin_cnt, 0, nbins);
-        hipLaunchKernelGGL(k_centroid_bounds, dim3(blocks(n)), dim3(256), 0, st, la);
+        hipLaunchKernelGGL(k_centroid_bounds, dim3(blocks(((size_t)n + PT_CB_SPAN - 1) / PT_CB_SPAN)), dim3(256), 0,
+                           st, la);
         hipLaunchKernelGGL(k_axis, dim3(blocks(num_active)), dim3(256), 0, st, la, num_active);
-        hipLaunchKernelGGL(k_bin, dim3(blocks(n)), dim3(256), 7 * num_bins * sizeof(uint32_t), st, la);
+        hipLaunchKernelGGL(k_bin, dim3(blocks(((size_t)n + PT_BIN_SPAN - 1) / PT_BIN_SPAN)), dim3(256),
+                           7 * num_bins * sizeof(uint32_t), st, la);
         SahArgs sa;
         sa.num_active = num_active;
         sa.num_bins = num_bins;
