@@ -82,7 +82,16 @@ EngineJob make_job(const pt_job_desc& d, int64_t id)
 int run_jobs(pt_engine& E, std::vector<EngineJob>& jobs, int iters_per_sync, pt_batch_stats* stats)
 {
     pt_context* ctx = E.ctx;
-    size_t next = 0;
+    // Admission order: largest job first (samples W x H x spp), so that the long jobs start
+    // early and the small ones fill the path slots around them and at the end; a job that
+    // does not fit yet is passed over for later ones that do (first fit). The images do not
+    // depend on the order.
+    std::vector<EngineJob*> pending;
+    pending.reserve(jobs.size());
+    for (EngineJob& j : jobs) pending.push_back(&j);
+    std::stable_sort(pending.begin(), pending.end(), [](const EngineJob* a, const EngineJob* b) {
+        return a->npix * a->desc.spp > b->npix * b->desc.spp;
+    });
     std::vector<EngineJob*> running;
     hipEvent_t ev0, ev1;
     PT_HIP(ctx, hipEventCreate(&ev0));
@@ -90,15 +99,18 @@ int run_jobs(pt_engine& E, std::vector<EngineJob>& jobs, int iters_per_sync, pt_
     PT_HIP(ctx, hipEventRecord(ev0, E.st));
     const auto t0 = std::chrono::steady_clock::now();
     int64_t replays = 0;
-    while (next < jobs.size() || !running.empty()) {
-        while (next < jobs.size() && (int64_t)running.size() < E.max_jobs && E.largest_free() >= jobs[next].npix &&
-               E.width_ok(pt_engine::job_width(jobs[next]))) {
-            EngineJob& j = jobs[next];
+    while (!pending.empty() || !running.empty()) {
+        for (size_t i = 0; i < pending.size() && (int64_t)running.size() < E.max_jobs;) {
+            EngineJob& j = *pending[i];
+            if (E.largest_free() < j.npix || !E.width_ok(pt_engine::job_width(j))) {
+                ++i;
+                continue;
+            }
             int rc = prepare_output(ctx, j);
             if (rc == PT_OK) rc = E.start(j);
             if (rc != PT_OK) return rc;
             running.push_back(&j);
-            ++next;
+            pending.erase(pending.begin() + i);
         }
         if (running.empty()) return pt_fail(ctx, PT_ERR_BUDGET, "job larger than the engine's path capacity");
         int rc = E.run(iters_per_sync);
