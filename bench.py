@@ -293,27 +293,32 @@ def batch_bench(args):
     env = scenes.sky_env()
     ctx = P.Context(0)
     t0 = time.perf_counter()
-    bvhs = []
+    # the autotuner picks the node width unless --bvh-width fixes it
+    widths = (args.bvh_width,) if args.bvh_width else (2, 8)
+    bvhs_by_width = {w: [] for w in widths}
     for i, sp in enumerate(specs):
         sc = scenes.scaled_scene(sp["tris"], seed=1000 + i, env=env)
         sp["tris"] = int(len(sc["tri_mat"]))
-        bvhs.append(ctx.scene(sc).build_bvh(16))
-    log("%d scenes (%d triangles) generated, uploaded and built in %.1f s"
+        s_ = ctx.scene(sc)
+        for w in bvhs_by_width:
+            bvhs_by_width[w].append(s_.build_bvh(16, width=w))
+    log("%d scenes (%d triangles) generated, uploaded and built (both widths) in %.1f s"
         % (len(specs), sum(sp["tris"] for sp in specs), time.perf_counter() - t0))
 
     def outs():
         return [torch.empty((sp["height"], sp["width"], 3), dtype=torch.float32, device="cuda") for sp in specs]
 
-    def jobs_with(o):
+    def jobs_with(o, bvhs):
         return [P.Job(b, sp["width"], sp["height"], sp["spp"], sp["depth"], P.INTEGRATOR_MIS, 7 + i, out=o[i])
                 for i, (b, sp) in enumerate(zip(bvhs, specs))]
 
     out_b = outs()
-    jobs = jobs_with(out_b)
-    best = autotune.tune(ctx, jobs, log=log)
+    best = autotune.tune(ctx, None, log=log, by_width={w: jobs_with(out_b, bvhs_by_width[w]) for w in widths})
+    jobs = best["jobs"]
+    bvhs = bvhs_by_width[best["width"]]
     eng = P.Engine(ctx, best["max_paths"], max_jobs=len(jobs))
     eng.set_tuning(**best["tuning"])
-    eng.render_batch(autotune.calibration_jobs(jobs))  # warm-up at 1 spp
+    eng.render_batch(autotune.calibration_jobs(jobs))  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     st = eng.render_batch(jobs)
@@ -360,7 +365,8 @@ def batch_bench(args):
         "batched_mrays_s": round(st["queries"] / t_batch / 1e6, 1),
         "queries": int(st["queries"]),
         "images_bit_identical": bool(same_plain and same_graph),
-        "autotune": {"max_paths": best["max_paths"], "tuning": best["tuning"], "trials": best["trials"],
+        "autotune": {"bvh_width": best["width"], "max_paths": best["max_paths"], "tuning": best["tuning"],
+                     "trials": best["trials"],
                      "calibration_ms": round(best["seconds"] * 1e3, 1)},
         "jobs_detail": specs,
     }
@@ -512,7 +518,8 @@ def main():
     ap.add_argument("--triangles", type=int, default=10_000_000, help="trace mode: triangle count")
     ap.add_argument("--bvh-width", type=int, default=None, choices=[2, 8],
                     help="BVH width: 2 walks the binary tree in the reference's order (frame default), "
-                         "8 the 8-wide tree (traversal microbenchmark default)")
+                         "8 the 8-wide tree (traversal microbenchmark default); batch mode: both are "
+                         "built and the autotuner picks one unless this is given")
     ap.add_argument("--jobs", type=int, default=64, help="batch mode: number of jobs")
     ap.add_argument("--tris-max", type=float, default=2e6, help="batch/serve modes: largest scene")
     ap.add_argument("--spp-max", type=int, default=256, help="batch/serve modes: largest spp")

@@ -299,6 +299,7 @@ class Bvh:
     def __init__(self, scene, bins=16, leaf_max=4, width=2, builder=BUILD_DEVICE):
         self.scene = scene
         self.ctx = scene.ctx
+        self.width = width
         d = BvhDesc(bins, leaf_max, width, builder)
         h = ctypes.c_void_p()
         self.ctx.check(self.ctx._lib.pt_bvh_build(scene.handle, ctypes.byref(d), ctypes.byref(h)), "pt_bvh_build")

@@ -1,11 +1,14 @@
 """Autotuner of the batched engine.
 
-Chooses, for a given set of render jobs, the path-pool size (the capacity of every ray
-queue), the graph replays per host synchronisation and the launch shapes of the trace
-and shading kernels (pt_tuning). It times pt_render_batch on a calibration batch - the
-same scenes and resolutions at one sample per pixel - with coordinate descent over the
-space below, for each pool size. Every candidate renders the same images (tuning only
-changes launch shapes and scheduling), so the fastest one is taken as is.
+Chooses, for a given set of render jobs, the BVH node width (when the jobs are given over
+BVHs of both widths), the path-pool size (the capacity of every ray queue), the graph
+replays per host synchronisation and the launch shapes of the trace and shading kernels
+(pt_tuning). It times pt_render_batch on a calibration batch - the same scenes and
+resolutions at a few samples per pixel - first for each width with the default launch
+shapes, then with coordinate descent over the space below for each pool size. Every
+candidate of one width renders the same images (tuning only changes launch shapes and
+scheduling), so the fastest one is taken as is; the two widths differ only on rays that
+graze a box within rounding of their hit.
 """
 import time
 
@@ -18,26 +21,30 @@ SPACE = {
     "shade_blocks_per_cu": [4, 8, 16],
 }
 
+# Path-pool sizes tried (at least the largest job's pixel count). Larger pools keep more
+# scenes in flight at once, whose BVHs then compete for L2 and the Infinity Cache.
+POOLS = (1 << 22, 6 << 20, 1 << 23)
 
-def calibration_jobs(jobs, spp=1):
+
+def calibration_jobs(jobs, spp=2):
     """The jobs at `spp` samples per pixel with host outputs dropped (device outputs kept)."""
     from . import Job
     return [Job(j.bvh, j.width, j.height, min(spp, j.spp), j.max_bounces, j.integrator, j.seed, j.out) for j in jobs]
 
 
-def tune(ctx, jobs, pool_sizes=None, passes=1, log=None):
-    """Returns {"max_paths", "tuning", "seconds", "trials"} for the fastest configuration
-    found on the calibration batch of `jobs`."""
+def tune(ctx, jobs, pool_sizes=None, passes=1, log=None, by_width=None, calib_spp=2):
+    """Returns {"width", "jobs", "max_paths", "tuning", "seconds", "trials"} for the fastest
+    configuration found on the calibration batch of `jobs`. by_width (optional) maps a BVH
+    width (2, 8) to the same jobs over BVHs of that width; "jobs" is then the chosen list."""
     from . import Engine
+    by_width = dict(by_width) if by_width else {getattr(jobs[0].bvh, "width", 2): jobs}
+    jobs = next(iter(by_width.values()))
     need = max(j.width * j.height for j in jobs)
     if pool_sizes is None:
-        pool_sizes = sorted({max(need, p) for p in (1 << 21, 1 << 22, 1 << 23)})
-    calib = calibration_jobs(jobs)
+        pool_sizes = sorted({max(need, p) for p in POOLS})
     trials = []
-    best = None
-    for pool in pool_sizes:
-        eng = Engine(ctx, pool, max_jobs=max(len(jobs), 1))
 
+    def timer(eng, calib, pool):
         def measure(t):
             eng.set_tuning(**t)
             t0 = time.perf_counter()
@@ -45,7 +52,30 @@ def tune(ctx, jobs, pool_sizes=None, passes=1, log=None):
             dt = time.perf_counter() - t0
             trials.append({"max_paths": pool, "tuning": dict(t), "seconds": dt})
             return dt
+        return measure
 
+    # node width: the default launch shapes on the middle pool size
+    width = next(iter(by_width))
+    if len(by_width) > 1:
+        pool = pool_sizes[len(pool_sizes) // 2]
+        eng = Engine(ctx, pool, max_jobs=max(len(jobs), 1))
+        best_w = None
+        for w, wj in sorted(by_width.items()):
+            measure = timer(eng, calibration_jobs(wj, calib_spp), pool)
+            measure(DEFAULT)  # first use of this width: graph instantiation and caches
+            t = measure(DEFAULT)
+            if log:
+                log("autotune: width %d -> %.1f ms" % (w, t * 1e3))
+            if best_w is None or t < best_w[1]:
+                best_w = (w, t)
+        eng.close()
+        width = best_w[0]
+    chosen = by_width[width]
+    calib = calibration_jobs(chosen, calib_spp)
+    best = None
+    for pool in pool_sizes:
+        eng = Engine(ctx, pool, max_jobs=max(len(chosen), 1))
+        measure = timer(eng, calib, pool)
         measure(DEFAULT)  # first use: graph instantiation and caches
         cur, cur_t = dict(DEFAULT), measure(DEFAULT)
         for _ in range(passes):
@@ -62,5 +92,7 @@ def tune(ctx, jobs, pool_sizes=None, passes=1, log=None):
         if best is None or cur_t < best["seconds"]:
             best = {"max_paths": pool, "tuning": cur, "seconds": cur_t}
         eng.close()
+    best["width"] = width
+    best["jobs"] = chosen
     best["trials"] = len(trials)
     return best

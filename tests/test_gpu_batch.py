@@ -42,3 +42,29 @@ def test_timed_render_matches_graph_render(gpu):
     assert sa["queries"] == stt["queries"]
     k = stt["kernels"]
     assert k["extension"]["items"] + k["shadow_any"]["items"] + k["shadow_closest"]["items"] == stt["queries"]
+
+
+def test_autotune_picks_a_width_whose_batch_equals_plain(gpu):
+    """The autotuner times the calibration batch over BVHs of both widths and returns one
+    width's jobs and launch shapes; the batch rendered with them equals the per-job plain
+    renders of the same BVHs."""
+    from petrichor_amd import autotune
+    c = ctx()
+    scs = [c.scene(scenes.cornell_box()),
+           c.scene(scenes.spheres_scene(n_spheres=8, seg=16, bands=9, terrain_n=32, env_w=256, env_h=128))]
+    specs = [(0, 48, 40, 6, 8, 11), (1, 40, 32, 5, 12, 12), (1, 24, 24, 7, 10, 14)]
+    by_width = {}
+    for w in (2, 8):
+        bv = [s.build_bvh(16, width=w) for s in scs]
+        by_width[w] = [P.Job(bv[i], W, H, spp, d, 0, seed) for (i, W, H, spp, d, seed) in specs]
+    best = autotune.tune(c, None, pool_sizes=[4096], passes=0, by_width=by_width)
+    log("autotune", {k: v for k, v in best.items() if k != "jobs"})
+    assert best["width"] in (2, 8) and best["jobs"] is by_width[best["width"]]
+    eng = P.Engine(c, best["max_paths"], max_jobs=4)
+    eng.set_tuning(**best["tuning"])
+    jobs = [P.Job(j.bvh, j.width, j.height, j.spp, j.max_bounces, j.integrator, j.seed) for j in best["jobs"]]
+    eng.render_batch(jobs)
+    for j in jobs:
+        plain, _ = j.bvh.render(j.width, j.height, j.spp, j.max_bounces, j.integrator, j.seed, plain=True)
+        assert j.bvh.width == best["width"]
+        assert bits_equal(j.result, plain)
