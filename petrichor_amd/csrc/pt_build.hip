@@ -123,7 +123,7 @@ __device__ __forceinline__ bool wave_uniform(int v, bool live)
 // Centroid bounds per node. A wave walks PT_CB_SPAN consecutive groups of 64 positions
 // (a node's positions are contiguous) and keeps a running min/max while its lanes share
 // one node, so it adds to a node's keys once per node met instead of once per group;
-// groups that straddle nodes add per lane.
+// groups that straddle nodes add once per node run.
 #define PT_CB_SPAN 16
 __device__ __forceinline__ void cb_flush(uint32_t* cbk, int node, uint32_t key[6])
 {
@@ -165,9 +165,24 @@ __global__ void k_centroid_bounds(LevelArgs a)
             }
             for (int q = 0; q < 3; ++q) acc[q] = min(acc[q], key[q]);
             for (int q = 3; q < 6; ++q) acc[q] = max(acc[q], key[q]);
-        } else if (live) {
-            for (int q = 0; q < 3; ++q) atomicMin(&a.cbk[6 * pn + q], key[q]);
-            for (int q = 3; q < 6; ++q) atomicMax(&a.cbk[6 * pn + q], key[q]);
+        } else {
+            // The group straddles nodes. A node's positions are contiguous, so its lanes form
+            // one run: a segmented scan leaves each run's bounds in its last lane, which adds
+            // them once.
+            uint32_t kk[6];
+            for (int q = 0; q < 6; ++q) kk[q] = key[q];
+            for (int off = 1; off < 64; off <<= 1) {
+                const int opn = __shfl_up(pn, off, 64);
+                for (int q = 0; q < 6; ++q) {
+                    const uint32_t o = __shfl_up(kk[q], off, 64);
+                    if (lane >= off && opn == pn) kk[q] = q < 3 ? min(kk[q], o) : max(kk[q], o);
+                }
+            }
+            const int npn = __shfl_down(pn, 1, 64);
+            if (live && (lane == 63 || npn != pn)) {
+                for (int q = 0; q < 3; ++q) atomicMin(&a.cbk[6 * pn + q], kk[q]);
+                for (int q = 3; q < 6; ++q) atomicMax(&a.cbk[6 * pn + q], kk[q]);
+            }
         }
     }
     if (acc_node >= 0) cb_flush(a.cbk, acc_node, acc);
@@ -199,31 +214,66 @@ __global__ void k_axis(LevelArgs a, int num_active)
 // consecutive groups of 256 positions. While a group's positions all belong to one node
 // (the common case near the root) they are reduced in LDS, across groups of the same
 // node, and the block adds one atomic per bin and value when the node changes or at the
-// end; other groups add per position.
+// end. A group over at most PT_BIN_NODES nodes (deeper levels; a node's positions are
+// contiguous and the active nodes are numbered in position order) is reduced in an LDS
+// table of those nodes' bins; other groups add per position.
 #define PT_BIN_SPAN 8
-__device__ __forceinline__ void bin_flush(const LevelArgs& a, const uint32_t* lds, int node)
+#define PT_BIN_NODES 8
+// LDS words of k_bin for nb bins: the one-node table, the PT_BIN_NODES-node table and
+// the group's lowest and highest node
+inline size_t bin_lds_words(int nb) { return (size_t)7 * nb * (1 + PT_BIN_NODES) + 2; }
+
+// Adds the bin tables of nodes [node0, node0 + count) held in lds (7 * nb words per node:
+// 6 keys per bin, then a count per bin) to the global bins.
+__device__ __forceinline__ void bin_flush(const LevelArgs& a, const uint32_t* lds, int node0, int count)
 {
     const int nb = a.num_bins;
-    for (int i = threadIdx.x; i < 7 * nb; i += blockDim.x) {
-        const int bin = i < 6 * nb ? i / 6 : i - 6 * nb;
-        if (lds[6 * nb + bin] == 0u) continue;
-        const size_t s = (size_t)node * nb + bin;
-        if (i < 6 * nb) {
-            const int q = i % 6;
-            if (q < 3) atomicMin(&a.bin_keys[6 * s + q], lds[i]);
-            else atomicMax(&a.bin_keys[6 * s + q], lds[i]);
+    const int per = 7 * nb;
+    for (int i = threadIdx.x; i < count * per; i += blockDim.x) {
+        const int off = i / per, j = i - off * per;
+        const uint32_t* t = lds + off * per;
+        const int bin = j < 6 * nb ? j / 6 : j - 6 * nb;
+        if (t[6 * nb + bin] == 0u) continue;
+        const size_t s = (size_t)(node0 + off) * nb + bin;
+        if (j < 6 * nb) {
+            const int q = j % 6;
+            if (q < 3) atomicMin(&a.bin_keys[6 * s + q], t[j]);
+            else atomicMax(&a.bin_keys[6 * s + q], t[j]);
         } else {
-            atomicAdd(&a.bin_cnt[s], (int)lds[i]);
+            atomicAdd(&a.bin_cnt[s], (int)t[j]);
         }
     }
 }
 
+__device__ __forceinline__ void bin_clear(uint32_t* t, int words, int nb)
+{
+    for (int i = threadIdx.x; i < words; i += blockDim.x) {
+        const int j = i % (7 * nb);
+        t[i] = j < 6 * nb ? ((j % 6) < 3 ? kKeyMax : kKeyMin) : 0u;
+    }
+}
+
+__device__ __forceinline__ void bin_add(uint32_t* t, int nb, int b, float4 blo, float4 bhi)
+{
+    uint32_t* bk = t + 6 * b;
+    atomicMin(&bk[0], fkey(blo.x));
+    atomicMin(&bk[1], fkey(blo.y));
+    atomicMin(&bk[2], fkey(blo.z));
+    atomicMax(&bk[3], fkey(bhi.x));
+    atomicMax(&bk[4], fkey(bhi.y));
+    atomicMax(&bk[5], fkey(bhi.z));
+    atomicAdd(&t[6 * nb + b], 1u);
+}
+
 __global__ __launch_bounds__(256) void k_bin(LevelArgs a)
 {
-    extern __shared__ uint32_t lds[]; // 6 keys then 1 count per bin
+    extern __shared__ uint32_t lds[];
     const int nb = a.num_bins;
+    uint32_t* one = lds;                                   // the one-node table
+    uint32_t* many = lds + 7 * nb;                         // the PT_BIN_NODES-node table
+    int* grp = (int*)(lds + 7 * nb * (1 + PT_BIN_NODES)); // lowest, highest node of a group
     const int first = (int)blockIdx.x * (int)blockDim.x * PT_BIN_SPAN;
-    int lds_node = -1; // block-uniform: the node whose bins the LDS holds
+    int one_node = -1; // block-uniform: the node whose bins the one-node table holds
     for (int it = 0; it < PT_BIN_SPAN; ++it) {
         const int base = first + it * (int)blockDim.x;
         if (base >= a.n) break;
@@ -255,25 +305,35 @@ __global__ __launch_bounds__(256) void k_bin(LevelArgs a)
             }
         }
         if (uni) {
-            if (pn0 != lds_node) {
+            if (pn0 != one_node) {
                 __syncthreads();
-                if (lds_node >= 0) bin_flush(a, lds, lds_node);
+                if (one_node >= 0) bin_flush(a, one, one_node, 1);
                 __syncthreads();
-                for (int i = threadIdx.x; i < 7 * nb; i += blockDim.x)
-                    lds[i] = i < 6 * nb ? ((i % 6) < 3 ? kKeyMax : kKeyMin) : 0u;
+                bin_clear(one, 7 * nb, nb);
                 __syncthreads();
-                lds_node = pn0;
+                one_node = pn0;
             }
-            if (b >= 0) {
-                uint32_t* bk = lds + 6 * b;
-                atomicMin(&bk[0], fkey(blo.x));
-                atomicMin(&bk[1], fkey(blo.y));
-                atomicMin(&bk[2], fkey(blo.z));
-                atomicMax(&bk[3], fkey(bhi.x));
-                atomicMax(&bk[4], fkey(bhi.y));
-                atomicMax(&bk[5], fkey(bhi.z));
-                atomicAdd(&lds[6 * nb + b], 1u);
-            }
+            if (b >= 0) bin_add(one, nb, b, blo, bhi);
+            continue;
+        }
+        // the group's node range
+        if (threadIdx.x == 0) {
+            grp[0] = 0x7fffffff;
+            grp[1] = -1;
+        }
+        __syncthreads();
+        if (pn >= 0) {
+            atomicMin(&grp[0], pn);
+            atomicMax(&grp[1], pn);
+        }
+        __syncthreads();
+        const int lo_node = grp[0], count = grp[1] - grp[0] + 1;
+        if (count >= 1 && count <= PT_BIN_NODES) {
+            bin_clear(many, count * 7 * nb, nb);
+            __syncthreads();
+            if (b >= 0) bin_add(many + (pn - lo_node) * 7 * nb, nb, b, blo, bhi);
+            __syncthreads();
+            bin_flush(a, many, lo_node, count);
         } else if (b >= 0) {
             const size_t s = (size_t)pn * nb + b;
             uint32_t* bk = a.bin_keys + 6 * s;
@@ -285,9 +345,10 @@ __global__ __launch_bounds__(256) void k_bin(LevelArgs a)
             atomicMax(&bk[5], fkey(bhi.z));
             atomicAdd(&a.bin_cnt[s], 1);
         }
+        __syncthreads(); // grp and the many-node table are reused by the next group
     }
     __syncthreads();
-    if (lds_node >= 0) bin_flush(a, lds, lds_node);
+    if (one_node >= 0) bin_flush(a, one, one_node, 1);
 }
 
 __device__ __forceinline__ float key_area(const uint32_t* lo, const uint32_t* hi)
@@ -488,21 +549,18 @@ struct PartArgs {
     int* pos_node_out;
 };
 
-__global__ void k_partition(PartArgs p)
+// Partition of one group of 64 positions (every lane of the wave calls it). Cursor
+// increments are aggregated per (node, side) key over the wave: one atomic for each of the
+// first four keys present, then one per remaining lane.
+__device__ __forceinline__ void partition_group(const PartArgs& p, int k, int lane)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const bool inr = k < p.n;
     const int pn = inr ? p.pos_node[k] : -1;
     int side = -1; // -1: stays, 0 left, 1 right
-    int id = inr ? p.ids[k] : 0;
+    const int id = inr ? p.ids[k] : 0;
     if (pn >= 0 && p.best_k[pn] != 0) side = p.pos_bin[k] < p.best_k[pn] ? 0 : 1;
-    // Cursor increments aggregated per (node, side) key over the wave: one atomic for each
-    // of the first four keys present (the two sides of one node at the top levels, where
-    // every wave would otherwise queue 64 atomics on the same two cursors), then one per
-    // remaining lane.
     const bool moving = side >= 0;
     const int key = moving ? 2 * pn + side : -1;
-    const int lane = (int)(threadIdx.x & 63);
     int slot = -1;
     bool pending = moving;
     for (int it = 0; it < 4; ++it) {
@@ -531,6 +589,91 @@ __global__ void k_partition(PartArgs p)
     const int dst = side == 0 ? nd.begin + slot : nd.begin + p.n_left[pn] + slot;
     p.ids_out[dst] = id;
     p.pos_node_out[dst] = p.child_next[2 * pn + side];
+}
+
+// Partition of the positions into the children. A wave handles PT_PART_SPAN groups of 64
+// positions. When all of them belong to one node (near the root), it counts both sides
+// over the span first and takes its slots with one atomic per side, then places the
+// positions in order; otherwise it partitions group by group.
+#define PT_PART_SPAN 16
+__global__ void k_partition(PartArgs p)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const unsigned long long lane_lt = (1ull << lane) - 1ull;
+    const int first = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * PT_PART_SPAN;
+    if (first >= p.n) return;
+    int lo = 0x7fffffff, hi = -1;
+    for (int it = 0; it < PT_PART_SPAN; ++it) {
+        const int k = first + 64 * it + lane;
+        const int pn = k < p.n ? p.pos_node[k] : -1;
+        if (pn >= 0) {
+            lo = min(lo, pn);
+            hi = max(hi, pn);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, __shfl_xor(lo, off, 64));
+        hi = max(hi, __shfl_xor(hi, off, 64));
+    }
+    if (hi < 0 || lo != hi) {
+        for (int it = 0; it < PT_PART_SPAN; ++it) partition_group(p, first + 64 * it + lane, lane);
+        return;
+    }
+    const int node = lo;
+    const int bk = p.best_k[node];
+    if (bk == 0) { // the node becomes a leaf: every position stays
+        for (int it = 0; it < PT_PART_SPAN; ++it) {
+            const int k = first + 64 * it + lane;
+            if (k < p.n) {
+                p.ids_out[k] = p.ids[k];
+                p.pos_node_out[k] = -1;
+            }
+        }
+        return;
+    }
+    int nl = 0, nr = 0;
+    for (int it = 0; it < PT_PART_SPAN; ++it) {
+        const int k = first + 64 * it + lane;
+        const bool live = k < p.n && p.pos_node[k] >= 0;
+        const bool left = live && p.pos_bin[k] < bk;
+        nl += __popcll(__ballot(left));
+        nr += __popcll(__ballot(live && !left));
+    }
+    int bl = 0, br = 0;
+    if (lane == 0) {
+        if (nl) bl = atomicAdd(&p.cursor[2 * node], nl);
+        if (nr) br = atomicAdd(&p.cursor[2 * node + 1], nr);
+    }
+    bl = __shfl(bl, 0, 64);
+    br = __shfl(br, 0, 64);
+    const BuildNode& nd = p.nodes[p.act[node]];
+    const int right0 = nd.begin + p.n_left[node];
+    const int child_l = p.child_next[2 * node], child_r = p.child_next[2 * node + 1];
+    for (int it = 0; it < PT_PART_SPAN; ++it) {
+        const int k = first + 64 * it + lane;
+        const bool inr = k < p.n;
+        const bool live = inr && p.pos_node[k] >= 0;
+        const bool left = live && p.pos_bin[k] < bk;
+        const bool right = live && !left;
+        const unsigned long long ml = __ballot(left), mr = __ballot(right);
+        if (inr) {
+            const int id = p.ids[k];
+            if (left) {
+                const int dst = nd.begin + bl + __popcll(ml & lane_lt);
+                p.ids_out[dst] = id;
+                p.pos_node_out[dst] = child_l;
+            } else if (right) {
+                const int dst = right0 + br + __popcll(mr & lane_lt);
+                p.ids_out[dst] = id;
+                p.pos_node_out[dst] = child_r;
+            } else {
+                p.ids_out[k] = id;
+                p.pos_node_out[k] = -1;
+            }
+        }
+        bl += __popcll(ml);
+        br += __popcll(mr);
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
@@ -843,7 +986,7 @@ hipError_t gpu_build_bvh(const float* tri_pos_d, int n, int num_bins, int leaf_m
                            st, la);
         hipLaunchKernelGGL(k_axis, dim3(blocks(num_active)), dim3(256), 0, st, la, num_active);
         hipLaunchKernelGGL(k_bin, dim3(blocks(((size_t)n + PT_BIN_SPAN - 1) / PT_BIN_SPAN)), dim3(256),
-                           7 * num_bins * sizeof(uint32_t), st, la);
+                           bin_lds_words(num_bins) * sizeof(uint32_t), st, la);
         SahArgs sa;
         sa.num_active = num_active;
         sa.num_bins = num_bins;
@@ -891,7 +1034,8 @@ hipError_t gpu_build_bvh(const float* tri_pos_d, int n, int num_bins, int leaf_m
         pa.cursor = cursor;
         pa.ids_out = ids[1 - cur];
         pa.pos_node_out = pos_node[1 - cur];
-        hipLaunchKernelGGL(k_partition, dim3(blocks(n)), dim3(256), 0, st, pa);
+        hipLaunchKernelGGL(k_partition, dim3(blocks(((size_t)n + PT_PART_SPAN - 1) / PT_PART_SPAN)), dim3(256), 0,
+                           st, pa);
         B_CHECK(hipGetLastError());
         B_CHECK(hipStreamSynchronize(st));
         const int splits = (int)(tot >> 32);
