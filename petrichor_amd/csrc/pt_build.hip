@@ -102,6 +102,7 @@ struct LevelArgs {
     const int* pos_node;  // active index of the node owning each position, -1 = in a leaf
     int n;
     int num_bins;
+    int bin_nodes;        // nodes of k_bin's many-node LDS table
     // per active node
     const int* act;       // global node id
     uint32_t* cbk;        // centroid bounds keys, 6 per node
@@ -214,14 +215,15 @@ __global__ void k_axis(LevelArgs a, int num_active)
 // consecutive groups of 256 positions. While a group's positions all belong to one node
 // (the common case near the root) they are reduced in LDS, across groups of the same
 // node, and the block adds one atomic per bin and value when the node changes or at the
-// end. A group over at most PT_BIN_NODES nodes (deeper levels; a node's positions are
+// end. A group over at most bin_table_nodes nodes (deeper levels; a node's positions are
 // contiguous and the active nodes are numbered in position order) is reduced in an LDS
 // table of those nodes' bins; other groups add per position.
 #define PT_BIN_SPAN 8
-#define PT_BIN_NODES 8
-// LDS words of k_bin for nb bins: the one-node table, the PT_BIN_NODES-node table and
-// the group's lowest and highest node
-inline size_t bin_lds_words(int nb) { return (size_t)7 * nb * (1 + PT_BIN_NODES) + 2; }
+// Nodes of the many-node table for nb bins: up to 32, within 16 KB of LDS.
+inline int bin_table_nodes(int nb) { return std::max(1, std::min(32, 16384 / (28 * nb))); }
+// LDS words of k_bin for nb bins: the one-node table, the many-node table and the group's
+// lowest and highest node
+inline size_t bin_lds_words(int nb) { return (size_t)7 * nb * (1 + bin_table_nodes(nb)) + 2; }
 
 // Adds the bin tables of nodes [node0, node0 + count) held in lds (7 * nb words per node:
 // 6 keys per bin, then a count per bin) to the global bins.
@@ -270,8 +272,8 @@ __global__ __launch_bounds__(256) void k_bin(LevelArgs a)
     extern __shared__ uint32_t lds[];
     const int nb = a.num_bins;
     uint32_t* one = lds;                                   // the one-node table
-    uint32_t* many = lds + 7 * nb;                         // the PT_BIN_NODES-node table
-    int* grp = (int*)(lds + 7 * nb * (1 + PT_BIN_NODES)); // lowest, highest node of a group
+    uint32_t* many = lds + 7 * nb;                          // the many-node table
+    int* grp = (int*)(lds + 7 * nb * (1 + a.bin_nodes)); // lowest, highest node of a group
     const int first = (int)blockIdx.x * (int)blockDim.x * PT_BIN_SPAN;
     int one_node = -1; // block-uniform: the node whose bins the one-node table holds
     for (int it = 0; it < PT_BIN_SPAN; ++it) {
@@ -328,7 +330,7 @@ __global__ __launch_bounds__(256) void k_bin(LevelArgs a)
         }
         __syncthreads();
         const int lo_node = grp[0], count = grp[1] - grp[0] + 1;
-        if (count >= 1 && count <= PT_BIN_NODES) {
+        if (count >= 1 && count <= a.bin_nodes) {
             bin_clear(many, count * 7 * nb, nb);
             __syncthreads();
             if (b >= 0) bin_add(many + (pn - lo_node) * 7 * nb, nb, b, blo, bhi);
@@ -969,6 +971,7 @@ hipError_t gpu_build_bvh(const float* tri_pos_d, int n, int num_bins, int leaf_m
         la.pos_node = pos_node[cur];
         la.n = n;
         la.num_bins = num_bins;
+        la.bin_nodes = bin_table_nodes(num_bins);
         la.act = act[cur];
         la.cbk = cbk;
         la.len = len;
