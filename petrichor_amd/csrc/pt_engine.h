@@ -86,6 +86,10 @@ struct pt_engine {
     int width = 0;             // BVH width of the jobs a multi-job engine runs (0: none yet)
     pt::LaunchTuning tune;     // launch shapes (captured in the graphs)
     int iters_per_sync = 8;    // default graph replays between host synchronisations
+    // Single-job engines: once at most this many paths are live at a synchronisation, the
+    // rest of the render runs as one path-completion launch (pt::launch_finish_paths).
+    int finish_threshold = 1 << 17;
+    int live_h = 0;            // live paths at the last synchronisation
 
     pt::Queue queue(int s, int kind) const
     {
@@ -365,7 +369,53 @@ struct pt_engine {
             ++iterations;
         }
         PT_HIP(ctx, hipMemcpyAsync(done_h.data(), done_d, sizeof(int) * (size_t)max_jobs, hipMemcpyDeviceToHost, st));
+        PT_HIP(ctx, hipMemcpyAsync(&live_h, counters + 4 * set + 3, sizeof(int), hipMemcpyDeviceToHost, st));
         PT_HIP(ctx, hipStreamSynchronize(st));
+        return PT_OK;
+    }
+
+    // Single-job engines: traces the queued rays and then completes every live path in
+    // one launch, and synchronises as run() does.
+    int finish_live()
+    {
+        PT_HIP(ctx, hipSetDevice(ctx->device));
+        const int r = set;
+        pt::TraceJob tj;
+        tj.num_seg = 3;
+        float* outs_t[3] = {hit_t, nullptr, sh_t};
+        int* outs_k[3] = {hit_k, sh_k, sh_k};
+        for (int k = 0; k < 3; ++k) {
+            pt::TraceSegment& g = tj.seg[k];
+            g.rays = q_rays[k];
+            g.path = q_path[k];
+            g.count = counters + 4 * r + k;
+            g.any = k == 1;
+            g.map_prim = 0;
+            g.hit_flag = 0;
+            g.out_t = outs_t[k];
+            g.out_k = outs_k[k];
+        }
+        tj.views = nullptr;
+        tj.job_of_path = nullptr;
+        tj.queries = queries_d;
+        PT_HIP(ctx, pt::launch_zero(counters + PT_CURSOR_STRIDE, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
+        PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + PT_CURSOR_STRIDE, nullptr, st, tune));
+        pt::ShadeArgs a{};
+        a.ps = ps;
+        a.active_in = act[r];
+        a.active_count = counters + 4 * r + 3;
+        a.hit_t = hit_t;
+        a.hit_k = hit_k;
+        a.sh_t = sh_t;
+        a.sh_k = sh_k;
+        const pt::HitBuffers h{hit_t, hit_k, sh_t, sh_k};
+        PT_HIP(ctx, pt::launch_finish_paths(integrator1, sc1, job1, a, h, queries_d, st));
+        // no live path and no queued ray is left: the counters of this set start the next job
+        PT_HIP(ctx, pt::launch_zero(counters + 4 * r, 4, st));
+        ++iterations;
+        PT_HIP(ctx, hipMemcpyAsync(done_h.data(), done_d, sizeof(int) * (size_t)max_jobs, hipMemcpyDeviceToHost, st));
+        PT_HIP(ctx, hipStreamSynchronize(st));
+        live_h = 0;
         return PT_OK;
     }
 

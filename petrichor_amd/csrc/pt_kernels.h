@@ -160,4 +160,19 @@ struct ShadeArgs {
 hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job, const JobDev* jobs,
                         const ShadeArgs& a, hipStream_t st, const LaunchTuning& tune = LaunchTuning());
 
+// The hit buffers of ShadeArgs, writable.
+struct HitBuffers {
+    float* hit_t;
+    int* hit_k;
+    float* sh_t;
+    int* sh_k;
+};
+// Runs the live paths a.active_in[0 .. *a.active_count) of a single-job render to the end
+// of their pixels, one thread per path (shading events and the rays they ask for, in the
+// wavefront's order, with the same results); adds the rays traced to *queries. The grid
+// holds PT_FINISH_BLOCKS_PER_CU blocks of 256 per CU, looping over the paths.
+#define PT_FINISH_BLOCKS_PER_CU 4
+hipError_t launch_finish_paths(int integrator, const SceneView& sc, const JobView& job, const ShadeArgs& a,
+                               const HitBuffers& h, unsigned long long* queries, hipStream_t st);
+
 } // namespace pt

@@ -5,6 +5,7 @@
 
 #include "pt_device.h"
 #include "pt_kernels.h"
+#include "pt_traverse.h"
 
 namespace pt {
 
@@ -564,6 +565,71 @@ __global__ void __launch_bounds__(256) k_shade_multi(const JobDev* jobs, ShadeAr
     shade_live_paths<0, true>(SceneView{}, JobView{}, jobs, A);
 }
 
+// One ray traced by one thread with the walk of the trace kernel; t and k as the trace
+// kernel writes them for the path queues (leaf slot, -1 on a miss).
+template <bool WIDE>
+__device__ __forceinline__ void trace_single(const BvhView& b, V3 o, float dmin, V3 d, float dmax, bool any,
+                                             uint2* lds, StackEntry* spill, float* t, int* k)
+{
+    const float4 r0 = make_float4(o.x, o.y, o.z, dmin), r1 = make_float4(d.x, d.y, d.z, dmax);
+    TraceRay r;
+    TraceState s;
+    if (trace_begin(b, r0, r1, any, r, s)) {
+        while (!(WIDE ? trace8_step<false>(b, r, s, lds, spill) : trace_step<false>(b, r, s, lds, spill))) {
+        }
+    }
+    *t = s.best_t;
+    *k = s.best;
+}
+
+// Completion of the live paths of a single-job render, one thread per path: every
+// shading event is followed at once by tracing its rays in the same thread, until the
+// pixel has all its samples. The events, their order and every result are those of the
+// wavefront iterations; what changes is that a path no longer waits for the slowest ray
+// of each iteration, which near the end of a frame (few live paths, each iteration as
+// long as its longest ray) is where the time goes.
+template <int INTEGRATOR, bool WIDE>
+__global__ void __launch_bounds__(256) k_finish_paths(SceneView sc, JobView job, ShadeArgs A, HitBuffers H,
+                                                      unsigned long long* queries)
+{
+    __shared__ uint2 lds_stack[lds_stack_entries<WIDE>() * 256];
+    uint2* lds = lds_stack + threadIdx.x;
+    StackEntry spill[(WIDE ? PT_STACK8 : PT_MAX_STACK) - lds_stack_entries<WIDE>()];
+    const int n = *A.active_count;
+    unsigned long long q = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int p = A.active_in[i];
+        for (;;) {
+            ShadeOut o;
+            o.keep = o.want_ext = o.want_any = o.want_closest = false;
+            o.done = nullptr;
+            shade_one<INTEGRATOR>(sc, job, A, p, o);
+            if (o.done) atomicAdd(o.done, 1);
+            if (!o.keep) break;
+            float t;
+            int k;
+            if (o.want_any) {
+                trace_single<WIDE>(sc.bvh, o.sh_o, 0.0f, o.sh_d, FLT_MAX, true, lds, spill, &t, &k);
+                H.sh_k[p] = k;
+                ++q;
+            }
+            if (o.want_closest) {
+                trace_single<WIDE>(sc.bvh, o.sh_o, 0.0f, o.sh_d, FLT_MAX, false, lds, spill, &t, &k);
+                H.sh_t[p] = t;
+                H.sh_k[p] = k;
+                ++q;
+            }
+            if (o.want_ext) {
+                trace_single<WIDE>(sc.bvh, o.ext_o, o.ext_dmin, o.ext_d, o.ext_dmax, false, lds, spill, &t, &k);
+                H.hit_t[p] = t;
+                H.hit_k[p] = k;
+                ++q;
+            }
+        }
+    }
+    if (q) atomicAdd(queries, q);
+}
+
 __global__ void k_zero_counters(int* c, int n, int stride)
 {
     const int i = threadIdx.x;
@@ -654,6 +720,21 @@ hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job,
     if (jobs) hipLaunchKernelGGL(k_shade_multi, grid, block, 0, st, jobs, a);
     else if (integrator == 0) hipLaunchKernelGGL((k_shade<0>), grid, block, 0, st, sc, job, a);
     else hipLaunchKernelGGL((k_shade<1>), grid, block, 0, st, sc, job, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish_paths(int integrator, const SceneView& sc, const JobView& job, const ShadeArgs& a,
+                               const HitBuffers& h, unsigned long long* queries, hipStream_t st)
+{
+    const dim3 grid((unsigned)(cu_count() * PT_FINISH_BLOCKS_PER_CU)), block(256);
+    const bool wide = sc.bvh.nodes8 != nullptr;
+    if (integrator == 0) {
+        if (wide) hipLaunchKernelGGL((k_finish_paths<0, true>), grid, block, 0, st, sc, job, a, h, queries);
+        else hipLaunchKernelGGL((k_finish_paths<0, false>), grid, block, 0, st, sc, job, a, h, queries);
+    } else {
+        if (wide) hipLaunchKernelGGL((k_finish_paths<1, true>), grid, block, 0, st, sc, job, a, h, queries);
+        else hipLaunchKernelGGL((k_finish_paths<1, false>), grid, block, 0, st, sc, job, a, h, queries);
+    }
     return hipGetLastError();
 }
 

@@ -1,394 +1,17 @@
-// pt_trace.hip — BVH traversal for gfx950: closest hit and any hit over the binary
-// tree in the reference's topology.
-//
-// Walk order and culling are the reference's (BinnedSAHBVH::Intersect,
-// Core/Accel/BinnedSAHBVH.cpp:272-374): from an inner node the child whose box centroid
-// is nearer the ray origin (squared distance, AABB::CalcCentroid, AABB.h:70-74) is
-// visited first, the other is pushed with its slab distance; a popped child is skipped
-// when a hit exists, the origin is not strictly inside the child's box and the hit is
-// nearer than the child's slab distance; leaf triangles are tested in leaf order and a
-// hit replaces the best one only when strictly nearer. The reference pushes children
-// whose box the ray misses and rejects them when popped; they are not pushed here.
-//
-// The 8-wide traversal (BVHs built with width 8, nodes in pt_wide.hip) tests the up to 8
-// child boxes of a node against [max(dist_min, 0), min(nearest hit, dist_max)], visits
-// the nearest hit child next and pushes the others farthest first with their entry
-// distance; a popped entry is skipped when the nearest hit is closer than its entry
-// distance. Leaves, triangle tests and the hit rule are the reference's, so it finds the
-// same nearest hit, and of several hits at exactly the same distance it keeps the one
-// the reference's walk reaches first (reference_first); only rays that graze a box
-// within rounding of their hit can differ.
+// pt_trace.hip — the persistent trace kernel over the ray queues (closest hit and any
+// hit; the walk itself is in pt_traverse.h).
 //
 // Execution: persistent waves. A wave takes rays in chunks from the cursor of its XCD's
 // part of the queues (then from the other parts) and refills its lanes as rays finish,
-// so lanes do not wait for the longest ray of the wave. A step handles one inner node or
-// one triangle. Each lane keeps
-// its traversal stack in LDS (the first lds_stack_entries entries) with deeper entries in
-// private memory; the LDS size per block sets the blocks per CU.
+// so lanes do not wait for the longest ray of the wave. The LDS size per block (the
+// traversal stacks) sets the blocks per CU.
 #include <hip/hip_runtime.h>
 
 #include "pt_device.h"
 #include "pt_kernels.h"
+#include "pt_traverse.h"
 
 namespace pt {
-
-// LDS stack entries per lane. Binary: 10 (20 KB per 256-thread block, 8 blocks or 32
-// waves per CU). 8-wide: 16 (32 KB, 5 blocks per CU, which its registers allow anyway),
-// as its stacks grow by up to 7 entries per node.
-template <bool WIDE>
-constexpr int lds_stack_entries()
-{
-    return WIDE ? 16 : 10;
-}
-#define PT_TRACE_BLOCK 256
-#define PT_STACK8 192 // 8-wide traversal: at most 7 pending children per level
-
-struct StackEntry {
-    uint32_t ref; // PT_REF_* word with the inside bit
-    float d;      // slab distance of that child's box
-};
-
-struct TraceRay {
-    V3 o, d, inv;
-    float dmin, dmax;
-    float tnear; // max(dist_min, 0): where child boxes start to matter (8-wide traversal)
-    int sx, sy, sz;
-    bool any;
-};
-
-struct TraceState {
-    int best;     // leaf slot of the nearest hit so far, -1 = none
-    float best_t;
-    uint32_t cur; // entry being processed
-    float cur_d;
-    int sp;
-    uint32_t nv, nt; // node visits (slab tests) and triangle tests
-    uint32_t leaf_k, leaf_end; // triangles of the current leaf still to test
-};
-
-template <bool WIDE>
-__device__ __forceinline__ void stack_push(uint2* lds, StackEntry* spill, int sp, uint32_t ref, float d)
-{
-    constexpr int NL = lds_stack_entries<WIDE>();
-    if (sp < NL) {
-        lds[sp * PT_TRACE_BLOCK] = make_uint2(ref, __float_as_uint(d));
-    } else {
-        spill[sp - NL].ref = ref;
-        spill[sp - NL].d = d;
-    }
-}
-
-template <bool WIDE>
-__device__ __forceinline__ void stack_pop(const uint2* lds, const StackEntry* spill, int sp, uint32_t* ref,
-                                          float* d)
-{
-    constexpr int NL = lds_stack_entries<WIDE>();
-    if (sp < NL) {
-        const uint2 e = lds[sp * PT_TRACE_BLOCK];
-        *ref = e.x;
-        *d = __uint_as_float(e.y);
-    } else {
-        *ref = spill[sp - NL].ref;
-        *d = spill[sp - NL].d;
-    }
-}
-
-// Starts a ray at the root: returns false when the root box is missed (or the scene is
-// empty), which finishes the ray with no hit.
-__device__ __forceinline__ bool trace_begin(const BvhView& b, const float4 r0, const float4 r1, bool any,
-                                            TraceRay& r, TraceState& s)
-{
-    r.o = v3(r0.x, r0.y, r0.z);
-    r.d = v3(r1.x, r1.y, r1.z);
-    r.dmin = r0.w;
-    r.dmax = r1.w;
-    r.tnear = fmaxf(r0.w, 0.0f);
-    r.any = any;
-    r.inv = v3(1.0f, 1.0f, 1.0f) / r.d; // Vector3f::One() / ray.dir (BinnedSAHBVH.cpp:280)
-    r.sx = r.inv.x < 0.0f;
-    r.sy = r.inv.y < 0.0f;
-    r.sz = r.inv.z < 0.0f;
-    s.best = -1;
-    s.best_t = 0.0f;
-    s.sp = 0;
-    s.nv = 0;
-    s.nt = 0;
-    s.leaf_k = s.leaf_end = 0;
-    if (b.root_ref == PT_REF_EMPTY) return false;
-    s.nv = 1;
-    s.cur = b.nodes8 ? b.root8 : b.root_ref;
-    return slab(r.o, r.inv, r.sx, r.sy, r.sz, b.root_lo, b.root_hi, &s.cur_d);
-}
-
-// Whether the reference's walk reaches leaf slot a before leaf slot c, both hit by the
-// ray: below their lowest common ancestor in the binary tree it descends first into the
-// child whose box centroid is nearer the ray origin (and finishes that subtree before
-// the other); inside one leaf the slots are tested in order. Used by the 8-wide
-// traversal to break exact distance ties the way the reference does.
-__device__ __noinline__ bool reference_first(const BvhView& b, const TraceRay& r, int a, int c)
-{
-    uint32_t ref = b.root_ref;
-    for (int depth = 0; depth < PT_MAX_STACK; ++depth) {
-        if (pt_ref_cnt(ref) != 0) return a < c;
-        const float4* nd = b.nodes + 4 * (size_t)pt_ref_index(ref);
-        const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], m = nd[3];
-        const uint32_t ml = __float_as_uint(m.x), mr = __float_as_uint(m.y);
-        int lb, le;
-        if (pt_ref_cnt(ml) == 0) {
-            const int2 rg = b.inner_range[pt_ref_index(ml)];
-            lb = rg.x;
-            le = rg.y;
-        } else if (pt_ref_cnt(ml) == PT_REF_BIG) {
-            const int2 bl = b.big_leaves[pt_ref_index(ml)];
-            lb = bl.x;
-            le = bl.x + bl.y;
-        } else {
-            lb = (int)pt_ref_index(ml);
-            le = lb + (int)pt_ref_cnt(ml);
-        }
-        const bool a_left = a >= lb && a < le, c_left = c >= lb && c < le;
-        if (a_left == c_left) {
-            ref = a_left ? ml : mr;
-            continue;
-        }
-        const V3 llo = v3(q0.x, q0.y, q0.z), lhi = v3(q0.w, q1.x, q1.y);
-        const V3 rlo = v3(q1.z, q1.w, q2.x), rhi = v3(q2.y, q2.z, q2.w);
-        const float dl = sq_length(0.5f * (llo + lhi) - r.o);
-        const float dr = sq_length(0.5f * (rlo + rhi) - r.o);
-        return a_left == (dl < dr);
-    }
-    return a < c;
-}
-
-// Tests triangle k (the reference's hit rule); true when an any-hit ray has found its
-// hit. The 8-wide traversal meets leaves out of the reference's order, so an exactly
-// equal distance goes to the triangle the reference would have reached first.
-template <bool STATS, bool WIDE>
-__device__ __forceinline__ bool tri_test(const BvhView& b, const TraceRay& r, TraceState& s, uint32_t k)
-{
-    if (STATS) ++s.nt;
-    const float4 t0 = b.tris[3 * (size_t)k];
-    const float4 t1 = b.tris[3 * (size_t)k + 1];
-    const float4 t2 = b.tris[3 * (size_t)k + 2];
-    TriDev tr;
-    tr.p0 = v3(t0.x, t0.y, t0.z);
-    tr.e1 = v3(t0.w, t1.x, t1.y);
-    tr.e2 = v3(t1.z, t1.w, t2.x);
-    tr.n = v3(t2.y, t2.z, t2.w);
-    float t;
-    if (tri_intersect(tr, r.o, r.d, &t) && !(t < r.dmin || t > r.dmax)) {
-        if (s.best < 0 || t < s.best_t) {
-            s.best = (int)k;
-            s.best_t = t;
-            if (r.any) return true;
-        } else if (WIDE && t == s.best_t && reference_first(b, r, (int)k, s.best)) {
-            s.best = (int)k;
-        }
-    }
-    return false;
-}
-
-// Enters the leaf s.cur: its triangles are tested in leaf order from the next call of
-// leaf_or_pop on.
-__device__ __forceinline__ void enter_leaf(const BvhView& b, TraceState& s)
-{
-    uint32_t first = pt_ref_index(s.cur), cnt = pt_ref_cnt(s.cur);
-    if (cnt == PT_REF_BIG) {
-        const int2 bl = b.big_leaves[first];
-        first = (uint32_t)bl.x;
-        cnt = (uint32_t)bl.y;
-    }
-    s.leaf_k = first;
-    s.leaf_end = first + cnt;
-}
-
-// End of a step: one triangle of the current leaf (so that lanes in leaves and lanes in
-// inner nodes do similar work per step), then, when the step did not move to a child
-// (`next`) and no triangle is left, the next entry from the stack. Returns true when the
-// ray is finished.
-template <bool STATS, bool WIDE>
-__device__ __forceinline__ bool leaf_or_pop(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
-                                            StackEntry* spill, bool next)
-{
-    if (s.leaf_k < s.leaf_end) {
-        if (WIDE) { // a wide node's work per step is larger: whole leaves keep steps alike
-            for (; s.leaf_k < s.leaf_end; ++s.leaf_k)
-                if (tri_test<STATS, WIDE>(b, r, s, s.leaf_k)) return true;
-        } else {
-            if (tri_test<STATS, WIDE>(b, r, s, s.leaf_k++)) return true;
-            if (s.leaf_k < s.leaf_end) return false;
-        }
-        next = false;
-    }
-    if (!next) {
-        if (s.sp == 0) return true;
-        --s.sp;
-        stack_pop<WIDE>(lds, spill, s.sp, &s.cur, &s.cur_d);
-    }
-    return false;
-}
-
-// One traversal step: an inner node or triangles of a leaf, then the next entry from the
-// stack when the current one is done. Returns true when the ray is finished.
-template <bool STATS>
-__device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
-                                           StackEntry* spill)
-{
-    bool next = false;
-    if (s.leaf_k < s.leaf_end) {
-        // in the middle of a leaf
-    } else if (s.best >= 0 && !(s.cur & PT_REF_INSIDE) && s.best_t < s.cur_d) {
-        // culled: a nearer hit exists and the origin is outside this box
-    } else if (pt_ref_cnt(s.cur) != 0) {
-        enter_leaf(b, s);
-    } else {
-        const float4* nd = b.nodes + 4 * (size_t)pt_ref_index(s.cur);
-        const float4 a = nd[0], bb = nd[1], c = nd[2], mf = nd[3];
-        const uint32_t ml = __float_as_uint(mf.x), mr = __float_as_uint(mf.y);
-        const V3 llo = v3(a.x, a.y, a.z), lhi = v3(a.w, bb.x, bb.y);
-        const V3 rlo = v3(bb.z, bb.w, c.x), rhi = v3(c.y, c.z, c.w);
-        if (STATS) s.nv += 2;
-        const float dl = sq_length(0.5f * (llo + lhi) - r.o);
-        const float dr = sq_length(0.5f * (rlo + rhi) - r.o);
-        float tl, tr_;
-        const bool hl = slab(r.o, r.inv, r.sx, r.sy, r.sz, llo, lhi, &tl);
-        const bool hr = slab(r.o, r.inv, r.sx, r.sy, r.sz, rlo, rhi, &tr_);
-        const uint32_t el = ml | (box_contains(llo, lhi, r.o) ? PT_REF_INSIDE : 0u);
-        const uint32_t er = mr | (box_contains(rlo, rhi, r.o) ? PT_REF_INSIDE : 0u);
-        // the child the reference pops first: left if dl < dr, else right
-        const bool left_first = dl < dr;
-        const bool h1 = left_first ? hl : hr, h2 = left_first ? hr : hl;
-        const uint32_t e1 = left_first ? el : er, e2 = left_first ? er : el;
-        const float d1 = left_first ? tl : tr_, d2 = left_first ? tr_ : tl;
-        if (h1) {
-            if (h2) {
-                stack_push<false>(lds, spill, s.sp, e2, d2);
-                ++s.sp;
-            }
-            s.cur = e1;
-            s.cur_d = d1;
-            next = true;
-        } else if (h2) {
-            s.cur = e2;
-            s.cur_d = d2;
-            next = true;
-        }
-    }
-    return leaf_or_pop<STATS, false>(b, r, s, lds, spill, next);
-}
-
-__device__ __forceinline__ void sort_pair(float& da, uint32_t& ra, float& db, uint32_t& rb)
-{
-    const bool sw = db < da;
-    const float lo_d = sw ? db : da, hi_d = sw ? da : db;
-    const uint32_t lo_r = sw ? rb : ra, hi_r = sw ? ra : rb;
-    da = lo_d;
-    ra = lo_r;
-    db = hi_d;
-    rb = hi_r;
-}
-
-__device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); }
-
-// One step of the 8-wide traversal: the current entry (a wide node or a leaf), then the
-// next entry from the stack. Returns true when the ray is finished.
-template <bool STATS>
-__device__ __forceinline__ bool trace8_step(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
-                                            StackEntry* spill)
-{
-    bool next = false;
-    if (s.leaf_k < s.leaf_end) {
-        // in the middle of a leaf
-    } else if (s.best >= 0 && s.best_t < s.cur_d) {
-        // culled: the box starts beyond the nearest hit
-    } else if (pt_ref_cnt(s.cur) != 0) {
-        enter_leaf(b, s);
-    } else {
-        const uint4* nd = b.nodes8 + 8 * (size_t)pt_ref_index(s.cur);
-        const uint4 w0 = nd[0], w1 = nd[1], w2 = nd[2], w3 = nd[3], w4 = nd[4], w5 = nd[5];
-        const int cnt = (int)(w0.w >> 24);
-        // the ray in the node's quantised frame: t = q * (inv * step) + (p - o) * inv
-        const float ix = r.inv.x * __uint_as_float((w0.w & 0xffu) << 23);
-        const float iy = r.inv.y * __uint_as_float(((w0.w >> 8) & 0xffu) << 23);
-        const float iz = r.inv.z * __uint_as_float(((w0.w >> 16) & 0xffu) << 23);
-        const float ox = (__uint_as_float(w0.x) - r.o.x) * r.inv.x;
-        const float oy = (__uint_as_float(w0.y) - r.o.y) * r.inv.y;
-        const float oz = (__uint_as_float(w0.z) - r.o.z) * r.inv.z;
-        // entry and exit planes by ray octant (qlo: w1.xy, w1.zw, w2.xy; qhi: w2.zw, w3.xy, w3.zw)
-        const uint32_t nx0 = r.sx ? w2.z : w1.x, nx1 = r.sx ? w2.w : w1.y;
-        const uint32_t fx0 = r.sx ? w1.x : w2.z, fx1 = r.sx ? w1.y : w2.w;
-        const uint32_t ny0 = r.sy ? w3.x : w1.z, ny1 = r.sy ? w3.y : w1.w;
-        const uint32_t fy0 = r.sy ? w1.z : w3.x, fy1 = r.sy ? w1.w : w3.y;
-        const uint32_t nz0 = r.sz ? w3.z : w2.x, nz1 = r.sz ? w3.w : w2.y;
-        const uint32_t fz0 = r.sz ? w2.x : w3.z, fz1 = r.sz ? w2.y : w3.w;
-        const float tfar = s.best >= 0 ? fminf(s.best_t, r.dmax) : r.dmax;
-        float d[8];
-        uint32_t ref[8] = {w4.x, w4.y, w4.z, w4.w, w5.x, w5.y, w5.z, w5.w};
-        int hits = 0;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = i & 3;
-            const float tx0 = fmaf(byte_f(i < 4 ? nx0 : nx1, k), ix, ox);
-            const float ty0 = fmaf(byte_f(i < 4 ? ny0 : ny1, k), iy, oy);
-            const float tz0 = fmaf(byte_f(i < 4 ? nz0 : nz1, k), iz, oz);
-            const float tx1 = fmaf(byte_f(i < 4 ? fx0 : fx1, k), ix, ox);
-            const float ty1 = fmaf(byte_f(i < 4 ? fy0 : fy1, k), iy, oy);
-            const float tz1 = fmaf(byte_f(i < 4 ? fz0 : fz1, k), iz, oz);
-            const float tmin = fmaxf(fmaxf(tx0, ty0), fmaxf(tz0, r.tnear));
-            const float tmax = fminf(fminf(tx1, ty1), fminf(tz1, tfar));
-            const bool hit = i < cnt && tmin <= tmax;
-            d[i] = hit ? tmin : __builtin_inff();
-            hits += hit ? 1 : 0;
-        }
-        if (STATS) s.nv += 1; // one 128-byte node record
-        // nearest first: 19-comparator sorting network for 8 keys
-        sort_pair(d[0], ref[0], d[2], ref[2]);
-        sort_pair(d[1], ref[1], d[3], ref[3]);
-        sort_pair(d[4], ref[4], d[6], ref[6]);
-        sort_pair(d[5], ref[5], d[7], ref[7]);
-        sort_pair(d[0], ref[0], d[4], ref[4]);
-        sort_pair(d[1], ref[1], d[5], ref[5]);
-        sort_pair(d[2], ref[2], d[6], ref[6]);
-        sort_pair(d[3], ref[3], d[7], ref[7]);
-        sort_pair(d[0], ref[0], d[1], ref[1]);
-        sort_pair(d[2], ref[2], d[3], ref[3]);
-        sort_pair(d[4], ref[4], d[5], ref[5]);
-        sort_pair(d[6], ref[6], d[7], ref[7]);
-        sort_pair(d[2], ref[2], d[4], ref[4]);
-        sort_pair(d[3], ref[3], d[5], ref[5]);
-        sort_pair(d[1], ref[1], d[4], ref[4]);
-        sort_pair(d[3], ref[3], d[6], ref[6]);
-        sort_pair(d[1], ref[1], d[2], ref[2]);
-        sort_pair(d[3], ref[3], d[4], ref[4]);
-        sort_pair(d[5], ref[5], d[6], ref[6]);
-#pragma unroll
-        for (int j = 7; j >= 1; --j) {
-            if (j < hits) {
-                stack_push<true>(lds, spill, s.sp, ref[j], d[j]);
-                ++s.sp;
-            }
-        }
-        if (hits > 0) {
-            s.cur = ref[0];
-            s.cur_d = d[0];
-            next = true;
-        }
-    }
-    return leaf_or_pop<STATS, true>(b, r, s, lds, spill, next);
-}
-
-__device__ __forceinline__ void trace_finish(const BvhView& b, const TraceSegment& seg, int i, const TraceState& s)
-{
-    const int p = seg.path ? seg.path[i] : i;
-    if (seg.hit_flag) {
-        seg.out_k[p] = s.best >= 0 ? 1 : 0;
-        return;
-    }
-    int k = s.best;
-    if (seg.map_prim && k >= 0) k = b.prim_ids[k];
-    if (seg.out_t) seg.out_t[p] = s.best >= 0 ? s.best_t : (seg.map_prim ? __builtin_inff() : s.best_t);
-    seg.out_k[p] = k;
-}
 
 // Part x of a segment of n rays: [min(x q, n), min(x q + q, n)) with q = ceil(n / parts).
 __device__ __forceinline__ void part_range(int n, int x, int* lo, int* hi)
