@@ -116,8 +116,8 @@ int run_jobs(pt_engine& E, std::vector<EngineJob>& jobs, int iters_per_sync, pt_
         int rc = E.run(iters_per_sync);
         if (rc != PT_OK) return rc;
         ++replays;
-        // a single-job render near its end: the remaining paths in one launch
-        if (!E.multi && pending.empty() && E.live_h > 0 && E.live_h <= E.finish_threshold) {
+        // every job admitted and few paths left: the remaining paths in one launch
+        if (pending.empty() && E.live_h > 0 && E.live_h <= E.finish_threshold) {
             rc = E.finish_live();
             if (rc != PT_OK) return rc;
             for (const EngineJob* j : running)

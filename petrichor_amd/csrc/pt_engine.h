@@ -86,8 +86,8 @@ struct pt_engine {
     int width = 0;             // BVH width of the jobs a multi-job engine runs (0: none yet)
     pt::LaunchTuning tune;     // launch shapes (captured in the graphs)
     int iters_per_sync = 8;    // default graph replays between host synchronisations
-    // Single-job engines: once at most this many paths are live at a synchronisation, the
-    // rest of the render runs as one path-completion launch (pt::launch_finish_paths).
+    // Once every job is admitted and at most this many paths are live at a
+    // synchronisation, the rest runs as one path-completion launch (pt::launch_finish_paths).
     int finish_threshold = 1 << 17;
     int live_h = 0;            // live paths at the last synchronisation
 
@@ -374,8 +374,8 @@ struct pt_engine {
         return PT_OK;
     }
 
-    // Single-job engines: traces the queued rays and then completes every live path in
-    // one launch, and synchronises as run() does.
+    // Traces the queued rays and then completes every live path in one launch, and
+    // synchronises as run() does.
     int finish_live()
     {
         PT_HIP(ctx, hipSetDevice(ctx->device));
@@ -395,8 +395,9 @@ struct pt_engine {
             g.out_t = outs_t[k];
             g.out_k = outs_k[k];
         }
-        tj.views = nullptr;
-        tj.job_of_path = nullptr;
+        tj.views = multi ? views_d : nullptr;
+        tj.wide = multi && width == 8;
+        tj.job_of_path = multi ? ps.job : nullptr;
         tj.queries = queries_d;
         PT_HIP(ctx, pt::launch_zero(counters + PT_CURSOR_STRIDE, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
         PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + PT_CURSOR_STRIDE, nullptr, st, tune));
@@ -409,7 +410,9 @@ struct pt_engine {
         a.sh_t = sh_t;
         a.sh_k = sh_k;
         const pt::HitBuffers h{hit_t, hit_k, sh_t, sh_k};
-        PT_HIP(ctx, pt::launch_finish_paths(integrator1, sc1, job1, a, h, queries_d, st));
+        const bool wide = multi ? width == 8 : sc1.bvh.nodes8 != nullptr;
+        PT_HIP(ctx, pt::launch_finish_paths(integrator1, sc1, job1, multi ? jobs_d : nullptr, wide, a, h, queries_d,
+                                            st));
         // no live path and no queued ray is left: the counters of this set start the next job
         PT_HIP(ctx, pt::launch_zero(counters + 4 * r, 4, st));
         ++iterations;

@@ -167,12 +167,15 @@ struct HitBuffers {
     float* sh_t;
     int* sh_k;
 };
-// Runs the live paths a.active_in[0 .. *a.active_count) of a single-job render to the end
-// of their pixels, one thread per path (shading events and the rays they ask for, in the
-// wavefront's order, with the same results); adds the rays traced to *queries. The grid
-// holds PT_FINISH_BLOCKS_PER_CU blocks of 256 per CU, looping over the paths.
+// Runs the live paths a.active_in[0 .. *a.active_count) to the end of their pixels, one
+// thread per path (shading events and the rays they ask for, in the wavefront's order,
+// with the same results); adds the rays traced to *queries. jobs: the multi-job table (the
+// job of each path, then sc and job are unused), or null for a single-job render. wide:
+// 8-wide BVHs. The grid holds PT_FINISH_BLOCKS_PER_CU blocks of 256 per CU, looping over
+// the paths.
 #define PT_FINISH_BLOCKS_PER_CU 4
-hipError_t launch_finish_paths(int integrator, const SceneView& sc, const JobView& job, const ShadeArgs& a,
-                               const HitBuffers& h, unsigned long long* queries, hipStream_t st);
+hipError_t launch_finish_paths(int integrator, const SceneView& sc, const JobView& job, const JobDev* jobs,
+                               bool wide, const ShadeArgs& a, const HitBuffers& h, unsigned long long* queries,
+                               hipStream_t st);
 
 } // namespace pt

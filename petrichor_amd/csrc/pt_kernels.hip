@@ -588,9 +588,52 @@ __device__ __forceinline__ void trace_single(const BvhView& b, V3 o, float dmin,
 // wavefront iterations; what changes is that a path no longer waits for the slowest ray
 // of each iteration, which near the end of a frame (few live paths, each iteration as
 // long as its longest ray) is where the time goes.
+// Events of path p from its current hits to the end of its pixel. Returns the rays traced.
 template <int INTEGRATOR, bool WIDE>
-__global__ void __launch_bounds__(256) k_finish_paths(SceneView sc, JobView job, ShadeArgs A, HitBuffers H,
-                                                      unsigned long long* queries)
+__device__ __forceinline__ unsigned long long finish_path(const SceneView& sc, const JobView& job,
+                                                          const ShadeArgs& A, const HitBuffers& H, int p,
+                                                          uint2* lds, StackEntry* spill)
+{
+    unsigned long long q = 0;
+    for (;;) {
+        ShadeOut o;
+        o.keep = o.want_ext = o.want_any = o.want_closest = false;
+        o.done = nullptr;
+        shade_one<INTEGRATOR>(sc, job, A, p, o);
+        if (o.done) atomicAdd(o.done, 1);
+        if (!o.keep) break;
+        float t;
+        int k;
+        if (o.want_any) {
+            trace_single<WIDE>(sc.bvh, o.sh_o, 0.0f, o.sh_d, FLT_MAX, true, lds, spill, &t, &k);
+            H.sh_k[p] = k;
+            ++q;
+        }
+        if (o.want_closest) {
+            trace_single<WIDE>(sc.bvh, o.sh_o, 0.0f, o.sh_d, FLT_MAX, false, lds, spill, &t, &k);
+            H.sh_t[p] = t;
+            H.sh_k[p] = k;
+            ++q;
+        }
+        if (o.want_ext) {
+            trace_single<WIDE>(sc.bvh, o.ext_o, o.ext_dmin, o.ext_d, o.ext_dmax, false, lds, spill, &t, &k);
+            H.hit_t[p] = t;
+            H.hit_k[p] = k;
+            ++q;
+        }
+    }
+    return q;
+}
+
+// Completion of the live paths of a render, one thread per path: every shading event is
+// followed at once by tracing its rays in the same thread, until the pixel has all its
+// samples. The events, their order and every result are those of the wavefront
+// iterations; what changes is that a path no longer waits for the slowest ray of each
+// iteration, which near the end of a render (few live paths, each iteration as long as
+// its longest ray) is where the time goes. Multi-job launches read each path's job.
+template <int INTEGRATOR, bool WIDE, bool MULTI>
+__global__ void __launch_bounds__(256) k_finish_paths(SceneView sc1, JobView job1, const JobDev* jobs, ShadeArgs A,
+                                                      HitBuffers H, unsigned long long* queries)
 {
     __shared__ uint2 lds_stack[lds_stack_entries<WIDE>() * 256];
     uint2* lds = lds_stack + threadIdx.x;
@@ -599,32 +642,12 @@ __global__ void __launch_bounds__(256) k_finish_paths(SceneView sc, JobView job,
     unsigned long long q = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int p = A.active_in[i];
-        for (;;) {
-            ShadeOut o;
-            o.keep = o.want_ext = o.want_any = o.want_closest = false;
-            o.done = nullptr;
-            shade_one<INTEGRATOR>(sc, job, A, p, o);
-            if (o.done) atomicAdd(o.done, 1);
-            if (!o.keep) break;
-            float t;
-            int k;
-            if (o.want_any) {
-                trace_single<WIDE>(sc.bvh, o.sh_o, 0.0f, o.sh_d, FLT_MAX, true, lds, spill, &t, &k);
-                H.sh_k[p] = k;
-                ++q;
-            }
-            if (o.want_closest) {
-                trace_single<WIDE>(sc.bvh, o.sh_o, 0.0f, o.sh_d, FLT_MAX, false, lds, spill, &t, &k);
-                H.sh_t[p] = t;
-                H.sh_k[p] = k;
-                ++q;
-            }
-            if (o.want_ext) {
-                trace_single<WIDE>(sc.bvh, o.ext_o, o.ext_dmin, o.ext_d, o.ext_dmax, false, lds, spill, &t, &k);
-                H.hit_t[p] = t;
-                H.hit_k[p] = k;
-                ++q;
-            }
+        if (MULTI) {
+            const JobDev* J = jobs + A.ps.job[p];
+            if (J->job.integrator == 0) q += finish_path<0, WIDE>(J->sc, J->job, A, H, p, lds, spill);
+            else q += finish_path<1, WIDE>(J->sc, J->job, A, H, p, lds, spill);
+        } else {
+            q += finish_path<INTEGRATOR, WIDE>(sc1, job1, A, H, p, lds, spill);
         }
     }
     if (q) atomicAdd(queries, q);
@@ -723,18 +746,24 @@ hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job,
     return hipGetLastError();
 }
 
-hipError_t launch_finish_paths(int integrator, const SceneView& sc, const JobView& job, const ShadeArgs& a,
-                               const HitBuffers& h, unsigned long long* queries, hipStream_t st)
+hipError_t launch_finish_paths(int integrator, const SceneView& sc, const JobView& job, const JobDev* jobs,
+                               bool wide, const ShadeArgs& a, const HitBuffers& h, unsigned long long* queries,
+                               hipStream_t st)
 {
     const dim3 grid((unsigned)(cu_count() * PT_FINISH_BLOCKS_PER_CU)), block(256);
-    const bool wide = sc.bvh.nodes8 != nullptr;
-    if (integrator == 0) {
-        if (wide) hipLaunchKernelGGL((k_finish_paths<0, true>), grid, block, 0, st, sc, job, a, h, queries);
-        else hipLaunchKernelGGL((k_finish_paths<0, false>), grid, block, 0, st, sc, job, a, h, queries);
+#define PT_LAUNCH_FINISH(I, W, M) \
+    hipLaunchKernelGGL((k_finish_paths<I, W, M>), grid, block, 0, st, sc, job, jobs, a, h, queries)
+    if (jobs) {
+        if (wide) PT_LAUNCH_FINISH(0, true, true);
+        else PT_LAUNCH_FINISH(0, false, true);
+    } else if (integrator == 0) {
+        if (wide) PT_LAUNCH_FINISH(0, true, false);
+        else PT_LAUNCH_FINISH(0, false, false);
     } else {
-        if (wide) hipLaunchKernelGGL((k_finish_paths<1, true>), grid, block, 0, st, sc, job, a, h, queries);
-        else hipLaunchKernelGGL((k_finish_paths<1, false>), grid, block, 0, st, sc, job, a, h, queries);
+        if (wide) PT_LAUNCH_FINISH(1, true, false);
+        else PT_LAUNCH_FINISH(1, false, false);
     }
+#undef PT_LAUNCH_FINISH
     return hipGetLastError();
 }
 
