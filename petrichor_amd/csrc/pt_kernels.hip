@@ -582,12 +582,6 @@ __device__ __forceinline__ void trace_single(const BvhView& b, V3 o, float dmin,
     *k = s.best;
 }
 
-// Completion of the live paths of a single-job render, one thread per path: every
-// shading event is followed at once by tracing its rays in the same thread, until the
-// pixel has all its samples. The events, their order and every result are those of the
-// wavefront iterations; what changes is that a path no longer waits for the slowest ray
-// of each iteration, which near the end of a frame (few live paths, each iteration as
-// long as its longest ray) is where the time goes.
 // Events of path p from its current hits to the end of its pixel. Returns the rays traced.
 template <int INTEGRATOR, bool WIDE>
 __device__ __forceinline__ unsigned long long finish_path(const SceneView& sc, const JobView& job,
