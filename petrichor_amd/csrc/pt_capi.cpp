@@ -255,6 +255,7 @@ int pt_ctx_destroy(pt_context* ctx)
 {
     if (!ctx) return PT_ERR_ARG;
     if (ctx->engine) pt_engine_destroy(ctx->engine);
+    if (ctx->engine2) pt_engine_destroy(ctx->engine2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PT_OK;

@@ -26,6 +26,9 @@
 
 using namespace pt;
 
+// Images of at least this many pixels render in two halves (run_halves).
+static constexpr int64_t kHalvesMinPixels = (int64_t)1 << 19;
+
 namespace pt {
 
 int check_job(pt_context* ctx, const pt_job_desc& j)
@@ -152,6 +155,64 @@ int run_jobs(pt_engine& E, std::vector<EngineJob>& jobs, int iters_per_sync, pt_
     return PT_OK;
 }
 
+// A large single-job render in two halves of its pixels, one per engine and stream, whose
+// launches the GPU overlaps: one half's kernels fill the CUs that the other's leave idle
+// at the end of each launch (the trace launches wait for their slowest rays). Each half is
+// the same job restricted to a pixel range, so every pixel is computed as in one piece.
+int run_halves(pt_context* ctx, pt_engine* E[2], const pt_job_desc& jd, int iters_per_sync, pt_batch_stats* stats)
+{
+    EngineJob full = make_job(jd, 0);
+    int rc = prepare_output(ctx, full);
+    if (rc != PT_OK) return rc;
+    EngineJob h[2] = {full, full};
+    const int64_t half = full.npix / 2;
+    h[0].npix = half;
+    h[1].npix = full.npix - half;
+    h[1].pixel_base = half;
+    h[0].own_out = h[1].own_out = false; // the image belongs to `full`
+    // the engines' streams must not start before the output exists (ctx stream work done)
+    PT_HIP(ctx, hipDeviceSynchronize());
+    const auto t0 = std::chrono::steady_clock::now();
+    bool done[2] = {false, false};
+    for (int i = 0; i < 2; ++i) {
+        E[i]->iterations = 0;
+        PT_HIP(ctx, hipMemsetAsync(E[i]->queries_d, 0, sizeof(unsigned long long), E[i]->st));
+        rc = E[i]->start(h[i]);
+        if (rc != PT_OK) return rc;
+        E[i]->live_h = (int)h[i].npix;
+    }
+    while (!done[0] || !done[1]) {
+        for (int i = 0; i < 2; ++i) {
+            if (done[i]) continue;
+            const bool last = E[i]->live_h > 0 && E[i]->live_h <= E[i]->finish_threshold / 2;
+            rc = last ? E[i]->launch_finish() : E[i]->launch(iters_per_sync);
+            if (rc != PT_OK) return rc;
+        }
+        for (int i = 0; i < 2; ++i) {
+            if (done[i]) continue;
+            const bool last = E[i]->live_h > 0 && E[i]->live_h <= E[i]->finish_threshold / 2;
+            rc = E[i]->wait();
+            if (rc != PT_OK) return rc;
+            done[i] = E[i]->done_h[h[i].slot] >= h[i].npix;
+            if (!done[i] && (last || E[i]->live_h == 0))
+                return pt_fail(ctx, PT_ERR_STATE, "render ended with pixels unfinished");
+        }
+    }
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (int i = 0; i < 2; ++i) E[i]->retire(h[i]);
+    rc = finish_output(ctx, full, E[0]->st);
+    if (rc != PT_OK) return rc;
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->queries = E[0]->queries() + E[1]->queries();
+        stats->iterations = std::max(E[0]->iterations, E[1]->iterations);
+        stats->elapsed_ms = wall;
+        stats->wall_ms = wall;
+        stats->syncs = 0;
+    }
+    return PT_OK;
+}
+
 } // namespace pt
 
 extern "C" {
@@ -192,6 +253,32 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
     int rc = check_job(ctx, jd);
     if (rc != PT_OK) return rc;
     const int64_t npix = (int64_t)d->width * d->height;
+    // Large images render in two halves on two engines (streams), whose launches overlap.
+    if (npix >= kHalvesMinPixels && !(d->flags & (PT_RENDER_PLAIN_LAUNCHES | PT_RENDER_TIME_KERNELS))) {
+        const int64_t cap = npix - npix / 2;
+        pt_engine** slots[2] = {&ctx->engine, &ctx->engine2};
+        pt_engine* E2[2];
+        for (int i = 0; i < 2; ++i) {
+            if (!*slots[i] || (*slots[i])->capacity < cap) {
+                delete *slots[i];
+                *slots[i] = nullptr;
+                rc = pt_engine_create(ctx, npix, 1, 0, slots[i]);
+                if (rc != PT_OK) return rc;
+                (*slots[i])->iters_per_sync = 16;
+            }
+            E2[i] = *slots[i];
+        }
+        pt_batch_stats bs;
+        rc = run_halves(ctx, E2, jd, E2[0]->iters_per_sync, &bs);
+        if (rc != PT_OK) return rc;
+        if (stats) {
+            std::memset(stats, 0, sizeof *stats);
+            stats->queries = bs.queries;
+            stats->iterations = bs.iterations;
+            stats->elapsed_ms = bs.elapsed_ms;
+        }
+        return PT_OK;
+    }
     // the context keeps a single-job engine, grown on demand
     if (!ctx->engine || ctx->engine->capacity < npix) {
         delete ctx->engine;

@@ -44,7 +44,8 @@ struct EngineJob {
     bool out_on_device = false;
     float* out_d = nullptr;    // device image (== user_out when on device)
     bool own_out = false;
-    int64_t base = 0, npix = 0;
+    int64_t base = 0, npix = 0; // path slots; the job renders npix pixels from pixel_base
+    int64_t pixel_base = 0;
     int slot = -1;             // job table index
     int64_t bytes = 0;         // working memory charged to the budget
     std::chrono::steady_clock::time_point submitted, started;
@@ -90,6 +91,9 @@ struct pt_engine {
     // synchronisation, the rest runs as one path-completion launch (pt::launch_finish_paths).
     int finish_threshold = 1 << 17;
     int live_h = 0;            // live paths at the last synchronisation
+    // pinned host copies of the finished-pixel counts and the live count, written by
+    // asynchronous copies (so that queueing them does not wait for the stream)
+    int* stage_h = nullptr;
 
     pt::Queue queue(int s, int kind) const
     {
@@ -113,6 +117,7 @@ struct pt_engine {
     {
         drop_graphs();
         if (st) (void)hipStreamDestroy(st);
+        if (stage_h) (void)hipHostFree(stage_h);
     }
 
     int init(pt_context* c, int64_t cap, int jobs, bool multi_job)
@@ -157,6 +162,7 @@ struct pt_engine {
         ALLOC(done_d, (size_t)jobs);
         ALLOC(queries_d, 1);
 #undef ALLOC
+        PT_HIP(ctx, hipHostMalloc((void**)&stage_h, sizeof(int) * ((size_t)jobs + 1)));
         jobs_h.assign(jobs, pt::JobDev{});
         done_h.assign(jobs, 0);
         slot_used.assign(jobs, 0);
@@ -271,6 +277,8 @@ struct pt_engine {
         v.done = done_d + slot;
         v.path_base = (int)j.base;
         v.job_index = slot;
+        v.pixel_base = (int)j.pixel_base;
+        v.num_pixels = (int)j.npix;
         jobs_h[slot] = jd;
         done_h[slot] = 0;
         PT_HIP(ctx, hipMemsetAsync(done_d + slot, 0, sizeof(int), st));
@@ -360,6 +368,13 @@ struct pt_engine {
     // per-job finished-pixel counts.
     int run(int k)
     {
+        const int rc = launch(k);
+        return rc != PT_OK ? rc : wait();
+    }
+
+    // k graph replays and the copies of the counts, queued without waiting.
+    int launch(int k)
+    {
         PT_HIP(ctx, hipSetDevice(ctx->device));
         int rc = build_graphs();
         if (rc != PT_OK) return rc;
@@ -368,15 +383,35 @@ struct pt_engine {
             set = 1 - set;
             ++iterations;
         }
-        PT_HIP(ctx, hipMemcpyAsync(done_h.data(), done_d, sizeof(int) * (size_t)max_jobs, hipMemcpyDeviceToHost, st));
-        PT_HIP(ctx, hipMemcpyAsync(&live_h, counters + 4 * set + 3, sizeof(int), hipMemcpyDeviceToHost, st));
+        return queue_counts();
+    }
+
+    int queue_counts()
+    {
+        PT_HIP(ctx, hipMemcpyAsync(stage_h, done_d, sizeof(int) * (size_t)max_jobs, hipMemcpyDeviceToHost, st));
+        PT_HIP(ctx, hipMemcpyAsync(stage_h + max_jobs, counters + 4 * set + 3, sizeof(int), hipMemcpyDeviceToHost,
+                                   st));
+        return PT_OK;
+    }
+
+    // Waits for the queued work and takes the counts.
+    int wait()
+    {
         PT_HIP(ctx, hipStreamSynchronize(st));
+        for (int i = 0; i < max_jobs; ++i) done_h[i] = stage_h[i];
+        live_h = stage_h[max_jobs];
         return PT_OK;
     }
 
     // Traces the queued rays and then completes every live path in one launch, and
     // synchronises as run() does.
     int finish_live()
+    {
+        const int rc = launch_finish();
+        return rc != PT_OK ? rc : wait();
+    }
+
+    int launch_finish()
     {
         PT_HIP(ctx, hipSetDevice(ctx->device));
         const int r = set;
@@ -416,10 +451,7 @@ struct pt_engine {
         // no live path and no queued ray is left: the counters of this set start the next job
         PT_HIP(ctx, pt::launch_zero(counters + 4 * r, 4, st));
         ++iterations;
-        PT_HIP(ctx, hipMemcpyAsync(done_h.data(), done_d, sizeof(int) * (size_t)max_jobs, hipMemcpyDeviceToHost, st));
-        PT_HIP(ctx, hipStreamSynchronize(st));
-        live_h = 0;
-        return PT_OK;
+        return queue_counts();
     }
 
     // One iteration with every launch timed by HIP events (kernel kinds: extension

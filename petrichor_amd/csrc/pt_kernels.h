@@ -42,10 +42,12 @@ struct SceneView {
 struct JobView {
     int width, height, spp, max_bounces, integrator;
     uint32_t seed;
-    float* out;      // width*height*3 floats of this job's image
+    float* out;      // width*height*3 floats of the image
     int* done;       // pixels finished so far
     int path_base;   // first path slot of this job
     int job_index;   // index in the engine's job table
+    int pixel_base;  // the job renders pixels [pixel_base, pixel_base + num_pixels) of the image
+    int num_pixels;
 };
 
 // A job as the kernels see it in the engine's job table.

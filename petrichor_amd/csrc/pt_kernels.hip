@@ -167,14 +167,14 @@ __global__ void __launch_bounds__(256) k_paths_init(SceneView sc, JobView job, P
                                                     int* active_count)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int num_paths = job.width * job.height;
-    const bool live = i < num_paths;
+    const bool live = i < job.num_pixels;
     const int p = job.path_base + i;
+    const int pixel = job.pixel_base + i;
     Samplers smp;
     Ray r;
     if (live) {
-        smp = samplers_for_pixel(job.seed, (uint32_t)i);
-        r = camera_ray(sc, job, i, smp);
+        smp = samplers_for_pixel(job.seed, (uint32_t)pixel);
+        r = camera_ray(sc, job, pixel, smp);
         ps.sum[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         ps.color[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         ps.ray_o[p] = st3(r.o, r.prob);
@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(256) k_paths_init(SceneView sc, JobView job, P
         ps.rng[3 * p] = pack_xs(smp.s1);
         ps.rng[3 * p + 1] = pack_xs(smp.s2x);
         ps.rng[3 * p + 2] = pack_xs(smp.s2y);
-        ps.pixel[p] = i;
+        ps.pixel[p] = pixel;
         ps.job[p] = job.job_index;
     }
     const int a = queue_slot(active_count, live);
@@ -668,7 +668,7 @@ hipError_t launch_zero(int* counters, int n, hipStream_t st, int stride)
 hipError_t launch_paths_init(const SceneView& sc, const JobView& job, const PathView& ps, const Queue& ext,
                              int* active, int* active_count, hipStream_t st)
 {
-    const int64_t n = (int64_t)job.width * job.height;
+    const int64_t n = job.num_pixels;
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_paths_init, dim3(grid_for(n, 256)), dim3(256), 0, st, sc, job, ps, ext, active,
                        active_count);
@@ -682,25 +682,26 @@ __global__ void __launch_bounds__(256) k_preview(JobView job, PathView ps, float
                                                  unsigned long long* samples)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = job.width * job.height;
+    const int n = job.num_pixels;
     unsigned long long c = 0;
     if (i < n) {
         const int p = job.path_base + i;
+        const size_t px = (size_t)job.pixel_base + i;
         const int s = ps.meta[p].x;
         float r = 0.0f, g = 0.0f, b = 0.0f;
         if (s >= job.spp) {
-            r = job.out[3 * (size_t)i];
-            g = job.out[3 * (size_t)i + 1];
-            b = job.out[3 * (size_t)i + 2];
+            r = job.out[3 * px];
+            g = job.out[3 * px + 1];
+            b = job.out[3 * px + 2];
         } else if (s > 0) {
             const float4 sum = ps.sum[p];
             r = sum.x / (float)s;
             g = sum.y / (float)s;
             b = sum.z / (float)s;
         }
-        out[3 * (size_t)i] = r;
-        out[3 * (size_t)i + 1] = g;
-        out[3 * (size_t)i + 2] = b;
+        out[3 * px] = r;
+        out[3 * px + 1] = g;
+        out[3 * px + 2] = b;
         c = (unsigned long long)min(s, job.spp);
     }
     for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
@@ -710,7 +711,7 @@ __global__ void __launch_bounds__(256) k_preview(JobView job, PathView ps, float
 hipError_t launch_preview(const JobView& job, const PathView& ps, float* out, unsigned long long* samples,
                           hipStream_t st)
 {
-    const int64_t n = (int64_t)job.width * job.height;
+    const int64_t n = job.num_pixels;
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_preview, dim3(grid_for(n, 256)), dim3(256), 0, st, job, ps, out, samples);
     return hipGetLastError();

@@ -68,3 +68,20 @@ def test_autotune_picks_a_width_whose_batch_equals_plain(gpu):
         plain, _ = j.bvh.render(j.width, j.height, j.spp, j.max_bounces, j.integrator, j.seed, plain=True)
         assert j.bvh.width == best["width"]
         assert bits_equal(j.result, plain)
+
+
+def test_large_image_in_two_halves_equals_one_piece(gpu):
+    """An image of at least 2^19 pixels renders in two halves on two engines (streams);
+    it equals the same render in one piece with plain launches, and the oracle."""
+    c = ctx()
+    b = c.scene(scenes.cornell_box()).build_bvh(16)
+    w, h = 1024, 576
+    halves, st = b.render(w, h, 2, 8, 0, 21)
+    whole, st1 = b.render(w, h, 2, 8, 0, 21, plain=True)
+    assert bits_equal(halves, whole)
+    assert st["queries"] == st1["queries"]
+    o = O.OracleScene(b.scene.src).use_tree(b.export_nodes() | {"prim_ids": b.export_order()})
+    rows = [0, 287, 288, 575]  # both halves' first and last rows
+    pix = np.array([r * w + x for r in rows for x in range(w)], np.int32)
+    ref, _ = o.render(w, h, 2, 8, 0, 21, 0, pixels=pix)
+    assert rel_err(halves.reshape(-1, 3)[pix], ref).max() <= 1e-4
