@@ -9,7 +9,9 @@
 // device memory, so an iteration is a fixed sequence of launches that is captured once
 // as a hipGraph and replayed several times between host synchronisations. Jobs are
 // admitted into free slots between replays (continuous batching) and retired when
-// all their pixels are done.
+// all their pixels are done. Once every job is admitted and few paths are live, one
+// path-completion launch runs them to the end; a large single-job render runs as two
+// halves of its pixels on two engines, whose launches overlap.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
