@@ -184,15 +184,17 @@ int run_halves(pt_context* ctx, pt_engine* E[2], const pt_job_desc& jd, int iter
         E[i]->live_h = (int)h[i].npix;
     }
     while (!done[0] || !done[1]) {
+        // both halves end together: their completion launches start in the same round, once
+        // the live paths of both fit one completion launch
+        const int live = (done[0] ? 0 : E[0]->live_h) + (done[1] ? 0 : E[1]->live_h);
+        const bool last = live > 0 && live <= E[0]->finish_threshold;
         for (int i = 0; i < 2; ++i) {
             if (done[i]) continue;
-            const bool last = E[i]->live_h > 0 && E[i]->live_h <= E[i]->finish_threshold / 2;
             rc = last ? E[i]->launch_finish() : E[i]->launch(iters_per_sync);
             if (rc != PT_OK) return rc;
         }
         for (int i = 0; i < 2; ++i) {
             if (done[i]) continue;
-            const bool last = E[i]->live_h > 0 && E[i]->live_h <= E[i]->finish_threshold / 2;
             rc = E[i]->wait();
             if (rc != PT_OK) return rc;
             done[i] = E[i]->done_h[h[i].slot] >= h[i].npix;
