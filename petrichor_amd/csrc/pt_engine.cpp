@@ -268,7 +268,7 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
                 *slots[i] = nullptr;
                 rc = pt_engine_create(ctx, npix, 1, 0, slots[i]);
                 if (rc != PT_OK) return rc;
-                (*slots[i])->iters_per_sync = 16;
+                (*slots[i])->iters_per_sync = 8;
             }
             E2[i] = *slots[i];
         }
@@ -289,7 +289,7 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
         ctx->engine = nullptr;
         rc = pt_engine_create(ctx, npix, 1, 0, &ctx->engine);
         if (rc != PT_OK) return rc;
-        ctx->engine->iters_per_sync = 16;
+        ctx->engine->iters_per_sync = 8;
     }
     pt_engine& E = *ctx->engine;
     E.iterations = 0;
