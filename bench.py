@@ -566,13 +566,13 @@ def main():
     W, H, SPP, D = args.width, args.height, args.spp, args.depth
     out = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
     for i in range(args.warmup):
-        _, st = bvh.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=out)
+        _, st = bvh.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=out, two_streams=True)
         log("warmup", i, st["queries"], "queries", "%.1f ms" % st["elapsed_ms"], st["iterations"], "iterations")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stats = []
     for i in range(args.steps):
-        _, st = bvh.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=out)
+        _, st = bvh.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=out, two_streams=True)
         stats.append(st)
         log("step", i, st["queries"], "queries", "%.1f ms" % st["elapsed_ms"], st["iterations"], "iterations")
     torch.cuda.synchronize()
@@ -655,12 +655,12 @@ def main():
     if width == 2 and not args.no_wide:
         b8 = s.build_bvh(16, width=8)
         out8 = torch.empty_like(out)
-        b8.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=out8)
+        b8.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=out8, two_streams=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         q8 = 0
         for _ in range(2):
-            _, st8 = b8.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=out8)
+            _, st8 = b8.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=out8, two_streams=True)
             q8 += st8["queries"]
         torch.cuda.synchronize()
         w8 = time.perf_counter() - t0
@@ -686,7 +686,8 @@ def main():
                    "integrator": "MIS",
                    "bvh": "binned SAH, 16 bins, leaf<=4, device builder, " +
                           ("collapsed to 8-wide nodes" if width == 8
-                           else "binary (reference topology and walk order)")},
+                           else "binary (reference topology and walk order)"),
+                   "render": "pt_render with PT_RENDER_TWO_STREAMS (two halves of the pixels on two streams)"},
         "queries_per_step": int(queries / args.steps),
         "iterations_per_step": int(stats[-1]["iterations"]),
         "bvh_build_ms": round(info["build_ms"], 1),

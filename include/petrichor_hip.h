@@ -95,6 +95,11 @@ typedef struct pt_render_desc {
 /* With PT_RENDER_TIME_KERNELS: count node visits and triangle tests per trace kind
  * (binary tree: child boxes tested, 32 B each; 8-wide tree: 128 B node records). */
 #define PT_RENDER_TRACE_STATS 4
+/* An image of at least 2^19 pixels renders as two halves of its pixels on two streams,
+ * whose launches the GPU overlaps (same image). It pays when the two streams get
+ * hardware queues of their own: with more streams alive in the process than the device
+ * has queues (4 by default), the halves can share one and run slower than one piece. */
+#define PT_RENDER_TWO_STREAMS 8
 
 /* Per-kernel figures: index 0 extension (closest hit), 1 environment shadow rays (any
  * hit), 2 area-light shadow rays (closest hit), 3 shading. Times are filled only with

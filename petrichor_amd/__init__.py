@@ -74,6 +74,7 @@ class Tuning(ctypes.Structure):
 RENDER_TIME_KERNELS = 1
 RENDER_PLAIN_LAUNCHES = 2
 RENDER_TRACE_STATS = 4
+RENDER_TWO_STREAMS = 8
 
 
 class ServerDesc(ctypes.Structure):
@@ -364,12 +365,14 @@ class Bvh:
         return t_out, prim_out, st
 
     def render(self, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1, out=None,
-               time_kernels=False, plain=False, trace_stats=False):
+               time_kernels=False, plain=False, trace_stats=False, two_streams=False):
         """Renders one image; returns (rgb [h, w, 3] float32 numpy array or the given device
         tensor, stats dict). time_kernels: every launch timed on its own; plain: plain
-        launches with one synchronisation per iteration (both give the same image)."""
+        launches with one synchronisation per iteration; two_streams: a large image in two
+        halves on two streams (all give the same image)."""
         flags = (RENDER_TIME_KERNELS if time_kernels or trace_stats else 0) | \
-            (RENDER_PLAIN_LAUNCHES if plain else 0) | (RENDER_TRACE_STATS if trace_stats else 0)
+            (RENDER_PLAIN_LAUNCHES if plain else 0) | (RENDER_TRACE_STATS if trace_stats else 0) | \
+            (RENDER_TWO_STREAMS if two_streams else 0)
         d = RenderDesc(width, height, spp, max_bounces, integrator, seed, flags)
         st = RenderStats()
         if out is None:

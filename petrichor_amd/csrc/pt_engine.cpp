@@ -28,7 +28,8 @@
 
 using namespace pt;
 
-// Images of at least this many pixels render in two halves (run_halves).
+// With PT_RENDER_TWO_STREAMS, images of at least this many pixels render in two halves
+// (run_halves).
 static constexpr int64_t kHalvesMinPixels = (int64_t)1 << 19;
 
 namespace pt {
@@ -257,8 +258,10 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
     int rc = check_job(ctx, jd);
     if (rc != PT_OK) return rc;
     const int64_t npix = (int64_t)d->width * d->height;
-    // Large images render in two halves on two engines (streams), whose launches overlap.
-    if (npix >= kHalvesMinPixels && !(d->flags & (PT_RENDER_PLAIN_LAUNCHES | PT_RENDER_TIME_KERNELS))) {
+    // On request, large images render in two halves on two engines (streams), whose
+    // launches overlap.
+    if ((d->flags & PT_RENDER_TWO_STREAMS) && npix >= kHalvesMinPixels &&
+        !(d->flags & (PT_RENDER_PLAIN_LAUNCHES | PT_RENDER_TIME_KERNELS))) {
         const int64_t cap = npix - npix / 2;
         pt_engine** slots[2] = {&ctx->engine, &ctx->engine2};
         pt_engine* E2[2];

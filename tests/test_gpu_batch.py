@@ -71,12 +71,15 @@ def test_autotune_picks_a_width_whose_batch_equals_plain(gpu):
 
 
 def test_large_image_in_two_halves_equals_one_piece(gpu):
-    """An image of at least 2^19 pixels renders in two halves on two engines (streams);
-    it equals the same render in one piece with plain launches, and the oracle."""
+    """With two_streams, an image of at least 2^19 pixels renders in two halves on two
+    engines (streams); it equals the same render in one piece, with plain launches and
+    with the graph path, and the oracle."""
     c = ctx()
     b = c.scene(scenes.cornell_box()).build_bvh(16)
     w, h = 1024, 576
-    halves, st = b.render(w, h, 2, 8, 0, 21)
+    halves, st = b.render(w, h, 2, 8, 0, 21, two_streams=True)
+    one, _ = b.render(w, h, 2, 8, 0, 21)
+    assert bits_equal(halves, one)
     whole, st1 = b.render(w, h, 2, 8, 0, 21, plain=True)
     assert bits_equal(halves, whole)
     assert st["queries"] == st1["queries"]
