@@ -41,6 +41,7 @@ CASES = {
     "cornell": (scenes.cornell_box, 16, 4),
     "random50k": (lambda: scenes.random_triangles(50000, seed=21, mu=-4.0), 16, 4),
     "random50k_32bins": (lambda: scenes.random_triangles(50000, seed=22, mu=-4.0), 32, 4),
+    "random50k_64bins": (lambda: scenes.random_triangles(50000, seed=25, mu=-4.0), 64, 4),
     "random_leaf1": (lambda: scenes.random_triangles(4000, seed=23, mu=-4.0), 16, 1),
     "random_leaf8": (lambda: scenes.random_triangles(4000, seed=24, mu=-4.0), 12, 8),
     "spheres": (lambda: scenes.spheres_scene(n_spheres=8, seg=16, bands=9, terrain_n=32, env_w=32, env_h=16), 16, 4),
