@@ -498,6 +498,67 @@ def serve_bench(args):
     print(json.dumps(res), flush=True)
 
 
+def tonemap_bench(args):
+    """The 8-bit output kernel (pt_tonemap_rgb8) on an image of --width x --height HDR
+    pixels resident in HBM: Mpixels/s, HIP-event launch times on the launch stream, the
+    HBM roofline (12 bytes read + 3 written per pixel) and the oracle on the host cores."""
+    import torch
+    import petrichor_amd as P
+    W, H = args.width, args.height
+    n = W * H
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    # log-normal HDR values around 0.3 (the range where the bytes change), plus exact zeros
+    img = torch.exp(torch.randn((H, W, 3), generator=gen, device="cuda") * 1.5 - 1.2)
+    img.view(-1)[::97] = 0.0
+    ctx = P.Context(0)
+    out = torch.empty((H, W, 3), dtype=torch.uint8, device="cuda")
+    st = torch.cuda.current_stream()
+    for _ in range(max(args.warmup, 1)):
+        ctx.tonemap_rgb8(img, out, stream=st.cuda_stream)
+    torch.cuda.synchronize()
+    steps = max(args.steps, 1) * 10
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record(st)
+    for _ in range(steps):
+        ctx.tonemap_rgb8(img, out, stream=st.cuda_stream)
+    ev[1].record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = ev[0].elapsed_time(ev[1]) / steps
+    alg = 15 * n
+    achieved = alg / (launch_ms * 1e-3) / 1e9
+    got = out.cpu().numpy().reshape(-1, 3)
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        sample = min(n, 1 << 24)
+        host = img.view(-1, 3)[:sample].cpu().numpy()
+        threads = min(os.cpu_count() or 1, 16)
+        t1 = time.perf_counter()
+        ref = O.tonemap_rgb8(host, threads)
+        cpu_s = time.perf_counter() - t1
+        cpu = {"value": round(sample / cpu_s / 1e6, 2), "unit": "Mpixels/s", "cores": threads, "kind": "port",
+               "sample": "the first %d pixels of the same image" % sample}
+        parity = {"pixels_checked": sample, "bytes_equal": int((got[:sample] == ref).sum()),
+                  "bytes_checked": int(ref.size)}
+    res = {"metric": "tonemap Mpixels/s (8-bit PNG bytes of an HDR image in HBM)",
+           "value": round(n * steps / wall / 1e6, 1), "unit": "Mpixels/s", "n_gpus": 1, "steps": steps,
+           "warmup": max(args.warmup, 1), "ms_per_step": round(wall / steps * 1e3, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 in, u8 out", "data": "synthetic",
+           "config": {"workload": "ACES + RTCamp7 + degamma to 8 bits of a %dx%d RGB float image" % (W, H),
+                      "width": W, "height": H},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "k_tonemap_rgb8", "algorithmic_bytes_per_launch": alg,
+                        "avg_launch_ms": round(launch_ms, 5)},
+           "cpu_baseline": cpu, "parity_vs_oracle": parity}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -510,11 +571,12 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--depth", type=int, default=12)
-    ap.add_argument("--mode", default="frame", choices=["frame", "trace", "batch", "serve", "copy"],
+    ap.add_argument("--mode", default="frame", choices=["frame", "trace", "batch", "serve", "copy", "tonemap"],
                     help="frame: the single large scene (default); trace: the BVH build and "
                          "traversal microbenchmark (10M triangles, 2^26 rays); copy: a plain 1 GiB "
                          "read + 1 GiB write elementwise kernel, the reference for scaling rocprofv3 "
-                         "FETCH_SIZE / WRITE_SIZE")
+                         "FETCH_SIZE / WRITE_SIZE; tonemap: the 8-bit output kernel on a "
+                         "--width x --height image")
     ap.add_argument("--triangles", type=int, default=10_000_000, help="trace mode: triangle count")
     ap.add_argument("--bvh-width", type=int, default=None, choices=[2, 8],
                     help="BVH width: 2 walks the binary tree in the reference's order (frame default), "
@@ -553,6 +615,8 @@ def main():
         return batch_bench(args)
     if args.mode == "serve":
         return serve_bench(args)
+    if args.mode == "tonemap":
+        return tonemap_bench(args)
     log("generating scene")
     t0 = time.perf_counter()
     sc = scenes.spheres_scene(seed=2)

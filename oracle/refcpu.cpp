@@ -1560,4 +1560,64 @@ void oracle_onb(const float* w_in, float theta, float phi, float* out12)
     std::memcpy(out12, o, sizeof o);
 }
 
+// ---- Texture2D::Save, PNG branch (Core/Texture2D.cpp:18-91, 199-215) ----------------
+// std::pow(float, float) binds to powf, called directly here.
+
+static float ref_aces(float x)
+{
+    // ACESFilm(float), Core/Texture2D.cpp:18-27
+    const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+    return (x * (a * x + b)) / (x * (c * x + d) + e);
+}
+
+static V ref_gamma(const V& c, float g) { return mk(powf(c.x, g), powf(c.y, g), powf(c.z, g)); } // :44-56
+
+static V ref_rtcamp7(const V& color)
+{
+    // RTCamp7Tonemap, Core/Texture2D.cpp:58-70; (c - x0) / (x1 - x0) is per component
+    V c = ref_gamma(color, 1.2f);
+    const V x0 = ref_gamma(mk(0.05f * 1.0f, 0.05f * 1.0f, 0.05f * 1.0f), 2.2f);
+    const V x1 = ref_gamma(mk(0.9f * 1.0f, 0.9f * 1.0f, 0.9f * 1.0f), 2.2f);
+    c = mk((c.x - x0.x) / (x1.x - x0.x), (c.y - x0.y) / (x1.y - x0.y), (c.z - x0.z) / (x1.z - x0.z));
+    c.y = powf(c.y, 1.3f);
+    c.z = powf(c.z, 1.6f);
+    return c;
+}
+
+static float ref_degamma(float value)
+{
+    // ApplyDegamma, Core/Texture2D.cpp:78-86; std::clamp keeps a NaN
+    value = powf(value, 1.1f);
+    value = value < 0.0f ? 0.0f : (1.0f < value ? 1.0f : value);
+    return powf(value, 1 / 2.2f);
+}
+
+static uint8_t ref_byte(float v)
+{
+    // static_cast<uint8_t>((kColorDepth8Bit - kEps) * v), Core/Texture2D.cpp:209-214. The
+    // x86-64 conversion goes through a 32-bit integer: a NaN becomes 0x80000000, low byte 0.
+    const float q = (255 - 1.0e-3f) * v;
+    if (q != q) return 0;
+    return (uint8_t)(int32_t)q;
+}
+
+void oracle_tonemap_rgb8(int64_t num_pixels, const float* rgb, uint8_t* out, int num_threads)
+{
+    int nt = num_threads > 0 ? num_threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    if (num_pixels < 4096) nt = 1;
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t) {
+        pool.emplace_back([=] {
+            for (int64_t i = num_pixels * t / nt; i < num_pixels * (t + 1) / nt; ++i) {
+                const V px = mk(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+                const V tm = ref_rtcamp7(mk(ref_aces(px.x), ref_aces(px.y), ref_aces(px.z))); // :75
+                out[3 * i] = ref_byte(ref_degamma(tm.x));
+                out[3 * i + 1] = ref_byte(ref_degamma(tm.y));
+                out[3 * i + 2] = ref_byte(ref_degamma(tm.z));
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+}
+
 } // extern "C"

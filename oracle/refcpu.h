@@ -95,6 +95,9 @@ void oracle_env_sample(void* scene, uint32_t seed0, uint32_t seed1, float* out4)
 float oracle_env_pdf(void* scene, const float* dir);
 /* OrthonormalBasis::Build(w) + GetDir(theta, phi): out u[3] v[3] w[3] dir[3]. */
 void oracle_onb(const float* w, float theta, float phi, float* out12);
+/* Texture2D::Save's PNG bytes of num_pixels RGB pixels (Core/Texture2D.cpp:199-215), on
+ * num_threads host threads (0: all cores). */
+void oracle_tonemap_rgb8(int64_t num_pixels, const float* rgb, uint8_t* out, int num_threads);
 
 #ifdef __cplusplus
 }

@@ -152,6 +152,7 @@ def lib():
     L.pt_server_preview.argtypes = [vp, ctypes.c_int64, _f, ctypes.POINTER(ctypes.c_float)]
     L.pt_server_fetch.argtypes = [vp, ctypes.c_int64, _f]
     L.pt_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    L.pt_tonemap_rgb8.argtypes = [vp, ctypes.c_int64, vp, vp, vp]
     _lib = L
     return L
 
@@ -162,7 +163,8 @@ EXPORTED = ["pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_scene_create
             "pt_trace_any", "pt_trace_stats", "pt_render", "pt_engine_create", "pt_engine_destroy",
             "pt_render_batch", "pt_engine_set_tuning", "pt_engine_get_tuning", "pt_server_create", "pt_server_destroy", "pt_server_last_error",
             "pt_server_add_scene", "pt_server_remove_scene", "pt_server_scene_resident", "pt_server_submit",
-            "pt_server_poll", "pt_server_wait", "pt_server_preview", "pt_server_fetch", "pt_server_get_stats"]
+            "pt_server_poll", "pt_server_wait", "pt_server_preview", "pt_server_fetch", "pt_server_get_stats",
+            "pt_tonemap_rgb8"]
 
 
 def _ptr(a, t):
@@ -190,6 +192,80 @@ def build_bvh_host(tri_pos, bins=16, leaf_max=4):
             "max_depth": depth.value}
 
 
+def save_png(path, rgb8):
+    """Writes an [H, W, 3] uint8 image as an 8-bit RGB PNG (the file Texture2D::Save
+    writes for a .png path, Core/Texture2D.cpp:217-222, without stb): one zlib stream of
+    unfiltered rows."""
+    import struct
+    import zlib
+    a = np.ascontiguousarray(np.asarray(rgb8, dtype=np.uint8))
+    assert a.ndim == 3 and a.shape[2] == 3
+    h, w = a.shape[:2]
+    raw = np.concatenate([np.zeros((h, 1), np.uint8), a.reshape(h, w * 3)], axis=1).tobytes()
+
+    def chunk(kind, data):
+        return struct.pack(">I", len(data)) + kind + data + struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF)
+
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n")
+        f.write(chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0)))
+        f.write(chunk(b"IDAT", zlib.compress(raw, 6)))
+        f.write(chunk(b"IEND", b""))
+
+
+def load_png_rgb8(path):
+    """Reads back an 8-bit RGB PNG whose rows are unfiltered, as save_png writes them."""
+    import struct
+    import zlib
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, w, h = 8, b"", 0, 0
+    while pos < len(data):
+        n = struct.unpack(">I", data[pos:pos + 4])[0]
+        kind, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        if kind == b"IHDR":
+            w, h, depth, ctype = struct.unpack(">IIBB", body[:10])
+            assert depth == 8 and ctype == 2
+        elif kind == b"IDAT":
+            idat += body
+        pos += 12 + n
+    rows = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 1 + 3 * w)
+    if np.any(rows[:, 0] != 0):
+        raise ValueError("filtered PNG rows are not supported")
+    return rows[:, 1:].reshape(h, w, 3).copy()
+
+
+def save_hdr(path, rgb):
+    """Writes an [H, W, 3] float image as a Radiance RGBE file (the format Texture2D::Save
+    writes for a .hdr path, Core/Texture2D.cpp:225-243, without stb): flat scanlines,
+    each pixel's shared exponent from frexp of its largest component (Ward's encoding)."""
+    a = np.ascontiguousarray(np.asarray(rgb, dtype=np.float32))
+    assert a.ndim == 3 and a.shape[2] == 3
+    h, w = a.shape[:2]
+    mx = a.max(axis=2)
+    mant, ex = np.frexp(mx)
+    ok = mx >= np.float32(1e-32)
+    scale = np.where(ok, (mant.astype(np.float32) * np.float32(256.0)) / np.where(ok, mx, 1), 0).astype(np.float32)
+    rgbe = np.zeros((h, w, 4), np.uint8)
+    rgbe[..., :3] = np.where(ok[..., None], (a * scale[..., None]), 0).astype(np.uint8)
+    rgbe[..., 3] = np.where(ok, ex + 128, 0).astype(np.uint8)
+    with open(path, "wb") as f:
+        f.write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n" % (h, w))
+        f.write(rgbe.tobytes())
+
+
+def load_hdr(path):
+    """Reads a flat (not run-length encoded) Radiance RGBE file as [H, W, 3] floats."""
+    data = open(path, "rb").read()
+    head, _, rest = data.partition(b"\n\n")
+    line, _, pix = rest.partition(b"\n")
+    parts = line.split()
+    h, w = int(parts[1]), int(parts[3])
+    e = np.frombuffer(pix, np.uint8, count=h * w * 4).reshape(h, w, 4)
+    f = np.where(e[..., 3:] > 0, np.ldexp(1.0, e[..., 3:].astype(np.int32) - (128 + 8)), 0.0)
+    return (e[..., :3] * f).astype(np.float32)
+
+
 class Context:
     """One device + one HIP stream (pt_ctx_create)."""
 
@@ -208,6 +284,20 @@ class Context:
 
     def scene(self, scene):
         return Scene(self, scene)
+
+    def tonemap_rgb8(self, rgb, out=None, stream=None):
+        """The reference's PNG bytes of an RGB float image (pt_tonemap_rgb8, the pixel
+        loop of Texture2D::Save): rgb is a float32 CUDA tensor [..., 3]; returns a uint8
+        tensor of the same shape on the same device."""
+        import torch
+        assert rgb.is_cuda and rgb.dtype == torch.float32 and rgb.is_contiguous() and rgb.shape[-1] == 3
+        if out is None:
+            out = torch.empty(rgb.shape, dtype=torch.uint8, device=rgb.device)
+        assert out.dtype == torch.uint8 and out.is_contiguous() and out.numel() == rgb.numel()
+        st = ctypes.c_void_p(stream) if stream else None
+        self.check(self._lib.pt_tonemap_rgb8(self.handle, rgb.numel() // 3, rgb.data_ptr(), out.data_ptr(), st),
+                   "pt_tonemap_rgb8")
+        return out
 
     def close(self):
         if self.handle:

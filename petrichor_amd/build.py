@@ -20,7 +20,7 @@ FP_FLAGS = ["-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt", "-f
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"] + FP_FLAGS
 
-SOURCES = ["pt_kernels.hip", "pt_trace.hip", "pt_build.hip", "pt_wide.hip", "pt_bvh_host.cpp", "pt_capi.cpp", "pt_engine.cpp", "pt_server.cpp"]
+SOURCES = ["pt_kernels.hip", "pt_trace.hip", "pt_build.hip", "pt_wide.hip", "pt_post.hip", "pt_bvh_host.cpp", "pt_capi.cpp", "pt_engine.cpp", "pt_server.cpp"]
 
 
 def _run(cmd):
@@ -59,5 +59,22 @@ def build(verbose=False):
     return LIB
 
 
+def gen_tonemap_lut():
+    """Regenerates csrc/pt_tonemap_lut.h from the host C library's powf (about 40 s on 8
+    cores: every 32-bit input pattern is evaluated). build() uses the committed table."""
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    exe = os.path.join(bdir, "gen_tonemap_lut")
+    _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fopenmp", os.path.join(CSRC, "gen_tonemap_lut.cpp"),
+          "-o", exe])
+    out = os.path.join(CSRC, "pt_tonemap_lut.h")
+    with open(out + ".tmp", "w") as f:
+        subprocess.check_call([exe], stdout=f)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 if __name__ == "__main__":
+    if "--tonemap-lut" in sys.argv[1:]:
+        print(gen_tonemap_lut())
     print(build(verbose=True))

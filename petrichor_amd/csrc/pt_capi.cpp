@@ -614,5 +614,16 @@ int pt_trace_stats(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_
     return PT_OK;
 }
 
+int pt_tonemap_rgb8(pt_context* ctx, int64_t num_pixels, const float* rgb, uint8_t* rgb8_out, void* stream)
+{
+    if (!ctx) return PT_ERR_ARG;
+    if (num_pixels < 0 || num_pixels > ((int64_t)1 << 40) || (num_pixels > 0 && (!rgb || !rgb8_out)))
+        return fail(ctx, PT_ERR_ARG, "bad tonemap arguments");
+    if (num_pixels == 0) return PT_OK;
+    PT_HIP(ctx, hipSetDevice(ctx->device));
+    PT_HIP(ctx, pt::launch_tonemap_rgb8(rgb, rgb8_out, num_pixels, stream ? (hipStream_t)stream : ctx->stream));
+    return PT_OK;
+}
+
 } // extern "C"
 

@@ -180,4 +180,7 @@ hipError_t launch_finish_paths(int integrator, const SceneView& sc, const JobVie
                                bool wide, const ShadeArgs& a, const HitBuffers& h, unsigned long long* queries,
                                hipStream_t st);
 
+// The reference's 8-bit PNG bytes of num_pixels RGB float pixels (pt_post.hip).
+hipError_t launch_tonemap_rgb8(const float* rgb, uint8_t* out, int64_t num_pixels, hipStream_t st);
+
 } // namespace pt

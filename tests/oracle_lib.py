@@ -65,6 +65,7 @@ def lib():
     L.oracle_env_pdf.restype = ctypes.c_float
     L.oracle_env_pdf.argtypes = [_v, _f]
     L.oracle_onb.argtypes = [_f, ctypes.c_float, ctypes.c_float, _f]
+    L.oracle_tonemap_rgb8.argtypes = [ctypes.c_int64, _f, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int]
     _lib = L
     return L
 
@@ -239,4 +240,12 @@ def triangle_sample(tri9, s0, s1):
 def onb(w, theta, phi):
     out = np.zeros(12, np.float32)
     lib().oracle_onb(_ptr(f32(w), _f), theta, phi, _ptr(out, _f))
+    return out
+
+
+def tonemap_rgb8(rgb, threads=0):
+    """Texture2D::Save's PNG bytes of float RGB values (any shape with a last axis of 3)."""
+    a = np.ascontiguousarray(rgb, np.float32)
+    out = np.zeros(a.shape, np.uint8)
+    lib().oracle_tonemap_rgb8(a.size // 3, _ptr(a, _f), _ptr(out, ctypes.POINTER(ctypes.c_uint8)), threads)
     return out
