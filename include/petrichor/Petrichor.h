@@ -6,17 +6,22 @@
 //
 // Only what the GPU path renders is here: triangles (flat or smooth), Lambert, GGX,
 // Glass, Emission and MixMaterial, an equirect environment given as a Texture2D,
-// the MIS and simple integrators. File loading (OBJ models, JSON scenes, images) and the
-// denoising AOVs are not part of this API.
+// the MIS and simple integrators, and Texture2D::Save (PNG bytes converted on the GPU,
+// Radiance HDR). File loading (OBJ models, JSON scenes, images) and the denoising AOVs are
+// not part of this API.
 //
 // Floating point: the camera set-up below must be compiled without FMA contraction
 // (-ffp-contract=off) to give the reference's camera state bit for bit.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <filesystem>
 #include <functional>
 #include <memory>
 #include <stdexcept>
@@ -256,7 +261,125 @@ public:
     const float* GetRawDataPtr() const { return &m_pixels[0].x; }
     float* GetRawDataPtr() { return &m_pixels[0].x; }
 
+    // Core/Texture2D.cpp:197-246: ".png" writes the reference's 8-bit bytes (ACES filmic,
+    // RTCamp7 curve, degamma, truncation), converted on the GPU by pt_tonemap_rgb8;
+    // ".hdr" writes the floats as Radiance RGBE. Returns false for another extension or a
+    // failure (the reference logs the error and returns).
+    bool Save(const std::filesystem::path& path, int device = 0) const
+    {
+        if (!IsValid()) return false;
+        const size_t n = (size_t)m_width * m_height;
+        if (path.extension() == ".png") {
+            std::vector<uint8_t> px(3 * n);
+            pt_context* ctx = nullptr;
+            if (pt_ctx_create(device, &ctx) != PT_OK) return false;
+            const int rc = pt_tonemap_rgb8(ctx, (int64_t)n, GetRawDataPtr(), px.data(), 0, nullptr);
+            pt_ctx_destroy(ctx);
+            return rc == PT_OK && writePng(path, px);
+        }
+        if (path.extension() == ".hdr") return writeHdr(path);
+        return false;
+    }
+
 private:
+    static uint32_t crc32(uint32_t crc, const uint8_t* p, size_t n)
+    {
+        // CRC-32 of the PNG chunks (ISO 3309, reflected polynomial 0xedb88320)
+        static const std::array<uint32_t, 256> table = [] {
+            std::array<uint32_t, 256> t{};
+            for (uint32_t i = 0; i < 256; ++i) {
+                uint32_t c = i;
+                for (int k = 0; k < 8; ++k) c = (c & 1) ? 0xedb88320u ^ (c >> 1) : c >> 1;
+                t[i] = c;
+            }
+            return t;
+        }();
+        crc = ~crc;
+        for (size_t i = 0; i < n; ++i) crc = table[(crc ^ p[i]) & 0xff] ^ (crc >> 8);
+        return ~crc;
+    }
+
+    static void put32(std::vector<uint8_t>& b, uint32_t v)
+    {
+        for (int s = 24; s >= 0; s -= 8) b.push_back((uint8_t)(v >> s));
+    }
+
+    // An 8-bit RGB PNG: unfiltered rows in one zlib stream of stored (uncompressed)
+    // deflate blocks.
+    bool writePng(const std::filesystem::path& path, const std::vector<uint8_t>& px) const
+    {
+        std::vector<uint8_t> raw;
+        raw.reserve((size_t)m_height * (1 + 3 * (size_t)m_width));
+        for (int y = 0; y < m_height; ++y) {
+            raw.push_back(0);
+            raw.insert(raw.end(), px.begin() + (size_t)y * 3 * m_width, px.begin() + (size_t)(y + 1) * 3 * m_width);
+        }
+        std::vector<uint8_t> z = {0x78, 0x01};
+        uint32_t a = 1, b = 0; // Adler-32 of the uncompressed stream
+        for (uint8_t c : raw) {
+            a = (a + c) % 65521u;
+            b = (b + a) % 65521u;
+        }
+        for (size_t pos = 0; pos < raw.size(); pos += 65535) {
+            const size_t len = std::min<size_t>(65535, raw.size() - pos);
+            z.push_back(pos + len >= raw.size() ? 1 : 0);
+            z.push_back((uint8_t)(len & 0xff));
+            z.push_back((uint8_t)(len >> 8));
+            z.push_back((uint8_t)(~len & 0xff));
+            z.push_back((uint8_t)((~len >> 8) & 0xff));
+            z.insert(z.end(), raw.begin() + pos, raw.begin() + pos + len);
+        }
+        put32(z, (b << 16) | a);
+        std::vector<uint8_t> file = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+        auto chunk = [&](const char* kind, const std::vector<uint8_t>& data) {
+            put32(file, (uint32_t)data.size());
+            const size_t at = file.size();
+            file.insert(file.end(), kind, kind + 4);
+            file.insert(file.end(), data.begin(), data.end());
+            put32(file, crc32(0, file.data() + at, 4 + data.size()));
+        };
+        std::vector<uint8_t> ihdr;
+        put32(ihdr, (uint32_t)m_width);
+        put32(ihdr, (uint32_t)m_height);
+        ihdr.insert(ihdr.end(), {8, 2, 0, 0, 0}); // 8-bit RGB, deflate, no filter, no interlace
+        chunk("IHDR", ihdr);
+        chunk("IDAT", z);
+        chunk("IEND", {});
+        FILE* f = std::fopen(path.string().c_str(), "wb");
+        if (!f) return false;
+        const bool ok = std::fwrite(file.data(), 1, file.size(), f) == file.size();
+        return std::fclose(f) == 0 && ok;
+    }
+
+    // Radiance RGBE (Ward): each pixel's shared exponent from frexp of its largest component.
+    bool writeHdr(const std::filesystem::path& path) const
+    {
+        FILE* f = std::fopen(path.string().c_str(), "wb");
+        if (!f) return false;
+        std::fprintf(f, "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n", m_height, m_width);
+        std::vector<uint8_t> row(4 * (size_t)m_width);
+        bool ok = true;
+        for (int y = 0; y < m_height && ok; ++y) {
+            for (int x = 0; x < m_width; ++x) {
+                const Color3f& c = m_pixels[(size_t)y * m_width + x];
+                const float m = std::max(c.x, std::max(c.y, c.z));
+                uint8_t* e = &row[4 * (size_t)x];
+                if (!(m >= 1e-32f)) {
+                    e[0] = e[1] = e[2] = e[3] = 0;
+                    continue;
+                }
+                int ex = 0;
+                const float scale = (float)std::frexp(m, &ex) * 256.0f / m;
+                e[0] = (uint8_t)(c.x * scale);
+                e[1] = (uint8_t)(c.y * scale);
+                e[2] = (uint8_t)(c.z * scale);
+                e[3] = (uint8_t)(ex + 128);
+            }
+            ok = std::fwrite(row.data(), 1, row.size(), f) == row.size();
+        }
+        return std::fclose(f) == 0 && ok;
+    }
+
     int m_width = 0, m_height = 0;
     std::vector<Color3f> m_pixels;
 };

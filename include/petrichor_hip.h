@@ -291,11 +291,13 @@ int pt_server_get_stats(pt_server* server, pt_server_stats* stats);
  * 8-bit RGB of an image as the reference writes it to a PNG (the pixel loop of
  * Texture2D::Save, Core/Texture2D.cpp:199-222: ACES filmic, the RTCamp7 curve, degamma
  * and truncation, Core/Texture2D.cpp:18-91): rgb8_out[3 i + c] is the byte of
- * rgb[3 i + c]. Device pointers, num_pixels * 3 values each; any float input, NaN and
- * infinities included. stream: a hipStream_t or NULL for the context's stream; the call
- * returns once the kernel is queued. Writing the PNG file (stbi_write_png,
- * Core/Texture2D.cpp:217-222) stays with the host. */
-int pt_tonemap_rgb8(pt_context* ctx, int64_t num_pixels, const float* rgb, uint8_t* rgb8_out, void* stream);
+ * rgb[3 i + c], for any float input, NaN and infinities included. num_pixels * 3 values
+ * each: device pointers when on_device != 0 (the call returns once the kernel is queued
+ * on stream, a hipStream_t, or the context's stream when NULL), else host pointers (the
+ * call stages them through device memory and returns when the bytes are on the host).
+ * Writing the PNG file (stbi_write_png, Core/Texture2D.cpp:217-222) stays with the host. */
+int pt_tonemap_rgb8(pt_context* ctx, int64_t num_pixels, const float* rgb, uint8_t* rgb8_out, int32_t on_device,
+                    void* stream);
 
 #ifdef __cplusplus
 }

@@ -152,7 +152,7 @@ def lib():
     L.pt_server_preview.argtypes = [vp, ctypes.c_int64, _f, ctypes.POINTER(ctypes.c_float)]
     L.pt_server_fetch.argtypes = [vp, ctypes.c_int64, _f]
     L.pt_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
-    L.pt_tonemap_rgb8.argtypes = [vp, ctypes.c_int64, vp, vp, vp]
+    L.pt_tonemap_rgb8.argtypes = [vp, ctypes.c_int64, vp, vp, ctypes.c_int32, vp]
     _lib = L
     return L
 
@@ -295,7 +295,7 @@ class Context:
             out = torch.empty(rgb.shape, dtype=torch.uint8, device=rgb.device)
         assert out.dtype == torch.uint8 and out.is_contiguous() and out.numel() == rgb.numel()
         st = ctypes.c_void_p(stream) if stream else None
-        self.check(self._lib.pt_tonemap_rgb8(self.handle, rgb.numel() // 3, rgb.data_ptr(), out.data_ptr(), st),
+        self.check(self._lib.pt_tonemap_rgb8(self.handle, rgb.numel() // 3, rgb.data_ptr(), out.data_ptr(), 1, st),
                    "pt_tonemap_rgb8")
         return out
 

@@ -614,14 +614,32 @@ int pt_trace_stats(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_
     return PT_OK;
 }
 
-int pt_tonemap_rgb8(pt_context* ctx, int64_t num_pixels, const float* rgb, uint8_t* rgb8_out, void* stream)
+int pt_tonemap_rgb8(pt_context* ctx, int64_t num_pixels, const float* rgb, uint8_t* rgb8_out, int32_t on_device,
+                    void* stream)
 {
     if (!ctx) return PT_ERR_ARG;
     if (num_pixels < 0 || num_pixels > ((int64_t)1 << 40) || (num_pixels > 0 && (!rgb || !rgb8_out)))
         return fail(ctx, PT_ERR_ARG, "bad tonemap arguments");
     if (num_pixels == 0) return PT_OK;
     PT_HIP(ctx, hipSetDevice(ctx->device));
-    PT_HIP(ctx, pt::launch_tonemap_rgb8(rgb, rgb8_out, num_pixels, stream ? (hipStream_t)stream : ctx->stream));
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    if (on_device) {
+        PT_HIP(ctx, pt::launch_tonemap_rgb8(rgb, rgb8_out, num_pixels, st));
+        return PT_OK;
+    }
+    // host buffers: staged through device memory, synchronous
+    const size_t in_b = (size_t)num_pixels * 3 * sizeof(float), out_b = (size_t)num_pixels * 3;
+    float* in_d = nullptr;
+    uint8_t* out_d = nullptr;
+    PT_HIP(ctx, hipMalloc((void**)&in_d, in_b));
+    hipError_t e = hipMalloc((void**)&out_d, out_b);
+    if (e == hipSuccess) e = hipMemcpyAsync(in_d, rgb, in_b, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = pt::launch_tonemap_rgb8(in_d, out_d, num_pixels, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(rgb8_out, out_d, out_b, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(in_d);
+    if (out_d) (void)hipFree(out_d);
+    if (e != hipSuccess) return fail(ctx, e == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP, hipGetErrorString(e));
     return PT_OK;
 }
 

@@ -87,6 +87,12 @@ int main(int argc, char** argv)
             rc = 1;
         } else {
             std::fwrite(target.GetRawDataPtr(), 4, 3 * (size_t)w * h, out);
+            // Texture2D::Save of the same image, next to the output file
+            const std::string base = argv[2];
+            if (!target.Save(base + ".png") || !target.Save(base + ".hdr")) {
+                std::fprintf(stderr, "Texture2D::Save failed\n");
+                rc = 1;
+            }
             std::printf("rendered %dx%d in %.3f s, %u of %u tiles\n", w, h, seconds, renderer.GetNumRenderedTiles(),
                         renderer.GetNumTiles());
         }

@@ -9,9 +9,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "petrichor_amd", "_build", "render_api")
 
 
-def run(scene, size=None):
+def run(scene, size=None, saved=None):
     """Builds the scene through the C++ API; returns (camera state [16], image or None).
-    size: (width, height, spp, depth, integrator, seed) to render."""
+    size: (width, height, spp, depth, integrator, seed) to render. saved: a dict that
+    receives the files Texture2D::Save wrote of the image ("png": uint8 [h, w, 3],
+    "hdr": float32 [h, w, 3])."""
     n = len(scene["tri_mat"])
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
@@ -24,6 +26,10 @@ def run(scene, size=None):
         r = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
         assert r.returncode == 0, r.stdout
         data = np.fromfile(fout, np.float32)
+        if size and saved is not None:
+            import petrichor_amd as P
+            saved["png"] = P.load_png_rgb8(fout + ".png")
+            saved["hdr"] = P.load_hdr(fout + ".hdr")
     cam = data[:16]
     img = data[16:].reshape(size[1], size[0], 3) if size else None
     return cam, img

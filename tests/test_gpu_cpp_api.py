@@ -21,3 +21,15 @@ def test_cpp_render_equals_python(gpu, integrator):
     assert bits_equal(cam, np.asarray(sc["camera"], np.float32))
     assert img.mean() > 0
     assert bits_equal(img, ref)
+
+
+def test_cpp_texture_save_png_and_hdr(gpu):
+    # Texture2D::Save: the PNG holds the oracle's tonemapped bytes of the rendered image;
+    # the HDR file decodes to it within RGBE precision (1/128 of a pixel's largest channel)
+    import oracle_lib as O
+    log("C++ API Texture2D::Save")
+    saved = {}
+    _, img = cpp_api.run(scenes.cornell_box(), (40, 24, 4, 8, 0, 3), saved)
+    assert np.array_equal(saved["png"], O.tonemap_rgb8(img))
+    peak = img.max(axis=2, keepdims=True)
+    assert np.all(np.abs(saved["hdr"] - img) <= peak / 128 + 1e-30)

@@ -67,7 +67,7 @@ def test_sizes_and_alignments(gpu, n, in_off, out_off):
     big_in[in_off:in_off + 3 * n] = torch.from_numpy(vals).cuda()
     big_out = torch.full((3 * n + 8,), 77, dtype=torch.uint8, device="cuda")
     src, dst = big_in[in_off:in_off + 3 * n], big_out[out_off:out_off + 3 * n]
-    c.check(c._lib.pt_tonemap_rgb8(c.handle, n, src.data_ptr(), dst.data_ptr(), None), "pt_tonemap_rgb8")
+    c.check(c._lib.pt_tonemap_rgb8(c.handle, n, src.data_ptr(), dst.data_ptr(), 1, None), "pt_tonemap_rgb8")
     torch.cuda.synchronize()
     out = big_out.cpu().numpy()
     assert np.array_equal(out[out_off:out_off + 3 * n], O.tonemap_rgb8(vals.reshape(-1, 3)).ravel())
@@ -75,11 +75,21 @@ def test_sizes_and_alignments(gpu, n, in_off, out_off):
     assert np.all(out[:out_off] == 77) and np.all(out[out_off + 3 * n:] == 77)
 
 
+def test_host_buffers(gpu):
+    # on_device = 0: host pointers, staged through device memory
+    rng = np.random.default_rng(3)
+    vals = (10.0 ** rng.uniform(-3, 1, (1001, 3))).astype(np.float32)
+    out = np.zeros((1001, 3), np.uint8)
+    c = ctx()
+    c.check(c._lib.pt_tonemap_rgb8(c.handle, 1001, vals.ctypes.data, out.ctypes.data, 0, None), "pt_tonemap_rgb8")
+    assert np.array_equal(out, O.tonemap_rgb8(vals))
+
+
 def test_bad_arguments_are_rejected(gpu):
     import torch
     c = ctx()
     x = torch.zeros(3, dtype=torch.float32, device="cuda")
-    assert c._lib.pt_tonemap_rgb8(c.handle, -1, x.data_ptr(), x.data_ptr(), None) == -1
-    assert c._lib.pt_tonemap_rgb8(c.handle, 1, None, x.data_ptr(), None) == -1
-    assert c._lib.pt_tonemap_rgb8(c.handle, 0, None, None, None) == 0
+    assert c._lib.pt_tonemap_rgb8(c.handle, -1, x.data_ptr(), x.data_ptr(), 1, None) == -1
+    assert c._lib.pt_tonemap_rgb8(c.handle, 1, None, x.data_ptr(), 1, None) == -1
+    assert c._lib.pt_tonemap_rgb8(c.handle, 0, None, None, 1, None) == 0
     assert re.search("tonemap", c._lib.pt_last_error(c.handle).decode())
