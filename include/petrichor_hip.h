@@ -299,6 +299,18 @@ int pt_server_get_stats(pt_server* server, pt_server_stats* stats);
 int pt_tonemap_rgb8(pt_context* ctx, int64_t num_pixels, const float* rgb, uint8_t* rgb8_out, int32_t on_device,
                     void* stream);
 
+/* ---- input -----------------------------------------------------------------------
+ * Triangles of a Wavefront OBJ text (len bytes) as Mesh::Load receives them from Assimp
+ * 4.1.0 read with no post-processing (Core/Geometry/Mesh.cpp:18-125): one triangle per
+ * face (its first three corners), in file order; positions[9 t..] the corners' positions,
+ * normals[9 t..] their normals normalised (+Z for a face without normal indices).
+ * Numbers are read as Assimp's fast_atof reads them. Fills at most max_triangles (pass 0
+ * to count) and sets *num_triangles to the file's count. Host pointers, no device.
+ * Errors (with the line in err, err_len bytes): PT_ERR_ARG for malformed text or an index
+ * out of range, PT_ERR_UNSUPPORTED for point and line elements. */
+int pt_obj_parse(const char* text, int64_t len, int64_t max_triangles, float* positions, float* normals,
+                 int64_t* num_triangles, char* err, int32_t err_len);
+
 #ifdef __cplusplus
 }
 #endif

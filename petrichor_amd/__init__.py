@@ -153,6 +153,8 @@ def lib():
     L.pt_server_fetch.argtypes = [vp, ctypes.c_int64, _f]
     L.pt_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
     L.pt_tonemap_rgb8.argtypes = [vp, ctypes.c_int64, vp, vp, ctypes.c_int32, vp]
+    L.pt_obj_parse.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, _f, _f, _l, ctypes.c_char_p,
+                               ctypes.c_int32]
     _lib = L
     return L
 
@@ -164,7 +166,7 @@ EXPORTED = ["pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_scene_create
             "pt_render_batch", "pt_engine_set_tuning", "pt_engine_get_tuning", "pt_server_create", "pt_server_destroy", "pt_server_last_error",
             "pt_server_add_scene", "pt_server_remove_scene", "pt_server_scene_resident", "pt_server_submit",
             "pt_server_poll", "pt_server_wait", "pt_server_preview", "pt_server_fetch", "pt_server_get_stats",
-            "pt_tonemap_rgb8"]
+            "pt_tonemap_rgb8", "pt_obj_parse"]
 
 
 def _ptr(a, t):
