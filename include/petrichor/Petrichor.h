@@ -6,8 +6,9 @@
 //
 // Only what the GPU path renders is here: triangles (flat or smooth), Lambert, GGX,
 // Glass, Emission and MixMaterial, an equirect environment given as a Texture2D,
-// the MIS and simple integrators, and Texture2D::Save (PNG bytes converted on the GPU,
-// Radiance HDR). File loading (OBJ models, JSON scenes, images) and the denoising AOVs are
+// the MIS and simple integrators, Texture2D::Save (PNG bytes converted on the GPU,
+// Radiance HDR), and the reference's file loading: JSON scene and render-setting files,
+// OBJ meshes and Radiance .hdr environment maps. Image textures and the denoising AOVs are
 // not part of this API.
 //
 // Floating point: the camera set-up below must be compiled without FMA contraction
@@ -21,6 +22,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <filesystem>
 #include <functional>
 #include <memory>
@@ -223,6 +225,231 @@ private:
     ShadingTypes m_shadingType;
 };
 
+namespace detail
+{
+
+inline bool readFile(const std::filesystem::path& path, std::string& out)
+{
+    FILE* f = std::fopen(path.string().c_str(), "rb");
+    if (!f) return false;
+    out.clear();
+    char buf[1 << 16];
+    size_t k;
+    while ((k = std::fread(buf, 1, sizeof buf, f)) > 0) out.append(buf, k);
+    std::fclose(f);
+    return true;
+}
+
+// A JSON value, as much of it as the reference's scene and setting files use (numbers
+// are read as doubles, as nlohmann::json reads them).
+struct Json
+{
+    enum Type
+    {
+        Null,
+        Bool,
+        Number,
+        String,
+        Array,
+        Object
+    } type = Null;
+    bool boolean = false;
+    double number = 0.0;
+    std::string str;
+    std::vector<Json> items;
+    std::vector<std::pair<std::string, Json>> members;
+
+    const Json* find(std::string_view key) const
+    {
+        if (type != Object) return nullptr;
+        for (const auto& m : members)
+            if (m.first == key) return &m.second;
+        return nullptr;
+    }
+};
+
+inline bool parseJson(const char*& p, const char* e, Json& out, int depth = 0)
+{
+    auto ws = [&] {
+        while (p < e && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')) ++p;
+    };
+    auto str = [&](std::string& s) {
+        if (p >= e || *p != '"') return false;
+        ++p;
+        s.clear();
+        while (p < e && *p != '"') {
+            char c = *p++;
+            if (c == '\\') {
+                if (p >= e) return false;
+                c = *p++;
+                switch (c) {
+                case 'n': c = '\n'; break;
+                case 't': c = '\t'; break;
+                case 'r': c = '\r'; break;
+                case 'b': c = '\b'; break;
+                case 'f': c = '\f'; break;
+                case 'u': {
+                    if (e - p < 4) return false;
+                    const unsigned cp = (unsigned)std::stoul(std::string(p, p + 4), nullptr, 16);
+                    p += 4;
+                    if (cp < 0x80) {
+                        s += (char)cp;
+                    } else if (cp < 0x800) {
+                        s += (char)(0xc0 | (cp >> 6));
+                        s += (char)(0x80 | (cp & 0x3f));
+                    } else {
+                        s += (char)(0xe0 | (cp >> 12));
+                        s += (char)(0x80 | ((cp >> 6) & 0x3f));
+                        s += (char)(0x80 | (cp & 0x3f));
+                    }
+                    continue;
+                }
+                default: break; // \" \\ \/
+                }
+            }
+            s += c;
+        }
+        if (p >= e) return false;
+        ++p;
+        return true;
+    };
+    if (depth > 64) return false;
+    ws();
+    if (p >= e) return false;
+    if (*p == '{') {
+        out.type = Json::Object;
+        ++p;
+        ws();
+        if (p < e && *p == '}') return ++p, true;
+        for (;;) {
+            std::string key;
+            ws();
+            if (!str(key)) return false;
+            ws();
+            if (p >= e || *p != ':') return false;
+            ++p;
+            Json v;
+            if (!parseJson(p, e, v, depth + 1)) return false;
+            out.members.emplace_back(std::move(key), std::move(v));
+            ws();
+            if (p < e && *p == ',') {
+                ++p;
+                continue;
+            }
+            if (p < e && *p == '}') return ++p, true;
+            return false;
+        }
+    }
+    if (*p == '[') {
+        out.type = Json::Array;
+        ++p;
+        ws();
+        if (p < e && *p == ']') return ++p, true;
+        for (;;) {
+            Json v;
+            if (!parseJson(p, e, v, depth + 1)) return false;
+            out.items.push_back(std::move(v));
+            ws();
+            if (p < e && *p == ',') {
+                ++p;
+                continue;
+            }
+            if (p < e && *p == ']') return ++p, true;
+            return false;
+        }
+    }
+    if (*p == '"') {
+        out.type = Json::String;
+        return str(out.str);
+    }
+    auto word = [&](const char* w) {
+        const size_t n = std::strlen(w);
+        if ((size_t)(e - p) >= n && std::strncmp(p, w, n) == 0) return p += n, true;
+        return false;
+    };
+    if (word("true")) return out.type = Json::Bool, out.boolean = true, true;
+    if (word("false")) return out.type = Json::Bool, true;
+    if (word("null")) return true;
+    const char* q = p;
+    while (q < e && (std::strchr("+-.eE", *q) || (*q >= '0' && *q <= '9'))) ++q;
+    if (q == p) return false;
+    out.type = Json::Number;
+    out.number = std::strtod(std::string(p, q).c_str(), nullptr);
+    p = q;
+    return true;
+}
+
+inline bool loadJson(const std::filesystem::path& path, Json& out)
+{
+    std::string text;
+    if (!readFile(path, text)) return false;
+    const char* p = text.data();
+    return parseJson(p, p + text.size(), out);
+}
+
+// stbi_loadf of a Radiance file (Core/Texture2D.cpp:165-190): flat or run-length RGBE
+// scanlines, byte * 2^(exponent - 136).
+inline bool loadHdr(const std::filesystem::path& path, int& w, int& h, std::vector<float>& rgb)
+{
+    std::string d;
+    if (!readFile(path, d)) return false;
+    if (d.rfind("#?RADIANCE\n", 0) != 0 && d.rfind("#?RGBE\n", 0) != 0) return false;
+    size_t pos = d.find('\n') + 1;
+    bool fmt = false;
+    for (;;) {
+        const size_t end = d.find('\n', pos);
+        if (end == std::string::npos) return false;
+        const std::string line = d.substr(pos, end - pos);
+        pos = end + 1;
+        if (line.empty()) break;
+        if (line == "FORMAT=32-bit_rle_rgbe") fmt = true;
+    }
+    const size_t end = d.find('\n', pos);
+    if (!fmt || end == std::string::npos) return false;
+    if (std::sscanf(d.substr(pos, end - pos).c_str(), "-Y %d +X %d", &h, &w) != 2 || w <= 0 || h <= 0) return false;
+    pos = end + 1;
+    const uint8_t* b = (const uint8_t*)d.data() + pos;
+    const size_t avail = d.size() - pos;
+    std::vector<uint8_t> e4((size_t)w * h * 4);
+    const bool rle = w >= 8 && w < 32768 && avail >= 4 && b[0] == 2 && b[1] == 2 && !(b[2] & 0x80);
+    if (!rle) {
+        if (avail < e4.size()) return false;
+        std::memcpy(e4.data(), b, e4.size());
+    } else {
+        size_t i = 0;
+        for (int y = 0; y < h; ++y) {
+            if (i + 4 > avail || b[i] != 2 || b[i + 1] != 2 || ((b[i + 2] << 8) | b[i + 3]) != w) return false;
+            i += 4;
+            for (int ch = 0; ch < 4; ++ch) {
+                for (int x = 0; x < w;) {
+                    if (i >= avail) return false;
+                    int count = b[i++];
+                    if (count > 128) {
+                        count -= 128;
+                        if (x + count > w || i >= avail) return false;
+                        for (int k = 0; k < count; ++k) e4[4 * ((size_t)y * w + x + k) + ch] = b[i];
+                        ++i;
+                    } else {
+                        if (count == 0 || x + count > w || i + count > avail) return false;
+                        for (int k = 0; k < count; ++k) e4[4 * ((size_t)y * w + x + k) + ch] = b[i + k];
+                        i += count;
+                    }
+                    x += count;
+                }
+            }
+        }
+    }
+    rgb.resize((size_t)w * h * 3);
+    for (size_t k = 0; k < (size_t)w * h; ++k) {
+        const uint8_t* q = &e4[4 * k];
+        const float f = q[3] ? (float)std::ldexp(1.0f, q[3] - 136) : 0.0f;
+        for (int c = 0; c < 3; ++c) rgb[3 * k + c] = q[3] ? q[c] * f : 0.0f;
+    }
+    return true;
+}
+
+} // namespace detail
+
 // Core/Geometry/Mesh.h, built in memory: vertices and triangles over them.
 class Mesh
 {
@@ -240,6 +467,45 @@ public:
     }
     void AddTriangle(const Triangle& t) { m_triangles.push_back(t); }
     const std::vector<Triangle>& GetTriangles() const { return m_triangles; }
+
+    // Core/Geometry/Mesh.cpp:18-125: the triangles of an OBJ file as Assimp 4.1 hands them
+    // over (pt_obj_parse), one vertex per face corner, all with the given material and
+    // shading. The mesh must be empty. Returns false (error text in *error) when the file
+    // cannot be read or parsed; the reference logs the error and leaves the mesh empty.
+    bool Load(const std::filesystem::path& path, const MaterialBase* material,
+              ShadingTypes shadingType = ShadingTypes::Flat, std::string* error = nullptr)
+    {
+        auto fail = [&](const std::string& msg) {
+            if (error) *error = msg;
+            return false;
+        };
+        if (!m_vertices.empty() || !m_triangles.empty()) return fail("Mesh::Load needs an empty mesh");
+        std::string text;
+        if (!detail::readFile(path, text)) return fail("cannot read " + path.string());
+        int64_t n = 0;
+        char err[256] = {0};
+        if (pt_obj_parse(text.data(), (int64_t)text.size(), 0, nullptr, nullptr, &n, err, sizeof err) != PT_OK)
+            return fail(err);
+        std::vector<float> p(9 * (size_t)n), nr(9 * (size_t)n);
+        if (n > 0 && pt_obj_parse(text.data(), (int64_t)text.size(), n, p.data(), nr.data(), &n, err, sizeof err) != PT_OK)
+            return fail(err);
+        m_vertices.reserve(3 * (size_t)n);
+        m_triangles.reserve((size_t)n);
+        for (size_t t = 0; t < (size_t)n; ++t) {
+            const Vertex* v[3];
+            for (int k = 0; k < 3; ++k) {
+                const float* q = &p[9 * t + 3 * k];
+                const float* m = &nr[9 * t + 3 * k];
+                m_vertices.emplace_back(Math::Vector3f(q[0], q[1], q[2]), Math::Vector3f(m[0], m[1], m[2]),
+                                        Math::Vector3f());
+                v[k] = &m_vertices.back();
+            }
+            Triangle tri(v[0], v[1], v[2], shadingType);
+            tri.SetMaterial(material);
+            m_triangles.push_back(tri);
+        }
+        return true;
+    }
 
 private:
     std::vector<Vertex> m_vertices;
@@ -398,8 +664,22 @@ public:
     const Color3f& GetBaseColor() const { return m_baseColor; }
     float GetZAxisRotation() const { return m_ZAxisRotation; }
     bool UseEnvImportanceSampling() const { return m_texEnv && m_texEnv->IsValid() && m_useImportanceSampling; }
+    // Environment::Load (Core/Environment.cpp:15-27) of a Radiance .hdr map, which the
+    // environment then owns (shared by its copies). False for another format or a read error.
+    bool Load(const std::filesystem::path& path)
+    {
+        int w = 0, h = 0;
+        std::vector<float> rgb;
+        if (path.extension() != ".hdr" || !detail::loadHdr(path, w, h, rgb)) return false;
+        auto tex = std::make_shared<Texture2D>(w, h);
+        std::memcpy(tex->GetRawDataPtr(), rgb.data(), rgb.size() * sizeof(float));
+        m_ownedTex = tex;
+        m_texEnv = tex.get();
+        return true;
+    }
 
 private:
+    std::shared_ptr<Texture2D> m_ownedTex;
     const Texture2D* m_texEnv = nullptr;
     Color3f m_baseColor;
     float m_ZAxisRotation = 0.0f;
@@ -516,10 +796,122 @@ public:
     void SetTargetTexture(AOVType::Value aovType, Texture2D* targetTex) { m_targetTextures[aovType] = targetTex; }
     Texture2D* GetTargetTexture(AOVType::Value aovType) const { return m_targetTextures[aovType]; }
     const RenderSetting& GetRenderSetting() const { return m_renderSetting; }
-    // The reference reads these from a settings file (Scene::LoadRenderSetting).
     void SetRenderSetting(const RenderSetting& renderSetting) { m_renderSetting = renderSetting; }
 
+    // Scene::LoadRenderSetting (Core/Scene.cpp:12-26, RenderSettingLoader.cpp:15-64): the
+    // keys present in the JSON file override the current values (numbers truncated to int).
+    bool LoadRenderSetting(const std::filesystem::path& path)
+    {
+        detail::Json j;
+        if (!detail::loadJson(path, j)) return false;
+        auto read = [&](int* v, const char* key) {
+            if (const detail::Json* x = j.find(key); x && x->type == detail::Json::Number) *v = (int)x->number;
+        };
+        read(&m_renderSetting.outputWidth, "outputWidth");
+        read(&m_renderSetting.outputHeight, "outputHeight");
+        read(&m_renderSetting.numSamplesPerPixel, "spp");
+        read(&m_renderSetting.numSppForDenoising, "denoising_spp");
+        read(&m_renderSetting.numMaxBounces, "maxBounces");
+        read(&m_renderSetting.tileWidth, "tileWidth");
+        read(&m_renderSetting.tileHeight, "tileHeight");
+        read(&m_renderSetting.numThreads, "numThreads");
+        return true;
+    }
+
+    // Scene::LoadModel (Core/Scene.cpp:28-51): an OBJ file's triangles, smooth-shaded, with
+    // the named material ("default" without a name). The scene owns the mesh. False (with
+    // *error) for an unregistered material or a file that cannot be read.
+    bool LoadModel(const std::filesystem::path& path, std::string_view materialName = "default",
+                   std::string* error = nullptr)
+    {
+        const MaterialBase* material = GetMaterial(materialName);
+        if (!material) {
+            if (error) *error = "material '" + std::string(materialName) + "' is not registered";
+            return false;
+        }
+        auto mesh = std::make_unique<Mesh>();
+        if (!mesh->Load(path, material, ShadingTypes::Smooth, error)) return false;
+        AppendMesh(*mesh);
+        m_loadedMeshes.push_back(std::move(mesh));
+        return true;
+    }
+
+    // Scene::LoadAssets = SceneLoaderJson::Load (Core/SceneLoader.cpp:17-296): materials,
+    // camera, environment and OBJ assets of a JSON scene file, paths relative to it. As in
+    // the reference: "default" is a white Lambert, "emission" materials register nothing,
+    // and the camera takes SetFNumber before SetFocalLength. False (with *error) where the
+    // reference would use a missing material or texture, for textured materials (not
+    // rendered by the GPU path) and for environment maps other than Radiance .hdr.
+    bool LoadAssets(const std::filesystem::path& path, std::string* error = nullptr)
+    {
+        auto fail = [&](const std::string& msg) {
+            if (error) *error = msg;
+            return false;
+        };
+        using detail::Json;
+        Json j;
+        if (!detail::loadJson(path, j)) return fail("cannot read or parse " + path.string());
+        const std::filesystem::path base = path.parent_path();
+        auto vec3 = [](const Json* o, const char* key) {
+            float v[3] = {0.0f, 0.0f, 0.0f};
+            if (const Json* a = o ? o->find(key) : nullptr; a && a->type == Json::Array)
+                for (size_t i = 0; i < a->items.size() && i < 3; ++i) v[i] = (float)a->items[i].number;
+            return Math::Vector3f(v[0], v[1], v[2]);
+        };
+        auto num = [](const Json* o, const char* key, float def) {
+            const Json* x = o ? o->find(key) : nullptr;
+            return x && x->type == Json::Number ? (float)x->number : def;
+        };
+        auto text = [](const Json* o, const char* key) {
+            const Json* x = o ? o->find(key) : nullptr;
+            return x && x->type == Json::String ? x->str : std::string();
+        };
+        RegisterMaterial("default", std::make_unique<Lambert>(Color3f::One()));
+        if (const Json* mats = j.find("materials"); mats && mats->type == Json::Array) {
+            for (const Json& m : mats->items) {
+                const std::string type = text(&m, "type"), name = text(&m, "name");
+                if (type.empty() || name.empty()) continue;
+                if (!text(&m, "color_tex").empty() || !text(&m, "roughness_tex").empty())
+                    return fail("material '" + name + "': textured materials are not rendered by the GPU path");
+                if (type == "lambert") {
+                    RegisterMaterial(name, std::make_unique<Lambert>(vec3(&m, "base_color")));
+                } else if (type == "glass") {
+                    RegisterMaterial(name, std::make_unique<Glass>(vec3(&m, "color"), num(&m, "ior", 1.0f)));
+                } else if (type == "glossy") {
+                    RegisterMaterial(name, std::make_unique<GGX>(vec3(&m, "color"), num(&m, "roughness", 0.0f)));
+                } else if (type == "mix") {
+                    const MaterialBase* m0 = GetMaterial(text(&m, "mat0_name"));
+                    const MaterialBase* m1 = GetMaterial(text(&m, "mat1_name"));
+                    if (!m0 || !m1) return fail("material '" + name + "' mixes unregistered materials");
+                    RegisterMaterial(name, std::make_unique<MixMaterial>(m0, m1, num(&m, "mix", 0.5f)));
+                }
+            }
+        }
+        const Json* cd = j.find("camera");
+        auto camera = std::make_unique<Camera>();
+        camera->SetPosition(vec3(cd, "position"));
+        camera->LookAt(vec3(cd, "look_at"));
+        camera->FocusTo(vec3(cd, "focus_pos"));
+        camera->SetFNumber(num(cd, "f_number", 2.8f));
+        camera->SetFocalLength(num(cd, "focal_length", 55e-3f));
+        SetMainCamera(std::move(camera));
+        const Json* ed = j.find("env");
+        Environment env;
+        env.SetBaseColor(vec3(ed, "base_color"));
+        if (const std::string tex = text(ed, "texture"); !tex.empty() && !env.Load(base / tex))
+            return fail("environment map " + (base / tex).string() + ": not a readable Radiance .hdr file");
+        SetEnvironment(env);
+        if (const Json* assets = j.find("assets"); assets && assets->type == Json::Array) {
+            for (const Json& a : assets->items) {
+                const std::string mat = a.find("mat") ? text(&a, "mat") : std::string("default");
+                if (!LoadModel(base / text(&a, "path"), mat, error)) return false;
+            }
+        }
+        return true;
+    }
+
 private:
+    std::vector<std::unique_ptr<Mesh>> m_loadedMeshes;
     std::vector<const GeometryBase*> m_geometries;
     std::vector<const GeometryBase*> m_lights;
     std::unordered_map<std::string, std::unique_ptr<const MaterialBase>> m_materials;

@@ -50,10 +50,13 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp] + objs)
     os.replace(tmp, LIB)
-    # the C++ host API's test driver (tests/cpp), linked against the library
+    # the C++ host API's test drivers (tests/cpp), linked against the library
     _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-I", os.path.join(ROOT, "include"),
           os.path.join(ROOT, "tests", "cpp", "render_api.cpp"), "-L", HERE, "-lpetrichor_hip",
           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib", "-o", os.path.join(bdir, "render_api")])
+    _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-I", os.path.join(ROOT, "include"),
+          os.path.join(ROOT, "tests", "cpp", "load_api.cpp"), "-L", HERE, "-lpetrichor_hip",
+          "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib", "-o", os.path.join(bdir, "load_api")])
     # test oracle (CPU restatement; also builds oracle/_ref where the reference exists)
     subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     return LIB

@@ -7,6 +7,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "petrichor_amd", "_build", "render_api")
+LOAD_DRIVER = os.path.join(ROOT, "petrichor_amd", "_build", "load_api")
 
 
 def run(scene, size=None, saved=None):
@@ -33,3 +34,18 @@ def run(scene, size=None, saved=None):
     cam = data[:16]
     img = data[16:].reshape(size[1], size[0], 3) if size else None
     return cam, img
+
+
+def run_loaded(scene_json, settings_json, integrator, seed):
+    """Loads and renders a scene file through the C++ API (Scene::LoadRenderSetting,
+    Scene::LoadAssets, Petrichor::Render); returns (camera state [16], image)."""
+    import json
+    with open(settings_json) as f:
+        rs = json.load(f)
+    with tempfile.TemporaryDirectory() as d:
+        fout = os.path.join(d, "out.bin")
+        r = subprocess.run([LOAD_DRIVER, scene_json, settings_json, fout, str(integrator), str(seed)],
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout
+        data = np.fromfile(fout, np.float32)
+    return data[:16], data[16:].reshape(int(rs["outputHeight"]), int(rs["outputWidth"]), 3)

@@ -10,6 +10,8 @@ import oracle_lib as O
 import petrichor_amd as P
 from gpu_util import ctx, log, rel_err
 from petrichor_amd import loader
+import cpp_api
+from gpu_util import bits_equal
 from test_cpu_loader import cornell_files, write_rle_hdr
 
 pytestmark = pytest.mark.gpu
@@ -53,6 +55,19 @@ def loaded_scene(tmp_path):
     doc["assets"].append({"path": "ball.obj", "mat": "mx"})
     (tmp_path / "scene.json").write_text(json.dumps(doc))
     return loader.load_scene(str(tmp_path / "scene.json"))
+
+
+def test_cpp_loaded_scene_equals_python_loaded(gpu, tmp_path):
+    # the same files through Scene::LoadRenderSetting + Scene::LoadAssets + Petrichor::Render
+    sc = loaded_scene(tmp_path)
+    (tmp_path / "settings.json").write_text(json.dumps({"outputWidth": 48, "outputHeight": 32, "spp": 4,
+                                                         "maxBounces": 8}))
+    log("C++ loaded scene")
+    cam, img = cpp_api.run_loaded(str(tmp_path / "scene.json"), str(tmp_path / "settings.json"), 0, 9)
+    ref, _ = ctx().scene(sc).build_bvh(16).render(48, 32, 4, 8, 0, 9)
+    assert bits_equal(cam, sc["camera"])
+    assert img.mean() > 0.01
+    assert bits_equal(img, ref)
 
 
 @pytest.mark.parametrize("integrator", [0, 1])
