@@ -559,6 +559,54 @@ def tonemap_bench(args):
     print(json.dumps(res), flush=True)
 
 
+def aov_bench(args):
+    """The denoising AOVs (pt_render_aov) of the single large scene at --width x --height:
+    --spp denoising samples per pixel, albedo then normal; Mrays/s and the oracle's rate on
+    a pixel sample."""
+    import torch
+    import petrichor_amd as P
+    from petrichor_amd import scenes
+    sc = scenes.spheres_scene(seed=2)
+    ctx = P.Context(0)
+    bvh = ctx.scene(sc).build_bvh(16)
+    W, H, SPP, D = args.width, args.height, args.spp, args.depth
+    out = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    res = {}
+    for name, aov in (("albedo", P.AOV_DENOISING_ALBEDO), ("normal", P.AOV_DENOISING_NORMAL)):
+        bvh.render_aov(W, H, SPP, D, aov, 1, out=out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        q = 0
+        for _ in range(max(args.steps, 1)):
+            _, st = bvh.render_aov(W, H, SPP, D, aov, 1, out=out)
+            q += st["queries"]
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        res[name] = {"mrays_s": round(q / wall / 1e6, 1), "ms": round(wall / max(args.steps, 1) * 1e3, 2),
+                     "queries": int(q / max(args.steps, 1))}
+        log(name, res[name])
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        o = O.OracleScene(sc).use_tree(bvh.export_nodes() | {"prim_ids": bvh.export_order()})
+        rng = np.random.default_rng(3)
+        pix = rng.choice(W * H, 4096, replace=False).astype(np.int32)
+        threads = min(os.cpu_count() or 1, 16)
+        t1 = time.perf_counter()
+        ref, q = o.render(W, H, SPP, D, 2, 1, threads, pixels=pix)
+        cpu_s = time.perf_counter() - t1
+        img = bvh.render_aov(W, H, SPP, D, P.AOV_DENOISING_ALBEDO, 1)[0].reshape(-1, 3)[pix]
+        cpu = {"value": round(q / cpu_s / 1e6, 2), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": "albedo AOV of 4096 random pixels", "pixels_bit_identical": int(np.all(img == ref, axis=1).sum())}
+    print(json.dumps({"metric": "denoising AOV Mrays/s (albedo, normal)", "value": res["albedo"]["mrays_s"],
+                      "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
+                      "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "denoising AOVs of the single large scene, %dx%d, %d spp, depth %d"
+                                             % (W, H, SPP, D)},
+                      "aovs": res, "cpu_baseline": cpu}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -571,12 +619,13 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--depth", type=int, default=12)
-    ap.add_argument("--mode", default="frame", choices=["frame", "trace", "batch", "serve", "copy", "tonemap"],
+    ap.add_argument("--mode", default="frame", choices=["frame", "trace", "batch", "serve", "copy", "tonemap", "aov"],
                     help="frame: the single large scene (default); trace: the BVH build and "
                          "traversal microbenchmark (10M triangles, 2^26 rays); copy: a plain 1 GiB "
                          "read + 1 GiB write elementwise kernel, the reference for scaling rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE; tonemap: the 8-bit output kernel on a "
-                         "--width x --height image")
+                         "--width x --height image; aov: the denoising AOVs of the single "
+                         "large scene")
     ap.add_argument("--triangles", type=int, default=10_000_000, help="trace mode: triangle count")
     ap.add_argument("--bvh-width", type=int, default=None, choices=[2, 8],
                     help="BVH width: 2 walks the binary tree in the reference's order (frame default), "
@@ -617,6 +666,8 @@ def main():
         return serve_bench(args)
     if args.mode == "tonemap":
         return tonemap_bench(args)
+    if args.mode == "aov":
+        return aov_bench(args)
     log("generating scene")
     t0 = time.perf_counter()
     sc = scenes.spheres_scene(seed=2)

@@ -962,6 +962,8 @@ public:
         Texture2D* target = scene.GetTargetTexture(Scene::AOVType::Rendered);
         if (!target || !target->IsValid()) return fail("Target texture has not been set.");
         if (!scene.GetMainCamera()) return fail("The scene has no camera.");
+        if (scene.GetTargetTexture(Scene::AOVType::UV))
+            return fail("The UV AOV is not rendered (the GPU scene has no texture coordinates).");
         const RenderSetting& rs = scene.GetRenderSetting();
         // TileManager.cpp:8-43 counts (W / tw + 1) * (H / th + 1) tiles
         m_numTiles = (uint32_t)((target->GetWidth() / rs.tileWidth + 1) * (target->GetHeight() / rs.tileHeight + 1));
@@ -981,7 +983,19 @@ public:
         pt_render_desc rd{target->GetWidth(), target->GetHeight(), rs.numSamplesPerPixel, rs.numMaxBounces,
                           m_integrator == Integrator::PathTracing ? PT_INTEGRATOR_MIS : PT_INTEGRATOR_SIMPLE, m_seed,
                           0};
-        const int rc = pt_render(s, bvh, &rd, target->GetRawDataPtr(), 0, nullptr);
+        int rc = pt_render(s, bvh, &rd, target->GetRawDataPtr(), 0, nullptr);
+        // the denoising AOV passes (Core/Petrichor.cpp:104-261) into their target textures,
+        // at RenderSetting::numSppForDenoising samples; the UV pass is not rendered
+        const std::pair<Scene::AOVType::Value, int32_t> aovs[2] = {
+            {Scene::AOVType::DenoisingAlbedo, PT_AOV_DENOISING_ALBEDO},
+            {Scene::AOVType::DenoisingNormal, PT_AOV_DENOISING_NORMAL}};
+        for (const auto& [type, aov] : aovs) {
+            Texture2D* tex = scene.GetTargetTexture(type);
+            if (rc != PT_OK || !tex || !tex->IsValid()) continue;
+            pt_render_desc ad{tex->GetWidth(), tex->GetHeight(), rs.numSppForDenoising, rs.numMaxBounces,
+                              PT_INTEGRATOR_MIS, m_seed, 0};
+            rc = pt_render_aov(s, bvh, aov, &ad, tex->GetRawDataPtr(), 0, nullptr);
+        }
         if (rc != PT_OK) fail(pt_last_error(m_ctx));
         pt_bvh_destroy(bvh);
         pt_scene_destroy(s);

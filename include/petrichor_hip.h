@@ -170,6 +170,20 @@ int pt_trace_stats(pt_bvh* bvh, int64_t n, const float* rays, float* t_out, int3
 int pt_render(pt_scene* scene, pt_bvh* bvh, const pt_render_desc* desc, float* rgb_out,
               int32_t out_on_device, pt_render_stats* stats);
 
+/* Denoising AOVs (the DenoisingAlbedo and DenoisingNormal passes of
+ * Core::Petrichor::Render, Core/Petrichor.cpp:104-261, with Core/AOV/AOVDenoisingAlbedo.cpp
+ * and AOVDenoisingNormal.cpp): each pixel averages desc->spp camera rays (the reference's
+ * RenderSetting::numSppForDenoising), followed through glass and zero-roughness GGX to the
+ * first other hit, for at most desc->max_bounces hits. Albedo: throughput x (1 for
+ * Lambert, F0 for GGX, the environment on a miss); normal: throughput x the shading normal
+ * (-direction on a miss). Seeds per pixel as pt_render; desc->integrator and flags are
+ * not read. rgb_out as pt_render; synchronous. The UV AOV is not rendered (the scene
+ * carries no texture coordinates). */
+#define PT_AOV_DENOISING_ALBEDO 0
+#define PT_AOV_DENOISING_NORMAL 1
+int pt_render_aov(pt_scene* scene, pt_bvh* bvh, int32_t aov, const pt_render_desc* desc, float* rgb_out,
+                  int32_t out_on_device, pt_render_stats* stats);
+
 /* ---- batched rendering ---------------------------------------------------------
  * An engine holds a pool of path slots (one per pixel of the jobs in flight) in device
  * memory. Jobs of different scenes, sizes and settings share every launch: one

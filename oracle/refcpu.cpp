@@ -1210,6 +1210,47 @@ static V render_simple(Ctx& c, int px, int py, int w, int h, int spp, int max_bo
     return divs(sum, (float)spp);
 }
 
+// AOVDenoisingAlbedo / AOVDenoisingNormal::CalcPathContribution for one pixel, averaged
+// over spp camera rays (AOV/AOVDenoisingAlbedo.cpp:19-133, AOV/AOVDenoisingNormal.cpp:
+// 19-128): the path goes through glass and zero-roughness GGX (GetAlpha() == 0) to the
+// first other hit. Albedo: throughput * GetAlbedo (One for an untextured Lambert), or * F0
+// (GGX), or * the environment on a miss. Normal: throughput * the shading normal, or *
+// -dir on a miss. An Emission hit falls to the reference's default branch (zero); a path
+// that has not ended after max_bounces hits gives zero.
+static V render_aov(Ctx& c, int px, int py, int w, int h, int spp, int max_bounces, Sampler1D& s1, Sampler2D& s2,
+                    bool normal_aov)
+{
+    const Scene& sc = *c.sc;
+    V sum = mk(0.0f, 0.0f, 0.0f);
+    for (int s = 0; s < spp; s++) {
+        Ray ray = generate_ray(sc.cam, px, py, w, h, s2);
+        V contrib = mk(0.0f, 0.0f, 0.0f);
+        for (int bounce = 0; bounce < max_bounces; bounce++) {
+            float t;
+            const int hit = c.closest(ray, kEps, kInfinity, &t);
+            if (hit < 0) {
+                contrib = normal_aov ? mul(ray.throughput, neg(ray.dir)) : mul(ray.throughput, sc.env.color(ray.dir));
+                break;
+            }
+            const int m = sc.material_of(hit, s1.Next());
+            const Shading si = interpolate(sc, hit, ray, t);
+            const Material& mat = sc.mats[m];
+            if (mat.type == kLambert) {
+                contrib = mul(ray.throughput, normal_aov ? si.normal : mk(1.0f, 1.0f, 1.0f));
+                break;
+            }
+            if (mat.type == kGlossy && mat.alpha != 0.0f) {
+                contrib = mul(ray.throughput, normal_aov ? si.normal : ggx_f0(mat));
+                break;
+            }
+            if (mat.type != kGlass && mat.type != kGlossy) break; // Emission: zero
+            ray = next_ray(mat, ray, si, s2); // Glass (alpha 0) and smooth GGX
+        }
+        sum = add(sum, contrib);
+    }
+    return divs(sum, (float)spp);
+}
+
 } // namespace oracle
 
 using namespace oracle;
@@ -1388,8 +1429,10 @@ int64_t oracle_render(void* s, void* b, int width, int height, int spp, int max_
                 const uint32_t pix = (uint32_t)((int64_t)y * width + x);
                 Sampler1D s1(pixel_seed(seed, pix, 0));
                 Sampler2D s2(pixel_seed(seed, pix, 1), pixel_seed(seed, pix, 2));
-                const V r = integrator == 0 ? render_mis(c, x, y, width, height, spp, max_bounces, s1, s2)
-                                            : render_simple(c, x, y, width, height, spp, max_bounces, s1, s2);
+                const V r = integrator == 0   ? render_mis(c, x, y, width, height, spp, max_bounces, s1, s2)
+                            : integrator == 1 ? render_simple(c, x, y, width, height, spp, max_bounces, s1, s2)
+                                              : render_aov(c, x, y, width, height, spp, max_bounces, s1, s2,
+                                                           integrator == 3);
                 rgb_out[3 * slot] = r.x;
                 rgb_out[3 * slot + 1] = r.y;
                 rgb_out[3 * slot + 2] = r.z;

@@ -59,8 +59,10 @@ void oracle_trace_closest(void* scene, void* bvh, int64_t n, const float* rays, 
 /* Core/Accel/BruteForce.cpp:21-50: layout-independent closest hit. */
 void oracle_trace_brute(void* scene, int64_t n, const float* rays, float* t_out, int32_t* prim_out);
 
-/* Render with the MIS integrator (integrator 0, Core/Integrator/PathTracing.cpp) or the
- * simple path tracer (integrator 1, Core/Integrator/SimplePathTracing.cpp), per-pixel
+/* Render with the MIS integrator (integrator 0, Core/Integrator/PathTracing.cpp), the
+ * simple path tracer (integrator 1, Core/Integrator/SimplePathTracing.cpp) or a denoising
+ * AOV (2: albedo, Core/AOV/AOVDenoisingAlbedo.cpp; 3: normal, AOVDenoisingNormal.cpp;
+ * spp is then the denoising sample count), per-pixel
  * seeded samplers, 16x16 tiles on num_threads workers (0 = all hardware threads).
  * If pixels != NULL only those num_pixels pixel indices are rendered (rgb_out then holds
  * 3 floats per listed pixel). Returns the number of BVH queries issued. work (optional,

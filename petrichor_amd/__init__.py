@@ -75,6 +75,7 @@ RENDER_TIME_KERNELS = 1
 RENDER_PLAIN_LAUNCHES = 2
 RENDER_TRACE_STATS = 4
 RENDER_TWO_STREAMS = 8
+AOV_DENOISING_ALBEDO, AOV_DENOISING_NORMAL = 0, 1
 
 
 class ServerDesc(ctypes.Structure):
@@ -132,6 +133,8 @@ def lib():
     L.pt_trace_any.argtypes = [vp, ctypes.c_int64, vp, vp, vp]
     L.pt_trace_stats.argtypes = [vp, ctypes.c_int64, vp, vp, vp, _l]
     L.pt_render.argtypes = [vp, vp, ctypes.POINTER(RenderDesc), vp, ctypes.c_int32, ctypes.POINTER(RenderStats)]
+    L.pt_render_aov.argtypes = [vp, vp, ctypes.c_int32, ctypes.POINTER(RenderDesc), vp, ctypes.c_int32,
+                                ctypes.POINTER(RenderStats)]
     L.pt_engine_create.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, pvp]
     L.pt_engine_destroy.argtypes = [vp]
     L.pt_render_batch.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(JobDesc), ctypes.c_int32,
@@ -166,7 +169,7 @@ EXPORTED = ["pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_scene_create
             "pt_render_batch", "pt_engine_set_tuning", "pt_engine_get_tuning", "pt_server_create", "pt_server_destroy", "pt_server_last_error",
             "pt_server_add_scene", "pt_server_remove_scene", "pt_server_scene_resident", "pt_server_submit",
             "pt_server_poll", "pt_server_wait", "pt_server_preview", "pt_server_fetch", "pt_server_get_stats",
-            "pt_tonemap_rgb8", "pt_obj_parse"]
+            "pt_tonemap_rgb8", "pt_obj_parse", "pt_render_aov"]
 
 
 def _ptr(a, t):
@@ -481,6 +484,20 @@ class Bvh:
                      "kernels": {k: {"items": st.kernel_items[i], "launches": st.kernel_launches[i],
                                      "ms": st.kernel_ms[i], "nodes": st.kernel_nodes[i], "tris": st.kernel_tris[i]}
                             for i, k in enumerate(kinds)}}
+
+    def render_aov(self, width, height, spp, max_bounces, aov=AOV_DENOISING_ALBEDO, seed=1, out=None):
+        """A denoising AOV (pt_render_aov): AOV_DENOISING_ALBEDO or AOV_DENOISING_NORMAL,
+        spp the denoising sample count. Returns (rgb, stats) as render()."""
+        d = RenderDesc(width, height, spp, max_bounces, INTEGRATOR_MIS, seed, 0)
+        st = RenderStats()
+        if out is None:
+            res = np.zeros((height, width, 3), np.float32)
+            ptr, on_dev = res.ctypes.data, 0
+        else:
+            res, ptr, on_dev = out, out.data_ptr(), 1
+        self.ctx.check(self.ctx._lib.pt_render_aov(self.scene.handle, self.handle, aov, ctypes.byref(d), ptr, on_dev,
+                                                   ctypes.byref(st)), "pt_render_aov")
+        return res, {"queries": st.queries, "elapsed_ms": st.elapsed_ms}
 
     def close(self):
         if getattr(self, "handle", None):

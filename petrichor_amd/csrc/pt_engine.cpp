@@ -244,6 +244,53 @@ int pt_engine_destroy(pt_engine* e)
     return PT_OK;
 }
 
+int pt_render_aov(pt_scene* s, pt_bvh* b, int32_t aov, const pt_render_desc* d, float* rgb_out,
+                  int32_t out_on_device, pt_render_stats* stats)
+{
+    if (!s || !b || !d || !rgb_out) return PT_ERR_ARG;
+    pt_context* ctx = s->ctx;
+    if (aov != PT_AOV_DENOISING_ALBEDO && aov != PT_AOV_DENOISING_NORMAL)
+        return pt_fail(ctx, PT_ERR_ARG, "unknown AOV");
+    pt_job_desc jd;
+    jd.scene = s;
+    jd.bvh = b;
+    jd.render = *d;
+    jd.render.integrator = PT_INTEGRATOR_MIS; // not read by the AOV passes
+    jd.rgb_out = rgb_out;
+    jd.out_on_device = out_on_device;
+    int rc = check_job(ctx, jd);
+    if (rc != PT_OK) return rc;
+    PT_HIP(ctx, hipSetDevice(ctx->device));
+    EngineJob j = make_job(jd, 0);
+    rc = prepare_output(ctx, j);
+    if (rc != PT_OK) return rc;
+    unsigned long long* q_d = nullptr;
+    hipError_t e = hipMalloc((void**)&q_d, sizeof(unsigned long long));
+    const auto t0 = std::chrono::steady_clock::now();
+    if (e == hipSuccess) e = hipMemsetAsync(q_d, 0, sizeof(unsigned long long), ctx->stream);
+    if (e == hipSuccess)
+        e = launch_render_aov(pt_engine::scene_view(s, b), d->width, d->height, d->spp, d->max_bounces, d->seed,
+                              aov == PT_AOV_DENOISING_NORMAL, b->view.nodes8 != nullptr, j.out_d, q_d, ctx->stream);
+    unsigned long long q = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&q, q_d, sizeof q, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (q_d) (void)hipFree(q_d);
+    if (e != hipSuccess) {
+        if (j.own_out) (void)hipFree(j.out_d);
+        return pt_fail(ctx, e == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP, hipGetErrorString(e));
+    }
+    rc = finish_output(ctx, j, ctx->stream);
+    if (rc != PT_OK) return rc;
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->queries = (int64_t)q;
+        stats->iterations = 1;
+        stats->elapsed_ms = wall;
+    }
+    return PT_OK;
+}
+
 int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, int32_t out_on_device,
               pt_render_stats* stats)
 {

@@ -180,6 +180,11 @@ hipError_t launch_finish_paths(int integrator, const SceneView& sc, const JobVie
                                bool wide, const ShadeArgs& a, const HitBuffers& h, unsigned long long* queries,
                                hipStream_t st);
 
+// A denoising AOV (albedo, or the shading normal when normal_aov) of a w x h image, one
+// thread per pixel (pt_kernels.hip); queries (optional) receives the rays traced.
+hipError_t launch_render_aov(const SceneView& sc, int w, int h, int spp, int max_bounces, uint32_t seed,
+                             int normal_aov, bool wide, float* out, unsigned long long* queries, hipStream_t st);
+
 // The reference's 8-bit PNG bytes of num_pixels RGB float pixels (pt_post.hip).
 hipError_t launch_tonemap_rgb8(const float* rgb, uint8_t* out, int64_t num_pixels, hipStream_t st);
 

@@ -582,6 +582,82 @@ __device__ __forceinline__ void trace_single(const BvhView& b, V3 o, float dmin,
     *k = s.best;
 }
 
+// Denoising AOVs (AOVDenoisingAlbedo / AOVDenoisingNormal::Render and
+// CalcPathContribution, Core/AOV/AOVDenoisingAlbedo.cpp:19-133,
+// Core/AOV/AOVDenoisingNormal.cpp:19-128), one thread per pixel: spp camera rays, each
+// followed through glass and zero-roughness GGX to its first other hit. Albedo: the
+// throughput times GetAlbedo (one for an untextured Lambert), times F0 (GGX), or times
+// the environment on a miss; normal: the throughput times the shading normal, or times
+// -dir on a miss. An Emission hit gives zero (the reference's default branch), and so
+// does a path still going after max_bounces hits. The paths are short (most end at the
+// first hit), so they are not run through the wavefront queues.
+template <bool WIDE>
+__global__ void __launch_bounds__(256) k_render_aov(SceneView sc, int w, int h, int spp, int max_bounces,
+                                                    uint32_t seed, int normal_aov, float* out,
+                                                    unsigned long long* queries)
+{
+    __shared__ uint2 lds_stack[lds_stack_entries<WIDE>() * 256];
+    uint2* lds = lds_stack + threadIdx.x;
+    StackEntry spill[(WIDE ? PT_STACK8 : PT_MAX_STACK) - lds_stack_entries<WIDE>()];
+    const int n = w * h;
+    unsigned long long q = 0;
+    for (int pix = blockIdx.x * blockDim.x + threadIdx.x; pix < n; pix += gridDim.x * blockDim.x) {
+        const int px = pix % w, py = pix / w;
+        Samplers smp = samplers_for_pixel(seed, (uint32_t)pix);
+        V3 sum = v3(0.0f, 0.0f, 0.0f);
+        for (int s = 0; s < spp; ++s) {
+            Ray ray = generate_ray(sc.cam, px, py, w, h, smp);
+            V3 contrib = v3(0.0f, 0.0f, 0.0f);
+            for (int bounce = 0; bounce < max_bounces; ++bounce) {
+                float t;
+                int k;
+                trace_single<WIDE>(sc.bvh, ray.o, PT_EPS, ray.dir, PT_INFINITY, false, lds, spill, &t, &k);
+                ++q;
+                if (k < 0) {
+                    contrib = ray.throughput * (normal_aov ? -ray.dir : env_color(sc.env, ray.dir));
+                    break;
+                }
+                const int m = material_of(sc, k, next1(smp));
+                V3 pos, sn;
+                shade_point(sc, k, ray.o, ray.dir, t, &pos, &sn);
+                const MaterialDev mat = load_mat(sc, m);
+                if (mat.type == kMatLambert) {
+                    contrib = ray.throughput * (normal_aov ? sn : v3(1.0f, 1.0f, 1.0f));
+                    break;
+                }
+                if (mat.type == kMatGlossy && mat.alpha != 0.0f) {
+                    contrib = ray.throughput * (normal_aov ? sn : ggx_f0(mat));
+                    break;
+                }
+                if (mat.type != kMatGlass && mat.type != kMatGlossy) break; // Emission: zero
+                ray = next_ray(mat, ray, pos, sn, smp); // glass, smooth GGX
+            }
+            sum = sum + contrib;
+        }
+        const V3 r = sum / (float)spp;
+        out[3 * (size_t)pix] = r.x;
+        out[3 * (size_t)pix + 1] = r.y;
+        out[3 * (size_t)pix + 2] = r.z;
+    }
+    if (queries && q) atomicAdd(queries, q);
+}
+
+hipError_t launch_render_aov(const SceneView& sc, int w, int h, int spp, int max_bounces, uint32_t seed,
+                             int normal_aov, bool wide, float* out, unsigned long long* queries, hipStream_t st)
+{
+    const int64_t n = (int64_t)w * h;
+    if (n <= 0) return hipSuccess;
+    const int64_t want = (n + 255) / 256, cap = (int64_t)cu_count() * PT_FINISH_BLOCKS_PER_CU;
+    const dim3 grid((unsigned)(want < cap ? want : cap)), block(256);
+    if (wide)
+        hipLaunchKernelGGL(k_render_aov<true>, grid, block, 0, st, sc, w, h, spp, max_bounces, seed, normal_aov, out,
+                           queries);
+    else
+        hipLaunchKernelGGL(k_render_aov<false>, grid, block, 0, st, sc, w, h, spp, max_bounces, seed, normal_aov, out,
+                           queries);
+    return hipGetLastError();
+}
+
 // Events of path p from its current hits to the end of its pixel. Returns the rays traced.
 template <int INTEGRATOR, bool WIDE>
 __device__ __forceinline__ unsigned long long finish_path(const SceneView& sc, const JobView& job,
