@@ -14,7 +14,8 @@ def run(scene, size=None, saved=None):
     """Builds the scene through the C++ API; returns (camera state [16], image or None).
     size: (width, height, spp, depth, integrator, seed) to render. saved: a dict that
     receives the files Texture2D::Save wrote of the image ("png": uint8 [h, w, 3],
-    "hdr": float32 [h, w, 3])."""
+    "hdr": float32 [h, w, 3]) and the denoising AOVs Petrichor::Render filled ("albedo",
+    "normal")."""
     n = len(scene["tri_mat"])
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
@@ -32,7 +33,13 @@ def run(scene, size=None, saved=None):
             saved["png"] = P.load_png_rgb8(fout + ".png")
             saved["hdr"] = P.load_hdr(fout + ".hdr")
     cam = data[:16]
-    img = data[16:].reshape(size[1], size[0], 3) if size else None
+    img = None
+    if size:
+        n = size[0] * size[1] * 3
+        img = data[16:16 + n].reshape(size[1], size[0], 3)
+        if saved is not None:
+            saved["albedo"] = data[16 + n:16 + 2 * n].reshape(size[1], size[0], 3)
+            saved["normal"] = data[16 + 2 * n:16 + 3 * n].reshape(size[1], size[0], 3)
     return cam, img
 
 

@@ -33,3 +33,18 @@ def test_cpp_texture_save_png_and_hdr(gpu):
     assert np.array_equal(saved["png"], O.tonemap_rgb8(img))
     peak = img.max(axis=2, keepdims=True)
     assert np.all(np.abs(saved["hdr"] - img) <= peak / 128 + 1e-30)
+
+
+def test_cpp_render_fills_denoising_aovs(gpu):
+    # DenoisingAlbedo / DenoisingNormal targets set on the Scene are rendered by
+    # Petrichor::Render as pt_render_aov renders them
+    import petrichor_amd as P
+    log("C++ API denoising AOVs")
+    saved = {}
+    sc = scenes.cornell_box()
+    cpp_api.run(sc, (40, 24, 4, 8, 0, 3), saved)
+    b = ctx().scene(sc).build_bvh(16)
+    alb, _ = b.render_aov(40, 24, 4, 8, P.AOV_DENOISING_ALBEDO, 3)
+    nrm, _ = b.render_aov(40, 24, 4, 8, P.AOV_DENOISING_NORMAL, 3)
+    assert bits_equal(saved["albedo"], alb) and bits_equal(saved["normal"], nrm)
+    assert np.abs(nrm).mean() > 0.1
