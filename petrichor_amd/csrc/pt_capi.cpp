@@ -467,7 +467,7 @@ int pt_bvh_destroy(pt_bvh* b)
 {
     if (!b) return PT_ERR_ARG;
     free_all({b->nodes8_d, b->nodes_d, b->big_leaves_d, b->inner_range_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
-              b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d});
+              b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d, b->order_keys_d, b->order_idx_d, b->order_temp_d});
     delete b;
     return PT_OK;
 }
@@ -534,6 +534,9 @@ int pt_bvh_info(pt_bvh* b, int32_t* num_nodes, int32_t* max_depth, double* build
 
 namespace {
 
+// API launches of at least this many rays are traced in coherence order (pt_order.hip).
+constexpr int64_t kOrderMinRays = (int64_t)1 << 20;
+
 TraceSegment api_segment(const int* count_d, const float* rays, bool any, float* t_out, int32_t* k_out)
 {
     TraceSegment g;
@@ -561,6 +564,30 @@ int api_trace(pt_bvh* b, int64_t n, const float* rays, bool any, float* t_out, i
     job.job_of_path = nullptr;
     job.queries = nullptr;
     const int n32 = (int)n;
+    if (n >= kOrderMinRays && b->view.root_ref != PT_REF_EMPTY) {
+        // large launches trace their rays in coherence order (pt_order.hip)
+        if (b->order_cap < n) {
+            const int64_t cap = n;
+            const size_t tmp = ray_order_temp_bytes((int)cap);
+            PT_HIP(ctx, hipStreamSynchronize(st)); // the old buffers may still be read
+            (void)hipFree(b->order_keys_d);
+            (void)hipFree(b->order_idx_d);
+            (void)hipFree(b->order_temp_d);
+            b->order_keys_d = nullptr;
+            b->order_idx_d = nullptr;
+            b->order_temp_d = nullptr;
+            b->order_cap = 0;
+            PT_HIP(ctx, hipMalloc((void**)&b->order_keys_d, 2 * sizeof(uint32_t) * (size_t)cap));
+            PT_HIP(ctx, hipMalloc((void**)&b->order_idx_d, 2 * sizeof(int32_t) * (size_t)cap));
+            PT_HIP(ctx, hipMalloc(&b->order_temp_d, tmp > 0 ? tmp : 1));
+            b->order_temp_bytes = tmp;
+            b->order_cap = cap;
+        }
+        PT_HIP(ctx, launch_ray_order((const float4*)rays, n32, b->view.root_lo, b->view.root_hi, b->order_keys_d,
+                                     b->order_idx_d, b->order_temp_d, b->order_temp_bytes, st));
+        job.seg[0].path = b->order_idx_d + n; // sorted indices: ray and hit of item i at path[i]
+        job.seg[0].rays_by_path = 1;
+    }
     PT_HIP(ctx, launch_zero(b->cursor_d, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
     PT_HIP(ctx, hipMemcpyAsync(b->cursor_d + PT_CURSOR_INTS, &n32, sizeof(int), hipMemcpyHostToDevice, st));
     PT_HIP(ctx, launch_trace_job(b->view, job, b->cursor_d, stats, st));

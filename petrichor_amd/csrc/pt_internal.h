@@ -46,6 +46,13 @@ struct pt_bvh {
     int2* big_leaves_d = nullptr;
     int2* inner_range_d = nullptr; // leaf-slot range of every inner node (preorder)
     int* cursor_d = nullptr; // trace work cursor of the API entry points
+    // coherence order of large API launches: keys and ray indices (2 x order_cap each) and
+    // the radix sort's temporary storage, grown on demand
+    uint32_t* order_keys_d = nullptr;
+    int32_t* order_idx_d = nullptr;
+    void* order_temp_d = nullptr;
+    size_t order_temp_bytes = 0;
+    int64_t order_cap = 0;
     float4* tris_d = nullptr;
     int32_t* prim_ids_d = nullptr;
     int32_t* tri_mat_d = nullptr;

@@ -93,6 +93,7 @@ struct TraceSegment {
     int hit_flag;
     float* out_t;
     int32_t* out_k;
+    int rays_by_path = 0; // ray i is read at rays[path[i]] (the API's coherence-ordered launches)
 };
 struct TraceJob {
     TraceSegment seg[3];
@@ -184,6 +185,13 @@ hipError_t launch_finish_paths(int integrator, const SceneView& sc, const JobVie
 // thread per pixel (pt_kernels.hip); queries (optional) receives the rays traced.
 hipError_t launch_render_aov(const SceneView& sc, int w, int h, int spp, int max_bounces, uint32_t seed,
                              int normal_aov, bool wide, float* out, unsigned long long* queries, hipStream_t st);
+
+// Coherence order of n rays (pt_order.hip): keys[0..n) and idx[0..n) are filled, then
+// idx[n..2n) receives the ray indices sorted by the Morton code of the origin in the box
+// (stable); keys and idx hold 2n entries, temp ray_order_temp_bytes(n) bytes.
+size_t ray_order_temp_bytes(int n);
+hipError_t launch_ray_order(const float4* rays, int n, V3 box_lo, V3 box_hi, uint32_t* keys, int32_t* idx,
+                            void* temp, size_t temp_bytes, hipStream_t st);
 
 // The reference's 8-bit PNG bytes of num_pixels RGB float pixels (pt_post.hip).
 hipError_t launch_tonemap_rgb8(const float* rgb, uint8_t* out, int64_t num_pixels, hipStream_t st);

@@ -1,0 +1,75 @@
+// pt_order.hip — coherence order for large launches of the trace entry points.
+//
+// Rays that start near each other walk the same subtrees. A launch of rays in arbitrary
+// order (the traversal microbenchmark's surface rays start anywhere in the scene) keeps
+// every wave's rays in different parts of the tree, so node reads miss the caches. The
+// trace entry points therefore sort the ray indices of a large launch by the Morton code
+// of the ray origin, quantised to 10 bits per axis over the BVH's root box (a stable
+// radix sort, so rays from one point, such as a pinhole camera's, keep their order), and
+// the trace kernel reads ray order[i] and writes its hit back at order[i]. Hits do not
+// depend on the order in which rays are traced.
+#include <hip/hip_runtime.h>
+#include <hipcub/device/device_radix_sort.hpp>
+
+#include "pt_kernels.h"
+
+namespace pt {
+
+namespace {
+
+__device__ __forceinline__ uint32_t spread10(uint32_t x)
+{
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t quantise(float v, float lo, float scale)
+{
+    const float q = (v - lo) * scale;
+    // NaN and values outside the box clamp to the box's faces
+    return q >= 1023.0f ? 1023u : (q > 0.0f ? (uint32_t)q : 0u);
+}
+
+__global__ void __launch_bounds__(256) k_ray_keys(const float4* __restrict__ rays, int n, float3 lo, float3 scale,
+                                                  uint32_t* __restrict__ keys, int32_t* __restrict__ idx)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 o = rays[2 * (size_t)i];
+        keys[i] = spread10(quantise(o.x, lo.x, scale.x)) | (spread10(quantise(o.y, lo.y, scale.y)) << 1) |
+                  (spread10(quantise(o.z, lo.z, scale.z)) << 2);
+        idx[i] = i;
+    }
+}
+
+} // namespace
+
+size_t ray_order_temp_bytes(int n)
+{
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (const int32_t*)nullptr, (int32_t*)nullptr, n, 0, 30);
+    return bytes;
+}
+
+hipError_t launch_ray_order(const float4* rays, int n, V3 box_lo, V3 box_hi, uint32_t* keys, int32_t* idx,
+                            void* temp, size_t temp_bytes, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    auto inv = [](float lo, float hi) { return hi > lo ? 1024.0f / (hi - lo) : 0.0f; };
+    const float3 lo = make_float3(box_lo.x, box_lo.y, box_lo.z);
+    const float3 scale = make_float3(inv(box_lo.x, box_hi.x), inv(box_lo.y, box_hi.y), inv(box_lo.z, box_hi.z));
+    const int64_t want = ((int64_t)n + 255) / 256, cap = (int64_t)cu_count() * 8;
+    hipLaunchKernelGGL(k_ray_keys, dim3((unsigned)(want < cap ? want : cap)), dim3(256), 0, st, rays, n, lo, scale,
+                       keys, idx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // keys and indices sorted into their second halves (keys + n, idx + n)
+    size_t bytes = temp_bytes;
+    return hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys, keys + n, idx, idx + n, n, 0, 30, st);
+}
+
+} // namespace pt

@@ -126,7 +126,8 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1,
                     part_item(seg_n, job.num_seg, gp, g, &seg_id, &seg_i);
                     const TraceSegment& sg = job.seg[seg_id];
                     if (MULTI) b = job.views[job.job_of_path[sg.path[seg_i]]];
-                    const float4 r0 = sg.rays[2 * (size_t)seg_i], r1 = sg.rays[2 * (size_t)seg_i + 1];
+                    const size_t ri = sg.rays_by_path ? (size_t)sg.path[seg_i] : (size_t)seg_i;
+                    const float4 r0 = sg.rays[2 * ri], r1 = sg.rays[2 * ri + 1];
                     if (!trace_begin(b, r0, r1, sg.any != 0, r, s)) {
                         if (STATS) nv += s.nv;
                         trace_finish(b, sg, seg_i, s);
