@@ -290,7 +290,16 @@ inline bool parseJson(const char*& p, const char* e, Json& out, int depth = 0)
                 case 'f': c = '\f'; break;
                 case 'u': {
                     if (e - p < 4) return false;
-                    const unsigned cp = (unsigned)std::stoul(std::string(p, p + 4), nullptr, 16);
+                    unsigned cp = 0;
+                    for (int k = 0; k < 4; ++k) {
+                        const char h = p[k];
+                        const int dgt = h >= '0' && h <= '9'   ? h - '0'
+                                        : h >= 'a' && h <= 'f' ? h - 'a' + 10
+                                        : h >= 'A' && h <= 'F' ? h - 'A' + 10
+                                                               : -1;
+                        if (dgt < 0) return false;
+                        cp = cp * 16 + (unsigned)dgt;
+                    }
                     p += 4;
                     if (cp < 0x80) {
                         s += (char)cp;

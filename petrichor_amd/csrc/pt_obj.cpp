@@ -107,13 +107,17 @@ void skip_blank(Cursor& c)
     while (c.p < c.end && (*c.p == ' ' || *c.p == '\t' || *c.p == '\r')) ++c.p;
 }
 
-// atoi of a face field: optional sign and digits (0 when there are none).
+// atoi of a face field: optional sign and digits (0 when there are none). Values beyond
+// 10^12 saturate there (they are out of range for any index anyway).
 long read_int(Cursor& c)
 {
     bool neg = false;
     if (c.p < c.end && (*c.p == '-' || *c.p == '+')) neg = *c.p++ == '-';
     long v = 0;
-    while (c.p < c.end && is_digit(*c.p)) v = v * 10 + (*c.p++ - '0');
+    while (c.p < c.end && is_digit(*c.p)) {
+        v = v < 1000000000000L ? v * 10 + (*c.p - '0') : v;
+        ++c.p;
+    }
     return neg ? -v : v;
 }
 
