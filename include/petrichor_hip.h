@@ -151,7 +151,11 @@ int pt_bvh_build_host(const float* tri_pos, int32_t num_triangles, const pt_bvh_
 /* Closest hit for n rays (replaces BinnedSAHBVH::Intersect, BinnedSAHBVH.cpp:272-374).
  * Device pointers. rays: 8 floats per ray (ox oy oz dist_min dx dy dz dist_max);
  * hits: t_out (+inf on a miss) and prim_out (triangle index, -1 on a miss).
- * stream: a hipStream_t or NULL for the context's stream. */
+ * stream: a hipStream_t or NULL for the context's stream. A launch of at least 2^20 rays
+ * traces them in coherence order (sorted by the Morton code of the origin) and writes
+ * every hit at its ray's index; the hits are those of any other order. The BVH keeps the
+ * sort's buffers (16 bytes per ray of its largest such launch, plus the sort's scratch)
+ * until pt_bvh_destroy; calls on one BVH must not overlap. Same for pt_trace_any. */
 int pt_trace_closest(pt_bvh* bvh, int64_t n, const float* rays, float* t_out, int32_t* prim_out,
                      void* stream);
 /* Any hit (occlusion) for n rays: hit_out[i] = 1 if some triangle is hit within
