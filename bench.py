@@ -753,8 +753,15 @@ def main():
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    counters = None  # L2 hit rate and VALU activity of the trace launches (rocprofv3 --pmc)
+    cf = os.path.join(ROOT, "profiles", "pmc_l2_valu_frame_w%d.json" % width)
+    if os.path.exists(cf):
+        with open(cf) as f:
+            c = json.load(f)["k_trace_persistent"]
+        counters = {"l2_hit": c["l2_hit"], "valu_active": c["valu_active"], "source": os.path.basename(cf)}
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "counters": counters,
                 "kernel": "k_trace_persistent, every trace launch of the instrumented frame",
                 "algorithmic_bytes_per_launch": round(alg_bytes / max(t_launches, 1), 1),
                 "bytes_per_query": round(alg_bytes / max(t_items, 1), 1),
