@@ -49,8 +49,7 @@ __device__ __forceinline__ void part_item(const int* seg_n, int num_seg, int x, 
 }
 
 template <bool STATS, bool MULTI, bool WIDE>
-__global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1, TraceJob job, int* cursor,
-                                                                     unsigned long long* stats)
+__device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int* cursor, unsigned long long* stats)
 {
     // segment sizes come from device counters (written by the previous stage)
     int seg_n[3] = {0, 0, 0};
@@ -162,6 +161,18 @@ __global__ void __launch_bounds__(PT_TRACE_BLOCK) k_trace_persistent(BvhView b1,
         atomicAdd(&stats[0], (unsigned long long)nv);
         atomicAdd(&stats[1], (unsigned long long)nt);
     }
+}
+
+// The multi-job binary walk is held to 64 VGPRs, so that its 8 LDS-bound blocks per CU
+// are all resident (8 waves per SIMD; unbounded it takes 67 and fits 7, 2.8 % slower on
+// the 64-job batch). The single-job binary walk needs 61 by itself; the 8-wide walk is
+// LDS-bound at 5 blocks per CU.
+template <bool STATS, bool MULTI, bool WIDE>
+__global__ void __launch_bounds__(PT_TRACE_BLOCK, (MULTI && !WIDE) ? 8 : 1) k_trace_persistent(BvhView b1, TraceJob job,
+                                                                                  int* cursor,
+                                                                                  unsigned long long* stats)
+{
+    trace_body<STATS, MULTI, WIDE>(b1, job, cursor, stats);
 }
 
 hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* cursor, unsigned long long* stats,
