@@ -545,6 +545,11 @@ def tonemap_bench(args):
                "sample": "the first %d pixels of the same image" % sample}
         parity = {"pixels_checked": sample, "bytes_equal": int((got[:sample] == ref).sum()),
                   "bytes_checked": int(ref.size)}
+    traffic = None  # rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, committed for one image size
+    tf = os.path.join(ROOT, "profiles", "pmc_traffic_tonemap_%dx%d.json" % (W, H))
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     res = {"metric": "tonemap Mpixels/s (8-bit PNG bytes of an HDR image in HBM)",
            "value": round(n * steps / wall / 1e6, 1), "unit": "Mpixels/s", "n_gpus": 1, "steps": steps,
            "warmup": max(args.warmup, 1), "ms_per_step": round(wall / steps * 1e3, 4), "higher_is_better": True,
@@ -552,7 +557,7 @@ def tonemap_bench(args):
            "config": {"workload": "ACES + RTCamp7 + degamma to 8 bits of a %dx%d RGB float image" % (W, H),
                       "width": W, "height": H},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                         "kernel": "k_tonemap_rgb8", "algorithmic_bytes_per_launch": alg,
                         "avg_launch_ms": round(launch_ms, 5)},
            "cpu_baseline": cpu, "parity_vs_oracle": parity}
