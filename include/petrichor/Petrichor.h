@@ -493,10 +493,11 @@ public:
         if (!detail::readFile(path, text)) return fail("cannot read " + path.string());
         int64_t n = 0;
         char err[256] = {0};
-        if (pt_obj_parse(text.data(), (int64_t)text.size(), 0, nullptr, nullptr, &n, err, sizeof err) != PT_OK)
+        if (pt_obj_parse(text.data(), (int64_t)text.size(), 0, nullptr, nullptr, nullptr, &n, err, sizeof err) != PT_OK)
             return fail(err);
-        std::vector<float> p(9 * (size_t)n), nr(9 * (size_t)n);
-        if (n > 0 && pt_obj_parse(text.data(), (int64_t)text.size(), n, p.data(), nr.data(), &n, err, sizeof err) != PT_OK)
+        std::vector<float> p(9 * (size_t)n), nr(9 * (size_t)n), uv(6 * (size_t)n);
+        if (n > 0 && pt_obj_parse(text.data(), (int64_t)text.size(), n, p.data(), nr.data(), uv.data(), &n, err,
+                                  sizeof err) != PT_OK)
             return fail(err);
         m_vertices.reserve(3 * (size_t)n);
         m_triangles.reserve((size_t)n);
@@ -506,7 +507,7 @@ public:
                 const float* q = &p[9 * t + 3 * k];
                 const float* m = &nr[9 * t + 3 * k];
                 m_vertices.emplace_back(Math::Vector3f(q[0], q[1], q[2]), Math::Vector3f(m[0], m[1], m[2]),
-                                        Math::Vector3f());
+                                        Math::Vector3f(uv[6 * t + 2 * k], uv[6 * t + 2 * k + 1], 0.0f));
                 v[k] = &m_vertices.back();
             }
             Triangle tri(v[0], v[1], v[2], shadingType);
@@ -971,8 +972,6 @@ public:
         Texture2D* target = scene.GetTargetTexture(Scene::AOVType::Rendered);
         if (!target || !target->IsValid()) return fail("Target texture has not been set.");
         if (!scene.GetMainCamera()) return fail("The scene has no camera.");
-        if (scene.GetTargetTexture(Scene::AOVType::UV))
-            return fail("The UV AOV is not rendered (the GPU scene has no texture coordinates).");
         const RenderSetting& rs = scene.GetRenderSetting();
         // TileManager.cpp:8-43 counts (W / tw + 1) * (H / th + 1) tiles
         m_numTiles = (uint32_t)((target->GetWidth() / rs.tileWidth + 1) * (target->GetHeight() / rs.tileHeight + 1));
@@ -993,15 +992,17 @@ public:
                           m_integrator == Integrator::PathTracing ? PT_INTEGRATOR_MIS : PT_INTEGRATOR_SIMPLE, m_seed,
                           0};
         int rc = pt_render(s, bvh, &rd, target->GetRawDataPtr(), 0, nullptr);
-        // the denoising AOV passes (Core/Petrichor.cpp:104-261) into their target textures,
-        // at RenderSetting::numSppForDenoising samples; the UV pass is not rendered
-        const std::pair<Scene::AOVType::Value, int32_t> aovs[2] = {
+        // the AOV passes (Core/Petrichor.cpp:104-261) into their target textures: UV at
+        // RenderSetting::numSamplesPerPixel samples, the denoising ones at numSppForDenoising
+        const std::pair<Scene::AOVType::Value, int32_t> aovs[3] = {
+            {Scene::AOVType::UV, PT_AOV_UV},
             {Scene::AOVType::DenoisingAlbedo, PT_AOV_DENOISING_ALBEDO},
             {Scene::AOVType::DenoisingNormal, PT_AOV_DENOISING_NORMAL}};
         for (const auto& [type, aov] : aovs) {
             Texture2D* tex = scene.GetTargetTexture(type);
             if (rc != PT_OK || !tex || !tex->IsValid()) continue;
-            pt_render_desc ad{tex->GetWidth(), tex->GetHeight(), rs.numSppForDenoising, rs.numMaxBounces,
+            pt_render_desc ad{tex->GetWidth(), tex->GetHeight(),
+                              aov == PT_AOV_UV ? rs.numSamplesPerPixel : rs.numSppForDenoising, rs.numMaxBounces,
                               PT_INTEGRATOR_MIS, m_seed, 0};
             rc = pt_render_aov(s, bvh, aov, &ad, tex->GetRawDataPtr(), 0, nullptr);
         }
@@ -1019,7 +1020,7 @@ public:
 
 private:
     struct Flat {
-        std::vector<float> pos, nrm, matParam, env;
+        std::vector<float> pos, nrm, uv, matParam, env;
         std::vector<int32_t> matId, smooth, matType, lights;
         pt_scene_desc desc{};
     };
@@ -1076,10 +1077,13 @@ private:
     {
         std::unordered_map<const MaterialBase*, int32_t> matIndex;
         std::unordered_map<const GeometryBase*, int32_t> triIndex;
-        bool anySmooth = false;
+        bool anySmooth = false, anyUV = false;
         for (const GeometryBase* g : scene.GetGeometries())
-            if (const auto* t = dynamic_cast<const Triangle*>(g))
+            if (const auto* t = dynamic_cast<const Triangle*>(g)) {
                 anySmooth |= t->GetShadingType() == ShadingTypes::Smooth;
+                for (int k = 0; k < 3; ++k)
+                    if (t->GetVertex(k) && (t->GetVertex(k)->uv.x != 0.0f || t->GetVertex(k)->uv.y != 0.0f)) anyUV = true;
+            }
         for (const GeometryBase* g : scene.GetGeometries()) {
             const auto* t = dynamic_cast<const Triangle*>(g);
             if (!t) return fail("the GPU path renders triangles only");
@@ -1089,6 +1093,7 @@ private:
                 const Vertex* v = t->GetVertex(k);
                 f.pos.insert(f.pos.end(), {v->pos.x, v->pos.y, v->pos.z});
                 if (anySmooth) f.nrm.insert(f.nrm.end(), {v->normal.x, v->normal.y, v->normal.z});
+                if (anyUV) f.uv.insert(f.uv.end(), {v->uv.x, v->uv.y});
             }
             if (anySmooth) f.smooth.push_back(t->GetShadingType() == ShadingTypes::Smooth ? 1 : 0);
             f.matId.push_back(material_index(t->GetMaterial(), matIndex, f));
@@ -1126,6 +1131,7 @@ private:
         d.env_base_color[2] = env.GetBaseColor().z;
         d.env_z_rotation = env.GetZAxisRotation();
         d.env_importance_sampling = env.UseEnvImportanceSampling() ? 1 : 0;
+        d.uvs = anyUV ? f.uv.data() : nullptr;
         return true;
     }
 

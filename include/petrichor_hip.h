@@ -65,6 +65,9 @@ typedef struct pt_scene_desc {
     float env_base_color[3];
     float env_z_rotation;
     int32_t env_importance_sampling; /* Environment::useEnvImportanceSampling */
+    /* Texture coordinates (Vertex::uv, Core/Geometry/Vertex.h:21): 6 floats per triangle
+     * (u, v of p0, p1, p2), or NULL for all zero. Read only by the UV AOV. */
+    const float* uvs;
 } pt_scene_desc;
 
 typedef struct pt_bvh_desc {
@@ -180,11 +183,12 @@ int pt_render(pt_scene* scene, pt_bvh* bvh, const pt_render_desc* desc, float* r
  * RenderSetting::numSppForDenoising), followed through glass and zero-roughness GGX to the
  * first other hit, for at most desc->max_bounces hits. Albedo: throughput x (1 for
  * Lambert, F0 for GGX, the environment on a miss); normal: throughput x the shading normal
- * (-direction on a miss). Seeds per pixel as pt_render; desc->integrator and flags are
- * not read. rgb_out as pt_render; synchronous. The UV AOV is not rendered (the scene
- * carries no texture coordinates). */
+ * (-direction on a miss). UV: the mean over desc->spp camera rays of the interpolated
+ * texture coordinates (u, v, 0) of the first hit (misses add zero). Seeds per pixel as
+ * pt_render; desc->integrator and flags are not read. rgb_out as pt_render; synchronous. */
 #define PT_AOV_DENOISING_ALBEDO 0
 #define PT_AOV_DENOISING_NORMAL 1
+#define PT_AOV_UV 2 /* Core/AOV/AOVUVCoordinate.cpp: mean uv of the first hit, spp = numSamplesPerPixel */
 int pt_render_aov(pt_scene* scene, pt_bvh* bvh, int32_t aov, const pt_render_desc* desc, float* rgb_out,
                   int32_t out_on_device, pt_render_stats* stats);
 
@@ -321,13 +325,14 @@ int pt_tonemap_rgb8(pt_context* ctx, int64_t num_pixels, const float* rgb, uint8
  * Triangles of a Wavefront OBJ text (len bytes) as Mesh::Load receives them from Assimp
  * 4.1.0 read with no post-processing (Core/Geometry/Mesh.cpp:18-125): one triangle per
  * face (its first three corners), in file order; positions[9 t..] the corners' positions,
- * normals[9 t..] their normals normalised (+Z for a face without normal indices).
+ * normals[9 t..] their normals normalised (+Z for a face without normal indices), uvs[6 t..]
+ * their texture coordinates (zero without; uvs may be NULL).
  * Numbers are read as Assimp's fast_atof reads them. Fills at most max_triangles (pass 0
  * to count) and sets *num_triangles to the file's count. Host pointers, no device.
  * Errors (with the line in err, err_len bytes): PT_ERR_ARG for malformed text or an index
  * out of range, PT_ERR_UNSUPPORTED for point and line elements. */
 int pt_obj_parse(const char* text, int64_t len, int64_t max_triangles, float* positions, float* normals,
-                 int64_t* num_triangles, char* err, int32_t err_len);
+                 float* uvs, int64_t* num_triangles, char* err, int32_t err_len);
 
 #ifdef __cplusplus
 }

@@ -665,6 +665,7 @@ static Ray generate_ray(const Camera& c, int i, int j, int w, int h, Sampler2D& 
 struct Tri {
     V p0, p1, p2;
     V n0, n1, n2;
+    V uv0 = {0.0f, 0.0f, 0.0f}, uv1 = {0.0f, 0.0f, 0.0f}, uv2 = {0.0f, 0.0f, 0.0f}; // Vertex::uv (z = 0)
     bool smooth = false;
     int material = 0;
 };
@@ -726,6 +727,20 @@ static Shading interpolate(const Scene& sc, int prim, const Ray& ray, float t)
     }
     s.material = sc.material_of(prim, 0.0f);
     return s;
+}
+
+// Triangle::Interpolate's texture coordinates (Triangle.cpp:90-108):
+// weightE1 * (uv1 - uv0) + weightE2 * (uv2 - uv0) + uv0.
+static V hit_uv(const Scene& sc, int prim, const Ray& ray)
+{
+    const Tri& tr = sc.tris[prim];
+    const V e1 = sub(tr.p1, tr.p0);
+    const V e2 = sub(tr.p2, tr.p0);
+    const float inv_det = 1.0f / dot(neg(ray.dir), cross(e1, e2));
+    const V vec = sub(ray.o, tr.p0);
+    const float w1 = dot(vec, cross(ray.dir, e2)) * inv_det;
+    const float w2 = dot(vec, cross(e1, ray.dir)) * inv_det;
+    return add(add(scl(sub(tr.uv1, tr.uv0), w1), scl(sub(tr.uv2, tr.uv0), w2)), tr.uv0);
 }
 
 // Triangle::SampleSurface (Triangle.cpp:155-182)
@@ -1251,6 +1266,21 @@ static V render_aov(Ctx& c, int px, int py, int w, int h, int spp, int max_bounc
     return divs(sum, (float)spp);
 }
 
+// AOVUVCoordinate::Render (AOV/AOVUVCoordinate.cpp:11-51) for one pixel: the mean over spp
+// camera rays of the first hit's texture coordinates; a miss adds nothing.
+static V render_uv(Ctx& c, int px, int py, int w, int h, int spp, Sampler2D& s2)
+{
+    const Scene& sc = *c.sc;
+    V sum = mk(0.0f, 0.0f, 0.0f);
+    for (int s = 0; s < spp; s++) {
+        const Ray ray = generate_ray(sc.cam, px, py, w, h, s2);
+        float t;
+        const int hit = c.closest(ray, kEps, kInfinity, &t);
+        if (hit >= 0) sum = add(sum, hit_uv(sc, hit, ray));
+    }
+    return divs(sum, (float)spp);
+}
+
 } // namespace oracle
 
 using namespace oracle;
@@ -1431,6 +1461,7 @@ int64_t oracle_render(void* s, void* b, int width, int height, int spp, int max_
                 Sampler2D s2(pixel_seed(seed, pix, 1), pixel_seed(seed, pix, 2));
                 const V r = integrator == 0   ? render_mis(c, x, y, width, height, spp, max_bounces, s1, s2)
                             : integrator == 1 ? render_simple(c, x, y, width, height, spp, max_bounces, s1, s2)
+                            : integrator == 4 ? render_uv(c, x, y, width, height, spp, s2)
                                               : render_aov(c, x, y, width, height, spp, max_bounces, s1, s2,
                                                            integrator == 3);
                 rgb_out[3 * slot] = r.x;
@@ -1661,6 +1692,17 @@ void oracle_tonemap_rgb8(int64_t num_pixels, const float* rgb, uint8_t* out, int
         });
     }
     for (auto& th : pool) th.join();
+}
+
+void oracle_scene_set_uv(void* s, const float* uv)
+{
+    Scene* sc = (Scene*)s;
+    for (size_t i = 0; i < sc->tris.size(); ++i) {
+        const float* q = uv + 6 * i;
+        sc->tris[i].uv0 = mk(q[0], q[1], 0.0f);
+        sc->tris[i].uv1 = mk(q[2], q[3], 0.0f);
+        sc->tris[i].uv2 = mk(q[4], q[5], 0.0f);
+    }
 }
 
 } // extern "C"

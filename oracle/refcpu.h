@@ -36,6 +36,8 @@ void* oracle_scene_create(int num_tris, const float* tri_pos, const float* tri_n
                           int num_lights, const int32_t* light_tris, const float* cam,
                           int env_w, int env_h, const float* env_rgb, const float* env_param);
 void oracle_scene_destroy(void* scene);
+/* Vertex texture coordinates, 6 floats per triangle (u, v of p0, p1, p2); zero without. */
+void oracle_scene_set_uv(void* scene, const float* uv);
 
 /* Binary binned-SAH BVH, Core/Accel/BinnedSAHBVH.cpp:15-225 (num_bins = 16 there). */
 void* oracle_bvh_build(void* scene, int num_bins);
@@ -62,7 +64,7 @@ void oracle_trace_brute(void* scene, int64_t n, const float* rays, float* t_out,
 /* Render with the MIS integrator (integrator 0, Core/Integrator/PathTracing.cpp), the
  * simple path tracer (integrator 1, Core/Integrator/SimplePathTracing.cpp) or a denoising
  * AOV (2: albedo, Core/AOV/AOVDenoisingAlbedo.cpp; 3: normal, AOVDenoisingNormal.cpp;
- * spp is then the denoising sample count), per-pixel
+ * spp is then the denoising sample count; 4: the UV AOV, Core/AOV/AOVUVCoordinate.cpp), per-pixel
  * seeded samplers, 16x16 tiles on num_threads workers (0 = all hardware threads).
  * If pixels != NULL only those num_pixels pixel indices are rendered (rgb_out then holds
  * 3 floats per listed pixel). Returns the number of BVH queries issued. work (optional,

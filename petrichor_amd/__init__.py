@@ -37,7 +37,7 @@ class SceneDesc(ctypes.Structure):
         ("num_lights", ctypes.c_int32), ("light_triangles", _i), ("camera", ctypes.c_float * 16),
         ("env_width", ctypes.c_int32), ("env_height", ctypes.c_int32), ("env_rgb", _f),
         ("env_base_color", ctypes.c_float * 3), ("env_z_rotation", ctypes.c_float),
-        ("env_importance_sampling", ctypes.c_int32),
+        ("env_importance_sampling", ctypes.c_int32), ("uvs", _f),
     ]
 
 
@@ -75,7 +75,7 @@ RENDER_TIME_KERNELS = 1
 RENDER_PLAIN_LAUNCHES = 2
 RENDER_TRACE_STATS = 4
 RENDER_TWO_STREAMS = 8
-AOV_DENOISING_ALBEDO, AOV_DENOISING_NORMAL = 0, 1
+AOV_DENOISING_ALBEDO, AOV_DENOISING_NORMAL, AOV_UV = 0, 1, 2
 
 
 class ServerDesc(ctypes.Structure):
@@ -156,7 +156,7 @@ def lib():
     L.pt_server_fetch.argtypes = [vp, ctypes.c_int64, _f]
     L.pt_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
     L.pt_tonemap_rgb8.argtypes = [vp, ctypes.c_int64, vp, vp, ctypes.c_int32, vp]
-    L.pt_obj_parse.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, _f, _f, _l, ctypes.c_char_p,
+    L.pt_obj_parse.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, _f, _f, _f, _l, ctypes.c_char_p,
                                ctypes.c_int32]
     _lib = L
     return L
@@ -386,6 +386,7 @@ def scene_desc(scene):
         d.env_base_color[i] = float(ep[i])
     d.env_z_rotation = float(ep[3])
     d.env_importance_sampling = int(ep[4] != 0)
+    d.uvs = _ptr(k(scene["tri_uv"], np.float32), _f) if scene.get("tri_uv") is not None else None
     return d, keep
 
 

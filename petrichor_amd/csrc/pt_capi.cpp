@@ -300,6 +300,7 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
     if (d->normals) s->tri_nrm.assign(d->normals, d->normals + 9 * nt);
     s->tri_mat.assign(d->material_ids, d->material_ids + nt);
     if (d->smooth) s->tri_smooth.assign(d->smooth, d->smooth + nt);
+    if (d->uvs) s->tri_uv.assign(d->uvs, d->uvs + 6 * nt);
     for (int m = 0; m < d->num_materials; ++m)
         s->mats.push_back(material_from(d->material_types[m], d->material_params + 8 * m));
     std::vector<float4> lights;
@@ -344,13 +345,14 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
     if (err == hipSuccess) err = upload(&s->tri_mat_d, s->tri_mat.data(), s->tri_mat.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&s->tri_smooth_d, s->tri_smooth.data(), s->tri_smooth.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&s->tri_nrm_d, s->tri_nrm.data(), s->tri_nrm.size(), ctx->stream);
+    if (err == hipSuccess) err = upload(&s->tri_uv_d, s->tri_uv.data(), s->tri_uv.size(), ctx->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
     s->device_bytes = (int64_t)(s->mats.size() * (sizeof(MaterialDev) + 4) + lights.size() * sizeof(float4)) +
                       (d->env_rgb ? 3 * (int64_t)e.w * e.h * 4 : 0) +
                       (int64_t)nt * 40 + (int64_t)(s->tri_smooth.size() * 4 + s->tri_nrm.size() * 4);
     if (err != hipSuccess) {
         free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->mat_shading_of_d, s->tri_pos_d,
-                  s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d});
+                  s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d, s->tri_uv_d});
         delete s;
         return fail(ctx, err == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
                     std::string("scene upload: ") + hipGetErrorString(err));
@@ -363,12 +365,32 @@ int pt_scene_destroy(pt_scene* s)
 {
     if (!s) return PT_ERR_ARG;
     free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->mat_shading_of_d, s->tri_pos_d,
-              s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d});
+              s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d, s->tri_uv_d});
     delete s;
     return PT_OK;
 }
 
+static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out);
+
 int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
+{
+    const int rc = bvh_build_impl(s, desc, out);
+    if (rc != PT_OK || !s->tri_uv_d || s->num_tris == 0) return rc;
+    // texture coordinates per leaf slot, gathered through the leaf order
+    pt_bvh* b = *out;
+    pt_context* ctx = s->ctx;
+    hipError_t e = hipMalloc((void**)&b->tri_uv_d, 6 * sizeof(float) * (size_t)s->num_tris);
+    if (e == hipSuccess) e = launch_gather_rows(s->tri_uv_d, b->prim_ids_d, s->num_tris, 6, b->tri_uv_d, ctx->stream);
+    if (e != hipSuccess) {
+        pt_bvh_destroy(b);
+        *out = nullptr;
+        return fail(ctx, e == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP, hipGetErrorString(e));
+    }
+    b->device_bytes += 24 * (int64_t)s->num_tris;
+    return PT_OK;
+}
+
+static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
 {
     if (!s || !out) return PT_ERR_ARG;
     pt_context* ctx = s->ctx;
@@ -467,7 +489,8 @@ int pt_bvh_destroy(pt_bvh* b)
 {
     if (!b) return PT_ERR_ARG;
     free_all({b->nodes8_d, b->nodes_d, b->big_leaves_d, b->inner_range_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
-              b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d, b->order_keys_d, b->order_idx_d, b->order_temp_d});
+              b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d, b->order_keys_d, b->order_idx_d, b->order_temp_d,
+              b->tri_uv_d});
     delete b;
     return PT_OK;
 }

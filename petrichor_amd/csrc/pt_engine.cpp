@@ -249,7 +249,7 @@ int pt_render_aov(pt_scene* s, pt_bvh* b, int32_t aov, const pt_render_desc* d, 
 {
     if (!s || !b || !d || !rgb_out) return PT_ERR_ARG;
     pt_context* ctx = s->ctx;
-    if (aov != PT_AOV_DENOISING_ALBEDO && aov != PT_AOV_DENOISING_NORMAL)
+    if (aov != PT_AOV_DENOISING_ALBEDO && aov != PT_AOV_DENOISING_NORMAL && aov != PT_AOV_UV)
         return pt_fail(ctx, PT_ERR_ARG, "unknown AOV");
     pt_job_desc jd;
     jd.scene = s;
@@ -270,7 +270,8 @@ int pt_render_aov(pt_scene* s, pt_bvh* b, int32_t aov, const pt_render_desc* d, 
     if (e == hipSuccess) e = hipMemsetAsync(q_d, 0, sizeof(unsigned long long), ctx->stream);
     if (e == hipSuccess)
         e = launch_render_aov(pt_engine::scene_view(s, b), d->width, d->height, d->spp, d->max_bounces, d->seed,
-                              aov == PT_AOV_DENOISING_NORMAL, b->view.nodes8 != nullptr, j.out_d, q_d, ctx->stream);
+                              aov == PT_AOV_UV ? 2 : (aov == PT_AOV_DENOISING_NORMAL ? 1 : 0), b->view.nodes8 != nullptr,
+                              j.out_d, q_d, ctx->stream);
     unsigned long long q = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&q, q_d, sizeof q, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

@@ -225,6 +225,7 @@ struct pt_engine {
         sc.tri_mat_shading = b->tri_mat_shading_d;
         sc.tri_smooth = b->tri_smooth_d;
         sc.tri_nrm = b->tri_nrm_d;
+        sc.tri_uv = b->tri_uv_d;
         sc.mats = s->mats_d;
         sc.lights = s->lights_d;
         sc.num_lights = s->num_lights;

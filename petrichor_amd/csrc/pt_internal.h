@@ -35,6 +35,8 @@ struct pt_scene {
     int32_t* tri_mat_d = nullptr;
     int32_t* tri_smooth_d = nullptr;   // null without per-vertex normals
     float* tri_nrm_d = nullptr;
+    std::vector<float> tri_uv;         // 6 floats per triangle, or empty
+    float* tri_uv_d = nullptr;
     int32_t* mat_shading_of_d = nullptr; // material -> material that shades it (mix resolved)
     int64_t device_bytes = 0; // materials, lights, environment, triangles
 };
@@ -60,6 +62,7 @@ struct pt_bvh {
     int32_t* tri_smooth_d = nullptr;
     float* tri_nrm_d = nullptr;
     uint4* nodes8_d = nullptr; // 8-wide nodes (width 8 only)
+    float* tri_uv_d = nullptr; // 6 floats per leaf slot, when the scene has texture coordinates
     int width = 2;
     int num_nodes8 = 0;
     int depth8 = 0;

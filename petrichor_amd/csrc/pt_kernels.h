@@ -32,6 +32,7 @@ struct SceneView {
     const int32_t* tri_mat_shading; // GetMaterial(0.0f) per leaf slot (Triangle.cpp:151)
     const int32_t* tri_smooth;      // per leaf slot, or null
     const float* tri_nrm;           // 9 floats per leaf slot, or null
+    const float* tri_uv = nullptr;  // 6 floats per leaf slot (texture coordinates), or null
     const MaterialDev* mats;
     const float4* lights;           // light triangles, AppendLight order, 3 float4 each
     int num_lights;
@@ -181,15 +182,19 @@ hipError_t launch_finish_paths(int integrator, const SceneView& sc, const JobVie
                                bool wide, const ShadeArgs& a, const HitBuffers& h, unsigned long long* queries,
                                hipStream_t st);
 
-// A denoising AOV (albedo, or the shading normal when normal_aov) of a w x h image, one
-// thread per pixel (pt_kernels.hip); queries (optional) receives the rays traced.
+// An AOV of a w x h image, one thread per pixel (pt_kernels.hip): mode 0 the denoising
+// albedo, 1 the denoising normal, 2 the first hit's texture coordinates; queries
+// (optional) receives the rays traced.
 hipError_t launch_render_aov(const SceneView& sc, int w, int h, int spp, int max_bounces, uint32_t seed,
-                             int normal_aov, bool wide, float* out, unsigned long long* queries, hipStream_t st);
+                             int mode, bool wide, float* out, unsigned long long* queries, hipStream_t st);
 
 // Coherence order of n rays (pt_order.hip): keys[0..n) and idx[0..n) are filled, then
 // idx[n..2n) receives the ray indices sorted by the Morton code of the origin in the box
 // (stable); keys and idx hold 2n entries, temp ray_order_temp_bytes(n) bytes.
 size_t ray_order_temp_bytes(int n);
+// dst[i * row + c] = src[idx[i] * row + c] for i < n (pt_order.hip): per-triangle rows put
+// in leaf-slot order.
+hipError_t launch_gather_rows(const float* src, const int32_t* idx, int n, int row, float* dst, hipStream_t st);
 hipError_t launch_ray_order(const float4* rays, int n, V3 box_lo, V3 box_hi, uint32_t* keys, int32_t* idx,
                             void* temp, size_t temp_bytes, hipStream_t st);
 

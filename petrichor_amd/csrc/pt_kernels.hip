@@ -591,11 +591,30 @@ __device__ __forceinline__ void trace_single(const BvhView& b, V3 o, float dmin,
 // -dir on a miss. An Emission hit gives zero (the reference's default branch), and so
 // does a path still going after max_bounces hits. The paths are short (most end at the
 // first hit), so they are not run through the wavefront queues.
+// Texture coordinates of a hit at leaf slot k (Triangle::Interpolate, Triangle.cpp:90-108):
+// the barycentric weights of the hit, uv = w1 (uv1 - uv0) + w2 (uv2 - uv0) + uv0, with the
+// third component zero; all zero when the scene has no texture coordinates.
+__device__ __forceinline__ V3 hit_uv(const SceneView& s, int k, V3 o, V3 d)
+{
+    if (s.tri_uv == nullptr) return v3(0.0f, 0.0f, 0.0f);
+    const TriDev tr = load_tri(s.bvh.tris, k);
+    const float inv_det = 1.0f / dot(-d, tr.n);
+    const V3 vec = o - tr.p0;
+    const float w1 = dot(vec, cross(d, tr.e2)) * inv_det;
+    const float w2 = dot(vec, cross(tr.e1, d)) * inv_det;
+    const float* q = s.tri_uv + 6 * (size_t)k;
+    const V3 uv0 = v3(q[0], q[1], 0.0f), uv1 = v3(q[2], q[3], 0.0f), uv2 = v3(q[4], q[5], 0.0f);
+    return w1 * (uv1 - uv0) + w2 * (uv2 - uv0) + uv0;
+}
+
+// The UV AOV (AOVUVCoordinate::Render, Core/AOV/AOVUVCoordinate.cpp:11-51) is mode 2: the
+// mean over spp camera rays of the first hit's texture coordinates (a miss adds zero).
 template <bool WIDE>
 __global__ void __launch_bounds__(256) k_render_aov(SceneView sc, int w, int h, int spp, int max_bounces,
-                                                    uint32_t seed, int normal_aov, float* out,
+                                                    uint32_t seed, int mode, float* out,
                                                     unsigned long long* queries)
 {
+    const bool normal_aov = mode == 1;
     __shared__ uint2 lds_stack[lds_stack_entries<WIDE>() * 256];
     uint2* lds = lds_stack + threadIdx.x;
     StackEntry spill[(WIDE ? PT_STACK8 : PT_MAX_STACK) - lds_stack_entries<WIDE>()];
@@ -608,6 +627,14 @@ __global__ void __launch_bounds__(256) k_render_aov(SceneView sc, int w, int h, 
         for (int s = 0; s < spp; ++s) {
             Ray ray = generate_ray(sc.cam, px, py, w, h, smp);
             V3 contrib = v3(0.0f, 0.0f, 0.0f);
+            if (mode == 2) {
+                float t;
+                int k;
+                trace_single<WIDE>(sc.bvh, ray.o, PT_EPS, ray.dir, PT_INFINITY, false, lds, spill, &t, &k);
+                ++q;
+                if (k >= 0) sum = sum + hit_uv(sc, k, ray.o, ray.dir);
+                continue;
+            }
             for (int bounce = 0; bounce < max_bounces; ++bounce) {
                 float t;
                 int k;
@@ -643,17 +670,17 @@ __global__ void __launch_bounds__(256) k_render_aov(SceneView sc, int w, int h, 
 }
 
 hipError_t launch_render_aov(const SceneView& sc, int w, int h, int spp, int max_bounces, uint32_t seed,
-                             int normal_aov, bool wide, float* out, unsigned long long* queries, hipStream_t st)
+                             int mode, bool wide, float* out, unsigned long long* queries, hipStream_t st)
 {
     const int64_t n = (int64_t)w * h;
     if (n <= 0) return hipSuccess;
     const int64_t want = (n + 255) / 256, cap = (int64_t)cu_count() * PT_FINISH_BLOCKS_PER_CU;
     const dim3 grid((unsigned)(want < cap ? want : cap)), block(256);
     if (wide)
-        hipLaunchKernelGGL(k_render_aov<true>, grid, block, 0, st, sc, w, h, spp, max_bounces, seed, normal_aov, out,
+        hipLaunchKernelGGL(k_render_aov<true>, grid, block, 0, st, sc, w, h, spp, max_bounces, seed, mode, out,
                            queries);
     else
-        hipLaunchKernelGGL(k_render_aov<false>, grid, block, 0, st, sc, w, h, spp, max_bounces, seed, normal_aov, out,
+        hipLaunchKernelGGL(k_render_aov<false>, grid, block, 0, st, sc, w, h, spp, max_bounces, seed, mode, out,
                            queries);
     return hipGetLastError();
 }

@@ -10,6 +10,7 @@
 //  - no triangulation, so a polygon keeps its first three corners;
 //  - indices are 1-based, negative ones count back from the elements read so far;
 //  - "v" takes 3 components, 4 (divided by w) or 6 (position and colour);
+//  - "vt" gives a corner's texture coordinates (u, v; zero without), Vertex::uv;
 //  - a corner's normal only when its face gives normal indices, else the Vertex default
 //    +Z (Core/Geometry/Vertex.h:19); Assimp fails the whole import (no triangles) when
 //    faces of one mesh mix both, which this rejects as an error;
@@ -122,13 +123,13 @@ long read_int(Cursor& c)
 }
 
 struct Corner {
-    long v, n; // -1: none
+    long v, t, n; // position, texture coordinate and normal index; -1: none
 };
 
 } // namespace
 
 extern "C" int pt_obj_parse(const char* text, int64_t len, int64_t max_triangles, float* positions, float* normals,
-                            int64_t* num_triangles, char* err, int32_t err_len)
+                            float* uvs, int64_t* num_triangles, char* err, int32_t err_len)
 {
     auto fail = [&](int code, const std::string& msg) {
         if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", msg.c_str());
@@ -137,7 +138,7 @@ extern "C" int pt_obj_parse(const char* text, int64_t len, int64_t max_triangles
     if (!text || len < 0 || !num_triangles || max_triangles < 0 ||
         (max_triangles > 0 && (!positions || !normals)))
         return fail(PT_ERR_ARG, "bad OBJ arguments");
-    std::vector<float> pos, nrm; // 3 per element
+    std::vector<float> pos, nrm, tex; // 3, 3 and 2 per element
     int64_t count = 0;
     int with_normals = -1; // faces so far: -1 none, 0 without normals, 1 with
     Cursor c{text, text + len};
@@ -153,7 +154,18 @@ extern "C" int pt_obj_parse(const char* text, int64_t len, int64_t max_triangles
         const char* tag = l.p;
         while (l.p < l.end && *l.p != ' ' && *l.p != '\t') ++l.p;
         const std::string t(tag, l.p);
-        if (t == "v" || t == "vn") {
+        if (t == "vt") {
+            float v[3] = {0.0f, 0.0f, 0.0f};
+            int k = 0;
+            for (;;) {
+                skip_blank(l);
+                if (l.p == l.end || *l.p == '#') break;
+                if (k == 3 || !read_float(l, &v[k])) return fail(PT_ERR_ARG, "OBJ line " + std::to_string(line) + ": bad number");
+                ++k;
+            }
+            if (k < 1) return fail(PT_ERR_ARG, "OBJ line " + std::to_string(line) + ": a texture coordinate needs a number");
+            tex.insert(tex.end(), v, v + 2);
+        } else if (t == "v" || t == "vn") {
             float v[6];
             int k = 0;
             for (;;) {
@@ -178,7 +190,7 @@ extern "C" int pt_obj_parse(const char* text, int64_t len, int64_t max_triangles
             Corner corners[3];
             int k = 0;
             bool has_n = false, no_n = false;
-            const long nv = (long)(pos.size() / 3), nn = (long)(nrm.size() / 3);
+            const long nv = (long)(pos.size() / 3), nn = (long)(nrm.size() / 3), nt = (long)(tex.size() / 2);
             for (;;) {
                 skip_blank(l);
                 if (l.p == l.end || *l.p == '#') break;
@@ -193,11 +205,13 @@ extern "C" int pt_obj_parse(const char* text, int64_t len, int64_t max_triangles
                 }
                 while (l.p < l.end && *l.p != ' ' && *l.p != '\t' && *l.p != '\r') ++l.p;
                 const long vi = fields[0] > 0 ? fields[0] - 1 : nv + fields[0];
+                const long ti = fields[1] > 0 ? fields[1] - 1 : (fields[1] < 0 ? nt + fields[1] : -1);
                 const long ni = fields[2] > 0 ? fields[2] - 1 : (fields[2] < 0 ? nn + fields[2] : -1);
-                if (fields[0] == 0 || vi < 0 || vi >= nv || (ni >= 0 && ni >= nn) || (fields[2] < 0 && ni < 0))
+                if (fields[0] == 0 || vi < 0 || vi >= nv || (ni >= 0 && ni >= nn) || (fields[2] < 0 && ni < 0) ||
+                    (ti >= 0 && ti >= nt) || (fields[1] < 0 && ti < 0))
                     return fail(PT_ERR_ARG, "OBJ line " + std::to_string(line) + ": index out of range");
                 (ni >= 0 ? has_n : no_n) = true;
-                if (k < 3) corners[k] = Corner{vi, ni};
+                if (k < 3) corners[k] = Corner{vi, ti, ni};
                 ++k;
             }
             if (k < 3) return fail(PT_ERR_ARG, "OBJ line " + std::to_string(line) + ": a face needs three corners");
@@ -210,6 +224,10 @@ extern "C" int pt_obj_parse(const char* text, int64_t len, int64_t max_triangles
                 float* N = normals + 9 * count;
                 for (int j = 0; j < 3; ++j) {
                     for (int a = 0; a < 3; ++a) P[3 * j + a] = pos[3 * corners[j].v + a];
+                    if (uvs) {
+                        uvs[6 * count + 2 * j] = corners[j].t >= 0 ? tex[2 * corners[j].t] : 0.0f;
+                        uvs[6 * count + 2 * j + 1] = corners[j].t >= 0 ? tex[2 * corners[j].t + 1] : 0.0f;
+                    }
                     if (corners[j].n >= 0) {
                         // Vector3f::Normalized: v / sqrt(dot(v, v)) (Math/Vector3f.h:170-189)
                         const float x = nrm[3 * corners[j].n], y = nrm[3 * corners[j].n + 1], z = nrm[3 * corners[j].n + 2];
@@ -229,7 +247,7 @@ extern "C" int pt_obj_parse(const char* text, int64_t len, int64_t max_triangles
         } else if (t == "l" || t == "p") {
             return fail(PT_ERR_UNSUPPORTED, "OBJ line " + std::to_string(line) + ": point and line elements are not triangles");
         }
-        // vt, o, g, s, usemtl, mtllib and others do not change the triangles
+        // o, g, s, usemtl, mtllib and others do not change the triangles
     }
     *num_triangles = count;
     return PT_OK;

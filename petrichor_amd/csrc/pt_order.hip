@@ -72,4 +72,26 @@ hipError_t launch_ray_order(const float4* rays, int n, V3 box_lo, V3 box_hi, uin
     return hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys, keys + n, idx, idx + n, n, 0, 30, st);
 }
 
+namespace {
+
+__global__ void __launch_bounds__(256) k_gather_rows(const float* __restrict__ src, const int32_t* __restrict__ idx,
+                                                     int n, int row, float* __restrict__ dst)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const size_t from = (size_t)idx[i] * row, to = (size_t)i * row;
+        for (int c = 0; c < row; ++c) dst[to + c] = src[from + c];
+    }
+}
+
+} // namespace
+
+hipError_t launch_gather_rows(const float* src, const int32_t* idx, int n, int row, float* dst, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t want = ((int64_t)n + 255) / 256, cap = (int64_t)cu_count() * 8;
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)(want < cap ? want : cap)), dim3(256), 0, st, src, idx, n, row,
+                       dst);
+    return hipGetLastError();
+}
+
 } // namespace pt

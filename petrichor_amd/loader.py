@@ -72,24 +72,26 @@ def _normalized(v):
 
 def parse_obj(text):
     """Triangles of an OBJ text as Mesh::Load builds them from Assimp's import
-    (pt_obj_parse): returns (positions [F, 9], normals [F, 9]) float32, one row per face
-    (its first three corners), normals normalised per corner, +Z where the face gives none.
-    Raises ValueError on malformed text."""
+    (pt_obj_parse): returns (positions [F, 9], normals [F, 9], uvs [F, 6]) float32, one row
+    per face (its first three corners), normals normalised per corner (+Z where the face
+    gives none), texture coordinates per corner (zero where it gives none). Raises
+    ValueError on malformed text."""
     import ctypes
     from . import lib, _ptr, _f, _l
     data = text.encode() if isinstance(text, str) else bytes(text)
     L = lib()
     n = np.zeros(1, np.int64)
     err = ctypes.create_string_buffer(256)
-    rc = L.pt_obj_parse(data, len(data), 0, None, None, _ptr(n, _l), err, 256)
+    rc = L.pt_obj_parse(data, len(data), 0, None, None, None, _ptr(n, _l), err, 256)
     if rc != 0:
         raise ValueError(err.value.decode())
     p = np.zeros((int(n[0]), 9), np.float32)
     nr = np.zeros((int(n[0]), 9), np.float32)
-    rc = L.pt_obj_parse(data, len(data), int(n[0]), _ptr(p, _f), _ptr(nr, _f), _ptr(n, _l), err, 256)
+    uv = np.zeros((int(n[0]), 6), np.float32)
+    rc = L.pt_obj_parse(data, len(data), int(n[0]), _ptr(p, _f), _ptr(nr, _f), _ptr(uv, _f), _ptr(n, _l), err, 256)
     if rc != 0:
         raise ValueError(err.value.decode())
-    return p, nr
+    return p, nr, uv
 
 
 def load_obj(path):
@@ -217,19 +219,22 @@ def load_scene(path):
             raise ValueError("environment texture %s: only Radiance .hdr maps are read" % tex)
         env_rgb = load_hdr_file(tex)
     # ---- assets (Core/SceneLoader.cpp:284-300): smooth triangles of each OBJ ----
-    tri_pos, tri_nrm, tri_mat = [], [], []
+    tri_pos, tri_nrm, tri_uv, tri_mat = [], [], [], []
     for a in data.get("assets", []):
         mat = material(a["mat"]) if "mat" in a else names["default"]
-        p, n = load_obj(os.path.join(base, a["path"]))
+        p, n, uv = load_obj(os.path.join(base, a["path"]))
         tri_pos.append(p)
         tri_nrm.append(n)
+        tri_uv.append(uv)
         tri_mat.append(np.full(len(p), mat, np.int32))
+    uv_all = np.concatenate(tri_uv) if tri_uv else np.zeros((0, 6), np.float32)
     tp = np.concatenate(tri_pos) if tri_pos else np.zeros((0, 9), np.float32)
     return {
         "tri_pos": tp,
         "tri_mat": np.concatenate(tri_mat) if tri_mat else np.zeros(0, np.int32),
         "tri_nrm": np.concatenate(tri_nrm) if tri_nrm else None,
         "tri_smooth": np.ones(len(tp), np.int32) if tri_pos else None,
+        "tri_uv": uv_all if np.any(uv_all != 0) else None,
         "mat_type": np.asarray(types, np.int32),
         "mat_param": np.stack(params).astype(np.float32),
         "light_tris": np.zeros(0, np.int32),

@@ -6,8 +6,8 @@
 //   render_api IN OUT [W H SPP DEPTH INTEGRATOR SEED]   (no size: camera state only)
 // IN:  int32 n, int32 n_lights, float pos[9 n], int32 material[n] (0 white, 1 red,
 //      2 green, 3 light), int32 lights[n_lights]
-// OUT: float camera[16], then W*H*3 floats each of the image and of the denoising albedo
-//      and normal AOVs (numSppForDenoising = SPP)
+// OUT: float camera[16], then W*H*3 floats each of the image, the denoising albedo and
+//      normal AOVs (numSppForDenoising = SPP) and the UV AOV
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -72,10 +72,11 @@ int main(int argc, char** argv)
     int rc = 0;
     if (argc == 9) {
         const int w = std::atoi(argv[3]), h = std::atoi(argv[4]);
-        Texture2D target(w, h), albedo(w, h), normal(w, h);
+        Texture2D target(w, h), albedo(w, h), normal(w, h), uvtex(w, h);
         scene.SetTargetTexture(Scene::AOVType::Rendered, &target);
         scene.SetTargetTexture(Scene::AOVType::DenoisingAlbedo, &albedo);
         scene.SetTargetTexture(Scene::AOVType::DenoisingNormal, &normal);
+        scene.SetTargetTexture(Scene::AOVType::UV, &uvtex);
         RenderSetting rs;
         rs.numSamplesPerPixel = std::atoi(argv[5]);
         rs.numSppForDenoising = std::atoi(argv[5]);
@@ -93,6 +94,7 @@ int main(int argc, char** argv)
             std::fwrite(target.GetRawDataPtr(), 4, 3 * (size_t)w * h, out);
             std::fwrite(albedo.GetRawDataPtr(), 4, 3 * (size_t)w * h, out);
             std::fwrite(normal.GetRawDataPtr(), 4, 3 * (size_t)w * h, out);
+            std::fwrite(uvtex.GetRawDataPtr(), 4, 3 * (size_t)w * h, out);
             // Texture2D::Save of the same image, next to the output file
             const std::string base = argv[2];
             if (!target.Save(base + ".png") || !target.Save(base + ".hdr")) {

@@ -40,6 +40,7 @@ def run(scene, size=None, saved=None):
         if saved is not None:
             saved["albedo"] = data[16 + n:16 + 2 * n].reshape(size[1], size[0], 3)
             saved["normal"] = data[16 + 2 * n:16 + 3 * n].reshape(size[1], size[0], 3)
+            saved["uv"] = data[16 + 3 * n:16 + 4 * n].reshape(size[1], size[0], 3)
     return cam, img
 
 

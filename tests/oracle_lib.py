@@ -37,6 +37,7 @@ def lib():
     L.oracle_scene_create.argtypes = [ctypes.c_int, _f, _f, _i, _i, ctypes.c_int, _i, _f,
                                       ctypes.c_int, _i, _f, ctypes.c_int, ctypes.c_int, _f, _f]
     L.oracle_scene_destroy.argtypes = [_v]
+    L.oracle_scene_set_uv.argtypes = [_v, _f]
     L.oracle_bvh_build.restype = _v
     L.oracle_bvh_build.argtypes = [_v, ctypes.c_int]
     L.oracle_bvh_destroy.argtypes = [_v]
@@ -105,6 +106,8 @@ class OracleScene:
             len(tri_mat), _ptr(tri_pos, _f), _ptr(tri_nrm, _f), _ptr(tri_mat, _i),
             _ptr(tri_smooth, _i), len(mat_type), _ptr(mat_type, _i), _ptr(mat_param, _f),
             len(lights), _ptr(lights, _i), _ptr(cam, _f), ew, eh, _ptr(env, _f), _ptr(env_param, _f))
+        if scene.get("tri_uv") is not None:
+            L.oracle_scene_set_uv(self.ptr, _ptr(k(f32(scene["tri_uv"])), _f))
         self.bvh = None
         self.num_tris = len(tri_mat)
 

@@ -25,7 +25,7 @@ o second
 usemtl other
 f 2/1 3/1 4/1
 """
-    p, n = loader.parse_obj(quad)
+    p, n, uv = loader.parse_obj(quad)
     assert p.shape == (3, 9)
     # the quad keeps its first three corners (no triangulation)
     assert np.array_equal(p[0], [0, 0, 0, 1, 0, 0, 1, 1, 0])
@@ -33,12 +33,16 @@ f 2/1 3/1 4/1
     assert np.array_equal(p[2], [1, 0, 0, 1, 1, 0, 0, 1, 0])
     # no normal indices: the Vertex default +Z
     assert np.array_equal(n, np.tile(np.array([0, 0, 1], f32), (3, 3)))
+    # texture coordinates where the face gives them (the last face: vt 1 on every corner)
+    assert np.array_equal(uv[:2], np.zeros((2, 6), f32)) and np.array_equal(uv[2], np.zeros(6, f32))
+    p, _, uv = loader.parse_obj("v 0 0 0\nv 1 0 0\nv 1 1 0\nvt 0.25 0.5\nvt 1 0.75 0\nf 1/1 2/2 3/-1\nf 1 2/1 3\n")
+    assert np.array_equal(uv, np.array([[0.25, 0.5, 1, 0.75, 1, 0.75], [0, 0, 0.25, 0.5, 0, 0]], f32))
     with_normals = "v 0 0 0\nv 1 0 0\nv 1 1 0\nvn 0 0 2\nvn 3 4 0\nf 1//1 2//2 3//-2\n"
-    p, n = loader.parse_obj(with_normals)
+    p, n, _ = loader.parse_obj(with_normals)
     # normalised per corner: v / sqrt(dot(v, v))
     assert np.array_equal(n[0], np.array([0, 0, 1, 0.6, 0.8, 0, 0, 0, 1], f32))
     # homogeneous and coloured vertices
-    p, _ = loader.parse_obj("v 2 4 6 2\nv 1 0 0 0.5 0.5 0.5\nv 0 1 0\nf 1 2 3\n")
+    p, _, _ = loader.parse_obj("v 2 4 6 2\nv 1 0 0 0.5 0.5 0.5\nv 0 1 0\nf 1 2 3\n")
     assert np.array_equal(p[0], [1, 2, 3, 1, 0, 0, 0, 1, 0])
     for bad in ("v 0 0 0\nv 1 0 0\nl 1 2\n", "v 0 0 0\nf 1 2 3\n", "v 0 0 0\nv 1 0 0\nf 1 2\n",
                 "v 0 0 0\nv 1 0 0\nv 1 1 0\nvn 0 0 1\nf 1 2 3\nf 1//1 2//1 3//1\n", "v 0 0 x\n"):
@@ -56,11 +60,11 @@ def test_obj_numbers_are_read_as_assimp_reads_them():
     texts += ["-0.5", "1e2", "2.5E-3", ".25", "7.", "3,5", "1.0000000000000000001"]
     want += [f32(-0.5), f32(100), f32(f32(2.5) * np.power(f32(10), f32(-3))), f32(0.25), f32(7), f32(3.5), f32(1)]
     lines = ["v %s 0 0" % t for t in texts] + ["f 1 2 3"]
-    p, _ = loader.parse_obj("\n".join(lines) + "\n")
+    p, _, _ = loader.parse_obj("\n".join(lines) + "\n")
     assert p.shape == (1, 9)
     # read the numbers back through the three corners of as many faces as needed
     faces = "\n".join("f %d %d %d" % (i + 1, i + 1, i + 1) for i in range(len(texts)))
-    p, _ = loader.parse_obj("\n".join(lines[:-1]) + "\n" + faces + "\n")
+    p, _, _ = loader.parse_obj("\n".join(lines[:-1]) + "\n" + faces + "\n")
     got = p[:, 0]
     assert np.array_equal(got.view(np.uint32), np.array(want, f32).view(np.uint32))
     # and the rounding differs from a correctly rounded parse on some of them

@@ -57,6 +57,32 @@ def test_aov_matches_oracle(gpu, name, aov, width):
         assert exact.mean() > 0.99
 
 
+@pytest.mark.parametrize("width", [2, 8])
+def test_uv_aov_matches_oracle(gpu, width):
+    # texture coordinates per corner (some outside [0, 1]); the UV AOV averages the first
+    # hit's interpolated (u, v, 0) over the camera rays
+    sc = dict(specular_spheres())
+    n = len(sc["tri_mat"])
+    sc["tri_uv"] = np.random.default_rng(4).uniform(-0.5, 1.5, (n, 6)).astype(np.float32)
+    b = ctx().scene(sc).build_bvh(16, width=width)
+    img, st = b.render_aov(64, 48, 6, 8, P.AOV_UV, 5)
+    o = O.OracleScene(sc).use_tree(b.export_nodes() | {"prim_ids": b.export_order()})
+    ref, q = o.render(64, 48, 6, 8, 4, 5, 0)
+    exact = np.all(img.view(np.uint32) == ref.view(np.uint32), axis=2)
+    log("uv aov width %d: queries %d / %d, bit-identical %d of %d" % (width, st["queries"], q, int(exact.sum()),
+                                                                     exact.size))
+    assert st["queries"] == q == 64 * 48 * 6
+    assert np.all(img[..., 2] == 0) and np.abs(ref[..., :2]).mean() > 0.1
+    if width == 2:
+        assert exact.all()
+    else:
+        assert exact.mean() > 0.99
+    # a scene without texture coordinates gives zero
+    b2 = ctx().scene(specular_spheres()).build_bvh(16, width=width)
+    z, _ = b2.render_aov(16, 12, 2, 8, P.AOV_UV, 5)
+    assert not np.any(z)
+
+
 def test_aov_seeded_and_on_device(gpu):
     import torch
     b = ctx().scene(scenes.cornell_box()).build_bvh(16)

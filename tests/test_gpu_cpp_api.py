@@ -48,3 +48,5 @@ def test_cpp_render_fills_denoising_aovs(gpu):
     nrm, _ = b.render_aov(40, 24, 4, 8, P.AOV_DENOISING_NORMAL, 3)
     assert bits_equal(saved["albedo"], alb) and bits_equal(saved["normal"], nrm)
     assert np.abs(nrm).mean() > 0.1
+    # the scene's vertices carry no texture coordinates: the UV target is all zero
+    assert not np.any(saved["uv"])
