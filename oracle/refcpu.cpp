@@ -120,7 +120,7 @@ struct Sampler2D { // RandomSampler2D.cpp:8-21 (x stream first, then y stream)
     }
 };
 
-// Per-pixel seeding defined by this build: the reference seeds one sampler pair per
+// Per-pixel seeding as the GPU library defines it (pt_device.h): the reference seeds one sampler pair per
 // tile and never advances the tile index (Core/Petrichor.cpp:72-77), so any
 // pixel-parallel renderer needs its own seeds. MurmurHash3's 32-bit finaliser
 // (fmix32) spreads (job seed, pixel, stream) into XorShift128's w word.

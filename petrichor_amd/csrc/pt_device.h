@@ -99,7 +99,7 @@ PT_HD void next2(Samplers& s, float* a, float* b)
     *b = xs_next(s.s2y);
 }
 
-// Per-pixel seeds (this build's definition; the reference seeds per tile):
+// Per-pixel seeds (this library's definition; the reference seeds per tile):
 // MurmurHash3 fmix32 over (job seed, pixel index, stream).
 PT_HD uint32_t fmix32(uint32_t h)
 {

@@ -30,7 +30,7 @@
 
 // Round to the nearest integer (ties to even) for |x| < 2^51, by adding and
 // subtracting 1.5*2^52. Exact on any IEEE double unit in round-to-nearest mode
-// (the sum is not re-associated: no -ffast-math anywhere in this build).
+// (the sum is not re-associated: nothing here is compiled with -ffast-math).
 PT_HD double pt_rint_small(double x)
 {
     const double kMagic = 6755399441055744.0;

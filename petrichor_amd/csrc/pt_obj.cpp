@@ -3,7 +3,7 @@
 // Mesh::Load (Core/Geometry/Mesh.cpp:18-125) reads a file through Assimp 4.1.0 (the
 // reference's pinned dependency, conanfile.txt:2) with no post-processing flags, copies
 // every vertex (position, normalised normal) and makes one triangle of the first three
-// indices of every face. Assimp is not part of this build; what its OBJ importer hands
+// indices of every face. Assimp is not linked here; what its OBJ importer hands
 // over for such a call is restated here:
 //  - one output vertex per face corner, faces in file order (objects, groups and
 //    material changes split meshes but keep that order);
