@@ -15,3 +15,32 @@ def test_camera_state_matches_python_restatement():
     sc = scenes.cornell_box()
     cam, _ = cpp_api.run(sc)
     assert np.array_equal(cam.view(np.uint32), np.asarray(sc["camera"], np.float32).view(np.uint32))
+
+
+def test_cpp_scene_files_load_without_a_device(tmp_path):
+    # Scene::LoadRenderSetting + Scene::LoadAssets (JSON, OBJ) run on the host: the driver
+    # gets past loading (and renders where a GPU exists); malformed or unsupported files
+    # stop at LoadAssets with the reason
+    import json
+    import subprocess
+
+    import cpp_api
+    from test_cpu_loader import cornell_files
+    _, scene_json = cornell_files(tmp_path)
+    settings = tmp_path / "settings.json"
+    settings.write_text(json.dumps({"outputWidth": 16, "outputHeight": 8, "spp": 1, "maxBounces": 2}))
+
+    def run(path):
+        return subprocess.run([cpp_api.LOAD_DRIVER, str(path), str(settings), str(tmp_path / "out.bin"), "0", "1"],
+                              stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+
+    r = run(scene_json)
+    assert r.returncode == 0 or (r.returncode == 1 and "render failed" in r.stdout), r.stdout
+    doc = json.loads(open(scene_json).read())
+    doc["materials"][0]["color_tex"] = "wood.png"
+    (tmp_path / "tex.json").write_text(json.dumps(doc))
+    r = run(tmp_path / "tex.json")
+    assert r.returncode == 1 and "LoadAssets" in r.stdout and "textured" in r.stdout, r.stdout
+    (tmp_path / "bad.json").write_text('{"materials": [ {"type": "lambert", ')
+    r = run(tmp_path / "bad.json")
+    assert r.returncode == 1 and "cannot read or parse" in r.stdout, r.stdout
