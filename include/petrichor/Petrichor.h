@@ -674,13 +674,30 @@ public:
     const Color3f& GetBaseColor() const { return m_baseColor; }
     float GetZAxisRotation() const { return m_ZAxisRotation; }
     bool UseEnvImportanceSampling() const { return m_texEnv && m_texEnv->IsValid() && m_useImportanceSampling; }
-    // Environment::Load (Core/Environment.cpp:15-27) of a Radiance .hdr map, which the
-    // environment then owns (shared by its copies). False for another format or a read error.
+    // Environment::Load (Core/Environment.cpp:15-27) of a Radiance .hdr or a .png map, which
+    // the environment then owns (shared by its copies). False for another format or a read
+    // error.
     bool Load(const std::filesystem::path& path)
     {
         int w = 0, h = 0;
         std::vector<float> rgb;
-        if (path.extension() != ".hdr" || !detail::loadHdr(path, w, h, rgb)) return false;
+        if (path.extension() == ".png") {
+            // a colour texture (Core/Environment.cpp:22): bytes / 255, gamma 2.2
+            std::string data;
+            if (!detail::readFile(path, data)) return false;
+            int32_t pw = 0, ph = 0;
+            if (pt_png_decode((const uint8_t*)data.data(), (int64_t)data.size(), 1, &pw, &ph, nullptr, nullptr, 0) !=
+                PT_OK)
+                return false;
+            rgb.resize(3 * (size_t)pw * ph);
+            if (pt_png_decode((const uint8_t*)data.data(), (int64_t)data.size(), 1, &pw, &ph, rgb.data(), nullptr, 0) !=
+                PT_OK)
+                return false;
+            w = pw;
+            h = ph;
+        } else if (path.extension() != ".hdr" || !detail::loadHdr(path, w, h, rgb)) {
+            return false;
+        }
         auto tex = std::make_shared<Texture2D>(w, h);
         std::memcpy(tex->GetRawDataPtr(), rgb.data(), rgb.size() * sizeof(float));
         m_ownedTex = tex;
@@ -909,7 +926,7 @@ public:
         Environment env;
         env.SetBaseColor(vec3(ed, "base_color"));
         if (const std::string tex = text(ed, "texture"); !tex.empty() && !env.Load(base / tex))
-            return fail("environment map " + (base / tex).string() + ": not a readable Radiance .hdr file");
+            return fail("environment map " + (base / tex).string() + ": not a readable .hdr or .png file");
         SetEnvironment(env);
         if (const Json* assets = j.find("assets"); assets && assets->type == Json::Array) {
             for (const Json& a : assets->items) {

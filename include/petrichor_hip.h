@@ -334,6 +334,16 @@ int pt_tonemap_rgb8(pt_context* ctx, int64_t num_pixels, const float* rgb, uint8
 int pt_obj_parse(const char* text, int64_t len, int64_t max_triangles, float* positions, float* normals,
                  float* uvs, int64_t* num_triangles, char* err, int32_t err_len);
 
+/* A PNG file's pixels as Texture2D::Load reads them (Core/Texture2D.cpp:107-160): 8-bit
+ * RGB as stbi_load(..., 3) gives it (grey replicated, alpha dropped, palettes expanded,
+ * 16-bit samples reduced to their high byte), each byte / 255, then powf(x, 2.2) per
+ * channel when color != 0 (ApplyGamma, Core/Texture2D.cpp:44-56). Sets *width, *height;
+ * with rgb_out == NULL only those, else fills width*height*3 floats (row-major, top row
+ * first). Non-interlaced images of bit depth 8 or 16. Host pointers, no device. Errors:
+ * PT_ERR_ARG with the reason in err. */
+int pt_png_decode(const uint8_t* data, int64_t len, int32_t color, int32_t* width, int32_t* height, float* rgb_out,
+                  char* err, int32_t err_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -158,6 +158,8 @@ def lib():
     L.pt_tonemap_rgb8.argtypes = [vp, ctypes.c_int64, vp, vp, ctypes.c_int32, vp]
     L.pt_obj_parse.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, _f, _f, _f, _l, ctypes.c_char_p,
                                ctypes.c_int32]
+    L.pt_png_decode.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int32, _i, _i, _f, ctypes.c_char_p,
+                                ctypes.c_int32]
     _lib = L
     return L
 
@@ -169,7 +171,7 @@ EXPORTED = ["pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_scene_create
             "pt_render_batch", "pt_engine_set_tuning", "pt_engine_get_tuning", "pt_server_create", "pt_server_destroy", "pt_server_last_error",
             "pt_server_add_scene", "pt_server_remove_scene", "pt_server_scene_resident", "pt_server_submit",
             "pt_server_poll", "pt_server_wait", "pt_server_preview", "pt_server_fetch", "pt_server_get_stats",
-            "pt_tonemap_rgb8", "pt_obj_parse", "pt_render_aov"]
+            "pt_tonemap_rgb8", "pt_obj_parse", "pt_render_aov", "pt_png_decode"]
 
 
 def _ptr(a, t):

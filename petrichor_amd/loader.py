@@ -10,7 +10,7 @@ meshes, turned into the scene dict of pt_scene_desc (see scenes.py).
   hands Mesh::Load for an OBJ file read with no post-processing flags: one output vertex
   per face corner, faces in file order, no triangulation, Assimp's number parsing.
 - The environment texture is read with load_hdr_file (Radiance RGBE, flat or run-length
-  scanlines, as stbi_loadf decodes them).
+  scanlines, as stbi_loadf decodes them) or, for a .png, with decode_png (pt_png_decode).
 
 Behaviour kept from the reference:
 - "emission" materials are never registered (Core/SceneLoader.cpp:219-221), so a
@@ -92,6 +92,24 @@ def parse_obj(text):
     if rc != 0:
         raise ValueError(err.value.decode())
     return p, nr, uv
+
+
+def decode_png(data, color=True):
+    """A PNG file's pixels as Texture2D::Load reads them (pt_png_decode): [H, W, 3] float32,
+    bytes / 255, then powf(x, 2.2) for a colour texture."""
+    import ctypes
+    from . import lib, _ptr, _f, _i
+    data = bytes(data)
+    L = lib()
+    w = np.zeros(1, np.int32)
+    h = np.zeros(1, np.int32)
+    err = ctypes.create_string_buffer(256)
+    if L.pt_png_decode(data, len(data), int(color), _ptr(w, _i), _ptr(h, _i), None, err, 256) != 0:
+        raise ValueError(err.value.decode())
+    out = np.zeros((int(h[0]), int(w[0]), 3), np.float32)
+    if L.pt_png_decode(data, len(data), int(color), _ptr(w, _i), _ptr(h, _i), _ptr(out, _f), err, 256) != 0:
+        raise ValueError(err.value.decode())
+    return out
 
 
 def load_obj(path):
@@ -215,9 +233,14 @@ def load_scene(path):
     env_rgb = None
     if ed.get("texture"):
         tex = os.path.join(base, ed["texture"])
-        if not tex.endswith(".hdr"):
-            raise ValueError("environment texture %s: only Radiance .hdr maps are read" % tex)
-        env_rgb = load_hdr_file(tex)
+        if tex.endswith(".hdr"):
+            env_rgb = load_hdr_file(tex)
+        elif tex.endswith(".png"):
+            # Environment::Load reads its texture as a colour texture (gamma 2.2)
+            with open(tex, "rb") as f:
+                env_rgb = decode_png(f.read(), color=True)
+        else:
+            raise ValueError("environment texture %s: only .hdr and .png maps are read" % tex)
     # ---- assets (Core/SceneLoader.cpp:284-300): smooth triangles of each OBJ ----
     tri_pos, tri_nrm, tri_uv, tri_mat = [], [], [], []
     for a in data.get("assets", []):

@@ -26,7 +26,10 @@ def test_cpp_scene_files_load_without_a_device(tmp_path):
 
     import cpp_api
     from test_cpu_loader import cornell_files
-    _, scene_json = cornell_files(tmp_path)
+    from test_cpu_png import encode
+    sky = np.random.default_rng(5).integers(0, 256, (8, 16, 3), dtype=np.uint8)
+    (tmp_path / "sky.png").write_bytes(encode(sky, 2))  # Environment::Load of a .png
+    _, scene_json = cornell_files(tmp_path, env_texture="sky.png")
     settings = tmp_path / "settings.json"
     settings.write_text(json.dumps({"outputWidth": 16, "outputHeight": 8, "spp": 1, "maxBounces": 2}))
 
