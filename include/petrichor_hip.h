@@ -66,8 +66,17 @@ typedef struct pt_scene_desc {
     float env_z_rotation;
     int32_t env_importance_sampling; /* Environment::useEnvImportanceSampling */
     /* Texture coordinates (Vertex::uv, Core/Geometry/Vertex.h:21): 6 floats per triangle
-     * (u, v of p0, p1, p2), or NULL for all zero. Read only by the UV AOV. */
+     * (u, v of p0, p1, p2), or NULL for all zero. Read by the UV AOV and the material maps. */
     const float* uvs;
+    /* Material maps (Lambert::SetTexAlbedo, GGX::SetF0Texture, GGX::SetRoughnessMap):
+     * num_textures textures, texture t of texture_sizes[2t] x texture_sizes[2t+1] RGB floats
+     * at textures[t] (row-major, as Texture2D holds them); material_maps, 2 per material:
+     * the albedo map of a Lambert or the reflectance map of a GGX, then the roughness map of
+     * a GGX; -1 for none. NULL material_maps: no maps. */
+    int32_t num_textures;
+    const int32_t* texture_sizes;
+    const float* const* textures;
+    const int32_t* material_maps;
 } pt_scene_desc;
 
 typedef struct pt_bvh_desc {

@@ -206,6 +206,8 @@ struct Material {
     float mix = 0.0f;             // Mix weight
     int mat0 = -1, mat1 = -1;     // Mix sub-materials
     bool is_enabled = true;       // MaterialBase::m_isImportanceSamplingEnabled
+    V albedo = {1.0f, 1.0f, 1.0f}; // Lambert::GetAlbedo: one, or kd x the albedo map (Lambert.cpp:117-126)
+    int map0 = -1, map1 = -1;     // albedo (Lambert) or reflectance (GGX) map; roughness map (GGX)
 };
 
 static Material make_material(int type, const float* p)
@@ -224,8 +226,9 @@ static Material make_material(int type, const float* p)
     return m;
 }
 
-struct Shading { // Core/HitInfo.h:20-27 (tangent and uv are not read by untextured materials)
+struct Shading { // Core/HitInfo.h:20-27 (the tangent is not read: no normal maps)
     V normal, pos;
+    V uv = {0.0f, 0.0f, 0.0f};
     int material = -1;
 };
 
@@ -253,7 +256,7 @@ static V bxdf(const Material& m, V din, V dout, const Shading& si)
 {
     switch (m.type) {
     case kLambert:
-        return scl(mul(m.color, mk(1.0f, 1.0f, 1.0f)), kInvPi);
+        return scl(mul(m.color, m.albedo), kInvPi); // m_kd * GetAlbedo(si) * kInvPi
     case kGlossy: {
         const V normal = si.normal;
         const V half = normalized(add(neg(din), dout));
@@ -382,7 +385,7 @@ static Ray next_ray(const Material& m, const Ray& in, const Shading& si, Sampler
         out.bounce = in.bounce + 1;
         if (m.is_enabled) {
             // throughput * GetAlbedo(): 1 for an untextured Lambert (Lambert.cpp:117-126)
-            out.throughput = mul(in.throughput, mk(1.0f, 1.0f, 1.0f));
+            out.throughput = mul(in.throughput, m.albedo);
         } else {
             out.throughput = in.throughput;
             const V f = bxdf(m, in.dir, out.dir, si);
@@ -673,6 +676,7 @@ struct Tri {
 struct Scene {
     std::vector<Tri> tris;
     std::vector<Material> mats;
+    std::vector<Texture> maps; // material map textures
     std::vector<int> lights;
     Camera cam;
     Environment env;
@@ -706,6 +710,20 @@ static inline bool tri_intersect(const Tri& tr, const Ray& ray, float* dist_out)
     return true;
 }
 
+// Triangle::Interpolate's texture coordinates (Triangle.cpp:90-108):
+// weightE1 * (uv1 - uv0) + weightE2 * (uv2 - uv0) + uv0.
+static V hit_uv_of(const Tri& tr, const Ray& ray)
+{
+    const V e1 = sub(tr.p1, tr.p0);
+    const V e2 = sub(tr.p2, tr.p0);
+    const float inv_det = 1.0f / dot(neg(ray.dir), cross(e1, e2));
+    const V vec = sub(ray.o, tr.p0);
+    const float w1 = dot(vec, cross(ray.dir, e2)) * inv_det;
+    const float w2 = dot(vec, cross(e1, ray.dir)) * inv_det;
+    return add(add(scl(sub(tr.uv1, tr.uv0), w1), scl(sub(tr.uv2, tr.uv0), w2)), tr.uv0);
+}
+static V hit_uv(const Scene& sc, int prim, const Ray& ray) { return hit_uv_of(sc.tris[prim], ray); }
+
 // Triangle::Interpolate (Triangle.cpp:90-153): position, shading normal, material.
 static Shading interpolate(const Scene& sc, int prim, const Ray& ray, float t)
 {
@@ -725,22 +743,31 @@ static Shading interpolate(const Scene& sc, int prim, const Ray& ray, float t)
         V n = add(add(scl(sub(tr.n1, tr.n0), w1), scl(sub(tr.n2, tr.n0), w2)), tr.n0);
         s.normal = divs(n, length(n));
     }
+    s.uv = hit_uv_of(tr, ray);
     s.material = sc.material_of(prim, 0.0f);
     return s;
 }
 
-// Triangle::Interpolate's texture coordinates (Triangle.cpp:90-108):
-// weightE1 * (uv1 - uv0) + weightE2 * (uv2 - uv0) + uv0.
-static V hit_uv(const Scene& sc, int prim, const Ray& ray)
+// GetLuminance (Core/Color3f.cpp:7-11)
+static inline float luminance(V c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+// Material m at a shading point with its maps applied: Lambert::GetAlbedo = kd x albedo map,
+// GGX::GetF0's reflectance x reflectance map, GGX::GetAlpha = (1 x luminance of the
+// roughness map)^2 (Lambert.cpp:117-126, GGX.h:57-75,126-139); maps read with
+// GetPixelByUV's default bilinear fetch.
+static Material material_at(const Scene& sc, int m, const V& uv)
 {
-    const Tri& tr = sc.tris[prim];
-    const V e1 = sub(tr.p1, tr.p0);
-    const V e2 = sub(tr.p2, tr.p0);
-    const float inv_det = 1.0f / dot(neg(ray.dir), cross(e1, e2));
-    const V vec = sub(ray.o, tr.p0);
-    const float w1 = dot(vec, cross(ray.dir, e2)) * inv_det;
-    const float w2 = dot(vec, cross(e1, ray.dir)) * inv_det;
-    return add(add(scl(sub(tr.uv1, tr.uv0), w1), scl(sub(tr.uv2, tr.uv0), w2)), tr.uv0);
+    Material md = sc.mats[m];
+    if (md.type == kLambert && md.map0 >= 0) {
+        md.albedo = mul(md.color, sc.maps[md.map0].by_uv(uv.x, uv.y));
+    } else if (md.type == kGlossy) {
+        if (md.map0 >= 0) md.color = mul(md.color, sc.maps[md.map0].by_uv(uv.x, uv.y));
+        if (md.map1 >= 0) {
+            const float r = 1.0f * luminance(sc.maps[md.map1].by_uv(uv.x, uv.y));
+            md.alpha = r * r;
+        }
+    }
+    return md;
 }
 
 // Triangle::SampleSurface (Triangle.cpp:155-182)
@@ -1064,7 +1091,7 @@ static V light_contribution(Ctx& c, const Shading& si, Sampler1D& s1, Sampler2D&
     float pdf_area = 1.0f;
     V pl_pos, pl_nrm;
     const bool is_area = sample_light(c, p, s1.Next(), s2, &pdf_area, true, &pl_pos, &pl_nrm);
-    const Material& mat = sc.mats[si.material];
+    const Material mat = material_at(sc, si.material, si.uv);
     if (is_area) {
         if (sc.lights.empty()) return mk(0.0f, 0.0f, 0.0f);
         Ray to_light;
@@ -1142,7 +1169,7 @@ static V render_mis(Ctx& c, int px, int py, int w, int h, int spp, int max_bounc
         for (int b = 0; b < max_bounces; b++) {
             color = add(color, light_contribution(c, si, s1, s2, ray));
             prev = ray;
-            const Material& mat = sc.mats[si.material];
+            const Material mat = material_at(sc, si.material, si.uv);
             ray = next_ray(mat, ray, si, s2);
             if (approx_eq(sqlen(ray.throughput), 0.0f, kEps)) break;
             float t;
@@ -1209,7 +1236,7 @@ static V render_simple(Ctx& c, int px, int py, int w, int h, int spp, int max_bo
                 break;
             }
             const Shading si = interpolate(sc, hit, ray, t);
-            ray = next_ray(sc.mats[m], ray, si, s2);
+            ray = next_ray(material_at(sc, m, si.uv), ray, si, s2);
             if (ray.bounce > max_bounces) {
                 ray.prob *= 0.9f;
                 ray.prob = smax(0.1f, ray.prob);
@@ -1249,9 +1276,9 @@ static V render_aov(Ctx& c, int px, int py, int w, int h, int spp, int max_bounc
             }
             const int m = sc.material_of(hit, s1.Next());
             const Shading si = interpolate(sc, hit, ray, t);
-            const Material& mat = sc.mats[m];
+            const Material mat = material_at(sc, m, si.uv);
             if (mat.type == kLambert) {
-                contrib = mul(ray.throughput, normal_aov ? si.normal : mk(1.0f, 1.0f, 1.0f));
+                contrib = mul(ray.throughput, normal_aov ? si.normal : mat.albedo);
                 break;
             }
             if (mat.type == kGlossy && mat.alpha != 0.0f) {
@@ -1702,6 +1729,25 @@ void oracle_scene_set_uv(void* s, const float* uv)
         sc->tris[i].uv0 = mk(q[0], q[1], 0.0f);
         sc->tris[i].uv1 = mk(q[2], q[3], 0.0f);
         sc->tris[i].uv2 = mk(q[4], q[5], 0.0f);
+    }
+}
+
+void oracle_scene_set_maps(void* s, int num_textures, const int32_t* sizes, const float* const* rgb,
+                           const int32_t* mat_maps)
+{
+    Scene* sc = (Scene*)s;
+    sc->maps.assign((size_t)num_textures, Texture());
+    for (int t = 0; t < num_textures; ++t) {
+        Texture& tx = sc->maps[t];
+        tx.w = sizes[2 * t];
+        tx.h = sizes[2 * t + 1];
+        tx.loaded = true;
+        tx.px.resize((size_t)tx.w * tx.h);
+        for (size_t i = 0; i < tx.px.size(); ++i) tx.px[i] = mk(rgb[t][3 * i], rgb[t][3 * i + 1], rgb[t][3 * i + 2]);
+    }
+    for (size_t m = 0; m < sc->mats.size(); ++m) {
+        sc->mats[m].map0 = mat_maps[2 * m];
+        sc->mats[m].map1 = mat_maps[2 * m + 1];
     }
 }
 

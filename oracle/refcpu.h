@@ -38,6 +38,10 @@ void* oracle_scene_create(int num_tris, const float* tri_pos, const float* tri_n
 void oracle_scene_destroy(void* scene);
 /* Vertex texture coordinates, 6 floats per triangle (u, v of p0, p1, p2); zero without. */
 void oracle_scene_set_uv(void* scene, const float* uv);
+/* Material maps: num_textures RGB textures (sizes: w, h each) and 2 map indices per
+ * material (albedo or reflectance map, roughness map; -1 none). */
+void oracle_scene_set_maps(void* scene, int num_textures, const int32_t* sizes, const float* const* rgb,
+                           const int32_t* mat_maps);
 
 /* Binary binned-SAH BVH, Core/Accel/BinnedSAHBVH.cpp:15-225 (num_bins = 16 there). */
 void* oracle_bvh_build(void* scene, int num_bins);

@@ -52,6 +52,7 @@ MaterialDev material_from(int type, const float* p)
     m.mat0 = type == kMatMix ? (int)p[4] : -1;
     m.mat1 = type == kMatMix ? (int)p[5] : -1;
     m.is_enabled = p[6] != 0.0f ? 1 : 0;
+    m.albedo = v3(1.0f, 1.0f, 1.0f);
     return m;
 }
 
@@ -290,6 +291,16 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
         }
     }
     if (d->env_rgb && (d->env_width <= 0 || d->env_height <= 0)) return fail(ctx, PT_ERR_ARG, "bad env size");
+    if (d->material_maps) {
+        if (d->num_textures < 0 || (d->num_textures > 0 && (!d->texture_sizes || !d->textures)))
+            return fail(ctx, PT_ERR_ARG, "bad material maps");
+        for (int t = 0; t < d->num_textures; ++t)
+            if (d->texture_sizes[2 * t] <= 0 || d->texture_sizes[2 * t + 1] <= 0 || !d->textures[t])
+                return fail(ctx, PT_ERR_ARG, "bad material map texture");
+        for (int i = 0; i < 2 * d->num_materials; ++i)
+            if (d->material_maps[i] < -1 || d->material_maps[i] >= d->num_textures)
+                return fail(ctx, PT_ERR_ARG, "material map index out of range");
+    }
     PT_HIP(ctx, hipSetDevice(ctx->device));
 
     pt_scene* s = new pt_scene();
@@ -346,13 +357,29 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
     if (err == hipSuccess) err = upload(&s->tri_smooth_d, s->tri_smooth.data(), s->tri_smooth.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&s->tri_nrm_d, s->tri_nrm.data(), s->tri_nrm.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&s->tri_uv_d, s->tri_uv.data(), s->tri_uv.size(), ctx->stream);
+    if (err == hipSuccess && d->material_maps) {
+        // material maps: the textures, their table and two map indices per material
+        std::vector<TexDev> tex;
+        for (int t = 0; t < d->num_textures && err == hipSuccess; ++t) {
+            const int tw = d->texture_sizes[2 * t], th = d->texture_sizes[2 * t + 1];
+            float* p = nullptr;
+            err = upload(&p, d->textures[t], 3 * (size_t)tw * th, ctx->stream);
+            s->tex_d.push_back(p);
+            tex.push_back(TexDev{p, tw, th});
+        }
+        std::vector<int2> maps(s->mats.size());
+        for (size_t m = 0; m < maps.size(); ++m) maps[m] = make_int2(d->material_maps[2 * m], d->material_maps[2 * m + 1]);
+        if (err == hipSuccess) err = upload(&s->textures_d, tex.data(), tex.size(), ctx->stream);
+        if (err == hipSuccess) err = upload(&s->mat_maps_d, maps.data(), maps.size(), ctx->stream);
+    }
     if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
     s->device_bytes = (int64_t)(s->mats.size() * (sizeof(MaterialDev) + 4) + lights.size() * sizeof(float4)) +
                       (d->env_rgb ? 3 * (int64_t)e.w * e.h * 4 : 0) +
                       (int64_t)nt * 40 + (int64_t)(s->tri_smooth.size() * 4 + s->tri_nrm.size() * 4);
     if (err != hipSuccess) {
+        for (float* p : s->tex_d) (void)hipFree(p);
         free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->mat_shading_of_d, s->tri_pos_d,
-                  s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d, s->tri_uv_d});
+                  s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d, s->tri_uv_d, s->textures_d, s->mat_maps_d});
         delete s;
         return fail(ctx, err == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
                     std::string("scene upload: ") + hipGetErrorString(err));
@@ -364,8 +391,9 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
 int pt_scene_destroy(pt_scene* s)
 {
     if (!s) return PT_ERR_ARG;
+    for (float* p : s->tex_d) (void)hipFree(p);
     free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->mat_shading_of_d, s->tri_pos_d,
-              s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d, s->tri_uv_d});
+              s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d, s->tri_uv_d, s->textures_d, s->mat_maps_d});
     delete s;
     return PT_OK;
 }

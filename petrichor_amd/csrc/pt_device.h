@@ -240,7 +240,27 @@ struct MaterialDev {
     float mix;    // Mix weight
     int mat0, mat1;
     int is_enabled;
+    V3 albedo;    // Lambert::GetAlbedo: one, or kd x the albedo map at the hit (Lambert.cpp:117-126)
 };
+
+// A texture of a material map: row-major RGB floats.
+struct TexDev {
+    const float* rgb;
+    int w, h;
+};
+
+// Texture2D::GetPixelByUV (Texture2D.cpp:325-333) with its default bilinear fetch: u and v
+// wrapped into [0, 1), scaled by the texture's size.
+PT_HD V3 tex_bilinear(const float* rgb, int w, int h, float x, float y);
+PT_HD V3 tex_by_uv(const TexDev& t, float u, float v)
+{
+    u = fmodp(u, 1.0f);
+    v = fmodp(v, 1.0f);
+    return tex_bilinear(t.rgb, t.w, t.h, u * t.w, v * t.h);
+}
+
+// GetLuminance (Core/Color3f.cpp:7-11)
+PT_HD float luminance(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
 // GGX::GetF0 (GGX.h:126-139), metalness 0, base colour 1, no map.
 PT_HD V3 ggx_f0(const MaterialDev& m)
@@ -261,7 +281,7 @@ PT_HD float ggx_lambda(V3 dir, V3 half, float alpha)
 // Material::BxDF (Lambert.cpp:27-33, GGX.cpp:22-57, Glass.cpp:11-19).
 PT_HD V3 bxdf(const MaterialDev& m, V3 din, V3 dout, V3 normal)
 {
-    if (m.type == kMatLambert) return m.color * v3(1.0f, 1.0f, 1.0f) * PT_INV_PI;
+    if (m.type == kMatLambert) return m.color * m.albedo * PT_INV_PI;
     if (m.type == kMatGlossy) {
         const V3 half = normalized(-din + dout);
         const float hn = fabsf_(dot(half, normal));
@@ -356,7 +376,7 @@ PT_HD Ray next_ray(const MaterialDev& m, const Ray& in, V3 pos, V3 sn, Samplers&
             const float cv = (float)((double)sin_theta * sp);
             const float cw = __builtin_sqrtf(1.0f - sin_theta * sin_theta);
             out.dir = cu * u + cv * vv + cw * w;
-            out.throughput = in.throughput * v3(1.0f, 1.0f, 1.0f);
+            out.throughput = in.throughput * m.albedo;
         } else {
             const float cos_theta = r0;
             const float sin_theta = __builtin_sqrtf(1.0f - cos_theta * cos_theta);

@@ -226,6 +226,8 @@ struct pt_engine {
         sc.tri_smooth = b->tri_smooth_d;
         sc.tri_nrm = b->tri_nrm_d;
         sc.tri_uv = b->tri_uv_d;
+        sc.textures = s->textures_d;
+        sc.mat_maps = s->mat_maps_d;
         sc.mats = s->mats_d;
         sc.lights = s->lights_d;
         sc.num_lights = s->num_lights;

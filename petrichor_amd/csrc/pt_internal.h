@@ -37,6 +37,9 @@ struct pt_scene {
     float* tri_nrm_d = nullptr;
     std::vector<float> tri_uv;         // 6 floats per triangle, or empty
     float* tri_uv_d = nullptr;
+    std::vector<float*> tex_d;         // material map textures
+    pt::TexDev* textures_d = nullptr;
+    int2* mat_maps_d = nullptr;        // per material, or null without maps
     int32_t* mat_shading_of_d = nullptr; // material -> material that shades it (mix resolved)
     int64_t device_bytes = 0; // materials, lights, environment, triangles
 };

@@ -33,6 +33,8 @@ struct SceneView {
     const int32_t* tri_smooth;      // per leaf slot, or null
     const float* tri_nrm;           // 9 floats per leaf slot, or null
     const float* tri_uv = nullptr;  // 6 floats per leaf slot (texture coordinates), or null
+    const TexDev* textures = nullptr; // material map textures
+    const int2* mat_maps = nullptr;   // per material: albedo / reflectance map, roughness map (-1: none); or null
     const MaterialDev* mats;
     const float4* lights;           // light triangles, AppendLight order, 3 float4 each
     int num_lights;

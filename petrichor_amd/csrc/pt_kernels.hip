@@ -72,6 +72,48 @@ __device__ __forceinline__ void shade_point(const SceneView& s, int k, V3 o, V3 
     shade_point_tr(s, load_tri(s.bvh.tris, k), k, o, d, t, pos, nrm);
 }
 
+// Texture coordinates of a hit at leaf slot k (Triangle::Interpolate, Triangle.cpp:90-108):
+// the barycentric weights of the hit, uv = w1 (uv1 - uv0) + w2 (uv2 - uv0) + uv0, with the
+// third component zero; all zero when the scene has no texture coordinates.
+__device__ __forceinline__ V3 hit_uv(const SceneView& s, int k, V3 o, V3 d)
+{
+    if (s.tri_uv == nullptr) return v3(0.0f, 0.0f, 0.0f);
+    const TriDev tr = load_tri(s.bvh.tris, k);
+    const float inv_det = 1.0f / dot(-d, tr.n);
+    const V3 vec = o - tr.p0;
+    const float w1 = dot(vec, cross(d, tr.e2)) * inv_det;
+    const float w2 = dot(vec, cross(tr.e1, d)) * inv_det;
+    const float* q = s.tri_uv + 6 * (size_t)k;
+    const V3 uv0 = v3(q[0], q[1], 0.0f), uv1 = v3(q[2], q[3], 0.0f), uv2 = v3(q[4], q[5], 0.0f);
+    return w1 * (uv1 - uv0) + w2 * (uv2 - uv0) + uv0;
+}
+
+// Material m at a hit with texture coordinates uv, its maps applied: Lambert::GetAlbedo =
+// kd x the albedo map, GGX::GetF0's reflectance x the reflectance map, GGX::GetAlpha =
+// (strength x luminance of the roughness map)^2 with the default strength 1 (Lambert.cpp:
+// 117-126, GGX.h:57-75,126-139). Without maps the material as stored.
+__device__ __forceinline__ MaterialDev material_at(const SceneView& s, int m, V3 uv)
+{
+    MaterialDev md = load_mat(s, m);
+    if (s.mat_maps == nullptr) return md;
+    const int2 mm = s.mat_maps[m];
+    if (md.type == kMatLambert && mm.x >= 0) {
+        md.albedo = md.color * tex_by_uv(s.textures[mm.x], uv.x, uv.y);
+    } else if (md.type == kMatGlossy) {
+        if (mm.x >= 0) md.color = md.color * tex_by_uv(s.textures[mm.x], uv.x, uv.y);
+        if (mm.y >= 0) {
+            const float r = 1.0f * luminance(tex_by_uv(s.textures[mm.y], uv.x, uv.y));
+            md.alpha = r * r;
+        }
+    }
+    return md;
+}
+// Texture coordinates of the hit, read only when the scene has maps.
+__device__ __forceinline__ V3 maps_uv(const SceneView& s, int k, V3 o, V3 d)
+{
+    return s.mat_maps != nullptr ? hit_uv(s, k, o, d) : v3(0.0f, 0.0f, 0.0f);
+}
+
 // PathTracing::SampleLight (PathTracing.cpp:329-386). Returns false for the env slot.
 __device__ __forceinline__ bool sample_light(const SceneView& s, float rnd, Samplers& smp, bool want_env,
                                              float* pdf_area, V3* pl_pos)
@@ -281,7 +323,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
 
         // 2. the extension hit
         bool at_surface = false;
-        V3 si_pos = v3(0.0f, 0.0f, 0.0f), si_n = v3(0.0f, 0.0f, 1.0f);
+        V3 si_pos = v3(0.0f, 0.0f, 0.0f), si_n = v3(0.0f, 0.0f, 1.0f), si_uv = v3(0.0f, 0.0f, 0.0f);
         int si_mat = 0;
         int loop_b = meta.w;
         if (phase == kPhWait) {
@@ -308,6 +350,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
                 } else {
                     shade_point_tr(sc, pre_tr, k, ray.o, ray.dir, t, &si_pos, &si_n);
                     si_mat = pre_mat;
+                    si_uv = maps_uv(sc, k, ray.o, ray.dir);
                     if (sc.mats[si_mat].type == kMatEmission) {
                         sum = sum + ray.throughput * sc.mats[si_mat].color;
                         sample_done = true;
@@ -340,6 +383,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
                             si_pos = npos;
                             si_n = nn;
                             si_mat = nmat;
+                            si_uv = maps_uv(sc, k, ray.o, ray.dir);
                         } else {
                             sample_done = true;
                         }
@@ -372,7 +416,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
                 sample_done = true;
             } else {
                 const int m = material_of(sc, k, next1(smp));
-                const MaterialDev mat = load_mat(sc, m);
+                const MaterialDev mat = material_at(sc, m, maps_uv(sc, k, ray.o, ray.dir));
                 if (mat.type == kMatEmission) {
                     color = color + ray.throughput * mat.color;
                     sample_done = true;
@@ -403,7 +447,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
         // 3. a surface of the MIS loop: NEE then BSDF sampling
         int new_pend = kPendNone;
         if (INTEGRATOR == 0 && at_surface) {
-            const MaterialDev mat = load_mat(sc, si_mat);
+            const MaterialDev mat = material_at(sc, si_mat, si_uv);
             const float dd = -dot(ray.dir, si_n);
             const V3 pnee = si_pos + PT_EPS * copysignf_(1.0f, dd) * si_n;
             float pdf_area = 1.0f;
@@ -591,22 +635,6 @@ __device__ __forceinline__ void trace_single(const BvhView& b, V3 o, float dmin,
 // -dir on a miss. An Emission hit gives zero (the reference's default branch), and so
 // does a path still going after max_bounces hits. The paths are short (most end at the
 // first hit), so they are not run through the wavefront queues.
-// Texture coordinates of a hit at leaf slot k (Triangle::Interpolate, Triangle.cpp:90-108):
-// the barycentric weights of the hit, uv = w1 (uv1 - uv0) + w2 (uv2 - uv0) + uv0, with the
-// third component zero; all zero when the scene has no texture coordinates.
-__device__ __forceinline__ V3 hit_uv(const SceneView& s, int k, V3 o, V3 d)
-{
-    if (s.tri_uv == nullptr) return v3(0.0f, 0.0f, 0.0f);
-    const TriDev tr = load_tri(s.bvh.tris, k);
-    const float inv_det = 1.0f / dot(-d, tr.n);
-    const V3 vec = o - tr.p0;
-    const float w1 = dot(vec, cross(d, tr.e2)) * inv_det;
-    const float w2 = dot(vec, cross(tr.e1, d)) * inv_det;
-    const float* q = s.tri_uv + 6 * (size_t)k;
-    const V3 uv0 = v3(q[0], q[1], 0.0f), uv1 = v3(q[2], q[3], 0.0f), uv2 = v3(q[4], q[5], 0.0f);
-    return w1 * (uv1 - uv0) + w2 * (uv2 - uv0) + uv0;
-}
-
 // The UV AOV (AOVUVCoordinate::Render, Core/AOV/AOVUVCoordinate.cpp:11-51) is mode 2: the
 // mean over spp camera rays of the first hit's texture coordinates (a miss adds zero).
 template <bool WIDE>
@@ -647,9 +675,9 @@ __global__ void __launch_bounds__(256) k_render_aov(SceneView sc, int w, int h, 
                 const int m = material_of(sc, k, next1(smp));
                 V3 pos, sn;
                 shade_point(sc, k, ray.o, ray.dir, t, &pos, &sn);
-                const MaterialDev mat = load_mat(sc, m);
+                const MaterialDev mat = material_at(sc, m, maps_uv(sc, k, ray.o, ray.dir));
                 if (mat.type == kMatLambert) {
-                    contrib = ray.throughput * (normal_aov ? sn : v3(1.0f, 1.0f, 1.0f));
+                    contrib = ray.throughput * (normal_aov ? sn : mat.albedo);
                     break;
                 }
                 if (mat.type == kMatGlossy && mat.alpha != 0.0f) {

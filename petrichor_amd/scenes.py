@@ -3,7 +3,9 @@
 A scene is a dict of arrays in the layout of pt_scene_desc (include/petrichor_hip.h):
   tri_pos [N, 9] f32, tri_mat [N] i32, tri_nrm / tri_smooth / tri_uv [N, 6] (optional),
   mat_type [M] i32, mat_param [M, 8] f32, light_tris [L] i32, camera [16] f32,
-  env_rgb [H, W, 3] f32 or None, env_param [5] f32 (base rgb, z rotation, importance sampling).
+  env_rgb [H, W, 3] f32 or None, env_param [5] f32 (base rgb, z rotation, importance sampling);
+  optional material maps: textures (list of [h, w, 3] f32) and mat_maps [M, 2] i32 (albedo or
+  reflectance map, roughness map; -1 none), read at the texture coordinates tri_uv.
 Every generator takes an explicit seed.
 """
 import numpy as np

@@ -38,6 +38,7 @@ def lib():
                                       ctypes.c_int, _i, _f, ctypes.c_int, ctypes.c_int, _f, _f]
     L.oracle_scene_destroy.argtypes = [_v]
     L.oracle_scene_set_uv.argtypes = [_v, _f]
+    L.oracle_scene_set_maps.argtypes = [_v, ctypes.c_int, _i, ctypes.POINTER(_f), _i]
     L.oracle_bvh_build.restype = _v
     L.oracle_bvh_build.argtypes = [_v, ctypes.c_int]
     L.oracle_bvh_destroy.argtypes = [_v]
@@ -108,6 +109,12 @@ class OracleScene:
             len(lights), _ptr(lights, _i), _ptr(cam, _f), ew, eh, _ptr(env, _f), _ptr(env_param, _f))
         if scene.get("tri_uv") is not None:
             L.oracle_scene_set_uv(self.ptr, _ptr(k(f32(scene["tri_uv"])), _f))
+        if scene.get("mat_maps") is not None:
+            texs = [k(f32(t)) for t in scene.get("textures", [])]
+            sizes = k(np.array([[t.shape[1], t.shape[0]] for t in texs], np.int32).reshape(-1))
+            ptrs = k((_f * max(len(texs), 1))(*[_ptr(t, _f) for t in texs]))
+            L.oracle_scene_set_maps(self.ptr, len(texs), _ptr(sizes, _i), ctypes.cast(ptrs, ctypes.POINTER(_f)),
+                                    _ptr(k(i32(scene["mat_maps"])), _i))
         self.bvh = None
         self.num_tris = len(tri_mat)
 

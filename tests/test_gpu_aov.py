@@ -95,3 +95,47 @@ def test_aov_seeded_and_on_device(gpu):
     assert not np.array_equal(a, c)
     with pytest.raises(P.PtError):
         b.render_aov(40, 30, 4, 8, 7, 3)
+
+
+def textured_spheres():
+    """The small spheres scene with texture coordinates and material maps: an albedo map on
+    every Lambert, a reflectance map on even and a roughness map on odd GGX materials."""
+    sc = dict(scenes.spheres_scene(n_spheres=8, seg=16, bands=9, terrain_n=32, env_w=32, env_h=16))
+    rng = np.random.default_rng(6)
+    n, m = len(sc["tri_mat"]), len(sc["mat_type"])
+    sc["tri_uv"] = rng.uniform(-0.25, 1.25, (n, 6)).astype(np.float32)
+    sc["textures"] = [rng.uniform(0.05, 1.0, (9, 13, 3)).astype(np.float32),
+                      rng.uniform(0.2, 1.0, (5, 7, 3)).astype(np.float32),
+                      rng.uniform(0.0, 1.0, (16, 4, 3)).astype(np.float32)]
+    maps = -np.ones((m, 2), np.int32)
+    for i, t in enumerate(sc["mat_type"]):
+        if t == P.PT_MAT_LAMBERT:
+            maps[i, 0] = 0
+        elif t == P.PT_MAT_GLOSSY:
+            maps[i, 0 if i % 2 == 0 else 1] = 1 if i % 2 == 0 else 2
+    sc["mat_maps"] = maps
+    return sc
+
+
+@pytest.mark.parametrize("integrator", [0, 1, "albedo"])
+def test_material_maps_match_oracle(gpu, integrator):
+    sc = textured_spheres()
+    b = ctx().scene(sc).build_bvh(16)
+    o = O.OracleScene(sc).use_tree(b.export_nodes() | {"prim_ids": b.export_order()})
+    if integrator == "albedo":
+        img, st = b.render_aov(64, 48, 4, 8, P.AOV_DENOISING_ALBEDO, 5)
+        ref, q = o.render(64, 48, 4, 8, 2, 5, 0)
+    else:
+        img, st = b.render(64, 48, 4, 8, integrator, 5)
+        ref, q = o.render(64, 48, 4, 8, integrator, 5, 0)
+    err = rel_err(img, ref).max(axis=2)
+    log("material maps, %s: queries %d / %d, max rel err %g" % (integrator, st["queries"], q, float(err.max())))
+    assert st["queries"] == q
+    assert (err <= REL_TOL).all()
+    # the maps change the image
+    plain = dict(sc)
+    plain.pop("mat_maps")
+    b2 = ctx().scene(plain).build_bvh(16)
+    img2 = (b2.render_aov(64, 48, 4, 8, P.AOV_DENOISING_ALBEDO, 5) if integrator == "albedo"
+            else b2.render(64, 48, 4, 8, integrator, 5))[0]
+    assert not np.array_equal(img, img2)
