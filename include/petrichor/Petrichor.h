@@ -108,17 +108,28 @@ private:
     bool m_isImportanceSamplingEnabled = true;
 };
 
+class Texture2D;
+
+// Core/Material/Lambert.h:13-49. With an albedo texture, GetAlbedo is kd x the texture at
+// the hit's uv (so the BxDF is kd x kd x texture / pi); without, 1.
 class Lambert : public MaterialBase
 {
 public:
     explicit Lambert(const Color3f& kd) : m_kd(kd) {}
+    Lambert(const Color3f& kd, const Texture2D& albedoTex) : m_kd(kd), m_texAlbedo(&albedoTex) {}
     MaterialTypes GetMaterialType() const override { return MaterialTypes::Lambert; }
     const Color3f& GetKd() const { return m_kd; }
+    void SetTexAlbedo(const Texture2D* texAlbedo) { m_texAlbedo = texAlbedo; }
+    const Texture2D* GetTexAlbedo() const { return m_texAlbedo; }
 
 private:
     Color3f m_kd;
+    const Texture2D* m_texAlbedo = nullptr;
 };
 
+// Core/Material/GGX.h:13-164 with metalness 0. A reflectance (F0) texture multiplies the
+// reflectance; a roughness map replaces the roughness by the luminance of the map. Normal
+// maps are not rendered: Render reports a material that has one.
 class GGX : public MaterialBase
 {
 public:
@@ -126,10 +137,19 @@ public:
     MaterialTypes GetMaterialType() const override { return MaterialTypes::Glossy; }
     const Color3f& GetReflectance() const { return m_f0; }
     float GetRoughness() const { return m_roughness; }
+    void SetF0Texture(const Texture2D* texture) { m_reflectanceMap = texture; }
+    void SetRoughnessMap(const Texture2D* roughnessMap) { m_roughnessMap = roughnessMap; }
+    void SetNormalMap(const Texture2D* normalMap) { m_normalMap = normalMap; }
+    const Texture2D* GetF0Texture() const { return m_reflectanceMap; }
+    const Texture2D* GetRoughnessMap() const { return m_roughnessMap; }
+    const Texture2D* GetNormalMap() const { return m_normalMap; }
 
 private:
     Color3f m_f0;
     float m_roughness;
+    const Texture2D* m_reflectanceMap = nullptr;
+    const Texture2D* m_roughnessMap = nullptr;
+    const Texture2D* m_normalMap = nullptr;
 };
 
 class Glass : public MaterialBase
@@ -1038,7 +1058,10 @@ public:
 private:
     struct Flat {
         std::vector<float> pos, nrm, uv, matParam, env;
-        std::vector<int32_t> matId, smooth, matType, lights;
+        std::vector<int32_t> matId, smooth, matType, lights, matMaps, texSizes;
+        std::vector<const float*> texPixels;
+        std::unordered_map<const Texture2D*, int32_t> texIndex;
+        bool anyMap = false;
         pt_scene_desc desc{};
     };
 
@@ -1056,17 +1079,28 @@ private:
         auto it = index.find(m);
         if (it != index.end()) return it->second;
         float p[8] = {0, 0, 0, 0, 0, 0, m->IsImportanceSamplingEnabled() ? 1.0f : 0.0f, 0};
+        int32_t maps[2] = {-1, -1};
         auto rgb = [&](const Color3f& c) {
             p[0] = c.x;
             p[1] = c.y;
             p[2] = c.z;
         };
         switch (m->GetMaterialType()) {
-        case MaterialTypes::Lambert: rgb(static_cast<const Lambert*>(m)->GetKd()); break;
-        case MaterialTypes::Glossy:
-            rgb(static_cast<const GGX*>(m)->GetReflectance());
-            p[3] = static_cast<const GGX*>(m)->GetRoughness();
+        case MaterialTypes::Lambert:
+            rgb(static_cast<const Lambert*>(m)->GetKd());
+            maps[0] = texture_index(static_cast<const Lambert*>(m)->GetTexAlbedo(), f);
+            if (maps[0] == -3) return -3;
             break;
+        case MaterialTypes::Glossy: {
+            const auto* g = static_cast<const GGX*>(m);
+            rgb(g->GetReflectance());
+            p[3] = g->GetRoughness();
+            if (g->GetNormalMap()) return -2;
+            maps[0] = texture_index(g->GetF0Texture(), f);
+            maps[1] = texture_index(g->GetRoughnessMap(), f);
+            if (maps[0] == -3 || maps[1] == -3) return -3;
+            break;
+        }
         case MaterialTypes::Glass:
             rgb(static_cast<const Glass*>(m)->GetColor());
             p[3] = static_cast<const Glass*>(m)->GetIor();
@@ -1077,6 +1111,7 @@ private:
             // sub-materials first, so that their indices are known
             const int32_t a = material_index(mix->GetMaterial0(), index, f);
             const int32_t b = material_index(mix->GetMaterial1(), index, f);
+            if (a < 0 || b < 0) return a < 0 ? a : b;
             p[3] = mix->GetMix();
             p[4] = (float)a;
             p[5] = (float)b;
@@ -1086,7 +1121,24 @@ private:
         const int32_t id = (int32_t)f.matType.size();
         f.matType.push_back(type_code(m->GetMaterialType()));
         f.matParam.insert(f.matParam.end(), p, p + 8);
+        f.matMaps.insert(f.matMaps.end(), maps, maps + 2);
         index[m] = id;
+        return id;
+    }
+
+    // Index of a material texture in the descriptor's list (-1: none; -3: empty texture).
+    static int32_t texture_index(const Texture2D* t, Flat& f)
+    {
+        if (!t) return -1;
+        if (!t->IsValid()) return -3;
+        auto it = f.texIndex.find(t);
+        if (it != f.texIndex.end()) return it->second;
+        const int32_t id = (int32_t)f.texPixels.size();
+        f.texPixels.push_back(t->GetRawDataPtr());
+        f.texSizes.push_back(t->GetWidth());
+        f.texSizes.push_back(t->GetHeight());
+        f.texIndex[t] = id;
+        f.anyMap = true;
         return id;
     }
 
@@ -1113,7 +1165,10 @@ private:
                 if (anyUV) f.uv.insert(f.uv.end(), {v->uv.x, v->uv.y});
             }
             if (anySmooth) f.smooth.push_back(t->GetShadingType() == ShadingTypes::Smooth ? 1 : 0);
-            f.matId.push_back(material_index(t->GetMaterial(), matIndex, f));
+            const int32_t mi = material_index(t->GetMaterial(), matIndex, f);
+            if (mi == -2) return fail("normal maps are not rendered by the GPU path");
+            if (mi == -3) return fail("a material texture is empty");
+            f.matId.push_back(mi);
         }
         for (const GeometryBase* l : scene.GetLights()) {
             auto it = triIndex.find(l);
@@ -1123,6 +1178,7 @@ private:
         if (f.matType.empty()) { // pt_scene_create wants one material even for an empty scene
             f.matType.push_back(type_code(MaterialTypes::Lambert));
             f.matParam.assign(8, 0.0f);
+            f.matMaps.assign(2, -1);
         }
         pt_scene_desc& d = f.desc;
         d.num_triangles = (int32_t)f.matId.size();
@@ -1149,6 +1205,12 @@ private:
         d.env_z_rotation = env.GetZAxisRotation();
         d.env_importance_sampling = env.UseEnvImportanceSampling() ? 1 : 0;
         d.uvs = anyUV ? f.uv.data() : nullptr;
+        if (f.anyMap) {
+            d.num_textures = (int32_t)f.texPixels.size();
+            d.texture_sizes = f.texSizes.data();
+            d.textures = f.texPixels.data();
+            d.material_maps = f.matMaps.data();
+        }
         return true;
     }
 

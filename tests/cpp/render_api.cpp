@@ -5,7 +5,9 @@
 //
 //   render_api IN OUT [W H SPP DEPTH INTEGRATOR SEED]   (no size: camera state only)
 // IN:  int32 n, int32 n_lights, float pos[9 n], int32 material[n] (0 white, 1 red,
-//      2 green, 3 light), int32 lights[n_lights]
+//      2 green, 3 light, 4 GGX), int32 lights[n_lights]; optionally float uv[6 n], int32
+//      tw, th, float texture[th][tw][3]: the vertices' uv, and the texture as the white
+//      Lambert's albedo texture and the GGX's F0 texture and roughness map
 // OUT: float camera[16], then W*H*3 floats each of the image, the denoising albedo and
 //      normal AOVs (numSppForDenoising = SPP) and the UV AOV
 #include <cstdio>
@@ -33,22 +35,45 @@ int main(int argc, char** argv)
     ok = ok && std::fread(pos.data(), 4, pos.size(), in) == pos.size();
     ok = ok && std::fread(mat.data(), 4, mat.size(), in) == mat.size();
     ok = ok && std::fread(lights.data(), 4, lights.size(), in) == lights.size();
+    std::vector<float> uv(6 * (size_t)n), texels;
+    int32_t tsize[2] = {0, 0};
+    const bool textured = ok && std::fread(uv.data(), 4, uv.size(), in) == uv.size() && n > 0;
+    if (textured) {
+        ok = std::fread(tsize, 4, 2, in) == 2 && tsize[0] > 0 && tsize[1] > 0;
+        texels.resize(ok ? 3 * (size_t)tsize[0] * tsize[1] : 0);
+        ok = ok && std::fread(texels.data(), 4, texels.size(), in) == texels.size();
+    }
     std::fclose(in);
     if (!ok) return 2;
+    Texture2D tex(textured ? tsize[0] : 0, textured ? tsize[1] : 0);
+    for (int j = 0; j < tsize[1]; ++j)
+        for (int i = 0; i < tsize[0]; ++i) {
+            const float* c = &texels[3 * ((size_t)j * tsize[0] + i)];
+            tex.SetPixel(i, j, Color3f(c[0], c[1], c[2]));
+        }
 
     Scene scene;
-    scene.RegisterMaterial("white", std::make_unique<Lambert>(Color3f(0.73f, 0.73f, 0.73f)));
+    auto white = std::make_unique<Lambert>(Color3f(0.73f, 0.73f, 0.73f));
+    auto ggx = std::make_unique<GGX>(Color3f(0.9f, 0.8f, 0.7f), 0.3f);
+    if (textured) {
+        white->SetTexAlbedo(&tex);
+        ggx->SetF0Texture(&tex);
+        ggx->SetRoughnessMap(&tex);
+    }
+    scene.RegisterMaterial("white", std::move(white));
     scene.RegisterMaterial("red", std::make_unique<Lambert>(Color3f(0.65f, 0.05f, 0.05f)));
     scene.RegisterMaterial("green", std::make_unique<Lambert>(Color3f(0.05f, 0.65f, 0.05f)));
     scene.RegisterMaterial("light", std::make_unique<Emission>(Color3f(15.0f, 15.0f, 15.0f)));
-    const MaterialBase* mats[4] = {scene.GetMaterial("white"), scene.GetMaterial("red"), scene.GetMaterial("green"),
-                                   scene.GetMaterial("light")};
+    scene.RegisterMaterial("ggx", std::move(ggx));
+    const MaterialBase* mats[5] = {scene.GetMaterial("white"), scene.GetMaterial("red"), scene.GetMaterial("green"),
+                                   scene.GetMaterial("light"), scene.GetMaterial("ggx")};
     Mesh mesh(3 * (size_t)n);
     for (int i = 0; i < n; ++i) {
         const float* p = &pos[9 * (size_t)i];
-        const Vertex* v0 = mesh.AddVertex(Vertex(p[0], p[1], p[2]));
-        const Vertex* v1 = mesh.AddVertex(Vertex(p[3], p[4], p[5]));
-        const Vertex* v2 = mesh.AddVertex(Vertex(p[6], p[7], p[8]));
+        const float* tc = &uv[6 * (size_t)i];
+        const Vertex* v0 = mesh.AddVertex(Vertex(Vector3f(p[0], p[1], p[2]), Vector3f::UnitZ(), Vector3f(tc[0], tc[1], 0.0f)));
+        const Vertex* v1 = mesh.AddVertex(Vertex(Vector3f(p[3], p[4], p[5]), Vector3f::UnitZ(), Vector3f(tc[2], tc[3], 0.0f)));
+        const Vertex* v2 = mesh.AddVertex(Vertex(Vector3f(p[6], p[7], p[8]), Vector3f::UnitZ(), Vector3f(tc[4], tc[5], 0.0f)));
         Triangle t(v0, v1, v2);
         t.SetMaterial(mats[mat[i]]);
         mesh.AddTriangle(t);

@@ -15,7 +15,9 @@ def run(scene, size=None, saved=None):
     size: (width, height, spp, depth, integrator, seed) to render. saved: a dict that
     receives the files Texture2D::Save wrote of the image ("png": uint8 [h, w, 3],
     "hdr": float32 [h, w, 3]) and the denoising AOVs Petrichor::Render filled ("albedo",
-    "normal")."""
+    "normal"). A scene with mat_maps also hands over tri_uv and its first texture, which
+    the driver sets as the white Lambert's albedo texture and the GGX (material 4) F0
+    texture and roughness map."""
     n = len(scene["tri_mat"])
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
@@ -24,6 +26,11 @@ def run(scene, size=None, saved=None):
             np.ascontiguousarray(scene["tri_pos"], np.float32).tofile(f)
             np.ascontiguousarray(scene["tri_mat"], np.int32).tofile(f)
             np.ascontiguousarray(scene["light_tris"], np.int32).tofile(f)
+            if scene.get("mat_maps") is not None:
+                tex = np.ascontiguousarray(scene["textures"][0], np.float32)
+                np.ascontiguousarray(scene["tri_uv"], np.float32).tofile(f)
+                np.array([tex.shape[1], tex.shape[0]], np.int32).tofile(f)
+                tex.tofile(f)
         args = [DRIVER, fin, fout] + ([str(v) for v in size] if size else [])
         r = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
         assert r.returncode == 0, r.stdout

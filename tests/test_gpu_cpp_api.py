@@ -50,3 +50,31 @@ def test_cpp_render_fills_denoising_aovs(gpu):
     assert np.abs(nrm).mean() > 0.1
     # the scene's vertices carry no texture coordinates: the UV target is all zero
     assert not np.any(saved["uv"])
+
+
+def test_cpp_render_with_material_maps(gpu):
+    # Vertex uv, Lambert::SetTexAlbedo and GGX::SetF0Texture / SetRoughnessMap through
+    # Petrichor::Render equal the Python binding's descriptor with the same maps, and the
+    # maps change the image
+    log("C++ API material maps")
+    sc = dict(scenes.cornell_box())
+    rng = np.random.default_rng(9)
+    n = len(sc["tri_mat"])
+    tri_mat = np.asarray(sc["tri_mat"], np.int32).copy()
+    tri_mat[(tri_mat == 0) & (np.arange(n) % 3 == 0)] = 4
+    mt = np.append(np.asarray(sc["mat_type"], np.int32), scenes.PT_MAT_GLOSSY).astype(np.int32)
+    mp = np.vstack([sc["mat_param"], np.array([0.9, 0.8, 0.7, 0.3, 0, 0, 1, 0], np.float32)])
+    assert mt[4] == scenes.PT_MAT_GLOSSY
+    maps = -np.ones((5, 2), np.int32)
+    maps[0, 0] = 0
+    maps[4] = (0, 0)
+    sc.update(tri_mat=tri_mat, mat_type=mt, mat_param=mp, tri_uv=rng.uniform(-0.3, 1.3, (n, 6)).astype(np.float32),
+              textures=[rng.uniform(0.1, 1.0, (6, 11, 3)).astype(np.float32)], mat_maps=maps)
+    for integrator in (0, 1):
+        _, img = cpp_api.run(sc, (48, 36, 4, 8, integrator, 5))
+        ref, _ = ctx().scene(sc).build_bvh(16).render(48, 36, 4, 8, integrator, 5)
+        assert bits_equal(img, ref)
+        plain = dict(sc)
+        plain.pop("mat_maps")
+        other, _ = ctx().scene(plain).build_bvh(16).render(48, 36, 4, 8, integrator, 5)
+        assert not np.array_equal(img, other)
