@@ -25,6 +25,7 @@
 #include <cstring>
 #include <filesystem>
 #include <functional>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -887,8 +888,13 @@ public:
     // camera, environment and OBJ assets of a JSON scene file, paths relative to it. As in
     // the reference: "default" is a white Lambert, "emission" materials register nothing,
     // and the camera takes SetFNumber before SetFocalLength. False (with *error) where the
-    // reference would use a missing material or texture, for textured materials (not
-    // rendered by the GPU path) and for environment maps other than Radiance .hdr.
+    // reference would use a missing material, for material textures other than .png and for
+    // environment maps other than Radiance .hdr and .png. Material textures are read as the
+    // reference reads them (Core/SceneLoader.cpp:106-189): a Lambert's and a GGX's
+    // "color_tex" from the scene's folder as colour textures, a GGX's "roughness_tex" from
+    // the path as written (relative to the working directory) as a non-colour texture; a
+    // file that does not exist leaves the material untextured; a texture registered under
+    // a name already taken reuses the first one (Core/Scene.h:131-143).
     bool LoadAssets(const std::filesystem::path& path, std::string* error = nullptr)
     {
         auto fail = [&](const std::string& msg) {
@@ -918,14 +924,33 @@ public:
             for (const Json& m : mats->items) {
                 const std::string type = text(&m, "type"), name = text(&m, "name");
                 if (type.empty() || name.empty()) continue;
-                if (!text(&m, "color_tex").empty() || !text(&m, "roughness_tex").empty())
-                    return fail("material '" + name + "': textured materials are not rendered by the GPU path");
+                // Texture2D::Load then RegisterTexture; nullptr where the file does not exist
+                std::string texError;
+                auto texture = [&](const std::filesystem::path& file, const std::string& key,
+                                   bool color) -> const Texture2D* {
+                    if (!std::filesystem::exists(file)) return nullptr;
+                    Texture2D t;
+                    if (!loadPngTexture(file, color, t)) {
+                        texError = "material '" + name + "': cannot read texture " + file.string() +
+                                   " (material textures are read from .png files)";
+                        return nullptr;
+                    }
+                    return &m_textures.emplace(key, std::move(t)).first->second;
+                };
+                const std::string colorTex = text(&m, "color_tex"), roughTex = text(&m, "roughness_tex");
                 if (type == "lambert") {
-                    RegisterMaterial(name, std::make_unique<Lambert>(vec3(&m, "base_color")));
+                    auto lm = std::make_unique<Lambert>(vec3(&m, "base_color"));
+                    if (!colorTex.empty()) lm->SetTexAlbedo(texture(base / colorTex, (base / colorTex).string(), true));
+                    if (!texError.empty()) return fail(texError);
+                    RegisterMaterial(name, std::move(lm));
                 } else if (type == "glass") {
                     RegisterMaterial(name, std::make_unique<Glass>(vec3(&m, "color"), num(&m, "ior", 1.0f)));
                 } else if (type == "glossy") {
-                    RegisterMaterial(name, std::make_unique<GGX>(vec3(&m, "color"), num(&m, "roughness", 0.0f)));
+                    auto gm = std::make_unique<GGX>(vec3(&m, "color"), num(&m, "roughness", 0.0f));
+                    if (!colorTex.empty()) gm->SetF0Texture(texture(base / colorTex, colorTex, true));
+                    if (!roughTex.empty()) gm->SetRoughnessMap(texture(roughTex, roughTex, false));
+                    if (!texError.empty()) return fail(texError);
+                    RegisterMaterial(name, std::move(gm));
                 } else if (type == "mix") {
                     const MaterialBase* m0 = GetMaterial(text(&m, "mat0_name"));
                     const MaterialBase* m1 = GetMaterial(text(&m, "mat1_name"));
@@ -962,7 +987,25 @@ private:
     std::vector<const GeometryBase*> m_geometries;
     std::vector<const GeometryBase*> m_lights;
     std::unordered_map<std::string, std::unique_ptr<const MaterialBase>> m_materials;
+    std::map<std::string, Texture2D> m_textures; // loaded material textures by name (nodes stay put)
     Environment m_environment;
+
+    // A .png file as Texture2D::Load reads it: RGB bytes / 255, then gamma 2.2 for a colour
+    // texture (Core/Texture2D.cpp:107-160).
+    static bool loadPngTexture(const std::filesystem::path& path, bool color, Texture2D& out)
+    {
+        std::string data;
+        if (path.extension() != ".png" || !detail::readFile(path, data)) return false;
+        int32_t w = 0, h = 0;
+        const auto* bytes = (const uint8_t*)data.data();
+        if (pt_png_decode(bytes, (int64_t)data.size(), color ? 1 : 0, &w, &h, nullptr, nullptr, 0) != PT_OK)
+            return false;
+        Texture2D t(w, h);
+        if (pt_png_decode(bytes, (int64_t)data.size(), color ? 1 : 0, &w, &h, t.GetRawDataPtr(), nullptr, 0) != PT_OK)
+            return false;
+        out = std::move(t);
+        return true;
+    }
     std::unique_ptr<Camera> m_mainCamera;
     std::array<Texture2D*, AOVType::NumAOVTypes> m_targetTextures = {};
     RenderSetting m_renderSetting;

@@ -22,10 +22,15 @@ Behaviour kept from the reference:
 - a vertex without a normal has +Z (Core/Geometry/Vertex.h:19);
 - every loaded triangle is smooth-shaded.
 
-Where the reference would dereference a missing material or texture (an unknown
-material name, a mix of unknown materials, a point or line element) or needs a feature
-the GPU path does not render (textured materials), these functions raise ValueError
-instead.
+Material textures are read as Core/SceneLoader.cpp:106-189 reads them: a Lambert's and a
+GGX's "color_tex" from the scene's folder as colour textures, a GGX's "roughness_tex"
+from the path as written (relative to the working directory) as a non-colour texture. A
+file that does not exist leaves the material untextured, and a texture registered under a
+name already taken reuses the first one (Core/Scene.h:131-143).
+
+Where the reference would dereference a missing material (an unknown material name, a
+mix of unknown materials, a point or line element) or reads a file this module does not
+(material textures other than .png), these functions raise ValueError instead.
 """
 import json
 import os
@@ -183,18 +188,34 @@ def load_scene(path):
     # ---- materials (Core/SceneLoader.cpp:81-240): "default" first, then in file order ----
     names, types, params = {}, [], []
 
-    def register(name, mtype, color, p3=0.0, sub=(0, 0)):
+    textures, tex_names, maps = [], {}, []
+
+    def texture(path, key, color, name):
+        # Texture2D::Load, then Scene::RegisterTexture (the first texture of a name stays)
+        if not os.path.exists(path):
+            return -1
+        if not path.endswith(".png"):
+            raise ValueError("material '%s': material textures are read from .png files (%s)" % (name, path))
+        with open(path, "rb") as f:
+            rgb = decode_png(f.read(), color=color)
+        if key not in tex_names:
+            tex_names[key] = len(textures)
+            textures.append(rgb)
+        return tex_names[key]
+
+    def register(name, mtype, color, p3=0.0, sub=(0, 0), mmap=(-1, -1)):
         p = np.zeros(8, np.float32)
         p[0:3] = color
         p[3] = p3
         p[4], p[5] = sub
         p[6] = 1.0  # importance sampling on (the materials' default)
         if name in names:  # Scene::RegisterMaterial replaces an entry of the same name
-            types[names[name]], params[names[name]] = mtype, p
+            types[names[name]], params[names[name]], maps[names[name]] = mtype, p, mmap
         else:
             names[name] = len(types)
             types.append(mtype)
             params.append(p)
+            maps.append(mmap)
 
     def material(name):
         if name not in names:
@@ -206,14 +227,16 @@ def load_scene(path):
         mtype, name = m.get("type", ""), m.get("name", "")
         if not mtype or not name:
             continue
-        if m.get("color_tex") or m.get("roughness_tex"):
-            raise ValueError("material '%s': textured materials are not rendered by the GPU path" % name)
+        ctex, rtex = m.get("color_tex") or "", m.get("roughness_tex") or ""
         if mtype == "lambert":
-            register(name, PT_MAT_LAMBERT, _vec3(m, "base_color"))
+            t = texture(os.path.join(base, ctex), os.path.join(base, ctex), True, name) if ctex else -1
+            register(name, PT_MAT_LAMBERT, _vec3(m, "base_color"), mmap=(t, -1))
         elif mtype == "glass":
             register(name, PT_MAT_GLASS, _vec3(m, "color"), f32(m.get("ior", 1.0)))
         elif mtype == "glossy":
-            register(name, PT_MAT_GLOSSY, _vec3(m, "color"), f32(m.get("roughness", 0.0)))
+            t0 = texture(os.path.join(base, ctex), ctex, True, name) if ctex else -1
+            t1 = texture(rtex, rtex, False, name) if rtex else -1
+            register(name, PT_MAT_GLOSSY, _vec3(m, "color"), f32(m.get("roughness", 0.0)), mmap=(t0, t1))
         elif mtype == "mix":
             i0, i1 = material(m.get("mat0_name", "")), material(m.get("mat1_name", ""))
             register(name, PT_MAT_MIX, np.zeros(3, np.float32), f32(m.get("mix", 0.5)), (i0, i1))
@@ -264,4 +287,6 @@ def load_scene(path):
         "camera": cam,
         "env_rgb": env_rgb,
         "env_param": env_param,
+        **({"textures": textures, "mat_maps": np.asarray(maps, np.int32).reshape(-1, 2)}
+           if any(t >= 0 for mm in maps for t in mm) else {}),
     }

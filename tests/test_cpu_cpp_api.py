@@ -40,10 +40,12 @@ def test_cpp_scene_files_load_without_a_device(tmp_path):
     r = run(scene_json)
     assert r.returncode == 0 or (r.returncode == 1 and "render failed" in r.stdout), r.stdout
     doc = json.loads(open(scene_json).read())
-    doc["materials"][0]["color_tex"] = "wood.png"
+    # material textures are read from .png files; another existing file is reported
+    (tmp_path / "wood.jpg").write_bytes(b"\xff\xd8\xff")
+    doc["materials"][0]["color_tex"] = "wood.jpg"
     (tmp_path / "tex.json").write_text(json.dumps(doc))
     r = run(tmp_path / "tex.json")
-    assert r.returncode == 1 and "LoadAssets" in r.stdout and "textured" in r.stdout, r.stdout
+    assert r.returncode == 1 and "LoadAssets" in r.stdout and ".png" in r.stdout, r.stdout
     (tmp_path / "bad.json").write_text('{"materials": [ {"type": "lambert", ')
     r = run(tmp_path / "bad.json")
     assert r.returncode == 1 and "cannot read or parse" in r.stdout, r.stdout

@@ -191,7 +191,12 @@ def test_scene_file_errors(tmp_path):
     with pytest.raises(ValueError, match="light"):
         loader.load_scene(str(tmp_path / "bad.json"))
     doc = json.loads(open(path).read())
+    # a missing texture file leaves the material untextured; an existing non-PNG one is reported
     doc["materials"][0]["color_tex"] = "wood.png"
     (tmp_path / "tex.json").write_text(json.dumps(doc))
-    with pytest.raises(ValueError, match="textured"):
+    assert "mat_maps" not in loader.load_scene(str(tmp_path / "tex.json"))
+    (tmp_path / "wood.jpg").write_bytes(b"\xff\xd8\xff")
+    doc["materials"][0]["color_tex"] = "wood.jpg"
+    (tmp_path / "tex.json").write_text(json.dumps(doc))
+    with pytest.raises(ValueError, match=".png"):
         loader.load_scene(str(tmp_path / "tex.json"))

@@ -1,0 +1,86 @@
+"""Material textures in the scene files (Core/SceneLoader.cpp:106-189): a Lambert's and a
+GGX's "color_tex" (colour textures, from the scene's folder) and a GGX's "roughness_tex"
+(a non-colour texture, from the path as written), read by the Python and the C++ loaders
+and rendered with OBJ texture coordinates."""
+import json
+
+import numpy as np
+import pytest
+
+import cpp_api
+import oracle_lib as O
+import petrichor_amd as P
+from gpu_util import bits_equal, ctx, log, rel_err
+from petrichor_amd import loader
+from test_gpu_loader import sphere_obj
+from test_cpu_loader import cornell_files
+
+REL_TOL = 1e-4
+
+
+def textured_sphere_obj(path, *args):
+    """sphere_obj with one texture coordinate per vertex (u around, v from pole to pole)."""
+    sphere_obj(path, *args)
+    lines = path.read_text().splitlines()
+    nv = sum(1 for l in lines if l.startswith("v "))
+    nu = 16
+    vts = ["vt %r %r" % ((k % nu) / nu, (k // nu) / (nv // nu - 1)) for k in range(nv)]
+    faces = [" ".join(["f"] + ["%s/%s/%s" % (c.split("//")[0], c.split("//")[0], c.split("//")[1]) for c in l.split()[1:]])
+             if l.startswith("f ") else l for l in lines]
+    path.write_text("\n".join(vts + faces) + "\n")
+
+
+def textured_scene(tmp_path):
+    rng = np.random.default_rng(4)
+    albedo = rng.integers(0, 256, (6, 10, 3), dtype=np.uint8)
+    rough = rng.integers(40, 256, (5, 7, 3), dtype=np.uint8)
+    P.save_png(str(tmp_path / "albedo.png"), albedo)
+    P.save_png(str(tmp_path / "rough.png"), rough)
+    _, path = cornell_files(tmp_path)
+    textured_sphere_obj(tmp_path / "ball.obj", 0.3, 0.35, 0.22, 0.18)
+    textured_sphere_obj(tmp_path / "ball2.obj", 0.7, 0.6, 0.25, 0.2)
+    doc = json.loads(open(path).read())
+    doc["materials"] += [
+        {"type": "lambert", "name": "tex_white", "base_color": [0.9, 0.85, 0.8], "color_tex": "albedo.png"},
+        {"type": "glossy", "name": "tex_ggx", "color": [0.9, 0.9, 0.8], "roughness": 0.3,
+         "color_tex": "albedo.png", "roughness_tex": str(tmp_path / "rough.png")},
+        {"type": "lambert", "name": "no_file", "base_color": [0.5, 0.5, 0.5], "color_tex": "missing.png"},
+    ]
+    doc["assets"] += [{"path": "ball.obj", "mat": "tex_white"}, {"path": "ball2.obj", "mat": "tex_ggx"}]
+    (tmp_path / "scene.json").write_text(json.dumps(doc))
+    return albedo, rough
+
+
+def test_loader_registers_material_textures(tmp_path):
+    albedo, rough = textured_scene(tmp_path)
+    sc = loader.load_scene(str(tmp_path / "scene.json"))
+    # "emission" materials register nothing
+    names = [m["name"] for m in json.loads((tmp_path / "scene.json").read_text())["materials"]
+             if m["type"] != "emission"]
+    assert len(sc["textures"]) == 3  # the albedo under two names (joined and as written), the roughness map
+    idx = {n: i + 1 for i, n in enumerate(names)}  # "default" comes first
+    maps = sc["mat_maps"]
+    lam, ggx, nofile = maps[idx["tex_white"]], maps[idx["tex_ggx"]], maps[idx["no_file"]]
+    assert lam[1] == -1 and nofile.tolist() == [-1, -1]
+    np.testing.assert_array_equal(sc["textures"][lam[0]], loader.decode_png((tmp_path / "albedo.png").read_bytes()))
+    np.testing.assert_array_equal(sc["textures"][ggx[0]], sc["textures"][lam[0]])
+    np.testing.assert_array_equal(sc["textures"][ggx[1]], rough.astype(np.float32) / np.float32(255))
+    assert sc["tri_uv"] is not None and sc["tri_uv"].max() > 0.9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_textured_scene_file_renders_as_the_oracle(gpu, tmp_path, integrator):
+    textured_scene(tmp_path)
+    sc = loader.load_scene(str(tmp_path / "scene.json"))
+    log("textured scene file, integrator", integrator)
+    b = ctx().scene(sc).build_bvh(16)
+    img, stats = b.render(48, 36, 4, 8, integrator, 3)
+    o = O.OracleScene(sc).use_tree(b.export_nodes() | {"prim_ids": b.export_order()})
+    ref, q = o.render(48, 36, 4, 8, integrator, 3, 0)
+    assert stats["queries"] == q
+    assert (rel_err(img, ref).max(axis=2) <= REL_TOL).all()
+    (tmp_path / "settings.json").write_text(json.dumps({"outputWidth": 48, "outputHeight": 36, "spp": 4,
+                                                         "maxBounces": 8}))
+    _, cimg = cpp_api.run_loaded(str(tmp_path / "scene.json"), str(tmp_path / "settings.json"), integrator, 3)
+    assert bits_equal(cimg, img)
