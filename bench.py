@@ -42,20 +42,50 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """Threads for the CPU baseline and a description of the host. The reference's pool
+    takes std::thread::hardware_concurrency() workers (Core/Thread/ThreadPool.cpp:19-26,
+    Core/RenderSetting.h:17-21); a shared host gives this process a share of its cores
+    (OMP_NUM_THREADS names it where it is set, else the affinity mask), and the baseline
+    uses that share."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = nproc
+    share = affinity
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        share = min(share, int(env))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return share, {"nproc": nproc, "affinity": affinity, "cpu_model": model,
+                   "omp_num_threads": env or None}
+
+
 def cpu_baseline(sc, tree, width, height, spp, depth, seconds):
     """The CPU oracle (clean-room restatement of the reference renderer, tile pool of
-    std::threads as Core/Petrichor.cpp) on a random pixel sample of the same frame."""
+    std::threads as Core/Petrichor.cpp) on a random pixel sample of the same frame: one
+    random permutation of the pixels, rendered in consecutive slices, so no pixel is
+    rendered twice."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    threads = min(os.cpu_count() or 1, 16)
+    threads, host = host_cpu()
     o = O.OracleScene(sc).use_tree(tree)
-    rng = np.random.default_rng(12345)
+    order = np.random.default_rng(12345).permutation(width * height).astype(np.int32)
     n = 2048
     total_q, total_t, total_px = 0, 0.0, 0
     work = np.zeros(9, np.int64)
     pixels, rgbs = [], []
     while True:
-        pix = np.sort(rng.choice(width * height, n, replace=False)).astype(np.int32)
+        pix = np.sort(order[total_px:total_px + n])
         w = np.zeros(9, np.int64)
         t0 = time.perf_counter()
         rgb, q = o.render(width, height, spp, depth, 0, 1, threads, pixels=pix, work=w)
@@ -70,11 +100,14 @@ def cpu_baseline(sc, tree, width, height, spp, depth, seconds):
             break
         n = int(min(max(n * 2, n * (seconds - total_t) / max(dt, 1e-3)), width * height - total_px, 1 << 20))
         n = max(n, 1)
+    pix_all = np.concatenate(pixels)
+    assert len(np.unique(pix_all)) == len(pix_all)
     return {
         "value": total_q / total_t / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-        "sample": "%d random pixels of the 1920x1080 frame at full 64 spp / depth 12 (%.1f s, %d queries)"
-                  % (total_px, total_t, total_q),
-    }, work, np.concatenate(pixels), np.concatenate(rgbs)
+        "sample": "%d distinct random pixels of the %dx%d frame at full %d spp / depth %d (%.1f s, %d queries)"
+                  % (total_px, width, height, spp, depth, total_t, total_q),
+        "host": host,
+    }, work, pix_all, np.concatenate(rgbs)
 
 
 def torch_primary_rays(cam, width, height, gen):
@@ -125,7 +158,7 @@ def cpu_trace_baseline(sc, tree, rays_host, seconds):
     from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    threads = min(os.cpu_count() or 1, 16)
+    threads, host = host_cpu()
     o = O.OracleScene(sc).use_tree(tree)
     n = 1 << 14
     done, total_t = 0, 0.0
@@ -147,7 +180,7 @@ def cpu_trace_baseline(sc, tree, rays_host, seconds):
     pool.shutdown()
     return {"value": done / total_t / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": "%d rays (a random half-primary, half-surface sample of the 2^26), closest hit, %.1f s"
-                      % (done, total_t)}, np.concatenate(ts), np.concatenate(ps)
+                      % (done, total_t), "host": host}, np.concatenate(ts), np.concatenate(ps)
 
 
 def trace_bench(args):
@@ -537,12 +570,12 @@ def tonemap_bench(args):
         import oracle_lib as O
         sample = min(n, 1 << 24)
         host = img.view(-1, 3)[:sample].cpu().numpy()
-        threads = min(os.cpu_count() or 1, 16)
+        threads, hinfo = host_cpu()
         t1 = time.perf_counter()
         ref = O.tonemap_rgb8(host, threads)
         cpu_s = time.perf_counter() - t1
         cpu = {"value": round(sample / cpu_s / 1e6, 2), "unit": "Mpixels/s", "cores": threads, "kind": "port",
-               "sample": "the first %d pixels of the same image" % sample}
+               "sample": "the first %d pixels of the same image" % sample, "host": hinfo}
         parity = {"pixels_checked": sample, "bytes_equal": int((got[:sample] == ref).sum()),
                   "bytes_checked": int(ref.size)}
     traffic = None  # rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, committed for one image size
@@ -597,13 +630,13 @@ def aov_bench(args):
         o = O.OracleScene(sc).use_tree(bvh.export_nodes() | {"prim_ids": bvh.export_order()})
         rng = np.random.default_rng(3)
         pix = rng.choice(W * H, 4096, replace=False).astype(np.int32)
-        threads = min(os.cpu_count() or 1, 16)
+        threads, hinfo = host_cpu()
         t1 = time.perf_counter()
         ref, q = o.render(W, H, SPP, D, 2, 1, threads, pixels=pix)
         cpu_s = time.perf_counter() - t1
         img = bvh.render_aov(W, H, SPP, D, P.AOV_DENOISING_ALBEDO, 1)[0].reshape(-1, 3)[pix]
         cpu = {"value": round(q / cpu_s / 1e6, 2), "unit": "Mrays/s", "cores": threads, "kind": "port",
-               "sample": "albedo AOV of 4096 random pixels", "pixels_bit_identical": int(np.all(img == ref, axis=1).sum())}
+               "sample": "albedo AOV of 4096 random pixels", "host": hinfo, "pixels_bit_identical": int(np.all(img == ref, axis=1).sum())}
     print(json.dumps({"metric": "denoising AOV Mrays/s (albedo, normal)", "value": res["albedo"]["mrays_s"],
                       "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
                       "dtype": "f32", "data": "synthetic",
