@@ -15,7 +15,6 @@
 // sorted here at the end by the key of the last sort the reference applied to it
 // (parent's widest axis, or its own for an SAH leaf), ties by triangle index.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cfloat>
@@ -929,7 +928,7 @@ hipError_t gpu_build_bvh(const float* tri_pos_d, int n, int num_bins, int leaf_m
     B_CHECK(tmp.alloc(&scan_val, max_active + 1));
     B_CHECK(tmp.alloc(&scan_excl, max_active + 1));
     size_t scan_bytes = 0;
-    B_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, scan_val, scan_excl, (int)max_active + 1, st));
+    B_CHECK(exclusive_sum(nullptr, scan_bytes, scan_val, scan_excl, (size_t)((int)max_active + 1), st));
     char* scan_tmp;
     B_CHECK(tmp.alloc(&scan_tmp, scan_bytes));
 
@@ -1006,7 +1005,7 @@ hipError_t gpu_build_bvh(const float* tri_pos_d, int n, int num_bins, int leaf_m
         sa.scan_val = scan_val;
         hipLaunchKernelGGL(k_sah, dim3(blocks(64 * (size_t)num_active)), dim3(256), 0, st, sa);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t*)(scan_val + num_active), 0u, (size_t)2);
-        B_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, scan_val, scan_excl, num_active + 1, st));
+        B_CHECK(exclusive_sum(scan_tmp, scan_bytes, scan_val, scan_excl, (size_t)(num_active + 1), st));
         unsigned long long tot = 0;
         B_CHECK(hipMemcpyAsync(&tot, scan_excl + num_active, sizeof tot, hipMemcpyDeviceToHost, st));
         ChildArgs ca;
@@ -1112,10 +1111,10 @@ hipError_t gpu_pack_bvh(const GpuBvhResult& r, int num_tris, const float* tri_po
                        (size_t)num_tris + 1);
     hipLaunchKernelGGL(k_mark_big, dim3(blocks(nn)), dim3(256), 0, st, r.nodes_d, nn, mark);
     size_t scan_bytes = 0;
-    B_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, mark, big_index, num_tris + 1, st));
+    B_CHECK(exclusive_sum(nullptr, scan_bytes, mark, big_index, (size_t)(num_tris + 1), st));
     char* scan_tmp;
     B_CHECK(tmp.alloc(&scan_tmp, scan_bytes));
-    B_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, mark, big_index, num_tris + 1, st));
+    B_CHECK(exclusive_sum(scan_tmp, scan_bytes, mark, big_index, (size_t)(num_tris + 1), st));
     int counts[2] = {0, 0};
     B_CHECK(hipMemcpyAsync(&counts[0], inner_count, sizeof(int), hipMemcpyDeviceToHost, st));
     B_CHECK(hipMemcpyAsync(&counts[1], big_index + num_tris, sizeof(int), hipMemcpyDeviceToHost, st));

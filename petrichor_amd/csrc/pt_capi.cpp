@@ -245,6 +245,7 @@ int pt_ctx_create(int device, pt_context** out)
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);
     if (e != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return PT_ERR_HIP;
     }
@@ -634,9 +635,10 @@ int api_trace(pt_bvh* b, int64_t n, const float* rays, bool any, float* t_out, i
             b->order_temp_bytes = tmp;
             b->order_cap = cap;
         }
+        int32_t* sorted = nullptr;
         PT_HIP(ctx, launch_ray_order((const float4*)rays, n32, b->view.root_lo, b->view.root_hi, b->order_keys_d,
-                                     b->order_idx_d, b->order_temp_d, b->order_temp_bytes, st));
-        job.seg[0].path = b->order_idx_d + n; // sorted indices: ray and hit of item i at path[i]
+                                     b->order_idx_d, b->order_temp_d, b->order_temp_bytes, st, &sorted));
+        job.seg[0].path = sorted; // sorted indices: ray and hit of item i at path[i]
         job.seg[0].rays_by_path = 1;
     }
     PT_HIP(ctx, launch_zero(b->cursor_d, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));

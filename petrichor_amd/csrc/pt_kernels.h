@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/petrichor_hip.h"
 #include "pt_device.h"
 
 // Traversal stack entries per ray. A walk holds at most one pending sibling per
@@ -137,6 +138,9 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job, int* cursor, 
                             hipStream_t st, const LaunchTuning& tune = LaunchTuning());
 // Zeroes counters[i * stride] for i < n (n <= 64).
 hipError_t launch_zero(int* counters, int n, hipStream_t st, int stride = 1);
+// Exclusive prefix sum of n integers (rocPRIM); call with temp == nullptr to size temp.
+template <typename T>
+hipError_t exclusive_sum(void* temp, size_t& temp_bytes, const T* in, T* out, size_t n, hipStream_t st);
 // Compute units of the current device (queried once, then cached; called before any
 // graph capture so that no device query happens while capturing).
 int cu_count();
@@ -190,15 +194,15 @@ hipError_t launch_finish_paths(int integrator, const SceneView& sc, const JobVie
 hipError_t launch_render_aov(const SceneView& sc, int w, int h, int spp, int max_bounces, uint32_t seed,
                              int mode, bool wide, float* out, unsigned long long* queries, hipStream_t st);
 
-// Coherence order of n rays (pt_order.hip): keys[0..n) and idx[0..n) are filled, then
-// idx[n..2n) receives the ray indices sorted by the Morton code of the origin in the box
-// (stable); keys and idx hold 2n entries, temp ray_order_temp_bytes(n) bytes.
+// Coherence order of n rays (pt_order.hip): the ray indices sorted by the Morton code of
+// the origin in the box (stable) end in *sorted, one of the two halves of idx; keys and
+// idx hold 2n entries, temp ray_order_temp_bytes(n) bytes.
 size_t ray_order_temp_bytes(int n);
 // dst[i * row + c] = src[idx[i] * row + c] for i < n (pt_order.hip): per-triangle rows put
 // in leaf-slot order.
 hipError_t launch_gather_rows(const float* src, const int32_t* idx, int n, int row, float* dst, hipStream_t st);
 hipError_t launch_ray_order(const float4* rays, int n, V3 box_lo, V3 box_hi, uint32_t* keys, int32_t* idx,
-                            void* temp, size_t temp_bytes, hipStream_t st);
+                            void* temp, size_t temp_bytes, hipStream_t st, int32_t** sorted);
 
 // The reference's 8-bit PNG bytes of num_pixels RGB float pixels (pt_post.hip).
 hipError_t launch_tonemap_rgb8(const float* rgb, uint8_t* out, int64_t num_pixels, hipStream_t st);

@@ -9,7 +9,8 @@
 // the trace kernel reads ray order[i] and writes its hit back at order[i]. Hits do not
 // depend on the order in which rays are traced.
 #include <hip/hip_runtime.h>
-#include <hipcub/device/device_radix_sort.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "pt_kernels.h"
 
@@ -49,15 +50,19 @@ __global__ void __launch_bounds__(256) k_ray_keys(const float4* __restrict__ ray
 
 size_t ray_order_temp_bytes(int n)
 {
+    // the sort ping-pongs between the two halves of keys and idx, so its own temporary
+    // storage holds only the digit histograms and the look-back state
     size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                             (const int32_t*)nullptr, (int32_t*)nullptr, n, 0, 30);
+    rocprim::double_buffer<uint32_t> k(nullptr, nullptr);
+    rocprim::double_buffer<int32_t> v(nullptr, nullptr);
+    if (rocprim::radix_sort_pairs(nullptr, bytes, k, v, n, 0, 30) != hipSuccess) return 0;
     return bytes;
 }
 
 hipError_t launch_ray_order(const float4* rays, int n, V3 box_lo, V3 box_hi, uint32_t* keys, int32_t* idx,
-                            void* temp, size_t temp_bytes, hipStream_t st)
+                            void* temp, size_t temp_bytes, hipStream_t st, int32_t** sorted)
 {
+    *sorted = idx;
     if (n <= 0) return hipSuccess;
     auto inv = [](float lo, float hi) { return hi > lo ? 1024.0f / (hi - lo) : 0.0f; };
     const float3 lo = make_float3(box_lo.x, box_lo.y, box_lo.z);
@@ -67,10 +72,24 @@ hipError_t launch_ray_order(const float4* rays, int n, V3 box_lo, V3 box_hi, uin
                        keys, idx);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // keys and indices sorted into their second halves (keys + n, idx + n)
+    // stable LSD radix sort over the 30 key bits; the sorted indices end in one of the two
+    // halves of idx, which the number of passes (known on the host) decides
     size_t bytes = temp_bytes;
-    return hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys, keys + n, idx, idx + n, n, 0, 30, st);
+    rocprim::double_buffer<uint32_t> k(keys, keys + n);
+    rocprim::double_buffer<int32_t> v(idx, idx + n);
+    e = rocprim::radix_sort_pairs(temp, bytes, k, v, n, 0, 30, st);
+    *sorted = v.current();
+    return e;
 }
+
+template <typename T>
+hipError_t exclusive_sum(void* temp, size_t& temp_bytes, const T* in, T* out, size_t n, hipStream_t st)
+{
+    return rocprim::exclusive_scan(temp, temp_bytes, in, out, T(0), n, rocprim::plus<T>(), st);
+}
+template hipError_t exclusive_sum<int>(void*, size_t&, const int*, int*, size_t, hipStream_t);
+template hipError_t exclusive_sum<unsigned long long>(void*, size_t&, const unsigned long long*, unsigned long long*,
+                                                      size_t, hipStream_t);
 
 namespace {
 

@@ -19,12 +19,12 @@
 // nodes are numbered level by level; the leaves and their triangle order are the
 // binary tree's.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cfloat>
 #include <vector>
 
+#include "pt_kernels.h"
 #include "pt_wide.h"
 
 namespace pt {
@@ -226,7 +226,7 @@ hipError_t gpu_collapse_bvh8(const float4* bnodes_d, int num_inner, uint32_t roo
     W_CHECK(hipMalloc((void**)&refs[1], cap * sizeof(uint32_t)));
     tmp.push_back(refs[1]);
     size_t scan_bytes = 0;
-    W_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, n_inner, base, (int)cap + 1, st));
+    W_CHECK(exclusive_sum(nullptr, scan_bytes, n_inner, base, (size_t)((int)cap + 1), st));
     void* scan_tmp;
     W_CHECK(hipMalloc(&scan_tmp, scan_bytes));
     tmp.push_back(scan_tmp);
@@ -240,7 +240,7 @@ hipError_t gpu_collapse_bvh8(const float4* bnodes_d, int num_inner, uint32_t roo
                            cand, n_cand, n_inner);
         const int zero = 0;
         W_CHECK(hipMemcpyAsync(n_inner + level_n, &zero, sizeof(int), hipMemcpyHostToDevice, st));
-        W_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, n_inner, base, level_n + 1, st));
+        W_CHECK(exclusive_sum(scan_tmp, scan_bytes, n_inner, base, (size_t)(level_n + 1), st));
         int next_n = 0;
         W_CHECK(hipMemcpyAsync(&next_n, base + level_n, sizeof(int), hipMemcpyDeviceToHost, st));
         hipLaunchKernelGGL(k_collapse_write, dim3(blocks(level_n)), dim3(256), 0, st, cand, n_cand, base, level_n,
