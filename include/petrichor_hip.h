@@ -175,6 +175,19 @@ int pt_trace_closest(pt_bvh* bvh, int64_t n, const float* rays, float* t_out, in
  * rays call the closest-hit query, Core/Integrator/PathTracing.cpp:293). */
 int pt_trace_any(pt_bvh* bvh, int64_t n, const float* rays, int32_t* hit_out, void* stream);
 
+/* Closest hit as pt_trace_closest, allocating nothing: no hipMalloc, hipFree or allocating
+ * library call, for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation). No counterpart in the reference. Its
+ * work cursors and, for launches of at least 2^20 rays, the coherence order's keys,
+ * indices and sort storage live in a fixed scratch array in each GPU's memory (about
+ * 36 MiB, held from the library's load on that GPU to the end of the process). n may be at
+ * most PT_NOALLOC_MAX_RAYS (PT_ERR_ARG above). Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (other streams, other contexts or threads)
+ * race and must be serialised by the caller. Hits equal pt_trace_closest's. */
+#define PT_NOALLOC_MAX_RAYS (1 << 21)
+int pt_trace_closest_noalloc(pt_bvh* bvh, int64_t n, const float* rays, float* t_out, int32_t* prim_out,
+                             void* stream);
+
 /* Closest hit as pt_trace_closest, synchronous, also returning the traversal work:
  * host_stats[0] = node slab tests, host_stats[1] = triangle tests, summed over rays. */
 int pt_trace_stats(pt_bvh* bvh, int64_t n, const float* rays, float* t_out, int32_t* prim_out,

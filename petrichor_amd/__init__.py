@@ -133,6 +133,7 @@ def lib():
     L.pt_bvh_build_host.argtypes = [_f, ctypes.c_int32, ctypes.POINTER(BvhDesc), _i, _f, _i, _i, _i, _i]
     L.pt_trace_closest.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp]
     L.pt_trace_any.argtypes = [vp, ctypes.c_int64, vp, vp, vp]
+    L.pt_trace_closest_noalloc.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp]
     L.pt_trace_stats.argtypes = [vp, ctypes.c_int64, vp, vp, vp, _l]
     L.pt_render.argtypes = [vp, vp, ctypes.POINTER(RenderDesc), vp, ctypes.c_int32, ctypes.POINTER(RenderStats)]
     L.pt_render_aov.argtypes = [vp, vp, ctypes.c_int32, ctypes.POINTER(RenderDesc), vp, ctypes.c_int32,
@@ -169,7 +170,7 @@ def lib():
 # every symbol include/petrichor_hip.h declares
 EXPORTED = ["pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_scene_create", "pt_scene_destroy",
             "pt_bvh_build", "pt_bvh_destroy", "pt_bvh_export_order", "pt_bvh_export_nodes", "pt_bvh_build_host", "pt_bvh_info", "pt_trace_closest",
-            "pt_trace_any", "pt_trace_stats", "pt_render", "pt_engine_create", "pt_engine_destroy",
+            "pt_trace_any", "pt_trace_closest_noalloc", "pt_trace_stats", "pt_render", "pt_engine_create", "pt_engine_destroy",
             "pt_render_batch", "pt_engine_set_tuning", "pt_engine_get_tuning", "pt_server_create", "pt_server_destroy", "pt_server_last_error",
             "pt_server_add_scene", "pt_server_remove_scene", "pt_server_scene_resident", "pt_server_submit",
             "pt_server_poll", "pt_server_wait", "pt_server_preview", "pt_server_fetch", "pt_server_get_stats",
@@ -450,6 +451,18 @@ class Bvh:
         st = ctypes.c_void_p(stream) if stream else None
         self.ctx.check(self.ctx._lib.pt_trace_closest(self.handle, n, rays.data_ptr(), t_out.data_ptr(),
                                                       prim_out.data_ptr(), st), "pt_trace_closest")
+        return t_out, prim_out
+
+    def trace_closest_noalloc(self, rays, t_out, prim_out, stream=None):
+        """pt_trace_closest_noalloc: as trace_closest on caller-given outputs, on the GPU's
+        fixed scratch (no allocation; at most 2^21 rays; overlapping calls on one GPU race)."""
+        import torch
+        n = rays.shape[0]
+        assert rays.is_cuda and rays.dtype == torch.float32 and rays.is_contiguous() and rays.shape[1] == 8
+        assert t_out.numel() >= n and prim_out.numel() >= n
+        st = ctypes.c_void_p(stream) if stream else None
+        self.ctx.check(self.ctx._lib.pt_trace_closest_noalloc(self.handle, n, rays.data_ptr(), t_out.data_ptr(),
+                                                              prim_out.data_ptr(), st), "pt_trace_closest_noalloc")
         return t_out, prim_out
 
     def trace_any(self, rays, hit_out=None, stream=None):

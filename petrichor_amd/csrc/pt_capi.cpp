@@ -244,6 +244,8 @@ int pt_ctx_create(int device, pt_context** out)
     c->device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);
+    // the device address of the allocation-free trace scratch, once per device
+    if (e == hipSuccess) e = noalloc_scratch_init(device);
     if (e != hipSuccess) {
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
@@ -647,9 +649,52 @@ int api_trace(pt_bvh* b, int64_t n, const float* rays, bool any, float* t_out, i
     return PT_OK;
 }
 
+// pt_trace_closest on the GPU's fixed scratch (pt_order.hip) instead of the BVH's buffers:
+// no allocation, and the ray count is written by a kernel rather than copied from the host.
+int api_trace_noalloc(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_t* k_out, hipStream_t st)
+{
+    pt_context* ctx = b->scene->ctx;
+    const NoallocScratch sc = noalloc_scratch(ctx->device);
+    if (!sc.cursor) return fail(ctx, PT_ERR_STATE, "the fixed trace scratch of this device is not set up");
+    TraceJob job;
+    job.seg[0] = api_segment(sc.cursor + PT_CURSOR_INTS, rays, false, t_out, k_out);
+    job.num_seg = 1;
+    job.views = nullptr;
+    job.job_of_path = nullptr;
+    job.queries = nullptr;
+    const int n32 = (int)n;
+    if (n >= kOrderMinRays && b->view.root_ref != PT_REF_EMPTY) {
+        if (ray_order_temp_bytes(n32) > sc.temp_bytes)
+            return fail(ctx, PT_ERR_UNSUPPORTED, "the sort storage of this ray count exceeds the fixed scratch");
+        int32_t* sorted = nullptr;
+        PT_HIP(ctx, launch_ray_order((const float4*)rays, n32, b->view.root_lo, b->view.root_hi, sc.keys, sc.idx,
+                                     sc.temp, sc.temp_bytes, st, &sorted));
+        job.seg[0].path = sorted;
+        job.seg[0].rays_by_path = 1;
+    }
+    PT_HIP(ctx, launch_zero(sc.cursor, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
+    PT_HIP(ctx, launch_set(sc.cursor + PT_CURSOR_INTS, n32, st));
+    PT_HIP(ctx, launch_trace_job(b->view, job, sc.cursor, nullptr, st));
+    return PT_OK;
+}
+
 } // namespace
 
 extern "C" {
+
+int pt_trace_closest_noalloc(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_t* prim_out, void* stream)
+{
+    if (!b) return PT_ERR_ARG;
+    pt_context* ctx = b->scene->ctx;
+    if (n < 0 || (n > 0 && (!rays || !t_out || !prim_out))) return fail(ctx, PT_ERR_ARG, "bad trace arguments");
+    if (n > PT_NOALLOC_MAX_RAYS)
+        return fail(ctx, PT_ERR_ARG, "pt_trace_closest_noalloc takes at most PT_NOALLOC_MAX_RAYS (2^21) rays, got " +
+                                         std::to_string(n));
+    if (n == 0) return PT_OK;
+    PT_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    return api_trace_noalloc(b, n, rays, t_out, prim_out, st);
+}
 
 int pt_trace_closest(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_t* prim_out, void* stream)
 {

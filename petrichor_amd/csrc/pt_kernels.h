@@ -138,6 +138,8 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job, int* cursor, 
                             hipStream_t st, const LaunchTuning& tune = LaunchTuning());
 // Zeroes counters[i * stride] for i < n (n <= 64).
 hipError_t launch_zero(int* counters, int n, hipStream_t st, int stride = 1);
+// *p = value, by a kernel (no host copy).
+hipError_t launch_set(int* p, int value, hipStream_t st);
 // Exclusive prefix sum of n integers (rocPRIM); call with temp == nullptr to size temp.
 template <typename T>
 hipError_t exclusive_sum(void* temp, size_t& temp_bytes, const T* in, T* out, size_t n, hipStream_t st);
@@ -203,6 +205,21 @@ size_t ray_order_temp_bytes(int n);
 hipError_t launch_gather_rows(const float* src, const int32_t* idx, int n, int row, float* dst, hipStream_t st);
 hipError_t launch_ray_order(const float4* rays, int n, V3 box_lo, V3 box_hi, uint32_t* keys, int32_t* idx,
                             void* temp, size_t temp_bytes, hipStream_t st, int32_t** sorted);
+
+// The fixed scratch of the allocation-free trace entry point (pt_order.hip): a static
+// device array per GPU, addressed once per device by noalloc_scratch_init (with
+// hipGetSymbolAddress; the array's name is not a device address in host code).
+#define PT_MAX_DEVICES 64
+struct NoallocScratch {
+    int* cursor;   // PT_CURSOR_INTS work cursors, then the ray count
+    uint32_t* keys; // 2 x PT_NOALLOC_MAX_RAYS
+    int32_t* idx;   // 2 x PT_NOALLOC_MAX_RAYS
+    void* temp;     // the radix sort's temporary storage
+    size_t temp_bytes;
+};
+hipError_t noalloc_scratch_init(int device);
+// Null members until noalloc_scratch_init succeeded for the device.
+NoallocScratch noalloc_scratch(int device);
 
 // The reference's 8-bit PNG bytes of num_pixels RGB float pixels (pt_post.hip).
 hipError_t launch_tonemap_rgb8(const float* rgb, uint8_t* out, int64_t num_pixels, hipStream_t st);

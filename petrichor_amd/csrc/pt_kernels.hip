@@ -784,6 +784,11 @@ __global__ void k_zero_counters(int* c, int n, int stride)
     if (i < n) c[i * stride] = 0;
 }
 
+__global__ void k_set_int(int* p, int v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
 // ---------------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------------
@@ -793,6 +798,12 @@ static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + b
 hipError_t launch_zero(int* counters, int n, hipStream_t st, int stride)
 {
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, counters, n, stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_set(int* p, int value, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, p, value);
     return hipGetLastError();
 }
 

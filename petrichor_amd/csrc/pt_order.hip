@@ -8,9 +8,15 @@
 // radix sort, so rays from one point, such as a pinhole camera's, keep their order), and
 // the trace kernel reads ray order[i] and writes its hit back at order[i]. Hits do not
 // depend on the order in which rays are traced.
+//
+// The file also holds the fixed device scratch of the allocation-free trace entry point
+// (pt_trace_closest_noalloc): its work cursors and the sort buffers of at most
+// PT_NOALLOC_MAX_RAYS rays live in one static device array per GPU.
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+
+#include <mutex>
 
 #include "pt_kernels.h"
 
@@ -90,6 +96,57 @@ hipError_t exclusive_sum(void* temp, size_t& temp_bytes, const T* in, T* out, si
 template hipError_t exclusive_sum<int>(void*, size_t&, const int*, int*, size_t, hipStream_t);
 template hipError_t exclusive_sum<unsigned long long>(void*, size_t&, const unsigned long long*, unsigned long long*,
                                                       size_t, hipStream_t);
+
+namespace {
+
+// Scratch of pt_trace_closest_noalloc, one copy per GPU (the loader places the array in
+// each device's memory when the code object is loaded there, and it stays until the
+// process ends). Layout: the work cursors and the ray count (PT_CURSOR_INTS + 1 ints,
+// padded to 4 KiB), then the keys and indices of the coherence order (2 x 2 x
+// PT_NOALLOC_MAX_RAYS words), then the radix sort's temporary storage.
+constexpr size_t kScratchHead = 4096;
+constexpr size_t kScratchSortTemp = (size_t)4 << 20;
+constexpr size_t kScratchBytes = kScratchHead + 16 * (size_t)PT_NOALLOC_MAX_RAYS + kScratchSortTemp;
+static_assert((PT_CURSOR_INTS + 1) * sizeof(int) <= kScratchHead, "cursor block");
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[PT_MAX_DEVICES] = {};
+
+} // namespace
+
+hipError_t noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= PT_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+    return hipSuccess;
+}
+
+NoallocScratch noalloc_scratch(int device)
+{
+    NoallocScratch s{};
+    if (device < 0 || device >= PT_MAX_DEVICES) return s;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    unsigned char* base = (unsigned char*)g_scratch_ptr[device];
+    if (!base) return s;
+    s.cursor = (int*)base;
+    s.keys = (uint32_t*)(base + kScratchHead);
+    s.idx = (int32_t*)(base + kScratchHead + 8 * (size_t)PT_NOALLOC_MAX_RAYS);
+    s.temp = base + kScratchHead + 16 * (size_t)PT_NOALLOC_MAX_RAYS;
+    s.temp_bytes = kScratchSortTemp;
+    return s;
+}
 
 namespace {
 
