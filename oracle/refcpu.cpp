@@ -1643,6 +1643,36 @@ float oracle_env_pdf(void* s, const float* dir)
     return ((Scene*)s)->env.pdf(mk(dir[0], dir[1], dir[2]));
 }
 
+void oracle_libm(int fn, int64_t n, const double* a, const double* b, double* out)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        switch (fn) {
+        case 0: out[i] = pt_sin(a[i]); break;
+        case 1: out[i] = pt_cos(a[i]); break;
+        case 2: out[i] = pt_acos(a[i]); break;
+        default: out[i] = pt_atan2(a[i], b[i]); break;
+        }
+    }
+}
+
+int oracle_sample_light(void* s, float rnd, uint32_t seed0, uint32_t seed1, int want_env, float* out8)
+{
+    Ctx c;
+    c.sc = (Scene*)s;
+    c.bvh = nullptr;
+    Sampler2D s2(seed0, seed1);
+    float pdf_area = -1.0f;
+    V pos = mk(0.0f, 0.0f, 0.0f), nrm = pos;
+    const bool area = sample_light(c, pos, rnd, s2, &pdf_area, want_env != 0, &pos, &nrm);
+    const float o[8] = {pos.x, pos.y, pos.z, nrm.x, nrm.y, nrm.z, pdf_area, (float)s2.draws};
+    std::memcpy(out8, o, sizeof o);
+    if (!area) return -1;
+    // the slot the reference's index arithmetic picks (PathTracing.cpp:339-362)
+    const size_t L = c.sc->lights.size();
+    int index = want_env ? std::min((int)(rnd * (L + 1)), (int)L) : std::min((int)(rnd * L), (int)(L - 1));
+    return index;
+}
+
 void oracle_onb(const float* w_in, float theta, float phi, float* out12)
 {
     // OrthonormalBasis::Build(w) then GetDir(theta, phi) (OrthonormalBasis.cpp:10-43)

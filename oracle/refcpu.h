@@ -101,6 +101,15 @@ void oracle_triangle_sample(const float* tri9, uint32_t seed0, uint32_t seed1, f
 void oracle_env_color(void* scene, const float* dir, float* rgb);
 void oracle_env_sample(void* scene, uint32_t seed0, uint32_t seed1, float* out4);
 float oracle_env_pdf(void* scene, const float* dir);
+/* The double-precision libm routines the oracle and the device share (pt_libm.h):
+ * fn 0 sin(a), 1 cos(a), 2 acos(a), 3 atan2(a, b), n values. */
+void oracle_libm(int fn, int64_t n, const double* a, const double* b, double* out);
+/* PathTracing::SampleLight (PathTracing.cpp:329-386) with RandomSampler2D(seed0, seed1)
+ * and the random value rnd; want_env != 0 for the call of CalcLightContribution (an extra
+ * environment slot), 0 for the MIS weight of an emitter hit. Returns the slot picked (a
+ * light index) or -1 for the environment slot; out: pos[3], normal[3], pdf_area (-1 when
+ * not written), then the number of 2D draws consumed. */
+int oracle_sample_light(void* scene, float rnd, uint32_t seed0, uint32_t seed1, int want_env, float* out8);
 /* OrthonormalBasis::Build(w) + GetDir(theta, phi): out u[3] v[3] w[3] dir[3]. */
 void oracle_onb(const float* w, float theta, float phi, float* out12);
 /* Texture2D::Save's PNG bytes of num_pixels RGB pixels (Core/Texture2D.cpp:199-215), on

@@ -53,7 +53,8 @@ PT_HD double pt_kernel_sin(double r)
     p = p * z + -0x1.a01a01a01a01ap-13;
     p = p * z + 0x1.1111111111111p-7;
     p = p * z + -0x1.5555555555555p-3;
-    return r + (r * z) * p;
+    // sin(-0) = -0: r + (-0 * z) * p would round to +0
+    return r == 0.0 ? r : r + (r * z) * p;
 }
 
 // cos(r) for |r| <= pi/4: 1 + z*Q(z), Q from (-1)^k/(2k)!, k = 1..10.

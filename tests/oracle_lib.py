@@ -67,6 +67,10 @@ def lib():
     L.oracle_env_pdf.restype = ctypes.c_float
     L.oracle_env_pdf.argtypes = [_v, _f]
     L.oracle_onb.argtypes = [_f, ctypes.c_float, ctypes.c_float, _f]
+    _d = ctypes.POINTER(ctypes.c_double)
+    L.oracle_libm.argtypes = [ctypes.c_int, ctypes.c_int64, _d, _d, _d]
+    L.oracle_sample_light.restype = ctypes.c_int
+    L.oracle_sample_light.argtypes = [_v, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _f]
     L.oracle_tonemap_rgb8.argtypes = [ctypes.c_int64, _f, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int]
     _lib = L
     return L
@@ -194,6 +198,13 @@ class OracleScene:
     def env_pdf(self, d):
         return lib().oracle_env_pdf(self.ptr, _ptr(f32(d), _f))
 
+    def sample_light(self, rnd, s0, s1, want_env=True):
+        """PathTracing::SampleLight: (slot or -1 for the environment slot, pos, normal,
+        pdf_area, 2D draws consumed)."""
+        out = np.zeros(8, np.float32)
+        k = lib().oracle_sample_light(self.ptr, rnd, s0, s1, 1 if want_env else 0, _ptr(out, _f))
+        return k, out[0:3], out[3:6], out[6], int(out[7])
+
     def __del__(self):
         try:
             L = lib()
@@ -244,6 +255,16 @@ def triangle_intersect(tri9, ray6):
 def triangle_sample(tri9, s0, s1):
     out = np.zeros(7, np.float32)
     lib().oracle_triangle_sample(_ptr(f32(tri9), _f), s0, s1, _ptr(out, _f))
+    return out
+
+
+def libm(fn, a, b=None):
+    """The oracle's double-precision sin (0), cos (1), acos (2) or atan2(a, b) (3)."""
+    _d = ctypes.POINTER(ctypes.c_double)
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b if b is not None else a, np.float64)
+    out = np.zeros(len(a), np.float64)
+    lib().oracle_libm(fn, len(a), a.ctypes.data_as(_d), b.ctypes.data_as(_d), out.ctypes.data_as(_d))
     return out
 
 
