@@ -110,47 +110,6 @@ def cpu_baseline(sc, tree, width, height, spp, depth, seconds):
     }, work, pix_all, np.concatenate(rgbs)
 
 
-def torch_primary_rays(cam, width, height, gen):
-    """Pinhole rays through jittered pixel positions, built on the device (8 floats per ray)."""
-    import torch
-    dev = "cuda"
-    pos, right, up, fwd = (torch.tensor(cam[a:a + 3], dtype=torch.float64, device=dev) for a in (0, 3, 6, 9))
-    focus, hp = float(cam[12]), float(cam[13])
-    n = width * height
-    idx = torch.arange(n, device=dev)
-    i, j = (idx % width).double(), (idx // width).double()
-    u = (i + torch.rand(n, generator=gen, device=dev, dtype=torch.float64)) / width - 0.5
-    v = (j + torch.rand(n, generator=gen, device=dev, dtype=torch.float64)) / height - 0.5
-    d = (right[None] * (u * hp * focus * width / height)[:, None] - up[None] * (v * hp * focus)[:, None]
-         + fwd[None] * focus)
-    d = d / d.norm(dim=1, keepdim=True)
-    r = torch.zeros((n, 8), dtype=torch.float32, device=dev)
-    r[:, 0:3] = pos.float()
-    r[:, 4:7] = d.float()
-    r[:, 7] = float(np.finfo(np.float32).max)
-    return r
-
-
-def torch_surface_rays(tri_pos_dev, n, gen, dist_min):
-    """Rays from uniform points on random triangles with uniform directions on the sphere."""
-    import torch
-    dev = tri_pos_dev.device
-    k = torch.randint(0, tri_pos_dev.shape[0], (n,), generator=gen, device=dev)
-    t = tri_pos_dev[k].view(n, 3, 3).double()
-    u = torch.rand(n, generator=gen, device=dev, dtype=torch.float64)
-    v = torch.rand(n, generator=gen, device=dev, dtype=torch.float64)
-    su = u.sqrt()
-    p = t[:, 0] * (1 - su)[:, None] + t[:, 1] * (su * (1 - v))[:, None] + t[:, 2] * (su * v)[:, None]
-    d = torch.randn((n, 3), generator=gen, device=dev, dtype=torch.float64)
-    d = d / d.norm(dim=1, keepdim=True)
-    r = torch.zeros((n, 8), dtype=torch.float32, device=dev)
-    r[:, 0:3] = p.float()
-    r[:, 3] = dist_min
-    r[:, 4:7] = d.float()
-    r[:, 7] = float(np.finfo(np.float32).max)
-    return r
-
-
 def cpu_trace_baseline(sc, tree, rays_host, seconds):
     """The CPU oracle's closest hit (restatement of BinnedSAHBVH::Intersect) on a sample of
     the same rays through the same tree. ctypes releases the GIL during the call, so the
@@ -215,9 +174,9 @@ def trace_bench(args):
     w = 1 << ((args.log2_rays - 1) // 2 + 1)
     h = (nr // 2) // w
     rays = torch.empty((nr, 8), dtype=torch.float32, device="cuda")
-    rays[: nr // 2] = torch_primary_rays(sc["camera"], w, h, gen)
+    rays[: nr // 2] = scenes.primary_rays_device(sc["camera"], w, h, gen)
     tri_dev = torch.from_numpy(sc["tri_pos"]).cuda()
-    rays[nr // 2:] = torch_surface_rays(tri_dev, nr - nr // 2, gen, 1e-5)
+    rays[nr // 2:] = scenes.surface_rays_device(tri_dev, nr - nr // 2, gen, 1e-5)
     del tri_dev
     t_out = torch.empty(nr, dtype=torch.float32, device="cuda")
     p_out = torch.empty(nr, dtype=torch.int32, device="cuda")
@@ -297,23 +256,6 @@ def trace_bench(args):
     print(json.dumps(res), flush=True)
 
 
-RESOLUTIONS = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
-
-
-def job_specs(n, rng, tris_max, spp_max):
-    """Render jobs of the batched and served workloads: triangles log-uniform in
-    [1e3, tris_max], resolution from RESOLUTIONS, spp log-uniform in [4, spp_max],
-    max depth uniform in 8..12."""
-    out = []
-    for _ in range(n):
-        tris = int(round(np.exp(rng.uniform(np.log(1e3), np.log(tris_max)))))
-        w, h = RESOLUTIONS[rng.integers(0, len(RESOLUTIONS))]
-        spp = int(round(np.exp(rng.uniform(np.log(4), np.log(spp_max)))))
-        depth = int(rng.integers(8, 13))
-        out.append({"tris": tris, "width": w, "height": h, "spp": spp, "depth": depth})
-    return out
-
-
 def batch_bench(args):
     """Batched form: jobs of uneven scene size, resolution and spp in one graph-replayed
     multi-job engine (autotuned), against the same jobs rendered one by one with plain
@@ -322,7 +264,7 @@ def batch_bench(args):
     import petrichor_amd as P
     from petrichor_amd import autotune, scenes
     rng = np.random.default_rng(2024)
-    specs = job_specs(args.jobs, rng, args.tris_max, args.spp_max)
+    specs = scenes.job_specs(args.jobs, rng, args.tris_max, args.spp_max)
     env = scenes.sky_env()
     ctx = P.Context(0)
     t0 = time.perf_counter()
@@ -416,7 +358,7 @@ def serve_bench(args):
     from petrichor_amd import scenes
     rng = np.random.default_rng(99)
     n_req = int(rng.poisson(args.rate * args.duration))
-    specs = job_specs(n_req, rng, args.tris_max, args.spp_max)
+    specs = scenes.job_specs(n_req, rng, args.tris_max, args.spp_max)
     arrivals = np.sort(rng.uniform(0, args.duration, n_req))
     reuse = rng.uniform(0, 1, n_req) < 0.25
     env = scenes.sky_env()

@@ -314,3 +314,62 @@ def surface_rays(tri_pos, n, seed=5):
     r[:, 4:7] = d
     r[:, 7] = np.finfo(np.float32).max
     return r
+
+
+# ---- ray sets and job mixes of the benchmark configurations --------------------------------
+def primary_rays_device(cam, width, height, gen):
+    """Pinhole rays through jittered pixel positions, built on the device (8 floats per ray)."""
+    import torch
+    dev = "cuda"
+    pos, right, up, fwd = (torch.tensor(cam[a:a + 3], dtype=torch.float64, device=dev) for a in (0, 3, 6, 9))
+    focus, hp = float(cam[12]), float(cam[13])
+    n = width * height
+    idx = torch.arange(n, device=dev)
+    i, j = (idx % width).double(), (idx // width).double()
+    u = (i + torch.rand(n, generator=gen, device=dev, dtype=torch.float64)) / width - 0.5
+    v = (j + torch.rand(n, generator=gen, device=dev, dtype=torch.float64)) / height - 0.5
+    d = (right[None] * (u * hp * focus * width / height)[:, None] - up[None] * (v * hp * focus)[:, None]
+         + fwd[None] * focus)
+    d = d / d.norm(dim=1, keepdim=True)
+    r = torch.zeros((n, 8), dtype=torch.float32, device=dev)
+    r[:, 0:3] = pos.float()
+    r[:, 4:7] = d.float()
+    r[:, 7] = float(np.finfo(np.float32).max)
+    return r
+
+
+def surface_rays_device(tri_pos_dev, n, gen, dist_min):
+    """Rays from uniform points on random triangles with uniform directions on the sphere."""
+    import torch
+    dev = tri_pos_dev.device
+    k = torch.randint(0, tri_pos_dev.shape[0], (n,), generator=gen, device=dev)
+    t = tri_pos_dev[k].view(n, 3, 3).double()
+    u = torch.rand(n, generator=gen, device=dev, dtype=torch.float64)
+    v = torch.rand(n, generator=gen, device=dev, dtype=torch.float64)
+    su = u.sqrt()
+    p = t[:, 0] * (1 - su)[:, None] + t[:, 1] * (su * (1 - v))[:, None] + t[:, 2] * (su * v)[:, None]
+    d = torch.randn((n, 3), generator=gen, device=dev, dtype=torch.float64)
+    d = d / d.norm(dim=1, keepdim=True)
+    r = torch.zeros((n, 8), dtype=torch.float32, device=dev)
+    r[:, 0:3] = p.float()
+    r[:, 3] = dist_min
+    r[:, 4:7] = d.float()
+    r[:, 7] = float(np.finfo(np.float32).max)
+    return r
+
+
+RESOLUTIONS = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+
+
+def job_specs(n, rng, tris_max, spp_max):
+    """Render jobs of the batched and served workloads: triangles log-uniform in
+    [1e3, tris_max], resolution from RESOLUTIONS, spp log-uniform in [4, spp_max],
+    max depth uniform in 8..12."""
+    out = []
+    for _ in range(n):
+        tris = int(round(np.exp(rng.uniform(np.log(1e3), np.log(tris_max)))))
+        w, h = RESOLUTIONS[rng.integers(0, len(RESOLUTIONS))]
+        spp = int(round(np.exp(rng.uniform(np.log(4), np.log(spp_max)))))
+        depth = int(rng.integers(8, 13))
+        out.append({"tris": tris, "width": w, "height": h, "spp": spp, "depth": depth})
+    return out

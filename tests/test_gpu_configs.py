@@ -1,0 +1,209 @@
+"""Every benchmark configuration at its full size on the GPU, checked against the oracle:
+
+C1  Cornell box, 32 triangles, 512 x 512, 16 spp, depth 8, seed 1: every pixel, both
+    integrators, on the GPU's tree and on the oracle's own tree.
+C2  the single large scene (3,145,728 triangles, 2048 x 1024 sky), 1920 x 1080, 64 spp,
+    depth 12: the full frame on the GPU, scattered crops (the sun's among them) against the
+    oracle on the GPU's tree and on the oracle's own tree (std::sort leaf order).
+C3  10M random triangles, 32 bins, binary and 8-wide trees, 2^26 rays (half pinhole, half
+    surface rays): closest and any hit on every ray, a 1M-ray sample against the oracle.
+C4  the 64-job batch: every batched image equals its per-job render, and the oracle on a
+    pixel sample of every job.
+C5  served requests from the batched distribution at 20 requests/s (25 % on a resident
+    scene, 200 GB budget) over a shortened arrival window: every result equals its
+    per-job render.
+"""
+import functools
+import time
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import petrichor_amd as P
+from gpu_util import bits_equal, ctx, log, rel_err
+from petrichor_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_on(bvh, sc=None):
+    return O.OracleScene(sc if sc is not None else bvh.scene.src).use_tree(
+        bvh.export_nodes() | {"prim_ids": bvh.export_order()})
+
+
+@pytest.mark.parametrize("integrator", [P.INTEGRATOR_MIS, P.INTEGRATOR_SIMPLE])
+def test_c1_cornell_every_pixel(gpu, integrator):
+    sc = scenes.cornell_box()
+    b = ctx().scene(sc).build_bvh(16)
+    W = H = 512
+    log("C1 cornell 512x512 16 spp depth 8, integrator %d" % integrator)
+    img, st = b.render(W, H, 16, 8, integrator, 1)
+    ref, q = oracle_on(b, sc).render(W, H, 16, 8, integrator, 1, 0)
+    assert st["queries"] == q
+    assert rel_err(img, ref).max() <= 1e-4
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+    # the oracle's own tree (the reference's std::sort leaf order)
+    own = O.OracleScene(sc).build_bvh(16)
+    ref2, _ = own.render(W, H, 16, 8, integrator, 1, 0)
+    over = int((rel_err(img, ref2).max(axis=2) > 1e-4).sum())
+    log("C1 pixels over 1e-4 against the oracle's own tree:", over)
+    assert over <= W * H // 1000
+
+
+def crops(img, size=48):
+    """Pixel indices of five crops: around the brightest pixel (the sun or its glints), the
+    centre, two corners and the middle of the bottom edge."""
+    H, W, _ = img.shape
+    lum = img.sum(axis=2)
+    by, bx = np.unravel_index(int(np.argmax(lum)), lum.shape)
+    centres = [(by, bx), (H // 2, W // 2), (size // 2, size // 2), (H - size // 2, W - size // 2), (H - 30, W // 3)]
+    pix = set()
+    for cy, cx in centres:
+        y0 = int(np.clip(cy - size // 2, 0, H - size))
+        x0 = int(np.clip(cx - size // 2, 0, W - size))
+        for y in range(y0, y0 + size):
+            pix.update(range(y * W + x0, y * W + x0 + size))
+    return np.array(sorted(pix), np.int32), (by, bx)
+
+
+def test_c2_single_large_scene_crops(gpu):
+    sc = scenes.spheres_scene(seed=2)
+    assert len(sc["tri_mat"]) == 3145728 and sc["env_rgb"].shape == (1024, 2048, 3)
+    b = ctx().scene(sc).build_bvh(16)
+    W, H, SPP, D = 1920, 1080, 64, 12
+    log("C2 full frame")
+    img, st = b.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, two_streams=True)
+    assert st["queries"] > 3e8
+    pix, bright = crops(img)
+    log("C2 oracle on %d crop pixels (brightest pixel %s)" % (len(pix), bright))
+    got = img.reshape(-1, 3)[pix]
+    ref, _ = oracle_on(b, sc).render(W, H, SPP, D, 0, 1, 0, pixels=pix)
+    assert rel_err(got, ref).max() <= 1e-4
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    # the oracle's own tree: same nodes, leaf order among equal centroid keys may differ
+    own = O.OracleScene(sc).build_bvh(16)
+    t_gpu, t_own = b.export_nodes(), own.bvh_export()
+    for k in ("boxes", "child", "range", "leaf"):
+        assert np.array_equal(t_gpu[k], t_own[k]), k
+    slots_differ = int((b.export_order() != t_own["prim_ids"]).sum())
+    ref2, _ = own.render(W, H, SPP, D, 0, 1, 0, pixels=pix)
+    over = int((rel_err(got, ref2).max(axis=1) > 1e-4).sum())
+    log("C2 leaf slots ordered differently from the oracle's own tree: %d of %d; crop pixels over 1e-4: %d of %d"
+        % (slots_differ, len(sc["tri_mat"]), over, len(pix)))
+    assert over <= len(pix) // 100
+
+
+@functools.lru_cache(maxsize=1)
+def c3_scene():
+    return scenes.random_triangles(10_000_000, seed=3, mu=-6.0, sigma=1.0)
+
+
+@pytest.mark.parametrize("width", [2, 8])
+def test_c3_microbenchmark_10M(gpu, width):
+    import torch
+    sc = c3_scene()
+    b = ctx().scene(sc).build_bvh(32, 4, width=width)
+    info = b.info()
+    log("C3 width %d: %d nodes, depth %d, build %.1f ms" % (width, info["num_nodes"], info["max_depth"],
+                                                            info["build_ms"]))
+    nr = 1 << 26
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    w = 1 << 13
+    rays = torch.empty((nr, 8), dtype=torch.float32, device="cuda")
+    rays[: nr // 2] = scenes.primary_rays_device(sc["camera"], w, (nr // 2) // w, gen)
+    tri = torch.from_numpy(sc["tri_pos"]).cuda()
+    rays[nr // 2:] = scenes.surface_rays_device(tri, nr - nr // 2, gen, 1e-5)
+    del tri
+    t, p = b.trace_closest(rays)
+    hit = b.trace_any(rays)
+    torch.cuda.synchronize()
+    assert torch.equal(hit, (p >= 0).int())
+    frac = (p >= 0).float().mean().item()
+    log("C3 hit fraction %.4f" % frac)
+    assert 0.3 < frac < 1.0
+    sel = torch.randperm(nr, generator=torch.Generator().manual_seed(11))[: 1 << 20].cuda()
+    sample = rays[sel].cpu().numpy()
+    to, po, _ = oracle_on(b, sc).trace_closest(sample)
+    pg, tg = p[sel].cpu().numpy(), t[sel].cpu().numpy()
+    same = int(((pg == po) & (tg.view(np.uint32) == to.view(np.uint32))).sum())
+    log("C3 width %d: %d of %d sampled rays equal the oracle" % (width, same, len(sel)))
+    assert same == len(sel)
+
+
+def test_c4_batch_of_64_jobs(gpu):
+    rng = np.random.default_rng(2024)
+    specs = scenes.job_specs(64, rng, 2e6, 256)
+    env = scenes.sky_env()
+    c = ctx()
+    bvhs = []
+    for i, sp in enumerate(specs):
+        bvhs.append(c.scene(scenes.scaled_scene(sp["tris"], seed=1000 + i, env=env)).build_bvh(16))
+    jobs = [P.Job(bv, sp["width"], sp["height"], sp["spp"], sp["depth"], P.INTEGRATOR_MIS, 7 + i)
+            for i, (bv, sp) in enumerate(zip(bvhs, specs))]
+    eng = P.Engine(c, 1 << 23, max_jobs=64)
+    t0 = time.perf_counter()
+    st = eng.render_batch(jobs)
+    log("C4 batch: %.2f s, %d queries" % (time.perf_counter() - t0, st["queries"]))
+    q = 0
+    prng = np.random.default_rng(5)
+    for i, (j, sp) in enumerate(zip(jobs, specs)):
+        one, s1 = j.bvh.render(sp["width"], sp["height"], sp["spp"], sp["depth"], P.INTEGRATOR_MIS, 7 + i)
+        q += s1["queries"]
+        assert bits_equal(j.result, one), i
+        pix = np.sort(prng.choice(sp["width"] * sp["height"], 32, replace=False)).astype(np.int32)
+        ref, _ = oracle_on(j.bvh).render(sp["width"], sp["height"], sp["spp"], sp["depth"], 0, 7 + i, 0,
+                                         pixels=pix)
+        assert rel_err(j.result.reshape(-1, 3)[pix], ref).max() <= 1e-4, i
+    assert q == st["queries"]
+
+
+def test_c5_served_poisson_stream(gpu):
+    rate, duration = 20.0, 6.0
+    rng = np.random.default_rng(99)
+    n = int(rng.poisson(rate * duration))
+    specs = scenes.job_specs(n, rng, 2e6, 256)
+    arrivals = np.sort(rng.uniform(0, duration, n))
+    reuse = rng.uniform(0, 1, n) < 0.25
+    env = scenes.sky_env()
+    pool_n = min(n, 24)
+    geo = [scenes.scaled_scene(specs[i]["tris"], seed=5000 + i, env=env) for i in range(pool_n)]
+    srv = P.Server(200 << 30, max_paths=1 << 23, max_jobs=128, iterations_per_sync=8)
+    scene_geo, live, reqs = {}, [], []
+    log("C5 %d requests over %.0f s" % (n, duration))
+    t0 = time.perf_counter()
+    for i in range(n):
+        dt = arrivals[i] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        if reuse[i] and live:
+            sid = live[int(rng.integers(0, len(live)))]
+        else:
+            sid = srv.add_scene(geo[i % pool_n])
+            scene_geo[sid] = i % pool_n
+            live.append(sid)
+        sp = specs[i]
+        reqs.append((srv.submit(sid, sp["width"], sp["height"], sp["spp"], sp["depth"], P.INTEGRATOR_MIS, 11 + i),
+                     sid, i))
+    lat = []
+    imgs = {}
+    for rid, sid, i in reqs:
+        info = srv.wait(rid, 600000)
+        assert info["state"] == P.REQ_DONE, info
+        lat.append(info["latency_ms"])
+        imgs[rid] = srv.fetch(rid)
+    st = srv.stats()
+    wall = time.perf_counter() - t0
+    srv.close()
+    log("C5 done in %.1f s: %.2f jobs/s, p50 %.0f ms, p99 %.0f ms, evicted %d" % (
+        wall, n / wall, np.percentile(lat, 50), np.percentile(lat, 99), st["evicted_scenes"]))
+    c = ctx()
+    plain_bvh = {}
+    for rid, sid, i in reqs:
+        g = scene_geo[sid]
+        if g not in plain_bvh:
+            plain_bvh[g] = c.scene(geo[g]).build_bvh(16)
+        sp = specs[i]
+        one, _ = plain_bvh[g].render(sp["width"], sp["height"], sp["spp"], sp["depth"], P.INTEGRATOR_MIS, 11 + i)
+        assert bits_equal(imgs[rid], one), (rid, i)
