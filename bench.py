@@ -256,10 +256,11 @@ def trace_bench(args):
     print(json.dumps(res), flush=True)
 
 
-def batch_bench(args):
+def run_batch(args):
     """Batched form: jobs of uneven scene size, resolution and spp in one graph-replayed
     multi-job engine (autotuned), against the same jobs rendered one by one with plain
-    launches (one synchronisation per iteration) and with the single-job graph path."""
+    launches (one synchronisation per iteration) and with the single-job graph path.
+    Returns the result line's dict."""
     import torch
     import petrichor_amd as P
     from petrichor_amd import autotune, scenes
@@ -269,7 +270,8 @@ def batch_bench(args):
     ctx = P.Context(0)
     t0 = time.perf_counter()
     # the autotuner picks the node width unless --bvh-width fixes it
-    widths = (args.bvh_width,) if args.bvh_width else (2, 8)
+    # the binary walk is the reference's; the 8-wide one only on request (--bvh-width 8)
+    widths = (args.bvh_width,) if args.bvh_width else (2,)
     bvhs_by_width = {w: [] for w in widths}
     for i, sp in enumerate(specs):
         sc = scenes.scaled_scene(sp["tris"], seed=1000 + i, env=env)
@@ -277,7 +279,7 @@ def batch_bench(args):
         s_ = ctx.scene(sc)
         for w in bvhs_by_width:
             bvhs_by_width[w].append(s_.build_bvh(16, width=w))
-    log("%d scenes (%d triangles) generated, uploaded and built (both widths) in %.1f s"
+    log("%d scenes (%d triangles) generated, uploaded and built in %.1f s"
         % (len(specs), sum(sp["tris"] for sp in specs), time.perf_counter() - t0))
 
     def outs():
@@ -288,7 +290,8 @@ def batch_bench(args):
                 for i, (b, sp) in enumerate(zip(bvhs, specs))]
 
     out_b = outs()
-    best = autotune.tune(ctx, None, log=log, by_width={w: jobs_with(out_b, bvhs_by_width[w]) for w in widths})
+    best = autotune.tune(ctx, None, log=log, by_width={w: jobs_with(out_b, bvhs_by_width[w]) for w in widths},
+                         exact_only=widths == (2,))
     jobs = best["jobs"]
     bvhs = bvhs_by_width[best["width"]]
     eng = P.Engine(ctx, best["max_paths"], max_jobs=len(jobs))
@@ -304,20 +307,41 @@ def batch_bench(args):
     out_p = outs()
     t0 = time.perf_counter()
     q_plain = 0
+    per_plain, per_q = [], []
     for i, (b, sp) in enumerate(zip(bvhs, specs)):
+        t1 = time.perf_counter()
         _, s1 = b.render(sp["width"], sp["height"], sp["spp"], sp["depth"], P.INTEGRATOR_MIS, 7 + i, out=out_p[i],
                          plain=True)
+        torch.cuda.synchronize()
+        per_plain.append(time.perf_counter() - t1)
+        per_q.append(s1["queries"])
         q_plain += s1["queries"]
-    torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
     log("plain launches, one job at a time: %.2f s" % t_plain)
     out_g = outs()
     t0 = time.perf_counter()
+    per_graph = []
     for i, (b, sp) in enumerate(zip(bvhs, specs)):
+        t1 = time.perf_counter()
         b.render(sp["width"], sp["height"], sp["spp"], sp["depth"], P.INTEGRATOR_MIS, 7 + i, out=out_g[i])
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        per_graph.append(time.perf_counter() - t1)
     t_graph = time.perf_counter() - t0
     log("single-job graph path, one job at a time: %.2f s" % t_graph)
+    # Amdahl bound of the mix: the largest jobs fill the GPU on their own, so batching gains
+    # little on them; if every other job cost nothing, batching could not beat
+    # plain_total / (time of the largest jobs alone at their single-job graph speed).
+    order = np.argsort([-(sp["width"] * sp["height"] * sp["spp"]) for sp in specs])
+    big = order[:4]
+    amdahl = {
+        "largest_4_jobs": [specs[k] | {"plain_s": round(per_plain[k], 3), "graph_s": round(per_graph[k], 3)}
+                           for k in big],
+        "largest_4_share_of_plain_time": round(sum(per_plain[k] for k in big) / t_plain, 3),
+        "largest_4_share_of_queries": round(sum(per_q[k] for k in big) / q_plain, 3),
+        "bound_if_other_jobs_were_free": round(t_plain / sum(per_graph[k] for k in big), 2),
+        "bound_at_batched_throughput": round(t_plain / (sum(per_q[k] for k in big) / (st["queries"] / t_batch)), 2),
+    }
+    log("amdahl:", {k: v for k, v in amdahl.items() if k != "largest_4_jobs"})
     same_plain = all(torch.equal(a, b) for a, b in zip(out_b, out_p))
     same_graph = all(torch.equal(a, b) for a, b in zip(out_b, out_g))
     assert same_plain and same_graph, "batched images must equal the per-job images bit for bit"
@@ -343,15 +367,16 @@ def batch_bench(args):
         "autotune": {"bvh_width": best["width"], "max_paths": best["max_paths"], "tuning": best["tuning"],
                      "trials": best["trials"],
                      "calibration_ms": round(best["seconds"] * 1e3, 1)},
+        "amdahl": amdahl,
         "jobs_detail": specs,
     }
-    print(json.dumps(res), flush=True)
+    return res
 
 
-def serve_bench(args):
+def run_serve(args):
     """Served form: Poisson arrivals of render requests (batched-form job distribution, a
     quarter reusing a resident scene) against a render server with an HBM budget; latency
-    is arrival to final image on the host."""
+    is arrival to final image on the host. Returns the result line's dict."""
     import threading
     from concurrent.futures import ThreadPoolExecutor
     import petrichor_amd as P
@@ -470,7 +495,7 @@ def serve_bench(args):
         "evicted_scenes": st["evicted_scenes"], "resubmitted_after_eviction": evicted_retry[0],
         "peak_note": "latency counts arrival to final image on the host, scene upload and BVH build included",
     }
-    print(json.dumps(res), flush=True)
+    return res
 
 
 def tonemap_bench(args):
@@ -594,7 +619,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-wide", action="store_true", help="skip the 8-wide comparison frame")
+    ap.add_argument("--compare-wide", action="store_true", help="frame mode: also render the 8-wide tree's frame")
+    ap.add_argument("--separate-launches", action="store_true",
+                    help="frame mode: also a frame with every launch on its own (per-kernel split)")
+    ap.add_argument("--no-batch", action="store_true", help="frame mode: skip the batched form")
+    ap.add_argument("--no-serve", action="store_true", help="frame mode: skip the served form")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="frame mode: seconds of Poisson arrivals of the served form")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=64)
@@ -641,9 +672,11 @@ def main():
     if args.mode == "trace":
         return trace_bench(args)
     if args.mode == "batch":
-        return batch_bench(args)
+        print(json.dumps(run_batch(args)), flush=True)
+        return
     if args.mode == "serve":
-        return serve_bench(args)
+        print(json.dumps(run_serve(args)), flush=True)
+        return
     if args.mode == "tonemap":
         return tonemap_bench(args)
     if args.mode == "aov":
@@ -677,30 +710,66 @@ def main():
     img = out.cpu().numpy()
     assert np.isfinite(img).all() and img.mean() > 0
 
-    # One more frame with every launch on its own and timed by HIP events on the render
-    # stream: the per-kernel durations of the roofline (not part of `value`).
-    timed = torch.empty_like(out)
-    _, kst = bvh.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=timed, time_kernels=True)
+    # The roofline of the traversal kernel on the production schedule: the same frame
+    # (graph replays, two halves on two streams) with a launch log, once for the span of
+    # every trace launch on the device clock and its rays, once with the traversal counted
+    # (node visits, triangle tests), since counting slows the launches. The rays left to
+    # the path-completion launches are counted and timed apart.
+    logged = torch.empty_like(out)
+    _, lst = bvh.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=logged, two_streams=True, launch_log=True)
     torch.cuda.synchronize()
-    assert torch.equal(timed, out), "the instrumented frame must match the production frames"
-    log("instrumented frame: %.1f ms" % kst["elapsed_ms"])
-    ext_items = kst["kernels"]["extension"]["items"]
-    ext_launches = kst["kernels"]["extension"]["launches"]
-    ext_ms = kst["kernels"]["extension"]["ms"]
-    kernels = {k: {"items": v["items"], "launches": v["launches"], "ms": round(v["ms"], 3)}
-               for k, v in kst["kernels"].items()}
-    # and one with the traversal counters on (node records and triangle tests per kind)
-    counted = torch.empty_like(out)
-    _, cst = bvh.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=counted, trace_stats=True)
+    assert torch.equal(logged, out), "the logged frame must match the production frames"
+    _, cst = bvh.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=logged, two_streams=True, launch_log=True,
+                        trace_stats=True)
     torch.cuda.synchronize()
-    assert torch.equal(counted, out), "the counting frame must match the production frames"
-    ext_c = cst["kernels"]["extension"]
-    n_node = ext_c["nodes"] / max(ext_c["items"], 1)
-    n_tri = ext_c["tris"] / max(ext_c["items"], 1)
-    for k, v in cst["kernels"].items():
-        if k != "shade":
-            kernels[k]["node_visits"] = v["nodes"]
-            kernels[k]["triangle_tests"] = v["tris"]
+    assert torch.equal(logged, out), "the counted frame must match the production frames"
+    L, C = lst["log"], cst["log"]
+    log("logged frame:", L)
+    assert C["trace_rays"] == L["trace_rays"] and C["trace_launches"] == L["trace_launches"]
+    assert L["trace_rays"] + L["finish_rays"] == lst["queries"]
+    node_b = NODE8_B if width == 8 else NODE_B
+    s_ray = RAY_IN_B + PATH_B + HIT_OUT_B
+    alg_bytes = L["trace_rays"] * s_ray + C["trace_nodes"] * node_b + C["trace_tris"] * TRI_B
+    n_launch = max(L["trace_launches"], 1)
+    avg_ms = L["trace_ms"] / n_launch
+    achieved = alg_bytes / (L["trace_ms"] * 1e-3) / 1e9 if L["trace_ms"] > 0 else 0.0
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "pmc_traffic_frame_production_w%d.json" % width)
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    counters = None  # L2 hit rate and VALU activity of the trace launches (rocprofv3 --pmc)
+    cf = os.path.join(ROOT, "profiles", "pmc_l2_valu_frame_production_w%d.json" % width)
+    if os.path.exists(cf):
+        with open(cf) as f:
+            c = json.load(f)["k_trace_persistent"]
+        counters = {"l2_hit": c["l2_hit"], "valu_active": c["valu_active"], "source": os.path.basename(cf)}
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "counters": counters,
+                "kernel": "k_trace_persistent, every trace launch of the production frame (graph replays, "
+                          "two halves on two streams)",
+                "timing": "span of each launch's blocks on the device clock (wall_clock64), logged live; "
+                          "rocprofv3 --kernel-trace of the same schedule in profiles/",
+                "algorithmic_bytes_per_launch": round(alg_bytes / n_launch, 1),
+                "bytes_per_query": round(alg_bytes / max(L["trace_rays"], 1), 1),
+                "n_node": round(C["trace_nodes"] / max(L["trace_rays"], 1), 2),
+                "n_tri": round(C["trace_tris"] / max(L["trace_rays"], 1), 3),
+                "s_node": node_b, "s_tri": TRI_B, "s_ray": s_ray,
+                "avg_launch_ms": round(avg_ms, 4), "launches": L["trace_launches"],
+                "queries_per_launch": round(L["trace_rays"] / n_launch, 1),
+                "path_completion": {"kernel": "k_finish_paths", "launches": L["finish_launches"],
+                                    "rays": L["finish_rays"], "ms": round(L["finish_ms"], 3)}}
+
+    kernels = None
+    if args.separate_launches:
+        # a frame with every launch on its own, timed by HIP events: the split of the time
+        # between the trace kinds and shading (not the production schedule)
+        timed = torch.empty_like(out)
+        _, kst = bvh.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=timed, trace_stats=True)
+        torch.cuda.synchronize()
+        assert torch.equal(timed, out), "the instrumented frame must match the production frames"
+        kernels = {k: {"items": v["items"], "launches": v["launches"], "ms": round(v["ms"], 3),
+                       "node_visits": v["nodes"], "triangle_tests": v["tris"]} for k, v in kst["kernels"].items()}
 
     cpu = None
     parity = None
@@ -713,48 +782,14 @@ def main():
         got = img.reshape(-1, 3)[pix].astype(np.float64)
         ref = rgb_ref.astype(np.float64)
         rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)
-        parity = {"pixels_checked": int(len(pix)), "max_rel_err": float(rel.max()),
-                  "pixels_bit_identical": int(np.all(got == ref, axis=1).sum())}
+        parity = {"distinct_pixels_checked": int(len(pix)), "max_rel_err": float(rel.max()),
+                  "pixels_bit_identical": int(np.all(got == ref, axis=1).sum()),
+                  "pixels_over_1e-4": int((rel.max(axis=1) > 1e-4).sum())}
         log("parity vs oracle:", parity)
-    # Roofline of the traversal kernel over every trace launch of the instrumented frame
-    # (extension, environment-shadow and area-shadow queues): algorithmic bytes from the
-    # GPU's own traversal counters, launch durations from HIP events.
-    node_b = NODE8_B if width == 8 else NODE_B
-    kinds = ("extension", "shadow_any", "shadow_closest")
-    t_items = sum(kst["kernels"][k]["items"] for k in kinds)
-    t_launches = sum(kst["kernels"][k]["launches"] for k in kinds)
-    t_ms = sum(kst["kernels"][k]["ms"] for k in kinds)
-    t_nodes = sum(cst["kernels"][k]["nodes"] for k in kinds)
-    t_tris = sum(cst["kernels"][k]["tris"] for k in kinds)
-    alg_bytes = t_items * (RAY_IN_B + PATH_B + HIT_OUT_B) + t_nodes * node_b + t_tris * TRI_B
-    achieved = alg_bytes / (t_ms * 1e-3) / 1e9 if t_ms > 0 else 0.0
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "pmc_traffic_frame_w%d.json" % width)
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    counters = None  # L2 hit rate and VALU activity of the trace launches (rocprofv3 --pmc)
-    cf = os.path.join(ROOT, "profiles", "pmc_l2_valu_frame_w%d.json" % width)
-    if os.path.exists(cf):
-        with open(cf) as f:
-            c = json.load(f)["k_trace_persistent"]
-        counters = {"l2_hit": c["l2_hit"], "valu_active": c["valu_active"], "source": os.path.basename(cf)}
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "counters": counters,
-                "kernel": "k_trace_persistent, every trace launch of the instrumented frame",
-                "algorithmic_bytes_per_launch": round(alg_bytes / max(t_launches, 1), 1),
-                "bytes_per_query": round(alg_bytes / max(t_items, 1), 1),
-                "n_node": round(t_nodes / max(t_items, 1), 2), "n_tri": round(t_tris / max(t_items, 1), 3),
-                "s_node": node_b, "s_tri": TRI_B, "s_ray": RAY_IN_B + PATH_B + HIT_OUT_B,
-                "avg_launch_ms": round(t_ms / max(t_launches, 1), 4), "launches": t_launches,
-                "queries_per_launch": round(t_items / max(t_launches, 1), 1),
-                "extension_only": {"n_node": round(n_node, 2), "n_tri": round(n_tri, 3),
-                                   "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4)}}
-    # The same frame through the 8-wide tree (a faster walk that differs from the
-    # reference's on rays grazing a box within rounding of their hit), for comparison.
+    # The same frame through the 8-wide tree, on request (a walk that differs from the
+    # reference's on rays grazing a box within rounding of their hit).
     other = None
-    if width == 2 and not args.no_wide:
+    if width == 2 and args.compare_wide:
         b8 = s.build_bvh(16, width=8)
         out8 = torch.empty_like(out)
         b8.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=out8, two_streams=True)
@@ -767,17 +802,31 @@ def main():
         torch.cuda.synchronize()
         w8 = time.perf_counter() - t0
         img8 = out8.cpu().numpy()
-        other = {"bvh": "collapsed to 8-wide nodes", "mrays_s": round(q8 / w8 / 1e6, 2),
+        other = {"bvh": "collapsed to 8-wide nodes (not the reference's walk)", "mrays_s": round(q8 / w8 / 1e6, 2),
                  "ms_per_frame": round(w8 / 2 * 1e3, 2), "bvh_build_ms": round(b8.info()["build_ms"], 1),
                  "pixels_identical_to_exact_frame": int(np.all(img8 == img, axis=2).sum())}
-        if parity is not None:
-            got8 = img8.reshape(-1, 3)[pix].astype(np.float64)
-            rel8 = np.abs(got8 - ref) / np.maximum(np.abs(ref), 1e-30)
-            other["parity_vs_oracle"] = {"pixels_checked": int(len(pix)), "max_rel_err": float(rel8.max()),
-                                         "pixels_bit_identical": int(np.all(got8 == ref, axis=1).sum()),
-                                         "pixels_over_1e-4": int((rel8.max(axis=1) > 1e-4).sum())}
         log("8-wide:", other)
         b8.close()
+    del bvh, s
+    torch.cuda.empty_cache()
+    batched = served = None
+    if not args.no_batch:
+        log("batched form")
+        br = run_batch(args)
+        batched = {k: br[k] for k in ("batched_jobs_s", "plain_launch_jobs_s", "single_graph_jobs_s",
+                                      "speedup_vs_plain_launches", "speedup_vs_single_graph", "batched_mrays_s",
+                                      "images_bit_identical", "amdahl")}
+        batched["workload"] = br["config"]["workload"]
+        batched["autotune"] = {k: br["autotune"][k] for k in ("bvh_width", "max_paths", "tuning")}
+    if not args.no_serve:
+        log("served form")
+        a2 = argparse.Namespace(**vars(args))
+        a2.duration = args.serve_seconds
+        sr = run_serve(a2)
+        served = {k: sr[k] for k in ("value", "completed", "failed", "p50_latency_ms", "p99_latency_ms",
+                                     "p50_queue_ms", "mrays_s", "busy_fraction", "evicted_scenes")}
+        served["jobs_s"] = served.pop("value")
+        served["workload"] = sr["config"]["workload"]
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 2), "higher_is_better": True,
@@ -793,14 +842,17 @@ def main():
         "queries_per_step": int(queries / args.steps),
         "iterations_per_step": int(stats[-1]["iterations"]),
         "bvh_build_ms": round(info["build_ms"], 1),
-        "kernels_instrumented_frame": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
-        "bvh8_frame": other,
+        "batched": batched,
+        "served": served,
     }
+    if kernels is not None:
+        res["kernels_separate_launches"] = kernels
+    if other is not None:
+        res["bvh8_frame"] = other
     print(json.dumps(res), flush=True)
-
 
 if __name__ == "__main__":
     main()

@@ -112,6 +112,13 @@ typedef struct pt_render_desc {
  * hardware queues of their own: with more streams alive in the process than the device
  * has queues (4 by default), the halves can share one and run slower than one piece. */
 #define PT_RENDER_TWO_STREAMS 8
+/* The production schedule (graph replays, and two streams with PT_RENDER_TWO_STREAMS) with a
+ * launch log: every trace launch records the span of its blocks on the device clock and
+ * its rays (with PT_RENDER_TRACE_STATS also its node visits and triangle tests), and every
+ * path-completion launch is timed with events; the sums fill the log_* and finish_* fields
+ * of pt_render_stats (same image). Its only extra work is one single-thread kernel after
+ * each trace launch. */
+#define PT_RENDER_LAUNCH_LOG 16
 
 /* Per-kernel figures: index 0 extension (closest hit), 1 environment shadow rays (any
  * hit), 2 area-light shadow rays (closest hit), 3 shading. Times are filled only with
@@ -125,6 +132,14 @@ typedef struct pt_render_stats {
     double kernel_ms[4];     /* summed launch durations */
     int64_t kernel_nodes[4]; /* node records visited by the trace kinds (PT_RENDER_TRACE_STATS) */
     int64_t kernel_tris[4];  /* triangle tests of the trace kinds (PT_RENDER_TRACE_STATS) */
+    /* PT_RENDER_LAUNCH_LOG: trace launches of the graphs, their rays, summed spans (ms), node
+     * visits and triangle tests (with PT_RENDER_TRACE_STATS); path-completion launches, the rays
+     * they traced and their summed durations (ms) */
+    int64_t log_launches, log_items;
+    double log_trace_ms;
+    int64_t log_nodes, log_tris;
+    int64_t finish_launches, finish_rays;
+    double finish_ms;
 } pt_render_stats;
 
 /* Context = one device + one HIP stream. */

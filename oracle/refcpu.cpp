@@ -120,10 +120,11 @@ struct Sampler2D { // RandomSampler2D.cpp:8-21 (x stream first, then y stream)
     }
 };
 
-// Per-pixel seeding as the GPU library defines it (pt_device.h): the reference seeds one sampler pair per
-// tile and never advances the tile index (Core/Petrichor.cpp:72-77), so any
-// pixel-parallel renderer needs its own seeds. MurmurHash3's 32-bit finaliser
-// (fmix32) spreads (job seed, pixel, stream) into XorShift128's w word.
+// Per-sample seeding as the GPU library defines it (pt_device.h): the reference seeds one
+// sampler pair per tile and never advances the tile index (Core/Petrichor.cpp:72-77), so
+// any parallel renderer needs its own seeds. MurmurHash3's 32-bit finaliser (fmix32)
+// spreads (job seed, pixel, stream, sample) into XorShift128's w word: every sample of a
+// pixel has its own three streams (1D, 2D x, 2D y); sample 0's are the pixel seeds.
 static inline uint32_t fmix32(uint32_t h)
 {
     h ^= h >> 16;
@@ -133,10 +134,34 @@ static inline uint32_t fmix32(uint32_t h)
     h ^= h >> 16;
     return h;
 }
+static inline uint32_t sample_seed(uint32_t job_seed, uint32_t pixel, uint32_t sample, uint32_t stream)
+{
+    return fmix32(fmix32(fmix32(job_seed) ^ pixel) + 0x9e3779b9u * (stream + 1u) + fmix32(sample));
+}
 static inline uint32_t pixel_seed(uint32_t job_seed, uint32_t pixel, uint32_t stream)
 {
-    return fmix32(fmix32(fmix32(job_seed) ^ pixel) + 0x9e3779b9u * (stream + 1u));
+    return sample_seed(job_seed, pixel, 0, stream);
 }
+// The samplers of sample s of a pixel (RandomSampler1D and RandomSampler2D,
+// Core/Petrichor.cpp:76-77, with this library's seeds). XorShift128(seed) sets only w
+// (Random/XorShift.h:17-18) and the first outputs of such a fresh stream are strongly
+// correlated, so every stream is used from its 24th output on (the GPU library's
+// PT_STREAM_WARMUP).
+static const int kStreamWarmup = 24;
+struct SampleSamplers {
+    Sampler1D s1;
+    Sampler2D s2;
+    SampleSamplers(uint32_t job_seed, uint32_t pixel, uint32_t sample)
+      : s1(sample_seed(job_seed, pixel, sample, 0)),
+        s2(sample_seed(job_seed, pixel, sample, 1), sample_seed(job_seed, pixel, sample, 2))
+    {
+        for (int i = 0; i < kStreamWarmup; ++i) {
+            (void)s1.s.random();
+            (void)s2.s0.random();
+            (void)s2.s1.random();
+        }
+    }
+};
 
 // ---- Ray: Core/Ray.h:20-49 ---------------------------------------------------
 struct Ray {
@@ -1143,12 +1168,14 @@ static V light_contribution(Ctx& c, const Shading& si, Sampler1D& s1, Sampler2D&
 }
 
 // PathTracing::Render (PathTracing.cpp:20-200) for one pixel.
-static V render_mis(Ctx& c, int px, int py, int w, int h, int spp, int max_bounces, Sampler1D& s1,
-                    Sampler2D& s2)
+static V render_mis(Ctx& c, int px, int py, int w, int h, int spp, int max_bounces, uint32_t seed)
 {
     const Scene& sc = *c.sc;
     V sum = mk(0.0f, 0.0f, 0.0f);
     for (int s = 0; s < spp; s++) {
+        SampleSamplers smp(seed, (uint32_t)(py * w + px), (uint32_t)s);
+        Sampler1D& s1 = smp.s1;
+        Sampler2D& s2 = smp.s2;
         V color = mk(0.0f, 0.0f, 0.0f);
         Ray ray = generate_ray(sc.cam, px, py, w, h, s2);
         Ray prev = ray;
@@ -1215,12 +1242,14 @@ static V render_mis(Ctx& c, int px, int py, int w, int h, int spp, int max_bounc
 }
 
 // SimplePathTracing::Render (SimplePathTracing.cpp:16-101) for one pixel.
-static V render_simple(Ctx& c, int px, int py, int w, int h, int spp, int max_bounces, Sampler1D& s1,
-                       Sampler2D& s2)
+static V render_simple(Ctx& c, int px, int py, int w, int h, int spp, int max_bounces, uint32_t seed)
 {
     const Scene& sc = *c.sc;
     V sum = mk(0.0f, 0.0f, 0.0f);
     for (int s = 0; s < spp; s++) {
+        SampleSamplers smp(seed, (uint32_t)(py * w + px), (uint32_t)s);
+        Sampler1D& s1 = smp.s1;
+        Sampler2D& s2 = smp.s2;
         Ray ray = generate_ray(sc.cam, px, py, w, h, s2);
         V contrib = mk(0.0f, 0.0f, 0.0f);
         for (;;) {
@@ -1259,12 +1288,14 @@ static V render_simple(Ctx& c, int px, int py, int w, int h, int spp, int max_bo
 // (GGX), or * the environment on a miss. Normal: throughput * the shading normal, or *
 // -dir on a miss. An Emission hit falls to the reference's default branch (zero); a path
 // that has not ended after max_bounces hits gives zero.
-static V render_aov(Ctx& c, int px, int py, int w, int h, int spp, int max_bounces, Sampler1D& s1, Sampler2D& s2,
-                    bool normal_aov)
+static V render_aov(Ctx& c, int px, int py, int w, int h, int spp, int max_bounces, uint32_t seed, bool normal_aov)
 {
     const Scene& sc = *c.sc;
     V sum = mk(0.0f, 0.0f, 0.0f);
     for (int s = 0; s < spp; s++) {
+        SampleSamplers smp(seed, (uint32_t)(py * w + px), (uint32_t)s);
+        Sampler1D& s1 = smp.s1;
+        Sampler2D& s2 = smp.s2;
         Ray ray = generate_ray(sc.cam, px, py, w, h, s2);
         V contrib = mk(0.0f, 0.0f, 0.0f);
         for (int bounce = 0; bounce < max_bounces; bounce++) {
@@ -1295,12 +1326,13 @@ static V render_aov(Ctx& c, int px, int py, int w, int h, int spp, int max_bounc
 
 // AOVUVCoordinate::Render (AOV/AOVUVCoordinate.cpp:11-51) for one pixel: the mean over spp
 // camera rays of the first hit's texture coordinates; a miss adds nothing.
-static V render_uv(Ctx& c, int px, int py, int w, int h, int spp, Sampler2D& s2)
+static V render_uv(Ctx& c, int px, int py, int w, int h, int spp, uint32_t seed)
 {
     const Scene& sc = *c.sc;
     V sum = mk(0.0f, 0.0f, 0.0f);
     for (int s = 0; s < spp; s++) {
-        const Ray ray = generate_ray(sc.cam, px, py, w, h, s2);
+        SampleSamplers smp(seed, (uint32_t)(py * w + px), (uint32_t)s);
+        const Ray ray = generate_ray(sc.cam, px, py, w, h, smp.s2);
         float t;
         const int hit = c.closest(ray, kEps, kInfinity, &t);
         if (hit >= 0) sum = add(sum, hit_uv(sc, hit, ray));
@@ -1483,13 +1515,10 @@ int64_t oracle_render(void* s, void* b, int width, int height, int spp, int max_
             const int64_t tile = next_tile.fetch_add(1);
             if (tile >= ntiles) break;
             auto do_pixel = [&](int64_t slot, int x, int y) {
-                const uint32_t pix = (uint32_t)((int64_t)y * width + x);
-                Sampler1D s1(pixel_seed(seed, pix, 0));
-                Sampler2D s2(pixel_seed(seed, pix, 1), pixel_seed(seed, pix, 2));
-                const V r = integrator == 0   ? render_mis(c, x, y, width, height, spp, max_bounces, s1, s2)
-                            : integrator == 1 ? render_simple(c, x, y, width, height, spp, max_bounces, s1, s2)
-                            : integrator == 4 ? render_uv(c, x, y, width, height, spp, s2)
-                                              : render_aov(c, x, y, width, height, spp, max_bounces, s1, s2,
+                const V r = integrator == 0   ? render_mis(c, x, y, width, height, spp, max_bounces, seed)
+                            : integrator == 1 ? render_simple(c, x, y, width, height, spp, max_bounces, seed)
+                            : integrator == 4 ? render_uv(c, x, y, width, height, spp, seed)
+                                              : render_aov(c, x, y, width, height, spp, max_bounces, seed,
                                                            integrator == 3);
                 rgb_out[3 * slot] = r.x;
                 rgb_out[3 * slot + 1] = r.y;
@@ -1516,6 +1545,11 @@ int64_t oracle_render(void* s, void* b, int width, int height, int spp, int max_
     if (work)
         for (int k = 0; k < 9; ++k) work[k] = wk[k].load();
     return queries.load();
+}
+
+uint32_t oracle_sample_seed(uint32_t job_seed, uint32_t pixel, uint32_t sample, uint32_t stream)
+{
+    return sample_seed(job_seed, pixel, sample, stream);
 }
 
 uint32_t oracle_pixel_seed(uint32_t job_seed, uint32_t pixel, uint32_t stream)
@@ -1552,6 +1586,16 @@ void oracle_camera_ray(const float* cam, int i, int j, int w, int h, uint32_t se
     const Ray r = generate_ray(cam_from(cam), i, j, w, h, s2);
     const float v[10] = {r.o.x, r.o.y, r.o.z, r.dir.x, r.dir.y, r.dir.z,
                          r.throughput.x, r.throughput.y, r.throughput.z, (float)s2.draws};
+    std::memcpy(out, v, sizeof v);
+}
+
+void oracle_sample_camera_ray(const float* cam, int i, int j, int w, int h, uint32_t job_seed, uint32_t sample,
+                              float* out)
+{
+    SampleSamplers smp(job_seed, (uint32_t)(j * w + i), sample);
+    const Ray r = generate_ray(cam_from(cam), i, j, w, h, smp.s2);
+    const float v[10] = {r.o.x, r.o.y, r.o.z, r.dir.x, r.dir.y, r.dir.z,
+                         r.throughput.x, r.throughput.y, r.throughput.z, (float)smp.s2.draws};
     std::memcpy(out, v, sizeof v);
 }
 

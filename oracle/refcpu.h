@@ -68,8 +68,9 @@ void oracle_trace_brute(void* scene, int64_t n, const float* rays, float* t_out,
 /* Render with the MIS integrator (integrator 0, Core/Integrator/PathTracing.cpp), the
  * simple path tracer (integrator 1, Core/Integrator/SimplePathTracing.cpp) or a denoising
  * AOV (2: albedo, Core/AOV/AOVDenoisingAlbedo.cpp; 3: normal, AOVDenoisingNormal.cpp;
- * spp is then the denoising sample count; 4: the UV AOV, Core/AOV/AOVUVCoordinate.cpp), per-pixel
- * seeded samplers, 16x16 tiles on num_threads workers (0 = all hardware threads).
+ * spp is then the denoising sample count; 4: the UV AOV, Core/AOV/AOVUVCoordinate.cpp), with
+ * samplers seeded per sample (oracle_sample_seed), 16x16 tiles on num_threads workers
+ * (0 = all hardware threads).
  * If pixels != NULL only those num_pixels pixel indices are rendered (rgb_out then holds
  * 3 floats per listed pixel). Returns the number of BVH queries issued. work (optional,
  * 9 int64): per query kind (extension, area-light shadow, environment shadow) the
@@ -80,11 +81,18 @@ int64_t oracle_render(void* scene, void* bvh, int width, int height, int spp, in
 
 /* Building blocks, exposed for unit tests against the reference translation units. */
 uint32_t oracle_pixel_seed(uint32_t job_seed, uint32_t pixel, uint32_t stream);
+/* The seed of stream `stream` (0: 1D, 1: 2D x, 2: 2D y) of sample `sample` of a pixel; sample
+ * 0's seeds are oracle_pixel_seed's. */
+uint32_t oracle_sample_seed(uint32_t job_seed, uint32_t pixel, uint32_t sample, uint32_t stream);
 void oracle_xorshift(uint32_t seed, int n, uint32_t* u32_out, float* f_out);
 /* Camera::GenerateRay with a RandomSampler2D(seed0, seed1): out 10 floats
  * (o[3], dir[3], throughput[3], samples consumed). */
 void oracle_camera_ray(const float* cam, int i, int j, int w, int h, uint32_t seed0, uint32_t seed1,
                        float* out);
+/* The camera ray of sample `sample` of pixel (i, j) as the renderers draw it: the sample's
+ * 2D sampler (seeded per sample, after the stream warm-up). out as oracle_camera_ray. */
+void oracle_sample_camera_ray(const float* cam, int i, int j, int w, int h, uint32_t job_seed, uint32_t sample,
+                              float* out);
 /* Material::CreateNextRay for material `mat` (mat_type/mat_param as above) on a shading
  * point {pos[3], normal[3]} for an incoming ray {o[3], dir[3], throughput[3], bounce, prob,
  * ior}; RandomSampler2D(seed0, seed1). out: o[3] dir[3] throughput[3] bounce prob ior. */

@@ -77,6 +77,7 @@ RENDER_TIME_KERNELS = 1
 RENDER_PLAIN_LAUNCHES = 2
 RENDER_TRACE_STATS = 4
 RENDER_TWO_STREAMS = 8
+RENDER_LAUNCH_LOG = 16
 AOV_DENOISING_ALBEDO, AOV_DENOISING_NORMAL, AOV_UV = 0, 1, 2
 
 
@@ -103,7 +104,9 @@ class RenderStats(ctypes.Structure):
     _fields_ = [("queries", ctypes.c_int64), ("iterations", ctypes.c_int64), ("elapsed_ms", ctypes.c_double),
                 ("kernel_items", ctypes.c_int64 * 4), ("kernel_launches", ctypes.c_int64 * 4),
                 ("kernel_ms", ctypes.c_double * 4), ("kernel_nodes", ctypes.c_int64 * 4),
-                ("kernel_tris", ctypes.c_int64 * 4)]
+                ("kernel_tris", ctypes.c_int64 * 4), ("log_launches", ctypes.c_int64), ("log_items", ctypes.c_int64),
+                ("log_trace_ms", ctypes.c_double), ("log_nodes", ctypes.c_int64), ("log_tris", ctypes.c_int64),
+                ("finish_launches", ctypes.c_int64), ("finish_rays", ctypes.c_int64), ("finish_ms", ctypes.c_double)]
 
 
 _lib = None
@@ -487,14 +490,15 @@ class Bvh:
         return t_out, prim_out, st
 
     def render(self, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1, out=None,
-               time_kernels=False, plain=False, trace_stats=False, two_streams=False):
+               time_kernels=False, plain=False, trace_stats=False, two_streams=False, launch_log=False):
         """Renders one image; returns (rgb [h, w, 3] float32 numpy array or the given device
         tensor, stats dict). time_kernels: every launch timed on its own; plain: plain
         launches with one synchronisation per iteration; two_streams: a large image in two
-        halves on two streams (all give the same image)."""
-        flags = (RENDER_TIME_KERNELS if time_kernels or trace_stats else 0) | \
+        halves on two streams; launch_log: the production schedule with every trace launch
+        logged (with trace_stats: and its work counted) (all give the same image)."""
+        flags = (RENDER_TIME_KERNELS if (time_kernels or trace_stats) and not launch_log else 0) | \
             (RENDER_PLAIN_LAUNCHES if plain else 0) | (RENDER_TRACE_STATS if trace_stats else 0) | \
-            (RENDER_TWO_STREAMS if two_streams else 0)
+            (RENDER_TWO_STREAMS if two_streams else 0) | (RENDER_LAUNCH_LOG if launch_log else 0)
         d = RenderDesc(width, height, spp, max_bounces, integrator, seed, flags)
         st = RenderStats()
         if out is None:
@@ -510,7 +514,11 @@ class Bvh:
         return res, {"queries": st.queries, "iterations": st.iterations, "elapsed_ms": st.elapsed_ms,
                      "kernels": {k: {"items": st.kernel_items[i], "launches": st.kernel_launches[i],
                                      "ms": st.kernel_ms[i], "nodes": st.kernel_nodes[i], "tris": st.kernel_tris[i]}
-                            for i, k in enumerate(kinds)}}
+                            for i, k in enumerate(kinds)},
+                     "log": {"trace_launches": st.log_launches, "trace_rays": st.log_items,
+                             "trace_ms": st.log_trace_ms, "trace_nodes": st.log_nodes, "trace_tris": st.log_tris,
+                             "finish_launches": st.finish_launches, "finish_rays": st.finish_rays,
+                             "finish_ms": st.finish_ms}}
 
     def render_aov(self, width, height, spp, max_bounces, aov=AOV_DENOISING_ALBEDO, seed=1, out=None):
         """A denoising AOV (pt_render_aov): AOV_DENOISING_ALBEDO or AOV_DENOISING_NORMAL,

@@ -7,8 +7,13 @@ replays per host synchronisation and the launch shapes of the trace and shading 
 resolutions at a few samples per pixel - first for each width with the default launch
 shapes, then with coordinate descent over the space below for each pool size. Every
 candidate of one width renders the same images (tuning only changes launch shapes and
-scheduling), so the fastest one is taken as is; the two widths differ only on rays that
-graze a box within rounding of their hit.
+scheduling), so the fastest one is taken as is.
+
+Only the binary width walks the tree exactly as the reference (BinnedSAHBVH::Intersect,
+Core/Accel/BinnedSAHBVH.cpp:272-374). The 8-wide walk can return another hit on rays that
+graze a box within rounding of their hit (33 of 1.8M pixels of the single large scene
+differ by more than 1e-4), so it is a candidate only when the caller accepts that
+(exact_only=False).
 """
 import time
 
@@ -32,12 +37,17 @@ def calibration_jobs(jobs, spp=2):
     return [Job(j.bvh, j.width, j.height, min(spp, j.spp), j.max_bounces, j.integrator, j.seed, j.out) for j in jobs]
 
 
-def tune(ctx, jobs, pool_sizes=None, passes=1, log=None, by_width=None, calib_spp=2):
+def tune(ctx, jobs, pool_sizes=None, passes=1, log=None, by_width=None, calib_spp=2, exact_only=True):
     """Returns {"width", "jobs", "max_paths", "tuning", "seconds", "trials"} for the fastest
     configuration found on the calibration batch of `jobs`. by_width (optional) maps a BVH
-    width (2, 8) to the same jobs over BVHs of that width; "jobs" is then the chosen list."""
+    width (2, 8) to the same jobs over BVHs of that width; "jobs" is then the chosen list.
+    exact_only: only binary BVHs (the reference's walk) are candidates."""
     from . import Engine
     by_width = dict(by_width) if by_width else {getattr(jobs[0].bvh, "width", 2): jobs}
+    if exact_only:
+        by_width = {w: j for w, j in by_width.items() if w == 2}
+        if not by_width:
+            raise ValueError("exact_only: the jobs need binary BVHs (width 2)")
     jobs = next(iter(by_width.values()))
     need = max(j.width * j.height for j in jobs)
     if pool_sizes is None:

@@ -99,8 +99,13 @@ PT_HD void next2(Samplers& s, float* a, float* b)
     *b = xs_next(s.s2y);
 }
 
-// Per-pixel seeds (this library's definition; the reference seeds per tile):
-// MurmurHash3 fmix32 over (job seed, pixel index, stream).
+// Per-sample seeds (this library's definition; the reference seeds one sampler pair per
+// tile, Core/Petrichor.cpp:72-77): MurmurHash3 fmix32 over (job seed, pixel index,
+// stream, sample index). Sample s of a pixel starts three fresh XorShift128 streams, so
+// the samples of one pixel are independent and run in parallel; sample 0's seeds are the
+// pixel seeds (fmix32(0) = 0). XorShift128(seed) sets only w (Random/XorShift.h:17-18),
+// and the first outputs of such a stream are strongly correlated (the first two have
+// correlation -0.88), so each stream is used from its PT_STREAM_WARMUP-th output on.
 PT_HD uint32_t fmix32(uint32_t h)
 {
     h ^= h >> 16;
@@ -110,16 +115,23 @@ PT_HD uint32_t fmix32(uint32_t h)
     h ^= h >> 16;
     return h;
 }
-PT_HD uint32_t pixel_seed(uint32_t job_seed, uint32_t pixel, uint32_t stream)
+PT_HD uint32_t sample_seed(uint32_t job_seed, uint32_t pixel, uint32_t sample, uint32_t stream)
 {
-    return fmix32(fmix32(fmix32(job_seed) ^ pixel) + 0x9e3779b9u * (stream + 1u));
+    return fmix32(fmix32(fmix32(job_seed) ^ pixel) + 0x9e3779b9u * (stream + 1u) + fmix32(sample));
 }
-PT_HD Samplers samplers_for_pixel(uint32_t job_seed, uint32_t pixel)
+#define PT_STREAM_WARMUP 24
+PT_HD XorShift xorshift_warmed(uint32_t seed)
+{
+    XorShift x = xorshift_seeded(seed);
+    for (int i = 0; i < PT_STREAM_WARMUP; ++i) (void)xs_random(x);
+    return x;
+}
+PT_HD Samplers samplers_for_sample(uint32_t job_seed, uint32_t pixel, uint32_t sample)
 {
     Samplers s;
-    s.s1 = xorshift_seeded(pixel_seed(job_seed, pixel, 0));
-    s.s2x = xorshift_seeded(pixel_seed(job_seed, pixel, 1));
-    s.s2y = xorshift_seeded(pixel_seed(job_seed, pixel, 2));
+    s.s1 = xorshift_warmed(sample_seed(job_seed, pixel, sample, 0));
+    s.s2x = xorshift_warmed(sample_seed(job_seed, pixel, sample, 1));
+    s.s2y = xorshift_warmed(sample_seed(job_seed, pixel, sample, 2));
     return s;
 }
 

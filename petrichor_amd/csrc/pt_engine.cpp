@@ -1,17 +1,19 @@
 // pt_engine.cpp — the wavefront render engine behind pt_render, pt_render_batch and
 // the render server.
 //
-// A pool of path slots is shared by any number of jobs (one path per pixel: the
-// samples of a pixel run one after another, as in PathTracing::Render,
-// Core/Integrator/PathTracing.cpp:38-196, so a pixel's sampler stream is consumed in the
-// reference's order). Every iteration runs one persistent trace launch over the three
-// ray queues and one shading launch over the live paths; both read their counts from
-// device memory, so an iteration is a fixed sequence of launches that is captured once
-// as a hipGraph and replayed several times between host synchronisations. Jobs are
-// admitted into free slots between replays (continuous batching) and retired when
-// all their pixels are done. Once every job is admitted and few paths are live, one
-// path-completion launch runs them to the end; a large single-job render runs as two
-// halves of its pixels on two engines, whose launches overlap.
+// A pool of path slots is shared by any number of jobs. A job's work is its samples (every
+// sample has sampler streams of its own, so the samples of a pixel are independent): each
+// of its path slots runs one sample at a time and then takes the job's next sample, and a
+// sample's colour goes to the job's sample buffer; when every sample is done the pixels
+// are summed in sample order (PathTracing::Render's accumulation,
+// Core/Integrator/PathTracing.cpp:38-198). Every iteration runs one persistent trace launch
+// over the three ray queues and one shading launch over the live paths; both read their
+// counts from device memory, so an iteration is a fixed sequence of launches that is
+// captured once as a hipGraph and replayed several times between host synchronisations.
+// Jobs are admitted into free slots between replays (continuous batching) and retired when
+// all their samples are done. Once every job is admitted, every sample handed out and few
+// paths are live, one path-completion launch runs them to the end; a large single-job
+// render runs as two halves of its pixels on two engines, whose launches overlap.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,6 +33,8 @@ using namespace pt;
 // With PT_RENDER_TWO_STREAMS, images of at least this many pixels render in two halves
 // (run_halves).
 static constexpr int64_t kHalvesMinPixels = (int64_t)1 << 19;
+// Path slots of pt_render's engines: a job runs on up to max(its pixels, this many) slots.
+static constexpr int64_t kSingleJobSlots = (int64_t)1 << 21;
 
 namespace pt {
 
@@ -43,6 +47,7 @@ int check_job(pt_context* ctx, const pt_job_desc& j)
         (d.integrator != PT_INTEGRATOR_MIS && d.integrator != PT_INTEGRATOR_SIMPLE))
         return pt_fail(ctx, PT_ERR_ARG, "bad render settings");
     if ((int64_t)d.width * d.height > (1 << 28)) return pt_fail(ctx, PT_ERR_ARG, "image too large");
+    if ((int64_t)d.spp > ((int64_t)1 << 31) - 1) return pt_fail(ctx, PT_ERR_ARG, "too many samples per pixel");
     return PT_OK;
 }
 
@@ -80,7 +85,9 @@ EngineJob make_job(const pt_job_desc& d, int64_t id)
     j.user_out = d.rgb_out;
     j.out_on_device = d.out_on_device != 0;
     j.npix = (int64_t)d.render.width * d.render.height;
-    j.bytes = j.npix * (kPathBytes + 12);
+    j.items = j.npix * d.render.spp;
+    j.slots = 0; // set at admission (pt_engine::job_slots)
+    j.bytes = std::min(j.items, std::max(j.npix, kMinJobSlots)) * kPathBytes + j.npix * 12 + j.items * kSampleBytes;
     j.submitted = std::chrono::steady_clock::now();
     return j;
 }
@@ -108,7 +115,8 @@ int run_jobs(pt_engine& E, std::vector<EngineJob>& jobs, int iters_per_sync, pt_
     while (!pending.empty() || !running.empty()) {
         for (size_t i = 0; i < pending.size() && (int64_t)running.size() < E.max_jobs;) {
             EngineJob& j = *pending[i];
-            if (E.largest_free() < j.npix || !E.width_ok(pt_engine::job_width(j))) {
+            j.slots = E.job_slots(j);
+            if (E.largest_free() < j.slots || !E.width_ok(pt_engine::job_width(j))) {
                 ++i;
                 continue;
             }
@@ -122,18 +130,21 @@ int run_jobs(pt_engine& E, std::vector<EngineJob>& jobs, int iters_per_sync, pt_
         int rc = E.run(iters_per_sync);
         if (rc != PT_OK) return rc;
         ++replays;
-        // every job admitted and few paths left: the remaining paths in one launch
-        if (pending.empty() && E.live_h > 0 && E.live_h <= E.finish_threshold) {
+        // every job admitted, every sample handed out and few paths left: the remaining
+        // paths in one launch
+        bool exhausted = pending.empty();
+        for (const EngineJob* j : running) exhausted = exhausted && E.job_exhausted(*j);
+        if (exhausted && E.live_h > 0 && E.live_h <= E.finish_threshold) {
             rc = E.finish_live();
             if (rc != PT_OK) return rc;
             for (const EngineJob* j : running)
-                if (E.done_h[j->slot] < j->npix)
-                    return pt_fail(ctx, PT_ERR_STATE, "path completion left pixels unfinished");
+                if (!E.job_done(*j)) return pt_fail(ctx, PT_ERR_STATE, "path completion left samples unfinished");
         }
         for (size_t i = 0; i < running.size();) {
             EngineJob& j = *running[i];
-            if (E.done_h[j.slot] >= j.npix) {
-                rc = finish_output(ctx, j, E.st);
+            if (E.job_done(j)) {
+                rc = E.resolve(j);
+                if (rc == PT_OK) rc = finish_output(ctx, j, E.st);
                 if (rc != PT_OK) return rc;
                 E.retire(j);
                 running.erase(running.begin() + i);
@@ -180,15 +191,19 @@ int run_halves(pt_context* ctx, pt_engine* E[2], const pt_job_desc& jd, int iter
     for (int i = 0; i < 2; ++i) {
         E[i]->iterations = 0;
         PT_HIP(ctx, hipMemsetAsync(E[i]->queries_d, 0, sizeof(unsigned long long), E[i]->st));
+        h[i].items = h[i].npix * h[i].desc.spp;
+        h[i].slots = 0;
         rc = E[i]->start(h[i]);
         if (rc != PT_OK) return rc;
-        E[i]->live_h = (int)h[i].npix;
+        E[i]->live_h = (int)h[i].slots;
     }
     while (!done[0] || !done[1]) {
         // both halves end together: their completion launches start in the same round, once
-        // the live paths of both fit one completion launch
+        // every sample of both is handed out and the live paths of both fit one completion
+        // launch
         const int live = (done[0] ? 0 : E[0]->live_h) + (done[1] ? 0 : E[1]->live_h);
-        const bool last = live > 0 && live <= E[0]->finish_threshold;
+        const bool exhausted = (done[0] || E[0]->job_exhausted(h[0])) && (done[1] || E[1]->job_exhausted(h[1]));
+        const bool last = exhausted && live > 0 && live <= E[0]->finish_threshold;
         for (int i = 0; i < 2; ++i) {
             if (done[i]) continue;
             rc = last ? E[i]->launch_finish() : E[i]->launch(iters_per_sync);
@@ -198,11 +213,16 @@ int run_halves(pt_context* ctx, pt_engine* E[2], const pt_job_desc& jd, int iter
             if (done[i]) continue;
             rc = E[i]->wait();
             if (rc != PT_OK) return rc;
-            done[i] = E[i]->done_h[h[i].slot] >= h[i].npix;
+            done[i] = E[i]->job_done(h[i]);
             if (!done[i] && (last || E[i]->live_h == 0))
-                return pt_fail(ctx, PT_ERR_STATE, "render ended with pixels unfinished");
+                return pt_fail(ctx, PT_ERR_STATE, "render ended with samples unfinished");
+            if (done[i]) {
+                rc = E[i]->resolve(h[i]);
+                if (rc != PT_OK) return rc;
+            }
         }
     }
+    for (int i = 0; i < 2; ++i) PT_HIP(ctx, hipStreamSynchronize(E[i]->st));
     const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     for (int i = 0; i < 2; ++i) E[i]->retire(h[i]);
     rc = finish_output(ctx, full, E[0]->st);
@@ -306,22 +326,30 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
     int rc = check_job(ctx, jd);
     if (rc != PT_OK) return rc;
     const int64_t npix = (int64_t)d->width * d->height;
+    // path slots of a single-job engine: every sample, up to max(pixels, kSingleJobSlots)
+    auto slots_for = [&](int64_t n) { return std::min(n * d->spp, std::max(n, kSingleJobSlots)); };
     // On request, large images render in two halves on two engines (streams), whose
     // launches overlap.
     if ((d->flags & PT_RENDER_TWO_STREAMS) && npix >= kHalvesMinPixels &&
         !(d->flags & (PT_RENDER_PLAIN_LAUNCHES | PT_RENDER_TIME_KERNELS))) {
-        const int64_t cap = npix - npix / 2;
+        const int64_t cap = slots_for(npix - npix / 2);
         pt_engine** slots[2] = {&ctx->engine, &ctx->engine2};
         pt_engine* E2[2];
         for (int i = 0; i < 2; ++i) {
             if (!*slots[i] || (*slots[i])->capacity < cap) {
                 delete *slots[i];
                 *slots[i] = nullptr;
-                rc = pt_engine_create(ctx, npix, 1, 0, slots[i]);
+                rc = pt_engine_create(ctx, cap, 1, 0, slots[i]);
                 if (rc != PT_OK) return rc;
                 (*slots[i])->iters_per_sync = 8;
+                (*slots[i])->min_job_slots = kSingleJobSlots;
             }
             E2[i] = *slots[i];
+        }
+        const bool log = (d->flags & PT_RENDER_LAUNCH_LOG) != 0;
+        for (int i = 0; i < 2; ++i) {
+            rc = E2[i]->set_logging(log, (d->flags & PT_RENDER_TRACE_STATS) != 0);
+            if (rc != PT_OK) return rc;
         }
         pt_batch_stats bs;
         rc = run_halves(ctx, E2, jd, E2[0]->iters_per_sync, &bs);
@@ -332,15 +360,23 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
             stats->iterations = bs.iterations;
             stats->elapsed_ms = bs.elapsed_ms;
         }
+        if (log) {
+            for (int i = 0; i < 2; ++i) {
+                if (stats) rc = E2[i]->read_log(stats);
+                if (rc == PT_OK) rc = E2[i]->set_logging(false, false);
+                if (rc != PT_OK) return rc;
+            }
+        }
         return PT_OK;
     }
     // the context keeps a single-job engine, grown on demand
-    if (!ctx->engine || ctx->engine->capacity < npix) {
+    if (!ctx->engine || ctx->engine->capacity < slots_for(npix)) {
         delete ctx->engine;
         ctx->engine = nullptr;
-        rc = pt_engine_create(ctx, npix, 1, 0, &ctx->engine);
+        rc = pt_engine_create(ctx, slots_for(npix), 1, 0, &ctx->engine);
         if (rc != PT_OK) return rc;
         ctx->engine->iters_per_sync = 8;
+        ctx->engine->min_job_slots = kSingleJobSlots;
     }
     pt_engine& E = *ctx->engine;
     E.iterations = 0;
@@ -357,8 +393,9 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
             rc = E.run_plain(&live);
             if (rc != PT_OK) return rc;
         }
+        rc = E.resolve(j);
+        if (rc == PT_OK) rc = finish_output(ctx, j, E.st);
         const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        rc = finish_output(ctx, j, E.st);
         E.retire(j);
         if (rc != PT_OK) return rc;
         if (stats) {
@@ -396,8 +433,9 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
             (void)hipFree(stats_d);
             PT_HIP(ctx, e);
         }
+        rc = E.resolve(j);
+        if (rc == PT_OK) rc = finish_output(ctx, j, E.st);
         const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        rc = finish_output(ctx, j, E.st);
         E.retire(j);
         if (rc != PT_OK) return rc;
         if (stats) {
@@ -414,6 +452,9 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
         }
         return PT_OK;
     }
+    const bool log = (d->flags & PT_RENDER_LAUNCH_LOG) != 0;
+    rc = E.set_logging(log, (d->flags & PT_RENDER_TRACE_STATS) != 0);
+    if (rc != PT_OK) return rc;
     pt_batch_stats bs;
     rc = run_jobs(E, jobs, E.iters_per_sync, &bs);
     if (rc != PT_OK) return rc;
@@ -422,6 +463,11 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
         stats->queries = bs.queries;
         stats->iterations = bs.iterations;
         stats->elapsed_ms = bs.elapsed_ms;
+    }
+    if (log) {
+        if (stats) rc = E.read_log(stats);
+        if (rc == PT_OK) rc = E.set_logging(false, false);
+        if (rc != PT_OK) return rc;
     }
     return PT_OK;
 }
@@ -438,7 +484,7 @@ int pt_render_batch(pt_engine* E, int32_t n, const pt_job_desc* jobs, int32_t it
         if (rc != PT_OK) return rc;
         if (jobs[i].scene->ctx != E->ctx) return pt_fail(E->ctx, PT_ERR_ARG, "job scene on another context");
         js.push_back(make_job(jobs[i], i));
-        if (js.back().npix > E->capacity) return pt_fail(E->ctx, PT_ERR_BUDGET, "job larger than the engine");
+
     }
     E->iterations = 0;
     PT_HIP(E->ctx, hipMemsetAsync(E->queries_d, 0, sizeof(unsigned long long), E->st));

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -15,7 +16,12 @@
 namespace pt {
 
 // Bytes of device memory one path slot uses (path state, queues, hits, live lists).
-constexpr int64_t kPathBytes = 3 * 16 + 6 * 16 + 16 + 8 + 6 * 16 + 4 * 4 + 2 * 4 + 3 * (32 + 4);
+constexpr int64_t kPathBytes = 3 * 16 + 5 * 16 + 16 + 8 + 6 * 16 + 4 * 4 + 2 * 4 + 3 * (32 + 4);
+// Bytes per sample of a job's sample buffer.
+constexpr int64_t kSampleBytes = 12;
+// Path slots a job gets at least (when it has that many samples): its samples, not only
+// its pixels, run in parallel.
+constexpr int64_t kMinJobSlots = (int64_t)1 << 20;
 
 struct Buffers {
     std::vector<void*> ptrs;
@@ -46,6 +52,9 @@ struct EngineJob {
     bool own_out = false;
     int64_t base = 0, npix = 0; // path slots; the job renders npix pixels from pixel_base
     int64_t pixel_base = 0;
+    int64_t items = 0;         // samples: npix * spp
+    int64_t slots = 0;         // path slots it runs on (set at admission)
+    float* samples_d = nullptr; // its sample buffer (kSampleBytes per sample), from the engine's pool
     int slot = -1;             // job table index
     int64_t bytes = 0;         // working memory charged to the budget
     std::chrono::steady_clock::time_point submitted, started;
@@ -71,10 +80,16 @@ struct pt_engine {
     int32_t* q_path[3] = {nullptr, nullptr, nullptr};
     pt::JobDev* jobs_d = nullptr;
     pt::BvhView* views_d = nullptr;
-    int* done_d = nullptr;
+    unsigned long long* done_d = nullptr;   // samples finished, per job slot
+    unsigned long long* cursor_d = nullptr; // next item handed out, per job slot
     unsigned long long* queries_d = nullptr;
     std::vector<pt::JobDev> jobs_h;
-    std::vector<int> done_h;
+    std::vector<unsigned long long> done_h, cursor_h;
+    // sample buffers of retired jobs, kept for the next jobs (bytes -> buffer)
+    std::multimap<size_t, void*> spare_samples;
+    std::vector<void*> sample_bufs;
+    bool clear_samples = false;     // fill new sample buffers with the preview's "not done" pattern
+    bool keep_spare_samples = true; // keep retired jobs' sample buffers for later jobs (else free them)
     std::vector<char> slot_used;
     std::map<int64_t, int64_t> free_ranges; // first path slot -> length
     int set = 1;                            // counter set the next iteration reads
@@ -91,9 +106,20 @@ struct pt_engine {
     // synchronisation, the rest runs as one path-completion launch (pt::launch_finish_paths).
     int finish_threshold = 1 << 17;
     int live_h = 0;            // live paths at the last synchronisation
-    // pinned host copies of the finished-pixel counts and the live count, written by
-    // asynchronous copies (so that queueing them does not wait for the stream)
-    int* stage_h = nullptr;
+    // pinned host copies of the per-job sample counts and cursors and of the live count,
+    // written by asynchronous copies (so that queueing them does not wait for the stream)
+    unsigned long long* stage_h = nullptr;
+    int* stage_live_h = nullptr;
+    // Launch log (PT_RENDER_LAUNCH_LOG): every trace launch of the graphs records its
+    // device-clock span and ray counts (log_counts: also its node visits and triangle
+    // tests); the completion launches are timed with events and their rays counted from
+    // the query counter (finish_q: before and after each, in stream order).
+    bool logging = false, log_counts = false;
+    pt::LaunchRec* log_d = nullptr;
+    int* log_n_d = nullptr;
+    unsigned long long* finish_q_d = nullptr; // 2 per completion launch
+    std::vector<hipEvent_t> finish_ev;        // 2 per completion launch
+    int finish_n = 0;
 
     pt::Queue queue(int s, int kind) const
     {
@@ -116,8 +142,95 @@ struct pt_engine {
     ~pt_engine()
     {
         drop_graphs();
+        for (hipEvent_t e : finish_ev) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamSynchronize(st);
+        for (void* b : sample_bufs) (void)hipFree(b);
         if (st) (void)hipStreamDestroy(st);
         if (stage_h) (void)hipHostFree(stage_h);
+        if (stage_live_h) (void)hipHostFree(stage_live_h);
+    }
+
+    // Path slots of a job: all its samples, up to max(its pixels, min_job_slots), within the
+    // engine's capacity; in a multi-job engine at most half of it, so that jobs arriving
+    // later find slots while a large one runs.
+    int64_t min_job_slots = pt::kMinJobSlots;
+    int64_t job_slots(const pt::EngineJob& j) const
+    {
+        const int64_t want = std::max(j.npix, min_job_slots);
+        const int64_t cap = multi ? std::max<int64_t>(1, capacity / 2) : capacity;
+        return std::max<int64_t>(1, std::min(std::min(j.items, want), cap));
+    }
+
+    static constexpr int kFinishLogCap = 64;
+
+    // Turns the launch log on (clearing it) or off; the graphs are captured again.
+    int set_logging(bool on, bool counts)
+    {
+        PT_HIP(ctx, hipSetDevice(ctx->device));
+        if (on && !log_d) {
+            PT_HIP(ctx, buf.alloc(&log_d, PT_LAUNCH_LOG_CAP));
+            PT_HIP(ctx, buf.alloc(&log_n_d, 1));
+            PT_HIP(ctx, buf.alloc(&finish_q_d, 2 * kFinishLogCap));
+            finish_ev.resize(2 * kFinishLogCap);
+            for (hipEvent_t& e : finish_ev) PT_HIP(ctx, hipEventCreate(&e));
+        }
+        if (on) {
+            std::vector<pt::LaunchRec> init(PT_LAUNCH_LOG_CAP);
+            for (pt::LaunchRec& r : init) {
+                std::memset(&r, 0, sizeof r);
+                r.t0 = ~0ull;
+            }
+            PT_HIP(ctx, hipMemcpyAsync(log_d, init.data(), init.size() * sizeof(pt::LaunchRec),
+                                       hipMemcpyHostToDevice, st));
+            PT_HIP(ctx, pt::launch_set(log_n_d, 0, st));
+            PT_HIP(ctx, hipStreamSynchronize(st));
+            finish_n = 0;
+        }
+        if (on != logging || counts != log_counts) drop_graphs();
+        logging = on;
+        log_counts = on && counts;
+        return PT_OK;
+    }
+
+    void log_job(pt::TraceJob& tj) const
+    {
+        tj.launch_log = logging ? log_d : nullptr;
+        tj.launch_index = logging ? log_n_d : nullptr;
+        tj.log_counts = log_counts ? 1 : 0;
+    }
+
+    // Sums of the log: trace launches, their rays, device-clock milliseconds, node visits,
+    // triangle tests; completion launches, their rays and event milliseconds.
+    int read_log(pt_render_stats* s)
+    {
+        if (!logging) return PT_OK;
+        PT_HIP(ctx, hipStreamSynchronize(st));
+        int n = 0;
+        PT_HIP(ctx, hipMemcpy(&n, log_n_d, sizeof n, hipMemcpyDeviceToHost));
+        std::vector<pt::LaunchRec> recs((size_t)n);
+        if (n > 0) PT_HIP(ctx, hipMemcpy(recs.data(), log_d, (size_t)n * sizeof(pt::LaunchRec), hipMemcpyDeviceToHost));
+        int khz = 0;
+        PT_HIP(ctx, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+        for (const pt::LaunchRec& r : recs) {
+            const int64_t items = (int64_t)r.items[0] + r.items[1] + r.items[2];
+            if (items == 0 || r.t1 < r.t0) continue; // an empty launch returns before logging
+            s->log_launches += 1;
+            s->log_items += items;
+            s->log_trace_ms += khz > 0 ? (double)(r.t1 - r.t0) / (double)khz : 0.0;
+            s->log_nodes += (int64_t)r.nodes;
+            s->log_tris += (int64_t)r.tris;
+        }
+        std::vector<unsigned long long> q((size_t)2 * finish_n);
+        if (finish_n > 0)
+            PT_HIP(ctx, hipMemcpy(q.data(), finish_q_d, q.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        for (int i = 0; i < finish_n; ++i) {
+            float ms = 0.0f;
+            PT_HIP(ctx, hipEventElapsedTime(&ms, finish_ev[2 * i], finish_ev[2 * i + 1]));
+            s->finish_launches += 1;
+            s->finish_rays += (int64_t)(q[2 * i + 1] - q[2 * i]);
+            s->finish_ms += ms;
+        }
+        return PT_OK;
     }
 
     int init(pt_context* c, int64_t cap, int jobs, bool multi_job)
@@ -132,7 +245,6 @@ struct pt_engine {
         const size_t P = (size_t)cap;
 #define ALLOC(p, n) PT_HIP(ctx, buf.alloc(&(p), (n)))
         ALLOC(ps.rng, 3 * P);
-        ALLOC(ps.sum, P);
         ALLOC(ps.color, P);
         ALLOC(ps.ray_o, P);
         ALLOC(ps.ray_d, P);
@@ -160,18 +272,22 @@ struct pt_engine {
         ALLOC(jobs_d, (size_t)jobs);
         ALLOC(views_d, (size_t)jobs);
         ALLOC(done_d, (size_t)jobs);
+        ALLOC(cursor_d, (size_t)jobs);
         ALLOC(queries_d, 1);
 #undef ALLOC
-        PT_HIP(ctx, hipHostMalloc((void**)&stage_h, sizeof(int) * ((size_t)jobs + 1)));
+        PT_HIP(ctx, hipHostMalloc((void**)&stage_h, sizeof(unsigned long long) * 2 * (size_t)jobs));
+        PT_HIP(ctx, hipHostMalloc((void**)&stage_live_h, sizeof(int)));
         jobs_h.assign(jobs, pt::JobDev{});
         done_h.assign(jobs, 0);
+        cursor_h.assign(jobs, 0);
         slot_used.assign(jobs, 0);
         free_ranges.clear();
         free_ranges[0] = cap;
         PT_HIP(ctx, pt::launch_zero(counters, 8, st));
         PT_HIP(ctx, pt::launch_zero(counters + PT_CURSOR_STRIDE, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
         PT_HIP(ctx, hipMemsetAsync(queries_d, 0, sizeof(unsigned long long), st));
-        PT_HIP(ctx, hipMemsetAsync(done_d, 0, sizeof(int) * (size_t)jobs, st));
+        PT_HIP(ctx, hipMemsetAsync(done_d, 0, sizeof(unsigned long long) * (size_t)jobs, st));
+        PT_HIP(ctx, hipMemsetAsync(cursor_d, 0, sizeof(unsigned long long) * (size_t)jobs, st));
         PT_HIP(ctx, hipStreamSynchronize(st));
         set = 1;
         return PT_OK;
@@ -249,6 +365,34 @@ struct pt_engine {
         return true;
     }
 
+    // A sample buffer of at least `bytes` from the spare ones, or a new one.
+    int take_samples(size_t bytes, float** out)
+    {
+        auto it = spare_samples.lower_bound(bytes);
+        if (it != spare_samples.end() && it->first <= 2 * bytes) {
+            *out = (float*)it->second;
+            spare_samples.erase(it);
+            return PT_OK;
+        }
+        void* b = nullptr;
+        hipError_t e = hipMalloc(&b, bytes);
+        if (e != hipSuccess && !spare_samples.empty()) {
+            // give back the spare buffers and try once more
+            PT_HIP(ctx, hipStreamSynchronize(st));
+            for (auto& kv : spare_samples) {
+                (void)hipFree(kv.second);
+                sample_bufs.erase(std::find(sample_bufs.begin(), sample_bufs.end(), kv.second));
+            }
+            spare_samples.clear();
+            e = hipMalloc(&b, bytes);
+        }
+        PT_HIP(ctx, e);
+        sample_bufs.push_back(b);
+        *out = (float*)b;
+        return PT_OK;
+    }
+    static size_t samples_bytes(const pt::EngineJob& j) { return (size_t)pt::kSampleBytes * (size_t)std::max<int64_t>(j.items, 1); }
+
     int start(pt::EngineJob& j)
     {
         const int jw = job_width(j);
@@ -264,7 +408,13 @@ struct pt_engine {
                 break;
             }
         if (slot < 0) return pt_fail(ctx, PT_ERR_BUDGET, "job table full");
-        if (!alloc_range(j.npix, &j.base)) return pt_fail(ctx, PT_ERR_BUDGET, "no free path slots");
+        if (j.slots <= 0) j.slots = job_slots(j);
+        if (!alloc_range(j.slots, &j.base)) return pt_fail(ctx, PT_ERR_BUDGET, "no free path slots");
+        int rc = take_samples(samples_bytes(j), &j.samples_d);
+        if (rc != PT_OK) {
+            free_range(j.base, j.slots);
+            return rc;
+        }
         slot_used[slot] = 1;
         j.slot = slot;
         pt::JobDev jd;
@@ -278,30 +428,56 @@ struct pt_engine {
         v.seed = j.desc.seed;
         v.out = j.out_d;
         v.done = done_d + slot;
+        v.cursor = cursor_d + slot;
+        v.samples = j.samples_d;
+        v.items = (unsigned long long)j.items;
         v.path_base = (int)j.base;
+        v.num_slots = (int)j.slots;
         v.job_index = slot;
         v.pixel_base = (int)j.pixel_base;
         v.num_pixels = (int)j.npix;
         jobs_h[slot] = jd;
         done_h[slot] = 0;
-        PT_HIP(ctx, hipMemsetAsync(done_d + slot, 0, sizeof(int), st));
-        PT_HIP(ctx, hipMemcpyAsync(jobs_d + slot, &jd, sizeof(pt::JobDev), hipMemcpyHostToDevice, st));
-        PT_HIP(ctx, hipMemcpyAsync(views_d + slot, &jd.sc.bvh, sizeof(pt::BvhView), hipMemcpyHostToDevice, st));
+        cursor_h[slot] = 0;
+        PT_HIP(ctx, hipMemcpyAsync(jobs_d + slot, &jobs_h[slot], sizeof(pt::JobDev), hipMemcpyHostToDevice, st));
+        PT_HIP(ctx, hipMemcpyAsync(views_d + slot, &jobs_h[slot].sc.bvh, sizeof(pt::BvhView), hipMemcpyHostToDevice, st));
         if (!multi) {
             sc1 = jd.sc;
             job1 = v;
             integrator1 = v.integrator;
             drop_graphs();
         }
+        if (clear_samples) PT_HIP(ctx, pt::launch_samples_clear(v, st));
         PT_HIP(ctx, pt::launch_paths_init(jd.sc, v, ps, queue(set, 0), act[set], counters + 4 * set + 3, st));
         j.started = std::chrono::steady_clock::now();
         return PT_OK;
     }
 
+    bool job_done(const pt::EngineJob& j) const { return done_h[j.slot] >= (unsigned long long)j.items; }
+    bool job_exhausted(const pt::EngineJob& j) const { return cursor_h[j.slot] >= (unsigned long long)j.items; }
+
+    // The image of a job whose samples are all done (in stream order after them).
+    int resolve(const pt::EngineJob& j)
+    {
+        PT_HIP(ctx, pt::launch_resolve(jobs_h[j.slot].job, st));
+        return PT_OK;
+    }
+
     void retire(pt::EngineJob& j)
     {
-        free_range(j.base, j.npix);
+        free_range(j.base, j.slots);
         slot_used[j.slot] = 0;
+        if (j.samples_d) {
+            if (keep_spare_samples) {
+                // reused by a later job only after the work queued so far (stream order)
+                spare_samples.emplace(samples_bytes(j), j.samples_d);
+            } else {
+                (void)hipStreamSynchronize(st);
+                (void)hipFree(j.samples_d);
+                sample_bufs.erase(std::find(sample_bufs.begin(), sample_bufs.end(), (void*)j.samples_d));
+            }
+            j.samples_d = nullptr;
+        }
     }
 
     // One wavefront iteration reading counter set r and writing set 1 - r.
@@ -327,8 +503,10 @@ struct pt_engine {
         tj.wide = multi && width == 8;
         tj.job_of_path = multi ? ps.job : nullptr;
         tj.queries = queries_d;
+        log_job(tj);
         PT_HIP(ctx, pt::launch_zero(counters + PT_CURSOR_STRIDE, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
         PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + PT_CURSOR_STRIDE, nullptr, st, tune));
+        if (logging) PT_HIP(ctx, pt::launch_log_tick(counters + 4 * r, log_d, log_n_d, st));
         PT_HIP(ctx, pt::launch_zero(counters + 4 * w, 4, st));
         pt::ShadeArgs a;
         a.ps = ps;
@@ -391,9 +569,10 @@ struct pt_engine {
 
     int queue_counts()
     {
-        PT_HIP(ctx, hipMemcpyAsync(stage_h, done_d, sizeof(int) * (size_t)max_jobs, hipMemcpyDeviceToHost, st));
-        PT_HIP(ctx, hipMemcpyAsync(stage_h + max_jobs, counters + 4 * set + 3, sizeof(int), hipMemcpyDeviceToHost,
-                                   st));
+        const size_t nb = sizeof(unsigned long long) * (size_t)max_jobs;
+        PT_HIP(ctx, hipMemcpyAsync(stage_h, done_d, nb, hipMemcpyDeviceToHost, st));
+        PT_HIP(ctx, hipMemcpyAsync(stage_h + max_jobs, cursor_d, nb, hipMemcpyDeviceToHost, st));
+        PT_HIP(ctx, hipMemcpyAsync(stage_live_h, counters + 4 * set + 3, sizeof(int), hipMemcpyDeviceToHost, st));
         return PT_OK;
     }
 
@@ -401,9 +580,19 @@ struct pt_engine {
     int wait()
     {
         PT_HIP(ctx, hipStreamSynchronize(st));
-        for (int i = 0; i < max_jobs; ++i) done_h[i] = stage_h[i];
-        live_h = stage_h[max_jobs];
+        for (int i = 0; i < max_jobs; ++i) {
+            done_h[i] = stage_h[i];
+            cursor_h[i] = stage_h[max_jobs + i];
+        }
+        live_h = *stage_live_h;
         return PT_OK;
+    }
+
+    // Counts of the jobs, synchronously (the timed and plain loops).
+    int read_counts()
+    {
+        const int rc = queue_counts();
+        return rc != PT_OK ? rc : wait();
     }
 
     // Traces the queued rays and then completes every live path in one launch, and
@@ -437,8 +626,10 @@ struct pt_engine {
         tj.wide = multi && width == 8;
         tj.job_of_path = multi ? ps.job : nullptr;
         tj.queries = queries_d;
+        log_job(tj);
         PT_HIP(ctx, pt::launch_zero(counters + PT_CURSOR_STRIDE, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
         PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + PT_CURSOR_STRIDE, nullptr, st, tune));
+        if (logging) PT_HIP(ctx, pt::launch_log_tick(counters + 4 * r, log_d, log_n_d, st));
         pt::ShadeArgs a{};
         a.ps = ps;
         a.active_in = act[r];
@@ -449,8 +640,18 @@ struct pt_engine {
         a.sh_k = sh_k;
         const pt::HitBuffers h{hit_t, hit_k, sh_t, sh_k};
         const bool wide = multi ? width == 8 : sc1.bvh.nodes8 != nullptr;
+        const bool log_finish = logging && finish_n < kFinishLogCap;
+        if (log_finish) {
+            PT_HIP(ctx, pt::launch_copy_u64(queries_d, finish_q_d + 2 * finish_n, st));
+            PT_HIP(ctx, hipEventRecord(finish_ev[2 * finish_n], st));
+        }
         PT_HIP(ctx, pt::launch_finish_paths(integrator1, sc1, job1, multi ? jobs_d : nullptr, wide, a, h, queries_d,
                                             st));
+        if (log_finish) {
+            PT_HIP(ctx, hipEventRecord(finish_ev[2 * finish_n + 1], st));
+            PT_HIP(ctx, pt::launch_copy_u64(queries_d, finish_q_d + 2 * finish_n + 1, st));
+            ++finish_n;
+        }
         // no live path and no queued ray is left: the counters of this set start the next job
         PT_HIP(ctx, pt::launch_zero(counters + 4 * r, 4, st));
         ++iterations;
@@ -512,7 +713,6 @@ struct pt_engine {
         PT_HIP(ctx, hipEventRecord(ev[6], st));
         PT_HIP(ctx, pt::launch_shade(integrator1, sc1, job1, multi ? jobs_d : nullptr, a, st, tune));
         PT_HIP(ctx, hipEventRecord(ev[7], st));
-        PT_HIP(ctx, hipMemcpyAsync(done_h.data(), done_d, sizeof(int) * (size_t)max_jobs, hipMemcpyDeviceToHost, st));
         PT_HIP(ctx, hipStreamSynchronize(st));
         for (int k = 0; k < 4; ++k) {
             if (h[k] <= 0) continue;
@@ -577,7 +777,6 @@ struct pt_engine {
         a.sh_t = sh_t;
         a.sh_k = sh_k;
         PT_HIP(ctx, pt::launch_shade(integrator1, sc1, job1, multi ? jobs_d : nullptr, a, st, tune));
-        PT_HIP(ctx, hipMemcpyAsync(done_h.data(), done_d, sizeof(int) * (size_t)max_jobs, hipMemcpyDeviceToHost, st));
         PT_HIP(ctx, hipStreamSynchronize(st));
         set = w;
         ++iterations;

@@ -43,12 +43,20 @@ struct SceneView {
     EnvDev env;
 };
 
+// A job's work is num_pixels * spp samples, handed out to its path slots one sample at a
+// time (sample-major: sample s of the job's pixel i is item s * num_pixels + i). A finished
+// sample's colour goes to `samples` at the same index; once every sample is done, the
+// pixel is the sum of its samples in sample order over spp (k_resolve).
 struct JobView {
     int width, height, spp, max_bounces, integrator;
     uint32_t seed;
-    float* out;      // width*height*3 floats of the image
-    int* done;       // pixels finished so far
+    float* out;                 // width*height*3 floats of the image
+    unsigned long long* done;   // samples finished so far
+    unsigned long long* cursor; // next item to hand out
+    float* samples;             // 3 floats per item
+    unsigned long long items;   // num_pixels * spp
     int path_base;   // first path slot of this job
+    int num_slots;   // path slots of this job
     int job_index;   // index in the engine's job table
     int pixel_base;  // the job renders pixels [pixel_base, pixel_base + num_pixels) of the image
     int num_pixels;
@@ -63,13 +71,12 @@ struct JobDev {
 // Per-path state, structure of arrays indexed by path slot.
 struct PathView {
     uint4* rng;    // 3 per path: 1D stream, 2D x stream, 2D y stream
-    float4* sum;   // pixel sum
     float4* color; // sample colour, w = sin of the last BSDF direction to the normal
     float4* ray_o; // origin, w = prob
     float4* ray_d; // direction, w = ior
     float4* thr;   // throughput, w = BSDF pdf of the last sampled direction
     int4* meta;    // sample index, ray bounce, phase | pending << 8, loop bounce
-    int32_t* pixel;
+    int32_t* pixel; // the job's pixel index of the current sample
     int32_t* job;   // engine job index of each path slot
     float4* pend_a; // pending shadow data (see k_shade)
     float4* pend_b;
@@ -99,6 +106,16 @@ struct TraceSegment {
     int32_t* out_k;
     int rays_by_path = 0; // ray i is read at rays[path[i]] (the API's coherence-ordered launches)
 };
+// One record of a launch log: the device-clock span of a trace launch (first block's start
+// to last block's end, in wall-clock ticks), the rays of its three segments and, when the
+// launch counts its work, its node visits and triangle tests.
+struct LaunchRec {
+    unsigned long long t0, t1;
+    unsigned long long nodes, tris;
+    int items[3];
+    int pad;
+};
+#define PT_LAUNCH_LOG_CAP 8192
 struct TraceJob {
     TraceSegment seg[3];
     int num_seg;
@@ -108,6 +125,12 @@ struct TraceJob {
     unsigned long long* queries; // optional: += rays traced by this launch
     int chunk = 64;              // rays a wave takes from the cursor at a time (>= 64; set by the launcher)
     int wide = 0;                // 8-wide traversal (every BVH of a multi-job launch has the same width)
+    // optional launch log: the launch writes its span (and with log_counts its node visits
+    // and triangle tests) into launch_log[*launch_index]; launch_log_tick then records the
+    // ray counts and advances the index
+    LaunchRec* launch_log = nullptr;
+    const int* launch_index = nullptr;
+    int log_counts = 0;
 };
 
 // Launch shapes chosen by the autotuner (defaults fit every workload reasonably).
@@ -140,6 +163,11 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job, int* cursor, 
 hipError_t launch_zero(int* counters, int n, hipStream_t st, int stride = 1);
 // *p = value, by a kernel (no host copy).
 hipError_t launch_set(int* p, int value, hipStream_t st);
+// After a logged trace launch: log[*index].items = the three segment counts, ++*index
+// (while *index < PT_LAUNCH_LOG_CAP - 1).
+hipError_t launch_log_tick(const int* seg_counts, LaunchRec* log, int* index, hipStream_t st);
+// dst[i] = src[0] (copies a device counter into a log slot in stream order).
+hipError_t launch_copy_u64(const unsigned long long* src, unsigned long long* dst, hipStream_t st);
 // Exclusive prefix sum of n integers (rocPRIM); call with temp == nullptr to size temp.
 template <typename T>
 hipError_t exclusive_sum(void* temp, size_t& temp_bytes, const T* in, T* out, size_t n, hipStream_t st);
@@ -147,14 +175,21 @@ hipError_t exclusive_sum(void* temp, size_t& temp_bytes, const T* in, T* out, si
 // graph capture so that no device query happens while capturing).
 int cu_count();
 
-// Progressive image of a running job into out (3 floats per pixel); adds the job's
-// completed samples to *samples.
+// Progressive image of a running job into out (3 floats per pixel): the mean of each
+// pixel's completed samples (the sample buffer must have been cleared when the job
+// started); adds the job's completed samples to *samples.
 hipError_t launch_preview(const JobView& job, const PathView& ps, float* out, unsigned long long* samples,
                           hipStream_t st);
-// Sets up every path of a job (seeds, sums, first camera ray) and appends the paths to
-// the live list and their camera rays to the extension queue.
+// Sets up every path slot of a job with the job's first items (seeds, first camera ray),
+// sets the job's item cursor and sample count, and appends the paths to the live list and
+// their camera rays to the extension queue.
 hipError_t launch_paths_init(const SceneView& sc, const JobView& job, const PathView& ps, const Queue& ext,
                              int* active, int* active_count, hipStream_t st);
+// The image of a job whose samples are all done: each pixel is the sum of its samples in
+// sample order, divided by spp (PathTracing.cpp:194-198).
+hipError_t launch_resolve(const JobView& job, hipStream_t st);
+// Fills a job's sample buffer with the "not done" pattern read by the progressive preview.
+hipError_t launch_samples_clear(const JobView& job, hipStream_t st);
 // Shading of every live path (count read on the device). With `jobs` non-null each
 // path takes its scene and settings from jobs[ps.job[p]], else from sc / job.
 struct ShadeArgs {

@@ -189,6 +189,41 @@ __device__ __forceinline__ void wave_count(int* counter, bool want)
     }
 }
 
+// 64-bit form of wave_count (the samples a job has finished).
+__device__ __forceinline__ void wave_count64(unsigned long long* counter, bool want)
+{
+    const int lane = threadIdx.x & 63;
+    unsigned long long left = __ballot(want);
+    while (left) {
+        const int leader = __ffsll((long long)left) - 1;
+        const unsigned long long lp = (unsigned long long)__shfl((long long)(uintptr_t)counter, leader, 64);
+        const unsigned long long same = __ballot(want && (unsigned long long)(uintptr_t)counter == lp);
+        if (lane == leader) atomicAdd(counter, (unsigned long long)__popcll(same));
+        left &= ~same;
+    }
+}
+
+// Takes one item per lane with `want` from that lane's cursor, one atomic per distinct
+// cursor in the wave; lanes without `want` get ~0. Every lane of the wave must call it.
+__device__ __forceinline__ unsigned long long wave_take(unsigned long long* cursor, bool want)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lane_lt = (1ull << lane) - 1ull;
+    unsigned long long left = __ballot(want);
+    unsigned long long mine = ~0ull;
+    while (left) {
+        const int leader = __ffsll((long long)left) - 1;
+        const unsigned long long lp = (unsigned long long)__shfl((long long)(uintptr_t)cursor, leader, 64);
+        const unsigned long long same = __ballot(want && (unsigned long long)(uintptr_t)cursor == lp);
+        long long base = 0;
+        if (lane == leader) base = (long long)atomicAdd(cursor, (unsigned long long)__popcll(same));
+        base = __shfl(base, leader, 64);
+        if ((same >> lane) & 1ull) mine = (unsigned long long)base + (unsigned long long)__popcll(same & lane_lt);
+        left &= ~same;
+    }
+    return mine;
+}
+
 __device__ __forceinline__ void push_ray(const Queue& q, int slot, int path, V3 o, float dmin, V3 d, float dmax)
 {
     q.rays[2 * slot] = make_float4(o.x, o.y, o.z, dmin);
@@ -203,37 +238,53 @@ __device__ __forceinline__ Ray camera_ray(const SceneView& sc, const JobView& jo
     return generate_ray(sc.cam, x, y, job.width, job.height, smp);
 }
 
-// One-time set-up of every path of a job: sampler seeds, sums and the first camera ray;
-// the path joins the live list and its camera ray the extension queue.
+// Starts item `item` (< job.items) on path p: sample s = item / num_pixels of the job's
+// pixel item % num_pixels, with that sample's own sampler streams and its camera ray.
+__device__ __forceinline__ Ray start_sample(const SceneView& sc, const JobView& job, const PathView& ps, int p,
+                                            unsigned long long item)
+{
+    const unsigned long long np = (unsigned long long)job.num_pixels;
+    const int i = (int)(item % np), s = (int)(item / np);
+    const int pixel = job.pixel_base + i;
+    Samplers smp = samplers_for_sample(job.seed, (uint32_t)pixel, (uint32_t)s);
+    const Ray r = camera_ray(sc, job, pixel, smp);
+    ps.color[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    ps.ray_o[p] = st3(r.o, r.prob);
+    ps.ray_d[p] = st3(r.dir, r.ior);
+    ps.thr[p] = st3(r.throughput, 0.0f);
+    ps.meta[p] = make_int4(s, r.bounce, kPhCamera, 0);
+    ps.rng[3 * p] = pack_xs(smp.s1);
+    ps.rng[3 * p + 1] = pack_xs(smp.s2x);
+    ps.rng[3 * p + 2] = pack_xs(smp.s2y);
+    ps.pixel[p] = pixel;
+    ps.job[p] = job.job_index;
+    return r;
+}
+
+// The extension-ray interval of a camera ray: [0, FLT_MAX] for MIS (AccelBase.h:44-48),
+// [kEps, kInfinity] for the simple integrator (SimplePathTracing.cpp:47).
+__device__ __forceinline__ float camera_dmin(int integrator) { return integrator == 0 ? 0.0f : PT_EPS; }
+__device__ __forceinline__ float camera_dmax(int integrator) { return integrator == 0 ? FLT_MAX : PT_INFINITY; }
+
+// One-time set-up of every path slot of a job with the job's first items; the path joins
+// the live list and its camera ray the extension queue. Thread 0 sets the job's cursor
+// past the items handed out here and zeroes its sample count.
 __global__ void __launch_bounds__(256) k_paths_init(SceneView sc, JobView job, PathView ps, Queue ext, int* active,
                                                     int* active_count)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < job.num_pixels;
+    const bool live = i < job.num_slots && (unsigned long long)i < job.items;
     const int p = job.path_base + i;
-    const int pixel = job.pixel_base + i;
-    Samplers smp;
-    Ray r;
-    if (live) {
-        smp = samplers_for_pixel(job.seed, (uint32_t)pixel);
-        r = camera_ray(sc, job, pixel, smp);
-        ps.sum[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        ps.color[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        ps.ray_o[p] = st3(r.o, r.prob);
-        ps.ray_d[p] = st3(r.dir, r.ior);
-        ps.thr[p] = st3(r.throughput, 0.0f);
-        ps.meta[p] = make_int4(0, r.bounce, kPhCamera, 0);
-        ps.rng[3 * p] = pack_xs(smp.s1);
-        ps.rng[3 * p + 1] = pack_xs(smp.s2x);
-        ps.rng[3 * p + 2] = pack_xs(smp.s2y);
-        ps.pixel[p] = pixel;
-        ps.job[p] = job.job_index;
+    if (i == 0) {
+        *job.cursor = (unsigned long long)job.num_slots < job.items ? (unsigned long long)job.num_slots : job.items;
+        *job.done = 0ull;
     }
+    Ray r;
+    if (live) r = start_sample(sc, job, ps, p, (unsigned long long)i);
     const int a = queue_slot(active_count, live);
     if (live) active[a] = p;
     const int slot = queue_slot(ext.count, live);
-    if (live) push_ray(ext, slot, p, r.o, job.integrator == 0 ? 0.0f : PT_EPS, r.dir,
-                       job.integrator == 0 ? FLT_MAX : PT_INFINITY);
+    if (live) push_ray(ext, slot, p, r.o, camera_dmin(job.integrator), r.dir, camera_dmax(job.integrator));
 }
 
 // The shading stage of the MIS integrator (PathTracing::Render, PathTracing.cpp:20-200,
@@ -246,7 +297,7 @@ __global__ void __launch_bounds__(256) k_paths_init(SceneView sc, JobView job, P
 // What shading one path asks of the queues.
 struct ShadeOut {
     bool keep, want_ext, want_any, want_closest;
-    int* done; // the job's finished-pixel counter when this path finished its pixel
+    bool sample_end; // the path finished its sample (its colour is in the job's sample buffer)
     V3 ext_o, ext_d, sh_o, sh_d;
     float ext_dmin, ext_dmax;
 };
@@ -260,8 +311,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
     const int* hit_k = A.hit_k;
     const float* sh_t = A.sh_t;
     const int* sh_k = A.sh_k;
-    bool keep = false, want_ext = false, want_any = false, want_closest = false;
-    int* done = nullptr;
+    bool keep = false, want_ext = false, want_any = false, want_closest = false, sample_end = false;
     V3 ext_o = v3(0.0f, 0.0f, 0.0f), ext_d = ext_o, sh_o = ext_o, sh_d = ext_o;
     float ext_dmin = 0.0f, ext_dmax = FLT_MAX;
     {
@@ -283,9 +333,8 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
         const int pend = (meta.z >> 8) & 0xff;
         V3 color = ld3(ps.color[p]);
         float sin_n = ps.color[p].w;
-        V3 sum = ld3(ps.sum[p]);
+        V3 direct = v3(0.0f, 0.0f, 0.0f); // a camera ray's miss or emitter: added unconditionally
         const int pixel = ps.pixel[p];
-        const int spp = job.spp;
         bool sample_done = false, add_color = true;
 
         if (INTEGRATOR == 0) {
@@ -344,7 +393,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
             }
             if (phase == kPhCamera) {
                 if (k < 0) {
-                    sum = sum + ray.throughput * env_color(sc.env, ray.dir);
+                    direct = ray.throughput * env_color(sc.env, ray.dir);
                     sample_done = true;
                     add_color = false;
                 } else {
@@ -352,7 +401,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
                     si_mat = pre_mat;
                     si_uv = maps_uv(sc, k, ray.o, ray.dir);
                     if (sc.mats[si_mat].type == kMatEmission) {
-                        sum = sum + ray.throughput * sc.mats[si_mat].color;
+                        direct = ray.throughput * sc.mats[si_mat].color;
                         sample_done = true;
                         add_color = false;
                     } else {
@@ -503,35 +552,20 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
             }
         }
 
-        // end of a sample: accumulate, then start the next one or finish the pixel
-        int sample = meta.x;
+        // end of a sample: its colour goes to the job's sample buffer (the MIS integrator
+        // drops a sample with a negative component, PathTracing.cpp:191-195: it adds zero)
+        const int sample = meta.x;
         if (sample_done) {
-            if (INTEGRATOR == 0) {
-                if (add_color && min_elem(color) >= 0.0f) sum = sum + color;
-            } else {
-                sum = sum + color;
-            }
-            color = v3(0.0f, 0.0f, 0.0f);
-            sample += 1;
-            if (sample < spp) {
-                ray = camera_ray(sc, job, pixel, smp);
-                want_ext = true;
-                ext_o = ray.o;
-                ext_d = ray.dir;
-                ext_dmin = INTEGRATOR == 0 ? 0.0f : PT_EPS;
-                ext_dmax = INTEGRATOR == 0 ? FLT_MAX : PT_INFINITY;
-                phase = kPhCamera;
-                loop_b = 0;
-            } else {
-                const V3 avg = sum / (float)spp;
-                float* o = job.out + 3 * (size_t)pixel;
-                o[0] = avg.x;
-                o[1] = avg.y;
-                o[2] = avg.z;
-                done = job.done;
-                ps.meta[p] = make_int4(spp, 0, kPhDone, 0); // read by the progressive preview
-                phase = kPhDone;
-            }
+            V3 v;
+            if (INTEGRATOR == 0) v = add_color ? (min_elem(color) >= 0.0f ? color : v3(0.0f, 0.0f, 0.0f)) : direct;
+            else v = color;
+            const size_t item = (size_t)sample * (size_t)job.num_pixels + (size_t)(pixel - job.pixel_base);
+            float* o = job.samples + 3 * item;
+            o[0] = v.x;
+            o[1] = v.y;
+            o[2] = v.z;
+            sample_end = true;
+            phase = kPhDone;
         }
         keep = phase != kPhDone;
         if (keep) {
@@ -543,11 +577,10 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
             ps.thr[p] = st3(ray.throughput, pdf_bsdf);
             ps.meta[p] = make_int4(sample, ray.bounce, phase | (new_pend << 8), loop_b);
             ps.color[p] = st3(color, sin_n);
-            ps.sum[p] = st3(sum, 0.0f);
         }
     }
     out.keep = keep;
-    out.done = done;
+    out.sample_end = sample_end;
     out.want_ext = want_ext;
     out.want_any = want_any;
     out.want_closest = want_closest;
@@ -573,16 +606,37 @@ __device__ __forceinline__ void shade_live_paths(const SceneView& sc1, const Job
         const bool in_range = i < n;
         const int p = in_range ? A.active_in[i] : 0;
         ShadeOut o;
-        o.keep = o.want_ext = o.want_any = o.want_closest = false;
-        o.done = nullptr;
+        o.keep = o.want_ext = o.want_any = o.want_closest = o.sample_end = false;
+        const JobDev* J = nullptr;
         if (in_range) {
             if (MULTI) {
                 // the integrator is a per-job setting here
-                const JobDev* J = jobs + A.ps.job[p];
+                J = jobs + A.ps.job[p];
                 if (J->job.integrator == 0) shade_one<0>(J->sc, J->job, A, p, o);
                 else shade_one<1>(J->sc, J->job, A, p, o);
             } else {
                 shade_one<INTEGRATOR>(sc1, job1, A, p, o);
+            }
+        }
+        // a finished sample is counted and the path takes the job's next item, if any
+        {
+            unsigned long long* done = nullptr;
+            unsigned long long* cursor = nullptr;
+            if (o.sample_end) {
+                done = MULTI ? J->job.done : job1.done;
+                cursor = MULTI ? J->job.cursor : job1.cursor;
+            }
+            wave_count64(done, o.sample_end);
+            const unsigned long long item = wave_take(cursor, o.sample_end);
+            if (o.sample_end && item < (MULTI ? J->job.items : job1.items)) {
+                const JobView& jv = MULTI ? J->job : job1;
+                const Ray r = start_sample(MULTI ? J->sc : sc1, jv, A.ps, p, item);
+                o.keep = true;
+                o.want_ext = true;
+                o.ext_o = r.o;
+                o.ext_d = r.dir;
+                o.ext_dmin = camera_dmin(jv.integrator);
+                o.ext_dmax = camera_dmax(jv.integrator);
             }
         }
         int* const counters[4] = {A.ext.count, A.sh_any.count, A.sh_closest.count, A.active_out_count};
@@ -593,7 +647,6 @@ __device__ __forceinline__ void shade_live_paths(const SceneView& sc1, const Job
         if (o.want_any) push_ray(A.sh_any, slot[1], p, o.sh_o, 0.0f, o.sh_d, FLT_MAX);
         if (o.want_closest) push_ray(A.sh_closest, slot[2], p, o.sh_o, 0.0f, o.sh_d, FLT_MAX);
         if (o.keep) A.active_out[slot[3]] = p;
-        wave_count(o.done, o.done != nullptr);
     }
 }
 
@@ -650,9 +703,9 @@ __global__ void __launch_bounds__(256) k_render_aov(SceneView sc, int w, int h, 
     unsigned long long q = 0;
     for (int pix = blockIdx.x * blockDim.x + threadIdx.x; pix < n; pix += gridDim.x * blockDim.x) {
         const int px = pix % w, py = pix / w;
-        Samplers smp = samplers_for_pixel(seed, (uint32_t)pix);
         V3 sum = v3(0.0f, 0.0f, 0.0f);
         for (int s = 0; s < spp; ++s) {
+            Samplers smp = samplers_for_sample(seed, (uint32_t)pix, (uint32_t)s);
             Ray ray = generate_ray(sc.cam, px, py, w, h, smp);
             V3 contrib = v3(0.0f, 0.0f, 0.0f);
             if (mode == 2) {
@@ -713,7 +766,8 @@ hipError_t launch_render_aov(const SceneView& sc, int w, int h, int spp, int max
     return hipGetLastError();
 }
 
-// Events of path p from its current hits to the end of its pixel. Returns the rays traced.
+// Events of path p from its current hits to the end of its sample, then of the job's next
+// items while there are any. Returns the rays traced.
 template <int INTEGRATOR, bool WIDE>
 __device__ __forceinline__ unsigned long long finish_path(const SceneView& sc, const JobView& job,
                                                           const ShadeArgs& A, const HitBuffers& H, int p,
@@ -722,10 +776,20 @@ __device__ __forceinline__ unsigned long long finish_path(const SceneView& sc, c
     unsigned long long q = 0;
     for (;;) {
         ShadeOut o;
-        o.keep = o.want_ext = o.want_any = o.want_closest = false;
-        o.done = nullptr;
+        o.keep = o.want_ext = o.want_any = o.want_closest = o.sample_end = false;
         shade_one<INTEGRATOR>(sc, job, A, p, o);
-        if (o.done) atomicAdd(o.done, 1);
+        if (o.sample_end) {
+            atomicAdd(job.done, 1ull);
+            const unsigned long long item = atomicAdd(job.cursor, 1ull);
+            if (item >= job.items) break;
+            const Ray r = start_sample(sc, job, A.ps, p, item);
+            o.keep = true;
+            o.want_ext = true;
+            o.ext_o = r.o;
+            o.ext_d = r.dir;
+            o.ext_dmin = camera_dmin(job.integrator);
+            o.ext_dmax = camera_dmax(job.integrator);
+        }
         if (!o.keep) break;
         float t;
         int k;
@@ -751,8 +815,7 @@ __device__ __forceinline__ unsigned long long finish_path(const SceneView& sc, c
 }
 
 // Completion of the live paths of a render, one thread per path: every shading event is
-// followed at once by tracing its rays in the same thread, until the pixel has all its
-// samples. The events, their order and every result are those of the wavefront
+// followed at once by tracing its rays in the same thread, until the job has no item left. The events, their order and every result are those of the wavefront
 // iterations; what changes is that a path no longer waits for the slowest ray of each
 // iteration, which near the end of a render (few live paths, each iteration as long as
 // its longest ray) is where the time goes. Multi-job launches read each path's job.
@@ -789,6 +852,19 @@ __global__ void k_set_int(int* p, int v)
     if (threadIdx.x == 0) *p = v;
 }
 
+__global__ void k_log_tick(const int* seg_counts, LaunchRec* log, int* index)
+{
+    if (threadIdx.x != 0) return;
+    const int i = *index;
+    for (int k = 0; k < 3; ++k) log[i].items[k] = seg_counts[k];
+    if (i < PT_LAUNCH_LOG_CAP - 1) *index = i + 1;
+}
+
+__global__ void k_copy_u64(const unsigned long long* src, unsigned long long* dst)
+{
+    if (threadIdx.x == 0) *dst = *src;
+}
+
 // ---------------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------------
@@ -807,44 +883,52 @@ hipError_t launch_set(int* p, int value, hipStream_t st)
     return hipGetLastError();
 }
 
+hipError_t launch_log_tick(const int* seg_counts, LaunchRec* log, int* index, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_log_tick, dim3(1), dim3(64), 0, st, seg_counts, log, index);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_u64(const unsigned long long* src, unsigned long long* dst, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_copy_u64, dim3(1), dim3(64), 0, st, src, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_paths_init(const SceneView& sc, const JobView& job, const PathView& ps, const Queue& ext,
                              int* active, int* active_count, hipStream_t st)
 {
-    const int64_t n = job.num_pixels;
+    const int64_t n = job.num_slots;
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_paths_init, dim3(grid_for(n, 256)), dim3(256), 0, st, sc, job, ps, ext, active,
                        active_count);
     return hipGetLastError();
 }
 
-// Progressive image of a running job: finished pixels from the output, the others as the
-// mean of their completed samples (black before the first); adds the completed samples
-// to *samples.
-__global__ void __launch_bounds__(256) k_preview(JobView job, PathView ps, float* out,
-                                                 unsigned long long* samples)
+// Progressive image of a running job: per pixel the mean of its completed samples (a
+// sample not done yet holds the all-ones pattern written by k_samples_clear), black before
+// the first; adds the completed samples to *samples.
+__global__ void __launch_bounds__(256) k_preview(JobView job, float* out, unsigned long long* samples)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = job.num_pixels;
     unsigned long long c = 0;
     if (i < n) {
-        const int p = job.path_base + i;
-        const size_t px = (size_t)job.pixel_base + i;
-        const int s = ps.meta[p].x;
         float r = 0.0f, g = 0.0f, b = 0.0f;
-        if (s >= job.spp) {
-            r = job.out[3 * px];
-            g = job.out[3 * px + 1];
-            b = job.out[3 * px + 2];
-        } else if (s > 0) {
-            const float4 sum = ps.sum[p];
-            r = sum.x / (float)s;
-            g = sum.y / (float)s;
-            b = sum.z / (float)s;
+        int k = 0;
+        for (int s = 0; s < job.spp; ++s) {
+            const float* v = job.samples + 3 * ((size_t)s * n + i);
+            if (__float_as_uint(v[0]) == 0xffffffffu) continue;
+            r += v[0];
+            g += v[1];
+            b += v[2];
+            ++k;
         }
-        out[3 * px] = r;
-        out[3 * px + 1] = g;
-        out[3 * px + 2] = b;
-        c = (unsigned long long)min(s, job.spp);
+        const size_t px = (size_t)job.pixel_base + i;
+        out[3 * px] = k ? r / (float)k : 0.0f;
+        out[3 * px + 1] = k ? g / (float)k : 0.0f;
+        out[3 * px + 2] = k ? b / (float)k : 0.0f;
+        c = (unsigned long long)k;
     }
     for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(samples, c);
@@ -853,9 +937,58 @@ __global__ void __launch_bounds__(256) k_preview(JobView job, PathView ps, float
 hipError_t launch_preview(const JobView& job, const PathView& ps, float* out, unsigned long long* samples,
                           hipStream_t st)
 {
+    (void)ps;
     const int64_t n = job.num_pixels;
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_preview, dim3(grid_for(n, 256)), dim3(256), 0, st, job, ps, out, samples);
+    hipLaunchKernelGGL(k_preview, dim3(grid_for(n, 256)), dim3(256), 0, st, job, out, samples);
+    return hipGetLastError();
+}
+
+// PathTracing::Render's average (PathTracing.cpp:194-198): the samples of a pixel summed in
+// sample order, then divided by spp.
+__global__ void __launch_bounds__(256) k_resolve(JobView job)
+{
+    const int n = job.num_pixels;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        V3 sum = v3(0.0f, 0.0f, 0.0f);
+        for (int s = 0; s < job.spp; ++s) {
+            const float* v = job.samples + 3 * ((size_t)s * n + i);
+            sum = sum + v3(v[0], v[1], v[2]);
+        }
+        const V3 avg = sum / (float)job.spp;
+        float* o = job.out + 3 * ((size_t)job.pixel_base + i);
+        o[0] = avg.x;
+        o[1] = avg.y;
+        o[2] = avg.z;
+    }
+}
+
+hipError_t launch_resolve(const JobView& job, hipStream_t st)
+{
+    const int64_t n = job.num_pixels;
+    if (n <= 0) return hipSuccess;
+    const int64_t want = (n + 255) / 256, cap = (int64_t)cu_count() * 16;
+    hipLaunchKernelGGL(k_resolve, dim3((unsigned)(want < cap ? want : cap)), dim3(256), 0, st, job);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_fill_ones(uint4* p, size_t n16, uint32_t* tail, size_t n_tail)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu);
+    if (blockIdx.x == 0 && threadIdx.x < n_tail) tail[threadIdx.x] = 0xffffffffu;
+}
+
+hipError_t launch_samples_clear(const JobView& job, hipStream_t st)
+{
+    const size_t words = 3 * (size_t)job.items;
+    if (words == 0) return hipSuccess;
+    // the buffer starts 256-byte aligned (hipMalloc); the last words that do not fill a
+    // 16-byte store are written one by one
+    const size_t n16 = words / 4, n_tail = words - 4 * n16;
+    const int64_t want = (int64_t)((n16 + 255) / 256), cap = (int64_t)cu_count() * 16;
+    hipLaunchKernelGGL(k_fill_ones, dim3((unsigned)(want < 1 ? 1 : (want < cap ? want : cap))), dim3(256), 0, st,
+                       (uint4*)job.samples, n16, (uint32_t*)job.samples + 4 * n16, n_tail);
     return hipGetLastError();
 }
 

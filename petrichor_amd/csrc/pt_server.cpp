@@ -160,8 +160,13 @@ struct pt_server {
         const bool head_only = !queue.empty() && ms_between(queue.front()->submitted, Clock::now()) > kHeadWaitMs;
         for (auto it = queue.begin(); it != queue.end() && scanned < kLookahead; ++scanned) {
             SRequest* r = *it;
-            const int64_t bytes = 12 * r->npix;
-            const bool fits = (int64_t)running.size() < E->max_jobs && E->largest_free() >= r->npix &&
+            // the image and the request's sample buffer (the engine allocates the latter)
+            const int64_t items = r->npix * r->desc.spp;
+            const int64_t bytes = 12 * r->npix + kSampleBytes * items;
+            EngineJob probe;
+            probe.npix = r->npix;
+            probe.items = items;
+            const bool fits = (int64_t)running.size() < E->max_jobs && E->largest_free() >= E->job_slots(probe) &&
                               E->width_ok(scenes.at(r->scene_id).bvh->view.nodes8 ? 8 : 2) && make_room(bytes);
             if (!fits) {
                 if (running.empty() && it == queue.begin() && !make_room(bytes)) {
@@ -175,7 +180,7 @@ struct pt_server {
                 ++it;
                 continue;
             }
-            hipError_t e = hipMalloc((void**)&r->img_d, (size_t)bytes);
+            hipError_t e = hipMalloc((void**)&r->img_d, 12 * (size_t)r->npix);
             if (e != hipSuccess) {
                 r->img_d = nullptr;
                 fail_request(r, std::string("image allocation: ") + hipGetErrorString(e));
@@ -263,10 +268,12 @@ struct pt_server {
             bool notify = false;
             for (size_t i = 0; i < running.size();) {
                 SRequest* r = running[i];
-                if (E->done_h[r->job.slot] >= r->npix) {
+                if (E->job_done(r->job)) {
                     r->image.resize(3 * (size_t)r->npix);
-                    hipError_t e = hipMemcpyAsync(r->image.data(), r->img_d, 12 * (size_t)r->npix,
-                                                  hipMemcpyDeviceToHost, E->st);
+                    hipError_t e = E->resolve(r->job) == PT_OK ? hipSuccess : hipErrorUnknown;
+                    if (e == hipSuccess)
+                        e = hipMemcpyAsync(r->image.data(), r->img_d, 12 * (size_t)r->npix, hipMemcpyDeviceToHost,
+                                           E->st);
                     if (e == hipSuccess) e = hipStreamSynchronize(E->st);
                     E->retire(r->job);
                     if (e != hipSuccess) {
@@ -338,6 +345,10 @@ int pt_server_create(const pt_server_desc* d, pt_server** out)
         return sfail(PT_ERR_HIP, "context creation failed");
     }
     int rc = pt_engine_create(s->run, max_paths, max_jobs, 1, &s->E);
+    if (rc == PT_OK) {
+        s->E->clear_samples = true;     // progressive previews read the sample buffers
+        s->E->keep_spare_samples = false; // their memory is charged to the budget per request
+    }
     if (rc == PT_OK && hipMalloc((void**)&s->samples_d, sizeof(unsigned long long)) != hipSuccess) rc = PT_ERR_OOM;
     if (rc != PT_OK) {
         const std::string why = s->run->err;
@@ -453,9 +464,8 @@ int pt_server_submit(pt_server* s, int64_t scene_id, const pt_render_desc* d, in
     if (s->stop) return sfail(PT_ERR_STATE, "server stopped");
     auto it = s->scenes.find(scene_id);
     if (it == s->scenes.end() || it->second.removing) return sfail(PT_ERR_STATE, "unknown or evicted scene");
-    if (npix > s->E->capacity) return sfail(PT_ERR_BUDGET, "request larger than the path pool");
-    if (s->pool + it->second.bytes + 12 * npix > s->budget)
-        return sfail(PT_ERR_BUDGET, "request image does not fit the budget next to its scene");
+    if (s->pool + it->second.bytes + 12 * npix + kSampleBytes * npix * d->spp > s->budget)
+        return sfail(PT_ERR_BUDGET, "request image and samples do not fit the budget next to its scene");
     auto r = std::make_unique<SRequest>();
     r->id = s->next_id++;
     r->scene_id = scene_id;

@@ -60,6 +60,9 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
     }
     if (job.queries && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(job.queries, (unsigned long long)total);
     if (total == 0) return;
+    // launch log: the index stays put during the launch (the tick after it advances it)
+    LaunchRec* rec = job.launch_log ? job.launch_log + *job.launch_index : nullptr;
+    if (rec && threadIdx.x == 0) atomicMin(&rec->t0, (unsigned long long)wall_clock64());
     __shared__ uint2 lds_stack[lds_stack_entries<WIDE>() * PT_TRACE_BLOCK];
     uint2* lds = lds_stack + threadIdx.x;
     StackEntry spill[(WIDE ? PT_STACK8 : PT_MAX_STACK) - lds_stack_entries<WIDE>()];
@@ -158,8 +161,12 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
         }
     }
     if (STATS) {
-        atomicAdd(&stats[0], (unsigned long long)nv);
-        atomicAdd(&stats[1], (unsigned long long)nt);
+        atomicAdd(rec ? &rec->nodes : &stats[0], (unsigned long long)nv);
+        atomicAdd(rec ? &rec->tris : &stats[1], (unsigned long long)nt);
+    }
+    if (rec) {
+        __syncthreads(); // the block's last wave is done
+        if (threadIdx.x == 0) atomicMax(&rec->t1, (unsigned long long)wall_clock64());
     }
 }
 
@@ -194,7 +201,7 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
     const dim3 grid((unsigned)(cu_count() * bpc)), block(PT_TRACE_BLOCK);
 #define PT_LAUNCH_TRACE(S, M, W) \
     hipLaunchKernelGGL((k_trace_persistent<S, M, W>), grid, block, 0, st, b, job, cursor, stats)
-    if (stats) {
+    if (stats || (job.launch_log && job.log_counts)) {
         if (multi) {
             if (wide) PT_LAUNCH_TRACE(true, true, true);
             else PT_LAUNCH_TRACE(true, true, false);

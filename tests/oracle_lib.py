@@ -54,9 +54,13 @@ def lib():
                                 ctypes.c_int, ctypes.c_uint32, ctypes.c_int, _i, ctypes.c_int64, _f, _l]
     L.oracle_pixel_seed.restype = ctypes.c_uint32
     L.oracle_pixel_seed.argtypes = [ctypes.c_uint32] * 3
+    L.oracle_sample_seed.restype = ctypes.c_uint32
+    L.oracle_sample_seed.argtypes = [ctypes.c_uint32] * 4
     L.oracle_xorshift.argtypes = [ctypes.c_uint32, ctypes.c_int, _u, _f]
     L.oracle_camera_ray.argtypes = [_f, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_uint32, ctypes.c_uint32, _f]
+    L.oracle_sample_camera_ray.argtypes = [_f, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_uint32, ctypes.c_uint32, _f]
     L.oracle_next_ray.argtypes = [ctypes.c_int, _f, _f, _f, ctypes.c_uint32, ctypes.c_uint32, _f]
     L.oracle_bsdf_eval.argtypes = [ctypes.c_int, _f, _f, _f, _f, _f]
     L.oracle_triangle_intersect.restype = ctypes.c_int
@@ -219,6 +223,10 @@ def pixel_seed(job, pixel, stream):
     return lib().oracle_pixel_seed(job, pixel, stream)
 
 
+def sample_seed(job, pixel, sample, stream):
+    return lib().oracle_sample_seed(job, pixel, sample, stream)
+
+
 def xorshift(seed, n):
     u = np.zeros(n, np.uint32)
     f = np.zeros(n, np.float32)
@@ -229,6 +237,13 @@ def xorshift(seed, n):
 def camera_ray(cam, i, j, w, h, s0, s1):
     out = np.zeros(10, np.float32)
     lib().oracle_camera_ray(_ptr(f32(cam), _f), i, j, w, h, s0, s1, _ptr(out, _f))
+    return out
+
+
+def sample_camera_ray(cam, i, j, w, h, job_seed, sample=0):
+    """The camera ray of sample `sample` of pixel (i, j) as the renderers draw it."""
+    out = np.zeros(10, np.float32)
+    lib().oracle_sample_camera_ray(_ptr(f32(cam), _f), i, j, w, h, job_seed, sample, _ptr(out, _f))
     return out
 
 

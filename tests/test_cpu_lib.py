@@ -53,7 +53,7 @@ def test_primary_emission_pixel_is_camera_throughput_times_le():
         assert q == w * h
         for (x, y) in [(0, 0), (5, 3), (15, 7)]:
             pix = y * w + x
-            r = O.camera_ray(sc["camera"], x, y, w, h, O.pixel_seed(7, pix, 1), O.pixel_seed(7, pix, 2))
+            r = O.sample_camera_ray(sc["camera"], x, y, w, h, 7)
             le = sc["mat_param"][0, :3]
             expect = (r[6:9] * le).astype(np.float32)
             assert np.array_equal(img[y, x].view(np.uint32), expect.view(np.uint32))
@@ -68,7 +68,7 @@ def test_empty_scene_pixel_is_throughput_times_base_colour():
     o = O.OracleScene(sc).build_bvh(16)
     img, q = o.render(8, 8, 1, 4, 0, 3, 1)
     pix = 2 * 8 + 5
-    r = O.camera_ray(sc["camera"], 5, 2, 8, 8, O.pixel_seed(3, pix, 1), O.pixel_seed(3, pix, 2))
+    r = O.sample_camera_ray(sc["camera"], 5, 2, 8, 8, 3)
     assert np.array_equal(img[2, 5], (r[6:9] * sc["env_param"][:3]).astype(np.float32))
 
 

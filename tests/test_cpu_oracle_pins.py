@@ -55,7 +55,7 @@ def camera_rays(cam, w, h, job_seed):
     for y in range(h):
         for x in range(w):
             pix = y * w + x
-            out[y, x] = O.camera_ray(cam, x, y, w, h, O.pixel_seed(job_seed, pix, 1), O.pixel_seed(job_seed, pix, 2))
+            out[y, x] = O.sample_camera_ray(cam, x, y, w, h, job_seed)
     return out
 
 

@@ -45,9 +45,9 @@ def test_timed_render_matches_graph_render(gpu):
 
 
 def test_autotune_picks_a_width_whose_batch_equals_plain(gpu):
-    """The autotuner times the calibration batch over BVHs of both widths and returns one
-    width's jobs and launch shapes; the batch rendered with them equals the per-job plain
-    renders of the same BVHs."""
+    """The autotuner keeps to binary BVHs unless told otherwise; asked to, it times the
+    calibration batch over BVHs of both widths and returns one width's jobs and launch
+    shapes; the batch rendered with them equals the per-job plain renders of the same BVHs."""
     from petrichor_amd import autotune
     c = ctx()
     scs = [c.scene(scenes.cornell_box()),
@@ -57,7 +57,9 @@ def test_autotune_picks_a_width_whose_batch_equals_plain(gpu):
     for w in (2, 8):
         bv = [s.build_bvh(16, width=w) for s in scs]
         by_width[w] = [P.Job(bv[i], W, H, spp, d, 0, seed) for (i, W, H, spp, d, seed) in specs]
-    best = autotune.tune(c, None, pool_sizes=[4096], passes=0, by_width=by_width)
+    exact = autotune.tune(c, None, pool_sizes=[4096], passes=0, by_width=by_width)
+    assert exact["width"] == 2  # the 8-wide walk is not the reference's: a candidate only on request
+    best = autotune.tune(c, None, pool_sizes=[4096], passes=0, by_width=by_width, exact_only=False)
     log("autotune", {k: v for k, v in best.items() if k != "jobs"})
     assert best["width"] in (2, 8) and best["jobs"] is by_width[best["width"]]
     eng = P.Engine(c, best["max_paths"], max_jobs=4)

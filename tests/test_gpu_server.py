@@ -68,7 +68,7 @@ def test_requests_join_a_running_one_and_previews_progress(gpu):
     srv = P.Server(8 * GiB, max_paths=1 << 20, max_jobs=16, iterations_per_sync=2)
     sp = small_spheres(4)
     sid = srv.add_scene(sp)
-    long_req = (512, 384, 512, 12, P.INTEGRATOR_MIS, 11)
+    long_req = (512, 384, 2048, 12, P.INTEGRATOR_MIS, 11)  # 400M samples on half the pool
     rl = srv.submit(sid, *long_req)
     t0 = time.time()
     while srv.poll(rl)["state"] == P.REQ_QUEUED and time.time() - t0 < 60:
@@ -113,7 +113,7 @@ def test_budget_admission_and_eviction(gpu):
     with pytest.raises(P.PtError):
         srv.submit(ids[0], 16, 16, 1, 2)  # evicted
     with pytest.raises(P.PtError):
-        srv.submit(ids[1], 512, 512, 1, 2)  # more pixels than the path pool
+        srv.submit(ids[1], 512, 512, 64, 2)  # its image and samples (12 B each) exceed the budget
     with pytest.raises(P.PtError):
         srv.add_scene(scenes.random_triangles(400000, seed=9))  # larger than the whole budget
     r = srv.submit(ids[2], 48, 48, 2, 4, P.INTEGRATOR_MIS, 2)
