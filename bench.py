@@ -290,8 +290,13 @@ def run_batch(args):
                 for i, (b, sp) in enumerate(zip(bvhs, specs))]
 
     out_b = outs()
-    best = autotune.tune(ctx, None, log=log, by_width={w: jobs_with(out_b, bvhs_by_width[w]) for w in widths},
-                         exact_only=widths == (2,))
+    if args.no_tune:
+        w0 = widths[0]
+        best = {"width": w0, "jobs": jobs_with(out_b, bvhs_by_width[w0]), "max_paths": args.pool,
+                "tuning": dict(autotune.DEFAULT), "trials": 0, "seconds": 0.0}
+    else:
+        best = autotune.tune(ctx, None, log=log, by_width={w: jobs_with(out_b, bvhs_by_width[w]) for w in widths},
+                             exact_only=widths == (2,))
     jobs = best["jobs"]
     bvhs = bvhs_by_width[best["width"]]
     eng = P.Engine(ctx, best["max_paths"], max_jobs=len(jobs))
@@ -304,6 +309,9 @@ def run_batch(args):
     t_batch = time.perf_counter() - t0
     log("batched: %.2f s, %d queries, %d iterations, %d syncs" % (t_batch, st["queries"], st["iterations"],
                                                                    st["syncs"]))
+    if args.batch_only:
+        return {"batched_jobs_s": round(len(specs) / t_batch, 3), "batched_s": round(t_batch, 3),
+                "queries": int(st["queries"])}
     out_p = outs()
     t0 = time.perf_counter()
     q_plain = 0
@@ -394,7 +402,7 @@ def run_serve(args):
     geo = [scenes.scaled_scene(specs[i]["tris"], seed=5000 + i, env=env) for i in range(pool_n)]
     log("%d requests over %.0f s; %d scene geometries generated in %.1f s" % (n_req, args.duration, pool_n,
                                                                             time.perf_counter() - t0))
-    srv = P.Server(int(args.budget_gb * (1 << 30)), max_paths=1 << 23, max_jobs=128, iterations_per_sync=8)
+    srv = P.Server(int(args.budget_gb * (1 << 30)), max_paths=1 << 22, max_jobs=128, iterations_per_sync=8)
     lock = threading.Lock()
     resident = []          # scene ids added so far (some may be evicted later)
     outstanding = {}       # request id -> (index, arrival, submission), seconds from the start
@@ -643,6 +651,9 @@ def main():
                          "8 the 8-wide tree (traversal microbenchmark default); batch mode: both are "
                          "built and the autotuner picks one unless this is given")
     ap.add_argument("--jobs", type=int, default=64, help="batch mode: number of jobs")
+    ap.add_argument("--no-tune", action="store_true", help="batch mode: default launch shapes, --pool path slots")
+    ap.add_argument("--pool", type=int, default=1 << 23, help="batch mode with --no-tune: path slots")
+    ap.add_argument("--batch-only", action="store_true", help="batch mode: only the batched run (no per-job runs)")
     ap.add_argument("--tris-max", type=float, default=2e6, help="batch/serve modes: largest scene")
     ap.add_argument("--spp-max", type=int, default=256, help="batch/serve modes: largest spp")
     ap.add_argument("--rate", type=float, default=20.0, help="serve mode: Poisson arrival rate (requests/s)")

@@ -291,7 +291,7 @@ typedef struct pt_server_desc {
     int32_t device;
     int32_t max_jobs;            /* requests running at once (0 = 64) */
     int64_t hbm_budget_bytes;    /* device memory for scenes, BVHs, the path pool and request images */
-    int64_t max_paths;           /* path slots shared by running requests (0 = 8M) */
+    int64_t max_paths;           /* path slots shared by running requests (0 = 4M) */
     int32_t iterations_per_sync; /* graph replays between host synchronisations (0 = 8) */
 } pt_server_desc;
 

@@ -4,8 +4,9 @@ Chooses, for a given set of render jobs, the BVH node width (when the jobs are g
 BVHs of both widths), the path-pool size (the capacity of every ray queue), the graph
 replays per host synchronisation and the launch shapes of the trace and shading kernels
 (pt_tuning). It times pt_render_batch on a calibration batch - the same scenes and
-resolutions at a few samples per pixel - first for each width with the default launch
-shapes, then with coordinate descent over the space below for each pool size. Every
+resolutions at a few samples per pixel - first for each width and then for each pool size
+with the default launch shapes, then with coordinate descent over the space below on the
+best pool size. Every
 candidate of one width renders the same images (tuning only changes launch shapes and
 scheduling), so the fastest one is taken as is.
 
@@ -26,9 +27,10 @@ SPACE = {
     "shade_blocks_per_cu": [4, 8, 16],
 }
 
-# Path-pool sizes tried (at least the largest job's pixel count). Larger pools keep more
-# scenes in flight at once, whose BVHs then compete for L2 and the Infinity Cache.
-POOLS = (1 << 22, 6 << 20, 1 << 23)
+# Path-pool sizes tried. Larger pools keep more jobs, and so more scenes, in flight at once,
+# whose BVHs and path state then compete for L2 and the Infinity Cache (the 64-job batch:
+# 2.71 s with 2M slots, 2.67 s with 4M, 3.38 s with 8M).
+POOLS = (1 << 21, 1 << 22, 6 << 20)
 
 
 def calibration_jobs(jobs, spp=2):
@@ -37,7 +39,7 @@ def calibration_jobs(jobs, spp=2):
     return [Job(j.bvh, j.width, j.height, min(spp, j.spp), j.max_bounces, j.integrator, j.seed, j.out) for j in jobs]
 
 
-def tune(ctx, jobs, pool_sizes=None, passes=1, log=None, by_width=None, calib_spp=2, exact_only=True):
+def tune(ctx, jobs, pool_sizes=None, passes=1, log=None, by_width=None, calib_spp=8, exact_only=True):
     """Returns {"width", "jobs", "max_paths", "tuning", "seconds", "trials"} for the fastest
     configuration found on the calibration batch of `jobs`. by_width (optional) maps a BVH
     width (2, 8) to the same jobs over BVHs of that width; "jobs" is then the chosen list.
@@ -49,9 +51,8 @@ def tune(ctx, jobs, pool_sizes=None, passes=1, log=None, by_width=None, calib_sp
         if not by_width:
             raise ValueError("exact_only: the jobs need binary BVHs (width 2)")
     jobs = next(iter(by_width.values()))
-    need = max(j.width * j.height for j in jobs)
     if pool_sizes is None:
-        pool_sizes = sorted({max(need, p) for p in POOLS})
+        pool_sizes = list(POOLS)
     trials = []
 
     def timer(eng, calib, pool):
@@ -82,26 +83,37 @@ def tune(ctx, jobs, pool_sizes=None, passes=1, log=None, by_width=None, calib_sp
         width = best_w[0]
     chosen = by_width[width]
     calib = calibration_jobs(chosen, calib_spp)
+    # the pool size with the default launch shapes, then coordinate descent over the launch
+    # shapes on the best pool
     best = None
     for pool in pool_sizes:
         eng = Engine(ctx, pool, max_jobs=max(len(chosen), 1))
         measure = timer(eng, calib, pool)
         measure(DEFAULT)  # first use: graph instantiation and caches
-        cur, cur_t = dict(DEFAULT), measure(DEFAULT)
-        for _ in range(passes):
-            for key, values in SPACE.items():
-                for v in values:
-                    if v == cur[key]:
-                        continue
-                    cand = dict(cur, **{key: v})
-                    t = measure(cand)
-                    if t < cur_t:
-                        cur, cur_t = cand, t
+        t = measure(DEFAULT)
         if log:
-            log("autotune: pool %d -> %s, %.1f ms" % (pool, cur, cur_t * 1e3))
-        if best is None or cur_t < best["seconds"]:
-            best = {"max_paths": pool, "tuning": cur, "seconds": cur_t}
+            log("autotune: pool %d -> %.1f ms" % (pool, t * 1e3))
+        if best is None or t < best["seconds"]:
+            best = {"max_paths": pool, "tuning": dict(DEFAULT), "seconds": t}
         eng.close()
+    pool = best["max_paths"]
+    eng = Engine(ctx, pool, max_jobs=max(len(chosen), 1))
+    measure = timer(eng, calib, pool)
+    measure(DEFAULT)
+    cur, cur_t = dict(DEFAULT), measure(DEFAULT)
+    for _ in range(passes):
+        for key, values in SPACE.items():
+            for v in values:
+                if v == cur[key]:
+                    continue
+                cand = dict(cur, **{key: v})
+                t = measure(cand)
+                if t < cur_t:
+                    cur, cur_t = cand, t
+    eng.close()
+    if log:
+        log("autotune: pool %d -> %s, %.1f ms" % (pool, cur, cur_t * 1e3))
+    best = {"max_paths": pool, "tuning": cur, "seconds": cur_t}
     best["width"] = width
     best["jobs"] = chosen
     best["trials"] = len(trials)

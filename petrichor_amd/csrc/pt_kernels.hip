@@ -596,8 +596,8 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
 // Per-wave ballots compact the new rays into the queues and the surviving paths into
 // the next live list.
 template <int INTEGRATOR, bool MULTI>
-__device__ __forceinline__ void shade_live_paths(const SceneView& sc1, const JobView& job1, const JobDev* jobs,
-                                                 const ShadeArgs& A)
+__device__ __forceinline__ void shade_live_paths(const SceneView& sc1, const JobView& job1,
+                                                 const JobDev* __restrict__ jobs, const ShadeArgs& A)
 {
     const int n = *A.active_count;
     const int stride = gridDim.x * blockDim.x;
@@ -608,15 +608,29 @@ __device__ __forceinline__ void shade_live_paths(const SceneView& sc1, const Job
         ShadeOut o;
         o.keep = o.want_ext = o.want_any = o.want_closest = o.sample_end = false;
         const JobDev* J = nullptr;
-        if (in_range) {
-            if (MULTI) {
+        if (MULTI) {
+            // A wave's paths usually belong to one job (a job's slots are contiguous and its
+            // paths stay together in the live list). Then the job's scene and settings are
+            // read at a wave-uniform address, with scalar loads; otherwise per lane.
+            const int my_job = in_range ? A.ps.job[p] : -1;
+            const int j0 = __builtin_amdgcn_readfirstlane(my_job);
+            const bool uniform = __ballot(in_range && my_job != j0) == 0ull;
+            if (uniform && j0 >= 0) {
+                const JobDev* Ju = jobs + j0;
+                J = Ju;
+                if (Ju->job.integrator == 0) {
+                    if (in_range) shade_one<0>(Ju->sc, Ju->job, A, p, o);
+                } else {
+                    if (in_range) shade_one<1>(Ju->sc, Ju->job, A, p, o);
+                }
+            } else if (in_range) {
                 // the integrator is a per-job setting here
-                J = jobs + A.ps.job[p];
+                J = jobs + my_job;
                 if (J->job.integrator == 0) shade_one<0>(J->sc, J->job, A, p, o);
                 else shade_one<1>(J->sc, J->job, A, p, o);
-            } else {
-                shade_one<INTEGRATOR>(sc1, job1, A, p, o);
             }
+        } else if (in_range) {
+            shade_one<INTEGRATOR>(sc1, job1, A, p, o);
         }
         // a finished sample is counted and the path takes the job's next item, if any
         {
@@ -657,7 +671,7 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc, JobView job, ShadeA
 }
 
 // The multi-job form: the integrator is a per-job setting.
-__global__ void __launch_bounds__(256) k_shade_multi(const JobDev* jobs, ShadeArgs A)
+__global__ void __launch_bounds__(256) k_shade_multi(const JobDev* __restrict__ jobs, ShadeArgs A)
 {
     shade_live_paths<0, true>(SceneView{}, JobView{}, jobs, A);
 }

@@ -331,7 +331,7 @@ int pt_server_create(const pt_server_desc* d, pt_server** out)
 {
     if (!d || !out) return sfail(PT_ERR_ARG, "null argument");
     *out = nullptr;
-    const int64_t max_paths = d->max_paths > 0 ? d->max_paths : ((int64_t)8 << 20);
+    const int64_t max_paths = d->max_paths > 0 ? d->max_paths : ((int64_t)4 << 20);
     const int max_jobs = d->max_jobs > 0 ? d->max_jobs : 64;
     if (d->hbm_budget_bytes <= 0) return sfail(PT_ERR_ARG, "the server needs a device-memory budget");
     const int64_t pool = max_paths * kPathBytes + (int64_t)max_jobs * 512;

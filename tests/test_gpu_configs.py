@@ -142,7 +142,7 @@ def test_c4_batch_of_64_jobs(gpu):
         bvhs.append(c.scene(scenes.scaled_scene(sp["tris"], seed=1000 + i, env=env)).build_bvh(16))
     jobs = [P.Job(bv, sp["width"], sp["height"], sp["spp"], sp["depth"], P.INTEGRATOR_MIS, 7 + i)
             for i, (bv, sp) in enumerate(zip(bvhs, specs))]
-    eng = P.Engine(c, 1 << 23, max_jobs=64)
+    eng = P.Engine(c, 1 << 22, max_jobs=64)
     t0 = time.perf_counter()
     st = eng.render_batch(jobs)
     log("C4 batch: %.2f s, %d queries" % (time.perf_counter() - t0, st["queries"]))
@@ -169,7 +169,7 @@ def test_c5_served_poisson_stream(gpu):
     env = scenes.sky_env()
     pool_n = min(n, 24)
     geo = [scenes.scaled_scene(specs[i]["tris"], seed=5000 + i, env=env) for i in range(pool_n)]
-    srv = P.Server(200 << 30, max_paths=1 << 23, max_jobs=128, iterations_per_sync=8)
+    srv = P.Server(200 << 30, max_paths=1 << 22, max_jobs=128, iterations_per_sync=8)
     scene_geo, live, reqs = {}, [], []
     log("C5 %d requests over %.0f s" % (n, duration))
     t0 = time.perf_counter()
