@@ -5,11 +5,11 @@
 // of petrichor_hip.h (libpetrichor_hip.so).
 //
 // Only what the GPU path renders is here: triangles (flat or smooth), Lambert, GGX,
-// Glass, Emission and MixMaterial, an equirect environment given as a Texture2D,
-// the MIS and simple integrators, Texture2D::Save (PNG bytes converted on the GPU,
-// Radiance HDR), and the reference's file loading: JSON scene and render-setting files,
-// OBJ meshes and Radiance .hdr environment maps. Image textures and the denoising AOVs are
-// not part of this API.
+// Glass, Emission and MixMaterial with their image textures (Lambert albedo, GGX
+// reflectance and roughness maps), an equirect environment given as a Texture2D, the MIS
+// and simple integrators, the denoising albedo / normal and UV AOVs, Texture2D::Save (PNG
+// bytes converted on the GPU, Radiance HDR), and the reference's file loading: JSON scene
+// and render-setting files, OBJ meshes, PNG textures and Radiance .hdr environment maps.
 //
 // Floating point: the camera set-up below must be compiled without FMA contraction
 // (-ffp-contract=off) to give the reference's camera state bit for bit.

@@ -2,6 +2,7 @@
 // build, trace entry points and the plain per-frame wavefront render loop.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -14,6 +15,15 @@
 #include "pt_wide.h"
 
 using namespace pt;
+
+int64_t pt_scene_map_bytes(const pt_scene_desc* d)
+{
+    if (!d->material_maps || d->num_textures <= 0 || !d->texture_sizes) return 0;
+    int64_t b = (int64_t)d->num_materials * (int64_t)sizeof(int2) + (int64_t)d->num_textures * (int64_t)sizeof(TexDev);
+    for (int t = 0; t < d->num_textures; ++t)
+        b += 12 * (int64_t)std::max(d->texture_sizes[2 * t], 0) * (int64_t)std::max(d->texture_sizes[2 * t + 1], 0);
+    return b;
+}
 
 int pt_fail(pt_context* ctx, int code, const std::string& msg)
 {
@@ -378,7 +388,8 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
     if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
     s->device_bytes = (int64_t)(s->mats.size() * (sizeof(MaterialDev) + 4) + lights.size() * sizeof(float4)) +
                       (d->env_rgb ? 3 * (int64_t)e.w * e.h * 4 : 0) +
-                      (int64_t)nt * 40 + (int64_t)(s->tri_smooth.size() * 4 + s->tri_nrm.size() * 4);
+                      (int64_t)nt * 40 + (int64_t)(s->tri_smooth.size() * 4 + s->tri_nrm.size() * 4) +
+                      (int64_t)s->tri_uv.size() * 4 + pt_scene_map_bytes(d);
     if (err != hipSuccess) {
         for (float* p : s->tex_d) (void)hipFree(p);
         free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->mat_shading_of_d, s->tri_pos_d,

@@ -82,6 +82,8 @@ struct pt_bvh {
 
 // Sets the context's error text and returns the code.
 int pt_fail(pt_context* ctx, int code, const std::string& msg);
+// Device bytes of a scene's material-map textures and tables (pt_scene_create).
+int64_t pt_scene_map_bytes(const pt_scene_desc* d);
 
 #define PT_HIP(ctx, call)                                                                              \
     do {                                                                                               \

@@ -122,3 +122,25 @@ def test_budget_admission_and_eviction(gpu):
     srv.remove_scene(ids[2])
     assert not srv.resident(ids[2])
     srv.close()
+
+
+def test_scene_textures_are_charged_to_the_budget(gpu):
+    """A scene's material-map textures (12 bytes per texel) and texture coordinates count
+    against the server's budget like its triangles do."""
+    log("server: texture bytes in the budget")
+    base = small_spheres(3)
+    tex = dict(base)
+    n, m = len(base["tri_mat"]), len(base["mat_type"])
+    tex["tri_uv"] = np.zeros((n, 6), np.float32)
+    tex["textures"] = [np.full((1024, 2048, 3), 0.5, np.float32)]  # 24 MiB
+    maps = -np.ones((m, 2), np.int32)
+    maps[:, 0] = [0 if t == P.PT_MAT_LAMBERT else -1 for t in base["mat_type"]]
+    tex["mat_maps"] = maps
+    srv = P.Server(8 * GiB, max_paths=1 << 16, max_jobs=4)
+    b0 = srv.stats()["bytes_in_use"]
+    srv.add_scene(base)
+    b1 = srv.stats()["bytes_in_use"]
+    srv.add_scene(tex)
+    b2 = srv.stats()["bytes_in_use"]
+    srv.close()
+    assert (b2 - b1) - (b1 - b0) >= 12 * 1024 * 2048 + 24 * n
