@@ -6,13 +6,16 @@
 // colour texture then goes through ApplyGamma, powf(x, 2.2) per channel
 // (Core/Texture2D.cpp:44-56). stb is not part of this library: the PNG format is decoded
 // here from its specification (zlib inflate, the five scanline filters), for
-// non-interlaced images of bit depth 8 or 16 and every colour type.
+// non-interlaced images of every colour type and bit depth: samples of 1, 2 or 4 bits are
+// unpacked (grey levels scaled to 8 bits by 255 / (2^depth - 1), palette indices as they
+// are), 16-bit samples reduced to their high byte.
 #include <zlib.h>
 
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -28,10 +31,8 @@ int paeth(int a, int b, int c)
     return pa <= pb && pa <= pc ? a : (pb <= pc ? b : c);
 }
 
-} // namespace
-
-extern "C" int pt_png_decode(const uint8_t* data, int64_t len, int32_t color, int32_t* width, int32_t* height,
-                             float* rgb_out, char* err, int32_t err_len)
+int decode(const uint8_t* data, int64_t len, int32_t color, int32_t* width, int32_t* height, float* rgb_out,
+           char* err, int32_t err_len)
 {
     auto fail = [&](const std::string& msg) {
         if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "PNG: %s", msg.c_str());
@@ -67,13 +68,21 @@ extern "C" int pt_png_decode(const uint8_t* data, int64_t len, int32_t color, in
     if (w == 0 || h == 0 || w > 65536 || h > 65536 || ctype < 0 || ctype > 6 || kChannels[ctype] == 0)
         return fail("bad header");
     if (interlace != 0) return fail("interlaced images are not supported");
-    if (!(depth == 8 || (depth == 16 && ctype != 3))) return fail("bit depth " + std::to_string(depth) + " is not supported");
+    // the depths the format allows per colour type
+    const bool depth_ok = (ctype == 0 && (depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16)) ||
+                          (ctype == 3 && (depth == 1 || depth == 2 || depth == 4 || depth == 8)) ||
+                          ((ctype == 2 || ctype == 4 || ctype == 6) && (depth == 8 || depth == 16));
+    if (!depth_ok) return fail("bit depth " + std::to_string(depth) + " is not valid for colour type " +
+                               std::to_string(ctype));
     if (ctype == 3 && plte.size() < 3) return fail("palette missing");
     *width = (int32_t)w;
     *height = (int32_t)h;
     if (!rgb_out) return PT_OK; // size query
-    const int ch = kChannels[ctype], bps = depth / 8, bpp = ch * bps;
-    const size_t stride = (size_t)w * bpp;
+    const int ch = kChannels[ctype];
+    // bytes per row and the filters' byte distance to the previous pixel (at least 1)
+    const size_t stride = ((size_t)w * ch * depth + 7) / 8;
+    const int bpp = depth >= 8 ? ch * (depth / 8) : 1, bps = depth >= 8 ? depth / 8 : 0;
+    if ((stride + 1) * (size_t)h > ((size_t)1 << 32)) return fail("image too large");
     std::vector<uint8_t> raw((stride + 1) * h);
     uLongf raw_len = (uLongf)raw.size();
     if (uncompress(raw.data(), &raw_len, idat.data(), (uLong)idat.size()) != Z_OK || raw_len != raw.size())
@@ -102,8 +111,16 @@ extern "C" int pt_png_decode(const uint8_t* data, int64_t len, int32_t color, in
     for (uint32_t y = 0; y < h; ++y) {
         const uint8_t* cur = &raw[(stride + 1) * y + 1];
         for (uint32_t x = 0; x < w; ++x) {
-            // 8-bit samples of this pixel: the high byte of 16-bit ones
-            auto sample = [&](int c) { return cur[(size_t)x * bpp + (size_t)c * bps]; };
+            // 8-bit samples of this pixel: the high byte of 16-bit ones; a packed sample of
+            // 1, 2 or 4 bits (most significant bits first) as it is
+            auto sample = [&](int c) -> int {
+                if (depth >= 8) return cur[(size_t)x * bpp + (size_t)c * bps];
+                const size_t bit = (size_t)x * depth;
+                return (cur[bit / 8] >> (8 - depth - (int)(bit % 8))) & ((1 << depth) - 1);
+            };
+            // grey levels of fewer than 8 bits scaled to 8 bits (stb's depth scale: x255,
+            // x85, x17)
+            const int grey_scale = depth == 1 ? 255 : depth == 2 ? 85 : depth == 4 ? 17 : 1;
             uint8_t rgb[3];
             if (ctype == 3) {
                 const size_t i = 3 * (size_t)sample(0);
@@ -112,7 +129,7 @@ extern "C" int pt_png_decode(const uint8_t* data, int64_t len, int32_t color, in
                 rgb[1] = plte[i + 1];
                 rgb[2] = plte[i + 2];
             } else if (ctype == 0 || ctype == 4) {
-                rgb[0] = rgb[1] = rgb[2] = sample(0);
+                rgb[0] = rgb[1] = rgb[2] = (uint8_t)(sample(0) * grey_scale);
             } else {
                 rgb[0] = sample(0);
                 rgb[1] = sample(1);
@@ -127,4 +144,17 @@ extern "C" int pt_png_decode(const uint8_t* data, int64_t len, int32_t color, in
         }
     }
     return PT_OK;
+}
+
+} // namespace
+
+extern "C" int pt_png_decode(const uint8_t* data, int64_t len, int32_t color, int32_t* width, int32_t* height,
+                             float* rgb_out, char* err, int32_t err_len)
+{
+    try {
+        return decode(data, len, color, width, height, rgb_out, err, err_len);
+    } catch (const std::bad_alloc&) {
+        if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "PNG: not enough host memory for the image");
+        return PT_ERR_OOM;
+    }
 }

@@ -1,6 +1,6 @@
 """PNG decoding as Texture2D::Load reads textures (pt_png_decode): 8-bit RGB as stbi_load
 gives it, bytes / 255, powf(x, 2.2) for colour textures. Files are encoded here with every
-scanline filter, colour type and bit depth the decoder takes."""
+scanline filter, colour type and bit depth (1 to 16) the format allows."""
 import ctypes
 import struct
 import zlib
@@ -23,15 +23,26 @@ def paeth(a, b, c):
     return a if pa <= pb and pa <= pc else (b if pb <= pc else c)
 
 
+def pack_row(values, depth):
+    """One row of samples of 1, 2 or 4 bits, most significant bits first, padded to a byte."""
+    out = bytearray((len(values) * depth + 7) // 8)
+    for x, v in enumerate(values):
+        bit = x * depth
+        out[bit // 8] |= int(v) << (8 - depth - bit % 8)
+    return out
+
+
 def encode(samples, ctype, depth=8, plte=None, interlace=0):
     """PNG of samples [h, w, channels] (uint8 or uint16), row y filtered with type y % 5."""
     h, w, ch = samples.shape
-    bps = depth // 8
-    bpp = ch * bps
+    bpp = max(1, ch * depth // 8)
     raw = bytearray()
-    prev = bytearray(w * bpp)
+    prev = bytearray((w * ch * depth + 7) // 8)
     for y in range(h):
-        row = bytearray(samples[y].astype(">u2" if depth == 16 else np.uint8).tobytes())
+        if depth < 8:
+            row = pack_row(samples[y, :, 0], depth)
+        else:
+            row = bytearray(samples[y].astype(">u2" if depth == 16 else np.uint8).tobytes())
         f = y % 5
         out = bytearray(len(row))
         for x in range(len(row)):
@@ -75,6 +86,12 @@ def test_colour_types_filters_and_depths(color):
     plte = rng.integers(0, 256, (16, 3), dtype=np.uint8)
     idx = rng.integers(0, 16, (h, w, 1), dtype=np.uint8)
     cases["palette"] = (encode(idx, 3, plte=plte), plte[idx[..., 0]])
+    # packed samples: grey levels scaled to 8 bits (x255, x85, x17), palette indices unpacked
+    for d in (1, 2, 4):
+        g = rng.integers(0, 1 << d, (h, w, 1), dtype=np.uint8)
+        cases["gray%d" % d] = (encode(g, 0, depth=d), np.repeat(g * (255 // ((1 << d) - 1)), 3, 2).astype(np.uint8))
+        k = rng.integers(0, 1 << d, (h, w, 1), dtype=np.uint8)
+        cases["palette%d" % d] = (encode(k, 3, depth=d, plte=plte[: 1 << d]), plte[k[..., 0]])
     for name, (png, want8) in cases.items():
         got = loader.decode_png(png, color=color)
         assert got.shape == (h, w, 3), name
@@ -86,11 +103,13 @@ def test_rejected_files():
     for bad in (b"not a png", encode(rgb, 2, interlace=1), encode(rgb, 2)[:40]):
         with pytest.raises(ValueError):
             loader.decode_png(bad)
-    # bit depth 4 (a grey image) is not read
-    png = bytearray(encode(np.zeros((2, 2, 1), np.uint8), 0))
-    png[24] = 4
-    with pytest.raises(ValueError):
-        loader.decode_png(bytes(png))
+    # bit depth 4 is not valid for an RGB image; 8-bit data under a 4-bit grey header has
+    # the wrong size
+    for ctype, samples in ((2, rgb), (0, np.zeros((2, 2, 1), np.uint8))):
+        png = bytearray(encode(samples, ctype))
+        png[24] = 4
+        with pytest.raises(ValueError):
+            loader.decode_png(bytes(png))
 
 
 def test_png_environment_in_a_scene_file(tmp_path):
