@@ -990,7 +990,8 @@ static int bvh_intersect(const Scene& sc, const BVH& bvh, const Ray& ray, float 
 {
     const V inv = divv(mk(1.0f, 1.0f, 1.0f), ray.dir);
     const int sign[3] = {inv.x < 0.0f, inv.y < 0.0f, inv.z < 0.0f};
-    int stack[1024];
+    // unbounded, as the reference's stack: any tree depth
+    std::vector<int> stack(64);
     int sp = 0;
     stack[sp++] = 0;
     int best = -1;
@@ -1022,7 +1023,7 @@ static int bvh_intersect(const Scene& sc, const BVH& bvh, const Ray& ray, float 
             const int l = node.child[0], r = node.child[1];
             const float dl = sqlen(sub(bvh.nodes[l].box.centroid(), ray.o));
             const float dr = sqlen(sub(bvh.nodes[r].box.centroid(), ray.o));
-            if (sp + 2 > 1024) abort();
+            if (sp + 2 > (int)stack.size()) stack.resize(2 * stack.size());
             if (dl < dr) {
                 stack[sp++] = r;
                 stack[sp++] = l;

@@ -72,6 +72,51 @@ MaterialDev material_from(int type, const float* p)
 
 namespace {
 
+// Trees deeper than the traversal stack (PT_MAX_STACK - 2 levels) are walked without one,
+// by parent links (pt_traverse.h). The links are made here on the host from the packed
+// nodes and checked to form one tree over every inner node, rooted at the root, so the
+// walk up always ends; a tree that fails the check is reported as an internal error.
+int add_parents(pt_bvh* b)
+{
+    if (b->max_depth <= PT_MAX_STACK - 2 || b->num_inner == 0) return PT_OK;
+    pt_context* ctx = b->scene->ctx;
+    const size_t ni = (size_t)b->num_inner;
+    const uint32_t root = b->view.root_ref;
+    std::vector<float4> nodes(4 * ni);
+    PT_HIP(ctx, hipMemcpyAsync(nodes.data(), b->nodes_d, nodes.size() * sizeof(float4), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    PT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (pt_ref_cnt(root) != 0 || pt_ref_index(root) >= ni) return fail(ctx, PT_ERR_STATE, "deep tree: bad root");
+    std::vector<int32_t> parent(ni, -2);
+    parent[pt_ref_index(root)] = -1;
+    // breadth first from the root: every inner node is reached exactly once
+    std::vector<uint32_t> level(1, pt_ref_index(root)), nextl;
+    size_t reached = 1;
+    while (!level.empty()) {
+        nextl.clear();
+        for (const uint32_t i : level) {
+            const float4 m = nodes[4 * (size_t)i + 3];
+            for (const float f : {m.x, m.y}) {
+                uint32_t ref;
+                std::memcpy(&ref, &f, 4);
+                if (pt_ref_cnt(ref) != 0) continue; // a leaf
+                const uint32_t c = pt_ref_index(ref);
+                if (c >= ni || parent[c] != -2) return fail(ctx, PT_ERR_STATE, "deep tree: bad child link");
+                parent[c] = (int32_t)i;
+                nextl.push_back(c);
+                ++reached;
+            }
+        }
+        level.swap(nextl);
+    }
+    if (reached != ni) return fail(ctx, PT_ERR_STATE, "deep tree: unreachable inner nodes");
+    PT_HIP(ctx, upload(&b->parents_d, parent.data(), ni, ctx->stream));
+    PT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    b->view.parents = b->parents_d;
+    b->device_bytes += (int64_t)ni * 4;
+    return PT_OK;
+}
+
 int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
 {
     pt_context* ctx = s->ctx;
@@ -83,10 +128,6 @@ int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
         gpu_bvh_free(&r);
         return fail(ctx, e == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
                     std::string("device BVH build: ") + hipGetErrorString(e));
-    }
-    if (r.max_depth > PT_MAX_STACK - 2) {
-        gpu_bvh_free(&r);
-        return fail(ctx, PT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     }
     GpuPacked pk;
     e = gpu_pack_bvh(r, s->num_tris, s->tri_pos_d, s->tri_mat_d, s->mat_shading_of_d, s->tri_smooth_d, s->tri_nrm_d,
@@ -129,6 +170,10 @@ int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
     const int64_t nt = s->num_tris;
     b->device_bytes = (int64_t)pk.num_inner * 72 + (int64_t)pk.num_big * 8 + nt * 60 +
                       (pk.smooth_d ? nt * 4 : 0) + (pk.nrm_d ? nt * 36 : 0);
+    if (const int rc = add_parents(b)) {
+        pt_bvh_destroy(b);
+        return rc;
+    }
     *out = b;
     return PT_OK;
 }
@@ -458,10 +503,6 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
         delete b;
         return fail(ctx, PT_ERR_ARG, "bad BVH parameters");
     }
-    if (b->host.max_depth > PT_MAX_STACK - 2) {
-        delete b;
-        return fail(ctx, PT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
-    }
     b->host_ready = true;
     b->num_nodes = (int)b->host.nodes.size();
     b->max_depth = b->host.max_depth;
@@ -522,6 +563,10 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
                       (int64_t)ranges.size() * 8 +
                       (int64_t)nt * 12 + (int64_t)(smooth.size() * 4 + nrm.size() * 4);
     b->num_inner = (int)(nodes.size() / 4);
+    if (const int rc = add_parents(b)) {
+        pt_bvh_destroy(b);
+        return rc;
+    }
     *out = b;
     if (width == 8) return make_wide(b, out);
     return PT_OK;
@@ -532,7 +577,7 @@ int pt_bvh_destroy(pt_bvh* b)
     if (!b) return PT_ERR_ARG;
     free_all({b->nodes8_d, b->nodes_d, b->big_leaves_d, b->inner_range_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
               b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d, b->order_keys_d, b->order_idx_d, b->order_temp_d,
-              b->tri_uv_d});
+              b->tri_uv_d, b->parents_d});
     delete b;
     return PT_OK;
 }

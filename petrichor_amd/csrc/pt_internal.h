@@ -66,6 +66,7 @@ struct pt_bvh {
     float* tri_nrm_d = nullptr;
     uint4* nodes8_d = nullptr; // 8-wide nodes (width 8 only)
     float* tri_uv_d = nullptr; // 6 floats per leaf slot, when the scene has texture coordinates
+    int32_t* parents_d = nullptr; // inner nodes' parents, for trees deeper than the traversal stack
     int width = 2;
     int num_nodes8 = 0;
     int depth8 = 0;

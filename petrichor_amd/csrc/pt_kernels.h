@@ -25,6 +25,9 @@ struct BvhView {
     const int2* inner_range = nullptr; // leaf-slot range of every binary inner node (ties, 8-wide)
     const uint4* nodes8 = nullptr;  // 8-wide nodes (pt_wide.hip), null for the binary traversal
     uint32_t root8 = PT_REF_EMPTY;  // root of the 8-wide tree (wide node 0, or a leaf)
+    // parent of every binary inner node (-1 for the root), set only for trees deeper than
+    // the traversal stack, which are then walked without one
+    const int32_t* parents = nullptr;
 };
 
 struct SceneView {

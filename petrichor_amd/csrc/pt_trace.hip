@@ -175,7 +175,7 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
 // the 64-job batch). The single-job binary walk needs 61 by itself; the 8-wide walk is
 // LDS-bound at 5 blocks per CU.
 template <bool STATS, bool MULTI, bool WIDE>
-__global__ void __launch_bounds__(PT_TRACE_BLOCK, (MULTI && !WIDE) ? 8 : 1) k_trace_persistent(BvhView b1, TraceJob job,
+__global__ void __launch_bounds__(PT_TRACE_BLOCK, !WIDE ? 8 : 1) k_trace_persistent(BvhView b1, TraceJob job,
                                                                                   int* cursor,
                                                                                   unsigned long long* stats)
 {

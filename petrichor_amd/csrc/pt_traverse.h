@@ -23,6 +23,14 @@
 // A step handles one inner node or one triangle (8-wide: one node or one whole leaf).
 // Each lane keeps its traversal stack in LDS (the first lds_stack_entries entries) with
 // deeper entries in private memory.
+//
+// Binary trees deeper than that stack (BvhView::parents set) are walked without one, in
+// the same order: the stack only ever holds the far children of the ancestors of the
+// current entry whose near child was entered while the far one was hit, deepest on top.
+// So when an entry is done, the next one is found by walking up the parent links to the
+// nearest ancestor entered through its near child whose far child the ray hits; its
+// distance and inside bit are the same slab and containment tests the push would have
+// stored. The lane's first LDS stack slot holds the current entry's parent.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -128,7 +136,8 @@ __device__ __forceinline__ bool trace_begin(const BvhView& b, const float4 r0, c
 __device__ __noinline__ bool reference_first(const BvhView& b, const TraceRay& r, int a, int c)
 {
     uint32_t ref = b.root_ref;
-    for (int depth = 0; depth < PT_MAX_STACK; ++depth) {
+    // down the binary tree: ends at a leaf or where a and c part, within its depth
+    for (int depth = 0; depth <= (int)PT_REF_INDEX_MASK; ++depth) {
         if (pt_ref_cnt(ref) != 0) return a < c;
         const float4* nd = b.nodes + 4 * (size_t)pt_ref_index(ref);
         const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], m = nd[3];
@@ -206,6 +215,42 @@ __device__ __forceinline__ void enter_leaf(const BvhView& b, TraceState& s)
 // inner nodes do similar work per step), then, when the step did not move to a child
 // (`next`) and no triangle is left, the next entry from the stack. Returns true when the
 // ray is finished.
+// Deep binary trees: moves to the entry the stack would pop next (see the header) and
+// returns true, or returns false when there is none (the ray is done).
+__device__ __forceinline__ bool deep_next(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds)
+{
+    uint32_t child = s.cur & ~PT_REF_INSIDE;
+    if (child == b.root_ref) return false;
+    uint32_t p = lds[0].x; // parent of the entry just done
+    // the parent links form a tree rooted at root_ref (checked when they are made), so
+    // the walk ends at the root; the bound only caps a walk over links that do not
+    for (int guard = 0; guard <= (int)PT_REF_INDEX_MASK; ++guard) {
+        const float4* nd = b.nodes + 4 * (size_t)p;
+        const float4 a = nd[0], bb = nd[1], c = nd[2], mf = nd[3];
+        const uint32_t ml = __float_as_uint(mf.x), mr = __float_as_uint(mf.y);
+        const V3 llo = v3(a.x, a.y, a.z), lhi = v3(a.w, bb.x, bb.y);
+        const V3 rlo = v3(bb.z, bb.w, c.x), rhi = v3(c.y, c.z, c.w);
+        const float dl = sq_length(0.5f * (llo + lhi) - r.o);
+        const float dr = sq_length(0.5f * (rlo + rhi) - r.o);
+        const bool left_first = dl < dr;
+        if (child == (left_first ? ml : mr)) {
+            // entered through the near child: the far one is next if the ray hits it
+            const V3 flo = left_first ? rlo : llo, fhi = left_first ? rhi : lhi;
+            float d2;
+            if (slab(r.o, r.inv, r.sx, r.sy, r.sz, flo, fhi, &d2)) {
+                s.cur = (left_first ? mr : ml) | (box_contains(flo, fhi, r.o) ? PT_REF_INSIDE : 0u);
+                s.cur_d = d2;
+                lds[0].x = p;
+                return true;
+            }
+        }
+        if (p == b.root_ref) return false;
+        child = p; // an inner node's reference is its index
+        p = (uint32_t)b.parents[p];
+    }
+    return false;
+}
+
 template <bool STATS, bool WIDE>
 __device__ __forceinline__ bool leaf_or_pop(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
                                             StackEntry* spill, bool next)
@@ -221,6 +266,7 @@ __device__ __forceinline__ bool leaf_or_pop(const BvhView& b, const TraceRay& r,
         next = false;
     }
     if (!next) {
+        if (!WIDE && b.parents) return !deep_next(b, r, s, lds);
         if (s.sp == 0) return true;
         --s.sp;
         stack_pop<WIDE>(lds, spill, s.sp, &s.cur, &s.cur_d);
@@ -260,17 +306,15 @@ __device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, 
         const bool h1 = left_first ? hl : hr, h2 = left_first ? hr : hl;
         const uint32_t e1 = left_first ? el : er, e2 = left_first ? er : el;
         const float d1 = left_first ? tl : tr_, d2 = left_first ? tr_ : tl;
-        if (h1) {
-            if (h2) {
+        if (h1 || h2) {
+            if (b.parents) {
+                lds[0].x = pt_ref_index(s.cur); // deep tree: the parent of the child entered
+            } else if (h1 && h2) {
                 stack_push<false>(lds, spill, s.sp, e2, d2);
                 ++s.sp;
             }
-            s.cur = e1;
-            s.cur_d = d1;
-            next = true;
-        } else if (h2) {
-            s.cur = e2;
-            s.cur_d = d2;
+            s.cur = h1 ? e1 : e2;
+            s.cur_d = h1 ? d1 : d2;
             next = true;
         }
     }

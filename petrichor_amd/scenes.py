@@ -259,6 +259,47 @@ def scaled_scene(n_tris, seed, env=None):
     return spheres_scene(seed=seed, n_spheres=n_spheres, seg=seg, bands=bands, terrain_n=terrain_n, env=env)
 
 
+def deep_chain(n=120):
+    """A scene whose binned-SAH tree with 2 bins is about n levels deep: n triangles with
+    centroids x = 2^-i (i = 0..n-1), y = 1, z = 2^-i / 48 (distinct depths, so no two are
+    hit at the same distance), each 2^-i / 2 wide, 3 tall and tilted so that its box is
+    2^-i / 8 thick (no box is flat, which slab tests meet only within rounding). With
+    two bins every split puts the rightmost centroid alone in the upper bin, so each level
+    peels one triangle. Lambert 0.7 under a uniform 8 x 4 sky of radiance 1; the camera
+    looks at the chain's wide end from z = -3."""
+    x = np.exp2(-np.arange(n, dtype=np.float64))
+    w = 0.25 * x
+    z = x / 64
+    y = np.zeros_like(x)
+    t = np.stack([x - w, y, z, x + w, y, z, x, y + 3.0, z + 0.5 * w], 1)
+    cam = camera(pos=(0.4, 1.0, -3.0), look_at=(0.4, 1.0, 0.0), focal_length=35e-3, aperture=0.0)
+    env = np.ones((4, 8, 3), np.float32)
+    return _finish(t, [_material(PT_MAT_LAMBERT, (0.7, 0.7, 0.7))], np.zeros(n, np.int32), [], cam, env,
+                   (1, 1, 1, 0, 1))
+
+
+def deep_chain_rays(tri_pos, seed=5):
+    """Rays for deep_chain: one along +z through every triangle's centroid (the walk to
+    the deepest leaves), one along -z from behind through each centroid offset by a
+    quarter of its width, and 512 random rays from in front of the plane towards x in
+    [2^-16, 1] (farther in, a ray from z = -2 meets the chain only within rounding)."""
+    rng = np.random.default_rng(seed)
+    t = tri_pos.reshape(-1, 3, 3).astype(np.float64)
+    c = t.mean(1)
+    w = (t[:, 1, 0] - t[:, 0, 0]) / 2
+    r = []
+    for k in range(len(c)):
+        r.append([c[k, 0], c[k, 1], -1.0, 0.0, 0.0, 0.0, 1.0, np.finfo(np.float32).max])
+        r.append([c[k, 0] + 0.25 * w[k], c[k, 1] + 0.5, 1.0, 0.0, 0.0, 0.0, -1.0, np.finfo(np.float32).max])
+    o = np.stack([rng.uniform(-0.2, 1.4, 512), rng.uniform(-0.5, 3.5, 512), np.full(512, -2.0)], 1)
+    tgt = np.stack([np.exp2(-rng.uniform(0, 16, 512)), rng.uniform(0, 3, 512), np.zeros(512)], 1)
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    for k in range(512):
+        r.append([*o[k], 0.0, *d[k], np.finfo(np.float32).max])
+    return np.asarray(r, np.float32)
+
+
 # ---- Config 3: random triangle soup + rays ---------------------------------------------------
 def random_triangles(n, seed=3, mu=-6.0, sigma=1.0):
     """n triangles with centroids uniform in [0,1]^3; vertex offsets a, b, -(a+b) with

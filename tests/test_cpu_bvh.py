@@ -165,3 +165,23 @@ def test_host_builder_32_bins_and_edge_cases():
     assert len(hs["leaf"]) == 1 and hs["leaf"][0] == 1
     empty = P.build_bvh_host(np.zeros((0, 9), np.float32))
     assert len(empty["leaf"]) == 1
+
+
+def test_deep_tree_host_builder_and_oracle():
+    """A 117-level tree (two bins over the halving chain): the product's host builder gives
+    the oracle's node table, and the oracle's walk (unbounded stack) finds the brute-force
+    hits on the chain's rays, down to its deepest leaf."""
+    sc = scenes.deep_chain(120)
+    e = O.OracleScene(sc).build_bvh(2).bvh_export()
+    assert e["max_depth"] == 117
+    h = P.build_bvh_host(sc["tri_pos"], 2)
+    for k in ("boxes", "child", "range", "leaf", "prim_ids"):
+        assert np.array_equal(e[k], h[k]), k
+    assert h["max_depth"] == 117
+    o = O.OracleScene(sc).build_bvh(2)
+    r = scenes.deep_chain_rays(sc["tri_pos"])
+    t1, p1, _ = o.trace_closest(r)
+    t2, p2 = o.trace_brute(r)
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(t1.view(np.uint32), t2.view(np.uint32))
+    assert np.array_equal(p1[0:240:2], np.arange(120))
