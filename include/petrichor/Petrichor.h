@@ -129,8 +129,9 @@ private:
 };
 
 // Core/Material/GGX.h:13-164 with metalness 0. A reflectance (F0) texture multiplies the
-// reflectance; a roughness map replaces the roughness by the luminance of the map. Normal
-// maps are not rendered: Render reports a material that has one.
+// reflectance; a roughness map replaces the roughness by strength x the luminance of the
+// map; a normal map replaces the normal the GGX functions use by GetNormal's, in the frame
+// of the hit's tangent (GGX.h:84-119).
 class GGX : public MaterialBase
 {
 public:
@@ -141,13 +142,19 @@ public:
     void SetF0Texture(const Texture2D* texture) { m_reflectanceMap = texture; }
     void SetRoughnessMap(const Texture2D* roughnessMap) { m_roughnessMap = roughnessMap; }
     void SetNormalMap(const Texture2D* normalMap) { m_normalMap = normalMap; }
+    void SetNormalMapStrength(float strength) { m_normalMapStrength = std::clamp(strength, 0.0f, 1.0f); }
+    void SetRoughnessMapStrength(float strength) { m_roughnessMapStrength = strength; }
     const Texture2D* GetF0Texture() const { return m_reflectanceMap; }
     const Texture2D* GetRoughnessMap() const { return m_roughnessMap; }
     const Texture2D* GetNormalMap() const { return m_normalMap; }
+    float GetNormalMapStrength() const { return m_normalMapStrength; }
+    float GetRoughnessMapStrength() const { return m_roughnessMapStrength; }
 
 private:
     Color3f m_f0;
     float m_roughness;
+    float m_normalMapStrength = 1.0f;
+    float m_roughnessMapStrength = 1.0f;
     const Texture2D* m_reflectanceMap = nullptr;
     const Texture2D* m_roughnessMap = nullptr;
     const Texture2D* m_normalMap = nullptr;
@@ -1101,7 +1108,9 @@ public:
 private:
     struct Flat {
         std::vector<float> pos, nrm, uv, matParam, env;
-        std::vector<int32_t> matId, smooth, matType, lights, matMaps, texSizes;
+        std::vector<int32_t> matId, smooth, matType, lights, matMaps, matNormalMaps, texSizes;
+        std::vector<float> matStrengths;
+        bool anyNormalMap = false, anyStrength = false;
         std::vector<const float*> texPixels;
         std::unordered_map<const Texture2D*, int32_t> texIndex;
         bool anyMap = false;
@@ -1122,7 +1131,8 @@ private:
         auto it = index.find(m);
         if (it != index.end()) return it->second;
         float p[8] = {0, 0, 0, 0, 0, 0, m->IsImportanceSamplingEnabled() ? 1.0f : 0.0f, 0};
-        int32_t maps[2] = {-1, -1};
+        int32_t maps[2] = {-1, -1}, normalMap = -1;
+        float strengths[2] = {1.0f, 1.0f};
         auto rgb = [&](const Color3f& c) {
             p[0] = c.x;
             p[1] = c.y;
@@ -1138,10 +1148,14 @@ private:
             const auto* g = static_cast<const GGX*>(m);
             rgb(g->GetReflectance());
             p[3] = g->GetRoughness();
-            if (g->GetNormalMap()) return -2;
             maps[0] = texture_index(g->GetF0Texture(), f);
             maps[1] = texture_index(g->GetRoughnessMap(), f);
-            if (maps[0] == -3 || maps[1] == -3) return -3;
+            normalMap = texture_index(g->GetNormalMap(), f);
+            if (maps[0] == -3 || maps[1] == -3 || normalMap == -3) return -3;
+            strengths[0] = g->GetNormalMapStrength();
+            strengths[1] = g->GetRoughnessMapStrength();
+            f.anyNormalMap |= normalMap >= 0;
+            f.anyStrength |= strengths[0] != 1.0f || strengths[1] != 1.0f;
             break;
         }
         case MaterialTypes::Glass:
@@ -1165,6 +1179,8 @@ private:
         f.matType.push_back(type_code(m->GetMaterialType()));
         f.matParam.insert(f.matParam.end(), p, p + 8);
         f.matMaps.insert(f.matMaps.end(), maps, maps + 2);
+        f.matNormalMaps.push_back(normalMap);
+        f.matStrengths.insert(f.matStrengths.end(), strengths, strengths + 2);
         index[m] = id;
         return id;
     }
@@ -1209,7 +1225,6 @@ private:
             }
             if (anySmooth) f.smooth.push_back(t->GetShadingType() == ShadingTypes::Smooth ? 1 : 0);
             const int32_t mi = material_index(t->GetMaterial(), matIndex, f);
-            if (mi == -2) return fail("normal maps are not rendered by the GPU path");
             if (mi == -3) return fail("a material texture is empty");
             f.matId.push_back(mi);
         }
@@ -1222,6 +1237,8 @@ private:
             f.matType.push_back(type_code(MaterialTypes::Lambert));
             f.matParam.assign(8, 0.0f);
             f.matMaps.assign(2, -1);
+            f.matNormalMaps.assign(1, -1);
+            f.matStrengths.assign(2, 1.0f);
         }
         pt_scene_desc& d = f.desc;
         d.num_triangles = (int32_t)f.matId.size();
@@ -1253,7 +1270,9 @@ private:
             d.texture_sizes = f.texSizes.data();
             d.textures = f.texPixels.data();
             d.material_maps = f.matMaps.data();
+            if (f.anyNormalMap) d.material_normal_maps = f.matNormalMaps.data();
         }
+        if (f.anyStrength) d.material_map_strengths = f.matStrengths.data();
         return true;
     }
 

@@ -77,6 +77,13 @@ typedef struct pt_scene_desc {
     const int32_t* texture_sizes;
     const float* const* textures;
     const int32_t* material_maps;
+    /* GGX normal maps (GGX::SetNormalMap, Core/Material/GGX.h:50-54,96-119): one texture
+     * index per material, read by GGX materials only, -1 for none; NULL for none. */
+    const int32_t* material_normal_maps;
+    /* Map strengths, 2 per material: GGX::SetNormalMapStrength (GGX.h:84-87; the setter
+     * clamps it to [0, 1], values outside are rejected here) and SetRoughnessMapStrength
+     * (GGX.h:89-93); NULL for 1 and 1, the reference's defaults. */
+    const float* material_map_strengths;
 } pt_scene_desc;
 
 typedef struct pt_bvh_desc {
@@ -95,7 +102,7 @@ typedef struct pt_render_desc {
     int32_t spp;           /* numSamplesPerPixel */
     int32_t max_bounces;   /* numMaxBounces */
     int32_t integrator;    /* PT_INTEGRATOR_* */
-    uint32_t seed;         /* job seed of the per-pixel sampler streams */
+    uint32_t seed;         /* job seed of the per-sample sampler streams */
     int32_t flags;         /* PT_RENDER_* */
 } pt_render_desc;
 

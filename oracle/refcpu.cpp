@@ -233,6 +233,11 @@ struct Material {
     bool is_enabled = true;       // MaterialBase::m_isImportanceSamplingEnabled
     V albedo = {1.0f, 1.0f, 1.0f}; // Lambert::GetAlbedo: one, or kd x the albedo map (Lambert.cpp:117-126)
     int map0 = -1, map1 = -1;     // albedo (Lambert) or reflectance (GGX) map; roughness map (GGX)
+    int nmap = -1;                // GGX normal map (GGX::SetNormalMap)
+    float nstrength = 1.0f;       // GGX::m_normalMapStrength (GGX.h:159)
+    float rstrength = 1.0f;       // GGX::m_roughnessMapStrength (GGX.h:160)
+    bool has_gn = false;          // GGX::GetNormal at the shading point differs from its normal:
+    V gn = {0.0f, 0.0f, 0.0f};    // the normal-mapped normal (set by material_at)
 };
 
 static Material make_material(int type, const float* p)
@@ -251,11 +256,16 @@ static Material make_material(int type, const float* p)
     return m;
 }
 
-struct Shading { // Core/HitInfo.h:20-27 (the tangent is not read: no normal maps)
+struct Shading { // Core/HitInfo.h:20-27
     V normal, pos;
+    V tangent = {0.0f, 0.0f, 0.0f};
     V uv = {0.0f, 0.0f, 0.0f};
     int material = -1;
 };
+
+// The normal the GGX functions use: GGX::GetNormal (GGX.h:96-119), the normal-mapped one
+// when the material has a normal map.
+static inline V ggx_normal(const Material& m, const Shading& si) { return m.has_gn ? m.gn : si.normal; }
 
 // GGX::GetF0 (GGX.h:126-139) with metalness 0, base colour 1, no reflectance map:
 // 0.16*r^2*(1-0) + 1*0, evaluated as written.
@@ -283,7 +293,7 @@ static V bxdf(const Material& m, V din, V dout, const Shading& si)
     case kLambert:
         return scl(mul(m.color, m.albedo), kInvPi); // m_kd * GetAlbedo(si) * kInvPi
     case kGlossy: {
-        const V normal = si.normal;
+        const V normal = ggx_normal(m, si);
         const V half = normalized(add(neg(din), dout));
         const float hn = std::fabs(dot(half, normal));
         const float alpha = m.alpha;
@@ -318,7 +328,7 @@ static float pdf(const Material& m, V din, V dout, const Shading& si)
         return 0.5f * kInvPi;
     case kGlossy: {
         const V half = normalized(add(neg(din), dout));
-        const float hn = std::fabs(dot(half, si.normal));
+        const float hn = std::fabs(dot(half, ggx_normal(m, si)));
         const float alpha = m.alpha;
         const float alpha2 = alpha * alpha;
         const float hn2 = hn * hn;
@@ -333,7 +343,7 @@ static float pdf(const Material& m, V din, V dout, const Shading& si)
 // GGX::SampleGGXVNDF (GGX.cpp:155-208), Heitz 2017 visible-normal sampling.
 static V ggx_sample_vndf(const Material& m, V view, const Shading& si, Sampler2D& s2)
 {
-    const V n0 = si.normal;
+    const V n0 = ggx_normal(m, si);
     const float hit_sign = dot(view, n0);
     const V normal = scl(n0, std::copysign(1.0f, hit_sign));
     const V t = (std::fabs(normal.z) < 0.9999f) ? cross(normal, mk(0.0f, 0.0f, 1.0f))
@@ -419,7 +429,7 @@ static Ray next_ray(const Material& m, const Ray& in, const Shading& si, Sampler
         return out;
     }
     case kGlossy: {
-        const V n0 = si.normal;
+        const V n0 = ggx_normal(m, si);
         const float hit_sign = -dot(in.dir, n0);
         const V normal = scl(n0, std::copysign(1.0f, hit_sign));
         const V h = ggx_sample_vndf(m, neg(in.dir), si, s2);
@@ -768,28 +778,50 @@ static Shading interpolate(const Scene& sc, int prim, const Ray& ray, float t)
         V n = add(add(scl(sub(tr.n1, tr.n0), w1), scl(sub(tr.n2, tr.n0), w2)), tr.n0);
         s.normal = divs(n, length(n));
     }
+    // the tangent: diffUV2.y e1 - diffUV1.y e2, made orthogonal to a smooth normal, UnitX
+    // when its squared length is 0 (Triangle.cpp:105-147)
+    const V duv1 = sub(tr.uv1, tr.uv0), duv2 = sub(tr.uv2, tr.uv0);
+    s.tangent = sub(scl(e1, duv2.y), scl(e2, duv1.y));
+    if (tr.smooth) s.tangent = sub(s.tangent, scl(s.normal, dot(s.normal, s.tangent)));
+    if (dot(s.tangent, s.tangent) == 0.0f) s.tangent = mk(1.0f, 0.0f, 0.0f);
     s.uv = hit_uv_of(tr, ray);
     s.material = sc.material_of(prim, 0.0f);
     return s;
+}
+
+// GGX::GetNormal with a normal map (GGX.h:96-119): the frame OrthonormalBasis::Build(normal,
+// tangent) (w, u normalised, v = w x u; OrthonormalBasis.cpp:29-35), l = normalize(2 t - 1)
+// of the texel t, then (1 - s) l.x U + l.y V + (1 - s) l.z W as written (not normalised).
+static V ggx_map_normal(V normal, V tangent, V texel, float s)
+{
+    const V w = normalized(normal), u = normalized(tangent);
+    const V v = cross(w, u);
+    const V l = normalized(sub(scl(texel, 2.0f), mk(1.0f, 1.0f, 1.0f)));
+    return add(add(scl(u, (1.0f - s) * l.x), scl(v, l.y)), scl(w, (1.0f - s) * l.z));
 }
 
 // GetLuminance (Core/Color3f.cpp:7-11)
 static inline float luminance(V c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
 // Material m at a shading point with its maps applied: Lambert::GetAlbedo = kd x albedo map,
-// GGX::GetF0's reflectance x reflectance map, GGX::GetAlpha = (1 x luminance of the
-// roughness map)^2 (Lambert.cpp:117-126, GGX.h:57-75,126-139); maps read with
-// GetPixelByUV's default bilinear fetch.
-static Material material_at(const Scene& sc, int m, const V& uv)
+// GGX::GetF0's reflectance x reflectance map, GGX::GetAlpha = (strength x luminance of the
+// roughness map)^2, GGX::GetNormal with a normal map (Lambert.cpp:117-126,
+// GGX.h:57-75,84-119,126-139); maps read with GetPixelByUV's default bilinear fetch.
+static Material material_at(const Scene& sc, int m, const Shading& si)
 {
+    const V& uv = si.uv;
     Material md = sc.mats[m];
     if (md.type == kLambert && md.map0 >= 0) {
         md.albedo = mul(md.color, sc.maps[md.map0].by_uv(uv.x, uv.y));
     } else if (md.type == kGlossy) {
         if (md.map0 >= 0) md.color = mul(md.color, sc.maps[md.map0].by_uv(uv.x, uv.y));
         if (md.map1 >= 0) {
-            const float r = 1.0f * luminance(sc.maps[md.map1].by_uv(uv.x, uv.y));
+            const float r = md.rstrength * luminance(sc.maps[md.map1].by_uv(uv.x, uv.y));
             md.alpha = r * r;
+        }
+        if (md.nmap >= 0) {
+            md.gn = ggx_map_normal(si.normal, si.tangent, sc.maps[md.nmap].by_uv(uv.x, uv.y), md.nstrength);
+            md.has_gn = true;
         }
     }
     return md;
@@ -1117,7 +1149,7 @@ static V light_contribution(Ctx& c, const Shading& si, Sampler1D& s1, Sampler2D&
     float pdf_area = 1.0f;
     V pl_pos, pl_nrm;
     const bool is_area = sample_light(c, p, s1.Next(), s2, &pdf_area, true, &pl_pos, &pl_nrm);
-    const Material mat = material_at(sc, si.material, si.uv);
+    const Material mat = material_at(sc, si.material, si);
     if (is_area) {
         if (sc.lights.empty()) return mk(0.0f, 0.0f, 0.0f);
         Ray to_light;
@@ -1197,7 +1229,7 @@ static V render_mis(Ctx& c, int px, int py, int w, int h, int spp, int max_bounc
         for (int b = 0; b < max_bounces; b++) {
             color = add(color, light_contribution(c, si, s1, s2, ray));
             prev = ray;
-            const Material mat = material_at(sc, si.material, si.uv);
+            const Material mat = material_at(sc, si.material, si);
             ray = next_ray(mat, ray, si, s2);
             if (approx_eq(sqlen(ray.throughput), 0.0f, kEps)) break;
             float t;
@@ -1266,7 +1298,7 @@ static V render_simple(Ctx& c, int px, int py, int w, int h, int spp, int max_bo
                 break;
             }
             const Shading si = interpolate(sc, hit, ray, t);
-            ray = next_ray(material_at(sc, m, si.uv), ray, si, s2);
+            ray = next_ray(material_at(sc, m, si), ray, si, s2);
             if (ray.bounce > max_bounces) {
                 ray.prob *= 0.9f;
                 ray.prob = smax(0.1f, ray.prob);
@@ -1308,7 +1340,7 @@ static V render_aov(Ctx& c, int px, int py, int w, int h, int spp, int max_bounc
             }
             const int m = sc.material_of(hit, s1.Next());
             const Shading si = interpolate(sc, hit, ray, t);
-            const Material mat = material_at(sc, m, si.uv);
+            const Material mat = material_at(sc, m, si);
             if (mat.type == kLambert) {
                 contrib = mul(ray.throughput, normal_aov ? si.normal : mat.albedo);
                 break;
@@ -1823,6 +1855,69 @@ void oracle_scene_set_maps(void* s, int num_textures, const int32_t* sizes, cons
     for (size_t m = 0; m < sc->mats.size(); ++m) {
         sc->mats[m].map0 = mat_maps[2 * m];
         sc->mats[m].map1 = mat_maps[2 * m + 1];
+    }
+}
+
+// GGX::GetNormal of a normal-mapped GGX for a given shading normal, tangent and texel.
+void oracle_ggx_map_normal(const float* normal, const float* tangent, const float* texel, float strength,
+                           float* out)
+{
+    const V g = ggx_map_normal(mk(normal[0], normal[1], normal[2]), mk(tangent[0], tangent[1], tangent[2]),
+                               mk(texel[0], texel[1], texel[2]), strength);
+    out[0] = g.x;
+    out[1] = g.y;
+    out[2] = g.z;
+}
+
+// Triangle::Interpolate's normal and tangent (Triangle.cpp:90-147) for the hit at distance t
+// of a ray (o, dir) on one triangle: positions (9 floats), vertex normals (9 floats, or
+// null: flat), texture coordinates (6 floats, or null: zero).
+void oracle_interpolate_frame(const float* pos, const float* nrm, const float* uv, const float* ray, float t,
+                              float* normal_out, float* tangent_out)
+{
+    Scene sc;
+    Tri tr;
+    tr.p0 = mk(pos[0], pos[1], pos[2]);
+    tr.p1 = mk(pos[3], pos[4], pos[5]);
+    tr.p2 = mk(pos[6], pos[7], pos[8]);
+    tr.smooth = nrm != nullptr;
+    if (nrm) {
+        tr.n0 = mk(nrm[0], nrm[1], nrm[2]);
+        tr.n1 = mk(nrm[3], nrm[4], nrm[5]);
+        tr.n2 = mk(nrm[6], nrm[7], nrm[8]);
+    }
+    if (uv) {
+        tr.uv0 = mk(uv[0], uv[1], 0.0f);
+        tr.uv1 = mk(uv[2], uv[3], 0.0f);
+        tr.uv2 = mk(uv[4], uv[5], 0.0f);
+    }
+    tr.material = 0;
+    sc.tris.push_back(tr);
+    Material m;
+    sc.mats.push_back(m);
+    Ray r;
+    r.o = mk(ray[0], ray[1], ray[2]);
+    r.dir = mk(ray[3], ray[4], ray[5]);
+    const Shading si = interpolate(sc, 0, r, t);
+    normal_out[0] = si.normal.x;
+    normal_out[1] = si.normal.y;
+    normal_out[2] = si.normal.z;
+    tangent_out[0] = si.tangent.x;
+    tangent_out[1] = si.tangent.y;
+    tangent_out[2] = si.tangent.z;
+}
+
+// GGX normal maps (one texture index per material, -1 none; or null) and map strengths (2
+// per material: normal, roughness; or null), over the textures of oracle_scene_set_maps.
+void oracle_scene_set_normal_maps(void* s, const int32_t* nmaps, const float* strengths)
+{
+    Scene* sc = (Scene*)s;
+    for (size_t m = 0; m < sc->mats.size(); ++m) {
+        if (nmaps) sc->mats[m].nmap = nmaps[m];
+        if (strengths) {
+            sc->mats[m].nstrength = strengths[2 * m];
+            sc->mats[m].rstrength = strengths[2 * m + 1];
+        }
     }
 }
 

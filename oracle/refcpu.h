@@ -42,6 +42,18 @@ void oracle_scene_set_uv(void* scene, const float* uv);
  * material (albedo or reflectance map, roughness map; -1 none). */
 void oracle_scene_set_maps(void* scene, int num_textures, const int32_t* sizes, const float* const* rgb,
                            const int32_t* mat_maps);
+/* GGX normal maps (one texture index per material, -1 none; or NULL) and map strengths
+ * (2 per material: normal, roughness; or NULL), over the textures set above. */
+void oracle_scene_set_normal_maps(void* scene, const int32_t* nmaps, const float* strengths);
+/* GGX::GetNormal of a normal-mapped GGX (GGX.h:96-119) for a shading normal, tangent, texel
+ * and strength. */
+void oracle_ggx_map_normal(const float* normal, const float* tangent, const float* texel, float strength,
+                           float* out);
+/* Triangle::Interpolate's normal and tangent (Triangle.cpp:90-147) at distance t along a ray
+ * (6 floats: o, dir) on one triangle (9 position floats; vertex normals or NULL for flat;
+ * 6 texture coordinates or NULL). */
+void oracle_interpolate_frame(const float* pos, const float* nrm, const float* uv, const float* ray, float t,
+                              float* normal_out, float* tangent_out);
 
 /* Binary binned-SAH BVH, Core/Accel/BinnedSAHBVH.cpp:15-225 (num_bins = 16 there). */
 void* oracle_bvh_build(void* scene, int num_bins);

@@ -39,7 +39,7 @@ class SceneDesc(ctypes.Structure):
         ("env_base_color", ctypes.c_float * 3), ("env_z_rotation", ctypes.c_float),
         ("env_importance_sampling", ctypes.c_int32), ("uvs", _f),
         ("num_textures", ctypes.c_int32), ("texture_sizes", _i), ("textures", ctypes.POINTER(_f)),
-        ("material_maps", _i),
+        ("material_maps", _i), ("material_normal_maps", _i), ("material_map_strengths", _f),
     ]
 
 
@@ -395,7 +395,7 @@ def scene_desc(scene):
     d.env_z_rotation = float(ep[3])
     d.env_importance_sampling = int(ep[4] != 0)
     d.uvs = _ptr(k(scene["tri_uv"], np.float32), _f) if scene.get("tri_uv") is not None else None
-    if scene.get("mat_maps") is not None:
+    if scene.get("mat_maps") is not None or scene.get("mat_nmaps") is not None:
         texs = [k(t, np.float32) for t in scene.get("textures", [])]
         sizes = k(np.array([[t.shape[1], t.shape[0]] for t in texs], np.int32).reshape(-1), np.int32)
         ptrs = (_f * max(len(texs), 1))(*[_ptr(t, _f) for t in texs])
@@ -403,7 +403,12 @@ def scene_desc(scene):
         d.num_textures = len(texs)
         d.texture_sizes = _ptr(sizes, _i)
         d.textures = ctypes.cast(ptrs, ctypes.POINTER(_f))
-        d.material_maps = _ptr(k(scene["mat_maps"], np.int32), _i)
+        if scene.get("mat_maps") is not None:
+            d.material_maps = _ptr(k(scene["mat_maps"], np.int32), _i)
+        if scene.get("mat_nmaps") is not None:
+            d.material_normal_maps = _ptr(k(scene["mat_nmaps"], np.int32), _i)
+    if scene.get("mat_map_strength") is not None:
+        d.material_map_strengths = _ptr(k(scene["mat_map_strength"], np.float32), _f)
     return d, keep
 
 

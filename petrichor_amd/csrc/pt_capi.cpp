@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -18,8 +19,10 @@ using namespace pt;
 
 int64_t pt_scene_map_bytes(const pt_scene_desc* d)
 {
-    if (!d->material_maps || d->num_textures <= 0 || !d->texture_sizes) return 0;
+    if (!(d->material_maps || d->material_normal_maps) || d->num_textures <= 0 || !d->texture_sizes) return 0;
     int64_t b = (int64_t)d->num_materials * (int64_t)sizeof(int2) + (int64_t)d->num_textures * (int64_t)sizeof(TexDev);
+    if (d->material_normal_maps) b += (int64_t)d->num_materials * 4;
+    if (d->material_map_strengths) b += (int64_t)d->num_materials * 8;
     for (int t = 0; t < d->num_textures; ++t)
         b += 12 * (int64_t)std::max(d->texture_sizes[2 * t], 0) * (int64_t)std::max(d->texture_sizes[2 * t + 1], 0);
     return b;
@@ -349,15 +352,23 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
         }
     }
     if (d->env_rgb && (d->env_width <= 0 || d->env_height <= 0)) return fail(ctx, PT_ERR_ARG, "bad env size");
-    if (d->material_maps) {
+    if (d->material_maps || d->material_normal_maps) {
         if (d->num_textures < 0 || (d->num_textures > 0 && (!d->texture_sizes || !d->textures)))
             return fail(ctx, PT_ERR_ARG, "bad material maps");
         for (int t = 0; t < d->num_textures; ++t)
             if (d->texture_sizes[2 * t] <= 0 || d->texture_sizes[2 * t + 1] <= 0 || !d->textures[t])
                 return fail(ctx, PT_ERR_ARG, "bad material map texture");
-        for (int i = 0; i < 2 * d->num_materials; ++i)
+        for (int i = 0; d->material_maps && i < 2 * d->num_materials; ++i)
             if (d->material_maps[i] < -1 || d->material_maps[i] >= d->num_textures)
                 return fail(ctx, PT_ERR_ARG, "material map index out of range");
+        for (int i = 0; d->material_normal_maps && i < d->num_materials; ++i)
+            if (d->material_normal_maps[i] < -1 || d->material_normal_maps[i] >= d->num_textures)
+                return fail(ctx, PT_ERR_ARG, "normal map index out of range");
+    }
+    for (int i = 0; d->material_map_strengths && i < d->num_materials; ++i) {
+        const float ns = d->material_map_strengths[2 * i], rs = d->material_map_strengths[2 * i + 1];
+        if (!(ns >= 0.0f && ns <= 1.0f) || !std::isfinite(rs))
+            return fail(ctx, PT_ERR_ARG, "normal map strength outside [0, 1] or roughness map strength not finite");
     }
     PT_HIP(ctx, hipSetDevice(ctx->device));
 
@@ -415,8 +426,9 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
     if (err == hipSuccess) err = upload(&s->tri_smooth_d, s->tri_smooth.data(), s->tri_smooth.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&s->tri_nrm_d, s->tri_nrm.data(), s->tri_nrm.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&s->tri_uv_d, s->tri_uv.data(), s->tri_uv.size(), ctx->stream);
-    if (err == hipSuccess && d->material_maps) {
-        // material maps: the textures, their table and two map indices per material
+    if (err == hipSuccess && (d->material_maps || d->material_normal_maps)) {
+        // material maps: the textures, their table and two map indices per material, the
+        // normal map of each material and the map strengths
         std::vector<TexDev> tex;
         for (int t = 0; t < d->num_textures && err == hipSuccess; ++t) {
             const int tw = d->texture_sizes[2 * t], th = d->texture_sizes[2 * t + 1];
@@ -425,11 +437,16 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
             s->tex_d.push_back(p);
             tex.push_back(TexDev{p, tw, th});
         }
-        std::vector<int2> maps(s->mats.size());
-        for (size_t m = 0; m < maps.size(); ++m) maps[m] = make_int2(d->material_maps[2 * m], d->material_maps[2 * m + 1]);
+        std::vector<int2> maps(s->mats.size(), make_int2(-1, -1));
+        for (size_t m = 0; d->material_maps && m < maps.size(); ++m)
+            maps[m] = make_int2(d->material_maps[2 * m], d->material_maps[2 * m + 1]);
         if (err == hipSuccess) err = upload(&s->textures_d, tex.data(), tex.size(), ctx->stream);
         if (err == hipSuccess) err = upload(&s->mat_maps_d, maps.data(), maps.size(), ctx->stream);
+        if (err == hipSuccess && d->material_normal_maps)
+            err = upload(&s->mat_nmaps_d, d->material_normal_maps, s->mats.size(), ctx->stream);
     }
+    if (err == hipSuccess && d->material_map_strengths)
+        err = upload(&s->map_strength_d, (const float2*)d->material_map_strengths, s->mats.size(), ctx->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
     s->device_bytes = (int64_t)(s->mats.size() * (sizeof(MaterialDev) + 4) + lights.size() * sizeof(float4)) +
                       (d->env_rgb ? 3 * (int64_t)e.w * e.h * 4 : 0) +
@@ -438,7 +455,8 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
     if (err != hipSuccess) {
         for (float* p : s->tex_d) (void)hipFree(p);
         free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->mat_shading_of_d, s->tri_pos_d,
-                  s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d, s->tri_uv_d, s->textures_d, s->mat_maps_d});
+                  s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d, s->tri_uv_d, s->textures_d, s->mat_maps_d,
+                  s->mat_nmaps_d, s->map_strength_d});
         delete s;
         return fail(ctx, err == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
                     std::string("scene upload: ") + hipGetErrorString(err));
@@ -452,7 +470,8 @@ int pt_scene_destroy(pt_scene* s)
     if (!s) return PT_ERR_ARG;
     for (float* p : s->tex_d) (void)hipFree(p);
     free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->mat_shading_of_d, s->tri_pos_d,
-              s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d, s->tri_uv_d, s->textures_d, s->mat_maps_d});
+              s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d, s->tri_uv_d, s->textures_d, s->mat_maps_d,
+              s->mat_nmaps_d, s->map_strength_d});
     delete s;
     return PT_OK;
 }

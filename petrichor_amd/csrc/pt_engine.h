@@ -344,6 +344,8 @@ struct pt_engine {
         sc.tri_uv = b->tri_uv_d;
         sc.textures = s->textures_d;
         sc.mat_maps = s->mat_maps_d;
+        sc.mat_nmaps = s->mat_nmaps_d;
+        sc.map_strength = s->map_strength_d;
         sc.mats = s->mats_d;
         sc.lights = s->lights_d;
         sc.num_lights = s->num_lights;

@@ -40,6 +40,8 @@ struct pt_scene {
     std::vector<float*> tex_d;         // material map textures
     pt::TexDev* textures_d = nullptr;
     int2* mat_maps_d = nullptr;        // per material, or null without maps
+    int* mat_nmaps_d = nullptr;        // per material: GGX normal map, or null without normal maps
+    float2* map_strength_d = nullptr;  // per material: normal / roughness map strength, or null (1, 1)
     int32_t* mat_shading_of_d = nullptr; // material -> material that shades it (mix resolved)
     int64_t device_bytes = 0; // materials, lights, environment, triangles
 };

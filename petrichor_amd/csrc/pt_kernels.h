@@ -39,6 +39,8 @@ struct SceneView {
     const float* tri_uv = nullptr;  // 6 floats per leaf slot (texture coordinates), or null
     const TexDev* textures = nullptr; // material map textures
     const int2* mat_maps = nullptr;   // per material: albedo / reflectance map, roughness map (-1: none); or null
+    const int* mat_nmaps = nullptr;   // per material: the normal map of a GGX (-1: none); or null
+    const float2* map_strength = nullptr; // per material: normal and roughness map strengths; or null (1, 1)
     const MaterialDev* mats;
     const float4* lights;           // light triangles, AppendLight order, 3 float4 each
     int num_lights;

@@ -90,8 +90,8 @@ __device__ __forceinline__ V3 hit_uv(const SceneView& s, int k, V3 o, V3 d)
 
 // Material m at a hit with texture coordinates uv, its maps applied: Lambert::GetAlbedo =
 // kd x the albedo map, GGX::GetF0's reflectance x the reflectance map, GGX::GetAlpha =
-// (strength x luminance of the roughness map)^2 with the default strength 1 (Lambert.cpp:
-// 117-126, GGX.h:57-75,126-139). Without maps the material as stored.
+// (strength x luminance of the roughness map)^2, the strength 1 unless set (Lambert.cpp:
+// 117-126, GGX.h:57-75,89-93,126-139). Without maps the material as stored.
 __device__ __forceinline__ MaterialDev material_at(const SceneView& s, int m, V3 uv)
 {
     MaterialDev md = load_mat(s, m);
@@ -102,11 +102,44 @@ __device__ __forceinline__ MaterialDev material_at(const SceneView& s, int m, V3
     } else if (md.type == kMatGlossy) {
         if (mm.x >= 0) md.color = md.color * tex_by_uv(s.textures[mm.x], uv.x, uv.y);
         if (mm.y >= 0) {
-            const float r = 1.0f * luminance(tex_by_uv(s.textures[mm.y], uv.x, uv.y));
+            const float k = s.map_strength != nullptr ? s.map_strength[m].y : 1.0f;
+            const float r = k * luminance(tex_by_uv(s.textures[mm.y], uv.x, uv.y));
             md.alpha = r * r;
         }
     }
     return md;
+}
+
+// The normal the GGX functions take at leaf slot k for material m (of type `type`) with
+// shading normal n and texture coordinates uv: GGX::GetNormal (GGX.h:96-119) when the
+// material has a normal map, else n. The tangent is Triangle::Interpolate's
+// (Triangle.cpp:110-147): diffUV2.y e1 - diffUV1.y e2, made orthogonal to a smooth
+// triangle's normal, UnitX when its squared length is 0. The frame is
+// OrthonormalBasis::Build(n, tangent) (OrthonormalBasis.cpp:29-35), the map's texel t
+// gives l = normalize(2 t - 1), and the normal is (1 - s) l.x U + l.y V + (1 - s) l.z W
+// with the strength s, not normalised again.
+__device__ __forceinline__ V3 ggx_map_normal_at(const SceneView& s, int m, int k, V3 n, V3 uv)
+{
+    const TriDev tr = load_tri(s.bvh.tris, k);
+    float du1 = 0.0f, du2 = 0.0f;
+    if (s.tri_uv != nullptr) {
+        const float* q = s.tri_uv + 6 * (size_t)k;
+        du1 = q[3] - q[1]; // diffUV1.y = uv1.v - uv0.v
+        du2 = q[5] - q[1]; // diffUV2.y = uv2.v - uv0.v
+    }
+    V3 tg = du2 * tr.e1 - du1 * tr.e2;
+    if (s.tri_smooth != nullptr && s.tri_smooth[k] != 0) tg = tg - dot(n, tg) * n;
+    if (sq_length(tg) == 0.0f) tg = v3(1.0f, 0.0f, 0.0f);
+    const V3 w = normalized(n), u = normalized(tg), v = cross(w, u);
+    const V3 t = tex_by_uv(s.textures[s.mat_nmaps[m]], uv.x, uv.y);
+    const V3 l = normalized(2.0f * t - v3(1.0f, 1.0f, 1.0f));
+    const float st = s.map_strength != nullptr ? s.map_strength[m].x : 1.0f;
+    return ((1.0f - st) * l.x) * u + l.y * v + ((1.0f - st) * l.z) * w;
+}
+__device__ __forceinline__ V3 ggx_map_normal(const SceneView& s, int m, int type, int k, V3 n, V3 uv)
+{
+    if (s.mat_nmaps == nullptr || type != kMatGlossy || s.mat_nmaps[m] < 0) return n;
+    return ggx_map_normal_at(s, m, k, n, uv);
 }
 // Texture coordinates of the hit, read only when the scene has maps.
 __device__ __forceinline__ V3 maps_uv(const SceneView& s, int k, V3 o, V3 d)
@@ -465,13 +498,14 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
                 sample_done = true;
             } else {
                 const int m = material_of(sc, k, next1(smp));
-                const MaterialDev mat = material_at(sc, m, maps_uv(sc, k, ray.o, ray.dir));
+                const V3 uv = maps_uv(sc, k, ray.o, ray.dir);
+                const MaterialDev mat = material_at(sc, m, uv);
                 if (mat.type == kMatEmission) {
                     color = color + ray.throughput * mat.color;
                     sample_done = true;
                 } else {
                     shade_point(sc, k, ray.o, ray.dir, t, &si_pos, &si_n);
-                    ray = next_ray(mat, ray, si_pos, si_n, smp);
+                    ray = next_ray(mat, ray, si_pos, ggx_map_normal(sc, m, mat.type, k, si_n, uv), smp);
                     bool cont = true;
                     if (ray.bounce > job.max_bounces) {
                         ray.prob *= 0.9f;
@@ -497,6 +531,9 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
         int new_pend = kPendNone;
         if (INTEGRATOR == 0 && at_surface) {
             const MaterialDev mat = material_at(sc, si_mat, si_uv);
+            // the GGX functions take the normal-mapped normal; the integrator's own cosines
+            // and offsets keep the shading normal (PathTracing.cpp:206-300)
+            const V3 bn = ggx_map_normal(sc, si_mat, mat.type, hit_k[p], si_n, si_uv);
             const float dd = -dot(ray.dir, si_n);
             const V3 pnee = si_pos + PT_EPS * copysignf_(1.0f, dd) * si_n;
             float pdf_area = 1.0f;
@@ -509,10 +546,10 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
                     sh_o = pnee;
                     sh_d = ldir;
                     new_pend = kPendArea;
-                    const V3 f = bxdf(mat, ray.dir, ldir, si_n);
+                    const V3 f = bxdf(mat, ray.dir, ldir, bn);
                     ps.pend_a[p] = st3(pl, pdf_area);
                     ps.pend_b[p] = st3(si_pos, fabsf_(dot(ldir, si_n)));
-                    ps.pend_c[p] = st3(ray.throughput, bsdf_pdf(mat, ray.dir, ldir, si_n));
+                    ps.pend_c[p] = st3(ray.throughput, bsdf_pdf(mat, ray.dir, ldir, bn));
                     ps.pend_d[p] = st3(f, 0.0f);
                     ps.pend_o[p] = st3(pnee, 0.0f);
                     ps.pend_dir[p] = st3(ldir, 0.0f);
@@ -527,20 +564,20 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
                 new_pend = kPendEnv;
                 const float cs = fabsf_(dot(edir, si_n));
                 const float sn = __builtin_sqrtf(smax(0.0f, 1.0f - cs * cs));
-                const float pb = 2.0f * PT_PI * PT_PI * sn * bsdf_pdf(mat, ray.dir, edir, si_n);
+                const float pb = 2.0f * PT_PI * PT_PI * sn * bsdf_pdf(mat, ray.dir, edir, bn);
                 const float mis = pdf_uv * pdf_uv / (pdf_uv * pdf_uv + pb * pb);
                 const V3 li = env_color(sc.env, edir);
-                const V3 f = bxdf(mat, ray.dir, edir, si_n);
+                const V3 f = bxdf(mat, ray.dir, edir, bn);
                 const V3 c = mis * ray.throughput * (li * f * sn * cs * 2.0f * PT_PI * PT_PI) / pdf_uv;
                 ps.pend_a[p] = st3(c, 0.0f);
             }
             const V3 prev_dir = ray.dir;
-            ray = next_ray(mat, ray, si_pos, si_n, smp);
+            ray = next_ray(mat, ray, si_pos, bn, smp);
             if (fabsf_(sq_length(ray.throughput) - 0.0f) < PT_EPS) {
                 if (new_pend != kPendNone) phase = kPhWait;
                 else sample_done = true;
             } else {
-                pdf_bsdf = bsdf_pdf(mat, prev_dir, ray.dir, si_n);
+                pdf_bsdf = bsdf_pdf(mat, prev_dir, ray.dir, bn);
                 const float cs = fabsf_(dot(si_n, ray.dir));
                 sin_n = __builtin_sqrtf(smax(0.0f, 1.0f - cs * cs));
                 want_ext = true;
@@ -705,7 +742,7 @@ __device__ __forceinline__ void trace_single(const BvhView& b, V3 o, float dmin,
 // The UV AOV (AOVUVCoordinate::Render, Core/AOV/AOVUVCoordinate.cpp:11-51) is mode 2: the
 // mean over spp camera rays of the first hit's texture coordinates (a miss adds zero).
 template <bool WIDE>
-__global__ void __launch_bounds__(256) k_render_aov(SceneView sc, int w, int h, int spp, int max_bounces,
+__global__ void __launch_bounds__(256, WIDE ? 1 : 4) k_render_aov(SceneView sc, int w, int h, int spp, int max_bounces,
                                                     uint32_t seed, int mode, float* out,
                                                     unsigned long long* queries)
 {
@@ -742,7 +779,8 @@ __global__ void __launch_bounds__(256) k_render_aov(SceneView sc, int w, int h, 
                 const int m = material_of(sc, k, next1(smp));
                 V3 pos, sn;
                 shade_point(sc, k, ray.o, ray.dir, t, &pos, &sn);
-                const MaterialDev mat = material_at(sc, m, maps_uv(sc, k, ray.o, ray.dir));
+                const V3 uv = maps_uv(sc, k, ray.o, ray.dir);
+                const MaterialDev mat = material_at(sc, m, uv);
                 if (mat.type == kMatLambert) {
                     contrib = ray.throughput * (normal_aov ? sn : mat.albedo);
                     break;
@@ -752,7 +790,7 @@ __global__ void __launch_bounds__(256) k_render_aov(SceneView sc, int w, int h, 
                     break;
                 }
                 if (mat.type != kMatGlass && mat.type != kMatGlossy) break; // Emission: zero
-                ray = next_ray(mat, ray, pos, sn, smp); // glass, smooth GGX
+                ray = next_ray(mat, ray, pos, ggx_map_normal(sc, m, mat.type, k, sn, uv), smp); // glass, smooth GGX
             }
             sum = sum + contrib;
         }

@@ -7,7 +7,9 @@
 // IN:  int32 n, int32 n_lights, float pos[9 n], int32 material[n] (0 white, 1 red,
 //      2 green, 3 light, 4 GGX), int32 lights[n_lights]; optionally float uv[6 n], int32
 //      tw, th, float texture[th][tw][3]: the vertices' uv, and the texture as the white
-//      Lambert's albedo texture and the GGX's F0 texture and roughness map
+//      Lambert's albedo texture and the GGX's F0 texture and roughness map; optionally
+//      then float normal_strength, roughness_strength: the texture is also the GGX's
+//      normal map, with those map strengths
 // OUT: float camera[16], then W*H*3 floats each of the image, the denoising albedo and
 //      normal AOVs (numSppForDenoising = SPP) and the UV AOV
 #include <cstdio>
@@ -43,6 +45,8 @@ int main(int argc, char** argv)
         texels.resize(ok ? 3 * (size_t)tsize[0] * tsize[1] : 0);
         ok = ok && std::fread(texels.data(), 4, texels.size(), in) == texels.size();
     }
+    float strengths[2] = {1.0f, 1.0f};
+    const bool normal_mapped = textured && ok && std::fread(strengths, 4, 2, in) == 2;
     std::fclose(in);
     if (!ok) return 2;
     Texture2D tex(textured ? tsize[0] : 0, textured ? tsize[1] : 0);
@@ -59,6 +63,11 @@ int main(int argc, char** argv)
         white->SetTexAlbedo(&tex);
         ggx->SetF0Texture(&tex);
         ggx->SetRoughnessMap(&tex);
+    }
+    if (normal_mapped) {
+        ggx->SetNormalMap(&tex);
+        ggx->SetNormalMapStrength(strengths[0]);
+        ggx->SetRoughnessMapStrength(strengths[1]);
     }
     scene.RegisterMaterial("white", std::move(white));
     scene.RegisterMaterial("red", std::make_unique<Lambert>(Color3f(0.65f, 0.05f, 0.05f)));

@@ -17,7 +17,8 @@ def run(scene, size=None, saved=None):
     "hdr": float32 [h, w, 3]) and the denoising AOVs Petrichor::Render filled ("albedo",
     "normal"). A scene with mat_maps also hands over tri_uv and its first texture, which
     the driver sets as the white Lambert's albedo texture and the GGX (material 4) F0
-    texture and roughness map."""
+    texture and roughness map; with mat_nmaps also as that GGX's normal map, at the map
+    strengths of mat_map_strength[4]."""
     n = len(scene["tri_mat"])
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
@@ -31,6 +32,8 @@ def run(scene, size=None, saved=None):
                 np.ascontiguousarray(scene["tri_uv"], np.float32).tofile(f)
                 np.array([tex.shape[1], tex.shape[0]], np.int32).tofile(f)
                 tex.tofile(f)
+                if scene.get("mat_nmaps") is not None:
+                    np.asarray(scene["mat_map_strength"][4], np.float32).tofile(f)
         args = [DRIVER, fin, fout] + ([str(v) for v in size] if size else [])
         r = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
         assert r.returncode == 0, r.stdout

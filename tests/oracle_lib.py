@@ -39,6 +39,9 @@ def lib():
     L.oracle_scene_destroy.argtypes = [_v]
     L.oracle_scene_set_uv.argtypes = [_v, _f]
     L.oracle_scene_set_maps.argtypes = [_v, ctypes.c_int, _i, ctypes.POINTER(_f), _i]
+    L.oracle_scene_set_normal_maps.argtypes = [_v, _i, _f]
+    L.oracle_ggx_map_normal.argtypes = [_f, _f, _f, ctypes.c_float, _f]
+    L.oracle_interpolate_frame.argtypes = [_f, _f, _f, _f, ctypes.c_float, _f, _f]
     L.oracle_bvh_build.restype = _v
     L.oracle_bvh_build.argtypes = [_v, ctypes.c_int]
     L.oracle_bvh_destroy.argtypes = [_v]
@@ -123,6 +126,11 @@ class OracleScene:
             ptrs = k((_f * max(len(texs), 1))(*[_ptr(t, _f) for t in texs]))
             L.oracle_scene_set_maps(self.ptr, len(texs), _ptr(sizes, _i), ctypes.cast(ptrs, ctypes.POINTER(_f)),
                                     _ptr(k(i32(scene["mat_maps"])), _i))
+        if scene.get("mat_nmaps") is not None or scene.get("mat_map_strength") is not None:
+            nm = scene.get("mat_nmaps")
+            ms = scene.get("mat_map_strength")
+            L.oracle_scene_set_normal_maps(self.ptr, _ptr(k(i32(nm)), _i) if nm is not None else None,
+                                           _ptr(k(f32(ms)), _f) if ms is not None else None)
         self.bvh = None
         self.num_tris = len(tri_mat)
 
@@ -295,3 +303,21 @@ def tonemap_rgb8(rgb, threads=0):
     out = np.zeros(a.shape, np.uint8)
     lib().oracle_tonemap_rgb8(a.size // 3, _ptr(a, _f), _ptr(out, ctypes.POINTER(ctypes.c_uint8)), threads)
     return out
+
+
+def ggx_map_normal(normal, tangent, texel, strength):
+    """GGX::GetNormal of a normal-mapped GGX (float32 in, float32 out)."""
+    out = np.zeros(3, np.float32)
+    lib().oracle_ggx_map_normal(_ptr(f32(normal), _f), _ptr(f32(tangent), _f), _ptr(f32(texel), _f),
+                                float(strength), _ptr(out, _f))
+    return out
+
+
+def interpolate_frame(tri9, ray6, t, nrm9=None, uv6=None):
+    """Triangle::Interpolate's (normal, tangent) at distance t along ray6 on tri9."""
+    n = np.zeros(3, np.float32)
+    tg = np.zeros(3, np.float32)
+    lib().oracle_interpolate_frame(_ptr(f32(tri9), _f), _ptr(f32(nrm9), _f) if nrm9 is not None else None,
+                                   _ptr(f32(uv6), _f) if uv6 is not None else None, _ptr(f32(ray6), _f),
+                                   float(t), _ptr(n, _f), _ptr(tg, _f))
+    return n, tg

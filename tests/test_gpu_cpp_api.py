@@ -78,3 +78,12 @@ def test_cpp_render_with_material_maps(gpu):
         plain.pop("mat_maps")
         other, _ = ctx().scene(plain).build_bvh(16).render(48, 36, 4, 8, integrator, 5)
         assert not np.array_equal(img, other)
+    # GGX::SetNormalMap, SetNormalMapStrength (clamped to [0, 1]) and SetRoughnessMapStrength
+    nm = dict(sc, mat_nmaps=np.array([-1, -1, -1, -1, 0], np.int32),
+              mat_map_strength=np.array([[1, 1]] * 4 + [[0.35, 0.8]], np.float32))
+    for integrator in (0, 1):
+        _, img = cpp_api.run(nm, (48, 36, 4, 8, integrator, 5))
+        ref, _ = ctx().scene(nm).build_bvh(16).render(48, 36, 4, 8, integrator, 5)
+        assert bits_equal(img, ref)
+        other, _ = ctx().scene(sc).build_bvh(16).render(48, 36, 4, 8, integrator, 5)
+        assert not np.array_equal(img, other)
