@@ -6,10 +6,11 @@
 //
 // Only what the GPU path renders is here: triangles (flat or smooth), Lambert, GGX,
 // Glass, Emission and MixMaterial with their image textures (Lambert albedo, GGX
-// reflectance and roughness maps), an equirect environment given as a Texture2D, the MIS
+// reflectance, roughness and normal maps), an equirect environment given as a Texture2D, the MIS
 // and simple integrators, the denoising albedo / normal and UV AOVs, Texture2D::Save (PNG
-// bytes converted on the GPU, Radiance HDR), and the reference's file loading: JSON scene
-// and render-setting files, OBJ meshes, PNG textures and Radiance .hdr environment maps.
+// bytes converted on the GPU, Radiance HDR), the reference's file loading (JSON scene and
+// render-setting files, OBJ meshes, PNG textures and Radiance .hdr environment maps), and
+// the 2D samplers (RandomSampler2D, MicroJitteredSampler) with XorShift128 and Halton.
 //
 // Floating point: the camera set-up below must be compiled without FMA contraction
 // (-ffp-contract=off) to give the reference's camera state bit for bit.
@@ -30,6 +31,7 @@
 #include <stdexcept>
 #include <string>
 #include <string_view>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -75,6 +77,55 @@ inline Vector3f Cross(const Vector3f& a, const Vector3f& b)
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
+// Random/XorShift.h: Marsaglia's xorshift128 with the reference's default x, y, z words and
+// the seed as w; next() is random() / (float)max().
+class XorShift128
+{
+public:
+    explicit XorShift128(uint32_t seed = 88675123u) : w(seed) {}
+    uint32_t random()
+    {
+        const uint32_t t = x ^ (x << 11);
+        x = y;
+        y = z;
+        z = w;
+        return w = (w ^ (w >> 19)) ^ (t ^ (t >> 8));
+    }
+    static constexpr uint32_t max() { return 0xFFFFFFFFu; }
+    float next() { return (float)random() / static_cast<float>(max()); }
+
+private:
+    uint32_t x = 123456789u, y = 362436069u, z = 521288629u, w;
+};
+
+// Math/Halton.h: the radical-inverse sequence in one base, tabulated by Initialize with the
+// reference's float steps (Math/Halton.cpp:18-37).
+class Halton
+{
+public:
+    void Initialize(size_t length, unsigned base)
+    {
+        m_values.assign(length, 0.0f);
+        for (size_t i = 0; i < length; ++i) {
+            unsigned rest = (unsigned)i;
+            float f = 1.0f, r = 0.0f;
+            for (; rest > 0; rest /= base) {
+                f /= (float)base;
+                r += f * (float)(rest % base);
+            }
+            m_values[i] = r;
+        }
+        m_isInitialized = true;
+    }
+    float GetValue(unsigned index) const { return m_values.at(index); }
+    size_t GetLength() const { return m_values.size(); }
+    bool IsInitialized() const { return m_isInitialized; }
+
+private:
+    std::vector<float> m_values;
+    bool m_isInitialized = false;
+};
+
 } // namespace Math
 
 using Color3f = Math::Vector3f;
@@ -86,6 +137,71 @@ struct RenderingResult
 
 namespace Core
 {
+
+// Core/Sampler/ISampler2D.h and its two implementations. The renders here draw their 2D
+// samples on the device; these host classes give the same sequences to host code.
+class ISampler2D
+{
+public:
+    virtual ~ISampler2D() = default;
+    virtual std::tuple<float, float> Next() = 0;
+};
+
+// Core/Sampler/RandomSampler2D.cpp:8-21: one stream per coordinate.
+class RandomSampler2D : public ISampler2D
+{
+public:
+    RandomSampler2D(uint32_t seedX, uint32_t seedY) : m_x(seedX), m_y(seedY) {}
+    std::tuple<float, float> Next() override
+    {
+        const float a = m_x.next();
+        return {a, m_y.next()};
+    }
+
+private:
+    Math::XorShift128 m_x, m_y;
+};
+
+// Core/Sampler/MicroJitteredSampler.cpp:10-48: Halton points in bases 2 and 3 plus offsets
+// that are redrawn (mu * (next() - 0.5), mu = 2.5 / sqrt(length)) after every pass over
+// the sequence, each coordinate taken modulo 1. Petrichor::SetMicroJitteredSampler renders
+// with it on the device.
+class MicroJitteredSampler : public ISampler2D
+{
+public:
+    explicit MicroJitteredSampler(unsigned seed) : m_xorShift(seed) {}
+    void Initialize(int lengthSeq)
+    {
+        m_lengthSeq = (size_t)lengthSeq;
+        // the reference's unqualified sqrt is the double C function
+        m_mu = (float)((double)kStarDiscrepancy / std::sqrt((double)static_cast<float>(lengthSeq)));
+        m_haltonBase2.Initialize(m_lengthSeq, 2);
+        m_haltonBase3.Initialize(m_lengthSeq, 3);
+    }
+    std::tuple<float, float> Next() override
+    {
+        float x = m_haltonBase2.GetValue(m_index) + m_offsetX;
+        float y = m_haltonBase3.GetValue(m_index) + m_offsetY;
+        x = x - 1.0f * std::floor(x / 1.0f); // Math::Mod(x, 1.0f)
+        y = y - 1.0f * std::floor(y / 1.0f);
+        if (++m_index >= m_lengthSeq) {
+            m_offsetX = m_mu * (m_xorShift.next() - 0.5f);
+            m_offsetY = m_mu * (m_xorShift.next() - 0.5f);
+            m_index = 0;
+        }
+        return {x, y};
+    }
+    unsigned GetIndex() const { return m_index; }
+
+private:
+    Math::Halton m_haltonBase2, m_haltonBase3;
+    Math::XorShift128 m_xorShift;
+    unsigned m_index = 0;
+    size_t m_lengthSeq = 0;
+    float m_mu = 0.0f;
+    float m_offsetX = 0.0f, m_offsetY = 0.0f;
+    static constexpr float kStarDiscrepancy = 2.5f;
+};
 
 // Core/Material/MaterialBase.h:15-22 (the order of the C-ABI's PT_MAT_* codes).
 enum class MaterialTypes
@@ -1044,9 +1160,13 @@ public:
     uint32_t GetNumRenderedTiles() const { return m_numRenderedTiles; }
     uint32_t GetNumTiles() const { return m_numTiles; }
 
-    // Extensions of the GPU path: integrator choice and the sampler seed.
+    // Extensions of the GPU path: integrator choice, the sampler seed, and the 2D sampler
+    // (0: RandomSampler2D as Core::Petrichor::Render uses; > 0: MicroJitteredSampler
+    // initialised with that sequence length, sample s of a pixel starting at Halton point
+    // s; the denoising AOVs keep the random sampler).
     void SetIntegrator(Integrator integrator) { m_integrator = integrator; }
     void SetSeed(uint32_t seed) { m_seed = seed; }
+    void SetMicroJitteredSampler(int lengthSeq) { m_samplerLength = lengthSeq; }
     // Error text of the last failed Render (empty after a success).
     const std::string& GetLastError() const { return m_error; }
 
@@ -1077,7 +1197,7 @@ public:
         }
         pt_render_desc rd{target->GetWidth(), target->GetHeight(), rs.numSamplesPerPixel, rs.numMaxBounces,
                           m_integrator == Integrator::PathTracing ? PT_INTEGRATOR_MIS : PT_INTEGRATOR_SIMPLE, m_seed,
-                          0};
+                          0, m_samplerLength};
         int rc = pt_render(s, bvh, &rd, target->GetRawDataPtr(), 0, nullptr);
         // the AOV passes (Core/Petrichor.cpp:104-261) into their target textures: UV at
         // RenderSetting::numSamplesPerPixel samples, the denoising ones at numSppForDenoising
@@ -1090,7 +1210,7 @@ public:
             if (rc != PT_OK || !tex || !tex->IsValid()) continue;
             pt_render_desc ad{tex->GetWidth(), tex->GetHeight(),
                               aov == PT_AOV_UV ? rs.numSamplesPerPixel : rs.numSppForDenoising, rs.numMaxBounces,
-                              PT_INTEGRATOR_MIS, m_seed, 0};
+                              PT_INTEGRATOR_MIS, m_seed, 0, 0};
             rc = pt_render_aov(s, bvh, aov, &ad, tex->GetRawDataPtr(), 0, nullptr);
         }
         if (rc != PT_OK) fail(pt_last_error(m_ctx));
@@ -1280,6 +1400,7 @@ private:
     pt_context* m_ctx = nullptr;
     Integrator m_integrator = Integrator::SimplePathTracing;
     uint32_t m_seed = 1;
+    int m_samplerLength = 0;
     std::string m_error;
     std::atomic<uint32_t> m_numRenderedTiles = 0;
     uint32_t m_numTiles = 0;

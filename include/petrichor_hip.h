@@ -104,7 +104,14 @@ typedef struct pt_render_desc {
     int32_t integrator;    /* PT_INTEGRATOR_* */
     uint32_t seed;         /* job seed of the per-sample sampler streams */
     int32_t flags;         /* PT_RENDER_* */
+    /* The 2D sampler of the integrators: 0 for RandomSampler2D (what Core::Petrichor::Render
+     * uses, Core/Petrichor.cpp:76-77); 1..PT_MAX_SAMPLER_LENGTH for MicroJitteredSampler
+     * (Core/Sampler/MicroJitteredSampler.cpp:10-48) initialised with that sequence length.
+     * Sample s of a pixel starts at Halton point s mod length with its own offsets. The
+     * denoising AOVs always use the random sampler. */
+    int32_t sampler_length;
 } pt_render_desc;
+#define PT_MAX_SAMPLER_LENGTH 65536
 
 /* Every launch on its own, timed with HIP events (same image). */
 #define PT_RENDER_TIME_KERNELS 1

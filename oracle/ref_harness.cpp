@@ -3,7 +3,7 @@
 //
 // TEST INFRASTRUCTURE ONLY (see oracle/Makefile). It is compiled against the
 // reference sources where they lie (LibPetrichor/Math, Core/Geometry/Triangle,
-// Core/Camera, Core/Material/{Lambert,GGX,Glass}, Core/Sampler) — the units that
+// Core/Camera, Core/Material/{Lambert,GGX,Glass}, Core/Sampler, Math/Halton) — the units that
 // build from their own sources with no third-party header. Units that need spdlog,
 // fmt or stb (BinnedSAHBVH, Environment, Texture2D, the integrators) are not built.
 // Private members are read through `#define private public`, applied after every
@@ -38,7 +38,9 @@
 #include "Core/Material/Glass.h"
 #include "Core/Material/Lambert.h"
 #include "Core/Ray.h"
+#include "Core/Sampler/MicroJitteredSampler.h"
 #include "Core/Sampler/RandomSampler2D.h"
+#include "Math/Halton.h"
 #include "Math/OrthonormalBasis.h"
 #include "Random/XorShift.h"
 #undef private
@@ -318,6 +320,35 @@ void case_onb()
     dump("onb_out.f32", out);
 }
 
+// Math::Halton tables and MicroJitteredSampler sequences: 600 draws (several sequence
+// wraps for the short lengths) per (seed, length).
+void case_microjittered()
+{
+    std::vector<float> halton;
+    for (unsigned base : {2u, 3u, 5u}) {
+        Math::Halton h;
+        h.Initialize(1000, base);
+        for (unsigned i = 0; i < 1000; ++i) halton.push_back(h.GetValue(i));
+    }
+    dump("halton.f32", halton);
+    std::vector<uint32_t> cfg;
+    std::vector<float> out;
+    for (unsigned seed : {1u, 7u, 12345u, 0xdeadbeefu})
+        for (int len : {1, 2, 3, 16, 100, 256}) {
+            MicroJitteredSampler smp(seed);
+            smp.Initialize(len);
+            cfg.push_back(seed);
+            cfg.push_back((uint32_t)len);
+            for (int i = 0; i < 600; ++i) {
+                const auto v = smp.Next();
+                out.push_back(std::get<0>(v));
+                out.push_back(std::get<1>(v));
+            }
+        }
+    dump("mj_cfg.u32", cfg);
+    dump("mj_out.f32", out);
+}
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -333,6 +364,7 @@ int main(int argc, char** argv)
     case_camera();
     case_materials();
     case_onb();
+    case_microjittered();
     std::printf("ok\n");
     return 0;
 }

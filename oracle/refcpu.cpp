@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <numeric>
 #include <stack>
 #include <thread>
@@ -108,15 +109,66 @@ struct Sampler1D { // RandomSampler1D.h:11-25
     explicit Sampler1D(uint32_t seed) : s(seed) {}
     float Next() { return s.next(); }
 };
+// Math::Halton::Initialize (Math/Halton.cpp:18-37): value i is the radical inverse of i in
+// `base`, accumulated digit by digit in float (f /= base, r += f * digit).
+static std::vector<float> halton_table(uint32_t length, unsigned base)
+{
+    std::vector<float> v(length);
+    for (uint32_t i = 0; i < length; ++i) {
+        unsigned rest = i;
+        float f = 1.0f, r = 0.0f;
+        for (; rest > 0; rest /= base) {
+            f /= (float)base;
+            r += f * (float)(rest % base);
+        }
+        v[i] = r;
+    }
+    return v;
+}
+
+// MicroJitteredSampler::Initialize (MicroJitteredSampler.cpp:20-26): Halton tables in bases
+// 2 and 3 and the jitter size mu = 2.5 / sqrt(length) (the unqualified sqrt is the double
+// C function, rounded to float once).
+struct MicroJitter {
+    std::vector<float> h2, h3;
+    uint32_t len;
+    float mu;
+    explicit MicroJitter(uint32_t length)
+      : h2(halton_table(length, 2)), h3(halton_table(length, 3)), len(length),
+        mu((float)((double)2.5f / pt_sqrt_d((double)(float)length)))
+    {
+    }
+};
+
 struct Sampler2D { // RandomSampler2D.cpp:8-21 (x stream first, then y stream)
     XorShift s0, s1;
     int64_t draws = 0;
+    // MicroJitteredSampler (MicroJitteredSampler.cpp:10-48) when mj is set: s0 is its
+    // XorShift128 for the offsets, s1 unused
+    const MicroJitter* mj = nullptr;
+    uint32_t index = 0;
+    float offx = 0.0f, offy = 0.0f;
     Sampler2D(uint32_t a, uint32_t b) : s0(a), s1(b) {}
     void Next(float* a, float* b)
     {
+        ++draws;
+        if (mj) {
+            // MicroJitteredSampler::Next: the Halton pair plus the offsets, Mod(x, 1) =
+            // x - 1 * floor(x / 1); new offsets after the last point of the sequence
+            float x = mj->h2[index] + offx, y = mj->h3[index] + offy;
+            x = x - 1.0f * std::floor(x / 1.0f);
+            y = y - 1.0f * std::floor(y / 1.0f);
+            if (++index >= mj->len) {
+                offx = mj->mu * (s0.next() - 0.5f);
+                offy = mj->mu * (s0.next() - 0.5f);
+                index = 0;
+            }
+            *a = x;
+            *b = y;
+            return;
+        }
         *a = s0.next();
         *b = s1.next();
-        ++draws;
     }
 };
 
@@ -148,10 +200,14 @@ static inline uint32_t pixel_seed(uint32_t job_seed, uint32_t pixel, uint32_t st
 // correlated, so every stream is used from its 24th output on (the GPU library's
 // PT_STREAM_WARMUP).
 static const int kStreamWarmup = 24;
+// With a micro-jittered 2D sampler (the library's definition, like the seeds): sample s
+// starts at Halton point s mod length, with offsets drawn from its 2D x stream as at the
+// end of a sequence, so the samples of a pixel take successive Halton points, each
+// jittered on its own.
 struct SampleSamplers {
     Sampler1D s1;
     Sampler2D s2;
-    SampleSamplers(uint32_t job_seed, uint32_t pixel, uint32_t sample)
+    SampleSamplers(uint32_t job_seed, uint32_t pixel, uint32_t sample, const MicroJitter* mj = nullptr)
       : s1(sample_seed(job_seed, pixel, sample, 0)),
         s2(sample_seed(job_seed, pixel, sample, 1), sample_seed(job_seed, pixel, sample, 2))
     {
@@ -159,6 +215,12 @@ struct SampleSamplers {
             (void)s1.s.random();
             (void)s2.s0.random();
             (void)s2.s1.random();
+        }
+        if (mj) {
+            s2.mj = mj;
+            s2.index = sample % mj->len;
+            s2.offx = mj->mu * (s2.s0.next() - 0.5f);
+            s2.offy = mj->mu * (s2.s0.next() - 0.5f);
         }
     }
 };
@@ -1096,6 +1158,7 @@ static int brute_intersect(const Scene& sc, const Ray& ray, float dmin, float dm
 struct Ctx {
     const Scene* sc;
     const BVH* bvh;
+    const MicroJitter* mj = nullptr; // the renders' 2D sampler: micro-jittered, or random when null
     int64_t queries = 0;
     int64_t work[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}; // per kind: queries, node visits, triangle tests
     int closest(const Ray& r, float dmin, float dmax, float* t, int kind = 0)
@@ -1206,7 +1269,7 @@ static V render_mis(Ctx& c, int px, int py, int w, int h, int spp, int max_bounc
     const Scene& sc = *c.sc;
     V sum = mk(0.0f, 0.0f, 0.0f);
     for (int s = 0; s < spp; s++) {
-        SampleSamplers smp(seed, (uint32_t)(py * w + px), (uint32_t)s);
+        SampleSamplers smp(seed, (uint32_t)(py * w + px), (uint32_t)s, c.mj);
         Sampler1D& s1 = smp.s1;
         Sampler2D& s2 = smp.s2;
         V color = mk(0.0f, 0.0f, 0.0f);
@@ -1280,7 +1343,7 @@ static V render_simple(Ctx& c, int px, int py, int w, int h, int spp, int max_bo
     const Scene& sc = *c.sc;
     V sum = mk(0.0f, 0.0f, 0.0f);
     for (int s = 0; s < spp; s++) {
-        SampleSamplers smp(seed, (uint32_t)(py * w + px), (uint32_t)s);
+        SampleSamplers smp(seed, (uint32_t)(py * w + px), (uint32_t)s, c.mj);
         Sampler1D& s1 = smp.s1;
         Sampler2D& s2 = smp.s2;
         Ray ray = generate_ray(sc.cam, px, py, w, h, s2);
@@ -1523,8 +1586,18 @@ int64_t oracle_render(void* s, void* b, int width, int height, int spp, int max_
                       int integrator, uint32_t seed, int num_threads, const int32_t* pixels,
                       int64_t num_pixels, float* rgb_out, int64_t* work)
 {
+    return oracle_render_sampler(s, b, width, height, spp, max_bounces, integrator, seed, num_threads, pixels,
+                                 num_pixels, rgb_out, work, 0);
+}
+
+int64_t oracle_render_sampler(void* s, void* b, int width, int height, int spp, int max_bounces,
+                              int integrator, uint32_t seed, int num_threads, const int32_t* pixels,
+                              int64_t num_pixels, float* rgb_out, int64_t* work, int sampler_length)
+{
     const Scene& sc = *(Scene*)s;
     const BVH& bvh = *(BVH*)b;
+    std::unique_ptr<MicroJitter> mj;
+    if (sampler_length > 0 && integrator <= 1) mj = std::make_unique<MicroJitter>((uint32_t)sampler_length);
     // Core/TileManager.cpp:8-43 (16x16 tiles) and Core/Thread/ThreadPool.cpp:17-26
     // (hardware_concurrency workers); tiles are taken from an atomic counter.
     const int tw = 16, th = 16;
@@ -1544,6 +1617,7 @@ int64_t oracle_render(void* s, void* b, int width, int height, int spp, int max_
         Ctx c;
         c.sc = &sc;
         c.bvh = &bvh;
+        c.mj = mj.get();
         for (;;) {
             const int64_t tile = next_tile.fetch_add(1);
             if (tile >= ntiles) break;
@@ -1856,6 +1930,22 @@ void oracle_scene_set_maps(void* s, int num_textures, const int32_t* sizes, cons
         sc->mats[m].map0 = mat_maps[2 * m];
         sc->mats[m].map1 = mat_maps[2 * m + 1];
     }
+}
+
+// Math::Halton::Initialize's table (length values in `base`).
+void oracle_halton(int length, int base, float* out)
+{
+    const std::vector<float> v = halton_table((uint32_t)length, (unsigned)base);
+    std::memcpy(out, v.data(), v.size() * sizeof(float));
+}
+
+// n draws (x, y) of MicroJitteredSampler(seed) after Initialize(length).
+void oracle_microjittered(uint32_t seed, int length, int n, float* out)
+{
+    const MicroJitter mj((uint32_t)length);
+    Sampler2D s(seed, 0);
+    s.mj = &mj;
+    for (int i = 0; i < n; ++i) s.Next(&out[2 * i], &out[2 * i + 1]);
 }
 
 // GGX::GetNormal of a normal-mapped GGX for a given shading normal, tangent and texel.

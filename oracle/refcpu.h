@@ -90,6 +90,16 @@ void oracle_trace_brute(void* scene, int64_t n, const float* rays, float* t_out,
 int64_t oracle_render(void* scene, void* bvh, int width, int height, int spp, int max_bounces,
                       int integrator, uint32_t seed, int num_threads, const int32_t* pixels,
                       int64_t num_pixels, float* rgb_out, int64_t* work);
+/* oracle_render with the renders' 2D sampler chosen: sampler_length 0 for the random
+ * sampler, > 0 for MicroJitteredSampler over Halton sequences of that length (the
+ * integrators 0 and 1 only). */
+int64_t oracle_render_sampler(void* scene, void* bvh, int width, int height, int spp, int max_bounces,
+                              int integrator, uint32_t seed, int num_threads, const int32_t* pixels,
+                              int64_t num_pixels, float* rgb_out, int64_t* work, int sampler_length);
+/* Math::Halton::Initialize's table (length values in `base`). */
+void oracle_halton(int length, int base, float* out);
+/* n draws (x, y pairs) of MicroJitteredSampler(seed) after Initialize(length). */
+void oracle_microjittered(uint32_t seed, int length, int n, float* out);
 
 /* Building blocks, exposed for unit tests against the reference translation units. */
 uint32_t oracle_pixel_seed(uint32_t job_seed, uint32_t pixel, uint32_t stream);

@@ -55,7 +55,7 @@ class BvhDesc(ctypes.Structure):
 class RenderDesc(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("spp", ctypes.c_int32),
                 ("max_bounces", ctypes.c_int32), ("integrator", ctypes.c_int32), ("seed", ctypes.c_uint32),
-                ("flags", ctypes.c_int32)]
+                ("flags", ctypes.c_int32), ("sampler_length", ctypes.c_int32)]
 
 
 class JobDesc(ctypes.Structure):
@@ -78,6 +78,7 @@ RENDER_PLAIN_LAUNCHES = 2
 RENDER_TRACE_STATS = 4
 RENDER_TWO_STREAMS = 8
 RENDER_LAUNCH_LOG = 16
+MAX_SAMPLER_LENGTH = 65536  # PT_MAX_SAMPLER_LENGTH
 AOV_DENOISING_ALBEDO, AOV_DENOISING_NORMAL, AOV_UV = 0, 1, 2
 
 
@@ -495,16 +496,19 @@ class Bvh:
         return t_out, prim_out, st
 
     def render(self, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1, out=None,
-               time_kernels=False, plain=False, trace_stats=False, two_streams=False, launch_log=False):
+               time_kernels=False, plain=False, trace_stats=False, two_streams=False, launch_log=False,
+               sampler_length=0):
         """Renders one image; returns (rgb [h, w, 3] float32 numpy array or the given device
         tensor, stats dict). time_kernels: every launch timed on its own; plain: plain
         launches with one synchronisation per iteration; two_streams: a large image in two
         halves on two streams; launch_log: the production schedule with every trace launch
-        logged (with trace_stats: and its work counted) (all give the same image)."""
+        logged (with trace_stats: and its work counted) (all give the same image).
+        sampler_length > 0: the 2D sampler is MicroJitteredSampler over Halton sequences of
+        that length instead of RandomSampler2D."""
         flags = (RENDER_TIME_KERNELS if (time_kernels or trace_stats) and not launch_log else 0) | \
             (RENDER_PLAIN_LAUNCHES if plain else 0) | (RENDER_TRACE_STATS if trace_stats else 0) | \
             (RENDER_TWO_STREAMS if two_streams else 0) | (RENDER_LAUNCH_LOG if launch_log else 0)
-        d = RenderDesc(width, height, spp, max_bounces, integrator, seed, flags)
+        d = RenderDesc(width, height, spp, max_bounces, integrator, seed, flags, sampler_length)
         st = RenderStats()
         if out is None:
             host = np.zeros((height, width, 3), np.float32)
@@ -554,9 +558,11 @@ class Bvh:
 class Job:
     """One render job of a batch: a BVH (and its scene) plus render settings."""
 
-    def __init__(self, bvh, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1, out=None):
+    def __init__(self, bvh, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1, out=None,
+                 sampler_length=0):
         self.bvh, self.width, self.height = bvh, width, height
         self.spp, self.max_bounces, self.integrator, self.seed = spp, max_bounces, integrator, seed
+        self.sampler_length = sampler_length
         self.out = out  # device tensor [h, w, 3] or None (host numpy result)
         self.result = None
 
@@ -588,7 +594,8 @@ class Engine:
             d = arr[i]
             d.scene = j.bvh.scene.handle
             d.bvh = j.bvh.handle
-            d.render = RenderDesc(j.width, j.height, j.spp, j.max_bounces, j.integrator, j.seed, 0)
+            d.render = RenderDesc(j.width, j.height, j.spp, j.max_bounces, j.integrator, j.seed, 0,
+                                  getattr(j, "sampler_length", 0))
             if j.out is None:
                 host = np.zeros((j.height, j.width, 3), np.float32)
                 keep.append(host)
@@ -649,8 +656,8 @@ class Server:
     def resident(self, scene_id):
         return bool(self._lib.pt_server_scene_resident(self.handle, scene_id))
 
-    def submit(self, scene_id, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1):
-        rd = RenderDesc(width, height, spp, max_bounces, integrator, seed, 0)
+    def submit(self, scene_id, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1, sampler_length=0):
+        rd = RenderDesc(width, height, spp, max_bounces, integrator, seed, 0, sampler_length)
         rid = ctypes.c_int64()
         self._check(self._lib.pt_server_submit(self.handle, scene_id, ctypes.byref(rd), ctypes.byref(rid)),
                     "pt_server_submit")

@@ -12,6 +12,7 @@
 #include "pt_libm.h"
 
 #include <float.h>
+#include <string.h>
 #include <stdint.h>
 
 namespace pt {
@@ -88,13 +89,53 @@ PT_HD uint32_t xs_random(XorShift& s)
 // random() / float(UINT_MAX) == (float)u32 / 2^32
 PT_HD float xs_next(XorShift& s) { return (float)xs_random(s) / 4294967296.0f; }
 
-// Sampler state of one path: the 1D stream and the two streams of the 2D sampler.
+PT_HD float bits_to_float(uint32_t u)
+{
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+PT_HD uint32_t float_to_bits(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+
+// Sampler state of one path: the 1D stream and the two streams of the 2D sampler
+// (RandomSampler2D: x stream, y stream). With a micro-jittered 2D sampler `mj` is the
+// job's Halton table (entry 0: the sequence length as bits and mu; then the pairs in bases
+// 2 and 3), s2x is the sampler's offset stream and s2y holds its index and the two offsets
+// (as bits).
 struct Samplers {
     XorShift s1, s2x, s2y;
+    const float2* mj = nullptr;
 };
 PT_HD float next1(Samplers& s) { return xs_next(s.s1); }
+// MicroJitteredSampler::Next (Core/Sampler/MicroJitteredSampler.cpp:28-48): the Halton pair
+// at the index plus the offsets, each taken Mod(x, 1) = x - 1 * floor(x / 1); after the
+// last point of the sequence two new offsets mu * (next() - 0.5) and index 0.
+PT_HD void mj_next(Samplers& s, float* a, float* b)
+{
+    const float2 h = s.mj[1 + s.s2y.x];
+    float x = h.x + bits_to_float(s.s2y.y), y = h.y + bits_to_float(s.s2y.z);
+    x = x - 1.0f * floorf(x / 1.0f);
+    y = y - 1.0f * floorf(y / 1.0f);
+    if (++s.s2y.x >= float_to_bits(s.mj[0].x)) {
+        const float mu = s.mj[0].y;
+        s.s2y.y = float_to_bits(mu * (xs_next(s.s2x) - 0.5f));
+        s.s2y.z = float_to_bits(mu * (xs_next(s.s2x) - 0.5f));
+        s.s2y.x = 0;
+    }
+    *a = x;
+    *b = y;
+}
 PT_HD void next2(Samplers& s, float* a, float* b)
 {
+    if (s.mj != nullptr) {
+        mj_next(s, a, b);
+        return;
+    }
     *a = xs_next(s.s2x);
     *b = xs_next(s.s2y);
 }
@@ -126,12 +167,25 @@ PT_HD XorShift xorshift_warmed(uint32_t seed)
     for (int i = 0; i < PT_STREAM_WARMUP; ++i) (void)xs_random(x);
     return x;
 }
-PT_HD Samplers samplers_for_sample(uint32_t job_seed, uint32_t pixel, uint32_t sample)
+// With a micro-jittered 2D sampler (this library's definition, like the seeds): sample s
+// starts at Halton point s mod length with offsets drawn from its x stream as at the end
+// of a sequence, so the samples of a pixel take successive Halton points, each jittered
+// on its own.
+PT_HD Samplers samplers_for_sample(uint32_t job_seed, uint32_t pixel, uint32_t sample,
+                                   const float2* mj = nullptr)
 {
     Samplers s;
     s.s1 = xorshift_warmed(sample_seed(job_seed, pixel, sample, 0));
     s.s2x = xorshift_warmed(sample_seed(job_seed, pixel, sample, 1));
     s.s2y = xorshift_warmed(sample_seed(job_seed, pixel, sample, 2));
+    if (mj != nullptr) {
+        s.mj = mj;
+        const float mu = mj[0].y;
+        s.s2y.x = sample % float_to_bits(mj[0].x);
+        s.s2y.y = float_to_bits(mu * (xs_next(s.s2x) - 0.5f));
+        s.s2y.z = float_to_bits(mu * (xs_next(s.s2x) - 0.5f));
+        s.s2y.w = 0;
+    }
     return s;
 }
 

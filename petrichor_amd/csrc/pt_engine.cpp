@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -38,6 +39,31 @@ static constexpr int64_t kSingleJobSlots = (int64_t)1 << 21;
 
 namespace pt {
 
+// Entry 0: the length (as float bits) and MicroJitteredSampler's mu = 2.5 / sqrt(length),
+// whose unqualified sqrt is the double C function (MicroJitteredSampler.cpp:20-26); entry
+// 1 + i: point i of Math::Halton in bases 2 and 3, accumulated digit by digit in float as
+// Halton::Initialize does (f /= base, r += f * digit; Math/Halton.cpp:18-37).
+std::vector<float2> micro_jitter_table(int length)
+{
+    std::vector<float2> t((size_t)length + 1);
+    t[0] = make_float2(bits_to_float((uint32_t)length), (float)((double)2.5f / std::sqrt((double)(float)length)));
+    for (int i = 0; i < length; ++i) {
+        float h[2];
+        const unsigned bases[2] = {2u, 3u};
+        for (int k = 0; k < 2; ++k) {
+            unsigned rest = (unsigned)i;
+            float f = 1.0f, r = 0.0f;
+            for (; rest > 0; rest /= bases[k]) {
+                f /= (float)bases[k];
+                r += f * (float)(rest % bases[k]);
+            }
+            h[k] = r;
+        }
+        t[1 + (size_t)i] = make_float2(h[0], h[1]);
+    }
+    return t;
+}
+
 int check_job(pt_context* ctx, const pt_job_desc& j)
 {
     if (!j.scene || !j.bvh || !j.rgb_out) return pt_fail(ctx, PT_ERR_ARG, "job needs a scene, a BVH and an output");
@@ -48,6 +74,8 @@ int check_job(pt_context* ctx, const pt_job_desc& j)
         return pt_fail(ctx, PT_ERR_ARG, "bad render settings");
     if ((int64_t)d.width * d.height > (1 << 28)) return pt_fail(ctx, PT_ERR_ARG, "image too large");
     if ((int64_t)d.spp > ((int64_t)1 << 31) - 1) return pt_fail(ctx, PT_ERR_ARG, "too many samples per pixel");
+    if (d.sampler_length < 0 || d.sampler_length > PT_MAX_SAMPLER_LENGTH)
+        return pt_fail(ctx, PT_ERR_ARG, "sampler length outside [0, PT_MAX_SAMPLER_LENGTH]");
     return PT_OK;
 }
 

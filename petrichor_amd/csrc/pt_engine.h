@@ -23,6 +23,9 @@ constexpr int64_t kSampleBytes = 12;
 // its pixels, run in parallel.
 constexpr int64_t kMinJobSlots = (int64_t)1 << 20;
 
+// The micro-jittered 2D sampler's table for a sequence length (1 + length entries).
+std::vector<float2> micro_jitter_table(int length);
+
 struct Buffers {
     std::vector<void*> ptrs;
     template <typename T>
@@ -145,6 +148,7 @@ struct pt_engine {
         for (hipEvent_t e : finish_ev) (void)hipEventDestroy(e);
         if (st) (void)hipStreamSynchronize(st);
         for (void* b : sample_bufs) (void)hipFree(b);
+        for (auto& t : halton_d) (void)hipFree(t.second);
         if (st) (void)hipStreamDestroy(st);
         if (stage_h) (void)hipHostFree(stage_h);
         if (stage_live_h) (void)hipHostFree(stage_live_h);
@@ -395,6 +399,28 @@ struct pt_engine {
     }
     static size_t samples_bytes(const pt::EngineJob& j) { return (size_t)pt::kSampleBytes * (size_t)std::max<int64_t>(j.items, 1); }
 
+    // The micro-jittered 2D sampler's table for a sequence length (pt::Samplers), made once
+    // per engine and length: entry 0 is (length as bits, mu), then the Halton pairs.
+    std::map<int, float2*> halton_d;
+    int halton_table(int length, const float2** out)
+    {
+        auto it = halton_d.find(length);
+        if (it == halton_d.end()) {
+            const std::vector<float2> t = pt::micro_jitter_table(length);
+            float2* d = nullptr;
+            PT_HIP(ctx, hipMalloc((void**)&d, t.size() * sizeof(float2)));
+            const hipError_t e = hipMemcpyAsync(d, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) (void)hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                (void)hipFree(d);
+                return pt_fail(ctx, PT_ERR_HIP, std::string("sampler table: ") + hipGetErrorString(e));
+            }
+            it = halton_d.emplace(length, d).first;
+        }
+        *out = it->second;
+        return PT_OK;
+    }
+
     int start(pt::EngineJob& j)
     {
         const int jw = job_width(j);
@@ -410,6 +436,10 @@ struct pt_engine {
                 break;
             }
         if (slot < 0) return pt_fail(ctx, PT_ERR_BUDGET, "job table full");
+        const float2* mj = nullptr;
+        if (j.desc.sampler_length > 0) {
+            if (const int rc = halton_table(j.desc.sampler_length, &mj)) return rc;
+        }
         if (j.slots <= 0) j.slots = job_slots(j);
         if (!alloc_range(j.slots, &j.base)) return pt_fail(ctx, PT_ERR_BUDGET, "no free path slots");
         int rc = take_samples(samples_bytes(j), &j.samples_d);
@@ -438,6 +468,7 @@ struct pt_engine {
         v.job_index = slot;
         v.pixel_base = (int)j.pixel_base;
         v.num_pixels = (int)j.npix;
+        v.mj = mj;
         jobs_h[slot] = jd;
         done_h[slot] = 0;
         cursor_h[slot] = 0;

@@ -65,6 +65,7 @@ struct JobView {
     int job_index;   // index in the engine's job table
     int pixel_base;  // the job renders pixels [pixel_base, pixel_base + num_pixels) of the image
     int num_pixels;
+    const float2* mj = nullptr; // micro-jittered 2D sampler's Halton table (Samplers), or null
 };
 
 // A job as the kernels see it in the engine's job table.

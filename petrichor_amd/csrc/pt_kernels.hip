@@ -279,7 +279,7 @@ __device__ __forceinline__ Ray start_sample(const SceneView& sc, const JobView& 
     const unsigned long long np = (unsigned long long)job.num_pixels;
     const int i = (int)(item % np), s = (int)(item / np);
     const int pixel = job.pixel_base + i;
-    Samplers smp = samplers_for_sample(job.seed, (uint32_t)pixel, (uint32_t)s);
+    Samplers smp = samplers_for_sample(job.seed, (uint32_t)pixel, (uint32_t)s, job.mj);
     const Ray r = camera_ray(sc, job, pixel, smp);
     ps.color[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     ps.ray_o[p] = st3(r.o, r.prob);
@@ -352,6 +352,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
         smp.s1 = unpack_xs(ps.rng[3 * p]);
         smp.s2x = unpack_xs(ps.rng[3 * p + 1]);
         smp.s2y = unpack_xs(ps.rng[3 * p + 2]);
+        smp.mj = job.mj;
         const float4 o4 = ps.ray_o[p], d4 = ps.ray_d[p], t4 = ps.thr[p];
         Ray ray;
         ray.o = ld3(o4);

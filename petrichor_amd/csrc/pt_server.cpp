@@ -457,7 +457,8 @@ int pt_server_submit(pt_server* s, int64_t scene_id, const pt_render_desc* d, in
 {
     if (!s || !d || !request_id) return sfail(PT_ERR_ARG, "null argument");
     if (d->width <= 0 || d->height <= 0 || d->spp <= 0 || d->max_bounces < 0 ||
-        (d->integrator != PT_INTEGRATOR_MIS && d->integrator != PT_INTEGRATOR_SIMPLE))
+        (d->integrator != PT_INTEGRATOR_MIS && d->integrator != PT_INTEGRATOR_SIMPLE) ||
+        d->sampler_length < 0 || d->sampler_length > PT_MAX_SAMPLER_LENGTH)
         return sfail(PT_ERR_ARG, "bad render settings");
     const int64_t npix = (int64_t)d->width * d->height;
     std::lock_guard<std::mutex> lk(s->mu);

@@ -3,7 +3,8 @@
 // Scene, a Camera set up with the reference's calls, a target Texture2D, then
 // Petrichor::Render. Used by the tests to compare with the Python binding.
 //
-//   render_api IN OUT [W H SPP DEPTH INTEGRATOR SEED]   (no size: camera state only)
+//   render_api IN OUT [W H SPP DEPTH INTEGRATOR SEED [SAMPLER_LENGTH]]   (no size: camera
+//   state only; SAMPLER_LENGTH > 0: Petrichor::SetMicroJitteredSampler)
 // IN:  int32 n, int32 n_lights, float pos[9 n], int32 material[n] (0 white, 1 red,
 //      2 green, 3 light, 4 GGX), int32 lights[n_lights]; optionally float uv[6 n], int32
 //      tw, th, float texture[th][tw][3]: the vertices' uv, and the texture as the white
@@ -24,8 +25,8 @@ using Math::Vector3f;
 
 int main(int argc, char** argv)
 {
-    if (argc != 3 && argc != 9) {
-        std::fprintf(stderr, "usage: render_api IN OUT [W H SPP DEPTH INTEGRATOR SEED]\n");
+    if (argc != 3 && argc != 9 && argc != 10) {
+        std::fprintf(stderr, "usage: render_api IN OUT [W H SPP DEPTH INTEGRATOR SEED [SAMPLER_LENGTH]]\n");
         return 2;
     }
     FILE* in = std::fopen(argv[1], "rb");
@@ -104,7 +105,7 @@ int main(int argc, char** argv)
     if (!out) return 2;
     std::fwrite(state, 4, 16, out);
     int rc = 0;
-    if (argc == 9) {
+    if (argc >= 9) {
         const int w = std::atoi(argv[3]), h = std::atoi(argv[4]);
         Texture2D target(w, h), albedo(w, h), normal(w, h), uvtex(w, h);
         scene.SetTargetTexture(Scene::AOVType::Rendered, &target);
@@ -119,6 +120,7 @@ int main(int argc, char** argv)
         Core::Petrichor renderer;
         renderer.SetIntegrator(std::atoi(argv[7]) == 0 ? Integrator::PathTracing : Integrator::SimplePathTracing);
         renderer.SetSeed((uint32_t)std::atoi(argv[8]));
+        if (argc == 10) renderer.SetMicroJitteredSampler(std::atoi(argv[9]));
         float seconds = -1.0f;
         renderer.SetRenderCallback([&](const RenderingResult& r) { seconds = r.totalSec; });
         if (!renderer.Render(scene)) {

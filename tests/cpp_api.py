@@ -12,7 +12,7 @@ LOAD_DRIVER = os.path.join(ROOT, "petrichor_amd", "_build", "load_api")
 
 def run(scene, size=None, saved=None):
     """Builds the scene through the C++ API; returns (camera state [16], image or None).
-    size: (width, height, spp, depth, integrator, seed) to render. saved: a dict that
+    size: (width, height, spp, depth, integrator, seed[, sampler_length]) to render. saved: a dict that
     receives the files Texture2D::Save wrote of the image ("png": uint8 [h, w, 3],
     "hdr": float32 [h, w, 3]) and the denoising AOVs Petrichor::Render filled ("albedo",
     "normal"). A scene with mat_maps also hands over tri_uv and its first texture, which

@@ -42,6 +42,9 @@ FILES = {
     "mat_eval.f32": (np.float32, 4, 4000),
     "onb_in.f32": (np.float32, 5, 2000),
     "onb_out.f32": (np.float32, 12, 2000),
+    "halton.f32": (np.float32, 1000, 3),
+    "mj_cfg.u32": (np.uint32, 2, 24),
+    "mj_out.f32": (np.float32, 1200, 24),
 }
 
 

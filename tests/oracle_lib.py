@@ -41,6 +41,11 @@ def lib():
     L.oracle_scene_set_maps.argtypes = [_v, ctypes.c_int, _i, ctypes.POINTER(_f), _i]
     L.oracle_scene_set_normal_maps.argtypes = [_v, _i, _f]
     L.oracle_ggx_map_normal.argtypes = [_f, _f, _f, ctypes.c_float, _f]
+    L.oracle_halton.argtypes = [ctypes.c_int, ctypes.c_int, _f]
+    L.oracle_microjittered.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, _f]
+    L.oracle_render_sampler.restype = ctypes.c_int64
+    L.oracle_render_sampler.argtypes = [_v, _v, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_uint32, ctypes.c_int, _i, ctypes.c_int64, _f, _l, ctypes.c_int]
     L.oracle_interpolate_frame.argtypes = [_f, _f, _f, _f, ctypes.c_float, _f, _f]
     L.oracle_bvh_build.restype = _v
     L.oracle_bvh_build.argtypes = [_v, ctypes.c_int]
@@ -182,10 +187,12 @@ class OracleScene:
         lib().oracle_trace_brute(self.ptr, n, _ptr(rays, _f), _ptr(t, _f), _ptr(p, _i))
         return t, p
 
-    def render(self, width, height, spp, max_bounces, integrator=0, seed=1, threads=0, pixels=None, work=None):
+    def render(self, width, height, spp, max_bounces, integrator=0, seed=1, threads=0, pixels=None, work=None,
+               sampler_length=0):
         """Returns (image or per-pixel rows, queries). If `work` is an int64[9] array it
         receives per query kind (extension, area shadow, env shadow): queries, node
-        visits, triangle tests."""
+        visits, triangle tests. sampler_length > 0: the 2D sampler is micro-jittered over
+        Halton sequences of that length."""
         if pixels is not None:
             pixels = i32(pixels)
             out = np.zeros((len(pixels), 3), np.float32)
@@ -193,8 +200,9 @@ class OracleScene:
         else:
             out = np.zeros((height, width, 3), np.float32)
             npx = 0
-        q = lib().oracle_render(self.ptr, self.bvh, width, height, spp, max_bounces, integrator, seed,
-                                threads, _ptr(pixels, _i), npx, _ptr(out, _f), _ptr(work, _l))
+        q = lib().oracle_render_sampler(self.ptr, self.bvh, width, height, spp, max_bounces, integrator, seed,
+                                        threads, _ptr(pixels, _i), npx, _ptr(out, _f), _ptr(work, _l),
+                                        int(sampler_length))
         return out, q
 
     def env_color(self, d):
@@ -321,3 +329,17 @@ def interpolate_frame(tri9, ray6, t, nrm9=None, uv6=None):
                                    _ptr(f32(uv6), _f) if uv6 is not None else None, _ptr(f32(ray6), _f),
                                    float(t), _ptr(n, _f), _ptr(tg, _f))
     return n, tg
+
+
+def halton(length, base):
+    """Math::Halton::Initialize's table."""
+    out = np.zeros(length, np.float32)
+    lib().oracle_halton(length, base, _ptr(out, _f))
+    return out
+
+
+def microjittered(seed, length, n):
+    """n (x, y) draws of MicroJitteredSampler(seed) after Initialize(length)."""
+    out = np.zeros((n, 2), np.float32)
+    lib().oracle_microjittered(seed, length, n, _ptr(out, _f))
+    return out

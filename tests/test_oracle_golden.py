@@ -121,3 +121,14 @@ def test_full_harness_run_matches(tmp_path):
     for k in range(len(mt)):
         o = O.next_ray(int(mt[k]), mp[k], sh[k], ri[k], int(sd[k, 0]), int(sd[k, 1]))
         assert np.array_equal(bits(o), bits(ro[k])), k
+
+
+def test_halton_matches_reference():
+    for k, base in enumerate((2, 3, 5)):
+        assert np.array_equal(bits(O.halton(1000, base)), bits(GOLD["halton"][k]))
+
+
+def test_microjittered_matches_reference():
+    for (seed, length), ref in zip(GOLD["mj_cfg"], GOLD["mj_out"]):
+        got = O.microjittered(int(seed), int(length), 600)
+        assert np.array_equal(bits(got.reshape(-1)), bits(ref)), (seed, length)
