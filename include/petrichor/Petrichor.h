@@ -4,7 +4,7 @@
 // reference builds against this header; Render flattens the scene and calls the C-ABI
 // of petrichor_hip.h (libpetrichor_hip.so).
 //
-// Only what the GPU path renders is here: triangles (flat or smooth), Lambert, GGX,
+// Only what the GPU path renders is here: triangles (flat or smooth), spheres, Lambert, GGX,
 // Glass, Emission and MixMaterial with their image textures (Lambert albedo, GGX
 // reflectance, roughness and normal maps), an equirect environment given as a Texture2D, the MIS
 // and simple integrators, the denoising albedo / normal and UV AOVs, Texture2D::Save (PNG
@@ -367,6 +367,21 @@ public:
 private:
     std::array<const Vertex*, 3> m_vertices{};
     ShadingTypes m_shadingType;
+};
+
+// Core/Geometry/Sphere.h: centre and radius. The GPU path renders spheres as primitives
+// after all the scene's triangles (Render numbers them so).
+class Sphere : public GeometryBase
+{
+public:
+    Sphere() = default;
+    Sphere(const Math::Vector3f& o, float r) : m_origin(o), m_radius(r) {}
+    const Math::Vector3f& GetOrigin() const { return m_origin; }
+    float GetRadius() const { return m_radius; }
+
+private:
+    Math::Vector3f m_origin;
+    float m_radius = 1.0f;
 };
 
 namespace detail
@@ -1227,7 +1242,8 @@ public:
 
 private:
     struct Flat {
-        std::vector<float> pos, nrm, uv, matParam, env;
+        std::vector<float> pos, nrm, uv, matParam, env, spheres;
+        std::vector<int32_t> sphereMat;
         std::vector<int32_t> matId, smooth, matType, lights, matMaps, matNormalMaps, texSizes;
         std::vector<float> matStrengths;
         bool anyNormalMap = false, anyStrength = false;
@@ -1332,9 +1348,15 @@ private:
                 for (int k = 0; k < 3; ++k)
                     if (t->GetVertex(k) && (t->GetVertex(k)->uv.x != 0.0f || t->GetVertex(k)->uv.y != 0.0f)) anyUV = true;
             }
+        std::vector<const Sphere*> spheres;
         for (const GeometryBase* g : scene.GetGeometries()) {
+            if (const auto* sp = dynamic_cast<const Sphere*>(g)) { // numbered after the triangles
+                if (!sp->GetMaterial()) return fail("a sphere has no material");
+                spheres.push_back(sp);
+                continue;
+            }
             const auto* t = dynamic_cast<const Triangle*>(g);
-            if (!t) return fail("the GPU path renders triangles only");
+            if (!t) return fail("the GPU path renders triangles and spheres only");
             if (!t->GetMaterial()) return fail("a triangle has no material");
             triIndex[g] = (int32_t)f.matId.size();
             for (int k = 0; k < 3; ++k) {
@@ -1348,9 +1370,17 @@ private:
             if (mi == -3) return fail("a material texture is empty");
             f.matId.push_back(mi);
         }
+        const int32_t numTris = (int32_t)f.matId.size();
+        for (const Sphere* sp : spheres) {
+            triIndex[sp] = numTris + (int32_t)f.sphereMat.size();
+            const int32_t mi = material_index(sp->GetMaterial(), matIndex, f);
+            if (mi == -3) return fail("a material texture is empty");
+            f.spheres.insert(f.spheres.end(), {sp->GetOrigin().x, sp->GetOrigin().y, sp->GetOrigin().z, sp->GetRadius()});
+            f.sphereMat.push_back(mi);
+        }
         for (const GeometryBase* l : scene.GetLights()) {
             auto it = triIndex.find(l);
-            if (it == triIndex.end()) return fail("a light is not one of the scene's triangles");
+            if (it == triIndex.end()) return fail("a light is not one of the scene's triangles or spheres");
             f.lights.push_back(it->second);
         }
         if (f.matType.empty()) { // pt_scene_create wants one material even for an empty scene
@@ -1362,6 +1392,9 @@ private:
         }
         pt_scene_desc& d = f.desc;
         d.num_triangles = (int32_t)f.matId.size();
+        d.num_spheres = (int32_t)f.sphereMat.size();
+        d.spheres = f.spheres.empty() ? nullptr : f.spheres.data();
+        d.sphere_material_ids = f.sphereMat.empty() ? nullptr : f.sphereMat.data();
         d.positions = f.pos.data();
         d.normals = anySmooth ? f.nrm.data() : nullptr;
         d.material_ids = f.matId.data();

@@ -56,7 +56,7 @@ typedef struct pt_scene_desc {
      * p6 (importance sampling enabled: 1 or 0), p7 unused */
     const float* material_params;
     int32_t num_lights;
-    const int32_t* light_triangles; /* Scene::AppendLight order (Core/Scene.h:86-92) */
+    const int32_t* light_triangles; /* Scene::AppendLight order (Core/Scene.h:86-92); primitive indices */
     /* Camera state of Core/Camera.h:121-130: pos[3] right[3] up[3] forward[3]
      * focus_dist h_perf aperture, then one unused float. */
     float camera[16];
@@ -84,6 +84,13 @@ typedef struct pt_scene_desc {
      * clamps it to [0, 1], values outside are rejected here) and SetRoughnessMapStrength
      * (GGX.h:89-93); NULL for 1 and 1, the reference's defaults. */
     const float* material_map_strengths;
+    /* Spheres (Core/Geometry/Sphere.cpp:17-118), primitives num_triangles ..
+     * num_triangles + num_spheres - 1 after the triangles: centre and radius (4 floats) and
+     * a material each. light_triangles may name them by that primitive index (Scene::AppendLight
+     * of a Sphere); primitive indices in hits count the same way. */
+    int32_t num_spheres;
+    const float* spheres;
+    const int32_t* sphere_material_ids;
 } pt_scene_desc;
 
 typedef struct pt_bvh_desc {
@@ -171,7 +178,8 @@ int pt_scene_destroy(pt_scene* scene);
 int pt_bvh_build(pt_scene* scene, const pt_bvh_desc* desc, pt_bvh** out);
 int pt_bvh_destroy(pt_bvh* bvh);
 /* Leaf ordering (the reference's m_primitiveIDs, BinnedSAHBVH.h:178): prim_order[k]
- * is the triangle at leaf slot k. Host pointer of num_triangles ints. */
+ * is the primitive at leaf slot k (triangles, then spheres). Host pointer of
+ * num_triangles + num_spheres ints. */
 int pt_bvh_export_order(pt_bvh* bvh, int32_t* prim_order);
 /* Node table of the binary tree in the reference's node order (BinnedSAHBVH::m_nodes,
  * BinnedSAHBVH.h:22-52): per node box[6] (lower, upper), child[2] (-1 for leaves),

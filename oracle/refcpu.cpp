@@ -770,8 +770,16 @@ struct Tri {
     int material = 0;
 };
 
+// Core/Geometry/Sphere.h: centre and radius; primitives after the triangles.
+struct Sph {
+    V c;
+    float r = 1.0f;
+    int material = 0;
+};
+
 struct Scene {
     std::vector<Tri> tris;
+    std::vector<Sph> spheres; // primitive tris.size() + k
     std::vector<Material> mats;
     std::vector<Texture> maps; // material map textures
     std::vector<int> lights;
@@ -780,9 +788,12 @@ struct Scene {
 
     // GeometryBase::GetMaterial (GeometryBase.h:59-69), MixMaterial::GetSingleMaterial
     // (MixMaterial.h:45-49)
+    int num_prims() const { return (int)(tris.size() + spheres.size()); }
+    bool is_sphere(int prim) const { return prim >= (int)tris.size(); }
+    const Sph& sphere(int prim) const { return spheres[(size_t)prim - tris.size()]; }
     int material_of(int prim, float rand) const
     {
-        const int m = tris[prim].material;
+        const int m = is_sphere(prim) ? sphere(prim).material : tris[prim].material;
         if (mats[m].type == kMix) return mats[m].mix <= rand ? mats[m].mat0 : mats[m].mat1;
         return m;
     }
@@ -807,6 +818,29 @@ static inline bool tri_intersect(const Tri& tr, const Ray& ray, float* dist_out)
     return true;
 }
 
+// Sphere::Intersect (Sphere.cpp:32-59): the quadratic in float; the roots use the double
+// C sqrt the unqualified call binds to and are rounded to float once each. Both roots
+// behind the origin: a miss; else min(max(l1, 0), max(l2, 0)), which is 0 from inside.
+static inline bool sph_intersect(const Sph& sp, const Ray& ray, float* dist_out)
+{
+    const V oc = sub(ray.o, sp.c);
+    const float a = dot(ray.dir, ray.dir);
+    const float b_2 = dot(ray.dir, oc);
+    const float c = dot(oc, oc) - sp.r * sp.r;
+    const float d_4 = b_2 * b_2 - a * c;
+    if (d_4 < 0.0f) return false;
+    const float l1 = (float)(((double)-b_2 - pt_sqrt_d((double)d_4)) / (double)a);
+    const float l2 = (float)(((double)-b_2 + pt_sqrt_d((double)d_4)) / (double)a);
+    if (l1 < 0.0f && l2 < 0.0f) return false;
+    *dist_out = std::min(std::max(l1, 0.0f), std::max(l2, 0.0f));
+    return true;
+}
+
+static inline bool prim_intersect(const Scene& sc, int prim, const Ray& ray, float* t)
+{
+    return sc.is_sphere(prim) ? sph_intersect(sc.sphere(prim), ray, t) : tri_intersect(sc.tris[prim], ray, t);
+}
+
 // Triangle::Interpolate's texture coordinates (Triangle.cpp:90-108):
 // weightE1 * (uv1 - uv0) + weightE2 * (uv2 - uv0) + uv0.
 static V hit_uv_of(const Tri& tr, const Ray& ray)
@@ -819,14 +853,27 @@ static V hit_uv_of(const Tri& tr, const Ray& ray)
     const float w2 = dot(vec, cross(e1, ray.dir)) * inv_det;
     return add(add(scl(sub(tr.uv1, tr.uv0), w1), scl(sub(tr.uv2, tr.uv0), w2)), tr.uv0);
 }
-static V hit_uv(const Scene& sc, int prim, const Ray& ray) { return hit_uv_of(sc.tris[prim], ray); }
+static V hit_uv(const Scene& sc, int prim, const Ray& ray)
+{
+    return sc.is_sphere(prim) ? mk(0.0f, 0.0f, 0.0f) : hit_uv_of(sc.tris[prim], ray); // Sphere.cpp:81: no UV
+}
 
 // Triangle::Interpolate (Triangle.cpp:90-153): position, shading normal, material.
 static Shading interpolate(const Scene& sc, int prim, const Ray& ray, float t)
 {
-    const Tri& tr = sc.tris[prim];
     Shading s;
     s.pos = add(ray.o, scl(ray.dir, t));
+    if (sc.is_sphere(prim)) {
+        // Sphere::Interpolate (Sphere.cpp:61-88): away from the centre, towards it from
+        // inside; no texture coordinates, the tangent left zero
+        const Sph& sp = sc.sphere(prim);
+        const float l2 = sqlen(sub(ray.o, sp.c));
+        const V n = normalized(sub(s.pos, sp.c));
+        s.normal = l2 > sp.r * sp.r ? n : neg(n);
+        s.material = sc.material_of(prim, 0.0f);
+        return s;
+    }
+    const Tri& tr = sc.tris[prim];
     const V e1 = sub(tr.p1, tr.p0);
     const V e2 = sub(tr.p2, tr.p0);
     const V ce = cross(e1, e2);
@@ -902,6 +949,33 @@ static void tri_sample(const Tri& tr, Sampler2D& s2, V* pos, V* normal, float* p
     *pdf_area = 1.0f / (0.5f * length(cross(e0, e1)));
 }
 
+// Sphere::SampleSurface (Sphere.cpp:90-115): the cone the sphere subtends from p, in
+// OrthonormalBasis::Build(p - c) (Duff et al.), theta = acos(1 - u0 (1 - r / l)) and phi =
+// 2 pi u1 through GetDir with the double libm; pdf l / (2 pi r^2 (l - r)).
+static void sph_sample(const Sph& sp, V p, Sampler2D& s2, V* pos, V* normal, float* pdf_area)
+{
+    const V op = sub(p, sp.c);
+    const V w = normalized(op);
+    const float sign = std::copysign(1.0f, w.z);
+    const float a = -1.0f / (sign + w.z);
+    const float b = w.x * w.y * a;
+    const V u = mk(1.0f + sign * w.x * w.x * a, sign * b, -sign * w.x);
+    const V v = mk(b, sign + w.y * w.y * a, -w.y);
+    float u0, u1;
+    s2.Next(&u0, &u1);
+    const float l = length(op);
+    const float theta = (float)pt_acos((double)(1.0f - u0 * (1.0f - sp.r / l)));
+    const float phi = 2.0f * kPi * u1;
+    double st, ct, sps, cps;
+    pt_sincos((double)theta, &st, &ct);
+    pt_sincos((double)phi, &sps, &cps);
+    const float cx = (float)(st * cps), cy = (float)(st * sps), cz = (float)ct;
+    const V dir = add(add(scl(u, cx), scl(v, cy)), scl(w, cz));
+    *pos = add(sp.c, scl(dir, sp.r));
+    *normal = dir;
+    *pdf_area = l / (2.0f * kPi * sp.r * sp.r * (l - sp.r));
+}
+
 // ---- BinnedSAHBVH: Core/Accel/BinnedSAHBVH.cpp ---------------------------------
 struct Node {
     AABB box;
@@ -921,11 +995,22 @@ struct BVH {
 // exact and order-free, so the left/right boxes and counts are identical.
 static void build_bvh(const Scene& sc, int num_bins, BVH* bvh)
 {
-    const int n = (int)sc.tris.size();
+    const int n = sc.num_prims();
     std::vector<AABB> pbox(n);
     std::vector<V> cen(n);
     std::vector<int> bin(n, -1);
     for (int i = 0; i < n; ++i) {
+        if (sc.is_sphere(i)) {
+            // Sphere::CalcBoundary (Sphere.cpp:24-30), GetCentroid = origin (Sphere.h)
+            const Sph& sp = sc.sphere(i);
+            const V rr = scl(mk(1.0f, 1.0f, 1.0f), sp.r);
+            AABB b;
+            b.lower = sub(sp.c, rr);
+            b.upper = add(sp.c, rr);
+            pbox[i] = b;
+            cen[i] = sp.c;
+            continue;
+        }
         const Tri& t = sc.tris[i];
         AABB b;
         b.merge(t.p0);
@@ -1105,7 +1190,7 @@ static int bvh_intersect(const Scene& sc, const BVH& bvh, const Ray& ray, float 
                 const int prim = bvh.ids[k];
                 ++nt;
                 float t;
-                if (tri_intersect(sc.tris[prim], ray, &t)) {
+                if (prim_intersect(sc, prim, ray, &t)) {
                     if (t < dmin || t > dmax) continue;
                     if (best < 0 || t < best_t) {
                         best = prim;
@@ -1138,9 +1223,9 @@ static int brute_intersect(const Scene& sc, const Ray& ray, float dmin, float dm
 {
     int best = -1;
     float best_t = 0.0f;
-    for (int i = 0; i < (int)sc.tris.size(); ++i) {
+    for (int i = 0; i < sc.num_prims(); ++i) {
         float t;
-        if (tri_intersect(sc.tris[i], ray, &t)) {
+        if (prim_intersect(sc, i, ray, &t)) {
             if (t < dmin || t > dmax) continue;
             if (best < 0 || t < best_t) {
                 best = i;
@@ -1184,14 +1269,14 @@ static bool sample_light(Ctx& c, V origin, float rnd, Sampler2D& s2, float* pdf_
         index = std::min(index, (int)L);
         if ((size_t)index >= L) return false;
     }
-    (void)origin;
     float sum_area = 0.0f;
     *pl_pos = mk(0.0f, 0.0f, 0.0f);
     *pl_nrm = mk(0.0f, 0.0f, 0.0f);
     for (size_t i = 0; i < L; i++) {
         float pa = 0.0f;
         V p, n;
-        tri_sample(sc.tris[sc.lights[i]], s2, &p, &n, &pa);
+        if (sc.is_sphere(sc.lights[i])) sph_sample(sc.sphere(sc.lights[i]), origin, s2, &p, &n, &pa);
+        else tri_sample(sc.tris[sc.lights[i]], s2, &p, &n, &pa);
         sum_area += 1.0f / pa;
         if ((int64_t)i == (int64_t)index) {
             *pl_pos = p;
@@ -1518,7 +1603,7 @@ void* oracle_bvh_from_tables(void* s, int num_nodes, const float* boxes, const i
         n.end = range[2 * i + 1];
         n.leaf = leaf[i] != 0;
     }
-    const int nt = (int)((Scene*)s)->tris.size();
+    const int nt = ((Scene*)s)->num_prims();
     b->ids.assign(prim_ids, prim_ids + nt);
     return b;
 }
@@ -1995,6 +2080,44 @@ void oracle_interpolate_frame(const float* pos, const float* nrm, const float* u
     tangent_out[0] = si.tangent.x;
     tangent_out[1] = si.tangent.y;
     tangent_out[2] = si.tangent.z;
+}
+
+// Spheres (centre, radius; material each) as primitives after the triangles.
+void oracle_scene_set_spheres(void* s, int n, const float* spheres, const int32_t* mats)
+{
+    Scene* sc = (Scene*)s;
+    sc->spheres.resize((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        sc->spheres[i].c = mk(spheres[4 * i], spheres[4 * i + 1], spheres[4 * i + 2]);
+        sc->spheres[i].r = spheres[4 * i + 3];
+        sc->spheres[i].material = mats[i];
+    }
+}
+
+// Sphere::Intersect, Interpolate and SampleSurface of one sphere: out8 = (hit, t, normal at
+// the hit, then (position, pdf) of SampleSurface from p with a RandomSampler2D(seed0, seed1)).
+void oracle_sphere(const float* sphere, const float* ray6, const float* p, uint32_t seed0, uint32_t seed1,
+                   float* out12)
+{
+    Sph sp;
+    sp.c = mk(sphere[0], sphere[1], sphere[2]);
+    sp.r = sphere[3];
+    Ray r;
+    r.o = mk(ray6[0], ray6[1], ray6[2]);
+    r.dir = mk(ray6[3], ray6[4], ray6[5]);
+    float t = 0.0f;
+    const bool hit = sph_intersect(sp, r, &t);
+    Scene sc;
+    sc.spheres.push_back(sp);
+    sc.mats.push_back(Material());
+    const Shading si = interpolate(sc, 0, r, t);
+    Sampler2D s2(seed0, seed1);
+    V pos, nrm;
+    float pa = 0.0f;
+    sph_sample(sp, mk(p[0], p[1], p[2]), s2, &pos, &nrm, &pa);
+    const float o[12] = {hit ? 1.0f : 0.0f, t, si.normal.x, si.normal.y, si.normal.z, pos.x, pos.y, pos.z, pa,
+                         nrm.x, nrm.y, nrm.z};
+    std::memcpy(out12, o, sizeof o);
 }
 
 // GGX normal maps (one texture index per material, -1 none; or null) and map strengths (2

@@ -42,6 +42,13 @@ void oracle_scene_set_uv(void* scene, const float* uv);
  * material (albedo or reflectance map, roughness map; -1 none). */
 void oracle_scene_set_maps(void* scene, int num_textures, const int32_t* sizes, const float* const* rgb,
                            const int32_t* mat_maps);
+/* Spheres (centre, radius; a material each) as primitives after the triangles. */
+void oracle_scene_set_spheres(void* scene, int n, const float* spheres, const int32_t* mats);
+/* Sphere::Intersect, Interpolate and SampleSurface of one sphere (4 floats): out12 = hit,
+ * t, the normal at the hit, the sampled position and its pdf from point p (RandomSampler2D
+ * seeds), the sampled normal. */
+void oracle_sphere(const float* sphere, const float* ray6, const float* p, uint32_t seed0, uint32_t seed1,
+                   float* out12);
 /* GGX normal maps (one texture index per material, -1 none; or NULL) and map strengths
  * (2 per material: normal, roughness; or NULL), over the textures set above. */
 void oracle_scene_set_normal_maps(void* scene, const int32_t* nmaps, const float* strengths);

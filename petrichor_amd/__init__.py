@@ -40,6 +40,7 @@ class SceneDesc(ctypes.Structure):
         ("env_importance_sampling", ctypes.c_int32), ("uvs", _f),
         ("num_textures", ctypes.c_int32), ("texture_sizes", _i), ("textures", ctypes.POINTER(_f)),
         ("material_maps", _i), ("material_normal_maps", _i), ("material_map_strengths", _f),
+        ("num_spheres", ctypes.c_int32), ("spheres", _f), ("sphere_material_ids", _i),
     ]
 
 
@@ -338,6 +339,7 @@ class Scene:
         ctx.check(L.pt_scene_create(ctx.handle, ctypes.byref(d), ctypes.byref(h)), "pt_scene_create")
         self.handle = h
         self.num_triangles = d.num_triangles
+        self.num_primitives = d.num_triangles + d.num_spheres  # triangles, then spheres
         self._keep = []  # the library copied everything
 
     def build_bvh(self, bins=16, leaf_max=4, width=2, builder=BUILD_DEVICE):
@@ -410,6 +412,11 @@ def scene_desc(scene):
             d.material_normal_maps = _ptr(k(scene["mat_nmaps"], np.int32), _i)
     if scene.get("mat_map_strength") is not None:
         d.material_map_strengths = _ptr(k(scene["mat_map_strength"], np.float32), _f)
+    if scene.get("spheres") is not None and len(scene["spheres"]) > 0:
+        sp = k(scene["spheres"], np.float32).reshape(-1, 4)
+        d.num_spheres = sp.shape[0]
+        d.spheres = _ptr(sp, _f)
+        d.sphere_material_ids = _ptr(k(scene["sphere_mat"], np.int32), _i)
     return d, keep
 
 
@@ -434,7 +441,7 @@ class Bvh:
         return {"num_nodes": nn.value, "max_depth": md.value, "build_ms": ms.value}
 
     def export_order(self):
-        out = np.zeros(self.scene.num_triangles, np.int32)
+        out = np.zeros(self.scene.num_primitives, np.int32)
         self.ctx.check(self.ctx._lib.pt_bvh_export_order(self.handle, _ptr(out, _i)), "pt_bvh_export_order")
         return out
 

@@ -57,9 +57,10 @@ def build(verbose=False):
     _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-I", os.path.join(ROOT, "include"),
           os.path.join(ROOT, "tests", "cpp", "load_api.cpp"), "-L", HERE, "-lpetrichor_hip",
           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib", "-o", os.path.join(bdir, "load_api")])
-    _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-I", os.path.join(ROOT, "include"),
-          os.path.join(ROOT, "tests", "cpp", "sampler_api.cpp"), "-L", HERE, "-lpetrichor_hip",
-          "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib", "-o", os.path.join(bdir, "sampler_api")])
+    for drv in ("sampler_api", "sphere_api"):
+        _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-I", os.path.join(ROOT, "include"),
+              os.path.join(ROOT, "tests", "cpp", drv + ".cpp"), "-L", HERE, "-lpetrichor_hip",
+              "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib", "-o", os.path.join(bdir, drv)])
     # test oracle (CPU restatement; also builds oracle/_ref where the reference exists)
     subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     return LIB

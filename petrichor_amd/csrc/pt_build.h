@@ -39,24 +39,26 @@ struct GpuPacked {
     int32_t* mat_sh_d = nullptr;
     int32_t* smooth_d = nullptr;  // only when the scene has per-vertex normals
     float* nrm_d = nullptr;
+    int8_t* kinds_d = nullptr;    // per leaf slot: 1 for a sphere; only when the scene has spheres
     int num_inner = 0;
     int num_big = 0;
     V3 root_lo, root_hi;
     uint32_t root_ref = 0;
 };
 
-// Builds the tree over num_tris triangles (9 floats each, device memory). The result's
-// device arrays belong to the caller (gpu_bvh_free). Synchronises the stream once per
-// level of the tree.
-hipError_t gpu_build_bvh(const float* tri_pos_d, int num_tris, int num_bins, int leaf_max, hipStream_t st,
-                         GpuBvhResult* out);
+// Builds the tree over num_tris triangles (9 floats each, device memory) followed by
+// num_spheres spheres (centre, radius), primitives num_tris... The result's device arrays
+// belong to the caller (gpu_bvh_free). Synchronises the stream once per level of the tree.
+hipError_t gpu_build_bvh(const float* tri_pos_d, int num_tris, const float4* spheres_d, int num_spheres, int num_bins,
+                         int leaf_max, hipStream_t st, GpuBvhResult* out);
 void gpu_bvh_free(GpuBvhResult* r);
 
 // Packs a built tree into the traversal layout and gathers the per-slot triangle data.
 // smooth_d / nrm_d may be null (flat shading only). The packed arrays belong to the caller.
-hipError_t gpu_pack_bvh(const GpuBvhResult& r, int num_tris, const float* tri_pos_d, const int32_t* tri_mat_d,
-                        const int32_t* mat_shading_of_d, const int32_t* smooth_d, const float* nrm_d,
-                        hipStream_t st, GpuPacked* out);
+// The per-primitive arrays (materials, smooth flags, normals) cover triangles and spheres.
+hipError_t gpu_pack_bvh(const GpuBvhResult& r, int num_tris, const float* tri_pos_d, const float4* spheres_d,
+                        int num_spheres, const int32_t* tri_mat_d, const int32_t* mat_shading_of_d,
+                        const int32_t* smooth_d, const float* nrm_d, hipStream_t st, GpuPacked* out);
 void gpu_packed_free(GpuPacked* p);
 
 } // namespace pt

@@ -55,8 +55,8 @@ __device__ __forceinline__ float next_toward_zero(float x)
 // 64 triangles and adds its running bounds to the root keys once (with one atomic per
 // wave and group, every wave of the grid would queue on the same six keys).
 #define PT_PRIMS_SPAN 16
-__global__ void k_prims(const float* __restrict__ tri_pos, int n, float4* cen, float4* blo, float4* bhi,
-                        uint32_t* root_keys)
+__global__ void k_prims(const float* __restrict__ tri_pos, int ntri, const float4* __restrict__ spheres, int n,
+                        float4* cen, float4* blo, float4* bhi, uint32_t* root_keys)
 {
     const int lane = (int)(threadIdx.x & 63);
     const int first = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * PT_PRIMS_SPAN;
@@ -64,13 +64,23 @@ __global__ void k_prims(const float* __restrict__ tri_pos, int n, float4* cen, f
     for (int it = 0; it < PT_PRIMS_SPAN; ++it) {
         const int i = first + 64 * it + lane;
         if (i >= n) break;
-        const float* p = tri_pos + 9 * (size_t)i;
-        const V3 p0 = v3(p[0], p[1], p[2]), p1 = v3(p[3], p[4], p[5]), p2 = v3(p[6], p[7], p[8]);
-        V3 lo = v3(smin(smin(FLT_MAX, p0.x), p1.x), smin(smin(FLT_MAX, p0.y), p1.y), smin(smin(FLT_MAX, p0.z), p1.z));
-        lo = v3(smin(lo.x, p2.x), smin(lo.y, p2.y), smin(lo.z, p2.z));
-        V3 hi = v3(smax(smax(-FLT_MAX, p0.x), p1.x), smax(smax(-FLT_MAX, p0.y), p1.y), smax(smax(-FLT_MAX, p0.z), p1.z));
-        hi = v3(smax(hi.x, p2.x), smax(hi.y, p2.y), smax(hi.z, p2.z));
-        const V3 c = (p0 + p1 + p2) / 3.0f; // Triangle::GetCentroid (Triangle.h:45-50)
+        V3 lo, hi, c;
+        if (i < ntri) {
+            const float* p = tri_pos + 9 * (size_t)i;
+            const V3 p0 = v3(p[0], p[1], p[2]), p1 = v3(p[3], p[4], p[5]), p2 = v3(p[6], p[7], p[8]);
+            lo = v3(smin(smin(FLT_MAX, p0.x), p1.x), smin(smin(FLT_MAX, p0.y), p1.y), smin(smin(FLT_MAX, p0.z), p1.z));
+            lo = v3(smin(lo.x, p2.x), smin(lo.y, p2.y), smin(lo.z, p2.z));
+            hi = v3(smax(smax(-FLT_MAX, p0.x), p1.x), smax(smax(-FLT_MAX, p0.y), p1.y), smax(smax(-FLT_MAX, p0.z), p1.z));
+            hi = v3(smax(hi.x, p2.x), smax(hi.y, p2.y), smax(hi.z, p2.z));
+            c = (p0 + p1 + p2) / 3.0f; // Triangle::GetCentroid (Triangle.h:45-50)
+        } else {
+            // Sphere::CalcBoundary (Sphere.cpp:24-30) and GetCentroid (Sphere.h): origin
+            const float4 q = spheres[i - ntri];
+            c = v3(q.x, q.y, q.z);
+            const V3 rr = v3(1.0f, 1.0f, 1.0f) * q.w;
+            lo = c - rr;
+            hi = c + rr;
+        }
         cen[i] = make_float4(c.x, c.y, c.z, 0.0f);
         blo[i] = make_float4(lo.x, lo.y, lo.z, 0.0f);
         bhi[i] = make_float4(hi.x, hi.y, hi.z, 0.0f);
@@ -807,6 +817,9 @@ __global__ void k_pack_nodes(const BuildNode* nodes, int num_nodes, const int* p
 
 struct GatherArgs {
     const float* tri_pos;
+    const float4* spheres; // primitives ntri.. are spheres (centre, radius)
+    int ntri;
+    int8_t* kinds;         // per leaf slot: 1 for a sphere; null without spheres
     const int* ids;
     int n;
     const int* tri_mat;
@@ -827,11 +840,19 @@ __global__ void k_gather(GatherArgs a)
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.n) return;
     const int id = a.ids[k];
-    const float* p = a.tri_pos + 9 * (size_t)id;
-    const TriDev t = tri_from_points(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), v3(p[6], p[7], p[8]));
-    a.tris[3 * (size_t)k] = make_float4(t.p0.x, t.p0.y, t.p0.z, t.e1.x);
-    a.tris[3 * (size_t)k + 1] = make_float4(t.e1.y, t.e1.z, t.e2.x, t.e2.y);
-    a.tris[3 * (size_t)k + 2] = make_float4(t.e2.z, t.n.x, t.n.y, t.n.z);
+    if (id < a.ntri) {
+        const float* p = a.tri_pos + 9 * (size_t)id;
+        const TriDev t = tri_from_points(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), v3(p[6], p[7], p[8]));
+        a.tris[3 * (size_t)k] = make_float4(t.p0.x, t.p0.y, t.p0.z, t.e1.x);
+        a.tris[3 * (size_t)k + 1] = make_float4(t.e1.y, t.e1.z, t.e2.x, t.e2.y);
+        a.tris[3 * (size_t)k + 2] = make_float4(t.e2.z, t.n.x, t.n.y, t.n.z);
+    } else {
+        // a sphere's slot: (centre, radius), then zeros
+        a.tris[3 * (size_t)k] = a.spheres[id - a.ntri];
+        a.tris[3 * (size_t)k + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.tris[3 * (size_t)k + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    if (a.kinds) a.kinds[k] = id < a.ntri ? 0 : 1;
     a.prim_ids[k] = id;
     const int m = a.tri_mat[id];
     a.mat[k] = m;
@@ -884,9 +905,10 @@ void gpu_bvh_free(GpuBvhResult* r)
     r->ids_d = nullptr;
 }
 
-hipError_t gpu_build_bvh(const float* tri_pos_d, int n, int num_bins, int leaf_max, hipStream_t st,
-                         GpuBvhResult* out)
+hipError_t gpu_build_bvh(const float* tri_pos_d, int ntri, const float4* spheres_d, int num_spheres, int num_bins,
+                         int leaf_max, hipStream_t st, GpuBvhResult* out)
 {
+    const int n = ntri + num_spheres;
     *out = GpuBvhResult();
     if (n <= 0 || num_bins < 2 || num_bins > 64 || leaf_max < 1) return hipErrorInvalidValue;
     Scratch tmp;
@@ -935,7 +957,7 @@ hipError_t gpu_build_bvh(const float* tri_pos_d, int n, int num_bins, int leaf_m
     // primitives, the root box and the identity order
     hipLaunchKernelGGL(k_fill_keyboxes, dim3(1), dim3(64), 0, st, root_keys, (size_t)1);
     hipLaunchKernelGGL(k_prims, dim3(blocks(((size_t)n + PT_PRIMS_SPAN - 1) / PT_PRIMS_SPAN)), dim3(256), 0, st,
-                       tri_pos_d, n, cen, blo, bhi, root_keys);
+                       tri_pos_d, ntri, spheres_d, n, cen, blo, bhi, root_keys);
     hipLaunchKernelGGL(k_iota, dim3(blocks(n)), dim3(256), 0, st, ids[0], n);
     B_CHECK(hipGetLastError());
     uint32_t rk[6];
@@ -1087,10 +1109,11 @@ hipError_t gpu_build_bvh(const float* tri_pos_d, int n, int num_bins, int leaf_m
     return hipSuccess;
 }
 
-hipError_t gpu_pack_bvh(const GpuBvhResult& r, int num_tris, const float* tri_pos_d, const int32_t* tri_mat_d,
-                        const int32_t* mat_shading_of_d, const int32_t* smooth_d, const float* nrm_d,
-                        hipStream_t st, GpuPacked* out)
+hipError_t gpu_pack_bvh(const GpuBvhResult& r, int ntri, const float* tri_pos_d, const float4* spheres_d,
+                        int num_spheres, const int32_t* tri_mat_d, const int32_t* mat_shading_of_d,
+                        const int32_t* smooth_d, const float* nrm_d, hipStream_t st, GpuPacked* out)
 {
+    const int num_tris = ntri + num_spheres; // leaf slots
     *out = GpuPacked();
     Scratch tmp;
     const int nn = r.num_nodes;
@@ -1135,8 +1158,12 @@ hipError_t gpu_pack_bvh(const GpuBvhResult& r, int num_tris, const float* tri_po
     B_CHECK(hipMalloc((void**)&out->mat_sh_d, sizeof(int) * (size_t)num_tris));
     if (smooth_d) B_CHECK(hipMalloc((void**)&out->smooth_d, sizeof(int) * (size_t)num_tris));
     if (nrm_d) B_CHECK(hipMalloc((void**)&out->nrm_d, sizeof(float) * 9 * (size_t)num_tris));
+    if (num_spheres > 0) B_CHECK(hipMalloc((void**)&out->kinds_d, (size_t)num_tris));
     GatherArgs ga;
     ga.tri_pos = tri_pos_d;
+    ga.spheres = spheres_d;
+    ga.ntri = ntri;
+    ga.kinds = out->kinds_d;
     ga.ids = r.ids_d;
     ga.n = num_tris;
     ga.tri_mat = tri_mat_d;
@@ -1163,7 +1190,7 @@ hipError_t gpu_pack_bvh(const GpuBvhResult& r, int num_tris, const float* tri_po
 void gpu_packed_free(GpuPacked* p)
 {
     for (void* q : {(void*)p->nodes_d, (void*)p->big_leaves_d, (void*)p->inner_range_d, (void*)p->tris_d, (void*)p->prim_ids_d,
-                    (void*)p->mat_d, (void*)p->mat_sh_d, (void*)p->smooth_d, (void*)p->nrm_d})
+                    (void*)p->mat_d, (void*)p->mat_sh_d, (void*)p->smooth_d, (void*)p->nrm_d, (void*)p->kinds_d})
         if (q) (void)hipFree(q);
     *p = GpuPacked();
 }

@@ -66,16 +66,17 @@ struct Box {
 
 } // namespace
 
-bool build_bvh_host(const float* tri_pos, int num_tris, int num_bins, int leaf_max, HostBvh* out)
+bool build_bvh_host(const float* tri_pos, int num_tris, int num_bins, int leaf_max, HostBvh* out,
+                    const float* spheres, int num_spheres)
 {
+    const int n = num_tris + num_spheres;
     out->nodes.clear();
-    out->order.resize(num_tris);
+    out->order.resize(n);
     out->max_depth = 0;
     if (num_bins < 2 || num_bins > 256 || leaf_max < 1) return false;
-    const int n = num_tris;
     std::vector<Box> pbox(n);
     std::vector<V3> cen(n);
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < num_tris; ++i) {
         const float* p = tri_pos + 9 * (size_t)i;
         const V3 p0 = v3(p[0], p[1], p[2]), p1 = v3(p[3], p[4], p[5]), p2 = v3(p[6], p[7], p[8]);
         Box b; // Triangle::CalcBoundary (Triangle.cpp:41-49)
@@ -84,6 +85,14 @@ bool build_bvh_host(const float* tri_pos, int num_tris, int num_bins, int leaf_m
         b.merge(p2);
         pbox[i] = b;
         cen[i] = (p0 + p1 + p2) / 3.0f; // Triangle::GetCentroid (Triangle.h:45-50)
+    }
+    for (int k = 0; k < num_spheres; ++k) {
+        // Sphere::CalcBoundary (Sphere.cpp:24-30): origin -+ One * radius; GetCentroid: origin
+        const float* q = spheres + 4 * (size_t)k;
+        const V3 c = v3(q[0], q[1], q[2]), rr = v3(1.0f, 1.0f, 1.0f) * q[3];
+        pbox[num_tris + k].lo = c - rr;
+        pbox[num_tris + k].hi = c + rr;
+        cen[num_tris + k] = c;
     }
     std::vector<int>& ids = out->order;
     std::iota(ids.begin(), ids.end(), 0);

@@ -22,7 +22,9 @@ struct HostBvh {
     int max_depth = 0;
 };
 
-bool build_bvh_host(const float* tri_pos, int num_tris, int num_bins, int leaf_max, HostBvh* out);
+// Primitives are the triangles (9 floats each) and then the spheres (centre, radius).
+bool build_bvh_host(const float* tri_pos, int num_tris, int num_bins, int leaf_max, HostBvh* out,
+                    const float* spheres = nullptr, int num_spheres = 0);
 void pack_bvh2(const HostBvh& h, std::vector<float4>& nodes, std::vector<int2>& big_leaves, V3* root_lo,
                V3* root_hi, uint32_t* root_ref, std::vector<int2>* inner_range = nullptr);
 

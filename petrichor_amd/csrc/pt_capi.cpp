@@ -126,15 +126,15 @@ int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
     PT_HIP(ctx, hipSetDevice(ctx->device));
     const auto t0 = std::chrono::steady_clock::now();
     GpuBvhResult r;
-    hipError_t e = gpu_build_bvh(s->tri_pos_d, s->num_tris, bins, leaf_max, ctx->stream, &r);
+    hipError_t e = gpu_build_bvh(s->tri_pos_d, s->num_tris, s->spheres_d, s->num_spheres, bins, leaf_max, ctx->stream, &r);
     if (e != hipSuccess) {
         gpu_bvh_free(&r);
         return fail(ctx, e == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
                     std::string("device BVH build: ") + hipGetErrorString(e));
     }
     GpuPacked pk;
-    e = gpu_pack_bvh(r, s->num_tris, s->tri_pos_d, s->tri_mat_d, s->mat_shading_of_d, s->tri_smooth_d, s->tri_nrm_d,
-                     ctx->stream, &pk);
+    e = gpu_pack_bvh(r, s->num_tris, s->tri_pos_d, s->spheres_d, s->num_spheres, s->tri_mat_d, s->mat_shading_of_d,
+                     s->tri_smooth_d, s->tri_nrm_d, ctx->stream, &pk);
     const int num_nodes = r.num_nodes, max_depth = r.max_depth;
     gpu_bvh_free(&r);
     pt_bvh* b = new pt_bvh();
@@ -156,6 +156,8 @@ int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
     b->tri_mat_shading_d = pk.mat_sh_d;
     b->tri_smooth_d = pk.smooth_d;
     b->tri_nrm_d = pk.nrm_d;
+    b->kinds_d = pk.kinds_d;
+    b->view.kinds = b->kinds_d;
     b->view.nodes = b->nodes_d;
     b->view.tris = b->tris_d;
     b->view.prim_ids = b->prim_ids_d;
@@ -170,9 +172,9 @@ int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
     b->num_big = pk.num_big;
     b->host_ready = false;
     b->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    const int64_t nt = s->num_tris;
+    const int64_t nt = s->num_prims();
     b->device_bytes = (int64_t)pk.num_inner * 72 + (int64_t)pk.num_big * 8 + nt * 60 +
-                      (pk.smooth_d ? nt * 4 : 0) + (pk.nrm_d ? nt * 36 : 0);
+                      (pk.smooth_d ? nt * 4 : 0) + (pk.nrm_d ? nt * 36 : 0) + (pk.kinds_d ? nt : 0);
     if (const int rc = add_parents(b)) {
         pt_bvh_destroy(b);
         return rc;
@@ -223,7 +225,7 @@ int ensure_host(pt_bvh* b)
 {
     if (b->host_ready) return PT_OK;
     pt_context* ctx = b->scene->ctx;
-    const int nt = b->scene->num_tris;
+    const int nt = b->scene->num_prims();
     std::vector<float4> nodes(4 * (size_t)b->num_inner);
     std::vector<int2> big((size_t)b->num_big);
     HostBvh& h = b->host;
@@ -338,9 +340,19 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
     for (int i = 0; i < d->num_triangles; ++i)
         if (d->material_ids[i] < 0 || d->material_ids[i] >= d->num_materials)
             return fail(ctx, PT_ERR_ARG, "material id out of range at triangle " + std::to_string(i));
+    if (d->num_spheres < 0 || (d->num_spheres > 0 && (!d->spheres || !d->sphere_material_ids)))
+        return fail(ctx, PT_ERR_ARG, "bad sphere arrays");
+    if ((int64_t)d->num_triangles + d->num_spheres > ((int64_t)1 << 27))
+        return fail(ctx, PT_ERR_ARG, "too many primitives");
+    for (int i = 0; i < d->num_spheres; ++i) {
+        if (d->sphere_material_ids[i] < 0 || d->sphere_material_ids[i] >= d->num_materials)
+            return fail(ctx, PT_ERR_ARG, "material id out of range at sphere " + std::to_string(i));
+        for (int q = 0; q < 4; ++q)
+            if (!std::isfinite(d->spheres[4 * (size_t)i + q])) return fail(ctx, PT_ERR_ARG, "sphere not finite");
+    }
     for (int i = 0; i < d->num_lights; ++i)
-        if (d->light_triangles[i] < 0 || d->light_triangles[i] >= d->num_triangles)
-            return fail(ctx, PT_ERR_ARG, "light triangle out of range");
+        if (d->light_triangles[i] < 0 || d->light_triangles[i] >= d->num_triangles + d->num_spheres)
+            return fail(ctx, PT_ERR_ARG, "light primitive out of range");
     for (int m = 0; m < d->num_materials; ++m) {
         const int t = d->material_types[m];
         if (t < 0 || t > PT_MAT_MIX) return fail(ctx, PT_ERR_UNSUPPORTED, "unknown material type");
@@ -375,21 +387,50 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
     pt_scene* s = new pt_scene();
     s->ctx = ctx;
     s->num_tris = d->num_triangles;
-    const size_t nt = (size_t)d->num_triangles;
+    s->num_spheres = d->num_spheres;
+    const size_t nt = (size_t)d->num_triangles, ns = (size_t)d->num_spheres;
     s->tri_pos.assign(d->positions, d->positions + 9 * nt);
-    if (d->normals) s->tri_nrm.assign(d->normals, d->normals + 9 * nt);
+    // per-primitive arrays: the triangles' entries, then zeros (or the material) per sphere
+    if (d->normals) {
+        s->tri_nrm.assign(d->normals, d->normals + 9 * nt);
+        s->tri_nrm.resize(9 * (nt + ns), 0.0f);
+    }
     s->tri_mat.assign(d->material_ids, d->material_ids + nt);
-    if (d->smooth) s->tri_smooth.assign(d->smooth, d->smooth + nt);
-    if (d->uvs) s->tri_uv.assign(d->uvs, d->uvs + 6 * nt);
+    if (ns) s->tri_mat.insert(s->tri_mat.end(), d->sphere_material_ids, d->sphere_material_ids + ns);
+    if (d->smooth) {
+        s->tri_smooth.assign(d->smooth, d->smooth + nt);
+        s->tri_smooth.resize(nt + ns, 0);
+    }
+    if (d->uvs) {
+        s->tri_uv.assign(d->uvs, d->uvs + 6 * nt);
+        s->tri_uv.resize(6 * (nt + ns), 0.0f);
+    }
+    for (size_t i = 0; i < ns; ++i) {
+        const float* q = d->spheres + 4 * i;
+        s->spheres.push_back(make_float4(q[0], q[1], q[2], q[3]));
+    }
     for (int m = 0; m < d->num_materials; ++m)
         s->mats.push_back(material_from(d->material_types[m], d->material_params + 8 * m));
+    // lights: a triangle record, or a sphere's (centre, radius) and zeros
     std::vector<float4> lights;
+    std::vector<int8_t> light_kinds;
+    bool sphere_lights = false;
     for (int i = 0; i < d->num_lights; ++i) {
-        const float* p = d->positions + 9 * (size_t)d->light_triangles[i];
+        const int id = d->light_triangles[i];
+        if (id >= d->num_triangles) {
+            lights.push_back(s->spheres[(size_t)(id - d->num_triangles)]);
+            lights.push_back(make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+            lights.push_back(make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+            light_kinds.push_back(1);
+            sphere_lights = true;
+            continue;
+        }
+        const float* p = d->positions + 9 * (size_t)id;
         const TriDev t = tri_from_points(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), v3(p[6], p[7], p[8]));
         lights.push_back(make_float4(t.p0.x, t.p0.y, t.p0.z, t.e1.x));
         lights.push_back(make_float4(t.e1.y, t.e1.z, t.e2.x, t.e2.y));
         lights.push_back(make_float4(t.e2.z, t.n.x, t.n.y, t.n.z));
+        light_kinds.push_back(0);
     }
     s->num_lights = d->num_lights;
     const float* c = d->camera;
@@ -409,6 +450,9 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
     e.use_is = d->env_importance_sampling;
     hipError_t err = upload(&s->mats_d, s->mats.data(), s->mats.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&s->lights_d, lights.data(), lights.size(), ctx->stream);
+    if (err == hipSuccess && sphere_lights)
+        err = upload(&s->light_kinds_d, light_kinds.data(), light_kinds.size(), ctx->stream);
+    if (err == hipSuccess) err = upload(&s->spheres_d, s->spheres.data(), s->spheres.size(), ctx->stream);
     if (err == hipSuccess && d->env_rgb) {
         e.w = d->env_width;
         e.h = d->env_height;
@@ -450,13 +494,13 @@ int pt_scene_create(pt_context* ctx, const pt_scene_desc* d, pt_scene** out)
     if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
     s->device_bytes = (int64_t)(s->mats.size() * (sizeof(MaterialDev) + 4) + lights.size() * sizeof(float4)) +
                       (d->env_rgb ? 3 * (int64_t)e.w * e.h * 4 : 0) +
-                      (int64_t)nt * 40 + (int64_t)(s->tri_smooth.size() * 4 + s->tri_nrm.size() * 4) +
+                      (int64_t)nt * 40 + (int64_t)ns * 20 + (int64_t)(s->tri_smooth.size() * 4 + s->tri_nrm.size() * 4) +
                       (int64_t)s->tri_uv.size() * 4 + pt_scene_map_bytes(d);
     if (err != hipSuccess) {
         for (float* p : s->tex_d) (void)hipFree(p);
         free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->mat_shading_of_d, s->tri_pos_d,
                   s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d, s->tri_uv_d, s->textures_d, s->mat_maps_d,
-                  s->mat_nmaps_d, s->map_strength_d});
+                  s->mat_nmaps_d, s->map_strength_d, s->spheres_d, s->light_kinds_d});
         delete s;
         return fail(ctx, err == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
                     std::string("scene upload: ") + hipGetErrorString(err));
@@ -471,7 +515,7 @@ int pt_scene_destroy(pt_scene* s)
     for (float* p : s->tex_d) (void)hipFree(p);
     free_all({s->mats_d, s->lights_d, s->env_rgb_d, s->mat_shading_of_d, s->tri_pos_d,
               s->tri_mat_d, s->tri_smooth_d, s->tri_nrm_d, s->tri_uv_d, s->textures_d, s->mat_maps_d,
-              s->mat_nmaps_d, s->map_strength_d});
+              s->mat_nmaps_d, s->map_strength_d, s->spheres_d, s->light_kinds_d});
     delete s;
     return PT_OK;
 }
@@ -481,18 +525,18 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out);
 int pt_bvh_build(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
 {
     const int rc = bvh_build_impl(s, desc, out);
-    if (rc != PT_OK || !s->tri_uv_d || s->num_tris == 0) return rc;
+    if (rc != PT_OK || !s->tri_uv_d || s->num_prims() == 0) return rc;
     // texture coordinates per leaf slot, gathered through the leaf order
     pt_bvh* b = *out;
     pt_context* ctx = s->ctx;
-    hipError_t e = hipMalloc((void**)&b->tri_uv_d, 6 * sizeof(float) * (size_t)s->num_tris);
-    if (e == hipSuccess) e = launch_gather_rows(s->tri_uv_d, b->prim_ids_d, s->num_tris, 6, b->tri_uv_d, ctx->stream);
+    hipError_t e = hipMalloc((void**)&b->tri_uv_d, 6 * sizeof(float) * (size_t)s->num_prims());
+    if (e == hipSuccess) e = launch_gather_rows(s->tri_uv_d, b->prim_ids_d, s->num_prims(), 6, b->tri_uv_d, ctx->stream);
     if (e != hipSuccess) {
         pt_bvh_destroy(b);
         *out = nullptr;
         return fail(ctx, e == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP, hipGetErrorString(e));
     }
-    b->device_bytes += 24 * (int64_t)s->num_tris;
+    b->device_bytes += 24 * (int64_t)s->num_prims();
     return PT_OK;
 }
 
@@ -510,7 +554,7 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     if (builder != PT_BUILD_DEVICE && builder != PT_BUILD_HOST) return fail(ctx, PT_ERR_ARG, "unknown builder");
     if (builder == PT_BUILD_DEVICE && bins > 64)
         return fail(ctx, PT_ERR_UNSUPPORTED, "the device builder takes at most 64 bins");
-    if (builder == PT_BUILD_DEVICE && s->num_tris > 0) {
+    if (builder == PT_BUILD_DEVICE && s->num_prims() > 0) {
         const int rc = build_device(s, bins, leaf_max, out);
         if (rc != PT_OK || width == 2) return rc;
         return make_wide(*out, out);
@@ -518,7 +562,8 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     pt_bvh* b = new pt_bvh();
     b->scene = s;
     const auto t0 = std::chrono::steady_clock::now();
-    if (!build_bvh_host(s->tri_pos.data(), s->num_tris, bins, leaf_max, &b->host)) {
+    if (!build_bvh_host(s->tri_pos.data(), s->num_tris, bins, leaf_max, &b->host, (const float*)s->spheres.data(),
+                        s->num_spheres)) {
         delete b;
         return fail(ctx, PT_ERR_ARG, "bad BVH parameters");
     }
@@ -530,19 +575,27 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     V3 rlo, rhi;
     uint32_t rref;
     pack_bvh2(b->host, nodes, big, &rlo, &rhi, &rref, &ranges);
-    const size_t nt = (size_t)s->num_tris;
+    const size_t nt = (size_t)s->num_prims();
     std::vector<float4> tris(3 * nt);
     std::vector<int32_t> mat(nt), mat_sh(nt), smooth;
     std::vector<float> nrm;
+    std::vector<int8_t> kinds;
     if (!s->tri_smooth.empty()) smooth.resize(nt);
     if (!s->tri_nrm.empty()) nrm.resize(9 * nt);
+    if (s->num_spheres > 0) kinds.resize(nt);
     for (size_t k = 0; k < nt; ++k) {
         const int id = b->host.order[k];
-        const float* p = &s->tri_pos[9 * (size_t)id];
-        const TriDev t = tri_from_points(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), v3(p[6], p[7], p[8]));
-        tris[3 * k] = make_float4(t.p0.x, t.p0.y, t.p0.z, t.e1.x);
-        tris[3 * k + 1] = make_float4(t.e1.y, t.e1.z, t.e2.x, t.e2.y);
-        tris[3 * k + 2] = make_float4(t.e2.z, t.n.x, t.n.y, t.n.z);
+        if (id >= s->num_tris) { // a sphere: (centre, radius), then zeros
+            tris[3 * k] = s->spheres[(size_t)(id - s->num_tris)];
+            tris[3 * k + 1] = tris[3 * k + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            kinds[k] = 1;
+        } else {
+            const float* p = &s->tri_pos[9 * (size_t)id];
+            const TriDev t = tri_from_points(v3(p[0], p[1], p[2]), v3(p[3], p[4], p[5]), v3(p[6], p[7], p[8]));
+            tris[3 * k] = make_float4(t.p0.x, t.p0.y, t.p0.z, t.e1.x);
+            tris[3 * k + 1] = make_float4(t.e1.y, t.e1.z, t.e2.x, t.e2.y);
+            tris[3 * k + 2] = make_float4(t.e2.z, t.n.x, t.n.y, t.n.z);
+        }
         const int m = s->tri_mat[id];
         mat[k] = m;
         const MaterialDev& md = s->mats[m];
@@ -561,10 +614,11 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     if (err == hipSuccess) err = upload(&b->tri_mat_shading_d, mat_sh.data(), nt, ctx->stream);
     if (err == hipSuccess) err = upload(&b->tri_smooth_d, smooth.data(), smooth.size(), ctx->stream);
     if (err == hipSuccess) err = upload(&b->tri_nrm_d, nrm.data(), nrm.size(), ctx->stream);
+    if (err == hipSuccess) err = upload(&b->kinds_d, kinds.data(), kinds.size(), ctx->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
     if (err != hipSuccess) {
         free_all({b->nodes_d, b->big_leaves_d, b->inner_range_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
-                  b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d});
+                  b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d, b->kinds_d});
         delete b;
         return fail(ctx, err == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
                     std::string("bvh upload: ") + hipGetErrorString(err));
@@ -572,6 +626,7 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     b->view.nodes = b->nodes_d;
     b->view.tris = b->tris_d;
     b->view.prim_ids = b->prim_ids_d;
+    b->view.kinds = b->kinds_d;
     b->view.root_lo = rlo;
     b->view.root_hi = rhi;
     b->view.root_ref = rref;
@@ -580,7 +635,7 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     b->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     b->device_bytes = (int64_t)(nodes.size() * sizeof(float4) + big.size() * sizeof(int2) + tris.size() * sizeof(float4)) +
                       (int64_t)ranges.size() * 8 +
-                      (int64_t)nt * 12 + (int64_t)(smooth.size() * 4 + nrm.size() * 4);
+                      (int64_t)nt * 12 + (int64_t)(smooth.size() * 4 + nrm.size() * 4 + kinds.size());
     b->num_inner = (int)(nodes.size() / 4);
     if (const int rc = add_parents(b)) {
         pt_bvh_destroy(b);
@@ -596,7 +651,7 @@ int pt_bvh_destroy(pt_bvh* b)
     if (!b) return PT_ERR_ARG;
     free_all({b->nodes8_d, b->nodes_d, b->big_leaves_d, b->inner_range_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
               b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d, b->order_keys_d, b->order_idx_d, b->order_temp_d,
-              b->tri_uv_d, b->parents_d});
+              b->tri_uv_d, b->parents_d, b->kinds_d});
     delete b;
     return PT_OK;
 }

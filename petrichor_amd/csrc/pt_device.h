@@ -295,6 +295,66 @@ PT_HD void tri_sample(const TriDev& tr, Samplers& s, V3* pos, V3* normal, float*
     *pdf_area = 1.0f / (0.5f * length(tr.n));
 }
 
+// ---- spheres (Core/Geometry/Sphere.cpp) --------------------------------------------
+// A sphere's leaf slot holds (centre, radius) in its first float4 (TriDev: p0 and e1.x).
+
+// Sphere::Intersect (Sphere.cpp:32-59): the quadratic in float, its roots with the double
+// sqrt the unqualified call binds to, each rounded to float once; a ray whose two roots
+// are both behind its origin misses; else min(max(l1, 0), max(l2, 0)), which is 0 for an
+// origin inside the sphere.
+PT_HD bool sphere_intersect(V3 c, float r, V3 o, V3 d, float* t)
+{
+    const V3 oc = o - c;
+    const float a = dot(d, d);
+    const float b2 = dot(d, oc);
+    const float cc = dot(oc, oc) - r * r;
+    const float d4 = b2 * b2 - a * cc;
+    if (d4 < 0.0f) return false;
+    const double sq = pt_sqrt_d((double)d4);
+    const float l1 = (float)(((double)-b2 - sq) / (double)a);
+    const float l2 = (float)(((double)-b2 + sq) / (double)a);
+    if (l1 < 0.0f && l2 < 0.0f) return false;
+    *t = smin(smax(l1, 0.0f), smax(l2, 0.0f));
+    return true;
+}
+
+// Sphere::Interpolate (Sphere.cpp:61-88): the normal points away from the centre, or
+// towards it when the ray starts inside (|o - c|^2 <= r^2).
+PT_HD void sphere_shading(V3 c, float r, V3 o, V3 d, float t, V3* pos, V3* normal)
+{
+    *pos = o + d * t;
+    const float l2 = sq_length(o - c);
+    const V3 n = normalized(*pos - c);
+    *normal = l2 > r * r ? n : -n;
+}
+
+// Sphere::SampleSurface (Sphere.cpp:90-115): a direction in the cone the sphere subtends
+// from p, in the frame OrthonormalBasis::Build(p - c) (Duff et al. 2017), theta = acos(1 -
+// u0 (1 - r / l)), phi = 2 pi u1, through GetDir; pdf l / (2 pi r^2 (l - r)).
+PT_HD void sphere_sample(V3 c, float r, V3 p, Samplers& s, V3* pos, V3* normal, float* pdf_area)
+{
+    const V3 op = p - c;
+    const V3 w = normalized(op);
+    const float sign = copysignf_(1.0f, w.z);
+    const float a = -1.0f / (sign + w.z);
+    const float b = w.x * w.y * a;
+    const V3 u = v3(1.0f + sign * w.x * w.x * a, sign * b, -sign * w.x);
+    const V3 vv = v3(b, sign + w.y * w.y * a, -w.y);
+    float u0, u1;
+    next2(s, &u0, &u1);
+    const float l = length(op);
+    const float theta = (float)pt_acos((double)(1.0f - u0 * (1.0f - r / l)));
+    const float phi = 2.0f * PT_PI * u1;
+    double st, ct, sp, cp;
+    pt_sincos((double)theta, &st, &ct);
+    pt_sincos((double)phi, &sp, &cp);
+    const float cx = (float)(st * cp), cy = (float)(st * sp), cz = (float)ct;
+    const V3 dir = cx * u + cy * vv + cz * w;
+    *pos = c + r * dir;
+    *normal = dir;
+    *pdf_area = l / (2.0f * PT_PI * r * r * (l - r));
+}
+
 // ---- materials (Core/Material) ----------------------------------------------------
 enum { kMatLambert = 0, kMatGlass = 1, kMatEmission = 2, kMatGlossy = 3, kMatMix = 4 };
 

@@ -352,6 +352,7 @@ struct pt_engine {
         sc.map_strength = s->map_strength_d;
         sc.mats = s->mats_d;
         sc.lights = s->lights_d;
+        sc.light_kinds = s->light_kinds_d;
         sc.num_lights = s->num_lights;
         sc.cam = s->cam;
         sc.env = s->env;

@@ -20,7 +20,12 @@ struct pt_context {
 
 struct pt_scene {
     pt_context* ctx = nullptr;
-    int num_tris = 0;
+    int num_tris = 0;    // triangles: primitives 0 .. num_tris - 1
+    int num_spheres = 0; // spheres: primitives num_tris .. num_prims() - 1
+    int num_prims() const { return num_tris + num_spheres; }
+    std::vector<float4> spheres;        // centre, radius
+    float4* spheres_d = nullptr;
+    int8_t* light_kinds_d = nullptr;    // per light: 1 for a sphere; null without sphere lights
     std::vector<float> tri_pos, tri_nrm;
     std::vector<int32_t> tri_mat, tri_smooth;
     std::vector<pt::MaterialDev> mats;
@@ -30,12 +35,14 @@ struct pt_scene {
     pt::CameraDev cam;
     pt::EnvDev env;
     float* env_rgb_d = nullptr;
-    // triangle data resident for the device BVH builder
+    // primitive data resident for the device BVH builder; the per-primitive arrays
+    // (materials, smooth flags, normals, texture coordinates) cover triangles and then
+    // spheres (zeros for a sphere)
     float* tri_pos_d = nullptr;        // 9 floats per triangle
     int32_t* tri_mat_d = nullptr;
     int32_t* tri_smooth_d = nullptr;   // null without per-vertex normals
     float* tri_nrm_d = nullptr;
-    std::vector<float> tri_uv;         // 6 floats per triangle, or empty
+    std::vector<float> tri_uv;         // 6 floats per primitive, or empty
     float* tri_uv_d = nullptr;
     std::vector<float*> tex_d;         // material map textures
     pt::TexDev* textures_d = nullptr;
@@ -69,6 +76,7 @@ struct pt_bvh {
     uint4* nodes8_d = nullptr; // 8-wide nodes (width 8 only)
     float* tri_uv_d = nullptr; // 6 floats per leaf slot, when the scene has texture coordinates
     int32_t* parents_d = nullptr; // inner nodes' parents, for trees deeper than the traversal stack
+    int8_t* kinds_d = nullptr;    // per leaf slot: 1 for a sphere; only when the scene has spheres
     int width = 2;
     int num_nodes8 = 0;
     int depth8 = 0;

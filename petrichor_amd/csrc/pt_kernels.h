@@ -28,6 +28,8 @@ struct BvhView {
     // parent of every binary inner node (-1 for the root), set only for trees deeper than
     // the traversal stack, which are then walked without one
     const int32_t* parents = nullptr;
+    // per leaf slot: 1 for a sphere (its record holds centre and radius); null without spheres
+    const int8_t* kinds = nullptr;
 };
 
 struct SceneView {
@@ -42,7 +44,8 @@ struct SceneView {
     const int* mat_nmaps = nullptr;   // per material: the normal map of a GGX (-1: none); or null
     const float2* map_strength = nullptr; // per material: normal and roughness map strengths; or null (1, 1)
     const MaterialDev* mats;
-    const float4* lights;           // light triangles, AppendLight order, 3 float4 each
+    const float4* lights;           // lights in AppendLight order, 3 float4 each: a triangle record, or a sphere's
+    const int8_t* light_kinds = nullptr; // per light: 1 for a sphere; null without sphere lights
     int num_lights;
     CameraDev cam;
     EnvDev env;

@@ -52,11 +52,18 @@ __device__ __forceinline__ int material_of(const SceneView& s, int slot, float r
     return m;
 }
 
-// Shading point of a hit (Triangle::Interpolate, Triangle.cpp:90-153) at leaf slot k, of
-// the triangle tr loaded from that slot.
+__device__ __forceinline__ bool is_sphere(const SceneView& s, int k) { return s.bvh.kinds != nullptr && s.bvh.kinds[k] != 0; }
+
+// Shading point of a hit (Triangle::Interpolate, Triangle.cpp:90-153, or
+// Sphere::Interpolate, Sphere.cpp:61-88) at leaf slot k, of the record tr loaded from that
+// slot.
 __device__ __forceinline__ void shade_point_tr(const SceneView& s, const TriDev& tr, int k, V3 o, V3 d, float t,
                                                V3* pos, V3* nrm)
 {
+    if (is_sphere(s, k)) {
+        sphere_shading(tr.p0, tr.e1.x, o, d, t, pos, nrm);
+        return;
+    }
     const bool smooth = s.tri_smooth != nullptr && s.tri_smooth[k] != 0;
     V3 n0 = v3(0.0f, 0.0f, 1.0f), n1 = n0, n2 = n0;
     if (smooth) {
@@ -77,7 +84,7 @@ __device__ __forceinline__ void shade_point(const SceneView& s, int k, V3 o, V3 
 // third component zero; all zero when the scene has no texture coordinates.
 __device__ __forceinline__ V3 hit_uv(const SceneView& s, int k, V3 o, V3 d)
 {
-    if (s.tri_uv == nullptr) return v3(0.0f, 0.0f, 0.0f);
+    if (s.tri_uv == nullptr || is_sphere(s, k)) return v3(0.0f, 0.0f, 0.0f); // a sphere has none (Sphere.cpp:81)
     const TriDev tr = load_tri(s.bvh.tris, k);
     const float inv_det = 1.0f / dot(-d, tr.n);
     const V3 vec = o - tr.p0;
@@ -120,16 +127,19 @@ __device__ __forceinline__ MaterialDev material_at(const SceneView& s, int m, V3
 // with the strength s, not normalised again.
 __device__ __forceinline__ V3 ggx_map_normal_at(const SceneView& s, int m, int k, V3 n, V3 uv)
 {
-    const TriDev tr = load_tri(s.bvh.tris, k);
-    float du1 = 0.0f, du2 = 0.0f;
-    if (s.tri_uv != nullptr) {
-        const float* q = s.tri_uv + 6 * (size_t)k;
-        du1 = q[3] - q[1]; // diffUV1.y = uv1.v - uv0.v
-        du2 = q[5] - q[1]; // diffUV2.y = uv2.v - uv0.v
+    V3 tg = v3(0.0f, 0.0f, 0.0f); // Sphere::Interpolate leaves the tangent zero
+    if (!is_sphere(s, k)) {
+        const TriDev tr = load_tri(s.bvh.tris, k);
+        float du1 = 0.0f, du2 = 0.0f;
+        if (s.tri_uv != nullptr) {
+            const float* q = s.tri_uv + 6 * (size_t)k;
+            du1 = q[3] - q[1]; // diffUV1.y = uv1.v - uv0.v
+            du2 = q[5] - q[1]; // diffUV2.y = uv2.v - uv0.v
+        }
+        tg = du2 * tr.e1 - du1 * tr.e2;
+        if (s.tri_smooth != nullptr && s.tri_smooth[k] != 0) tg = tg - dot(n, tg) * n;
+        if (sq_length(tg) == 0.0f) tg = v3(1.0f, 0.0f, 0.0f);
     }
-    V3 tg = du2 * tr.e1 - du1 * tr.e2;
-    if (s.tri_smooth != nullptr && s.tri_smooth[k] != 0) tg = tg - dot(n, tg) * n;
-    if (sq_length(tg) == 0.0f) tg = v3(1.0f, 0.0f, 0.0f);
     const V3 w = normalized(n), u = normalized(tg), v = cross(w, u);
     const V3 t = tex_by_uv(s.textures[s.mat_nmaps[m]], uv.x, uv.y);
     const V3 l = normalized(2.0f * t - v3(1.0f, 1.0f, 1.0f));
@@ -147,8 +157,9 @@ __device__ __forceinline__ V3 maps_uv(const SceneView& s, int k, V3 o, V3 d)
     return s.mat_maps != nullptr ? hit_uv(s, k, o, d) : v3(0.0f, 0.0f, 0.0f);
 }
 
-// PathTracing::SampleLight (PathTracing.cpp:329-386). Returns false for the env slot.
-__device__ __forceinline__ bool sample_light(const SceneView& s, float rnd, Samplers& smp, bool want_env,
+// PathTracing::SampleLight (PathTracing.cpp:329-386) for the shadow-ray origin p (which only
+// sphere lights read). Returns false for the env slot.
+__device__ __forceinline__ bool sample_light(const SceneView& s, V3 p0, float rnd, Samplers& smp, bool want_env,
                                              float* pdf_area, V3* pl_pos)
 {
     const int L = s.num_lights;
@@ -167,7 +178,8 @@ __device__ __forceinline__ bool sample_light(const SceneView& s, float rnd, Samp
         const TriDev tr = load_tri(s.lights, i);
         V3 p, n;
         float pa;
-        tri_sample(tr, smp, &p, &n, &pa);
+        if (s.light_kinds != nullptr && s.light_kinds[i] != 0) sphere_sample(tr.p0, tr.e1.x, p0, smp, &p, &n, &pa);
+        else tri_sample(tr, smp, &p, &n, &pa);
         sum_area += 1.0f / pa;
         if (i == index) *pl_pos = p;
     }
@@ -453,7 +465,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
                         const float cos_p = fabsf_(dot(ray.dir, nn));
                         float pdf_area = 0.0f;
                         V3 dummy;
-                        sample_light(sc, next1(smp), smp, false, &pdf_area, &dummy);
+                        sample_light(sc, ray.o, next1(smp), smp, false, &pdf_area, &dummy);
                         const float pb = pdf_bsdf;
                         const float pl = l2 / cos_p * pdf_area;
                         const float mis = pb * pb / (pl * pl + pb * pb);
@@ -539,7 +551,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const JobView& jo
             const V3 pnee = si_pos + PT_EPS * copysignf_(1.0f, dd) * si_n;
             float pdf_area = 1.0f;
             V3 pl;
-            const bool is_area = sample_light(sc, next1(smp), smp, true, &pdf_area, &pl);
+            const bool is_area = sample_light(sc, pnee, next1(smp), smp, true, &pdf_area, &pl);
             if (is_area) {
                 if (sc.num_lights > 0) {
                     const V3 ldir = normalized(pl - pnee);

@@ -395,9 +395,11 @@ int pt_server_add_scene(pt_server* s, const pt_scene_desc* d, const pt_bvh_desc*
     if (!s || !d || !scene_id) return sfail(PT_ERR_ARG, "null argument");
     if (d->num_triangles < 0) return sfail(PT_ERR_ARG, "bad triangle count");
     // upper bound of the scene's and the BVH's device bytes, reserved while they are built
-    const int64_t estimate = (int64_t)d->num_triangles * 320 + (int64_t)d->num_materials * 64 + (int64_t)d->num_lights * 48 +
+    const int64_t estimate = ((int64_t)d->num_triangles + std::max(d->num_spheres, 0)) * 320 +
+                             (int64_t)d->num_materials * 64 + (int64_t)d->num_lights * 48 +
                              (d->env_rgb ? (int64_t)d->env_width * d->env_height * 12 + 8 * (int64_t)d->env_height : 0) +
-                             (d->uvs ? (int64_t)d->num_triangles * 48 : 0) + pt_scene_map_bytes(d) + (1 << 20);
+                             (d->uvs ? ((int64_t)d->num_triangles + std::max(d->num_spheres, 0)) * 48 : 0) +
+                             pt_scene_map_bytes(d) + (1 << 20);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stop) return sfail(PT_ERR_STATE, "server stopped");

@@ -169,23 +169,29 @@ __device__ __noinline__ bool reference_first(const BvhView& b, const TraceRay& r
     return a < c;
 }
 
-// Tests triangle k (the reference's hit rule); true when an any-hit ray has found its
-// hit. The 8-wide traversal meets leaves out of the reference's order, so an exactly
+// Tests the primitive of leaf slot k, a triangle or a sphere (the reference's hit rule);
+// true when an any-hit ray has found its hit. The 8-wide traversal meets leaves out of the reference's order, so an exactly
 // equal distance goes to the triangle the reference would have reached first.
 template <bool STATS, bool WIDE>
 __device__ __forceinline__ bool tri_test(const BvhView& b, const TraceRay& r, TraceState& s, uint32_t k)
 {
     if (STATS) ++s.nt;
     const float4 t0 = b.tris[3 * (size_t)k];
-    const float4 t1 = b.tris[3 * (size_t)k + 1];
-    const float4 t2 = b.tris[3 * (size_t)k + 2];
-    TriDev tr;
-    tr.p0 = v3(t0.x, t0.y, t0.z);
-    tr.e1 = v3(t0.w, t1.x, t1.y);
-    tr.e2 = v3(t1.z, t1.w, t2.x);
-    tr.n = v3(t2.y, t2.z, t2.w);
     float t;
-    if (tri_intersect(tr, r.o, r.d, &t) && !(t < r.dmin || t > r.dmax)) {
+    bool hit;
+    if (b.kinds != nullptr && b.kinds[k] != 0) {
+        hit = sphere_intersect(v3(t0.x, t0.y, t0.z), t0.w, r.o, r.d, &t);
+    } else {
+        const float4 t1 = b.tris[3 * (size_t)k + 1];
+        const float4 t2 = b.tris[3 * (size_t)k + 2];
+        TriDev tr;
+        tr.p0 = v3(t0.x, t0.y, t0.z);
+        tr.e1 = v3(t0.w, t1.x, t1.y);
+        tr.e2 = v3(t1.z, t1.w, t2.x);
+        tr.n = v3(t2.y, t2.z, t2.w);
+        hit = tri_intersect(tr, r.o, r.d, &t);
+    }
+    if (hit && !(t < r.dmin || t > r.dmax)) {
         if (s.best < 0 || t < s.best_t) {
             s.best = (int)k;
             s.best_t = t;

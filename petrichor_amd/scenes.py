@@ -259,6 +259,44 @@ def scaled_scene(n_tris, seed, env=None):
     return spheres_scene(seed=seed, n_spheres=n_spheres, seg=seg, bands=bands, terrain_n=terrain_n, env=env)
 
 
+def sphere_primitives(seed=5, n_spheres=24, env_w=64, env_h=32):
+    """Sphere primitives (Core/Geometry/Sphere.cpp) over a triangle floor under a sky: a
+    4 x 4 Lambert floor (2 triangles), n_spheres spheres with centres in [-1.5, 1.5]^2 x
+    [0.2, 1.2] and radii U(0.08, 0.35) of Lambert, GGX (roughness U(0.05, 0.6)) and glass
+    materials, one emissive sphere light (radius 0.25, radiance 12) and one emissive quad
+    (radiance 8), both registered as lights (the sphere first). Spheres are the primitives
+    after the triangles: 2 floor + 2 quad triangles, then the spheres."""
+    rng = np.random.default_rng(seed)
+    P = lambda *v: np.array(v, np.float32)
+    tris = [np.asarray(q, np.float32).reshape(-1, 9) for q in (
+        _quad(P(-2, -2, 0), P(2, -2, 0), P(2, 2, 0), P(-2, 2, 0)),
+        _quad(P(-0.6, 0.9, 2.2), P(-0.1, 0.9, 2.2), P(-0.1, 1.4, 2.2), P(-0.6, 1.4, 2.2)))]
+    mats = [_material(PT_MAT_LAMBERT, (0.6, 0.55, 0.5)), _material(PT_MAT_EMISSION, (8.0, 8.0, 8.0)),
+            _material(PT_MAT_EMISSION, (12.0, 11.0, 10.0))]
+    tri_mat = np.array([0, 0, 1, 1], np.int32)
+    sph, sph_mat = [], []
+    for _ in range(n_spheres):
+        c = (rng.uniform(-1.5, 1.5), rng.uniform(-1.5, 1.5), rng.uniform(0.2, 1.2))
+        r = rng.uniform(0.08, 0.35)
+        kind = rng.uniform()
+        if kind < 0.4:
+            mats.append(_material(PT_MAT_LAMBERT, rng.uniform(0.2, 0.9, 3)))
+        elif kind < 0.8:
+            mats.append(_material(PT_MAT_GLOSSY, rng.uniform(0.3, 1.0, 3), rng.uniform(0.05, 0.6)))
+        else:
+            mats.append(_material(PT_MAT_GLASS, (1, 1, 1), 1.5))
+        sph.append((*c, r))
+        sph_mat.append(len(mats) - 1)
+    sph.append((0.9, -0.4, 1.9, 0.25))  # the sphere light
+    sph_mat.append(2)
+    cam = camera(pos=(0.2, -4.5, 2.2), look_at=(0.0, 0.0, 0.5), focal_length=30e-3, aperture=0.0)
+    sc = _finish(np.concatenate(tris), mats, tri_mat, [4 + n_spheres, 2, 3], cam, sky_env(env_w, env_h),
+                 (1, 1, 1, 0.3, 1))
+    sc["spheres"] = np.asarray(sph, np.float32)
+    sc["sphere_mat"] = np.asarray(sph_mat, np.int32)
+    return sc
+
+
 def deep_chain(n=120):
     """A scene whose binned-SAH tree with 2 bins is about n levels deep: n triangles with
     centroids x = 2^-i (i = 0..n-1), y = 1, z = 2^-i / 48 (distinct depths, so no two are

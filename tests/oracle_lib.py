@@ -42,6 +42,8 @@ def lib():
     L.oracle_scene_set_normal_maps.argtypes = [_v, _i, _f]
     L.oracle_ggx_map_normal.argtypes = [_f, _f, _f, ctypes.c_float, _f]
     L.oracle_halton.argtypes = [ctypes.c_int, ctypes.c_int, _f]
+    L.oracle_scene_set_spheres.argtypes = [_v, ctypes.c_int, _f, _i]
+    L.oracle_sphere.argtypes = [_f, _f, _f, ctypes.c_uint32, ctypes.c_uint32, _f]
     L.oracle_microjittered.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, _f]
     L.oracle_render_sampler.restype = ctypes.c_int64
     L.oracle_render_sampler.argtypes = [_v, _v, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -125,6 +127,11 @@ class OracleScene:
             len(lights), _ptr(lights, _i), _ptr(cam, _f), ew, eh, _ptr(env, _f), _ptr(env_param, _f))
         if scene.get("tri_uv") is not None:
             L.oracle_scene_set_uv(self.ptr, _ptr(k(f32(scene["tri_uv"])), _f))
+        ns = 0
+        if scene.get("spheres") is not None and len(scene["spheres"]) > 0:
+            sp = k(f32(scene["spheres"]).reshape(-1, 4))
+            ns = sp.shape[0]
+            L.oracle_scene_set_spheres(self.ptr, ns, _ptr(sp, _f), _ptr(k(i32(scene["sphere_mat"])), _i))
         if scene.get("mat_maps") is not None:
             texs = [k(f32(t)) for t in scene.get("textures", [])]
             sizes = k(np.array([[t.shape[1], t.shape[0]] for t in texs], np.int32).reshape(-1))
@@ -137,7 +144,7 @@ class OracleScene:
             L.oracle_scene_set_normal_maps(self.ptr, _ptr(k(i32(nm)), _i) if nm is not None else None,
                                            _ptr(k(f32(ms)), _f) if ms is not None else None)
         self.bvh = None
-        self.num_tris = len(tri_mat)
+        self.num_tris = len(tri_mat) + ns  # primitives: triangles, then spheres
 
     def build_bvh(self, bins=16):
         if self.bvh is not None:
@@ -343,3 +350,11 @@ def microjittered(seed, length, n):
     out = np.zeros((n, 2), np.float32)
     lib().oracle_microjittered(seed, length, n, _ptr(out, _f))
     return out
+
+
+def sphere(sph4, ray6, p, seed0=1, seed1=2):
+    """Sphere::Intersect / Interpolate / SampleSurface of one sphere: dict of hit, t,
+    normal, sampled pos, pdf, sampled normal."""
+    out = np.zeros(12, np.float32)
+    lib().oracle_sphere(_ptr(f32(sph4), _f), _ptr(f32(ray6), _f), _ptr(f32(p), _f), seed0, seed1, _ptr(out, _f))
+    return {"hit": bool(out[0]), "t": out[1], "normal": out[2:5], "pos": out[5:8], "pdf": out[8], "sn": out[9:12]}

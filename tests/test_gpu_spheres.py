@@ -1,0 +1,138 @@
+"""Sphere primitives (Core/Geometry/Sphere.cpp:17-118) on the GPU against the oracle: the
+tree over triangles and spheres equals the oracle's on both builders; closest hit, any
+hit and the work counters bit-identical; renders on both integrators with an emissive
+sphere light beside a triangle light, the denoising AOVs, the camera-inside-a-sphere
+quirk, a batch beside a triangle scene, the 8-wide walk, and the C++ API's Sphere."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import petrichor_amd as P
+from gpu_util import bits_equal, ctx, log, rel_err, to_dev
+from petrichor_amd import scenes
+from test_cpu_spheres import emissive_sphere
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-4
+
+
+def rays(n, seed):
+    rng = np.random.default_rng(seed)
+    org = rng.uniform(-3, 3, (n, 3)) + np.array([0, 0, 1.5])
+    tgt = rng.uniform(-1.5, 1.5, (n, 3)) * np.array([1, 1, 0.6]) + np.array([0, 0, 0.7])
+    d = (tgt - org) / np.linalg.norm(tgt - org, axis=1, keepdims=True)
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3], r[:, 4:7], r[:, 7] = org, d, np.finfo(np.float32).max
+    return r
+
+
+@pytest.mark.parametrize("builder", [P.BUILD_DEVICE, P.BUILD_HOST])
+def test_sphere_tree_and_trace_match_oracle(gpu, builder):
+    sc = scenes.sphere_primitives()
+    b = ctx().scene(sc).build_bvh(16, builder=builder)
+    own = O.OracleScene(sc).build_bvh(16).bvh_export()
+    gpu_tree = b.export_nodes()
+    for k in ("boxes", "child", "range", "leaf"):
+        assert np.array_equal(own[k], gpu_tree[k]), k
+    o = O.OracleScene(sc).use_tree(gpu_tree | {"prim_ids": b.export_order()})
+    r = rays(20000, 3)
+    tg, pg, st = b.trace_stats(to_dev(r))
+    tg, pg = tg.cpu().numpy(), pg.cpu().numpy()
+    to, po, so = o.trace_closest(r)
+    log("spheres, builder %d: %d sphere hits of %d rays, work %s / %s" % (builder, int((po >= 4).sum()), len(r), st, so))
+    assert (po >= 4).sum() > 2000
+    assert np.array_equal(pg, po) and bits_equal(tg, to) and np.array_equal(st, so)
+    hit = b.trace_any(to_dev(r)).cpu().numpy()
+    assert np.array_equal(hit, (po >= 0).astype(np.int32))
+
+
+@pytest.mark.parametrize("mode", [0, 1, "albedo", "normal"])
+def test_sphere_renders_match_oracle(gpu, mode):
+    sc = scenes.sphere_primitives()
+    b = ctx().scene(sc).build_bvh(16)
+    o = O.OracleScene(sc).use_tree(b.export_nodes() | {"prim_ids": b.export_order()})
+    if mode in ("albedo", "normal"):
+        aov = P.AOV_DENOISING_ALBEDO if mode == "albedo" else P.AOV_DENOISING_NORMAL
+        img, st = b.render_aov(64, 48, 4, 8, aov, 5)
+        ref, q = o.render(64, 48, 4, 8, 2 + aov, 5, 0)
+    else:
+        img, st = b.render(64, 48, 6, 8, mode, 5)
+        ref, q = o.render(64, 48, 6, 8, mode, 5, 0)
+    err = rel_err(img, ref).max(axis=2)
+    exact = np.all(img.view(np.uint32) == ref.view(np.uint32), axis=2)
+    log("spheres %s: queries %d / %d, bit-identical %d of %d, max rel err %g" %
+        (mode, st["queries"], q, int(exact.sum()), exact.size, float(err.max())))
+    assert np.abs(ref).mean() > 0.01
+    assert st["queries"] == q
+    assert (err <= REL_TOL).all()
+
+
+@pytest.mark.parametrize("inside", [False, True])
+def test_emissive_sphere_camera_quirk(gpu, inside):
+    sc = emissive_sphere((1.0, -2.0, 0.5) if inside else (0.0, -30.0, 0.0))
+    b = ctx().scene(sc).build_bvh(16)
+    o = O.OracleScene(sc).use_tree(b.export_nodes() | {"prim_ids": b.export_order()})
+    for integ in (0, 1):
+        img, _ = b.render(16, 12, 2, 4, integ, 7)
+        ref, _ = o.render(16, 12, 2, 4, integ, 7, 0)
+        assert bits_equal(img, ref), (inside, integ)
+
+
+def test_sphere_scene_in_a_batch_and_8_wide(gpu):
+    c = ctx()
+    sp = c.scene(scenes.sphere_primitives()).build_bvh(16)
+    cb = c.scene(scenes.cornell_box()).build_bvh(16)
+    specs = [(sp, 48, 32, 4, 8, 0, 3), (cb, 32, 32, 4, 8, 0, 4), (sp, 32, 24, 3, 8, 1, 5)]
+    jobs = [P.Job(b, w, h, spp, d, i, sd) for (b, w, h, spp, d, i, sd) in specs]
+    eng = P.Engine(c, max_paths=1 << 14, max_jobs=4)
+    eng.render_batch(jobs, iterations_per_sync=4)
+    for j, (b, w, h, spp, d, i, sd) in zip(jobs, specs):
+        plain, _ = b.render(w, h, spp, d, i, sd)
+        assert bits_equal(j.result, plain)
+    eng.close()
+    sc = scenes.sphere_primitives()
+    b8 = c.scene(sc).build_bvh(16, width=8)
+    o = O.OracleScene(sc).use_tree(b8.export_nodes() | {"prim_ids": b8.export_order()})
+    r = rays(20000, 8)
+    t8, p8 = b8.trace_closest(to_dev(r))
+    to, po, _ = o.trace_closest(r)
+    same = (p8.cpu().numpy() == po).mean()
+    log("8-wide on spheres: %.6f of rays equal the oracle" % same)
+    assert same > 0.999
+
+
+def test_cpp_api_spheres(gpu):
+    sc = dict(scenes.cornell_box())
+    mt = np.append(np.asarray(sc["mat_type"], np.int32), [P.PT_MAT_GLOSSY, P.PT_MAT_GLASS]).astype(np.int32)
+    mp = np.vstack([sc["mat_param"], np.array([[0.9, 0.8, 0.7, 0.3, 0, 0, 1, 0], [1, 1, 1, 1.5, 0, 0, 1, 0]],
+                                              np.float32)])
+    spheres = np.array([[0.3, 0.35, 0.15, 0.12], [0.7, 0.6, 0.2, 0.15], [0.5, 0.3, 0.55, 0.1],
+                        [0.75, 0.25, 0.75, 0.06]], np.float32)
+    sph_mat = np.array([4, 5, 0, 3], np.int32)
+    nt = len(sc["tri_mat"])
+    lights = np.append(np.asarray(sc["light_tris"], np.int32), nt + 3).astype(np.int32)
+    sc.update(mat_type=mt, mat_param=mp, spheres=spheres, sphere_mat=sph_mat, light_tris=lights)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "petrichor_amd", "_build",
+                       "sphere_api")
+    for integrator in (0, 1):
+        with tempfile.TemporaryDirectory() as d:
+            fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+            with open(fin, "wb") as f:
+                np.array([nt, len(spheres), len(lights)], np.int32).tofile(f)
+                np.ascontiguousarray(sc["tri_pos"], np.float32).tofile(f)
+                np.ascontiguousarray(sc["tri_mat"], np.int32).tofile(f)
+                spheres.tofile(f)
+                sph_mat.tofile(f)
+                lights.tofile(f)
+            rr = subprocess.run([exe, fin, fout, "48", "36", "4", "8", str(integrator), "5"], stdout=subprocess.PIPE,
+                                stderr=subprocess.STDOUT, text=True, timeout=300)
+            assert rr.returncode == 0, rr.stdout
+            img = np.fromfile(fout, np.float32).reshape(36, 48, 3)
+        ref, _ = ctx().scene(sc).build_bvh(16).render(48, 36, 4, 8, integrator, 5)
+        assert bits_equal(img, ref)
+        assert np.abs(img).mean() > 0.01
