@@ -26,8 +26,10 @@
 #include <cstring>
 #include <filesystem>
 #include <functional>
+#include <limits>
 #include <map>
 #include <memory>
+#include <optional>
 #include <stdexcept>
 #include <string>
 #include <string_view>
@@ -1147,6 +1149,184 @@ private:
     std::unique_ptr<Camera> m_mainCamera;
     std::array<Texture2D*, AOVType::NumAOVTypes> m_targetTextures = {};
     RenderSetting m_renderSetting;
+};
+
+// Core/Ray.h:9-49: the ray of the integrators (origin, unit direction and the path's
+// throughput, bounce count, survival probability and index of refraction).
+enum class RayTypes : uint8_t
+{
+    Camera,
+    Shadow,
+    Diffuse,
+    Translucent,
+    Glossy,
+    Refract,
+};
+struct Ray
+{
+    Ray() = default;
+    Ray(const Math::Vector3f& o, const Math::Vector3f& dir, RayTypes rayType = RayTypes::Camera,
+        Color3f throughput = Color3f::One(), int bounce = 0, float prob = 1.0f, float ior = 1.0f)
+      : o(o), dir(dir), throughput(throughput), bounce(bounce), prob(prob), ior(ior), rayType(rayType)
+    {
+    }
+    Math::Vector3f o;
+    Math::Vector3f dir;
+    Color3f throughput = Color3f::One();
+    int bounce = 0;
+    float prob = 1.0f;
+    float ior = 1.0f;
+    RayTypes rayType = RayTypes::Camera;
+};
+
+// Core/HitInfo.h:15-19: the nearest hit's distance and geometry.
+struct HitInfo
+{
+    float distance = std::numeric_limits<float>::infinity();
+    const GeometryBase* hitObj = nullptr;
+};
+
+// Core/Accel/AccelBase.h:17-56: Build over a scene's geometries, then closest-hit queries
+// within [distMin, distMax] (defaults [0, FLT_MAX], or [distMin, inf]).
+enum class AccelType
+{
+    BruteForce,
+    BVH
+};
+class AccelBase
+{
+public:
+    virtual ~AccelBase() = default;
+    virtual void Build(const Scene& scene) = 0;
+    virtual std::optional<HitInfo> Intersect(const Ray& ray, const Scene& scene, float distMin,
+                                             float distMax) const = 0;
+    std::optional<HitInfo> Intersect(const Ray& ray, const Scene& scene) const
+    {
+        return Intersect(ray, scene, 0.0f, std::numeric_limits<float>::max());
+    }
+    std::optional<HitInfo> Intersect(const Ray& ray, const Scene& scene, float distMin) const
+    {
+        return Intersect(ray, scene, distMin, std::numeric_limits<float>::infinity());
+    }
+};
+
+// Core/Accel/BinnedSAHBVH.h: the binned-SAH tree (16 bins, leaves of up to 4) over the
+// scene's triangles and spheres, built and traversed on the GPU (pt_bvh_build,
+// pt_trace_closest_host). Build keeps the primitives in the geometries' order (triangles
+// first, then spheres, as Render numbers them); a hit reports its geometry. Intersect
+// traces one ray per call; IntersectBatch traces many in one launch. Build returns with
+// an empty tree (every query a miss) and GetLastError set when the device fails.
+class BinnedSAHBVH : public AccelBase
+{
+public:
+    explicit BinnedSAHBVH(int device = 0) : m_device(device) {}
+    ~BinnedSAHBVH() override
+    {
+        release();
+        if (m_ctx) pt_ctx_destroy(m_ctx);
+    }
+    BinnedSAHBVH(const BinnedSAHBVH&) = delete;
+    BinnedSAHBVH& operator=(const BinnedSAHBVH&) = delete;
+
+    void Build(const Scene& scene) override
+    {
+        release();
+        m_error.clear();
+        std::vector<float> pos, sph;
+        std::vector<int32_t> tmat, smat;
+        std::vector<const GeometryBase*> spheres;
+        for (const GeometryBase* g : scene.GetGeometries()) {
+            if (dynamic_cast<const Sphere*>(g)) {
+                spheres.push_back(g);
+            } else if (const auto* t = dynamic_cast<const Triangle*>(g)) {
+                for (int k = 0; k < 3; ++k) {
+                    const Math::Vector3f& p = t->GetVertex(k)->pos;
+                    pos.insert(pos.end(), {p.x, p.y, p.z});
+                }
+                m_prims.push_back(g);
+                tmat.push_back(0);
+            }
+        }
+        for (const GeometryBase* g : spheres) {
+            const auto* sp = static_cast<const Sphere*>(g);
+            sph.insert(sph.end(), {sp->GetOrigin().x, sp->GetOrigin().y, sp->GetOrigin().z, sp->GetRadius()});
+            m_prims.push_back(g);
+            smat.push_back(0);
+        }
+        // geometry only: one placeholder material, no lights, no camera
+        const int32_t matType = PT_MAT_LAMBERT;
+        const float matParam[8] = {1.0f, 1.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f};
+        pt_scene_desc d{};
+        d.num_triangles = (int32_t)tmat.size();
+        d.positions = pos.empty() ? nullptr : pos.data();
+        d.material_ids = tmat.empty() ? nullptr : tmat.data();
+        d.num_materials = 1;
+        d.material_types = &matType;
+        d.material_params = matParam;
+        d.num_spheres = (int32_t)smat.size();
+        d.spheres = sph.empty() ? nullptr : sph.data();
+        d.sphere_material_ids = smat.empty() ? nullptr : smat.data();
+        if (!m_ctx && pt_ctx_create(m_device, &m_ctx) != PT_OK) {
+            m_error = "device context creation failed";
+            m_prims.clear();
+            return;
+        }
+        pt_bvh_desc bd{16, 4, 2, PT_BUILD_DEVICE}; // BinnedSAHBVH::kNumBins, leaves of up to 4
+        if (pt_scene_create(m_ctx, &d, &m_scene) != PT_OK || pt_bvh_build(m_scene, &bd, &m_bvh) != PT_OK) {
+            m_error = pt_last_error(m_ctx);
+            release();
+        }
+    }
+
+    std::optional<HitInfo> Intersect(const Ray& ray, const Scene& scene, float distMin,
+                                     float distMax) const override
+    {
+        std::vector<std::optional<HitInfo>> h = IntersectBatch(std::vector<Ray>{ray}, distMin, distMax);
+        (void)scene;
+        return h[0];
+    }
+    using AccelBase::Intersect;
+
+    // Closest hits of many rays in one launch (the same query per ray).
+    std::vector<std::optional<HitInfo>> IntersectBatch(const std::vector<Ray>& rays, float distMin,
+                                                       float distMax) const
+    {
+        std::vector<std::optional<HitInfo>> out(rays.size());
+        if (!m_bvh || rays.empty()) return out;
+        std::vector<float> r(8 * rays.size()), t(rays.size());
+        std::vector<int32_t> prim(rays.size());
+        for (size_t i = 0; i < rays.size(); ++i) {
+            const Ray& q = rays[i];
+            const float v[8] = {q.o.x, q.o.y, q.o.z, distMin, q.dir.x, q.dir.y, q.dir.z, distMax};
+            std::memcpy(&r[8 * i], v, sizeof v);
+        }
+        if (pt_trace_closest_host(m_bvh, (int64_t)rays.size(), r.data(), t.data(), prim.data()) != PT_OK) {
+            m_error = pt_last_error(m_ctx);
+            return out;
+        }
+        for (size_t i = 0; i < rays.size(); ++i)
+            if (prim[i] >= 0) out[i] = HitInfo{t[i], m_prims[(size_t)prim[i]]};
+        return out;
+    }
+
+    const std::string& GetLastError() const { return m_error; }
+
+private:
+    void release()
+    {
+        if (m_bvh) pt_bvh_destroy(m_bvh);
+        if (m_scene) pt_scene_destroy(m_scene);
+        m_bvh = nullptr;
+        m_scene = nullptr;
+        m_prims.clear();
+    }
+
+    int m_device;
+    pt_context* m_ctx = nullptr;
+    pt_scene* m_scene = nullptr;
+    pt_bvh* m_bvh = nullptr;
+    std::vector<const GeometryBase*> m_prims; // primitive index -> geometry
+    mutable std::string m_error;
 };
 
 enum class Integrator

@@ -207,6 +207,12 @@ int pt_bvh_build_host(const float* tri_pos, int32_t num_triangles, const pt_bvh_
  * until pt_bvh_destroy; calls on one BVH must not overlap. Same for pt_trace_any. */
 int pt_trace_closest(pt_bvh* bvh, int64_t n, const float* rays, float* t_out, int32_t* prim_out,
                      void* stream);
+/* pt_trace_closest over host memory: rays, t_out and prim_out are host pointers; the rays
+ * go through device buffers the BVH keeps (40 bytes per ray of its largest such call) and
+ * the call returns when the hits are in t_out / prim_out. The BVH query of
+ * AccelBase::Intersect (Core/Accel/AccelBase.h:37-56) for host code that holds its rays on
+ * the host (the C++ BinnedSAHBVH class of petrichor/Petrichor.h). */
+int pt_trace_closest_host(pt_bvh* bvh, int64_t n, const float* rays, float* t_out, int32_t* prim_out);
 /* Any hit (occlusion) for n rays: hit_out[i] = 1 if some triangle is hit within
  * [dist_min, dist_max], else 0. No reference counterpart (the reference's shadow
  * rays call the closest-hit query, Core/Integrator/PathTracing.cpp:293). */

@@ -174,7 +174,7 @@ def lib():
 
 # every symbol include/petrichor_hip.h declares
 EXPORTED = ["pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_scene_create", "pt_scene_destroy",
-            "pt_bvh_build", "pt_bvh_destroy", "pt_bvh_export_order", "pt_bvh_export_nodes", "pt_bvh_build_host", "pt_bvh_info", "pt_trace_closest",
+            "pt_bvh_build", "pt_bvh_destroy", "pt_bvh_export_order", "pt_bvh_export_nodes", "pt_bvh_build_host", "pt_bvh_info", "pt_trace_closest", "pt_trace_closest_host",
             "pt_trace_any", "pt_trace_closest_noalloc", "pt_trace_stats", "pt_render", "pt_engine_create", "pt_engine_destroy",
             "pt_render_batch", "pt_engine_set_tuning", "pt_engine_get_tuning", "pt_server_create", "pt_server_destroy", "pt_server_last_error",
             "pt_server_add_scene", "pt_server_remove_scene", "pt_server_scene_resident", "pt_server_submit",

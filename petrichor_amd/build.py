@@ -57,7 +57,7 @@ def build(verbose=False):
     _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-I", os.path.join(ROOT, "include"),
           os.path.join(ROOT, "tests", "cpp", "load_api.cpp"), "-L", HERE, "-lpetrichor_hip",
           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib", "-o", os.path.join(bdir, "load_api")])
-    for drv in ("sampler_api", "sphere_api"):
+    for drv in ("sampler_api", "sphere_api", "accel_api"):
         _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-I", os.path.join(ROOT, "include"),
               os.path.join(ROOT, "tests", "cpp", drv + ".cpp"), "-L", HERE, "-lpetrichor_hip",
               "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib", "-o", os.path.join(bdir, drv)])

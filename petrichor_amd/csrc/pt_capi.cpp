@@ -651,7 +651,7 @@ int pt_bvh_destroy(pt_bvh* b)
     if (!b) return PT_ERR_ARG;
     free_all({b->nodes8_d, b->nodes_d, b->big_leaves_d, b->inner_range_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
               b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d, b->order_keys_d, b->order_idx_d, b->order_temp_d,
-              b->tri_uv_d, b->parents_d, b->kinds_d});
+              b->tri_uv_d, b->parents_d, b->kinds_d, b->host_io_d});
     delete b;
     return PT_OK;
 }
@@ -835,6 +835,33 @@ int pt_trace_closest(pt_bvh* b, int64_t n, const float* rays, float* t_out, int3
     if (n == 0) return PT_OK;
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     return api_trace(b, n, rays, false, t_out, prim_out, nullptr, st);
+}
+
+int pt_trace_closest_host(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_t* prim_out)
+{
+    if (!b) return PT_ERR_ARG;
+    pt_context* ctx = b->scene->ctx;
+    if (n < 0 || n > 0x7fffff00 || (n > 0 && (!rays || !t_out || !prim_out)))
+        return fail(ctx, PT_ERR_ARG, "bad trace arguments");
+    if (n == 0) return PT_OK;
+    PT_HIP(ctx, hipSetDevice(ctx->device));
+    if (b->host_io_cap < n) {
+        if (b->host_io_d) PT_HIP(ctx, hipFree(b->host_io_d));
+        b->host_io_d = nullptr;
+        b->host_io_cap = 0;
+        PT_HIP(ctx, hipMalloc(&b->host_io_d, 40 * (size_t)n));
+        b->host_io_cap = n;
+    }
+    float* rays_d = (float*)b->host_io_d;
+    float* t_d = rays_d + 8 * (size_t)n;
+    int32_t* prim_d = (int32_t*)(t_d + n);
+    PT_HIP(ctx, hipMemcpyAsync(rays_d, rays, 32 * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    const int rc = api_trace(b, n, rays_d, false, t_d, prim_d, nullptr, ctx->stream);
+    if (rc != PT_OK) return rc;
+    PT_HIP(ctx, hipMemcpyAsync(t_out, t_d, 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+    PT_HIP(ctx, hipMemcpyAsync(prim_out, prim_d, 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+    PT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PT_OK;
 }
 
 int pt_trace_any(pt_bvh* b, int64_t n, const float* rays, int32_t* hit_out, void* stream)

@@ -103,6 +103,7 @@ struct pt_engine {
     hipGraphExec_t graphs[2] = {nullptr, nullptr};
     int64_t iterations = 0;
     int width = 0;             // BVH width of the jobs a multi-job engine runs (0: none yet)
+    bool ext = false;          // a multi-job engine's launches take the extended walk (a job's BVH is deep or has spheres)
     pt::LaunchTuning tune;     // launch shapes (captured in the graphs)
     int iters_per_sync = 8;    // default graph replays between host synchronisations
     // Once every job is admitted and at most this many paths are live at a
@@ -430,6 +431,17 @@ struct pt_engine {
             width = jw;
             drop_graphs(); // the trace kernel is captured in the graphs
         }
+        if (multi) {
+            // the extended walk from the first job that needs it until the engine runs empty
+            const bool je = j.bvh->view.parents != nullptr || j.bvh->view.kinds != nullptr;
+            bool running = false;
+            for (int i = 0; i < max_jobs; ++i) running |= slot_used[i] != 0;
+            const bool want = running ? (ext || je) : je;
+            if (want != ext) {
+                ext = want;
+                drop_graphs();
+            }
+        }
         int slot = -1;
         for (int i = 0; i < max_jobs; ++i)
             if (!slot_used[i]) {
@@ -535,6 +547,7 @@ struct pt_engine {
         }
         tj.views = multi ? views_d : nullptr;
         tj.wide = multi && width == 8;
+        tj.ext = multi && ext;
         tj.job_of_path = multi ? ps.job : nullptr;
         tj.queries = queries_d;
         log_job(tj);
@@ -658,6 +671,7 @@ struct pt_engine {
         }
         tj.views = multi ? views_d : nullptr;
         tj.wide = multi && width == 8;
+        tj.ext = multi && ext;
         tj.job_of_path = multi ? ps.job : nullptr;
         tj.queries = queries_d;
         log_job(tj);
@@ -723,6 +737,7 @@ struct pt_engine {
             g.out_k = outs_k[k];
             tj.views = multi ? views_d : nullptr;
             tj.wide = multi && width == 8;
+        tj.ext = multi && ext;
             tj.job_of_path = multi ? ps.job : nullptr;
             tj.queries = queries_d;
             PT_HIP(ctx, pt::launch_zero(counters + PT_CURSOR_STRIDE, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
@@ -791,6 +806,7 @@ struct pt_engine {
             g.out_k = outs_k[k];
             tj.views = multi ? views_d : nullptr;
             tj.wide = multi && width == 8;
+        tj.ext = multi && ext;
             tj.job_of_path = multi ? ps.job : nullptr;
             tj.queries = queries_d;
             PT_HIP(ctx, pt::launch_zero(counters + PT_CURSOR_STRIDE, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));

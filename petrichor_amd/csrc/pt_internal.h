@@ -77,6 +77,8 @@ struct pt_bvh {
     float* tri_uv_d = nullptr; // 6 floats per leaf slot, when the scene has texture coordinates
     int32_t* parents_d = nullptr; // inner nodes' parents, for trees deeper than the traversal stack
     int8_t* kinds_d = nullptr;    // per leaf slot: 1 for a sphere; only when the scene has spheres
+    void* host_io_d = nullptr;    // pt_trace_closest_host's device staging (40 bytes per ray)
+    int64_t host_io_cap = 0;
     int width = 2;
     int num_nodes8 = 0;
     int depth8 = 0;

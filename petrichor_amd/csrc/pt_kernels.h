@@ -134,6 +134,7 @@ struct TraceJob {
     unsigned long long* queries; // optional: += rays traced by this launch
     int chunk = 64;              // rays a wave takes from the cursor at a time (>= 64; set by the launcher)
     int wide = 0;                // 8-wide traversal (every BVH of a multi-job launch has the same width)
+    int ext = 0;                 // multi-job launches: some BVH has a deep tree or spheres (the extended walk)
     // optional launch log: the launch writes its span (and with log_counts its node visits
     // and triangle tests) into launch_log[*launch_index]; launch_log_tick then records the
     // ray counts and advances the index

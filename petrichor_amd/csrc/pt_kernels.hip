@@ -736,7 +736,7 @@ __device__ __forceinline__ void trace_single(const BvhView& b, V3 o, float dmin,
     TraceRay r;
     TraceState s;
     if (trace_begin(b, r0, r1, any, r, s)) {
-        while (!(WIDE ? trace8_step<false>(b, r, s, lds, spill) : trace_step<false>(b, r, s, lds, spill))) {
+        while (!(WIDE ? trace8_step<false, true>(b, r, s, lds, spill) : trace_step<false, true>(b, r, s, lds, spill))) {
         }
     }
     *t = s.best_t;

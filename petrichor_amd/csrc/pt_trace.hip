@@ -48,7 +48,7 @@ __device__ __forceinline__ void part_item(const int* seg_n, int num_seg, int x, 
     }
 }
 
-template <bool STATS, bool MULTI, bool WIDE>
+template <bool STATS, bool MULTI, bool WIDE, bool EXT>
 __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int* cursor, unsigned long long* stats)
 {
     // segment sizes come from device counters (written by the previous stage)
@@ -150,7 +150,7 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
             continue;
         }
         if (item >= 0) {
-            if (WIDE ? trace8_step<STATS>(b, r, s, lds, spill) : trace_step<STATS>(b, r, s, lds, spill)) {
+            if (WIDE ? trace8_step<STATS, EXT>(b, r, s, lds, spill) : trace_step<STATS, EXT>(b, r, s, lds, spill)) {
                 if (STATS) {
                     nv += s.nv;
                     nt += s.nt;
@@ -174,12 +174,12 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
 // are all resident (8 waves per SIMD; unbounded it takes 67 and fits 7, 2.8 % slower on
 // the 64-job batch). The single-job binary walk needs 61 by itself; the 8-wide walk is
 // LDS-bound at 5 blocks per CU.
-template <bool STATS, bool MULTI, bool WIDE>
+template <bool STATS, bool MULTI, bool WIDE, bool EXT>
 __global__ void __launch_bounds__(PT_TRACE_BLOCK, !WIDE ? 8 : 1) k_trace_persistent(BvhView b1, TraceJob job,
                                                                                   int* cursor,
                                                                                   unsigned long long* stats)
 {
-    trace_body<STATS, MULTI, WIDE>(b1, job, cursor, stats);
+    trace_body<STATS, MULTI, WIDE, EXT>(b1, job, cursor, stats);
 }
 
 hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* cursor, unsigned long long* stats,
@@ -199,25 +199,34 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
     const int bpc = tune.trace_blocks_per_cu < 1 || tune.trace_blocks_per_cu > max_bpc ? max_bpc
                                                                                      : tune.trace_blocks_per_cu;
     const dim3 grid((unsigned)(cu_count() * bpc)), block(PT_TRACE_BLOCK);
-#define PT_LAUNCH_TRACE(S, M, W) \
-    hipLaunchKernelGGL((k_trace_persistent<S, M, W>), grid, block, 0, st, b, job, cursor, stats)
-    if (stats || (job.launch_log && job.log_counts)) {
+    // the extended walk (deep trees, sphere leaves) only where a BVH needs it
+    const bool ext = multi ? job.ext != 0 : (b.parents != nullptr || b.kinds != nullptr);
+    const bool counted = stats || (job.launch_log && job.log_counts);
+#define PT_LAUNCH_TRACE(S, M, W, E) \
+    hipLaunchKernelGGL((k_trace_persistent<S, M, W, E>), grid, block, 0, st, b, job, cursor, stats)
+#define PT_LAUNCH_TRACE_E(S, M, W) \
+    do {                          \
+        if (ext) PT_LAUNCH_TRACE(S, M, W, true); \
+        else PT_LAUNCH_TRACE(S, M, W, false);    \
+    } while (0)
+    if (counted) {
         if (multi) {
-            if (wide) PT_LAUNCH_TRACE(true, true, true);
-            else PT_LAUNCH_TRACE(true, true, false);
+            if (wide) PT_LAUNCH_TRACE_E(true, true, true);
+            else PT_LAUNCH_TRACE_E(true, true, false);
         } else {
-            if (wide) PT_LAUNCH_TRACE(true, false, true);
-            else PT_LAUNCH_TRACE(true, false, false);
+            if (wide) PT_LAUNCH_TRACE_E(true, false, true);
+            else PT_LAUNCH_TRACE_E(true, false, false);
         }
     } else {
         if (multi) {
-            if (wide) PT_LAUNCH_TRACE(false, true, true);
-            else PT_LAUNCH_TRACE(false, true, false);
+            if (wide) PT_LAUNCH_TRACE_E(false, true, true);
+            else PT_LAUNCH_TRACE_E(false, true, false);
         } else {
-            if (wide) PT_LAUNCH_TRACE(false, false, true);
-            else PT_LAUNCH_TRACE(false, false, false);
+            if (wide) PT_LAUNCH_TRACE_E(false, false, true);
+            else PT_LAUNCH_TRACE_E(false, false, false);
         }
     }
+#undef PT_LAUNCH_TRACE_E
 #undef PT_LAUNCH_TRACE
     return hipGetLastError();
 }

@@ -172,24 +172,25 @@ __device__ __noinline__ bool reference_first(const BvhView& b, const TraceRay& r
 // Tests the primitive of leaf slot k, a triangle or a sphere (the reference's hit rule);
 // true when an any-hit ray has found its hit. The 8-wide traversal meets leaves out of the reference's order, so an exactly
 // equal distance goes to the triangle the reference would have reached first.
-template <bool STATS, bool WIDE>
+template <bool STATS, bool WIDE, bool EXT>
 __device__ __forceinline__ bool tri_test(const BvhView& b, const TraceRay& r, TraceState& s, uint32_t k)
 {
     if (STATS) ++s.nt;
     const float4 t0 = b.tris[3 * (size_t)k];
+    const float4 t1 = b.tris[3 * (size_t)k + 1];
+    const float4 t2 = b.tris[3 * (size_t)k + 2];
+    TriDev tr;
+    tr.p0 = v3(t0.x, t0.y, t0.z);
+    tr.e1 = v3(t0.w, t1.x, t1.y);
+    tr.e2 = v3(t1.z, t1.w, t2.x);
+    tr.n = v3(t2.y, t2.z, t2.w);
     float t;
-    bool hit;
-    if (b.kinds != nullptr && b.kinds[k] != 0) {
-        hit = sphere_intersect(v3(t0.x, t0.y, t0.z), t0.w, r.o, r.d, &t);
-    } else {
-        const float4 t1 = b.tris[3 * (size_t)k + 1];
-        const float4 t2 = b.tris[3 * (size_t)k + 2];
-        TriDev tr;
-        tr.p0 = v3(t0.x, t0.y, t0.z);
-        tr.e1 = v3(t0.w, t1.x, t1.y);
-        tr.e2 = v3(t1.z, t1.w, t2.x);
-        tr.n = v3(t2.y, t2.z, t2.w);
-        hit = tri_intersect(tr, r.o, r.d, &t);
+    bool hit = tri_intersect(tr, r.o, r.d, &t);
+    // a sphere's slot (only in scenes with spheres): its record read again, so that the
+    // triangle path keeps its registers
+    if (EXT && b.kinds != nullptr && b.kinds[k] != 0) {
+        const float4 q = b.tris[3 * (size_t)k];
+        hit = sphere_intersect(v3(q.x, q.y, q.z), q.w, r.o, r.d, &t);
     }
     if (hit && !(t < r.dmin || t > r.dmax)) {
         if (s.best < 0 || t < s.best_t) {
@@ -257,22 +258,22 @@ __device__ __forceinline__ bool deep_next(const BvhView& b, const TraceRay& r, T
     return false;
 }
 
-template <bool STATS, bool WIDE>
+template <bool STATS, bool WIDE, bool EXT>
 __device__ __forceinline__ bool leaf_or_pop(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
                                             StackEntry* spill, bool next)
 {
     if (s.leaf_k < s.leaf_end) {
         if (WIDE) { // a wide node's work per step is larger: whole leaves keep steps alike
             for (; s.leaf_k < s.leaf_end; ++s.leaf_k)
-                if (tri_test<STATS, WIDE>(b, r, s, s.leaf_k)) return true;
+                if (tri_test<STATS, WIDE, EXT>(b, r, s, s.leaf_k)) return true;
         } else {
-            if (tri_test<STATS, WIDE>(b, r, s, s.leaf_k++)) return true;
+            if (tri_test<STATS, WIDE, EXT>(b, r, s, s.leaf_k++)) return true;
             if (s.leaf_k < s.leaf_end) return false;
         }
         next = false;
     }
     if (!next) {
-        if (!WIDE && b.parents) return !deep_next(b, r, s, lds);
+        if (!WIDE && EXT && b.parents) return !deep_next(b, r, s, lds);
         if (s.sp == 0) return true;
         --s.sp;
         stack_pop<WIDE>(lds, spill, s.sp, &s.cur, &s.cur_d);
@@ -281,8 +282,10 @@ __device__ __forceinline__ bool leaf_or_pop(const BvhView& b, const TraceRay& r,
 }
 
 // One traversal step: an inner node or triangles of a leaf, then the next entry from the
-// stack when the current one is done. Returns true when the ray is finished.
-template <bool STATS>
+// stack when the current one is done. Returns true when the ray is finished. EXT: the walk
+// handles trees deeper than the stack and sphere leaves (BvhView::parents, ::kinds); the
+// kernels take it only for BVHs that have them, so ordinary trees keep the plain walk.
+template <bool STATS, bool EXT>
 __device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
                                            StackEntry* spill)
 {
@@ -312,19 +315,28 @@ __device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, 
         const bool h1 = left_first ? hl : hr, h2 = left_first ? hr : hl;
         const uint32_t e1 = left_first ? el : er, e2 = left_first ? er : el;
         const float d1 = left_first ? tl : tr_, d2 = left_first ? tr_ : tl;
-        if (h1 || h2) {
-            if (b.parents) {
+        if (EXT && b.parents) {
+            if (h1 || h2) {
                 lds[0].x = pt_ref_index(s.cur); // deep tree: the parent of the child entered
-            } else if (h1 && h2) {
+                s.cur = h1 ? e1 : e2;
+                s.cur_d = h1 ? d1 : d2;
+                next = true;
+            }
+        } else if (h1) {
+            if (h2) {
                 stack_push<false>(lds, spill, s.sp, e2, d2);
                 ++s.sp;
             }
-            s.cur = h1 ? e1 : e2;
-            s.cur_d = h1 ? d1 : d2;
+            s.cur = e1;
+            s.cur_d = d1;
+            next = true;
+        } else if (h2) {
+            s.cur = e2;
+            s.cur_d = d2;
             next = true;
         }
     }
-    return leaf_or_pop<STATS, false>(b, r, s, lds, spill, next);
+    return leaf_or_pop<STATS, false, EXT>(b, r, s, lds, spill, next);
 }
 
 __device__ __forceinline__ void sort_pair(float& da, uint32_t& ra, float& db, uint32_t& rb)
@@ -342,7 +354,7 @@ __device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >
 
 // One step of the 8-wide traversal: the current entry (a wide node or a leaf), then the
 // next entry from the stack. Returns true when the ray is finished.
-template <bool STATS>
+template <bool STATS, bool EXT>
 __device__ __forceinline__ bool trace8_step(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
                                             StackEntry* spill)
 {
@@ -424,7 +436,7 @@ __device__ __forceinline__ bool trace8_step(const BvhView& b, const TraceRay& r,
             next = true;
         }
     }
-    return leaf_or_pop<STATS, true>(b, r, s, lds, spill, next);
+    return leaf_or_pop<STATS, true, EXT>(b, r, s, lds, spill, next);
 }
 
 __device__ __forceinline__ void trace_finish(const BvhView& b, const TraceSegment& seg, int i, const TraceState& s)

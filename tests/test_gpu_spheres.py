@@ -136,3 +136,39 @@ def test_cpp_api_spheres(gpu):
         ref, _ = ctx().scene(sc).build_bvh(16).render(48, 36, 4, 8, integrator, 5)
         assert bits_equal(img, ref)
         assert np.abs(img).mean() > 0.01
+
+
+def test_cpp_binned_sah_bvh_class(gpu):
+    """BinnedSAHBVH::Build / Intersect / IntersectBatch of the C++ API (AccelBase,
+    Core/Accel/AccelBase.h:29-56) over host rays equal pt_trace_closest on the same tree."""
+    sc = scenes.sphere_primitives()
+    r = rays(3000, 21)
+    nt, ns = len(sc["tri_mat"]), len(sc["spheres"])
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "petrichor_amd", "_build",
+                       "accel_api")
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([nt, ns, len(r)], np.int32).tofile(f)
+            np.ascontiguousarray(sc["tri_pos"], np.float32).tofile(f)
+            np.ascontiguousarray(sc["spheres"], np.float32).tofile(f)
+            r.tofile(f)
+        rr = subprocess.run([exe, fin, fout], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+        assert rr.returncode == 0, rr.stdout
+        raw = np.fromfile(fout, np.uint8)
+    m = len(r) + 64 + 2
+    prim = raw[:4 * m].view(np.int32)
+    t = raw[4 * m:8 * m].view(np.float32)
+    b = ctx().scene(sc).build_bvh(16)
+    tg, pg = b.trace_closest(to_dev(r))
+    tg, pg = tg.cpu().numpy(), pg.cpu().numpy()
+    hit = pg >= 0
+    assert hit.sum() > 1000 and (pg >= nt).sum() > 500
+    assert np.array_equal(prim[:len(r)], pg)
+    assert bits_equal(t[:len(r)][hit], tg[hit])
+    assert np.array_equal(prim[len(r):len(r) + 64], pg[:64])
+    assert np.array_equal(prim[len(r) + 64], pg[0])
+    r0 = r[:1].copy()
+    r0[0, 3], r0[0, 7] = 0.5, np.inf
+    t5, p5 = b.trace_closest(to_dev(r0))
+    assert prim[len(r) + 65] == int(p5.cpu().numpy()[0])
