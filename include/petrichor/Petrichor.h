@@ -1228,6 +1228,11 @@ public:
     BinnedSAHBVH(const BinnedSAHBVH&) = delete;
     BinnedSAHBVH& operator=(const BinnedSAHBVH&) = delete;
 
+    // Extension: true builds on the host with the reference's sort at every node, so the
+    // primitives of a leaf are also in the reference's order among equal centroid keys;
+    // false (default) builds on the device (same nodes, ties by primitive index).
+    void SetReferenceLeafOrder(bool on) { m_referenceOrder = on; }
+
     void Build(const Scene& scene) override
     {
         release();
@@ -1271,7 +1276,8 @@ public:
             m_prims.clear();
             return;
         }
-        pt_bvh_desc bd{16, 4, 2, PT_BUILD_DEVICE}; // BinnedSAHBVH::kNumBins, leaves of up to 4
+        // BinnedSAHBVH::kNumBins, leaves of up to 4
+        pt_bvh_desc bd{16, 4, 2, m_referenceOrder ? PT_BUILD_HOST_REFERENCE_ORDER : PT_BUILD_DEVICE};
         if (pt_scene_create(m_ctx, &d, &m_scene) != PT_OK || pt_bvh_build(m_scene, &bd, &m_bvh) != PT_OK) {
             m_error = pt_last_error(m_ctx);
             release();
@@ -1322,6 +1328,7 @@ private:
     }
 
     int m_device;
+    bool m_referenceOrder = false;
     pt_context* m_ctx = nullptr;
     pt_scene* m_scene = nullptr;
     pt_bvh* m_bvh = nullptr;
@@ -1362,6 +1369,9 @@ public:
     void SetIntegrator(Integrator integrator) { m_integrator = integrator; }
     void SetSeed(uint32_t seed) { m_seed = seed; }
     void SetMicroJitteredSampler(int lengthSeq) { m_samplerLength = lengthSeq; }
+    // true: the BVH is built on the host with the reference's sort at every node, so the
+    // leaf order is the reference's also among equal centroid keys (see BinnedSAHBVH).
+    void SetReferenceLeafOrder(bool on) { m_referenceOrder = on; }
     // Error text of the last failed Render (empty after a success).
     const std::string& GetLastError() const { return m_error; }
 
@@ -1385,7 +1395,8 @@ public:
         pt_scene* s = nullptr;
         pt_bvh* bvh = nullptr;
         if (pt_scene_create(m_ctx, &f.desc, &s) != PT_OK) return fail(pt_last_error(m_ctx));
-        pt_bvh_desc bd{16, 4, 2, PT_BUILD_DEVICE}; // BinnedSAHBVH: 16 bins, leaves of up to 4
+        // BinnedSAHBVH: 16 bins, leaves of up to 4
+        pt_bvh_desc bd{16, 4, 2, m_referenceOrder ? PT_BUILD_HOST_REFERENCE_ORDER : PT_BUILD_DEVICE};
         if (pt_bvh_build(s, &bd, &bvh) != PT_OK) {
             pt_scene_destroy(s);
             return fail(pt_last_error(m_ctx));
@@ -1614,6 +1625,7 @@ private:
     Integrator m_integrator = Integrator::SimplePathTracing;
     uint32_t m_seed = 1;
     int m_samplerLength = 0;
+    bool m_referenceOrder = false;
     std::string m_error;
     std::atomic<uint32_t> m_numRenderedTiles = 0;
     uint32_t m_numTiles = 0;

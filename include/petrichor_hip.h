@@ -97,12 +97,19 @@ typedef struct pt_bvh_desc {
     int32_t num_bins; /* 16 in the reference (Core/Accel/BinnedSAHBVH.h:174); 32 for the microbenchmark */
     int32_t leaf_max; /* 4 in the reference (BinnedSAHBVH.cpp:79) */
     int32_t width;    /* traversal node width: 2 (reference order) */
-    int32_t builder;  /* PT_BUILD_DEVICE (default) or PT_BUILD_HOST; both give the same tree */
+    int32_t builder;  /* PT_BUILD_* below */
 } pt_bvh_desc;
 
-/* pt_bvh_desc.builder. The device builder takes 2..64 bins, the host builder 2..256. */
+/* pt_bvh_desc.builder. The device builder takes 2..64 bins, the host builders 2..256.
+ * PT_BUILD_DEVICE and PT_BUILD_HOST give the same tree: the reference's nodes, with the
+ * primitives of a leaf ordered by the reference's sort key and, among equal keys, by
+ * primitive index. PT_BUILD_HOST_REFERENCE_ORDER sorts every split node's range as
+ * BinnedSAHBVH::Build does (std::sort by the centroid on the widest axis,
+ * Core/Accel/BinnedSAHBVH.cpp:104-107), so the leaf order among equal keys is the
+ * reference's too (one host sort per node: several seconds for millions of triangles). */
 #define PT_BUILD_DEVICE 0
 #define PT_BUILD_HOST 1
+#define PT_BUILD_HOST_REFERENCE_ORDER 2
 
 typedef struct pt_render_desc {
     int32_t width, height; /* Core/RenderSetting.h:12-13 */

@@ -46,6 +46,7 @@ class SceneDesc(ctypes.Structure):
 
 BUILD_DEVICE = 0
 BUILD_HOST = 1
+BUILD_HOST_REFERENCE_ORDER = 2  # the reference's sort at every node: its leaf order among equal keys
 
 
 class BvhDesc(ctypes.Structure):
@@ -186,9 +187,10 @@ def _ptr(a, t):
     return ctypes.cast(None, t) if a is None else a.ctypes.data_as(t)
 
 
-def build_bvh_host(tri_pos, bins=16, leaf_max=4):
+def build_bvh_host(tri_pos, bins=16, leaf_max=4, builder=BUILD_HOST):
     """Binary BVH built on the host only (pt_bvh_build_host): the reference's node table
-    and leaf order, without a device."""
+    and leaf order, without a device. builder=BUILD_HOST_REFERENCE_ORDER also keeps the
+    reference's order among equal centroid keys."""
     tri_pos = np.ascontiguousarray(tri_pos, np.float32).reshape(-1, 9)
     n = tri_pos.shape[0]
     cap = max(2 * n - 1, 1)
@@ -198,7 +200,7 @@ def build_bvh_host(tri_pos, bins=16, leaf_max=4):
     rng = np.zeros((cap, 2), np.int32)
     leaf = np.zeros(cap, np.int32)
     depth = ctypes.c_int32()
-    d = BvhDesc(bins, leaf_max, 2, BUILD_HOST)
+    d = BvhDesc(bins, leaf_max, 2, builder)
     nn = lib().pt_bvh_build_host(_ptr(tri_pos, _f), n, ctypes.byref(d), _ptr(order, _i), _ptr(boxes, _f),
                                  _ptr(child, _i), _ptr(rng, _i), _ptr(leaf, _i), ctypes.byref(depth))
     if nn < 0:

@@ -5,11 +5,15 @@
 // processed first), same bins, same SAH costs and same split. The reference sorts
 // every node's primitive range by centroid before binning; bins are monotone in the
 // centroid, so the child *sets* follow from the bins alone and only the order inside
-// a leaf depends on the sort. Here a node's range is partitioned by bin and each leaf
-// is sorted once by the axis of the last sort the reference applied to it (the
+// a leaf depends on the sort. By default a node's range is partitioned by bin and each
+// leaf is sorted once by the axis of the last sort the reference applied to it (the
 // parent's axis, or its own for a leaf chosen by the SAH), ties broken by triangle
 // index. The leaf order then equals the reference's wherever the centroid keys are
-// distinct; with equal keys libstdc++'s introsort order is unspecified.
+// distinct. With reference_order, every node of more than leaf_max primitives is
+// sorted as the reference sorts it (std::sort by the centroid on the widest axis, no
+// tie-break, BinnedSAHBVH.cpp:104-107) and split at the SAH count: the leaf order is then
+// the reference's also among equal keys, where it is whatever libstdc++'s introsort
+// leaves (the same algorithm in every libstdc++ release the reference builds against).
 #include "pt_bvh_host.h"
 
 #include <algorithm>
@@ -67,7 +71,7 @@ struct Box {
 } // namespace
 
 bool build_bvh_host(const float* tri_pos, int num_tris, int num_bins, int leaf_max, HostBvh* out,
-                    const float* spheres, int num_spheres)
+                    const float* spheres, int num_spheres, bool reference_order)
 {
     const int n = num_tris + num_spheres;
     out->nodes.clear();
@@ -130,13 +134,16 @@ bool build_bvh_host(const float* tri_pos, int num_tris, int num_bins, int leaf_m
         out->max_depth = std::max(out->max_depth, depth[ni]);
         if (cnt <= leaf_max) {
             out->nodes[ni].leaf = true;
-            sort_leaf(begin, end, parent_axis[ni]);
+            if (!reference_order) sort_leaf(begin, end, parent_axis[ni]);
             continue;
         }
         Box cb;
         for (int k = begin; k < end; ++k) cb.merge(cen[ids[k]]);
         const int axis = cb.widest_axis();
         const float len = comp(cb.hi, axis) - comp(cb.lo, axis);
+        if (reference_order) // BinnedSAHBVH.cpp:104-107: bins are then contiguous runs of the range
+            std::sort(ids.begin() + begin, ids.begin() + end,
+                      [&](int a, int b) { return comp(cen[a], axis) < comp(cen[b], axis); });
         for (int b = 0; b < num_bins; ++b) {
             bbox[b] = Box();
             bcnt[b] = 0;
@@ -187,23 +194,31 @@ bool build_bvh_host(const float* tri_pos, int num_tris, int num_bins, int leaf_m
         }
         if (best_k == 0) {
             out->nodes[ni].leaf = true;
-            sort_leaf(begin, end, axis);
+            if (!reference_order) sort_leaf(begin, end, axis);
             continue;
         }
         Box lb, rb;
-        tmp.clear();
-        int w = begin;
-        for (int k = begin; k < end; ++k) {
-            const int id = ids[k];
-            if (bin[id] < best_k) {
-                lb.merge(pbox[id]);
-                ids[w++] = id;
-            } else {
-                rb.merge(pbox[id]);
-                tmp.push_back(id);
+        if (reference_order) { // sorted: the first best_left primitives are the left bins
+            for (int k = begin; k < end; ++k) {
+                const int id = ids[k];
+                if (bin[id] < best_k) lb.merge(pbox[id]);
+                else rb.merge(pbox[id]);
             }
+        } else {
+            tmp.clear();
+            int w = begin;
+            for (int k = begin; k < end; ++k) {
+                const int id = ids[k];
+                if (bin[id] < best_k) {
+                    lb.merge(pbox[id]);
+                    ids[w++] = id;
+                } else {
+                    rb.merge(pbox[id]);
+                    tmp.push_back(id);
+                }
+            }
+            std::copy(tmp.begin(), tmp.end(), ids.begin() + w);
         }
-        std::copy(tmp.begin(), tmp.end(), ids.begin() + w);
         HostNode left, right;
         left.lo = lb.lo;
         left.hi = lb.hi;

@@ -551,7 +551,8 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     const int builder = desc ? desc->builder : PT_BUILD_DEVICE;
     if (width != 2 && width != 8) return fail(ctx, PT_ERR_UNSUPPORTED, "BVH width must be 2 or 8");
     if (bins < 2 || bins > 256 || leaf_max < 1) return fail(ctx, PT_ERR_ARG, "bad BVH parameters");
-    if (builder != PT_BUILD_DEVICE && builder != PT_BUILD_HOST) return fail(ctx, PT_ERR_ARG, "unknown builder");
+    if (builder != PT_BUILD_DEVICE && builder != PT_BUILD_HOST && builder != PT_BUILD_HOST_REFERENCE_ORDER)
+        return fail(ctx, PT_ERR_ARG, "unknown builder");
     if (builder == PT_BUILD_DEVICE && bins > 64)
         return fail(ctx, PT_ERR_UNSUPPORTED, "the device builder takes at most 64 bins");
     if (builder == PT_BUILD_DEVICE && s->num_prims() > 0) {
@@ -563,7 +564,7 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     b->scene = s;
     const auto t0 = std::chrono::steady_clock::now();
     if (!build_bvh_host(s->tri_pos.data(), s->num_tris, bins, leaf_max, &b->host, (const float*)s->spheres.data(),
-                        s->num_spheres)) {
+                        s->num_spheres, builder == PT_BUILD_HOST_REFERENCE_ORDER)) {
         delete b;
         return fail(ctx, PT_ERR_ARG, "bad BVH parameters");
     }
@@ -687,7 +688,11 @@ int pt_bvh_build_host(const float* tri_pos, int32_t num_triangles, const pt_bvh_
 {
     if (num_triangles < 0 || (num_triangles > 0 && !tri_pos)) return PT_ERR_ARG;
     HostBvh h;
-    if (!build_bvh_host(tri_pos, num_triangles, desc ? desc->num_bins : 16, desc ? desc->leaf_max : 4, &h))
+    const int builder = desc ? desc->builder : PT_BUILD_HOST;
+    if (builder != PT_BUILD_DEVICE && builder != PT_BUILD_HOST && builder != PT_BUILD_HOST_REFERENCE_ORDER)
+        return PT_ERR_ARG;
+    if (!build_bvh_host(tri_pos, num_triangles, desc ? desc->num_bins : 16, desc ? desc->leaf_max : 4, &h, nullptr, 0,
+                        builder == PT_BUILD_HOST_REFERENCE_ORDER))
         return PT_ERR_ARG;
     if (prim_order) std::memcpy(prim_order, h.order.data(), h.order.size() * sizeof(int32_t));
     for (size_t i = 0; i < h.nodes.size(); ++i) {

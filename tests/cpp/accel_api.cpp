@@ -3,7 +3,7 @@
 // ray (the three overloads) and IntersectBatch. Used by the tests to compare with the
 // Python binding's pt_trace_closest.
 //
-//   accel_api IN OUT
+//   accel_api IN OUT [reference]   (reference: BinnedSAHBVH::SetReferenceLeafOrder(true))
 // IN:  int32 n_tris, n_spheres, n_rays, float pos[9 n_tris], float sphere[4 n_spheres],
 //      float ray[8 n_rays] (origin, dist_min, direction, dist_max)
 // OUT: per ray int32 primitive (triangles first, then spheres; -1 on a miss) and float t
@@ -23,8 +23,8 @@ using Math::Vector3f;
 
 int main(int argc, char** argv)
 {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: accel_api IN OUT\n");
+    if (argc != 3 && argc != 4) {
+        std::fprintf(stderr, "usage: accel_api IN OUT [reference]\n");
         return 2;
     }
     FILE* in = std::fopen(argv[1], "rb");
@@ -58,6 +58,7 @@ int main(int argc, char** argv)
     for (int i = 0; i < ns; ++i) index[&spheres[i]] = nt + i;
 
     BinnedSAHBVH bvh;
+    bvh.SetReferenceLeafOrder(argc == 4);
     bvh.Build(scene);
     if (!bvh.GetLastError().empty()) {
         std::fprintf(stderr, "build failed: %s\n", bvh.GetLastError().c_str());

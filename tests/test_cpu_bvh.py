@@ -29,6 +29,14 @@ def rays_at(tri_pos, n, seed):
     return r
 
 
+def bare_scene(tri_pos):
+    """A scene of the given triangles with one Lambert material (geometry tests only)."""
+    sc = dict(scenes.random_triangles(1, seed=1))
+    sc["tri_pos"] = np.ascontiguousarray(tri_pos, np.float32).reshape(-1, 9)
+    sc["tri_mat"] = np.zeros(sc["tri_pos"].shape[0], np.int32)
+    return sc
+
+
 def prim_boxes(tri_pos):
     t = tri_pos.reshape(-1, 3, 3)
     return t.min(1), t.max(1)
@@ -185,3 +193,43 @@ def test_deep_tree_host_builder_and_oracle():
     assert np.array_equal(p1, p2)
     assert np.array_equal(t1.view(np.uint32), t2.view(np.uint32))
     assert np.array_equal(p1[0:240:2], np.arange(120))
+
+
+@pytest.mark.parametrize("name", ["cornell", "spheres", "grid_ties", "identical"])
+def test_reference_order_host_builder_equals_oracle_exactly(name):
+    """PT_BUILD_HOST_REFERENCE_ORDER sorts every split node's range as
+    BinnedSAHBVH::Build does (std::sort, Core/Accel/BinnedSAHBVH.cpp:104-107), so its leaf
+    order equals the oracle's (which follows the same sort sequence) also among equal
+    centroid keys, where the default builders order by triangle index. The scenes are
+    chosen so that the default order differs: the Cornell box, a terrain grid, a shuffled
+    grid of triangles whose centroids tie on every axis in rows, and 40 identical
+    triangles (one leaf of equal keys, permuted by introsort's partitions)."""
+    if name == "cornell":
+        sc = scenes.cornell_box()
+    elif name == "spheres":
+        sc = scenes.spheres_scene(n_spheres=6, seg=16, bands=9, terrain_n=48, env_w=32, env_h=16)
+    elif name == "grid_ties":
+        g = np.arange(40, dtype=np.float32)
+        x, y = np.meshgrid(g, g)
+        x, y = x.ravel(), y.ravel()
+        t = np.zeros((x.size, 3, 3), np.float32)
+        t[:, 0, :2] = np.stack([x, y], 1)
+        t[:, 1, :2] = np.stack([x + 1, y], 1)
+        t[:, 2, :2] = np.stack([x, y + 1], 1)
+        t[:, :, 2] = (x % 3)[:, None]
+        t = t[np.random.default_rng(5).permutation(len(t))]  # ties not in index order
+        sc = bare_scene(t.reshape(-1, 9))
+    else:
+        sc = bare_scene(np.repeat(scenes.random_triangles(1, seed=1)["tri_pos"], 40, axis=0))
+    tri = np.ascontiguousarray(sc["tri_pos"], np.float32).reshape(-1, 9)
+    e = O.OracleScene(sc).build_bvh(16).bvh_export()
+    r = P.build_bvh_host(tri, 16, builder=P.BUILD_HOST_REFERENCE_ORDER)
+    for k in ("boxes", "range", "leaf", "prim_ids"):
+        assert np.array_equal(e[k], r[k]), k
+    inner = e["leaf"] == 0  # a leaf's children are never read (the reference's root leaf keeps {0, 0})
+    assert np.array_equal(e["child"][inner], r["child"][inner])
+    assert e["max_depth"] == r["max_depth"]
+    d = P.build_bvh_host(tri, 16)
+    for k in ("boxes", "child", "range", "leaf"):
+        assert np.array_equal(d[k], r[k]), k
+    assert not np.array_equal(d["prim_ids"], r["prim_ids"]), "scene has no tied keys"

@@ -4,7 +4,8 @@ C1  Cornell box, 32 triangles, 512 x 512, 16 spp, depth 8, seed 1: every pixel, 
     integrators, on the GPU's tree and on the oracle's own tree.
 C2  the single large scene (3,145,728 triangles, 2048 x 1024 sky), 1920 x 1080, 64 spp,
     depth 12: the full frame on the GPU, scattered crops (the sun's among them) against the
-    oracle on the GPU's tree and on the oracle's own tree (std::sort leaf order).
+    oracle on the GPU's tree and on the oracle's own tree (std::sort leaf order), which the
+    product's reference-order build reproduces exactly.
 C3  10M random triangles, 32 bins, binary and 8-wide trees, 2^26 rays (half pinhole, half
     surface rays): closest and any hit on every ray, a 1M-ray sample against the oracle.
 C4  the 64-job batch: every batched image equals its per-job render, and the oracle on a
@@ -49,6 +50,11 @@ def test_c1_cornell_every_pixel(gpu, integrator):
     over = int((rel_err(img, ref2).max(axis=2) > 1e-4).sum())
     log("C1 pixels over 1e-4 against the oracle's own tree:", over)
     assert over <= W * H // 1000
+    # the product's reference-order build is that tree, and renders the oracle's image of it
+    br = ctx().scene(sc).build_bvh(16, builder=P.BUILD_HOST_REFERENCE_ORDER)
+    assert np.array_equal(br.export_order(), own.bvh_export()["prim_ids"])
+    img3, _ = br.render(W, H, 16, 8, integrator, 1)
+    assert np.array_equal(img3.view(np.uint32), ref2.view(np.uint32))
 
 
 def crops(img, size=48):
@@ -92,6 +98,18 @@ def test_c2_single_large_scene_crops(gpu):
     log("C2 leaf slots ordered differently from the oracle's own tree: %d of %d; crop pixels over 1e-4: %d of %d"
         % (slots_differ, len(sc["tri_mat"]), over, len(pix)))
     assert over <= len(pix) // 100
+    # the product's reference-order build (the reference's sort at every node) is the
+    # oracle's own tree, leaf order included, and its frame equals the oracle's on it
+    br = ctx().scene(sc).build_bvh(16, builder=P.BUILD_HOST_REFERENCE_ORDER)
+    t_ref = br.export_nodes()
+    for k in ("boxes", "child", "range", "leaf"):
+        assert np.array_equal(t_ref[k], t_own[k]), k
+    assert np.array_equal(br.export_order(), t_own["prim_ids"])
+    img3, st3 = br.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, two_streams=True)
+    got3 = img3.reshape(-1, 3)[pix]
+    log("C2 reference-order tree: crop pixels bit-identical to the oracle's own tree: %d of %d"
+        % (int((got3.view(np.uint32) == ref2.view(np.uint32)).all(axis=1).sum()), len(pix)))
+    assert np.array_equal(got3.view(np.uint32), ref2.view(np.uint32))
 
 
 @functools.lru_cache(maxsize=1)

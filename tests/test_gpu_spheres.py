@@ -138,9 +138,11 @@ def test_cpp_api_spheres(gpu):
         assert np.abs(img).mean() > 0.01
 
 
-def test_cpp_binned_sah_bvh_class(gpu):
+@pytest.mark.parametrize("reference_order", [False, True])
+def test_cpp_binned_sah_bvh_class(gpu, reference_order):
     """BinnedSAHBVH::Build / Intersect / IntersectBatch of the C++ API (AccelBase,
-    Core/Accel/AccelBase.h:29-56) over host rays equal pt_trace_closest on the same tree."""
+    Core/Accel/AccelBase.h:29-56) over host rays equal pt_trace_closest on the same tree
+    (built on the device, or on the host in the reference's leaf order)."""
     sc = scenes.sphere_primitives()
     r = rays(3000, 21)
     nt, ns = len(sc["tri_mat"]), len(sc["spheres"])
@@ -153,13 +155,14 @@ def test_cpp_binned_sah_bvh_class(gpu):
             np.ascontiguousarray(sc["tri_pos"], np.float32).tofile(f)
             np.ascontiguousarray(sc["spheres"], np.float32).tofile(f)
             r.tofile(f)
-        rr = subprocess.run([exe, fin, fout], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+        rr = subprocess.run([exe, fin, fout] + (["reference"] if reference_order else []), stdout=subprocess.PIPE,
+                            stderr=subprocess.STDOUT, text=True, timeout=300)
         assert rr.returncode == 0, rr.stdout
         raw = np.fromfile(fout, np.uint8)
     m = len(r) + 64 + 2
     prim = raw[:4 * m].view(np.int32)
     t = raw[4 * m:8 * m].view(np.float32)
-    b = ctx().scene(sc).build_bvh(16)
+    b = ctx().scene(sc).build_bvh(16, builder=P.BUILD_HOST_REFERENCE_ORDER if reference_order else P.BUILD_DEVICE)
     tg, pg = b.trace_closest(to_dev(r))
     tg, pg = tg.cpu().numpy(), pg.cpu().numpy()
     hit = pg >= 0
