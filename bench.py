@@ -8,7 +8,7 @@ queries (camera, bounce and shadow rays) per second over the timed steps, in Mra
 The oracle renders a random sample of the frame's pixels on the host cores: that is the
 CPU baseline, and the same pixels check the GPU frame (per-pixel relative error).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S] [--bvh-width 2|8]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S] [--bvh-width 2|4|8]
   python bench.py --mode trace    # BVH build and traversal microbenchmark (10M triangles)
   python bench.py --mode batch    # 64 uneven jobs: batched vs one by one
   python bench.py --mode serve    # Poisson request stream against the render server
@@ -34,8 +34,14 @@ HBM_PEAK_GBS = 8000.0
 # holds the boxes of both children, and each child box is one visit); 48 B per triangle
 # test (p0, e1, e2, n).
 RAY_IN_B, PATH_B, HIT_OUT_B, NODE_B, TRI_B = 32, 4, 8, 32, 48
-# 8-wide layout: one 128 B record per node visit (8 quantised child boxes and references)
+# wide layouts: one record per node visit (quantised child boxes and references): 128 B for
+# 8 children, 64 B for 4
 NODE8_B = 128
+NODE4_B = 64
+
+
+def node_bytes(width):
+    return {2: NODE_B, 4: NODE4_B, 8: NODE8_B}[width]
 
 
 def log(*a):
@@ -206,7 +212,7 @@ def trace_bench(args):
     log("closest: %.2f ms/launch, any: %.2f ms/launch, %d hits" % (np.mean(ms_c), np.mean(ms_a), hits))
     _, _, st = bvh.trace_stats(rays)
     n_node, n_tri = st[0] / nr, st[1] / nr
-    node_b = NODE8_B if width == 8 else NODE_B
+    node_b = node_bytes(width)
     b_ray = RAY_IN_B + HIT_OUT_B + n_node * node_b + n_tri * TRI_B
     avg_ms = float(np.mean(ms_c))
     achieved = nr * b_ray / (avg_ms * 1e-3) / 1e9
@@ -241,7 +247,7 @@ def trace_bench(args):
                                "2^%d rays (half pinhole primary, half surface rays with uniform directions), "
                                "closest hit" % (N, args.log2_rays),
                    "triangles": N, "rays": nr, "bvh": "binned SAH, 32 bins, leaf<=4, device builder, %s"
-                                                   % ("collapsed to 8-wide nodes" if width == 8
+                                                   % ("collapsed to %d-wide nodes" % width if width != 2
                                                       else "binary (reference topology and walk order)")},
         "closest_mrays_s": round(value, 2),
         "any_mrays_s": round(nr * args.steps / wall_a / 1e6, 2),
@@ -627,7 +633,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--compare-wide", action="store_true", help="frame mode: also render the 8-wide tree's frame")
+    ap.add_argument("--compare-wide", action="store_true",
+                    help="frame mode: also render the frame through the 4- and 8-wide trees")
     ap.add_argument("--separate-launches", action="store_true",
                     help="frame mode: also a frame with every launch on its own (per-kernel split)")
     ap.add_argument("--no-batch", action="store_true", help="frame mode: skip the batched form")
@@ -646,7 +653,7 @@ def main():
                          "--width x --height image; aov: the denoising AOVs of the single "
                          "large scene")
     ap.add_argument("--triangles", type=int, default=10_000_000, help="trace mode: triangle count")
-    ap.add_argument("--bvh-width", type=int, default=None, choices=[2, 8],
+    ap.add_argument("--bvh-width", type=int, default=None, choices=[2, 4, 8],
                     help="BVH width: 2 walks the binary tree in the reference's order (frame default), "
                          "8 the 8-wide tree (traversal microbenchmark default); batch mode: both are "
                          "built and the autotuner picks one unless this is given")
@@ -738,7 +745,7 @@ def main():
     log("logged frame:", L)
     assert C["trace_rays"] == L["trace_rays"] and C["trace_launches"] == L["trace_launches"]
     assert L["trace_rays"] + L["finish_rays"] == lst["queries"]
-    node_b = NODE8_B if width == 8 else NODE_B
+    node_b = node_bytes(width)
     s_ray = RAY_IN_B + PATH_B + HIT_OUT_B
     alg_bytes = L["trace_rays"] * s_ray + C["trace_nodes"] * node_b + C["trace_tris"] * TRI_B
     n_launch = max(L["trace_launches"], 1)
@@ -797,27 +804,35 @@ def main():
                   "pixels_bit_identical": int(np.all(got == ref, axis=1).sum()),
                   "pixels_over_1e-4": int((rel.max(axis=1) > 1e-4).sum())}
         log("parity vs oracle:", parity)
-    # The same frame through the 8-wide tree, on request (a walk that differs from the
-    # reference's on rays grazing a box within rounding of their hit).
+    # The same frame through the 4- and 8-wide trees, on request (walks that differ from the
+    # reference's on rays grazing a box within rounding of their hit). The exact frame equals
+    # the oracle's bit for bit, so differences from it are differences from the oracle.
     other = None
     if width == 2 and args.compare_wide:
-        b8 = s.build_bvh(16, width=8)
-        out8 = torch.empty_like(out)
-        b8.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=out8, two_streams=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        q8 = 0
-        for _ in range(2):
-            _, st8 = b8.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=out8, two_streams=True)
-            q8 += st8["queries"]
-        torch.cuda.synchronize()
-        w8 = time.perf_counter() - t0
-        img8 = out8.cpu().numpy()
-        other = {"bvh": "collapsed to 8-wide nodes (not the reference's walk)", "mrays_s": round(q8 / w8 / 1e6, 2),
-                 "ms_per_frame": round(w8 / 2 * 1e3, 2), "bvh_build_ms": round(b8.info()["build_ms"], 1),
-                 "pixels_identical_to_exact_frame": int(np.all(img8 == img, axis=2).sum())}
-        log("8-wide:", other)
-        b8.close()
+        other = {}
+        for ww in (4, 8):
+            bw = s.build_bvh(16, width=ww)
+            outw = torch.empty_like(out)
+            bw.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=outw, two_streams=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            qw = 0
+            for _ in range(2):
+                _, stw = bw.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=outw, two_streams=True)
+                qw += stw["queries"]
+            torch.cuda.synchronize()
+            tw = time.perf_counter() - t0
+            imgw = outw.cpu().numpy()
+            relw = np.abs(imgw.astype(np.float64) - img) / np.maximum(np.abs(img.astype(np.float64)), 1e-30)
+            other["width%d" % ww] = {
+                "bvh": "collapsed to %d-wide nodes (not the reference's walk)" % ww,
+                "mrays_s": round(qw / tw / 1e6, 2), "ms_per_frame": round(tw / 2 * 1e3, 2),
+                "bvh_build_ms": round(bw.info()["build_ms"], 1),
+                "pixels_identical_to_exact_frame": int(np.all(imgw == img, axis=2).sum()),
+                "pixels_over_1e-4": int((relw.max(axis=2) > 1e-4).sum()),
+                "max_rel_err": float(relw.max())}
+            log("%d-wide:" % ww, other["width%d" % ww])
+            bw.close()
     del bvh, s
     torch.cuda.empty_cache()
     batched = served = None
@@ -862,7 +877,7 @@ def main():
     if kernels is not None:
         res["kernels_separate_launches"] = kernels
     if other is not None:
-        res["bvh8_frame"] = other
+        res["wide_frames"] = other
     print(json.dumps(res), flush=True)
 
 if __name__ == "__main__":

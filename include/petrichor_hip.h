@@ -96,7 +96,10 @@ typedef struct pt_scene_desc {
 typedef struct pt_bvh_desc {
     int32_t num_bins; /* 16 in the reference (Core/Accel/BinnedSAHBVH.h:174); 32 for the microbenchmark */
     int32_t leaf_max; /* 4 in the reference (BinnedSAHBVH.cpp:79) */
-    int32_t width;    /* traversal node width: 2 (reference order) */
+    int32_t width;    /* traversal node width: 2 walks the binary tree in the reference's
+                       * order; 4 and 8 add wide nodes (64 B / 128 B, quantised child boxes,
+                       * nearest child first), which reach the same nearest hit except on rays
+                       * that graze a box within rounding of their hit */
     int32_t builder;  /* PT_BUILD_* below */
 } pt_bvh_desc;
 
@@ -224,6 +227,29 @@ int pt_trace_closest_host(pt_bvh* bvh, int64_t n, const float* rays, float* t_ou
  * [dist_min, dist_max], else 0. No reference counterpart (the reference's shadow
  * rays call the closest-hit query, Core/Integrator/PathTracing.cpp:293). */
 int pt_trace_any(pt_bvh* bvh, int64_t n, const float* rays, int32_t* hit_out, void* stream);
+
+/* The same queries over structure-of-arrays buffers (BinnedSAHBVH::Intersect over many
+ * rays, Core/Accel/BinnedSAHBVH.cpp:272-374; the Ray fields of Core/Ray.h:10-49 and the
+ * HitInfo of Core/HitInfo.h:14-27). Device pointers of n floats (ints) each. dist_min and
+ * dist_max may be NULL: 0 and FLT_MAX, as AccelBase::Intersect(ray, scene)
+ * (Core/Accel/AccelBase.h:44-48). Hits: t (+inf on a miss), prim (triangle index, then
+ * spheres; -1 on a miss) and, when not NULL, u and v: the barycentric coordinates of the
+ * hit as Triangle::Intersect forms them (Core/Geometry/Triangle.cpp:57-72; 0 for spheres
+ * and misses). The BVH keeps a staging copy of the rays (32 bytes per ray of its largest
+ * such call) until pt_bvh_destroy; calls on one BVH must not overlap. Hits equal
+ * pt_trace_closest's / pt_trace_any's for the same rays. */
+typedef struct pt_rays_soa {
+    const float *ox, *oy, *oz;
+    const float *dx, *dy, *dz;
+    const float *dist_min, *dist_max;
+} pt_rays_soa;
+typedef struct pt_hits_soa {
+    float* t;
+    int32_t* prim;
+    float *u, *v;
+} pt_hits_soa;
+int pt_trace_closest_soa(pt_bvh* bvh, int64_t n, const pt_rays_soa* rays, const pt_hits_soa* hits, void* stream);
+int pt_trace_any_soa(pt_bvh* bvh, int64_t n, const pt_rays_soa* rays, int32_t* hit_out, void* stream);
 
 /* Closest hit as pt_trace_closest, allocating nothing: no hipMalloc, hipFree or allocating
  * library call, for callers that cannot allocate (code under HIP graph capture, tight

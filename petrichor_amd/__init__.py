@@ -140,6 +140,8 @@ def lib():
     L.pt_trace_closest.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp]
     L.pt_trace_any.argtypes = [vp, ctypes.c_int64, vp, vp, vp]
     L.pt_trace_closest_noalloc.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp]
+    L.pt_trace_closest_soa.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(RaysSoa), ctypes.POINTER(HitsSoa), vp]
+    L.pt_trace_any_soa.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(RaysSoa), vp, vp]
     L.pt_trace_stats.argtypes = [vp, ctypes.c_int64, vp, vp, vp, _l]
     L.pt_render.argtypes = [vp, vp, ctypes.POINTER(RenderDesc), vp, ctypes.c_int32, ctypes.POINTER(RenderStats)]
     L.pt_render_aov.argtypes = [vp, vp, ctypes.c_int32, ctypes.POINTER(RenderDesc), vp, ctypes.c_int32,
@@ -176,11 +178,19 @@ def lib():
 # every symbol include/petrichor_hip.h declares
 EXPORTED = ["pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_scene_create", "pt_scene_destroy",
             "pt_bvh_build", "pt_bvh_destroy", "pt_bvh_export_order", "pt_bvh_export_nodes", "pt_bvh_build_host", "pt_bvh_info", "pt_trace_closest", "pt_trace_closest_host",
-            "pt_trace_any", "pt_trace_closest_noalloc", "pt_trace_stats", "pt_render", "pt_engine_create", "pt_engine_destroy",
+            "pt_trace_any", "pt_trace_closest_noalloc", "pt_trace_closest_soa", "pt_trace_any_soa", "pt_trace_stats", "pt_render", "pt_engine_create", "pt_engine_destroy",
             "pt_render_batch", "pt_engine_set_tuning", "pt_engine_get_tuning", "pt_server_create", "pt_server_destroy", "pt_server_last_error",
             "pt_server_add_scene", "pt_server_remove_scene", "pt_server_scene_resident", "pt_server_submit",
             "pt_server_poll", "pt_server_wait", "pt_server_preview", "pt_server_fetch", "pt_server_get_stats",
             "pt_tonemap_rgb8", "pt_obj_parse", "pt_render_aov", "pt_png_decode"]
+
+
+class RaysSoa(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "dist_min", "dist_max")]
+
+
+class HitsSoa(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("t", "prim", "u", "v")]
 
 
 def _ptr(a, t):
@@ -470,6 +480,48 @@ class Bvh:
         self.ctx.check(self.ctx._lib.pt_trace_closest(self.handle, n, rays.data_ptr(), t_out.data_ptr(),
                                                       prim_out.data_ptr(), st), "pt_trace_closest")
         return t_out, prim_out
+
+    @staticmethod
+    def _rays_soa(rays, default_window=False):
+        """pt_rays_soa over the columns of an [n, 8] ray tensor's structure-of-arrays copy;
+        default_window: null dist_min / dist_max (0 and FLT_MAX)."""
+        cols = rays.t().contiguous()  # [8, n]: ox oy oz dist_min dx dy dz dist_max
+        ptrs = [cols[k].data_ptr() for k in (0, 1, 2, 4, 5, 6, 3, 7)]
+        if default_window:
+            ptrs[6] = ptrs[7] = None
+        return RaysSoa(*ptrs), cols
+
+    def trace_closest_soa(self, rays, uv=True, stream=None, default_window=False):
+        """pt_trace_closest_soa on the columns of `rays` ([n, 8] as trace_closest) given as
+        separate arrays. Returns (t, prim, u, v) tensors (u, v None unless uv)."""
+        import torch
+        n = rays.shape[0]
+        assert rays.is_cuda and rays.dtype == torch.float32 and rays.shape[1] == 8
+        r, cols = self._rays_soa(rays, default_window)
+        t = torch.empty(n, dtype=torch.float32, device=rays.device)
+        p = torch.empty(n, dtype=torch.int32, device=rays.device)
+        u = torch.empty(n, dtype=torch.float32, device=rays.device) if uv else None
+        v = torch.empty(n, dtype=torch.float32, device=rays.device) if uv else None
+        h = HitsSoa(t.data_ptr(), p.data_ptr(), u.data_ptr() if uv else None, v.data_ptr() if uv else None)
+        st = ctypes.c_void_p(stream) if stream else None
+        self.ctx.check(self.ctx._lib.pt_trace_closest_soa(self.handle, n, ctypes.byref(r), ctypes.byref(h), st),
+                       "pt_trace_closest_soa")
+        if stream is None:
+            torch.cuda.synchronize()  # cols must outlive the kernels
+        return t, p, u, v
+
+    def trace_any_soa(self, rays, stream=None):
+        """pt_trace_any_soa on the columns of `rays`; returns the int32 hit flags."""
+        import torch
+        n = rays.shape[0]
+        r, cols = self._rays_soa(rays)
+        out = torch.empty(n, dtype=torch.int32, device=rays.device)
+        st = ctypes.c_void_p(stream) if stream else None
+        self.ctx.check(self.ctx._lib.pt_trace_any_soa(self.handle, n, ctypes.byref(r), out.data_ptr(), st),
+                       "pt_trace_any_soa")
+        if stream is None:
+            torch.cuda.synchronize()
+        return out
 
     def trace_closest_noalloc(self, rays, t_out, prim_out, stream=None):
         """pt_trace_closest_noalloc: as trace_closest on caller-given outputs, on the GPU's

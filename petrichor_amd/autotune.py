@@ -1,7 +1,7 @@
 """Autotuner of the batched engine.
 
 Chooses, for a given set of render jobs, the BVH node width (when the jobs are given over
-BVHs of both widths), the path-pool size (the capacity of every ray queue), the graph
+BVHs of several widths: 2, 4, 8), the path-pool size (the capacity of every ray queue), the graph
 replays per host synchronisation and the launch shapes of the trace and shading kernels
 (pt_tuning). It times pt_render_batch on a calibration batch - the same scenes and
 resolutions at a few samples per pixel - first for each width and then for each pool size
@@ -11,10 +11,10 @@ candidate of one width renders the same images (tuning only changes launch shape
 scheduling), so the fastest one is taken as is.
 
 Only the binary width walks the tree exactly as the reference (BinnedSAHBVH::Intersect,
-Core/Accel/BinnedSAHBVH.cpp:272-374). The 8-wide walk can return another hit on rays that
-graze a box within rounding of their hit (33 of 1.8M pixels of the single large scene
-differ by more than 1e-4), so it is a candidate only when the caller accepts that
-(exact_only=False).
+Core/Accel/BinnedSAHBVH.cpp:272-374). The 4- and 8-wide walks can return another hit on
+rays that graze a box within rounding of their hit (with 8-wide nodes 35 of 2,073,600
+pixels of the single large scene differ by more than 1e-4), so they are candidates only
+when the caller accepts that (exact_only=False).
 """
 import time
 
@@ -42,7 +42,7 @@ def calibration_jobs(jobs, spp=2):
 def tune(ctx, jobs, pool_sizes=None, passes=1, log=None, by_width=None, calib_spp=8, exact_only=True):
     """Returns {"width", "jobs", "max_paths", "tuning", "seconds", "trials"} for the fastest
     configuration found on the calibration batch of `jobs`. by_width (optional) maps a BVH
-    width (2, 8) to the same jobs over BVHs of that width; "jobs" is then the chosen list.
+    width (2, 4, 8) to the same jobs over BVHs of that width; "jobs" is then the chosen list.
     exact_only: only binary BVHs (the reference's walk) are candidates."""
     from . import Engine
     by_width = dict(by_width) if by_width else {getattr(jobs[0].bvh, "width", 2): jobs}

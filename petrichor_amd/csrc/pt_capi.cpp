@@ -183,34 +183,36 @@ int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
     return PT_OK;
 }
 
-// Adds the 8-wide nodes to a built binary tree (bvh width 8). On failure the tree is
+// Adds the wide nodes (width 4 or 8) to a built binary tree. On failure the tree is
 // destroyed and *out cleared.
-int make_wide(pt_bvh* b, pt_bvh** out)
+int make_wide(pt_bvh* b, int width, pt_bvh** out)
 {
     pt_context* ctx = b->scene->ctx;
     const auto t0 = std::chrono::steady_clock::now();
     Wide8 w;
-    hipError_t e = gpu_collapse_bvh8(b->nodes_d, b->num_inner, b->view.root_ref, ctx->stream, &w);
-    // a wide level leaves at most 7 children pending on the stack
-    if (e == hipSuccess && 7 * w.depth + 1 > 192) {
+    hipError_t e = gpu_collapse_wide(width, b->nodes_d, b->num_inner, b->view.root_ref, ctx->stream, &w);
+    // a wide level leaves at most width - 1 children pending on the stack
+    const int stack = width == 8 ? PT_STACK8 : PT_STACK4;
+    if (e == hipSuccess && (width - 1) * w.depth + 1 > stack) {
         if (w.nodes_d) (void)hipFree(w.nodes_d);
         pt_bvh_destroy(b);
         *out = nullptr;
-        return fail(ctx, PT_ERR_UNSUPPORTED, "8-wide tree too deep for the traversal stack");
+        return fail(ctx, PT_ERR_UNSUPPORTED, "wide tree too deep for the traversal stack");
     }
     if (e != hipSuccess) {
         pt_bvh_destroy(b);
         *out = nullptr;
         return fail(ctx, e == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP,
-                    std::string("8-wide collapse: ") + hipGetErrorString(e));
+                    std::string("wide collapse: ") + hipGetErrorString(e));
     }
     b->nodes8_d = w.nodes_d;
     b->num_nodes8 = w.num_nodes;
     b->depth8 = w.depth;
-    b->width = 8;
+    b->width = width;
     b->view.nodes8 = w.nodes_d;
     b->view.root8 = w.root;
-    b->device_bytes += (int64_t)w.num_nodes * 128;
+    b->view.width = width;
+    b->device_bytes += (int64_t)w.num_nodes * wide_node_bytes(width);
     b->build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = b;
     return PT_OK;
@@ -549,7 +551,7 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     const int leaf_max = desc ? desc->leaf_max : 4;
     const int width = desc ? desc->width : 2;
     const int builder = desc ? desc->builder : PT_BUILD_DEVICE;
-    if (width != 2 && width != 8) return fail(ctx, PT_ERR_UNSUPPORTED, "BVH width must be 2 or 8");
+    if (width != 2 && width != 4 && width != 8) return fail(ctx, PT_ERR_UNSUPPORTED, "BVH width must be 2, 4 or 8");
     if (bins < 2 || bins > 256 || leaf_max < 1) return fail(ctx, PT_ERR_ARG, "bad BVH parameters");
     if (builder != PT_BUILD_DEVICE && builder != PT_BUILD_HOST && builder != PT_BUILD_HOST_REFERENCE_ORDER)
         return fail(ctx, PT_ERR_ARG, "unknown builder");
@@ -558,7 +560,7 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
     if (builder == PT_BUILD_DEVICE && s->num_prims() > 0) {
         const int rc = build_device(s, bins, leaf_max, out);
         if (rc != PT_OK || width == 2) return rc;
-        return make_wide(*out, out);
+        return make_wide(*out, width, out);
     }
     pt_bvh* b = new pt_bvh();
     b->scene = s;
@@ -643,7 +645,7 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
         return rc;
     }
     *out = b;
-    if (width == 8) return make_wide(b, out);
+    if (width != 2) return make_wide(b, width, out);
     return PT_OK;
 }
 
@@ -652,7 +654,7 @@ int pt_bvh_destroy(pt_bvh* b)
     if (!b) return PT_ERR_ARG;
     free_all({b->nodes8_d, b->nodes_d, b->big_leaves_d, b->inner_range_d, b->cursor_d, b->tris_d, b->prim_ids_d, b->tri_mat_d,
               b->tri_mat_shading_d, b->tri_smooth_d, b->tri_nrm_d, b->order_keys_d, b->order_idx_d, b->order_temp_d,
-              b->tri_uv_d, b->parents_d, b->kinds_d, b->host_io_d});
+              b->tri_uv_d, b->parents_d, b->kinds_d, b->host_io_d, b->soa_rays_d});
     delete b;
     return PT_OK;
 }
@@ -742,12 +744,15 @@ TraceSegment api_segment(const int* count_d, const float* rays, bool any, float*
 
 // cursor_d holds the work cursors (PT_CURSOR_STRIDE apart), cursor_d[PT_CURSOR_INTS] the ray
 // count of the launch.
+// slots: closest hits give the leaf slot (and t on a miss as the walk leaves it) instead
+// of the primitive index.
 int api_trace(pt_bvh* b, int64_t n, const float* rays, bool any, float* t_out, int32_t* k_out,
-              unsigned long long* stats, hipStream_t st)
+              unsigned long long* stats, hipStream_t st, bool slots = false)
 {
     pt_context* ctx = b->scene->ctx;
     TraceJob job;
     job.seg[0] = api_segment(b->cursor_d + PT_CURSOR_INTS, rays, any, t_out, k_out);
+    if (slots) job.seg[0].map_prim = 0;
     job.num_seg = 1;
     job.views = nullptr;
     job.job_of_path = nullptr;
@@ -867,6 +872,53 @@ int pt_trace_closest_host(pt_bvh* b, int64_t n, const float* rays, float* t_out,
     PT_HIP(ctx, hipMemcpyAsync(prim_out, prim_d, 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
     PT_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return PT_OK;
+}
+
+// The structure-of-arrays entry points: rays packed into the BVH's staging records, traced
+// as by pt_trace_closest / pt_trace_any, hits written per array.
+static int trace_soa(pt_bvh* b, int64_t n, const pt_rays_soa* r, bool any, float* t, int32_t* k, float* u, float* v,
+                     void* stream)
+{
+    pt_context* ctx = b->scene->ctx;
+    if (n == 0) return PT_OK;
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    if (b->soa_rays_cap < n) {
+        PT_HIP(ctx, hipStreamSynchronize(st)); // the old records may still be read
+        (void)hipFree(b->soa_rays_d);
+        b->soa_rays_d = nullptr;
+        b->soa_rays_cap = 0;
+        PT_HIP(ctx, hipMalloc((void**)&b->soa_rays_d, 2 * sizeof(float4) * (size_t)n));
+        b->soa_rays_cap = n;
+    }
+    PT_HIP(ctx, launch_pack_rays_soa(n, r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, r->dist_min, r->dist_max,
+                                     b->soa_rays_d, st));
+    const int rc = api_trace(b, n, (const float*)b->soa_rays_d, any, t, k, nullptr, st, !any);
+    if (rc != PT_OK || any) return rc;
+    PT_HIP(ctx, launch_hits_soa(n, b->view, b->soa_rays_d, t, k, u, v, st));
+    return PT_OK;
+}
+
+static bool soa_rays_ok(const pt_rays_soa* r)
+{
+    return r && r->ox && r->oy && r->oz && r->dx && r->dy && r->dz;
+}
+
+int pt_trace_closest_soa(pt_bvh* b, int64_t n, const pt_rays_soa* rays, const pt_hits_soa* hits, void* stream)
+{
+    if (!b) return PT_ERR_ARG;
+    pt_context* ctx = b->scene->ctx;
+    if (n < 0 || n > 0x7fffff00 || (n > 0 && (!soa_rays_ok(rays) || !hits || !hits->t || !hits->prim)))
+        return fail(ctx, PT_ERR_ARG, "bad trace arguments");
+    return trace_soa(b, n, rays, false, hits->t, hits->prim, hits->u, hits->v, stream);
+}
+
+int pt_trace_any_soa(pt_bvh* b, int64_t n, const pt_rays_soa* rays, int32_t* hit_out, void* stream)
+{
+    if (!b) return PT_ERR_ARG;
+    pt_context* ctx = b->scene->ctx;
+    if (n < 0 || n > 0x7fffff00 || (n > 0 && (!soa_rays_ok(rays) || !hit_out)))
+        return fail(ctx, PT_ERR_ARG, "bad trace arguments");
+    return trace_soa(b, n, rays, true, nullptr, hit_out, nullptr, nullptr, stream);
 }
 
 int pt_trace_any(pt_bvh* b, int64_t n, const float* rays, int32_t* hit_out, void* stream)

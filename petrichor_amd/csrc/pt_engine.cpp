@@ -318,7 +318,7 @@ int pt_render_aov(pt_scene* s, pt_bvh* b, int32_t aov, const pt_render_desc* d, 
     if (e == hipSuccess) e = hipMemsetAsync(q_d, 0, sizeof(unsigned long long), ctx->stream);
     if (e == hipSuccess)
         e = launch_render_aov(pt_engine::scene_view(s, b), d->width, d->height, d->spp, d->max_bounces, d->seed,
-                              aov == PT_AOV_UV ? 2 : (aov == PT_AOV_DENOISING_NORMAL ? 1 : 0), b->view.nodes8 != nullptr,
+                              aov == PT_AOV_UV ? 2 : (aov == PT_AOV_DENOISING_NORMAL ? 1 : 0), b->view.nodes8 ? b->view.width : 2,
                               j.out_d, q_d, ctx->stream);
     unsigned long long q = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&q, q_d, sizeof q, hipMemcpyDeviceToHost, ctx->stream);

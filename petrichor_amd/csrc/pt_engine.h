@@ -362,7 +362,8 @@ struct pt_engine {
 
     // Starts a job: path slots, job table entry, set-up kernel appending its paths and
     // camera rays to the queues the next iteration reads.
-    static int job_width(const pt::EngineJob& j) { return j.bvh->view.nodes8 ? 8 : 2; }
+    static int job_width_of(const pt_bvh* b) { return b->view.nodes8 ? b->view.width : 2; }
+    static int job_width(const pt::EngineJob& j) { return job_width_of(j.bvh); }
 
     // A multi-job engine runs BVHs of one width at a time (one trace kernel per launch).
     bool width_ok(int w) const
@@ -546,7 +547,7 @@ struct pt_engine {
             g.out_k = outs_k[k];
         }
         tj.views = multi ? views_d : nullptr;
-        tj.wide = multi && width == 8;
+        tj.width = multi ? width : 2;
         tj.ext = multi && ext;
         tj.job_of_path = multi ? ps.job : nullptr;
         tj.queries = queries_d;
@@ -670,7 +671,7 @@ struct pt_engine {
             g.out_k = outs_k[k];
         }
         tj.views = multi ? views_d : nullptr;
-        tj.wide = multi && width == 8;
+        tj.width = multi ? width : 2;
         tj.ext = multi && ext;
         tj.job_of_path = multi ? ps.job : nullptr;
         tj.queries = queries_d;
@@ -687,13 +688,13 @@ struct pt_engine {
         a.sh_t = sh_t;
         a.sh_k = sh_k;
         const pt::HitBuffers h{hit_t, hit_k, sh_t, sh_k};
-        const bool wide = multi ? width == 8 : sc1.bvh.nodes8 != nullptr;
+        const int walk_width = multi ? width : (sc1.bvh.nodes8 != nullptr ? sc1.bvh.width : 2);
         const bool log_finish = logging && finish_n < kFinishLogCap;
         if (log_finish) {
             PT_HIP(ctx, pt::launch_copy_u64(queries_d, finish_q_d + 2 * finish_n, st));
             PT_HIP(ctx, hipEventRecord(finish_ev[2 * finish_n], st));
         }
-        PT_HIP(ctx, pt::launch_finish_paths(integrator1, sc1, job1, multi ? jobs_d : nullptr, wide, a, h, queries_d,
+        PT_HIP(ctx, pt::launch_finish_paths(integrator1, sc1, job1, multi ? jobs_d : nullptr, walk_width, a, h, queries_d,
                                             st));
         if (log_finish) {
             PT_HIP(ctx, hipEventRecord(finish_ev[2 * finish_n + 1], st));
@@ -736,7 +737,7 @@ struct pt_engine {
             g.out_t = outs_t[k];
             g.out_k = outs_k[k];
             tj.views = multi ? views_d : nullptr;
-            tj.wide = multi && width == 8;
+            tj.width = multi ? width : 2;
         tj.ext = multi && ext;
             tj.job_of_path = multi ? ps.job : nullptr;
             tj.queries = queries_d;
@@ -805,7 +806,7 @@ struct pt_engine {
             g.out_t = outs_t[k];
             g.out_k = outs_k[k];
             tj.views = multi ? views_d : nullptr;
-            tj.wide = multi && width == 8;
+            tj.width = multi ? width : 2;
         tj.ext = multi && ext;
             tj.job_of_path = multi ? ps.job : nullptr;
             tj.queries = queries_d;

@@ -73,12 +73,14 @@ struct pt_bvh {
     int32_t* tri_mat_shading_d = nullptr;
     int32_t* tri_smooth_d = nullptr;
     float* tri_nrm_d = nullptr;
-    uint4* nodes8_d = nullptr; // 8-wide nodes (width 8 only)
+    uint4* nodes8_d = nullptr; // wide nodes (width 4 or 8 only)
     float* tri_uv_d = nullptr; // 6 floats per leaf slot, when the scene has texture coordinates
     int32_t* parents_d = nullptr; // inner nodes' parents, for trees deeper than the traversal stack
     int8_t* kinds_d = nullptr;    // per leaf slot: 1 for a sphere; only when the scene has spheres
     void* host_io_d = nullptr;    // pt_trace_closest_host's device staging (40 bytes per ray)
     int64_t host_io_cap = 0;
+    float4* soa_rays_d = nullptr; // the structure-of-arrays entry points' ray records (32 bytes per ray)
+    int64_t soa_rays_cap = 0;
     int width = 2;
     int num_nodes8 = 0;
     int depth8 = 0;

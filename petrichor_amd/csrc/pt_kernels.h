@@ -10,6 +10,10 @@
 // tree level plus the current node, so trees up to PT_MAX_STACK - 2 levels deep
 // are accepted (checked when the BVH is built).
 #define PT_MAX_STACK 64
+// Wide trees: a level leaves at most 7 (8-wide) or 3 (4-wide) children pending, so those
+// stacks take trees of up to 27 and 31 wide levels (checked when the wide nodes are made).
+#define PT_STACK8 192
+#define PT_STACK4 96
 
 namespace pt {
 
@@ -22,9 +26,10 @@ struct BvhView {
     const int2* big_leaves;   // (first slot, count) of leaves over PT_REF_MAX_SMALL_LEAF triangles
     V3 root_lo, root_hi;
     uint32_t root_ref;        // PT_REF_* encoding, PT_REF_EMPTY for an empty scene
-    const int2* inner_range = nullptr; // leaf-slot range of every binary inner node (ties, 8-wide)
-    const uint4* nodes8 = nullptr;  // 8-wide nodes (pt_wide.hip), null for the binary traversal
-    uint32_t root8 = PT_REF_EMPTY;  // root of the 8-wide tree (wide node 0, or a leaf)
+    const int2* inner_range = nullptr; // leaf-slot range of every binary inner node (ties, wide walks)
+    const uint4* nodes8 = nullptr;  // wide nodes (pt_wide.hip), null for the binary traversal
+    uint32_t root8 = PT_REF_EMPTY;  // root of the wide tree (wide node 0, or a leaf)
+    int width = 2;                  // node width of the walk: 2, or 4 / 8 with nodes8
     // parent of every binary inner node (-1 for the root), set only for trees deeper than
     // the traversal stack, which are then walked without one
     const int32_t* parents = nullptr;
@@ -133,7 +138,7 @@ struct TraceJob {
     const int32_t* job_of_path;
     unsigned long long* queries; // optional: += rays traced by this launch
     int chunk = 64;              // rays a wave takes from the cursor at a time (>= 64; set by the launcher)
-    int wide = 0;                // 8-wide traversal (every BVH of a multi-job launch has the same width)
+    int width = 2;               // node width of the walk (every BVH of a multi-job launch has the same)
     int ext = 0;                 // multi-job launches: some BVH has a deep tree or spheres (the extended walk)
     // optional launch log: the launch writes its span (and with log_counts its node visits
     // and triangle tests) into launch_log[*launch_index]; launch_log_tick then records the
@@ -171,6 +176,14 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job, int* cursor, 
                             hipStream_t st, const LaunchTuning& tune = LaunchTuning());
 // Zeroes counters[i * stride] for i < n (n <= 64).
 hipError_t launch_zero(int* counters, int n, hipStream_t st, int stride = 1);
+// Structure-of-arrays rays to the trace kernel's ray records, and its per-ray (t, leaf
+// slot) results to structure-of-arrays hits (primitive index, +inf / -1 on a miss, and
+// the barycentric u, v when u / v are not null), in place (pt_trace.hip).
+hipError_t launch_pack_rays_soa(int64_t n, const float* ox, const float* oy, const float* oz, const float* dx,
+                                const float* dy, const float* dz, const float* dmin, const float* dmax, float4* out,
+                                hipStream_t st);
+hipError_t launch_hits_soa(int64_t n, const BvhView& b, const float4* rays, float* t, int32_t* prim, float* u, float* v,
+                           hipStream_t st);
 // *p = value, by a kernel (no host copy).
 hipError_t launch_set(int* p, int value, hipStream_t st);
 // After a logged trace launch: log[*index].items = the three segment counts, ++*index
@@ -227,19 +240,19 @@ struct HitBuffers {
 // Runs the live paths a.active_in[0 .. *a.active_count) to the end of their pixels, one
 // thread per path (shading events and the rays they ask for, in the wavefront's order,
 // with the same results); adds the rays traced to *queries. jobs: the multi-job table (the
-// job of each path, then sc and job are unused), or null for a single-job render. wide:
-// 8-wide BVHs. The grid holds PT_FINISH_BLOCKS_PER_CU blocks of 256 per CU, looping over
+// job of each path, then sc and job are unused), or null for a single-job render. width:
+// the BVHs' node width (2, 4 or 8). The grid holds PT_FINISH_BLOCKS_PER_CU blocks of 256 per CU, looping over
 // the paths.
 #define PT_FINISH_BLOCKS_PER_CU 4
 hipError_t launch_finish_paths(int integrator, const SceneView& sc, const JobView& job, const JobDev* jobs,
-                               bool wide, const ShadeArgs& a, const HitBuffers& h, unsigned long long* queries,
+                               int width, const ShadeArgs& a, const HitBuffers& h, unsigned long long* queries,
                                hipStream_t st);
 
 // An AOV of a w x h image, one thread per pixel (pt_kernels.hip): mode 0 the denoising
 // albedo, 1 the denoising normal, 2 the first hit's texture coordinates; queries
 // (optional) receives the rays traced.
 hipError_t launch_render_aov(const SceneView& sc, int w, int h, int spp, int max_bounces, uint32_t seed,
-                             int mode, bool wide, float* out, unsigned long long* queries, hipStream_t st);
+                             int mode, int width, float* out, unsigned long long* queries, hipStream_t st);
 
 // Coherence order of n rays (pt_order.hip): the ray indices sorted by the Morton code of
 // the origin in the box (stable) end in *sorted, one of the two halves of idx; keys and

@@ -728,7 +728,7 @@ __global__ void __launch_bounds__(256) k_shade_multi(const JobDev* __restrict__ 
 
 // One ray traced by one thread with the walk of the trace kernel; t and k as the trace
 // kernel writes them for the path queues (leaf slot, -1 on a miss).
-template <bool WIDE>
+template <int W>
 __device__ __forceinline__ void trace_single(const BvhView& b, V3 o, float dmin, V3 d, float dmax, bool any,
                                              uint2* lds, StackEntry* spill, float* t, int* k)
 {
@@ -736,7 +736,7 @@ __device__ __forceinline__ void trace_single(const BvhView& b, V3 o, float dmin,
     TraceRay r;
     TraceState s;
     if (trace_begin(b, r0, r1, any, r, s)) {
-        while (!(WIDE ? trace8_step<false, true>(b, r, s, lds, spill) : trace_step<false, true>(b, r, s, lds, spill))) {
+        while (!trace_any_width_step<false, true, W>(b, r, s, lds, spill)) {
         }
     }
     *t = s.best_t;
@@ -754,15 +754,15 @@ __device__ __forceinline__ void trace_single(const BvhView& b, V3 o, float dmin,
 // first hit), so they are not run through the wavefront queues.
 // The UV AOV (AOVUVCoordinate::Render, Core/AOV/AOVUVCoordinate.cpp:11-51) is mode 2: the
 // mean over spp camera rays of the first hit's texture coordinates (a miss adds zero).
-template <bool WIDE>
-__global__ void __launch_bounds__(256, WIDE ? 1 : 4) k_render_aov(SceneView sc, int w, int h, int spp, int max_bounces,
+template <int W>
+__global__ void __launch_bounds__(256, W == 8 ? 1 : 4) k_render_aov(SceneView sc, int w, int h, int spp, int max_bounces,
                                                     uint32_t seed, int mode, float* out,
                                                     unsigned long long* queries)
 {
     const bool normal_aov = mode == 1;
-    __shared__ uint2 lds_stack[lds_stack_entries<WIDE>() * 256];
+    __shared__ uint2 lds_stack[lds_stack_entries<W>() * 256];
     uint2* lds = lds_stack + threadIdx.x;
-    StackEntry spill[(WIDE ? PT_STACK8 : PT_MAX_STACK) - lds_stack_entries<WIDE>()];
+    StackEntry spill[stack_entries<W>() - lds_stack_entries<W>()];
     const int n = w * h;
     unsigned long long q = 0;
     for (int pix = blockIdx.x * blockDim.x + threadIdx.x; pix < n; pix += gridDim.x * blockDim.x) {
@@ -775,7 +775,7 @@ __global__ void __launch_bounds__(256, WIDE ? 1 : 4) k_render_aov(SceneView sc, 
             if (mode == 2) {
                 float t;
                 int k;
-                trace_single<WIDE>(sc.bvh, ray.o, PT_EPS, ray.dir, PT_INFINITY, false, lds, spill, &t, &k);
+                trace_single<W>(sc.bvh, ray.o, PT_EPS, ray.dir, PT_INFINITY, false, lds, spill, &t, &k);
                 ++q;
                 if (k >= 0) sum = sum + hit_uv(sc, k, ray.o, ray.dir);
                 continue;
@@ -783,7 +783,7 @@ __global__ void __launch_bounds__(256, WIDE ? 1 : 4) k_render_aov(SceneView sc, 
             for (int bounce = 0; bounce < max_bounces; ++bounce) {
                 float t;
                 int k;
-                trace_single<WIDE>(sc.bvh, ray.o, PT_EPS, ray.dir, PT_INFINITY, false, lds, spill, &t, &k);
+                trace_single<W>(sc.bvh, ray.o, PT_EPS, ray.dir, PT_INFINITY, false, lds, spill, &t, &k);
                 ++q;
                 if (k < 0) {
                     contrib = ray.throughput * (normal_aov ? -ray.dir : env_color(sc.env, ray.dir));
@@ -816,24 +816,24 @@ __global__ void __launch_bounds__(256, WIDE ? 1 : 4) k_render_aov(SceneView sc, 
 }
 
 hipError_t launch_render_aov(const SceneView& sc, int w, int h, int spp, int max_bounces, uint32_t seed,
-                             int mode, bool wide, float* out, unsigned long long* queries, hipStream_t st)
+                             int mode, int width, float* out, unsigned long long* queries, hipStream_t st)
 {
     const int64_t n = (int64_t)w * h;
     if (n <= 0) return hipSuccess;
     const int64_t want = (n + 255) / 256, cap = (int64_t)cu_count() * PT_FINISH_BLOCKS_PER_CU;
     const dim3 grid((unsigned)(want < cap ? want : cap)), block(256);
-    if (wide)
-        hipLaunchKernelGGL(k_render_aov<true>, grid, block, 0, st, sc, w, h, spp, max_bounces, seed, mode, out,
-                           queries);
+    if (width == 8)
+        hipLaunchKernelGGL(k_render_aov<8>, grid, block, 0, st, sc, w, h, spp, max_bounces, seed, mode, out, queries);
+    else if (width == 4)
+        hipLaunchKernelGGL(k_render_aov<4>, grid, block, 0, st, sc, w, h, spp, max_bounces, seed, mode, out, queries);
     else
-        hipLaunchKernelGGL(k_render_aov<false>, grid, block, 0, st, sc, w, h, spp, max_bounces, seed, mode, out,
-                           queries);
+        hipLaunchKernelGGL(k_render_aov<2>, grid, block, 0, st, sc, w, h, spp, max_bounces, seed, mode, out, queries);
     return hipGetLastError();
 }
 
 // Events of path p from its current hits to the end of its sample, then of the job's next
 // items while there are any. Returns the rays traced.
-template <int INTEGRATOR, bool WIDE>
+template <int INTEGRATOR, int W>
 __device__ __forceinline__ unsigned long long finish_path(const SceneView& sc, const JobView& job,
                                                           const ShadeArgs& A, const HitBuffers& H, int p,
                                                           uint2* lds, StackEntry* spill)
@@ -859,18 +859,18 @@ __device__ __forceinline__ unsigned long long finish_path(const SceneView& sc, c
         float t;
         int k;
         if (o.want_any) {
-            trace_single<WIDE>(sc.bvh, o.sh_o, 0.0f, o.sh_d, FLT_MAX, true, lds, spill, &t, &k);
+            trace_single<W>(sc.bvh, o.sh_o, 0.0f, o.sh_d, FLT_MAX, true, lds, spill, &t, &k);
             H.sh_k[p] = k;
             ++q;
         }
         if (o.want_closest) {
-            trace_single<WIDE>(sc.bvh, o.sh_o, 0.0f, o.sh_d, FLT_MAX, false, lds, spill, &t, &k);
+            trace_single<W>(sc.bvh, o.sh_o, 0.0f, o.sh_d, FLT_MAX, false, lds, spill, &t, &k);
             H.sh_t[p] = t;
             H.sh_k[p] = k;
             ++q;
         }
         if (o.want_ext) {
-            trace_single<WIDE>(sc.bvh, o.ext_o, o.ext_dmin, o.ext_d, o.ext_dmax, false, lds, spill, &t, &k);
+            trace_single<W>(sc.bvh, o.ext_o, o.ext_dmin, o.ext_d, o.ext_dmax, false, lds, spill, &t, &k);
             H.hit_t[p] = t;
             H.hit_k[p] = k;
             ++q;
@@ -884,23 +884,23 @@ __device__ __forceinline__ unsigned long long finish_path(const SceneView& sc, c
 // iterations; what changes is that a path no longer waits for the slowest ray of each
 // iteration, which near the end of a render (few live paths, each iteration as long as
 // its longest ray) is where the time goes. Multi-job launches read each path's job.
-template <int INTEGRATOR, bool WIDE, bool MULTI>
+template <int INTEGRATOR, int W, bool MULTI>
 __global__ void __launch_bounds__(256) k_finish_paths(SceneView sc1, JobView job1, const JobDev* jobs, ShadeArgs A,
                                                       HitBuffers H, unsigned long long* queries)
 {
-    __shared__ uint2 lds_stack[lds_stack_entries<WIDE>() * 256];
+    __shared__ uint2 lds_stack[lds_stack_entries<W>() * 256];
     uint2* lds = lds_stack + threadIdx.x;
-    StackEntry spill[(WIDE ? PT_STACK8 : PT_MAX_STACK) - lds_stack_entries<WIDE>()];
+    StackEntry spill[stack_entries<W>() - lds_stack_entries<W>()];
     const int n = *A.active_count;
     unsigned long long q = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int p = A.active_in[i];
         if (MULTI) {
             const JobDev* J = jobs + A.ps.job[p];
-            if (J->job.integrator == 0) q += finish_path<0, WIDE>(J->sc, J->job, A, H, p, lds, spill);
-            else q += finish_path<1, WIDE>(J->sc, J->job, A, H, p, lds, spill);
+            if (J->job.integrator == 0) q += finish_path<0, W>(J->sc, J->job, A, H, p, lds, spill);
+            else q += finish_path<1, W>(J->sc, J->job, A, H, p, lds, spill);
         } else {
-            q += finish_path<INTEGRATOR, WIDE>(sc1, job1, A, H, p, lds, spill);
+            q += finish_path<INTEGRATOR, W>(sc1, job1, A, H, p, lds, spill);
         }
     }
     if (q) atomicAdd(queries, q);
@@ -1082,22 +1082,22 @@ hipError_t launch_shade(int integrator, const SceneView& sc, const JobView& job,
 }
 
 hipError_t launch_finish_paths(int integrator, const SceneView& sc, const JobView& job, const JobDev* jobs,
-                               bool wide, const ShadeArgs& a, const HitBuffers& h, unsigned long long* queries,
+                               int width, const ShadeArgs& a, const HitBuffers& h, unsigned long long* queries,
                                hipStream_t st)
 {
     const dim3 grid((unsigned)(cu_count() * PT_FINISH_BLOCKS_PER_CU)), block(256);
 #define PT_LAUNCH_FINISH(I, W, M) \
     hipLaunchKernelGGL((k_finish_paths<I, W, M>), grid, block, 0, st, sc, job, jobs, a, h, queries)
-    if (jobs) {
-        if (wide) PT_LAUNCH_FINISH(0, true, true);
-        else PT_LAUNCH_FINISH(0, false, true);
-    } else if (integrator == 0) {
-        if (wide) PT_LAUNCH_FINISH(0, true, false);
-        else PT_LAUNCH_FINISH(0, false, false);
-    } else {
-        if (wide) PT_LAUNCH_FINISH(1, true, false);
-        else PT_LAUNCH_FINISH(1, false, false);
-    }
+#define PT_LAUNCH_FINISH_W(I, M)                  \
+    do {                                          \
+        if (width == 8) PT_LAUNCH_FINISH(I, 8, M);      \
+        else if (width == 4) PT_LAUNCH_FINISH(I, 4, M); \
+        else PT_LAUNCH_FINISH(I, 2, M);                 \
+    } while (0)
+    if (jobs) PT_LAUNCH_FINISH_W(0, true);
+    else if (integrator == 0) PT_LAUNCH_FINISH_W(0, false);
+    else PT_LAUNCH_FINISH_W(1, false);
+#undef PT_LAUNCH_FINISH_W
 #undef PT_LAUNCH_FINISH
     return hipGetLastError();
 }

@@ -167,7 +167,7 @@ struct pt_server {
             probe.npix = r->npix;
             probe.items = items;
             const bool fits = (int64_t)running.size() < E->max_jobs && E->largest_free() >= E->job_slots(probe) &&
-                              E->width_ok(scenes.at(r->scene_id).bvh->view.nodes8 ? 8 : 2) && make_room(bytes);
+                              E->width_ok(pt_engine::job_width_of(scenes.at(r->scene_id).bvh)) && make_room(bytes);
             if (!fits) {
                 if (running.empty() && it == queue.begin() && !make_room(bytes)) {
                     // nothing will free memory for it: no request runs and no scene is idle

@@ -7,6 +7,8 @@
 // traversal stacks) sets the blocks per CU.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
+
 #include "pt_device.h"
 #include "pt_kernels.h"
 #include "pt_traverse.h"
@@ -48,7 +50,7 @@ __device__ __forceinline__ void part_item(const int* seg_n, int num_seg, int x, 
     }
 }
 
-template <bool STATS, bool MULTI, bool WIDE, bool EXT>
+template <bool STATS, bool MULTI, int W, bool EXT>
 __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int* cursor, unsigned long long* stats)
 {
     // segment sizes come from device counters (written by the previous stage)
@@ -63,9 +65,9 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
     // launch log: the index stays put during the launch (the tick after it advances it)
     LaunchRec* rec = job.launch_log ? job.launch_log + *job.launch_index : nullptr;
     if (rec && threadIdx.x == 0) atomicMin(&rec->t0, (unsigned long long)wall_clock64());
-    __shared__ uint2 lds_stack[lds_stack_entries<WIDE>() * PT_TRACE_BLOCK];
+    __shared__ uint2 lds_stack[lds_stack_entries<W>() * PT_TRACE_BLOCK];
     uint2* lds = lds_stack + threadIdx.x;
-    StackEntry spill[(WIDE ? PT_STACK8 : PT_MAX_STACK) - lds_stack_entries<WIDE>()];
+    StackEntry spill[stack_entries<W>() - lds_stack_entries<W>()];
     const int lane = threadIdx.x & 63;
     const unsigned long long lane_lt = (1ull << lane) - 1ull;
     // wave-uniform: this wave's share of rays [pool_next, pool_end) in part pool_part, and
@@ -150,7 +152,7 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
             continue;
         }
         if (item >= 0) {
-            if (WIDE ? trace8_step<STATS, EXT>(b, r, s, lds, spill) : trace_step<STATS, EXT>(b, r, s, lds, spill)) {
+            if (trace_any_width_step<STATS, EXT, W>(b, r, s, lds, spill)) {
                 if (STATS) {
                     nv += s.nv;
                     nt += s.nt;
@@ -172,14 +174,84 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
 
 // The multi-job binary walk is held to 64 VGPRs, so that its 8 LDS-bound blocks per CU
 // are all resident (8 waves per SIMD; unbounded it takes 67 and fits 7, 2.8 % slower on
-// the 64-job batch). The single-job binary walk needs 61 by itself; the 8-wide walk is
-// LDS-bound at 5 blocks per CU.
-template <bool STATS, bool MULTI, bool WIDE, bool EXT>
-__global__ void __launch_bounds__(PT_TRACE_BLOCK, !WIDE ? 8 : 1) k_trace_persistent(BvhView b1, TraceJob job,
-                                                                                  int* cursor,
-                                                                                  unsigned long long* stats)
+// the 64-job batch). The single-job binary walk needs 61 by itself, and the 4-wide walk
+// is held to the same 8 blocks; the 8-wide walk is LDS-bound at 5 blocks per CU.
+template <bool STATS, bool MULTI, int W, bool EXT>
+__global__ void __launch_bounds__(PT_TRACE_BLOCK, W == 8 ? 1 : (W == 4 ? 6 : 8)) k_trace_persistent(BvhView b1, TraceJob job,
+                                                                                    int* cursor,
+                                                                                    unsigned long long* stats)
 {
-    trace_body<STATS, MULTI, WIDE, EXT>(b1, job, cursor, stats);
+    trace_body<STATS, MULTI, W, EXT>(b1, job, cursor, stats);
+}
+
+// Ray records (o, dist_min), (d, dist_max) from structure-of-arrays inputs; a null
+// dist_min / dist_max array reads as 0 / FLT_MAX (AccelBase::Intersect(ray, scene),
+// Core/Accel/AccelBase.h:44-48).
+__global__ void k_pack_rays_soa(int64_t n, const float* __restrict__ ox, const float* __restrict__ oy,
+                                const float* __restrict__ oz, const float* __restrict__ dx,
+                                const float* __restrict__ dy, const float* __restrict__ dz,
+                                const float* __restrict__ dmin, const float* __restrict__ dmax, float4* out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        out[2 * i] = make_float4(ox[i], oy[i], oz[i], dmin ? dmin[i] : 0.0f);
+        out[2 * i + 1] = make_float4(dx[i], dy[i], dz[i], dmax ? dmax[i] : FLT_MAX);
+    }
+}
+
+// Structure-of-arrays hits from a trace that wrote t and the leaf slot (-1 on a miss) per
+// ray: the slot becomes the primitive index, a miss gets t = +inf, and u, v (when asked
+// for) are the hit triangle's barycentric coordinates as Triangle::Intersect forms them
+// (Core/Geometry/Triangle.cpp:57-72; 0 for spheres and misses).
+__global__ void k_hits_soa(int64_t n, BvhView b, const float4* __restrict__ rays, float* t, int32_t* prim, float* u,
+                           float* v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int k = prim[i];
+        float bu = 0.0f, bv = 0.0f;
+        if (k < 0) {
+            t[i] = __builtin_inff();
+            prim[i] = -1;
+        } else {
+            prim[i] = b.prim_ids[k];
+            if (u || v) {
+                if (b.kinds == nullptr || b.kinds[k] == 0) {
+                    const float4 t0 = b.tris[3 * (size_t)k], t1 = b.tris[3 * (size_t)k + 1],
+                                 t2 = b.tris[3 * (size_t)k + 2];
+                    const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+                    const V3 o = v3(r0.x, r0.y, r0.z), d = v3(r1.x, r1.y, r1.z);
+                    const V3 p0 = v3(t0.x, t0.y, t0.z), e1 = v3(t0.w, t1.x, t1.y);
+                    const V3 e2 = v3(t1.z, t1.w, t2.x), nn = v3(t2.y, t2.z, t2.w);
+                    const float inv_det = 1.0f / dot(-d, nn);
+                    const V3 vec = o - p0;
+                    bu = dot(vec, cross(d, e2)) * inv_det;
+                    bv = dot(vec, cross(e1, d)) * inv_det;
+                }
+            }
+        }
+        if (u) u[i] = bu;
+        if (v) v[i] = bv;
+    }
+}
+
+hipError_t launch_pack_rays_soa(int64_t n, const float* ox, const float* oy, const float* oz, const float* dx,
+                                const float* dy, const float* dz, const float* dmin, const float* dmax, float4* out,
+                                hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t want = (n + 255) / 256, cap = (int64_t)cu_count() * 16;
+    hipLaunchKernelGGL(k_pack_rays_soa, dim3((unsigned)(want < cap ? want : cap)), dim3(256), 0, st, n, ox, oy, oz,
+                       dx, dy, dz, dmin, dmax, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_hits_soa(int64_t n, const BvhView& b, const float4* rays, float* t, int32_t* prim, float* u, float* v,
+                           hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t want = (n + 255) / 256, cap = (int64_t)cu_count() * 16;
+    hipLaunchKernelGGL(k_hits_soa, dim3((unsigned)(want < cap ? want : cap)), dim3(256), 0, st, n, b, rays, t, prim,
+                       u, v);
+    return hipGetLastError();
 }
 
 hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* cursor, unsigned long long* stats,
@@ -190,10 +262,10 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
     // a wave refills at most 64 lanes from one chunk, so chunks hold at least 64 rays
     job.chunk = tune.trace_chunk < 64 ? 64 : tune.trace_chunk;
     const bool multi = job.views != nullptr;
-    const bool wide = multi ? job.wide != 0 : b.nodes8 != nullptr;
+    const int width = multi ? job.width : (b.nodes8 != nullptr ? b.width : 2);
     // persistent grid: blocks of 4 waves; the LDS stacks of a block bound the blocks per CU
     // (160 KB of LDS, at most 8 waves per SIMD)
-    const int lds_block = (wide ? lds_stack_entries<true>() : lds_stack_entries<false>()) * PT_TRACE_BLOCK *
+    const int lds_block = (width == 8 ? lds_stack_entries<8>() : lds_stack_entries<2>()) * PT_TRACE_BLOCK *
                           (int)sizeof(uint2);
     const int max_bpc = (160 * 1024) / lds_block < 8 ? (160 * 1024) / lds_block : 8;
     const int bpc = tune.trace_blocks_per_cu < 1 || tune.trace_blocks_per_cu > max_bpc ? max_bpc
@@ -209,23 +281,20 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
         if (ext) PT_LAUNCH_TRACE(S, M, W, true); \
         else PT_LAUNCH_TRACE(S, M, W, false);    \
     } while (0)
+#define PT_LAUNCH_TRACE_W(S, M) \
+    do {                       \
+        if (width == 8) PT_LAUNCH_TRACE_E(S, M, 8);      \
+        else if (width == 4) PT_LAUNCH_TRACE_E(S, M, 4); \
+        else PT_LAUNCH_TRACE_E(S, M, 2);                 \
+    } while (0)
     if (counted) {
-        if (multi) {
-            if (wide) PT_LAUNCH_TRACE_E(true, true, true);
-            else PT_LAUNCH_TRACE_E(true, true, false);
-        } else {
-            if (wide) PT_LAUNCH_TRACE_E(true, false, true);
-            else PT_LAUNCH_TRACE_E(true, false, false);
-        }
+        if (multi) PT_LAUNCH_TRACE_W(true, true);
+        else PT_LAUNCH_TRACE_W(true, false);
     } else {
-        if (multi) {
-            if (wide) PT_LAUNCH_TRACE_E(false, true, true);
-            else PT_LAUNCH_TRACE_E(false, true, false);
-        } else {
-            if (wide) PT_LAUNCH_TRACE_E(false, false, true);
-            else PT_LAUNCH_TRACE_E(false, false, false);
-        }
+        if (multi) PT_LAUNCH_TRACE_W(false, true);
+        else PT_LAUNCH_TRACE_W(false, false);
     }
+#undef PT_LAUNCH_TRACE_W
 #undef PT_LAUNCH_TRACE_E
 #undef PT_LAUNCH_TRACE
     return hipGetLastError();

@@ -11,8 +11,8 @@
 // hit replaces the best one only when strictly nearer. The reference pushes children
 // whose box the ray misses and rejects them when popped; they are not pushed here.
 //
-// The 8-wide traversal (BVHs built with width 8, nodes in pt_wide.hip) tests the up to 8
-// child boxes of a node against [max(dist_min, 0), min(nearest hit, dist_max)], visits
+// The wide traversal (BVHs built with width 4 or 8, nodes in pt_wide.hip) tests the up
+// to 4 or 8 child boxes of a node against [max(dist_min, 0), min(nearest hit, dist_max)], visits
 // the nearest hit child next and pushes the others farthest first with their entry
 // distance; a popped entry is skipped when the nearest hit is closer than its entry
 // distance. Leaves, triangle tests and the hit rule are the reference's, so it finds the
@@ -20,7 +20,7 @@
 // the reference's walk reaches first (reference_first); only rays that graze a box
 // within rounding of their hit can differ.
 //
-// A step handles one inner node or one triangle (8-wide: one node or one whole leaf).
+// A step handles one inner node or one triangle (wide: one node or one whole leaf).
 // Each lane keeps its traversal stack in LDS (the first lds_stack_entries entries) with
 // deeper entries in private memory.
 //
@@ -40,16 +40,21 @@
 
 namespace pt {
 
-// LDS stack entries per lane. Binary: 10 (20 KB per 256-thread block, 8 blocks or 32
-// waves per CU). 8-wide: 16 (32 KB, 5 blocks per CU, which its registers allow anyway),
-// as its stacks grow by up to 7 entries per node.
-template <bool WIDE>
+// LDS stack entries per lane, by node width W. Binary and 4-wide: 10 (20 KB per
+// 256-thread block, 8 blocks or 32 waves per CU). 8-wide: 16 (32 KB, 5 blocks per CU,
+// which its registers allow anyway), as its stacks grow by up to 7 entries per node.
+template <int W>
 constexpr int lds_stack_entries()
 {
-    return WIDE ? 16 : 10;
+    return W == 8 ? 16 : 10;
 }
 #define PT_TRACE_BLOCK 256
-#define PT_STACK8 192 // 8-wide traversal: at most 7 pending children per level
+// Whole traversal stack per lane (LDS entries first, the rest in private memory).
+template <int W>
+constexpr int stack_entries()
+{
+    return W == 8 ? PT_STACK8 : (W == 4 ? PT_STACK4 : PT_MAX_STACK);
+}
 
 struct StackEntry {
     uint32_t ref; // PT_REF_* word with the inside bit
@@ -74,10 +79,10 @@ struct TraceState {
     uint32_t leaf_k, leaf_end; // triangles of the current leaf still to test
 };
 
-template <bool WIDE>
+template <int W>
 __device__ __forceinline__ void stack_push(uint2* lds, StackEntry* spill, int sp, uint32_t ref, float d)
 {
-    constexpr int NL = lds_stack_entries<WIDE>();
+    constexpr int NL = lds_stack_entries<W>();
     if (sp < NL) {
         lds[sp * PT_TRACE_BLOCK] = make_uint2(ref, __float_as_uint(d));
     } else {
@@ -86,11 +91,11 @@ __device__ __forceinline__ void stack_push(uint2* lds, StackEntry* spill, int sp
     }
 }
 
-template <bool WIDE>
+template <int W>
 __device__ __forceinline__ void stack_pop(const uint2* lds, const StackEntry* spill, int sp, uint32_t* ref,
                                           float* d)
 {
-    constexpr int NL = lds_stack_entries<WIDE>();
+    constexpr int NL = lds_stack_entries<W>();
     if (sp < NL) {
         const uint2 e = lds[sp * PT_TRACE_BLOCK];
         *ref = e.x;
@@ -172,7 +177,7 @@ __device__ __noinline__ bool reference_first(const BvhView& b, const TraceRay& r
 // Tests the primitive of leaf slot k, a triangle or a sphere (the reference's hit rule);
 // true when an any-hit ray has found its hit. The 8-wide traversal meets leaves out of the reference's order, so an exactly
 // equal distance goes to the triangle the reference would have reached first.
-template <bool STATS, bool WIDE, bool EXT>
+template <bool STATS, int W, bool EXT>
 __device__ __forceinline__ bool tri_test(const BvhView& b, const TraceRay& r, TraceState& s, uint32_t k)
 {
     if (STATS) ++s.nt;
@@ -197,7 +202,7 @@ __device__ __forceinline__ bool tri_test(const BvhView& b, const TraceRay& r, Tr
             s.best = (int)k;
             s.best_t = t;
             if (r.any) return true;
-        } else if (WIDE && t == s.best_t && reference_first(b, r, (int)k, s.best)) {
+        } else if (W != 2 && t == s.best_t && reference_first(b, r, (int)k, s.best)) {
             s.best = (int)k;
         }
     }
@@ -258,25 +263,25 @@ __device__ __forceinline__ bool deep_next(const BvhView& b, const TraceRay& r, T
     return false;
 }
 
-template <bool STATS, bool WIDE, bool EXT>
+template <bool STATS, int W, bool EXT>
 __device__ __forceinline__ bool leaf_or_pop(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
                                             StackEntry* spill, bool next)
 {
     if (s.leaf_k < s.leaf_end) {
-        if (WIDE) { // a wide node's work per step is larger: whole leaves keep steps alike
+        if (W != 2) { // a wide node's work per step is larger: whole leaves keep steps alike
             for (; s.leaf_k < s.leaf_end; ++s.leaf_k)
-                if (tri_test<STATS, WIDE, EXT>(b, r, s, s.leaf_k)) return true;
+                if (tri_test<STATS, W, EXT>(b, r, s, s.leaf_k)) return true;
         } else {
-            if (tri_test<STATS, WIDE, EXT>(b, r, s, s.leaf_k++)) return true;
+            if (tri_test<STATS, W, EXT>(b, r, s, s.leaf_k++)) return true;
             if (s.leaf_k < s.leaf_end) return false;
         }
         next = false;
     }
     if (!next) {
-        if (!WIDE && EXT && b.parents) return !deep_next(b, r, s, lds);
+        if (W == 2 && EXT && b.parents) return !deep_next(b, r, s, lds);
         if (s.sp == 0) return true;
         --s.sp;
-        stack_pop<WIDE>(lds, spill, s.sp, &s.cur, &s.cur_d);
+        stack_pop<W>(lds, spill, s.sp, &s.cur, &s.cur_d);
     }
     return false;
 }
@@ -324,7 +329,7 @@ __device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, 
             }
         } else if (h1) {
             if (h2) {
-                stack_push<false>(lds, spill, s.sp, e2, d2);
+                stack_push<2>(lds, spill, s.sp, e2, d2);
                 ++s.sp;
             }
             s.cur = e1;
@@ -336,7 +341,7 @@ __device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, 
             next = true;
         }
     }
-    return leaf_or_pop<STATS, false, EXT>(b, r, s, lds, spill, next);
+    return leaf_or_pop<STATS, 2, EXT>(b, r, s, lds, spill, next);
 }
 
 __device__ __forceinline__ void sort_pair(float& da, uint32_t& ra, float& db, uint32_t& rb)
@@ -352,12 +357,14 @@ __device__ __forceinline__ void sort_pair(float& da, uint32_t& ra, float& db, ui
 
 __device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); }
 
-// One step of the 8-wide traversal: the current entry (a wide node or a leaf), then the
-// next entry from the stack. Returns true when the ray is finished.
-template <bool STATS, bool EXT>
-__device__ __forceinline__ bool trace8_step(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
-                                            StackEntry* spill)
+// One step of the wide traversal (W = 4 or 8 children per node): the current entry (a
+// wide node or a leaf), then the next entry from the stack. Returns true when the ray is
+// finished.
+template <bool STATS, bool EXT, int W>
+__device__ __forceinline__ bool trace_wide_step(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
+                                                StackEntry* spill)
 {
+    static_assert(W == 4 || W == 8, "wide nodes hold 4 or 8 children");
     bool next = false;
     if (s.leaf_k < s.leaf_end) {
         // in the middle of a leaf
@@ -366,8 +373,40 @@ __device__ __forceinline__ bool trace8_step(const BvhView& b, const TraceRay& r,
     } else if (pt_ref_cnt(s.cur) != 0) {
         enter_leaf(b, s);
     } else {
-        const uint4* nd = b.nodes8 + 8 * (size_t)pt_ref_index(s.cur);
-        const uint4 w0 = nd[0], w1 = nd[1], w2 = nd[2], w3 = nd[3], w4 = nd[4], w5 = nd[5];
+        // quantised near / far plane bytes by axis (q[0..2] the near planes, q[3..5] the
+        // far ones for this ray's octant) and the child references, per 4 children
+        uint32_t nq[3][W / 4], fq[3][W / 4], ref[W];
+        uint4 w0;
+        if constexpr (W == 8) {
+            const uint4* nd = b.nodes8 + 8 * (size_t)pt_ref_index(s.cur);
+            w0 = nd[0];
+            const uint4 w1 = nd[1], w2 = nd[2], w3 = nd[3], w4 = nd[4], w5 = nd[5];
+            // qlo: w1.xy, w1.zw, w2.xy; qhi: w2.zw, w3.xy, w3.zw
+            const uint32_t lo[3][2] = {{w1.x, w1.y}, {w1.z, w1.w}, {w2.x, w2.y}};
+            const uint32_t hi[3][2] = {{w2.z, w2.w}, {w3.x, w3.y}, {w3.z, w3.w}};
+            const int sg[3] = {r.sx, r.sy, r.sz};
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    nq[a][h] = sg[a] ? hi[a][h] : lo[a][h];
+                    fq[a][h] = sg[a] ? lo[a][h] : hi[a][h];
+                }
+            ref[0] = w4.x; ref[1] = w4.y; ref[2] = w4.z; ref[3] = w4.w;
+            ref[4] = w5.x; ref[5] = w5.y; ref[6] = w5.z; ref[7] = w5.w;
+        } else {
+            const uint4* nd = b.nodes8 + 4 * (size_t)pt_ref_index(s.cur);
+            w0 = nd[0];
+            const uint4 w1 = nd[1], w2 = nd[2], w3 = nd[3];
+            // qlo: w1.x, w1.y, w1.z; qhi: w1.w, w2.x, w2.y; references w2.zw, w3.xy
+            nq[0][0] = r.sx ? w1.w : w1.x;
+            fq[0][0] = r.sx ? w1.x : w1.w;
+            nq[1][0] = r.sy ? w2.x : w1.y;
+            fq[1][0] = r.sy ? w1.y : w2.x;
+            nq[2][0] = r.sz ? w2.y : w1.z;
+            fq[2][0] = r.sz ? w1.z : w2.y;
+            ref[0] = w2.z; ref[1] = w2.w; ref[2] = w3.x; ref[3] = w3.y;
+        }
         const int cnt = (int)(w0.w >> 24);
         // the ray in the node's quantised frame: t = q * (inv * step) + (p - o) * inv
         const float ix = r.inv.x * __uint_as_float((w0.w & 0xffu) << 23);
@@ -376,57 +415,58 @@ __device__ __forceinline__ bool trace8_step(const BvhView& b, const TraceRay& r,
         const float ox = (__uint_as_float(w0.x) - r.o.x) * r.inv.x;
         const float oy = (__uint_as_float(w0.y) - r.o.y) * r.inv.y;
         const float oz = (__uint_as_float(w0.z) - r.o.z) * r.inv.z;
-        // entry and exit planes by ray octant (qlo: w1.xy, w1.zw, w2.xy; qhi: w2.zw, w3.xy, w3.zw)
-        const uint32_t nx0 = r.sx ? w2.z : w1.x, nx1 = r.sx ? w2.w : w1.y;
-        const uint32_t fx0 = r.sx ? w1.x : w2.z, fx1 = r.sx ? w1.y : w2.w;
-        const uint32_t ny0 = r.sy ? w3.x : w1.z, ny1 = r.sy ? w3.y : w1.w;
-        const uint32_t fy0 = r.sy ? w1.z : w3.x, fy1 = r.sy ? w1.w : w3.y;
-        const uint32_t nz0 = r.sz ? w3.z : w2.x, nz1 = r.sz ? w3.w : w2.y;
-        const uint32_t fz0 = r.sz ? w2.x : w3.z, fz1 = r.sz ? w2.y : w3.w;
         const float tfar = s.best >= 0 ? fminf(s.best_t, r.dmax) : r.dmax;
-        float d[8];
-        uint32_t ref[8] = {w4.x, w4.y, w4.z, w4.w, w5.x, w5.y, w5.z, w5.w};
+        float d[W];
         int hits = 0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = i & 3;
-            const float tx0 = fmaf(byte_f(i < 4 ? nx0 : nx1, k), ix, ox);
-            const float ty0 = fmaf(byte_f(i < 4 ? ny0 : ny1, k), iy, oy);
-            const float tz0 = fmaf(byte_f(i < 4 ? nz0 : nz1, k), iz, oz);
-            const float tx1 = fmaf(byte_f(i < 4 ? fx0 : fx1, k), ix, ox);
-            const float ty1 = fmaf(byte_f(i < 4 ? fy0 : fy1, k), iy, oy);
-            const float tz1 = fmaf(byte_f(i < 4 ? fz0 : fz1, k), iz, oz);
+        for (int i = 0; i < W; ++i) {
+            const int h = i >> 2, k = i & 3;
+            const float tx0 = fmaf(byte_f(nq[0][h], k), ix, ox);
+            const float ty0 = fmaf(byte_f(nq[1][h], k), iy, oy);
+            const float tz0 = fmaf(byte_f(nq[2][h], k), iz, oz);
+            const float tx1 = fmaf(byte_f(fq[0][h], k), ix, ox);
+            const float ty1 = fmaf(byte_f(fq[1][h], k), iy, oy);
+            const float tz1 = fmaf(byte_f(fq[2][h], k), iz, oz);
             const float tmin = fmaxf(fmaxf(tx0, ty0), fmaxf(tz0, r.tnear));
             const float tmax = fminf(fminf(tx1, ty1), fminf(tz1, tfar));
             const bool hit = i < cnt && tmin <= tmax;
             d[i] = hit ? tmin : __builtin_inff();
             hits += hit ? 1 : 0;
         }
-        if (STATS) s.nv += 1; // one 128-byte node record
-        // nearest first: 19-comparator sorting network for 8 keys
-        sort_pair(d[0], ref[0], d[2], ref[2]);
-        sort_pair(d[1], ref[1], d[3], ref[3]);
-        sort_pair(d[4], ref[4], d[6], ref[6]);
-        sort_pair(d[5], ref[5], d[7], ref[7]);
-        sort_pair(d[0], ref[0], d[4], ref[4]);
-        sort_pair(d[1], ref[1], d[5], ref[5]);
-        sort_pair(d[2], ref[2], d[6], ref[6]);
-        sort_pair(d[3], ref[3], d[7], ref[7]);
-        sort_pair(d[0], ref[0], d[1], ref[1]);
-        sort_pair(d[2], ref[2], d[3], ref[3]);
-        sort_pair(d[4], ref[4], d[5], ref[5]);
-        sort_pair(d[6], ref[6], d[7], ref[7]);
-        sort_pair(d[2], ref[2], d[4], ref[4]);
-        sort_pair(d[3], ref[3], d[5], ref[5]);
-        sort_pair(d[1], ref[1], d[4], ref[4]);
-        sort_pair(d[3], ref[3], d[6], ref[6]);
-        sort_pair(d[1], ref[1], d[2], ref[2]);
-        sort_pair(d[3], ref[3], d[4], ref[4]);
-        sort_pair(d[5], ref[5], d[6], ref[6]);
+        if (STATS) s.nv += 1; // one node record (128 B at width 8, 64 B at width 4)
+        if constexpr (W == 8) {
+            // nearest first: 19-comparator sorting network for 8 keys
+            sort_pair(d[0], ref[0], d[2], ref[2]);
+            sort_pair(d[1], ref[1], d[3], ref[3]);
+            sort_pair(d[4], ref[4], d[6], ref[6]);
+            sort_pair(d[5], ref[5], d[7], ref[7]);
+            sort_pair(d[0], ref[0], d[4], ref[4]);
+            sort_pair(d[1], ref[1], d[5], ref[5]);
+            sort_pair(d[2], ref[2], d[6], ref[6]);
+            sort_pair(d[3], ref[3], d[7], ref[7]);
+            sort_pair(d[0], ref[0], d[1], ref[1]);
+            sort_pair(d[2], ref[2], d[3], ref[3]);
+            sort_pair(d[4], ref[4], d[5], ref[5]);
+            sort_pair(d[6], ref[6], d[7], ref[7]);
+            sort_pair(d[2], ref[2], d[4], ref[4]);
+            sort_pair(d[3], ref[3], d[5], ref[5]);
+            sort_pair(d[1], ref[1], d[4], ref[4]);
+            sort_pair(d[3], ref[3], d[6], ref[6]);
+            sort_pair(d[1], ref[1], d[2], ref[2]);
+            sort_pair(d[3], ref[3], d[4], ref[4]);
+            sort_pair(d[5], ref[5], d[6], ref[6]);
+        } else {
+            // 5-comparator sorting network for 4 keys
+            sort_pair(d[0], ref[0], d[1], ref[1]);
+            sort_pair(d[2], ref[2], d[3], ref[3]);
+            sort_pair(d[0], ref[0], d[2], ref[2]);
+            sort_pair(d[1], ref[1], d[3], ref[3]);
+            sort_pair(d[1], ref[1], d[2], ref[2]);
+        }
 #pragma unroll
-        for (int j = 7; j >= 1; --j) {
+        for (int j = W - 1; j >= 1; --j) {
             if (j < hits) {
-                stack_push<true>(lds, spill, s.sp, ref[j], d[j]);
+                stack_push<W>(lds, spill, s.sp, ref[j], d[j]);
                 ++s.sp;
             }
         }
@@ -436,7 +476,16 @@ __device__ __forceinline__ bool trace8_step(const BvhView& b, const TraceRay& r,
             next = true;
         }
     }
-    return leaf_or_pop<STATS, true, EXT>(b, r, s, lds, spill, next);
+    return leaf_or_pop<STATS, W, EXT>(b, r, s, lds, spill, next);
+}
+
+// One step of the walk of node width W (2: the reference's binary walk).
+template <bool STATS, bool EXT, int W>
+__device__ __forceinline__ bool trace_any_width_step(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
+                                                     StackEntry* spill)
+{
+    if constexpr (W == 2) return trace_step<STATS, EXT>(b, r, s, lds, spill);
+    else return trace_wide_step<STATS, EXT, W>(b, r, s, lds, spill);
 }
 
 __device__ __forceinline__ void trace_finish(const BvhView& b, const TraceSegment& seg, int i, const TraceState& s)

@@ -1,7 +1,7 @@
-// pt_wide.hip — the 8-wide BVH: collapse of the binary tree into 8-ary nodes with
-// quantised child boxes, built on the device.
+// pt_wide.hip — the 4- and 8-wide BVHs: collapse of the binary tree into 4- or 8-ary
+// nodes with quantised child boxes, built on the device.
 //
-// Node record (128 B, one cache line; 8 x uint4):
+// 8-wide node record (128 B, one cache line; 8 x uint4):
 //   w0  origin p (3 x f32) | exponents ex, ey, ez (biased by 127) and child count (4 x u8)
 //   w1  near-plane bytes qlo.x[0..7], qlo.y[0..7]
 //   w2  qlo.z[0..7], qhi.x[0..7]
@@ -9,12 +9,17 @@
 //   w4  child references 0..3, w5 child references 4..7 (PT_REF_* encoding: an inner
 //       child is a wide-node index, a leaf child is the binary tree's leaf reference)
 //   w6, w7 unused
+// 4-wide node record (64 B; 4 x uint4):
+//   w0  as above
+//   w1  qlo.x[0..3], qlo.y[0..3], qlo.z[0..3], qhi.x[0..3]
+//   w2  qhi.y[0..3], qhi.z[0..3], child references 0, 1
+//   w3  child references 2, 3, unused
 // Child box i is [p + qlo_i * s, p + qhi_i * s] with s = 2^(e - 127) per axis, rounded
 // outwards so that it contains the exact box.
 //
 // Collapse: every wide node starts from one inner node of the binary tree and keeps
-// opening the inner candidate with the largest surface area until it holds 8 children
-// or only leaves (Ylitie, Karras and Laine, "Efficient Incoherent Ray Traversal on
+// opening the inner candidate with the largest surface area until it holds 4 or 8
+// children or only leaves (Ylitie, Karras and Laine, "Efficient Incoherent Ray Traversal on
 // GPUs Through Compressed Wide BVHs", HPG 2017, collapse by greedy expansion). Wide
 // nodes are numbered level by level; the leaves and their triangle order are the
 // binary tree's.
@@ -56,16 +61,17 @@ __device__ __forceinline__ void read_children(const float4* bnodes, uint32_t idx
     r.ref = __float_as_uint(m.y);
 }
 
-// Pass A: greedy expansion of every wide node of the level into at most 8 candidates.
+// Pass A: greedy expansion of every wide node of the level into at most W candidates.
+template <int W>
 __global__ void k_collapse_expand(const float4* bnodes, const uint32_t* in_ref, int n, float4* cand, int* n_cand,
                                   int* n_inner)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= n) return;
-    Cand c[8];
+    Cand c[W];
     read_children(bnodes, pt_ref_index(in_ref[w]), c[0], c[1]);
     int cnt = 2;
-    while (cnt < 8) {
+    while (cnt < W) {
         int best = -1;
         float best_a = -1.0f;
         for (int i = 0; i < cnt; ++i) {
@@ -125,6 +131,7 @@ __device__ __forceinline__ uint32_t quant_hi(float p, float s, float v)
 
 // Pass B: node records, indices of the inner children (the next level, in order) and
 // the next level's input.
+template <int W>
 __global__ void k_collapse_write(const float4* cand, const int* n_cand, const int* inner_base, int n, int node_base,
                                  int next_base, uint4* out, uint32_t* next_ref)
 {
@@ -146,7 +153,7 @@ __global__ void k_collapse_write(const float4* cand, const int* n_cand, const in
     uint32_t qlo[3][2] = {{0, 0}, {0, 0}, {0, 0}}, qhi[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     uint32_t refs[8];
     int next = inner_base[w];
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < W; ++i) {
         uint32_t bl[3] = {255, 255, 255}, bh[3] = {0, 0, 0};
         uint32_t ref = PT_REF_EMPTY;
         if (i < cnt) {
@@ -169,17 +176,26 @@ __global__ void k_collapse_write(const float4* cand, const int* n_cand, const in
         }
         refs[i] = ref;
     }
-    uint4* o = out + 8 * (size_t)(node_base + w);
-    o[0] = make_uint4(__float_as_uint(lo[0]), __float_as_uint(lo[1]), __float_as_uint(lo[2]),
-                      (uint32_t)(e[0] + 127) | ((uint32_t)(e[1] + 127) << 8) | ((uint32_t)(e[2] + 127) << 16) |
-                          ((uint32_t)cnt << 24));
-    o[1] = make_uint4(qlo[0][0], qlo[0][1], qlo[1][0], qlo[1][1]);
-    o[2] = make_uint4(qlo[2][0], qlo[2][1], qhi[0][0], qhi[0][1]);
-    o[3] = make_uint4(qhi[1][0], qhi[1][1], qhi[2][0], qhi[2][1]);
-    o[4] = make_uint4(refs[0], refs[1], refs[2], refs[3]);
-    o[5] = make_uint4(refs[4], refs[5], refs[6], refs[7]);
-    o[6] = make_uint4(0, 0, 0, 0);
-    o[7] = make_uint4(0, 0, 0, 0);
+    const uint4 w0 = make_uint4(__float_as_uint(lo[0]), __float_as_uint(lo[1]), __float_as_uint(lo[2]),
+                                (uint32_t)(e[0] + 127) | ((uint32_t)(e[1] + 127) << 8) |
+                                    ((uint32_t)(e[2] + 127) << 16) | ((uint32_t)cnt << 24));
+    if (W == 8) {
+        uint4* o = out + 8 * (size_t)(node_base + w);
+        o[0] = w0;
+        o[1] = make_uint4(qlo[0][0], qlo[0][1], qlo[1][0], qlo[1][1]);
+        o[2] = make_uint4(qlo[2][0], qlo[2][1], qhi[0][0], qhi[0][1]);
+        o[3] = make_uint4(qhi[1][0], qhi[1][1], qhi[2][0], qhi[2][1]);
+        o[4] = make_uint4(refs[0], refs[1], refs[2], refs[3]);
+        o[5] = make_uint4(refs[4], refs[5], refs[6], refs[7]);
+        o[6] = make_uint4(0, 0, 0, 0);
+        o[7] = make_uint4(0, 0, 0, 0);
+    } else {
+        uint4* o = out + 4 * (size_t)(node_base + w);
+        o[0] = w0;
+        o[1] = make_uint4(qlo[0][0], qlo[1][0], qlo[2][0], qhi[0][0]);
+        o[2] = make_uint4(qhi[1][0], qhi[2][0], refs[0], refs[1]);
+        o[3] = make_uint4(refs[2], refs[3], 0, 0);
+    }
 }
 
 inline unsigned blocks(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -195,14 +211,17 @@ inline unsigned blocks(size_t n, int b = 256) { return (unsigned)((n + b - 1) / 
 
 } // namespace
 
-hipError_t gpu_collapse_bvh8(const float4* bnodes_d, int num_inner, uint32_t root_ref, hipStream_t st, Wide8* out)
+hipError_t gpu_collapse_wide(int width, const float4* bnodes_d, int num_inner, uint32_t root_ref, hipStream_t st,
+                             Wide8* out)
 {
     *out = Wide8();
+    if (width != 4 && width != 8) return hipErrorInvalidValue;
+    const int words = width == 8 ? 8 : 4; // uint4 per node record
     if (pt_ref_cnt(root_ref) != 0 || root_ref == PT_REF_EMPTY || num_inner <= 0) {
         // a leaf (or empty) root: the traversal starts at the leaf; one unused node keeps
         // the layout non-empty
-        hipError_t e = hipMalloc((void**)&out->nodes_d, 8 * sizeof(uint4));
-        if (e == hipSuccess) e = hipMemsetAsync(out->nodes_d, 0, 8 * sizeof(uint4), st);
+        hipError_t e = hipMalloc((void**)&out->nodes_d, words * sizeof(uint4));
+        if (e == hipSuccess) e = hipMemsetAsync(out->nodes_d, 0, words * sizeof(uint4), st);
         if (e != hipSuccess) return e;
         out->root = root_ref;
         return hipStreamSynchronize(st);
@@ -231,20 +250,28 @@ hipError_t gpu_collapse_bvh8(const float4* bnodes_d, int num_inner, uint32_t roo
     W_CHECK(hipMalloc(&scan_tmp, scan_bytes));
     tmp.push_back(scan_tmp);
     uint4* nodes;
-    W_CHECK(hipMalloc((void**)&nodes, 8 * cap * sizeof(uint4)));
+    W_CHECK(hipMalloc((void**)&nodes, words * cap * sizeof(uint4)));
     tmp.push_back(nodes); // freed with the rest on failure
     W_CHECK(hipMemcpyAsync(refs[0], &root_ref, sizeof(uint32_t), hipMemcpyHostToDevice, st));
     int level_n = 1, total = 0, depth = 0, cur = 0;
     while (level_n > 0) {
-        hipLaunchKernelGGL(k_collapse_expand, dim3(blocks(level_n)), dim3(256), 0, st, bnodes_d, refs[cur], level_n,
-                           cand, n_cand, n_inner);
+        if (width == 8)
+            hipLaunchKernelGGL(k_collapse_expand<8>, dim3(blocks(level_n)), dim3(256), 0, st, bnodes_d, refs[cur],
+                               level_n, cand, n_cand, n_inner);
+        else
+            hipLaunchKernelGGL(k_collapse_expand<4>, dim3(blocks(level_n)), dim3(256), 0, st, bnodes_d, refs[cur],
+                               level_n, cand, n_cand, n_inner);
         const int zero = 0;
         W_CHECK(hipMemcpyAsync(n_inner + level_n, &zero, sizeof(int), hipMemcpyHostToDevice, st));
         W_CHECK(exclusive_sum(scan_tmp, scan_bytes, n_inner, base, (size_t)(level_n + 1), st));
         int next_n = 0;
         W_CHECK(hipMemcpyAsync(&next_n, base + level_n, sizeof(int), hipMemcpyDeviceToHost, st));
-        hipLaunchKernelGGL(k_collapse_write, dim3(blocks(level_n)), dim3(256), 0, st, cand, n_cand, base, level_n,
-                           total, total + level_n, nodes, refs[1 - cur]);
+        if (width == 8)
+            hipLaunchKernelGGL(k_collapse_write<8>, dim3(blocks(level_n)), dim3(256), 0, st, cand, n_cand, base,
+                               level_n, total, total + level_n, nodes, refs[1 - cur]);
+        else
+            hipLaunchKernelGGL(k_collapse_write<4>, dim3(blocks(level_n)), dim3(256), 0, st, cand, n_cand, base,
+                               level_n, total, total + level_n, nodes, refs[1 - cur]);
         W_CHECK(hipGetLastError());
         W_CHECK(hipStreamSynchronize(st));
         total += level_n;

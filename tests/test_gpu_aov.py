@@ -37,7 +37,7 @@ CASES = {"cornell": scenes.cornell_box, "specular_spheres": specular_spheres}
 
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("aov", [P.AOV_DENOISING_ALBEDO, P.AOV_DENOISING_NORMAL])
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_aov_matches_oracle(gpu, name, aov, width):
     sc = CASES[name]()
     b = ctx().scene(sc).build_bvh(16, width=width)
@@ -57,7 +57,7 @@ def test_aov_matches_oracle(gpu, name, aov, width):
         assert exact.mean() > 0.99
 
 
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_uv_aov_matches_oracle(gpu, width):
     # texture coordinates per corner (some outside [0, 1]); the UV AOV averages the first
     # hit's interpolated (u, v, 0) over the camera rays

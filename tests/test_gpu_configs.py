@@ -6,7 +6,7 @@ C2  the single large scene (3,145,728 triangles, 2048 x 1024 sky), 1920 x 1080, 
     depth 12: the full frame on the GPU, scattered crops (the sun's among them) against the
     oracle on the GPU's tree and on the oracle's own tree (std::sort leaf order), which the
     product's reference-order build reproduces exactly.
-C3  10M random triangles, 32 bins, binary and 8-wide trees, 2^26 rays (half pinhole, half
+C3  10M random triangles, 32 bins, binary, 4- and 8-wide trees, 2^26 rays (half pinhole, half
     surface rays): closest and any hit on every ray, a 1M-ray sample against the oracle.
 C4  the 64-job batch: every batched image equals its per-job render, and the oracle on a
     pixel sample of every job.
@@ -117,7 +117,7 @@ def c3_scene():
     return scenes.random_triangles(10_000_000, seed=3, mu=-6.0, sigma=1.0)
 
 
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_c3_microbenchmark_10M(gpu, width):
     import torch
     sc = c3_scene()

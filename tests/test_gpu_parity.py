@@ -112,3 +112,75 @@ def test_render_is_deterministic_and_seeded(gpu):
     d, _ = b.render(32, 32, 4, 8, 0, 6)
     assert bits_equal(a, a2)
     assert not bits_equal(a, d)
+
+
+def barycentrics_f32(tri_pos, prim, r):
+    """Triangle::Intersect's u and v (Core/Geometry/Triangle.cpp:57-72) in float32, each
+    product and sum rounded in the reference's order: e1 = p1 - p0, e2 = p2 - p0,
+    n = e1 x e2, inv_det = 1 / dot(-d, n), u = dot(o - p0, d x e2) inv_det,
+    v = dot(o - p0, e1 x d) inv_det, with dot = (x x' + y y') + z z'."""
+    f = np.float32
+    t = tri_pos.reshape(-1, 3, 3)[prim].astype(f)
+    p0, e1, e2 = t[:, 0], t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]
+
+    def cross(a, b):
+        return np.stack([a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1], a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2],
+                         a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], 1)
+
+    def dot(a, b):
+        return (a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2]
+
+    o, d = r[:, 0:3].astype(f), r[:, 4:7].astype(f)
+    n = cross(e1, e2)
+    inv_det = f(1.0) / dot(-d, n)
+    vec = o - p0
+    return dot(vec, cross(d, e2)) * inv_det, dot(vec, cross(e1, d)) * inv_det
+
+
+@pytest.mark.parametrize("name", ["cornell", "random"])
+def test_soa_entry_points_equal_aos(gpu, name):
+    """pt_trace_closest_soa / pt_trace_any_soa (rays and hits as separate arrays) give the
+    hits of pt_trace_closest / pt_trace_any, and u, v as Triangle::Intersect forms them:
+    bit for bit against a float32 evaluation in its order, and the hit point
+    p0 + u e1 + v e2 equals o + t d to rounding. Null distance arrays mean [0, FLT_MAX];
+    a launch over 2^20 rays takes the coherence-ordered path."""
+    sc = SCENES[name]()
+    b = ctx().scene(sc).build_bvh(16)
+    r = rays_into(sc, 20000, 5)
+    r[::7, 3] = 0.25  # distance windows
+    r[::5, 7] = 2.0
+    rd = to_dev(r)
+    t, p = b.trace_closest(rd)
+    ts, ps, u, v = b.trace_closest_soa(rd)
+    t, p, ts, ps, u, v = (x.cpu().numpy() for x in (t, p, ts, ps, u, v))
+    assert np.array_equal(ps, p) and bits_equal(ts, t)
+    hit = p >= 0
+    assert hit.sum() > 1000 and np.all(np.isinf(ts[~hit]))
+    assert np.all(u[~hit] == 0) and np.all(v[~hit] == 0)
+    ue, ve = barycentrics_f32(sc["tri_pos"], p[hit], r[hit])
+    assert bits_equal(u[hit], ue) and bits_equal(v[hit], ve)
+    assert np.all((u[hit] >= 0) & (v[hit] >= 0) & (u[hit] + v[hit] < 1))
+    tri = sc["tri_pos"].reshape(-1, 3, 3)[p[hit]].astype(np.float64)
+    q1 = tri[:, 0] + u[hit, None] * (tri[:, 1] - tri[:, 0]) + v[hit, None] * (tri[:, 2] - tri[:, 0])
+    q2 = r[hit, 0:3].astype(np.float64) + ts[hit, None] * r[hit, 4:7].astype(np.float64)
+    scale = np.abs(tri).max() + 1.0
+    assert np.abs(q1 - q2).max() <= 1e-4 * scale
+    a = b.trace_any(rd).cpu().numpy()
+    assert np.array_equal(b.trace_any_soa(rd).cpu().numpy(), a)
+    # null dist_min / dist_max: the default window [0, FLT_MAX]
+    r0 = r.copy()
+    r0[:, 3], r0[:, 7] = 0.0, np.finfo(np.float32).max
+    t0, p0 = b.trace_closest(to_dev(r0))
+    t1, p1, _, _ = b.trace_closest_soa(rd, uv=False, default_window=True)
+    assert np.array_equal(p1.cpu().numpy(), p0.cpu().numpy()) and bits_equal(t1.cpu().numpy(), t0.cpu().numpy())
+    if name == "random":
+        big = np.concatenate([r] * 53)[: (1 << 20) + 5]
+        bd = to_dev(big)
+        tb, pb = b.trace_closest(bd)
+        tbs, pbs, _, _ = b.trace_closest_soa(bd, uv=False)
+        assert torch_equal(pbs, pb) and bits_equal(tbs.cpu().numpy(), tb.cpu().numpy())
+
+
+def torch_equal(a, b):
+    import torch
+    return bool(torch.equal(a, b))

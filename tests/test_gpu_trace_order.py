@@ -29,7 +29,7 @@ def surface_rays(sc, n, seed):
     return r
 
 
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_large_launch_equals_small_launches_and_oracle(gpu, width):
     import torch
     sc = scenes.random_triangles(200000, seed=5, mu=-5.0)

@@ -1,6 +1,6 @@
-"""8-wide BVH (width 8): nearest hits and renders against the oracle.
+"""Wide BVHs (widths 4 and 8): nearest hits and renders against the oracle.
 
-The 8-wide traversal visits boxes nearest first and culls by the nearest hit so far,
+The wide traversal visits boxes nearest first and culls by the nearest hit so far,
 not in the reference's order (BinnedSAHBVH::Intersect, Core/Accel/BinnedSAHBVH.cpp:
 272-374); leaves, triangle tests and the strictly-nearer rule are the reference's. The
 nearest hit distance is therefore the reference's for every ray that does not graze a
@@ -29,15 +29,16 @@ SCENES = {
 }
 
 
+@pytest.mark.parametrize("width", [4, 8])
 @pytest.mark.parametrize("name", list(SCENES))
 @pytest.mark.parametrize("builder", [P.BUILD_DEVICE, P.BUILD_HOST])
-def test_wide_nearest_hits_match_oracle(gpu, name, builder):
-    log("8-wide trace", name, builder)
+def test_wide_nearest_hits_match_oracle(gpu, name, builder, width):
+    log("%d-wide trace" % width, name, builder)
     sc = SCENES[name]()
     c = ctx()
     s = c.scene(sc)
     b2 = s.build_bvh(16, builder=builder)
-    b8 = s.build_bvh(16, width=8, builder=builder)
+    b8 = s.build_bvh(16, width=width, builder=builder)
     o = O.OracleScene(sc).use_tree(b2.export_nodes() | {"prim_ids": b2.export_order()})
     r = np.concatenate([rays_into(sc, 20000, 5), scenes.surface_rays(sc["tri_pos"], 5000, seed=6)])
     r[20000:, 3] = 1e-5  # surface rays leave their own triangle
@@ -60,15 +61,16 @@ def test_wide_nearest_hits_match_oracle(gpu, name, builder):
     assert np.array_equal(hit8, hits.astype(np.int32))
 
 
+@pytest.mark.parametrize("width", [4, 8])
 @pytest.mark.parametrize("integrator", [0, 1])
 @pytest.mark.parametrize("name", ["cornell", "spheres"])
-def test_wide_render_matches_oracle(gpu, name, integrator):
-    log("8-wide render", name, integrator)
+def test_wide_render_matches_oracle(gpu, name, integrator, width):
+    log("%d-wide render" % width, name, integrator)
     sc = SCENES[name]()
     c = ctx()
     s = c.scene(sc)
     b2 = s.build_bvh(16)
-    b8 = s.build_bvh(16, width=8)
+    b8 = s.build_bvh(16, width=width)
     w, h, spp, depth = (96, 96, 8, 8) if name == "cornell" else (128, 72, 8, 12)
     img8, _ = b8.render(w, h, spp, depth, integrator, 3)
     img2, _ = b2.render(w, h, spp, depth, integrator, 3)
@@ -80,11 +82,12 @@ def test_wide_render_matches_oracle(gpu, name, integrator):
     assert err.max() <= 1e-4
 
 
-def test_wide_batch_equals_single(gpu):
-    log("8-wide batch")
+@pytest.mark.parametrize("width", [4, 8])
+def test_wide_batch_equals_single(gpu, width):
+    log("%d-wide batch" % width)
     c = ctx()
-    cb = c.scene(scenes.cornell_box()).build_bvh(16, width=8)
-    sp = c.scene(SCENES["spheres"]()).build_bvh(16, width=8)
+    cb = c.scene(scenes.cornell_box()).build_bvh(16, width=width)
+    sp = c.scene(SCENES["spheres"]()).build_bvh(16, width=width)
     jobs = [P.Job(cb, 64, 64, 4, 8, 0, 5), P.Job(sp, 80, 60, 6, 12, 0, 6), P.Job(cb, 48, 40, 3, 8, 1, 7)]
     eng = P.Engine(c, 1 << 16, max_jobs=8)
     eng.render_batch(jobs)
