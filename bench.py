@@ -277,7 +277,7 @@ def run_batch(args):
     t0 = time.perf_counter()
     # the autotuner picks the node width unless --bvh-width fixes it
     # the binary walk is the reference's; the 8-wide one only on request (--bvh-width 8)
-    widths = (args.bvh_width,) if args.bvh_width else (2,)
+    widths = (args.bvh_width,) if args.bvh_width else ((2, 4, 8) if args.tune_widths else (2,))
     bvhs_by_width = {w: [] for w in widths}
     for i, sp in enumerate(specs):
         sc = scenes.scaled_scene(sp["tris"], seed=1000 + i, env=env)
@@ -657,6 +657,8 @@ def main():
                     help="BVH width: 2 walks the binary tree in the reference's order (frame default), "
                          "8 the 8-wide tree (traversal microbenchmark default); batch mode: both are "
                          "built and the autotuner picks one unless this is given")
+    ap.add_argument("--tune-widths", action="store_true",
+                    help="batch mode: the autotuner also tries the 4- and 8-wide trees (not the reference's walk)")
     ap.add_argument("--jobs", type=int, default=64, help="batch mode: number of jobs")
     ap.add_argument("--no-tune", action="store_true", help="batch mode: default launch shapes, --pool path slots")
     ap.add_argument("--pool", type=int, default=1 << 23, help="batch mode with --no-tune: path slots")

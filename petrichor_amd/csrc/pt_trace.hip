@@ -265,8 +265,9 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
     const int width = multi ? job.width : (b.nodes8 != nullptr ? b.width : 2);
     // persistent grid: blocks of 4 waves; the LDS stacks of a block bound the blocks per CU
     // (160 KB of LDS, at most 8 waves per SIMD)
-    const int lds_block = (width == 8 ? lds_stack_entries<8>() : lds_stack_entries<2>()) * PT_TRACE_BLOCK *
-                          (int)sizeof(uint2);
+    const int lds_block =
+        (width == 8 ? lds_stack_entries<8>() : (width == 4 ? lds_stack_entries<4>() : lds_stack_entries<2>())) *
+        PT_TRACE_BLOCK * (int)sizeof(uint2);
     const int max_bpc = (160 * 1024) / lds_block < 8 ? (160 * 1024) / lds_block : 8;
     const int bpc = tune.trace_blocks_per_cu < 1 || tune.trace_blocks_per_cu > max_bpc ? max_bpc
                                                                                      : tune.trace_blocks_per_cu;

@@ -40,13 +40,14 @@
 
 namespace pt {
 
-// LDS stack entries per lane, by node width W. Binary and 4-wide: 10 (20 KB per
-// 256-thread block, 8 blocks or 32 waves per CU). 8-wide: 16 (32 KB, 5 blocks per CU,
-// which its registers allow anyway), as its stacks grow by up to 7 entries per node.
+// LDS stack entries per lane, by node width W. Binary: 10 (20 KB per 256-thread block, 8
+// blocks or 32 waves per CU). 4-wide: 13 (26 KB, the 6 blocks per CU its registers
+// allow). 8-wide: 16 (32 KB, 5 blocks per CU, which its registers allow anyway), as its
+// stacks grow by up to 7 entries per node.
 template <int W>
 constexpr int lds_stack_entries()
 {
-    return W == 8 ? 16 : 10;
+    return W == 8 ? 16 : (W == 4 ? 13 : 10);
 }
 #define PT_TRACE_BLOCK 256
 // Whole traversal stack per lane (LDS entries first, the rest in private memory).
