@@ -687,6 +687,13 @@ private:
 class Texture2D
 {
 public:
+    // Core/Texture2D.h:33-37: colour textures get gamma 2.2 when loaded, non-colour ones not
+    enum class TextureColorType
+    {
+        Color,
+        NonColor
+    };
+
     Texture2D() = default;
     Texture2D(int width, int height) : m_width(width), m_height(height), m_pixels((size_t)width * height) {}
     void Clear(const Color3f& color = Color3f::Zero()) { std::fill(m_pixels.begin(), m_pixels.end(), color); }
@@ -697,6 +704,39 @@ public:
     bool IsValid() const { return m_width > 0 && m_height > 0; }
     const float* GetRawDataPtr() const { return &m_pixels[0].x; }
     float* GetRawDataPtr() { return &m_pixels[0].x; }
+
+    // Core/Texture2D.cpp:107-195: a ".png" path is read as stbi_load(..., 3) reads it, the
+    // format told by the file's header (pt_image_decode: PNG, BMP, TGA, binary PNM), bytes /
+    // 255, then gamma 2.2 for a colour texture; a ".hdr" path as Radiance RGBE floats, no
+    // gamma; any other extension, a missing file or an unreadable one gives false and
+    // leaves the texture as it was.
+    bool Load(const std::filesystem::path& path, TextureColorType textureColorType)
+    {
+        int32_t w = 0, h = 0;
+        std::vector<float> rgb;
+        if (path.extension() == ".png") {
+            std::string data;
+            if (!detail::readFile(path, data)) return false;
+            const auto* bytes = (const uint8_t*)data.data();
+            const int32_t color = textureColorType == TextureColorType::Color ? 1 : 0;
+            if (pt_image_decode(bytes, (int64_t)data.size(), color, &w, &h, nullptr, nullptr, 0) != PT_OK) return false;
+            rgb.resize(3 * (size_t)w * h);
+            if (pt_image_decode(bytes, (int64_t)data.size(), color, &w, &h, rgb.data(), nullptr, 0) != PT_OK)
+                return false;
+        } else if (path.extension() == ".hdr") {
+            int hw = 0, hh = 0;
+            if (!detail::loadHdr(path, hw, hh, rgb)) return false;
+            w = hw;
+            h = hh;
+        } else {
+            return false;
+        }
+        m_width = w;
+        m_height = h;
+        m_pixels.assign((size_t)w * h, Color3f::Zero());
+        std::memcpy(GetRawDataPtr(), rgb.data(), rgb.size() * sizeof(float));
+        return true;
+    }
 
     // Core/Texture2D.cpp:197-246: ".png" writes the reference's 8-bit bytes (ACES filmic,
     // RTCamp7 curve, degamma, truncation), converted on the GPU by pt_tonemap_rgb8;
@@ -835,32 +875,13 @@ public:
     const Color3f& GetBaseColor() const { return m_baseColor; }
     float GetZAxisRotation() const { return m_ZAxisRotation; }
     bool UseEnvImportanceSampling() const { return m_texEnv && m_texEnv->IsValid() && m_useImportanceSampling; }
-    // Environment::Load (Core/Environment.cpp:15-27) of a Radiance .hdr or a .png map, which
-    // the environment then owns (shared by its copies). False for another format or a read
-    // error.
+    // Environment::Load (Core/Environment.cpp:15-27): Texture2D::Load of the map as a colour
+    // texture (a Radiance .hdr, or a .png path), which the environment then owns (shared by
+    // its copies). False for another format or a read error.
     bool Load(const std::filesystem::path& path)
     {
-        int w = 0, h = 0;
-        std::vector<float> rgb;
-        if (path.extension() == ".png") {
-            // a colour texture (Core/Environment.cpp:22): bytes / 255, gamma 2.2
-            std::string data;
-            if (!detail::readFile(path, data)) return false;
-            int32_t pw = 0, ph = 0;
-            if (pt_png_decode((const uint8_t*)data.data(), (int64_t)data.size(), 1, &pw, &ph, nullptr, nullptr, 0) !=
-                PT_OK)
-                return false;
-            rgb.resize(3 * (size_t)pw * ph);
-            if (pt_png_decode((const uint8_t*)data.data(), (int64_t)data.size(), 1, &pw, &ph, rgb.data(), nullptr, 0) !=
-                PT_OK)
-                return false;
-            w = pw;
-            h = ph;
-        } else if (path.extension() != ".hdr" || !detail::loadHdr(path, w, h, rgb)) {
-            return false;
-        }
-        auto tex = std::make_shared<Texture2D>(w, h);
-        std::memcpy(tex->GetRawDataPtr(), rgb.data(), rgb.size() * sizeof(float));
+        auto tex = std::make_shared<Texture2D>();
+        if (!tex->Load(path, Texture2D::TextureColorType::Color)) return false;
         m_ownedTex = tex;
         m_texEnv = tex.get();
         return true;
@@ -1070,9 +1091,10 @@ public:
                                    bool color) -> const Texture2D* {
                     if (!std::filesystem::exists(file)) return nullptr;
                     Texture2D t;
-                    if (!loadPngTexture(file, color, t)) {
+                    if (!t.Load(file, color ? Texture2D::TextureColorType::Color
+                                            : Texture2D::TextureColorType::NonColor)) {
                         texError = "material '" + name + "': cannot read texture " + file.string() +
-                                   " (material textures are read from .png files)";
+                                   " (Texture2D::Load reads .png and .hdr paths)";
                         return nullptr;
                     }
                     return &m_textures.emplace(key, std::move(t)).first->second;
@@ -1130,22 +1152,6 @@ private:
     std::map<std::string, Texture2D> m_textures; // loaded material textures by name (nodes stay put)
     Environment m_environment;
 
-    // A .png file as Texture2D::Load reads it: RGB bytes / 255, then gamma 2.2 for a colour
-    // texture (Core/Texture2D.cpp:107-160).
-    static bool loadPngTexture(const std::filesystem::path& path, bool color, Texture2D& out)
-    {
-        std::string data;
-        if (path.extension() != ".png" || !detail::readFile(path, data)) return false;
-        int32_t w = 0, h = 0;
-        const auto* bytes = (const uint8_t*)data.data();
-        if (pt_png_decode(bytes, (int64_t)data.size(), color ? 1 : 0, &w, &h, nullptr, nullptr, 0) != PT_OK)
-            return false;
-        Texture2D t(w, h);
-        if (pt_png_decode(bytes, (int64_t)data.size(), color ? 1 : 0, &w, &h, t.GetRawDataPtr(), nullptr, 0) != PT_OK)
-            return false;
-        out = std::move(t);
-        return true;
-    }
     std::unique_ptr<Camera> m_mainCamera;
     std::array<Texture2D*, AOVType::NumAOVTypes> m_targetTextures = {};
     RenderSetting m_renderSetting;

@@ -442,6 +442,16 @@ int pt_obj_parse(const char* text, int64_t len, int64_t max_triangles, float* po
 int pt_png_decode(const uint8_t* data, int64_t len, int32_t color, int32_t* width, int32_t* height, float* rgb_out,
                   char* err, int32_t err_len);
 
+/* An image file's pixels as Texture2D::Load reads a non-HDR texture with stbi_load(..., 3)
+ * (Core/Texture2D.cpp:107-160), for the formats whose pixels are fixed by the file: PNG
+ * (as pt_png_decode), BMP (1/4/8-bit palettes, 16/24/32-bit, bit-field masks; not
+ * run-length encoded), TGA (colour-mapped, true-colour and grey, raw or run-length) and
+ * binary PNM (P5, P6, maximum value <= 255), told apart by their headers. 8-bit RGB as stb
+ * gives it, each byte / 255, then powf(x, 2.2) when color != 0. Outputs as pt_png_decode.
+ * JPEG, GIF and PSD files: PT_ERR_UNSUPPORTED; malformed files: PT_ERR_ARG (reason in err). */
+int pt_image_decode(const uint8_t* data, int64_t len, int32_t color, int32_t* width, int32_t* height, float* rgb_out,
+                    char* err, int32_t err_len);
+
 #ifdef __cplusplus
 }
 #endif

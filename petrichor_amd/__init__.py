@@ -171,6 +171,8 @@ def lib():
                                ctypes.c_int32]
     L.pt_png_decode.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int32, _i, _i, _f, ctypes.c_char_p,
                                 ctypes.c_int32]
+    L.pt_image_decode.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int32, _i, _i, _f, ctypes.c_char_p,
+                                  ctypes.c_int32]
     _lib = L
     return L
 
@@ -182,7 +184,7 @@ EXPORTED = ["pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_scene_create
             "pt_render_batch", "pt_engine_set_tuning", "pt_engine_get_tuning", "pt_server_create", "pt_server_destroy", "pt_server_last_error",
             "pt_server_add_scene", "pt_server_remove_scene", "pt_server_scene_resident", "pt_server_submit",
             "pt_server_poll", "pt_server_wait", "pt_server_preview", "pt_server_fetch", "pt_server_get_stats",
-            "pt_tonemap_rgb8", "pt_obj_parse", "pt_render_aov", "pt_png_decode"]
+            "pt_tonemap_rgb8", "pt_obj_parse", "pt_render_aov", "pt_png_decode", "pt_image_decode"]
 
 
 class RaysSoa(ctypes.Structure):

@@ -20,7 +20,7 @@ FP_FLAGS = ["-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt", "-f
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"] + FP_FLAGS
 
-SOURCES = ["pt_kernels.hip", "pt_trace.hip", "pt_build.hip", "pt_wide.hip", "pt_post.hip", "pt_order.hip", "pt_obj.cpp", "pt_png.cpp", "pt_bvh_host.cpp", "pt_capi.cpp", "pt_engine.cpp", "pt_server.cpp"]
+SOURCES = ["pt_kernels.hip", "pt_trace.hip", "pt_build.hip", "pt_wide.hip", "pt_post.hip", "pt_order.hip", "pt_obj.cpp", "pt_png.cpp", "pt_image.cpp", "pt_bvh_host.cpp", "pt_capi.cpp", "pt_engine.cpp", "pt_server.cpp"]
 
 
 def _run(cmd):

@@ -9,8 +9,11 @@ meshes, turned into the scene dict of pt_scene_desc (see scenes.py).
   Assimp 4.1.0 (the reference's pinned dependency, conanfile.txt:2; not present here)
   hands Mesh::Load for an OBJ file read with no post-processing flags: one output vertex
   per face corner, faces in file order, no triangulation, Assimp's number parsing.
-- The environment texture is read with load_hdr_file (Radiance RGBE, flat or run-length
-  scanlines, as stbi_loadf decodes them) or, for a .png, with decode_png (pt_png_decode).
+- Textures are read as Texture2D::Load reads them (Core/Texture2D.cpp:107-195), by the
+  path's extension: a .hdr with load_hdr_file (Radiance RGBE, flat or run-length
+  scanlines, as stbi_loadf decodes them, no gamma); a .png with decode_image
+  (pt_image_decode), which, as stbi_load, tells the format by the file's header (PNG,
+  BMP, TGA, binary PNM); any other extension is a file Texture2D::Load refuses.
 
 Behaviour kept from the reference:
 - "emission" materials are never registered (Core/SceneLoader.cpp:219-221), so a
@@ -29,8 +32,10 @@ file that does not exist leaves the material untextured, and a texture registere
 name already taken reuses the first one (Core/Scene.h:131-143).
 
 Where the reference would dereference a missing material (an unknown material name, a
-mix of unknown materials, a point or line element) or reads a file this module does not
-(material textures other than .png), these functions raise ValueError instead.
+mix of unknown materials, a point or line element), or logs an error and goes on without
+the texture (a texture path that is not .png or .hdr, an unreadable file, or a .png path
+holding a JPEG, GIF or PSD image, whose pixels depend on stb's own decoder), these
+functions raise ValueError instead.
 """
 import json
 import os
@@ -99,22 +104,44 @@ def parse_obj(text):
     return p, nr, uv
 
 
-def decode_png(data, color=True):
-    """A PNG file's pixels as Texture2D::Load reads them (pt_png_decode): [H, W, 3] float32,
-    bytes / 255, then powf(x, 2.2) for a colour texture."""
+def _decode_ldr(fn, data, color):
     import ctypes
     from . import lib, _ptr, _f, _i
     data = bytes(data)
-    L = lib()
+    f = getattr(lib(), fn)
     w = np.zeros(1, np.int32)
     h = np.zeros(1, np.int32)
     err = ctypes.create_string_buffer(256)
-    if L.pt_png_decode(data, len(data), int(color), _ptr(w, _i), _ptr(h, _i), None, err, 256) != 0:
+    if f(data, len(data), int(color), _ptr(w, _i), _ptr(h, _i), None, err, 256) != 0:
         raise ValueError(err.value.decode())
     out = np.zeros((int(h[0]), int(w[0]), 3), np.float32)
-    if L.pt_png_decode(data, len(data), int(color), _ptr(w, _i), _ptr(h, _i), _ptr(out, _f), err, 256) != 0:
+    if f(data, len(data), int(color), _ptr(w, _i), _ptr(h, _i), _ptr(out, _f), err, 256) != 0:
         raise ValueError(err.value.decode())
     return out
+
+
+def decode_png(data, color=True):
+    """A PNG file's pixels as Texture2D::Load reads them (pt_png_decode): [H, W, 3] float32,
+    bytes / 255, then powf(x, 2.2) for a colour texture."""
+    return _decode_ldr("pt_png_decode", data, color)
+
+
+def decode_image(data, color=True):
+    """The pixels of the file at a .png path as Texture2D::Load reads them
+    (pt_image_decode: PNG, BMP, TGA or binary PNM, told by the header, as stbi_load does):
+    [H, W, 3] float32, bytes / 255, then powf(x, 2.2) for a colour texture."""
+    return _decode_ldr("pt_image_decode", data, color)
+
+
+def load_texture(path, color=True):
+    """Texture2D::Load (Core/Texture2D.cpp:107-195) by the path's extension: .png through
+    decode_image, .hdr through load_hdr_file (no gamma); ValueError for anything else."""
+    if path.endswith(".png"):
+        with open(path, "rb") as f:
+            return decode_image(f.read(), color=color)
+    if path.endswith(".hdr"):
+        return load_hdr_file(path)
+    raise ValueError("%s: Texture2D::Load reads .png and .hdr paths only" % path)
 
 
 def load_obj(path):
@@ -194,10 +221,10 @@ def load_scene(path):
         # Texture2D::Load, then Scene::RegisterTexture (the first texture of a name stays)
         if not os.path.exists(path):
             return -1
-        if not path.endswith(".png"):
-            raise ValueError("material '%s': material textures are read from .png files (%s)" % (name, path))
-        with open(path, "rb") as f:
-            rgb = decode_png(f.read(), color=color)
+        try:
+            rgb = load_texture(path, color=color)
+        except ValueError as e:
+            raise ValueError("material '%s': %s" % (name, e))
         if key not in tex_names:
             tex_names[key] = len(textures)
             textures.append(rgb)
@@ -256,14 +283,8 @@ def load_scene(path):
     env_rgb = None
     if ed.get("texture"):
         tex = os.path.join(base, ed["texture"])
-        if tex.endswith(".hdr"):
-            env_rgb = load_hdr_file(tex)
-        elif tex.endswith(".png"):
-            # Environment::Load reads its texture as a colour texture (gamma 2.2)
-            with open(tex, "rb") as f:
-                env_rgb = decode_png(f.read(), color=True)
-        else:
-            raise ValueError("environment texture %s: only .hdr and .png maps are read" % tex)
+        # Environment::Load reads its texture as a colour texture (Core/Environment.cpp:22)
+        env_rgb = load_texture(tex, color=True)
     # ---- assets (Core/SceneLoader.cpp:284-300): smooth triangles of each OBJ ----
     tri_pos, tri_nrm, tri_uv, tri_mat = [], [], [], []
     for a in data.get("assets", []):

@@ -40,7 +40,18 @@ def test_cpp_scene_files_load_without_a_device(tmp_path):
     r = run(scene_json)
     assert r.returncode == 0 or (r.returncode == 1 and "render failed" in r.stdout), r.stdout
     doc = json.loads(open(scene_json).read())
-    # material textures are read from .png files; another existing file is reported
+    # Texture2D::Load reads .png and .hdr paths; a .png path holding a BMP is read by its
+    # header (as stbi_load reads it); another extension is reported
+    import io
+    from PIL import Image
+    b = io.BytesIO()
+    Image.fromarray(sky, "RGB").save(b, format="BMP")
+    (tmp_path / "bmp_inside.png").write_bytes(b.getvalue())
+    doc0 = json.loads(open(scene_json).read())
+    doc0["materials"][0]["color_tex"] = "bmp_inside.png"
+    (tmp_path / "bmp.json").write_text(json.dumps(doc0))
+    r = run(tmp_path / "bmp.json")
+    assert r.returncode == 0 or (r.returncode == 1 and "render failed" in r.stdout), r.stdout
     (tmp_path / "wood.jpg").write_bytes(b"\xff\xd8\xff")
     doc["materials"][0]["color_tex"] = "wood.jpg"
     (tmp_path / "tex.json").write_text(json.dumps(doc))

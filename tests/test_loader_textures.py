@@ -1,7 +1,9 @@
 """Material textures in the scene files (Core/SceneLoader.cpp:106-189): a Lambert's and a
 GGX's "color_tex" (colour textures, from the scene's folder) and a GGX's "roughness_tex"
 (a non-colour texture, from the path as written), read by the Python and the C++ loaders
-and rendered with OBJ texture coordinates."""
+and rendered with OBJ texture coordinates. Texture2D::Load reads .png and .hdr paths
+(Core/Texture2D.cpp:107-195); the "tga_hdr" files are a run-length TGA at a .png path
+(read by its header, as stbi_load reads it) and a Radiance .hdr roughness map (no gamma)."""
 import json
 
 import numpy as np
@@ -30,12 +32,22 @@ def textured_sphere_obj(path, *args):
     path.write_text("\n".join(vts + faces) + "\n")
 
 
-def textured_scene(tmp_path):
+def textured_scene(tmp_path, fmt="png"):
     rng = np.random.default_rng(4)
     albedo = rng.integers(0, 256, (6, 10, 3), dtype=np.uint8)
     rough = rng.integers(40, 256, (5, 7, 3), dtype=np.uint8)
-    P.save_png(str(tmp_path / "albedo.png"), albedo)
-    P.save_png(str(tmp_path / "rough.png"), rough)
+    rough_name = "rough.png"
+    if fmt == "png":
+        P.save_png(str(tmp_path / "albedo.png"), albedo)
+        P.save_png(str(tmp_path / rough_name), rough)
+    else:
+        import io
+        from PIL import Image
+        b = io.BytesIO()
+        Image.fromarray(albedo, "RGB").save(b, format="TGA", compression="tga_rle")
+        (tmp_path / "albedo.png").write_bytes(b.getvalue())
+        rough_name = "rough.hdr"
+        P.save_hdr(str(tmp_path / rough_name), rough.astype(np.float32) / np.float32(255))
     _, path = cornell_files(tmp_path)
     textured_sphere_obj(tmp_path / "ball.obj", 0.3, 0.35, 0.22, 0.18)
     textured_sphere_obj(tmp_path / "ball2.obj", 0.7, 0.6, 0.25, 0.2)
@@ -43,7 +55,7 @@ def textured_scene(tmp_path):
     doc["materials"] += [
         {"type": "lambert", "name": "tex_white", "base_color": [0.9, 0.85, 0.8], "color_tex": "albedo.png"},
         {"type": "glossy", "name": "tex_ggx", "color": [0.9, 0.9, 0.8], "roughness": 0.3,
-         "color_tex": "albedo.png", "roughness_tex": str(tmp_path / "rough.png")},
+         "color_tex": "albedo.png", "roughness_tex": str(tmp_path / rough_name)},
         {"type": "lambert", "name": "no_file", "base_color": [0.5, 0.5, 0.5], "color_tex": "missing.png"},
     ]
     doc["assets"] += [{"path": "ball.obj", "mat": "tex_white"}, {"path": "ball2.obj", "mat": "tex_ggx"}]
@@ -51,8 +63,9 @@ def textured_scene(tmp_path):
     return albedo, rough
 
 
-def test_loader_registers_material_textures(tmp_path):
-    albedo, rough = textured_scene(tmp_path)
+@pytest.mark.parametrize("fmt", ["png", "tga_hdr"])
+def test_loader_registers_material_textures(tmp_path, fmt):
+    albedo, rough = textured_scene(tmp_path, fmt)
     sc = loader.load_scene(str(tmp_path / "scene.json"))
     # "emission" materials register nothing
     names = [m["name"] for m in json.loads((tmp_path / "scene.json").read_text())["materials"]
@@ -62,16 +75,25 @@ def test_loader_registers_material_textures(tmp_path):
     maps = sc["mat_maps"]
     lam, ggx, nofile = maps[idx["tex_white"]], maps[idx["tex_ggx"]], maps[idx["no_file"]]
     assert lam[1] == -1 and nofile.tolist() == [-1, -1]
-    np.testing.assert_array_equal(sc["textures"][lam[0]], loader.decode_png((tmp_path / "albedo.png").read_bytes()))
+    gamma = loader.decode_image(b"P6 10 6 255\n" + albedo.tobytes())  # the same bytes as a PNM
+    np.testing.assert_array_equal(sc["textures"][lam[0]], gamma)
+    if fmt == "png":
+        np.testing.assert_array_equal(sc["textures"][lam[0]], loader.decode_png((tmp_path / "albedo.png").read_bytes()))
     np.testing.assert_array_equal(sc["textures"][ggx[0]], sc["textures"][lam[0]])
-    np.testing.assert_array_equal(sc["textures"][ggx[1]], rough.astype(np.float32) / np.float32(255))
+    lin = rough.astype(np.float32) / np.float32(255)
+    if fmt == "png":
+        np.testing.assert_array_equal(sc["textures"][ggx[1]], lin)
+    else:  # the floats of the .hdr file as written (RGBE keeps 8 mantissa bits)
+        np.testing.assert_array_equal(sc["textures"][ggx[1]], loader.load_hdr_file(str(tmp_path / "rough.hdr")))
+        assert np.abs(sc["textures"][ggx[1]] - lin).max() <= lin.max() / 128
     assert sc["tri_uv"] is not None and sc["tri_uv"].max() > 0.9
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("fmt", ["png", "tga_hdr"])
 @pytest.mark.parametrize("integrator", [0, 1])
-def test_textured_scene_file_renders_as_the_oracle(gpu, tmp_path, integrator):
-    textured_scene(tmp_path)
+def test_textured_scene_file_renders_as_the_oracle(gpu, tmp_path, integrator, fmt):
+    textured_scene(tmp_path, fmt)
     sc = loader.load_scene(str(tmp_path / "scene.json"))
     log("textured scene file, integrator", integrator)
     b = ctx().scene(sc).build_bvh(16)
