@@ -299,7 +299,8 @@ def run_batch(args):
     if args.no_tune:
         w0 = widths[0]
         best = {"width": w0, "jobs": jobs_with(out_b, bvhs_by_width[w0]), "max_paths": args.pool,
-                "tuning": dict(autotune.DEFAULT), "trials": 0, "seconds": 0.0}
+                "tuning": dict(autotune.DEFAULT, admission=args.admission, solo_log2_samples=args.solo),
+                "trials": 0, "seconds": 0.0}
     else:
         best = autotune.tune(ctx, None, log=log, by_width={w: jobs_with(out_b, bvhs_by_width[w]) for w in widths},
                              exact_only=widths == (2,))
@@ -662,6 +663,12 @@ def main():
     ap.add_argument("--jobs", type=int, default=64, help="batch mode: number of jobs")
     ap.add_argument("--no-tune", action="store_true", help="batch mode: default launch shapes, --pool path slots")
     ap.add_argument("--pool", type=int, default=1 << 23, help="batch mode with --no-tune: path slots")
+    ap.add_argument("--solo", type=int, default=0,
+                    help="batch mode with --no-tune: jobs of at least 2^SOLO samples render alone on the "
+                         "single-job engine beside the shared one (0: none)")
+    ap.add_argument("--admission", type=int, default=0, choices=[0, 1],
+                    help="batch mode with --no-tune: 0 admits any job that fits, 1 waits for the running "
+                         "jobs to hand out their samples")
     ap.add_argument("--batch-only", action="store_true", help="batch mode: only the batched run (no per-job runs)")
     ap.add_argument("--tris-max", type=float, default=2e6, help="batch/serve modes: largest scene")
     ap.add_argument("--spp-max", type=int, default=256, help="batch/serve modes: largest spp")

@@ -329,7 +329,19 @@ typedef struct pt_tuning {
                                     as the LDS stacks allow; larger values are clamped to that) */
     int32_t trace_chunk;         /* rays a wave takes from the shared cursor at a time, 64..4096 */
     int32_t shade_blocks_per_cu; /* shading blocks of 256 threads per CU, 1..32 */
+    int32_t admission;           /* PT_ADMIT_*: when pt_render_batch admits the next job */
+    int32_t solo_log2_samples;   /* 0, or k in 10..40: pt_render_batch renders each job of at least
+                                    2^k samples on its own with the single-job kernels (one job at a
+                                    time, on the context's single-job engine and stream) while the
+                                    other jobs share this engine; the two streams' launches overlap.
+                                    Images do not depend on it. */
 } pt_tuning;
+/* pt_tuning.admission. PT_ADMIT_FIT: any pending job whose path slots are free (largest
+ * first). PT_ADMIT_TAILS: a job only once every running job has handed out all its samples,
+ * so that large jobs run one after another with up to half the pool each and the next job
+ * starts in the slots the previous one's last paths leave free. Images do not depend on it. */
+#define PT_ADMIT_FIT 0
+#define PT_ADMIT_TAILS 1
 int pt_engine_set_tuning(pt_engine* engine, const pt_tuning* tuning);
 int pt_engine_get_tuning(pt_engine* engine, pt_tuning* tuning);
 /* Renders n jobs to completion, admitting them in order as slots free up;

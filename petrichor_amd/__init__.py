@@ -72,7 +72,12 @@ class BatchStats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("iterations_per_sync", ctypes.c_int32), ("trace_blocks_per_cu", ctypes.c_int32),
-                ("trace_chunk", ctypes.c_int32), ("shade_blocks_per_cu", ctypes.c_int32)]
+                ("trace_chunk", ctypes.c_int32), ("shade_blocks_per_cu", ctypes.c_int32),
+                ("admission", ctypes.c_int32), ("solo_log2_samples", ctypes.c_int32)]
+
+
+ADMIT_FIT = 0    # pt_render_batch admits any pending job whose slots are free
+ADMIT_TAILS = 1  # ... only once every running job has handed out all its samples
 
 
 RENDER_TIME_KERNELS = 1
@@ -641,8 +646,12 @@ class Engine:
                                             ctypes.byref(h)), "pt_engine_create")
         self.handle = h
 
-    def set_tuning(self, iterations_per_sync=8, trace_blocks_per_cu=0, trace_chunk=64, shade_blocks_per_cu=8):
-        t = Tuning(iterations_per_sync, trace_blocks_per_cu, trace_chunk, shade_blocks_per_cu)
+    def set_tuning(self, iterations_per_sync=8, trace_blocks_per_cu=0, trace_chunk=64, shade_blocks_per_cu=8,
+                   admission=ADMIT_FIT, solo_log2_samples=0):
+        """pt_engine_set_tuning. solo_log2_samples = k > 0: render_batch renders each job of
+        at least 2^k samples alone on the context's single-job engine, beside the others."""
+        t = Tuning(iterations_per_sync, trace_blocks_per_cu, trace_chunk, shade_blocks_per_cu, admission,
+                   solo_log2_samples)
         self.ctx.check(self.ctx._lib.pt_engine_set_tuning(self.handle, ctypes.byref(t)), "pt_engine_set_tuning")
 
     def get_tuning(self):

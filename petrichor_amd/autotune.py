@@ -6,9 +6,11 @@ replays per host synchronisation and the launch shapes of the trace and shading 
 (pt_tuning). It times pt_render_batch on a calibration batch - the same scenes and
 resolutions at a few samples per pixel - first for each width and then for each pool size
 with the default launch shapes, then with coordinate descent over the space below on the
-best pool size. Every
-candidate of one width renders the same images (tuning only changes launch shapes and
-scheduling), so the fastest one is taken as is.
+best pool size. Last, on the full batch, it picks the size from which a job renders alone
+on the context's single-job engine (the single-job kernels, beside the shared engine on a
+second stream) rather than in the shared pool. Every candidate of one width renders the
+same images (tuning only changes launch shapes and scheduling), so the fastest one is taken
+as is.
 
 Only the binary width walks the tree exactly as the reference (BinnedSAHBVH::Intersect,
 Core/Accel/BinnedSAHBVH.cpp:272-374). The 4- and 8-wide walks can return another hit on
@@ -19,6 +21,10 @@ when the caller accepts that (exact_only=False).
 import time
 
 DEFAULT = {"iterations_per_sync": 8, "trace_blocks_per_cu": 0, "trace_chunk": 64, "shade_blocks_per_cu": 8}
+# Thresholds tried for rendering the largest jobs alone on the single-job engine, in log2
+# samples (0: every job in the shared engine). They are timed on the full batch, since the
+# calibration batch's fewer samples per pixel would move every job across them.
+SOLO = (0, 23, 25, 27)
 
 SPACE = {
     "iterations_per_sync": [2, 4, 8, 16, 32],
@@ -39,11 +45,13 @@ def calibration_jobs(jobs, spp=2):
     return [Job(j.bvh, j.width, j.height, min(spp, j.spp), j.max_bounces, j.integrator, j.seed, j.out) for j in jobs]
 
 
-def tune(ctx, jobs, pool_sizes=None, passes=1, log=None, by_width=None, calib_spp=8, exact_only=True):
+def tune(ctx, jobs, pool_sizes=None, passes=1, log=None, by_width=None, calib_spp=8, exact_only=True, solo=SOLO):
     """Returns {"width", "jobs", "max_paths", "tuning", "seconds", "trials"} for the fastest
     configuration found on the calibration batch of `jobs`. by_width (optional) maps a BVH
     width (2, 4, 8) to the same jobs over BVHs of that width; "jobs" is then the chosen list.
-    exact_only: only binary BVHs (the reference's walk) are candidates."""
+    exact_only: only binary BVHs (the reference's walk) are candidates. solo: thresholds
+    (log2 samples) for rendering the largest jobs alone on the single-job engine, timed on
+    the full batch last ("solo_seconds" then holds the full batch's time per threshold)."""
     from . import Engine
     by_width = dict(by_width) if by_width else {getattr(jobs[0].bvh, "width", 2): jobs}
     if exact_only:
@@ -110,10 +118,25 @@ def tune(ctx, jobs, pool_sizes=None, passes=1, log=None, by_width=None, calib_sp
                 t = measure(cand)
                 if t < cur_t:
                     cur, cur_t = cand, t
+    solo_seconds = {}
+    if solo and len(solo) > 1:
+        # first use of the single-job engine (allocation, graph capture) outside the timing
+        eng.set_tuning(**dict(cur, solo_log2_samples=min(k for k in solo if k > 0)))
+        eng.render_batch(calib)
+        for k in solo:
+            cand = dict(cur, solo_log2_samples=k)
+            eng.set_tuning(**cand)
+            t0 = time.perf_counter()
+            eng.render_batch(chosen)
+            solo_seconds[k] = time.perf_counter() - t0
+            trials.append({"max_paths": pool, "tuning": cand, "seconds": solo_seconds[k], "full_batch": True})
+            if log:
+                log("autotune: solo 2^%d samples -> %.1f ms (full batch)" % (k, solo_seconds[k] * 1e3))
+        cur = dict(cur, solo_log2_samples=min(solo_seconds, key=solo_seconds.get))
     eng.close()
     if log:
         log("autotune: pool %d -> %s, %.1f ms" % (pool, cur, cur_t * 1e3))
-    best = {"max_paths": pool, "tuning": cur, "seconds": cur_t}
+    best = {"max_paths": pool, "tuning": cur, "seconds": cur_t, "solo_seconds": solo_seconds}
     best["width"] = width
     best["jobs"] = chosen
     best["trials"] = len(trials)

@@ -35,7 +35,13 @@ using namespace pt;
 // (run_halves).
 static constexpr int64_t kHalvesMinPixels = (int64_t)1 << 19;
 // Path slots of pt_render's engines: a job runs on up to max(its pixels, this many) slots.
-static constexpr int64_t kSingleJobSlots = (int64_t)1 << 21;
+// 8M slots per engine (about 4 GB of path state and queues): on the single large scene a
+// half of the frame ran at 1749 / 1887 / 1904 / 1934 / 1935 Mrays/s with 2M / 4M / 6M / 8M /
+// 12M slots, as fewer, larger trace launches spend less of their time in the slowest rays.
+#ifndef PT_SINGLE_JOB_SLOTS
+#define PT_SINGLE_JOB_SLOTS ((int64_t)1 << 23)
+#endif
+static constexpr int64_t kSingleJobSlots = PT_SINGLE_JOB_SLOTS;
 
 namespace pt {
 
@@ -140,8 +146,15 @@ int run_jobs(pt_engine& E, std::vector<EngineJob>& jobs, int iters_per_sync, pt_
     PT_HIP(ctx, hipEventRecord(ev0, E.st));
     const auto t0 = std::chrono::steady_clock::now();
     int64_t replays = 0;
+    // PT_ADMIT_TAILS: the next job waits until every running job has handed out its samples
+    auto admit_open = [&]() {
+        if (E.tune.admission != PT_ADMIT_TAILS) return true;
+        for (const EngineJob* r : running)
+            if (!E.job_exhausted(*r)) return false;
+        return true;
+    };
     while (!pending.empty() || !running.empty()) {
-        for (size_t i = 0; i < pending.size() && (int64_t)running.size() < E.max_jobs;) {
+        for (size_t i = 0; i < pending.size() && (int64_t)running.size() < E.max_jobs && admit_open();) {
             EngineJob& j = *pending[i];
             j.slots = E.job_slots(j);
             if (E.largest_free() < j.slots || !E.width_ok(pt_engine::job_width(j))) {
@@ -192,6 +205,116 @@ int run_jobs(pt_engine& E, std::vector<EngineJob>& jobs, int iters_per_sync, pt_
         stats->iterations = E.iterations;
         stats->elapsed_ms = ms;
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->syncs = replays;
+    }
+    return PT_OK;
+}
+
+// pt_render_batch with solo jobs: the jobs of at least solo_min samples render one at a
+// time, largest first, on the single-job engine S (the single-job kernels, the scene as
+// kernel arguments, S's stream), while the other jobs share the multi-job engine E as in
+// run_jobs. Both engines' graph replays are queued before either is waited for, so the GPU
+// overlaps the two streams' launches (one engine's kernels fill the CUs that the other's
+// trace launch leaves idle while it waits for its slowest rays). Every job's image is the
+// same as in one piece.
+int run_jobs_solo(pt_engine& E, pt_engine& S, std::vector<EngineJob>& jobs, int64_t solo_min, int iters_per_sync,
+                  pt_batch_stats* stats)
+{
+    pt_context* ctx = E.ctx;
+    std::vector<EngineJob*> solo_q, pending;
+    for (EngineJob& j : jobs) (j.items >= solo_min ? solo_q : pending).push_back(&j);
+    auto by_size = [](const EngineJob* a, const EngineJob* b) {
+        return a->npix * a->desc.spp > b->npix * b->desc.spp;
+    };
+    std::stable_sort(solo_q.begin(), solo_q.end(), by_size);
+    std::stable_sort(pending.begin(), pending.end(), by_size);
+    // the engines' streams must not start before the context's stream work is done
+    PT_HIP(ctx, hipDeviceSynchronize());
+    S.iterations = 0;
+    PT_HIP(ctx, hipMemsetAsync(S.queries_d, 0, sizeof(unsigned long long), S.st));
+    const auto t0 = std::chrono::steady_clock::now();
+    size_t next_solo = 0;
+    EngineJob* solo = nullptr;
+    std::vector<EngineJob*> running;
+    int64_t replays = 0;
+    while (solo || next_solo < solo_q.size() || !pending.empty() || !running.empty()) {
+        if (!solo && next_solo < solo_q.size()) {
+            EngineJob& j = *solo_q[next_solo++];
+            j.slots = S.job_slots(j);
+            int rc = prepare_output(ctx, j);
+            if (rc == PT_OK) rc = S.start(j);
+            if (rc != PT_OK) return rc;
+            S.live_h = (int)j.slots;
+            solo = &j;
+        }
+        for (size_t i = 0; i < pending.size() && (int64_t)running.size() < E.max_jobs;) {
+            EngineJob& j = *pending[i];
+            j.slots = E.job_slots(j);
+            if (E.largest_free() < j.slots || !E.width_ok(pt_engine::job_width(j))) {
+                ++i;
+                continue;
+            }
+            int rc = prepare_output(ctx, j);
+            if (rc == PT_OK) rc = E.start(j);
+            if (rc != PT_OK) return rc;
+            E.live_h += (int)j.slots;
+            running.push_back(&j);
+            pending.erase(pending.begin() + i);
+        }
+        if (!solo && running.empty()) return pt_fail(ctx, PT_ERR_BUDGET, "job larger than the engine's path capacity");
+        // a solo job's last live paths, and the shared engine's once every job is admitted
+        // and exhausted, complete in one launch
+        const bool s_last = solo && S.job_exhausted(*solo) && S.live_h > 0 && S.live_h <= S.finish_threshold;
+        bool e_exhausted = pending.empty();
+        for (const EngineJob* j : running) e_exhausted = e_exhausted && E.job_exhausted(*j);
+        const bool e_last = !running.empty() && e_exhausted && E.live_h > 0 && E.live_h <= E.finish_threshold;
+        int rc = PT_OK;
+        if (solo) rc = s_last ? S.launch_finish() : S.launch(iters_per_sync);
+        if (rc == PT_OK && !running.empty()) rc = e_last ? E.launch_finish() : E.launch(iters_per_sync);
+        if (rc != PT_OK) return rc;
+        ++replays;
+        if (solo) {
+            rc = S.wait();
+            if (rc != PT_OK) return rc;
+            if (S.job_done(*solo)) {
+                rc = S.resolve(*solo);
+                if (rc == PT_OK) rc = finish_output(ctx, *solo, S.st);
+                if (rc != PT_OK) return rc;
+                S.retire(*solo);
+                solo = nullptr;
+            } else if (s_last || S.live_h == 0) {
+                return pt_fail(ctx, PT_ERR_STATE, "solo job ended with samples unfinished");
+            }
+        }
+        if (!running.empty()) {
+            rc = E.wait();
+            if (rc != PT_OK) return rc;
+            if (e_last)
+                for (const EngineJob* j : running)
+                    if (!E.job_done(*j)) return pt_fail(ctx, PT_ERR_STATE, "path completion left samples unfinished");
+            for (size_t i = 0; i < running.size();) {
+                EngineJob& j = *running[i];
+                if (E.job_done(j)) {
+                    rc = E.resolve(j);
+                    if (rc == PT_OK) rc = finish_output(ctx, j, E.st);
+                    if (rc != PT_OK) return rc;
+                    E.retire(j);
+                    running.erase(running.begin() + i);
+                } else {
+                    ++i;
+                }
+            }
+        }
+    }
+    PT_HIP(ctx, hipStreamSynchronize(S.st));
+    PT_HIP(ctx, hipStreamSynchronize(E.st));
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->queries = E.queries() + S.queries();
+        stats->iterations = E.iterations + S.iterations;
+        stats->elapsed_ms = wall;
+        stats->wall_ms = wall;
         stats->syncs = replays;
     }
     return PT_OK;
@@ -516,7 +639,25 @@ int pt_render_batch(pt_engine* E, int32_t n, const pt_job_desc* jobs, int32_t it
     }
     E->iterations = 0;
     PT_HIP(E->ctx, hipMemsetAsync(E->queries_d, 0, sizeof(unsigned long long), E->st));
-    return run_jobs(*E, js, iterations_per_sync > 0 ? iterations_per_sync : E->iters_per_sync, stats);
+    const int iters = iterations_per_sync > 0 ? iterations_per_sync : E->iters_per_sync;
+    if (E->tune.solo_log2_samples > 0) {
+        const int64_t solo_min = (int64_t)1 << E->tune.solo_log2_samples;
+        bool any_solo = false;
+        for (const EngineJob& j : js) any_solo = any_solo || j.items >= solo_min;
+        if (any_solo) {
+            pt_context* ctx = E->ctx;
+            if (!ctx->engine || ctx->engine->capacity < kSingleJobSlots) {
+                delete ctx->engine;
+                ctx->engine = nullptr;
+                const int rc = pt_engine_create(ctx, kSingleJobSlots, 1, 0, &ctx->engine);
+                if (rc != PT_OK) return rc;
+                ctx->engine->min_job_slots = kSingleJobSlots;
+            }
+            ctx->engine->iters_per_sync = 8;
+            return run_jobs_solo(*E, *ctx->engine, js, solo_min, iters, stats);
+        }
+    }
+    return run_jobs(*E, js, iters, stats);
 }
 
 int pt_engine_set_tuning(pt_engine* E, const pt_tuning* t)
@@ -524,12 +665,15 @@ int pt_engine_set_tuning(pt_engine* E, const pt_tuning* t)
     if (!E || !t) return PT_ERR_ARG;
     if (t->iterations_per_sync < 1 || t->iterations_per_sync > 256 || t->trace_blocks_per_cu < 0 ||
         t->trace_blocks_per_cu > 8 || t->trace_chunk < 64 || t->trace_chunk > 4096 || t->shade_blocks_per_cu < 1 ||
-        t->shade_blocks_per_cu > 32)
+        t->shade_blocks_per_cu > 32 || (t->admission != PT_ADMIT_FIT && t->admission != PT_ADMIT_TAILS) ||
+        (t->solo_log2_samples != 0 && (t->solo_log2_samples < 10 || t->solo_log2_samples > 40)))
         return pt_fail(E->ctx, PT_ERR_ARG, "tuning out of range");
     E->iters_per_sync = t->iterations_per_sync;
     E->tune.trace_blocks_per_cu = t->trace_blocks_per_cu;
     E->tune.trace_chunk = t->trace_chunk;
     E->tune.shade_blocks_per_cu = t->shade_blocks_per_cu;
+    E->tune.admission = t->admission;
+    E->tune.solo_log2_samples = t->solo_log2_samples;
     E->drop_graphs(); // the launch shapes are captured in the graphs
     return PT_OK;
 }
@@ -541,6 +685,8 @@ int pt_engine_get_tuning(pt_engine* E, pt_tuning* t)
     t->trace_blocks_per_cu = E->tune.trace_blocks_per_cu;
     t->trace_chunk = E->tune.trace_chunk;
     t->shade_blocks_per_cu = E->tune.shade_blocks_per_cu;
+    t->admission = E->tune.admission;
+    t->solo_log2_samples = E->tune.solo_log2_samples;
     return PT_OK;
 }
 

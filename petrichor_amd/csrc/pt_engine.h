@@ -485,7 +485,8 @@ struct pt_engine {
         v.mj = mj;
         jobs_h[slot] = jd;
         done_h[slot] = 0;
-        cursor_h[slot] = 0;
+        // k_paths_init hands out the first items, one per slot
+        cursor_h[slot] = (unsigned long long)std::min(j.slots, j.items);
         PT_HIP(ctx, hipMemcpyAsync(jobs_d + slot, &jobs_h[slot], sizeof(pt::JobDev), hipMemcpyHostToDevice, st));
         PT_HIP(ctx, hipMemcpyAsync(views_d + slot, &jobs_h[slot].sc.bvh, sizeof(pt::BvhView), hipMemcpyHostToDevice, st));
         if (!multi) {

@@ -153,6 +153,8 @@ struct LaunchTuning {
     int trace_blocks_per_cu = 0; // 256-thread blocks per CU (0: as many as their LDS stacks allow)
     int trace_chunk = 64;        // >= 64, the wave width
     int shade_blocks_per_cu = 8;
+    int admission = 0; // PT_ADMIT_* (host-side scheduling of pt_render_batch)
+    int solo_log2_samples = 0; // jobs of at least 2^k samples render alone on the single-job engine (0: none)
 };
 // The rays of a trace launch are handed out in PT_TRACE_PARTS parts, one per XCD: part x
 // holds the x-th eighth of every segment, and the blocks that the hardware places on XCD

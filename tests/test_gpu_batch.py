@@ -90,3 +90,31 @@ def test_large_image_in_two_halves_equals_one_piece(gpu):
     pix = np.array([r * w + x for r in rows for x in range(w)], np.int32)
     ref, _ = o.render(w, h, 2, 8, 0, 21, 0, pixels=pix)
     assert rel_err(halves.reshape(-1, 3)[pix], ref).max() <= 1e-4
+
+
+@pytest.mark.parametrize("solo,admission", [(12, P.ADMIT_FIT), (0, P.ADMIT_TAILS), (14, P.ADMIT_TAILS)])
+def test_batch_scheduling_options_keep_images(gpu, solo, admission):
+    """pt_render_batch's scheduling options: jobs of at least 2^solo samples rendered alone
+    on the context's single-job engine beside the shared engine, and admission only once the
+    running jobs have handed out their samples. Every image equals its plain per-job render
+    (both integrators, two scenes)."""
+    log("batch scheduling: solo %d, admission %d" % (solo, admission))
+    c = ctx()
+    cb = c.scene(scenes.cornell_box()).build_bvh(16)
+    sp = c.scene(scenes.spheres_scene(n_spheres=8, seg=16, bands=9, terrain_n=32, env_w=64, env_h=32)).build_bvh(16)
+    jobs = [P.Job(cb, 64, 64, 4, 8, 0, 5), P.Job(sp, 80, 60, 6, 12, 0, 6), P.Job(cb, 48, 40, 3, 8, 1, 7),
+            P.Job(sp, 32, 32, 2, 12, 0, 9), P.Job(sp, 72, 40, 9, 10, 1, 3)]
+    eng = P.Engine(c, 1 << 14, max_jobs=4)
+    eng.set_tuning(solo_log2_samples=solo, admission=admission)
+    assert eng.get_tuning()["solo_log2_samples"] == solo and eng.get_tuning()["admission"] == admission
+    st = eng.render_batch(jobs)
+    q = 0
+    for j in jobs:
+        single, s1 = j.bvh.render(j.width, j.height, j.spp, j.max_bounces, j.integrator, j.seed, plain=True)
+        assert bits_equal(j.result, single)
+        q += s1["queries"]
+    assert st["queries"] == q
+    eng.render_batch(jobs)  # again on the same engines
+    for j in jobs:
+        single, _ = j.bvh.render(j.width, j.height, j.spp, j.max_bounces, j.integrator, j.seed)
+        assert bits_equal(j.result, single)
