@@ -409,7 +409,8 @@ def run_serve(args):
     geo = [scenes.scaled_scene(specs[i]["tris"], seed=5000 + i, env=env) for i in range(pool_n)]
     log("%d requests over %.0f s; %d scene geometries generated in %.1f s" % (n_req, args.duration, pool_n,
                                                                             time.perf_counter() - t0))
-    srv = P.Server(int(args.budget_gb * (1 << 30)), max_paths=1 << 22, max_jobs=128, iterations_per_sync=8)
+    srv = P.Server(int(args.budget_gb * (1 << 30)), max_paths=1 << 22, max_jobs=128, iterations_per_sync=8,
+                   solo_log2_samples=args.serve_solo)
     lock = threading.Lock()
     resident = []          # scene ids added so far (some may be evicted later)
     outstanding = {}       # request id -> (index, arrival, submission), seconds from the start
@@ -658,6 +659,9 @@ def main():
                     help="BVH width: 2 walks the binary tree in the reference's order (frame default), "
                          "8 the 8-wide tree (traversal microbenchmark default); batch mode: both are "
                          "built and the autotuner picks one unless this is given")
+    ap.add_argument("--serve-solo", type=int, default=23,
+                    help="serve mode: requests of at least 2^k samples render alone on the server's single-job "
+                         "engine beside the shared pool (0: none)")
     ap.add_argument("--tune-widths", action="store_true",
                     help="batch mode: the autotuner also tries the 4- and 8-wide trees (not the reference's walk)")
     ap.add_argument("--jobs", type=int, default=64, help="batch mode: number of jobs")

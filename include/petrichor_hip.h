@@ -366,6 +366,10 @@ typedef struct pt_server_desc {
     int64_t hbm_budget_bytes;    /* device memory for scenes, BVHs, the path pool and request images */
     int64_t max_paths;           /* path slots shared by running requests (0 = 4M) */
     int32_t iterations_per_sync; /* graph replays between host synchronisations (0 = 8) */
+    int32_t solo_log2_samples;   /* 0, or k in 10..40: a request of at least 2^k samples renders alone on
+                                    a second, single-job engine (the single-job kernels, its own stream)
+                                    beside the shared pool, one such request at a time */
+    int64_t solo_paths;          /* path slots of that engine (0 = 8M), charged to the budget */
 } pt_server_desc;
 
 #define PT_REQ_QUEUED 0

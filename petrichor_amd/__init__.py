@@ -91,7 +91,8 @@ AOV_DENOISING_ALBEDO, AOV_DENOISING_NORMAL, AOV_UV = 0, 1, 2
 
 class ServerDesc(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("max_jobs", ctypes.c_int32), ("hbm_budget_bytes", ctypes.c_int64),
-                ("max_paths", ctypes.c_int64), ("iterations_per_sync", ctypes.c_int32)]
+                ("max_paths", ctypes.c_int64), ("iterations_per_sync", ctypes.c_int32),
+                ("solo_log2_samples", ctypes.c_int32), ("solo_paths", ctypes.c_int64)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -700,9 +701,13 @@ class Server:
     """Render server (pt_server_*): resident scenes, queued requests and a request loop
     with continuous batching."""
 
-    def __init__(self, budget_bytes, device=0, max_paths=0, max_jobs=0, iterations_per_sync=0):
+    def __init__(self, budget_bytes, device=0, max_paths=0, max_jobs=0, iterations_per_sync=0, solo_log2_samples=0,
+                 solo_paths=0):
+        """solo_log2_samples = k > 0: requests of at least 2^k samples render one at a time on a
+        second, single-job engine of solo_paths path slots (0: 8M) beside the shared pool."""
         self._lib = lib()
-        d = ServerDesc(device, max_jobs, int(budget_bytes), int(max_paths), iterations_per_sync)
+        d = ServerDesc(device, max_jobs, int(budget_bytes), int(max_paths), iterations_per_sync, solo_log2_samples,
+                       int(solo_paths))
         h = ctypes.c_void_p()
         self._check(self._lib.pt_server_create(ctypes.byref(d), ctypes.byref(h)), "pt_server_create")
         self.handle = h

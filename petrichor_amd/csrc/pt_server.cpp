@@ -9,6 +9,12 @@
 // the request is done. Scenes are charged to the budget when added; idle scenes (no
 // queued or running request) are evicted least-recently-used first when a new scene
 // or a request image does not fit.
+//
+// Optionally, requests of at least 2^solo_log2_samples samples render one at a time on a
+// second, single-job engine S (the single-job kernels, with the scene as kernel arguments)
+// while the others share the multi-job engine E; each round queues both engines' graph
+// replays before waiting for either, so their launches overlap on the GPU. A request's
+// image does not depend on the engine.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -64,6 +70,7 @@ struct SRequest {
     int64_t npix = 0;
     int64_t bytes = 0; // device image, plus the preview buffer once one was asked for
     EngineJob job;
+    bool on_solo = false; // runs on the single-job engine S
     float* img_d = nullptr;
     float* prev_d = nullptr;
     std::vector<float> image;   // final image (host)
@@ -81,6 +88,9 @@ struct pt_server {
     pt_context* up = nullptr;  // scene uploads and BVH builds, under up_mu
     pt_context* run = nullptr; // the request loop's context
     pt_engine* E = nullptr;
+    pt_engine* S = nullptr;     // single-job engine of the solo requests, or null
+    SRequest* solo = nullptr;   // the request running on S
+    int64_t solo_min = 0;       // samples from which a request runs on S (0: never)
     int iters = 8;
     std::mutex mu; // guards everything below
     std::mutex up_mu;
@@ -166,10 +176,13 @@ struct pt_server {
             EngineJob probe;
             probe.npix = r->npix;
             probe.items = items;
-            const bool fits = (int64_t)running.size() < E->max_jobs && E->largest_free() >= E->job_slots(probe) &&
-                              E->width_ok(pt_engine::job_width_of(scenes.at(r->scene_id).bvh)) && make_room(bytes);
+            const bool to_solo = S && items >= solo_min;
+            const bool fits = to_solo ? (solo == nullptr && make_room(bytes))
+                                      : ((int64_t)running.size() < E->max_jobs && E->largest_free() >= E->job_slots(probe) &&
+                                         E->width_ok(pt_engine::job_width_of(scenes.at(r->scene_id).bvh)) &&
+                                         make_room(bytes));
             if (!fits) {
-                if (running.empty() && it == queue.begin() && !make_room(bytes)) {
+                if (running.empty() && !solo && it == queue.begin() && !make_room(bytes)) {
                     // nothing will free memory for it: no request runs and no scene is idle
                     fail_request(r, "request does not fit the memory budget next to the resident scenes");
                     it = queue.erase(it);
@@ -199,8 +212,10 @@ struct pt_server {
             jd.rgb_out = r->img_d;
             jd.out_on_device = 1;
             r->job = make_job(jd, r->id);
+            pt_engine* eng = to_solo ? S : E;
+            if (to_solo) r->job.slots = S->job_slots(r->job);
             int rc = prepare_output(run, r->job);
-            if (rc == PT_OK) rc = E->start(r->job);
+            if (rc == PT_OK) rc = eng->start(r->job);
             if (rc != PT_OK) {
                 fail_request(r, run->err);
                 it = queue.erase(it);
@@ -209,17 +224,24 @@ struct pt_server {
             }
             r->state = PT_REQ_RUNNING;
             r->started = Clock::now();
-            running.push_back(r);
+            r->on_solo = to_solo;
+            if (to_solo) {
+                solo = r;
+                S->live_h = (int)r->job.slots;
+            } else {
+                running.push_back(r);
+            }
             it = queue.erase(it);
             if (head_only) break;
         }
-        st.running = (int64_t)running.size();
+        st.running = (int64_t)running.size() + (solo ? 1 : 0);
         st.queued = (int64_t)queue.size();
     }
 
     // Progressive estimate of a running request into its host preview (under mu).
     int make_preview(SRequest* r)
     {
+        pt_engine* E = r->on_solo ? this->S : this->E;
         if (!r->prev_d) {
             PT_HIP(run, hipMalloc((void**)&r->prev_d, 12 * (size_t)r->npix));
             used += 12 * r->npix;
@@ -236,24 +258,63 @@ struct pt_server {
         return PT_OK;
     }
 
+    // A finished request's image to host memory; the request leaves its engine (under mu).
+    void complete(SRequest* r, pt_engine* eng)
+    {
+        r->image.resize(3 * (size_t)r->npix);
+        hipError_t e = eng->resolve(r->job) == PT_OK ? hipSuccess : hipErrorUnknown;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(r->image.data(), r->img_d, 12 * (size_t)r->npix, hipMemcpyDeviceToHost, eng->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(eng->st);
+        eng->retire(r->job);
+        if (e != hipSuccess) {
+            fail_request(r, std::string("image copy: ") + hipGetErrorString(e));
+        } else {
+            r->state = PT_REQ_DONE;
+            r->progress = 1.0f;
+            r->finished = Clock::now();
+            free_request_memory(r);
+            release_scene_user(r->scene_id);
+            st.completed += 1;
+        }
+    }
+
+    void serve_preview(SRequest* r, bool* notify)
+    {
+        if (!r->preview_wanted) return;
+        if (make_preview(r) != PT_OK) r->progress = -1.0f;
+        r->preview_wanted = false;
+        r->preview_gen += 1;
+        *notify = true;
+    }
+
     void loop()
     {
         (void)hipSetDevice(run->device);
         std::unique_lock<std::mutex> lk(mu);
         while (!stop) {
             admit();
-            if (running.empty()) {
+            if (running.empty() && !solo) {
                 cv_work.wait(lk, [&] { return stop || !queue.empty(); });
                 continue;
             }
             const auto t0 = Clock::now();
+            // only this thread changes `running` and `solo`
+            const bool e_run = !running.empty();
+            SRequest* const sr = solo;
+            // the solo request's last live paths complete in one launch
+            const bool s_last = sr && S->job_exhausted(sr->job) && S->live_h > 0 && S->live_h <= S->finish_threshold;
             lk.unlock();
-            int rc = E->run(iters);
-            const int64_t q = rc == PT_OK ? E->queries() : -1;
+            int rc = PT_OK;
+            if (sr) rc = s_last ? S->launch_finish() : S->launch(iters);
+            if (rc == PT_OK && e_run) rc = E->launch(iters);
+            if (rc == PT_OK && sr) rc = S->wait();
+            if (rc == PT_OK && e_run) rc = E->wait();
+            const int64_t q = rc == PT_OK ? E->queries() + (S ? S->queries() : 0) : -1;
             lk.lock();
             st.busy_ms += ms_between(t0, Clock::now());
             st.syncs += 1;
-            st.iterations = E->iterations;
+            st.iterations = E->iterations + (S ? S->iterations : 0);
             if (q >= 0) st.queries = q;
             if (rc != PT_OK || q < 0) {
                 const std::string why = "request loop: " + run->err;
@@ -262,43 +323,41 @@ struct pt_server {
                     fail_request(r, why);
                 }
                 running.clear();
+                if (solo) {
+                    S->retire(solo->job);
+                    fail_request(solo, why);
+                    solo = nullptr;
+                }
                 cv_done.notify_all();
                 continue;
             }
             bool notify = false;
+            if (sr) {
+                if (S->job_done(sr->job)) {
+                    complete(sr, S);
+                    solo = nullptr;
+                    notify = true;
+                } else if (s_last || S->live_h == 0) {
+                    S->retire(sr->job);
+                    fail_request(sr, "request loop: path completion left samples unfinished");
+                    solo = nullptr;
+                    notify = true;
+                } else {
+                    serve_preview(sr, &notify);
+                }
+            }
             for (size_t i = 0; i < running.size();) {
                 SRequest* r = running[i];
                 if (E->job_done(r->job)) {
-                    r->image.resize(3 * (size_t)r->npix);
-                    hipError_t e = E->resolve(r->job) == PT_OK ? hipSuccess : hipErrorUnknown;
-                    if (e == hipSuccess)
-                        e = hipMemcpyAsync(r->image.data(), r->img_d, 12 * (size_t)r->npix, hipMemcpyDeviceToHost,
-                                           E->st);
-                    if (e == hipSuccess) e = hipStreamSynchronize(E->st);
-                    E->retire(r->job);
-                    if (e != hipSuccess) {
-                        fail_request(r, std::string("image copy: ") + hipGetErrorString(e));
-                    } else {
-                        r->state = PT_REQ_DONE;
-                        r->progress = 1.0f;
-                        r->finished = Clock::now();
-                        free_request_memory(r);
-                        release_scene_user(r->scene_id);
-                        st.completed += 1;
-                    }
+                    complete(r, E);
                     running.erase(running.begin() + i);
                     notify = true;
                     continue;
                 }
-                if (r->preview_wanted) {
-                    if (make_preview(r) != PT_OK) r->progress = -1.0f;
-                    r->preview_wanted = false;
-                    r->preview_gen += 1;
-                    notify = true;
-                }
+                serve_preview(r, &notify);
                 ++i;
             }
-            st.running = (int64_t)running.size();
+            st.running = (int64_t)running.size() + (solo ? 1 : 0);
             if (notify) cv_done.notify_all();
         }
     }
@@ -334,7 +393,10 @@ int pt_server_create(const pt_server_desc* d, pt_server** out)
     const int64_t max_paths = d->max_paths > 0 ? d->max_paths : ((int64_t)4 << 20);
     const int max_jobs = d->max_jobs > 0 ? d->max_jobs : 64;
     if (d->hbm_budget_bytes <= 0) return sfail(PT_ERR_ARG, "the server needs a device-memory budget");
-    const int64_t pool = max_paths * kPathBytes + (int64_t)max_jobs * 512;
+    if (d->solo_log2_samples != 0 && (d->solo_log2_samples < 10 || d->solo_log2_samples > 40))
+        return sfail(PT_ERR_ARG, "solo_log2_samples must be 0 or 10..40");
+    const int64_t solo_paths = d->solo_log2_samples ? (d->solo_paths > 0 ? d->solo_paths : ((int64_t)8 << 20)) : 0;
+    const int64_t pool = (max_paths + solo_paths) * kPathBytes + (int64_t)max_jobs * 512;
     if (pool > d->hbm_budget_bytes) return sfail(PT_ERR_BUDGET, "the path pool alone exceeds the budget");
     auto s = std::make_unique<pt_server>();
     s->budget = d->hbm_budget_bytes;
@@ -349,9 +411,19 @@ int pt_server_create(const pt_server_desc* d, pt_server** out)
         s->E->clear_samples = true;     // progressive previews read the sample buffers
         s->E->keep_spare_samples = false; // their memory is charged to the budget per request
     }
+    if (rc == PT_OK && solo_paths > 0) {
+        rc = pt_engine_create(s->run, solo_paths, 1, 0, &s->S);
+        if (rc == PT_OK) {
+            s->S->clear_samples = true;
+            s->S->keep_spare_samples = false;
+            s->S->min_job_slots = solo_paths;
+            s->solo_min = (int64_t)1 << d->solo_log2_samples;
+        }
+    }
     if (rc == PT_OK && hipMalloc((void**)&s->samples_d, sizeof(unsigned long long)) != hipSuccess) rc = PT_ERR_OOM;
     if (rc != PT_OK) {
         const std::string why = s->run->err;
+        if (s->S) pt_engine_destroy(s->S);
         if (s->E) pt_engine_destroy(s->E);
         pt_ctx_destroy(s->up);
         pt_ctx_destroy(s->run);
@@ -378,11 +450,14 @@ int pt_server_destroy(pt_server* s)
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->run->device);
     (void)hipStreamSynchronize(s->E->st);
+    if (s->S) (void)hipStreamSynchronize(s->S->st);
     for (SRequest* r : s->running) s->E->retire(r->job);
+    if (s->solo) s->S->retire(s->solo->job);
     for (auto& kv : s->reqs) s->free_request_memory(kv.second.get());
     s->reqs.clear();
     while (!s->scenes.empty()) s->drop_scene(s->scenes.begin());
     if (s->samples_d) (void)hipFree(s->samples_d);
+    if (s->S) pt_engine_destroy(s->S);
     pt_engine_destroy(s->E);
     pt_ctx_destroy(s->up);
     pt_ctx_destroy(s->run);
@@ -569,7 +644,7 @@ int pt_server_get_stats(pt_server* s, pt_server_stats* out)
     out->bytes_in_use = s->used;
     out->budget_bytes = s->budget;
     out->queued = (int64_t)s->queue.size();
-    out->running = (int64_t)s->running.size();
+    out->running = (int64_t)s->running.size() + (s->solo ? 1 : 0);
     return PT_OK;
 }
 

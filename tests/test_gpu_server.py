@@ -37,9 +37,14 @@ def done(srv, rid):
     return info
 
 
-def test_request_alone_equals_mix_and_plain(gpu):
-    log("server: alone vs mix")
-    srv = P.Server(8 * GiB, max_paths=1 << 20, max_jobs=16, iterations_per_sync=4)
+@pytest.mark.parametrize("solo", [0, 15])
+def test_request_alone_equals_mix_and_plain(gpu, solo):
+    """A request's image alone, in a mix of requests and through pt_render are the same;
+    with solo = 15 the requests of at least 2^15 samples run one at a time on the server's
+    single-job engine beside the shared pool."""
+    log("server: alone vs mix, solo %d" % solo)
+    srv = P.Server(8 * GiB, max_paths=1 << 20, max_jobs=16, iterations_per_sync=4, solo_log2_samples=solo,
+                   solo_paths=1 << 20)
     cb, sp = scenes.cornell_box(), small_spheres()
     a_id, b_id = srv.add_scene(cb), srv.add_scene(sp)
     req_a = (128, 96, 8, 8, P.INTEGRATOR_MIS, 3)
@@ -63,9 +68,11 @@ def test_request_alone_equals_mix_and_plain(gpu):
     srv.close()
 
 
-def test_requests_join_a_running_one_and_previews_progress(gpu):
-    log("server: continuous admission and progressive results")
-    srv = P.Server(8 * GiB, max_paths=1 << 20, max_jobs=16, iterations_per_sync=2)
+@pytest.mark.parametrize("solo", [0, 20])
+def test_requests_join_a_running_one_and_previews_progress(gpu, solo):
+    log("server: continuous admission and progressive results, solo %d" % solo)
+    srv = P.Server(8 * GiB, max_paths=1 << 20, max_jobs=16, iterations_per_sync=2, solo_log2_samples=solo,
+                   solo_paths=1 << 19)
     sp = small_spheres(4)
     sid = srv.add_scene(sp)
     long_req = (512, 384, 2048, 12, P.INTEGRATOR_MIS, 11)  # 400M samples on half the pool
