@@ -168,6 +168,8 @@ def trace_bench(args):
         if bvh is not None:
             bvh.close()
         bvh = s.build_bvh(32, 4, width=width, builder=P.BUILD_DEVICE)
+        if args.threads_per_ray != 1:
+            bvh.set_trace_tuning(threads_per_ray=args.threads_per_ray)
         info = bvh.info()
         if i >= args.warmup:
             builds.append(info["build_ms"])
@@ -248,7 +250,8 @@ def trace_bench(args):
                                "closest hit" % (N, args.log2_rays),
                    "triangles": N, "rays": nr, "bvh": "binned SAH, 32 bins, leaf<=4, device builder, %s"
                                                    % ("collapsed to %d-wide nodes" % width if width != 2
-                                                      else "binary (reference topology and walk order)")},
+                                                      else "binary (reference topology and walk order)"),
+                   "threads_per_ray": args.threads_per_ray},
         "closest_mrays_s": round(value, 2),
         "any_mrays_s": round(nr * args.steps / wall_a / 1e6, 2),
         "closest_ms_per_launch": round(avg_ms, 3), "any_ms_per_launch": round(float(np.mean(ms_a)), 3),
@@ -662,6 +665,8 @@ def main():
     ap.add_argument("--serve-solo", type=int, default=23,
                     help="serve mode: requests of at least 2^k samples render alone on the server's single-job "
                          "engine beside the shared pool (0: none)")
+    ap.add_argument("--threads-per-ray", type=int, default=1, choices=[1, 2],
+                    help="frame and trace modes: lanes per ray of the binary walk (pt_bvh_set_trace_tuning)")
     ap.add_argument("--tune-widths", action="store_true",
                     help="batch mode: the autotuner also tries the 4- and 8-wide trees (not the reference's walk)")
     ap.add_argument("--jobs", type=int, default=64, help="batch mode: number of jobs")
@@ -720,6 +725,8 @@ def main():
     s = ctx.scene(sc)
     width = args.bvh_width or 2
     bvh = s.build_bvh(16, width=width)
+    if args.threads_per_ray != 1:
+        bvh.set_trace_tuning(threads_per_ray=args.threads_per_ray)
     info = bvh.info()
     log("bvh:", info)
     W, H, SPP, D = args.width, args.height, args.spp, args.depth
@@ -875,8 +882,9 @@ def main():
                    "triangles": int(len(sc["tri_mat"])), "width": W, "height": H, "spp": SPP, "max_depth": D,
                    "integrator": "MIS",
                    "bvh": "binned SAH, 16 bins, leaf<=4, device builder, " +
-                          ("collapsed to 8-wide nodes" if width == 8
+                          ("collapsed to %d-wide nodes" % width if width != 2
                            else "binary (reference topology and walk order)"),
+                   "threads_per_ray": args.threads_per_ray,
                    "render": "pt_render with PT_RENDER_TWO_STREAMS (two halves of the pixels on two streams)"},
         "queries_per_step": int(queries / args.steps),
         "iterations_per_step": int(stats[-1]["iterations"]),

@@ -335,6 +335,10 @@ typedef struct pt_tuning {
                                     time, on the context's single-job engine and stream) while the
                                     other jobs share this engine; the two streams' launches overlap.
                                     Images do not depend on it. */
+    int32_t threads_per_ray;     /* 1 (0 reads as 1), or 2: the two lanes of a pair walk one ray of a
+                                    binary BVH of a plain tree, each testing one child box or one
+                                    triangle per step (the same walk and hits). Wide walks, deep
+                                    trees and scenes with spheres keep one lane per ray. */
 } pt_tuning;
 /* pt_tuning.admission. PT_ADMIT_FIT: any pending job whose path slots are free (largest
  * first). PT_ADMIT_TAILS: a job only once every running job has handed out all its samples,
@@ -344,6 +348,10 @@ typedef struct pt_tuning {
 #define PT_ADMIT_TAILS 1
 int pt_engine_set_tuning(pt_engine* engine, const pt_tuning* tuning);
 int pt_engine_get_tuning(pt_engine* engine, pt_tuning* tuning);
+/* Launch shapes of this BVH's traces (pt_trace_* and pt_render of this BVH): its
+ * trace_blocks_per_cu, trace_chunk and threads_per_ray (the other fields are ignored). Hits
+ * and images do not depend on them. */
+int pt_bvh_set_trace_tuning(pt_bvh* bvh, const pt_tuning* tuning);
 /* Renders n jobs to completion, admitting them in order as slots free up;
  * iterations_per_sync graph replays run between host synchronisations (0 = the
  * engine's tuning). */

@@ -73,7 +73,8 @@ class BatchStats(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("iterations_per_sync", ctypes.c_int32), ("trace_blocks_per_cu", ctypes.c_int32),
                 ("trace_chunk", ctypes.c_int32), ("shade_blocks_per_cu", ctypes.c_int32),
-                ("admission", ctypes.c_int32), ("solo_log2_samples", ctypes.c_int32)]
+                ("admission", ctypes.c_int32), ("solo_log2_samples", ctypes.c_int32),
+                ("threads_per_ray", ctypes.c_int32)]
 
 
 ADMIT_FIT = 0    # pt_render_batch admits any pending job whose slots are free
@@ -146,6 +147,7 @@ def lib():
     L.pt_trace_closest.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp]
     L.pt_trace_any.argtypes = [vp, ctypes.c_int64, vp, vp, vp]
     L.pt_trace_closest_noalloc.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp]
+    L.pt_bvh_set_trace_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     L.pt_trace_closest_soa.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(RaysSoa), ctypes.POINTER(HitsSoa), vp]
     L.pt_trace_any_soa.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(RaysSoa), vp, vp]
     L.pt_trace_stats.argtypes = [vp, ctypes.c_int64, vp, vp, vp, _l]
@@ -186,7 +188,8 @@ def lib():
 # every symbol include/petrichor_hip.h declares
 EXPORTED = ["pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_scene_create", "pt_scene_destroy",
             "pt_bvh_build", "pt_bvh_destroy", "pt_bvh_export_order", "pt_bvh_export_nodes", "pt_bvh_build_host", "pt_bvh_info", "pt_trace_closest", "pt_trace_closest_host",
-            "pt_trace_any", "pt_trace_closest_noalloc", "pt_trace_closest_soa", "pt_trace_any_soa", "pt_trace_stats", "pt_render", "pt_engine_create", "pt_engine_destroy",
+            "pt_trace_any", "pt_trace_closest_noalloc", "pt_trace_closest_soa", "pt_trace_any_soa",
+            "pt_bvh_set_trace_tuning", "pt_trace_stats", "pt_render", "pt_engine_create", "pt_engine_destroy",
             "pt_render_batch", "pt_engine_set_tuning", "pt_engine_get_tuning", "pt_server_create", "pt_server_destroy", "pt_server_last_error",
             "pt_server_add_scene", "pt_server_remove_scene", "pt_server_scene_resident", "pt_server_submit",
             "pt_server_poll", "pt_server_wait", "pt_server_preview", "pt_server_fetch", "pt_server_get_stats",
@@ -475,6 +478,11 @@ class Bvh:
                                                          _ptr(rng, _i), _ptr(leaf, _i)), "pt_bvh_export_nodes")
         return {"boxes": boxes, "child": child, "range": rng, "leaf": leaf}
 
+    def set_trace_tuning(self, trace_blocks_per_cu=0, trace_chunk=64, threads_per_ray=1):
+        """pt_bvh_set_trace_tuning: launch shapes of this BVH's traces and renders."""
+        t = Tuning(8, trace_blocks_per_cu, trace_chunk, 8, 0, 0, threads_per_ray)
+        self.ctx.check(self.ctx._lib.pt_bvh_set_trace_tuning(self.handle, ctypes.byref(t)), "pt_bvh_set_trace_tuning")
+
     def trace_closest(self, rays, t_out=None, prim_out=None, stream=None):
         """rays: torch float32 CUDA tensor [n, 8]. Returns (t, prim) tensors."""
         import torch
@@ -648,11 +656,12 @@ class Engine:
         self.handle = h
 
     def set_tuning(self, iterations_per_sync=8, trace_blocks_per_cu=0, trace_chunk=64, shade_blocks_per_cu=8,
-                   admission=ADMIT_FIT, solo_log2_samples=0):
+                   admission=ADMIT_FIT, solo_log2_samples=0, threads_per_ray=1):
         """pt_engine_set_tuning. solo_log2_samples = k > 0: render_batch renders each job of
-        at least 2^k samples alone on the context's single-job engine, beside the others."""
+        at least 2^k samples alone on the context's single-job engine, beside the others.
+        threads_per_ray = 2: two lanes walk each ray of a binary BVH (the same hits)."""
         t = Tuning(iterations_per_sync, trace_blocks_per_cu, trace_chunk, shade_blocks_per_cu, admission,
-                   solo_log2_samples)
+                   solo_log2_samples, threads_per_ray)
         self.ctx.check(self.ctx._lib.pt_engine_set_tuning(self.handle, ctypes.byref(t)), "pt_engine_set_tuning")
 
     def get_tuning(self):

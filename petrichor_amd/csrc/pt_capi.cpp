@@ -785,7 +785,7 @@ int api_trace(pt_bvh* b, int64_t n, const float* rays, bool any, float* t_out, i
     }
     PT_HIP(ctx, launch_zero(b->cursor_d, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
     PT_HIP(ctx, hipMemcpyAsync(b->cursor_d + PT_CURSOR_INTS, &n32, sizeof(int), hipMemcpyHostToDevice, st));
-    PT_HIP(ctx, launch_trace_job(b->view, job, b->cursor_d, stats, st));
+    PT_HIP(ctx, launch_trace_job(b->view, job, b->cursor_d, stats, st, b->trace_tune));
     return PT_OK;
 }
 
@@ -814,7 +814,7 @@ int api_trace_noalloc(pt_bvh* b, int64_t n, const float* rays, float* t_out, int
     }
     PT_HIP(ctx, launch_zero(sc.cursor, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
     PT_HIP(ctx, launch_set(sc.cursor + PT_CURSOR_INTS, n32, st));
-    PT_HIP(ctx, launch_trace_job(b->view, job, sc.cursor, nullptr, st));
+    PT_HIP(ctx, launch_trace_job(b->view, job, sc.cursor, nullptr, st, b->trace_tune));
     return PT_OK;
 }
 
@@ -834,6 +834,20 @@ int pt_trace_closest_noalloc(pt_bvh* b, int64_t n, const float* rays, float* t_o
     PT_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     return api_trace_noalloc(b, n, rays, t_out, prim_out, st);
+}
+
+int pt_bvh_set_trace_tuning(pt_bvh* b, const pt_tuning* t)
+{
+    if (!b || !t) return PT_ERR_ARG;
+    pt_context* ctx = b->scene->ctx;
+    const int tpr = t->threads_per_ray == 0 ? 1 : t->threads_per_ray;
+    if (t->trace_blocks_per_cu < 0 || t->trace_blocks_per_cu > 8 || t->trace_chunk < 64 || t->trace_chunk > 4096 ||
+        (tpr != 1 && tpr != 2))
+        return fail(ctx, PT_ERR_ARG, "trace tuning out of range");
+    b->trace_tune.trace_blocks_per_cu = t->trace_blocks_per_cu;
+    b->trace_tune.trace_chunk = t->trace_chunk;
+    b->trace_tune.threads_per_ray = tpr;
+    return PT_OK;
 }
 
 int pt_trace_closest(pt_bvh* b, int64_t n, const float* rays, float* t_out, int32_t* prim_out, void* stream)

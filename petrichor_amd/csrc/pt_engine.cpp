@@ -499,6 +499,7 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
         }
         const bool log = (d->flags & PT_RENDER_LAUNCH_LOG) != 0;
         for (int i = 0; i < 2; ++i) {
+            E2[i]->use_trace_tuning(b->trace_tune);
             rc = E2[i]->set_logging(log, (d->flags & PT_RENDER_TRACE_STATS) != 0);
             if (rc != PT_OK) return rc;
         }
@@ -530,6 +531,7 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
         ctx->engine->min_job_slots = kSingleJobSlots;
     }
     pt_engine& E = *ctx->engine;
+    E.use_trace_tuning(b->trace_tune);
     E.iterations = 0;
     PT_HIP(ctx, hipMemsetAsync(E.queries_d, 0, sizeof(unsigned long long), E.st));
     std::vector<EngineJob> jobs(1, make_job(jd, 0));
@@ -654,6 +656,7 @@ int pt_render_batch(pt_engine* E, int32_t n, const pt_job_desc* jobs, int32_t it
                 ctx->engine->min_job_slots = kSingleJobSlots;
             }
             ctx->engine->iters_per_sync = 8;
+            ctx->engine->use_trace_tuning(E->tune);
             return run_jobs_solo(*E, *ctx->engine, js, solo_min, iters, stats);
         }
     }
@@ -666,7 +669,8 @@ int pt_engine_set_tuning(pt_engine* E, const pt_tuning* t)
     if (t->iterations_per_sync < 1 || t->iterations_per_sync > 256 || t->trace_blocks_per_cu < 0 ||
         t->trace_blocks_per_cu > 8 || t->trace_chunk < 64 || t->trace_chunk > 4096 || t->shade_blocks_per_cu < 1 ||
         t->shade_blocks_per_cu > 32 || (t->admission != PT_ADMIT_FIT && t->admission != PT_ADMIT_TAILS) ||
-        (t->solo_log2_samples != 0 && (t->solo_log2_samples < 10 || t->solo_log2_samples > 40)))
+        (t->solo_log2_samples != 0 && (t->solo_log2_samples < 10 || t->solo_log2_samples > 40)) ||
+        t->threads_per_ray < 0 || t->threads_per_ray > 2)
         return pt_fail(E->ctx, PT_ERR_ARG, "tuning out of range");
     E->iters_per_sync = t->iterations_per_sync;
     E->tune.trace_blocks_per_cu = t->trace_blocks_per_cu;
@@ -674,6 +678,7 @@ int pt_engine_set_tuning(pt_engine* E, const pt_tuning* t)
     E->tune.shade_blocks_per_cu = t->shade_blocks_per_cu;
     E->tune.admission = t->admission;
     E->tune.solo_log2_samples = t->solo_log2_samples;
+    E->tune.threads_per_ray = t->threads_per_ray == 0 ? 1 : t->threads_per_ray;
     E->drop_graphs(); // the launch shapes are captured in the graphs
     return PT_OK;
 }
@@ -687,6 +692,7 @@ int pt_engine_get_tuning(pt_engine* E, pt_tuning* t)
     t->shade_blocks_per_cu = E->tune.shade_blocks_per_cu;
     t->admission = E->tune.admission;
     t->solo_log2_samples = E->tune.solo_log2_samples;
+    t->threads_per_ray = E->tune.threads_per_ray;
     return PT_OK;
 }
 

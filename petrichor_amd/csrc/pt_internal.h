@@ -79,6 +79,7 @@ struct pt_bvh {
     int8_t* kinds_d = nullptr;    // per leaf slot: 1 for a sphere; only when the scene has spheres
     void* host_io_d = nullptr;    // pt_trace_closest_host's device staging (40 bytes per ray)
     int64_t host_io_cap = 0;
+    pt::LaunchTuning trace_tune;  // launch shapes of this BVH's traces (pt_bvh_set_trace_tuning)
     float4* soa_rays_d = nullptr; // the structure-of-arrays entry points' ray records (32 bytes per ray)
     int64_t soa_rays_cap = 0;
     int width = 2;

@@ -155,6 +155,7 @@ struct LaunchTuning {
     int shade_blocks_per_cu = 8;
     int admission = 0; // PT_ADMIT_* (host-side scheduling of pt_render_batch)
     int solo_log2_samples = 0; // jobs of at least 2^k samples render alone on the single-job engine (0: none)
+    int threads_per_ray = 1;   // 1, or 2 lanes per ray for the binary walk of plain trees (trace_step_pair)
 };
 // The rays of a trace launch are handed out in PT_TRACE_PARTS parts, one per XCD: part x
 // holds the x-th eighth of every segment, and the blocks that the hardware places on XCD

@@ -50,9 +50,12 @@ __device__ __forceinline__ void part_item(const int* seg_n, int num_seg, int x, 
     }
 }
 
-template <bool STATS, bool MULTI, int W, bool EXT>
+// TPR: lanes per ray, 1, or 2 for the binary walk of plain trees (trace_step_pair): the two
+// lanes of a pair hold the same ray and take it from the pool together.
+template <bool STATS, bool MULTI, int W, bool EXT, int TPR>
 __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int* cursor, unsigned long long* stats)
 {
+    static_assert(TPR == 1 || (TPR == 2 && W == 2 && !EXT), "two lanes per ray: binary walk of plain trees");
     // segment sizes come from device counters (written by the previous stage)
     int seg_n[3] = {0, 0, 0};
     int total = 0;
@@ -69,7 +72,10 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
     uint2* lds = lds_stack + threadIdx.x;
     StackEntry spill[stack_entries<W>() - lds_stack_entries<W>()];
     const int lane = threadIdx.x & 63;
-    const unsigned long long lane_lt = (1ull << lane) - 1ull;
+    // rays are ranked by their first lane: every lane (one lane per ray), or the even lanes
+    const unsigned long long ray_lanes = TPR == 2 ? 0x5555555555555555ull : ~0ull;
+    const unsigned long long lane_lt = ((1ull << (lane & ~(TPR - 1))) - 1ull) & ray_lanes;
+    const int sub = TPR == 2 ? (lane & 1) : 0;
     // wave-uniform: this wave's share of rays [pool_next, pool_end) in part pool_part, and
     // how many parts it has found empty
     int pool_next = 0, pool_end = 0, pool_part = (int)(blockIdx.x % PT_TRACE_PARTS), parts_empty = 0;
@@ -91,8 +97,8 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
         // once a quarter of the wave is idle (the refill's bookkeeping then runs every few
         // steps instead of whenever one ray ends)
         const bool need = item < 0;
-        const unsigned long long need_mask = __ballot(need);
-        if (__popcll(need_mask) >= 16 && !exhausted) {
+        const unsigned long long need_mask = __ballot(need) & ray_lanes;
+        if (__popcll(need_mask) >= 16 / TPR && !exhausted) {
             const int n_need = __popcll(need_mask);
             const int rank = __popcll(need_mask & lane_lt);
             const int leader = __ffsll((long long)need_mask) - 1;
@@ -133,8 +139,8 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
                     const size_t ri = sg.rays_by_path ? (size_t)sg.path[seg_i] : (size_t)seg_i;
                     const float4 r0 = sg.rays[2 * ri], r1 = sg.rays[2 * ri + 1];
                     if (!trace_begin(b, r0, r1, sg.any != 0, r, s)) {
-                        if (STATS) nv += s.nv;
-                        trace_finish(b, sg, seg_i, s);
+                        if (STATS && sub == 0) nv += s.nv;
+                        if (sub == 0) trace_finish(b, sg, seg_i, s);
                         item = -1;
                     }
                 }
@@ -152,12 +158,15 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
             continue;
         }
         if (item >= 0) {
-            if (trace_any_width_step<STATS, EXT, W>(b, r, s, lds, spill)) {
-                if (STATS) {
+            bool done;
+            if constexpr (TPR == 2) done = trace_step_pair<STATS>(b, r, s, lds, spill, sub);
+            else done = trace_any_width_step<STATS, EXT, W>(b, r, s, lds, spill);
+            if (done) {
+                if (STATS && sub == 0) {
                     nv += s.nv;
                     nt += s.nt;
                 }
-                trace_finish(b, job.seg[seg_id], seg_i, s);
+                if (sub == 0) trace_finish(b, job.seg[seg_id], seg_i, s);
                 item = -1;
             }
         }
@@ -176,12 +185,12 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
 // are all resident (8 waves per SIMD; unbounded it takes 67 and fits 7, 2.8 % slower on
 // the 64-job batch). The single-job binary walk needs 61 by itself, and the 4-wide walk
 // is held to the same 8 blocks; the 8-wide walk is LDS-bound at 5 blocks per CU.
-template <bool STATS, bool MULTI, int W, bool EXT>
+template <bool STATS, bool MULTI, int W, bool EXT, int TPR>
 __global__ void __launch_bounds__(PT_TRACE_BLOCK, W == 8 ? 1 : (W == 4 ? 6 : 8)) k_trace_persistent(BvhView b1, TraceJob job,
                                                                                     int* cursor,
                                                                                     unsigned long long* stats)
 {
-    trace_body<STATS, MULTI, W, EXT>(b1, job, cursor, stats);
+    trace_body<STATS, MULTI, W, EXT, TPR>(b1, job, cursor, stats);
 }
 
 // Ray records (o, dist_min), (d, dist_max) from structure-of-arrays inputs; a null
@@ -275,8 +284,10 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
     // the extended walk (deep trees, sphere leaves) only where a BVH needs it
     const bool ext = multi ? job.ext != 0 : (b.parents != nullptr || b.kinds != nullptr);
     const bool counted = stats || (job.launch_log && job.log_counts);
+    // two lanes per ray on request, for the binary walk of plain trees
+    const bool pair = tune.threads_per_ray == 2 && width == 2 && !ext;
 #define PT_LAUNCH_TRACE(S, M, W, E) \
-    hipLaunchKernelGGL((k_trace_persistent<S, M, W, E>), grid, block, 0, st, b, job, cursor, stats)
+    hipLaunchKernelGGL((k_trace_persistent<S, M, W, E, 1>), grid, block, 0, st, b, job, cursor, stats)
 #define PT_LAUNCH_TRACE_E(S, M, W) \
     do {                          \
         if (ext) PT_LAUNCH_TRACE(S, M, W, true); \
@@ -288,13 +299,24 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
         else if (width == 4) PT_LAUNCH_TRACE_E(S, M, 4); \
         else PT_LAUNCH_TRACE_E(S, M, 2);                 \
     } while (0)
-    if (counted) {
+#define PT_LAUNCH_PAIR(S, M) \
+    hipLaunchKernelGGL((k_trace_persistent<S, M, 2, false, 2>), grid, block, 0, st, b, job, cursor, stats)
+    if (pair) {
+        if (counted) {
+            if (multi) PT_LAUNCH_PAIR(true, true);
+            else PT_LAUNCH_PAIR(true, false);
+        } else {
+            if (multi) PT_LAUNCH_PAIR(false, true);
+            else PT_LAUNCH_PAIR(false, false);
+        }
+    } else if (counted) {
         if (multi) PT_LAUNCH_TRACE_W(true, true);
         else PT_LAUNCH_TRACE_W(true, false);
     } else {
         if (multi) PT_LAUNCH_TRACE_W(false, true);
         else PT_LAUNCH_TRACE_W(false, false);
     }
+#undef PT_LAUNCH_PAIR
 #undef PT_LAUNCH_TRACE_W
 #undef PT_LAUNCH_TRACE_E
 #undef PT_LAUNCH_TRACE

@@ -489,6 +489,109 @@ __device__ __forceinline__ bool trace_any_width_step(const BvhView& b, const Tra
     else return trace_wide_step<STATS, EXT, W>(b, r, s, lds, spill);
 }
 
+// Two lanes per ray (the binary walk of plain trees only: no deep-tree links, no spheres).
+// The two lanes of a pair (sub 0 and 1) hold the same ray, state and stack and make the
+// same decisions; what they split is the work of a step: at an inner node lane sub tests
+// child sub's box (sub 0 the left child, sub 1 the right), at a leaf the pair tests two
+// triangles at once. Each lane reads only its half of the node record, and the pair swaps
+// the results with one shuffle per value before the reference's decision (near child by
+// centroid distance, cull, strictly-nearer hits in leaf order), so the walk and every hit
+// are the one-lane walk's. Returns true when the ray is finished.
+template <bool STATS>
+__device__ __forceinline__ bool trace_step_pair(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
+                                                StackEntry* spill, int sub)
+{
+    bool next = false;
+    if (s.leaf_k < s.leaf_end) {
+        // in the middle of a leaf
+    } else if (s.best >= 0 && !(s.cur & PT_REF_INSIDE) && s.best_t < s.cur_d) {
+        // culled: a nearer hit exists and the origin is outside this box
+    } else if (pt_ref_cnt(s.cur) != 0) {
+        enter_leaf(b, s);
+    } else {
+        // node record: (llo, lhi.x), (lhi.yz, rlo.xy), (rlo.z, rhi), (ref_l, ref_r)
+        const float4* nd = b.nodes + 4 * (size_t)pt_ref_index(s.cur);
+        const float4 q0 = nd[sub], q1 = nd[sub + 1];
+        const uint32_t m = reinterpret_cast<const uint32_t*>(nd + 3)[sub];
+        const V3 lo = sub ? v3(q0.z, q0.w, q1.x) : v3(q0.x, q0.y, q0.z);
+        const V3 hi = sub ? v3(q1.y, q1.z, q1.w) : v3(q0.w, q1.x, q1.y);
+        const float dc = sq_length(0.5f * (lo + hi) - r.o);
+        float t;
+        const bool h = slab(r.o, r.inv, r.sx, r.sy, r.sz, lo, hi, &t);
+        const uint32_t e = m | (box_contains(lo, hi, r.o) ? PT_REF_INSIDE : 0u);
+        const float dc_o = __shfl_xor(dc, 1, 64), t_o = __shfl_xor(t, 1, 64);
+        const int h_o = __shfl_xor(h ? 1 : 0, 1, 64);
+        const uint32_t e_o = (uint32_t)__shfl_xor((int)e, 1, 64);
+        if (STATS) s.nv += 2;
+        const float dl = sub ? dc_o : dc, dr = sub ? dc : dc_o;
+        const bool hl = sub ? h_o != 0 : h, hr = sub ? h : h_o != 0;
+        const float tl = sub ? t_o : t, tr_ = sub ? t : t_o;
+        const uint32_t el = sub ? e_o : e, er = sub ? e : e_o;
+        // the child the reference pops first: left if dl < dr, else right
+        const bool left_first = dl < dr;
+        const bool h1 = left_first ? hl : hr, h2 = left_first ? hr : hl;
+        const uint32_t e1 = left_first ? el : er, e2 = left_first ? er : el;
+        const float d1 = left_first ? tl : tr_, d2 = left_first ? tr_ : tl;
+        if (h1) {
+            if (h2) {
+                stack_push<2>(lds, spill, s.sp, e2, d2);
+                ++s.sp;
+            }
+            s.cur = e1;
+            s.cur_d = d1;
+            next = true;
+        } else if (h2) {
+            s.cur = e2;
+            s.cur_d = d2;
+            next = true;
+        }
+    }
+    if (s.leaf_k < s.leaf_end) {
+        // two triangles of the leaf: lane sub tests slot leaf_k + sub, then both lanes take
+        // the two results in leaf order with the reference's strictly-nearer rule
+        const uint32_t k = s.leaf_k + (uint32_t)sub;
+        const bool have = k < s.leaf_end;
+        bool hit = false;
+        float t = 0.0f;
+        if (have) {
+            const float4 t0 = b.tris[3 * (size_t)k];
+            const float4 t1 = b.tris[3 * (size_t)k + 1];
+            const float4 t2 = b.tris[3 * (size_t)k + 2];
+            TriDev tr;
+            tr.p0 = v3(t0.x, t0.y, t0.z);
+            tr.e1 = v3(t0.w, t1.x, t1.y);
+            tr.e2 = v3(t1.z, t1.w, t2.x);
+            tr.n = v3(t2.y, t2.z, t2.w);
+            hit = tri_intersect(tr, r.o, r.d, &t) && !(t < r.dmin || t > r.dmax);
+        }
+        const float t_o = __shfl_xor(t, 1, 64);
+        const int hit_o = __shfl_xor(hit ? 1 : 0, 1, 64);
+        const bool have1 = s.leaf_k + 1 < s.leaf_end;
+        if (STATS) s.nt += have1 ? 2 : 1;
+        const bool hit0 = sub ? hit_o != 0 : hit, hit1 = sub ? hit : hit_o != 0;
+        const float ta = sub ? t_o : t, tb = sub ? t : t_o;
+        if (hit0 && (s.best < 0 || ta < s.best_t)) {
+            s.best = (int)s.leaf_k;
+            s.best_t = ta;
+            if (r.any) return true;
+        }
+        if (have1 && hit1 && (s.best < 0 || tb < s.best_t)) {
+            s.best = (int)s.leaf_k + 1;
+            s.best_t = tb;
+            if (r.any) return true;
+        }
+        s.leaf_k = have1 ? s.leaf_k + 2 : s.leaf_end;
+        if (s.leaf_k < s.leaf_end) return false;
+        next = false;
+    }
+    if (!next) {
+        if (s.sp == 0) return true;
+        --s.sp;
+        stack_pop<2>(lds, spill, s.sp, &s.cur, &s.cur_d);
+    }
+    return false;
+}
+
 __device__ __forceinline__ void trace_finish(const BvhView& b, const TraceSegment& seg, int i, const TraceState& s)
 {
     const int p = seg.path ? seg.path[i] : i;
