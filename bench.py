@@ -168,8 +168,8 @@ def trace_bench(args):
         if bvh is not None:
             bvh.close()
         bvh = s.build_bvh(32, 4, width=width, builder=P.BUILD_DEVICE)
-        if args.threads_per_ray != 1:
-            bvh.set_trace_tuning(threads_per_ray=args.threads_per_ray)
+        if args.threads_per_ray != 1 or args.leaf_batch:
+            bvh.set_trace_tuning(threads_per_ray=args.threads_per_ray, leaf_batch=args.leaf_batch)
         info = bvh.info()
         if i >= args.warmup:
             builds.append(info["build_ms"])
@@ -251,7 +251,7 @@ def trace_bench(args):
                    "triangles": N, "rays": nr, "bvh": "binned SAH, 32 bins, leaf<=4, device builder, %s"
                                                    % ("collapsed to %d-wide nodes" % width if width != 2
                                                       else "binary (reference topology and walk order)"),
-                   "threads_per_ray": args.threads_per_ray},
+                   "threads_per_ray": args.threads_per_ray, "leaf_batch": args.leaf_batch or 3},
         "closest_mrays_s": round(value, 2),
         "any_mrays_s": round(nr * args.steps / wall_a / 1e6, 2),
         "closest_ms_per_launch": round(avg_ms, 3), "any_ms_per_launch": round(float(np.mean(ms_a)), 3),
@@ -667,6 +667,11 @@ def main():
                          "engine beside the shared pool (0: none)")
     ap.add_argument("--threads-per-ray", type=int, default=1, choices=[1, 2],
                     help="frame and trace modes: lanes per ray of the binary walk (pt_bvh_set_trace_tuning)")
+    ap.add_argument("--trace-blocks-per-cu", type=int, default=0, choices=range(0, 9),
+                    help="frame mode: persistent trace blocks per CU (0: as many as the LDS stacks allow)")
+    ap.add_argument("--leaf-batch", type=int, default=0, choices=[-1, 0, 1, 2, 3, 4],
+                    help="trace/frame modes: phased binary walk, triangle steps once 1/2^(k-1) of a wave's rays "
+                         "are in a leaf (0: the library's default, 3; -1: off)")
     ap.add_argument("--tune-widths", action="store_true",
                     help="batch mode: the autotuner also tries the 4- and 8-wide trees (not the reference's walk)")
     ap.add_argument("--jobs", type=int, default=64, help="batch mode: number of jobs")
@@ -725,8 +730,9 @@ def main():
     s = ctx.scene(sc)
     width = args.bvh_width or 2
     bvh = s.build_bvh(16, width=width)
-    if args.threads_per_ray != 1:
-        bvh.set_trace_tuning(threads_per_ray=args.threads_per_ray)
+    if args.threads_per_ray != 1 or args.leaf_batch or args.trace_blocks_per_cu:
+        bvh.set_trace_tuning(trace_blocks_per_cu=args.trace_blocks_per_cu, threads_per_ray=args.threads_per_ray,
+                             leaf_batch=args.leaf_batch)
     info = bvh.info()
     log("bvh:", info)
     W, H, SPP, D = args.width, args.height, args.spp, args.depth
@@ -884,7 +890,8 @@ def main():
                    "bvh": "binned SAH, 16 bins, leaf<=4, device builder, " +
                           ("collapsed to %d-wide nodes" % width if width != 2
                            else "binary (reference topology and walk order)"),
-                   "threads_per_ray": args.threads_per_ray,
+                   "threads_per_ray": args.threads_per_ray, "leaf_batch": args.leaf_batch or 3,
+                   "trace_blocks_per_cu": args.trace_blocks_per_cu or 8,
                    "render": "pt_render with PT_RENDER_TWO_STREAMS (two halves of the pixels on two streams)"},
         "queries_per_step": int(queries / args.steps),
         "iterations_per_step": int(stats[-1]["iterations"]),

@@ -339,6 +339,11 @@ typedef struct pt_tuning {
                                     binary BVH of a plain tree, each testing one child box or one
                                     triangle per step (the same walk and hits). Wide walks, deep
                                     trees and scenes with spheres keep one lane per ray. */
+    int32_t leaf_batch;          /* k in 1..4 (0 reads as 3), or -1: the one-lane binary walk of a
+                                    plain tree runs in wave-wide phases, node steps until at least
+                                    1/2^(k-1) of a wave's rays are in a leaf, then triangle steps (the
+                                    same walk and hits per ray; a wave runs one kind of step at a
+                                    time); -1: every step runs both kinds. */
 } pt_tuning;
 /* pt_tuning.admission. PT_ADMIT_FIT: any pending job whose path slots are free (largest
  * first). PT_ADMIT_TAILS: a job only once every running job has handed out all its samples,
@@ -349,7 +354,7 @@ typedef struct pt_tuning {
 int pt_engine_set_tuning(pt_engine* engine, const pt_tuning* tuning);
 int pt_engine_get_tuning(pt_engine* engine, pt_tuning* tuning);
 /* Launch shapes of this BVH's traces (pt_trace_* and pt_render of this BVH): its
- * trace_blocks_per_cu, trace_chunk and threads_per_ray (the other fields are ignored). Hits
+ * trace_blocks_per_cu, trace_chunk, threads_per_ray and leaf_batch (the other fields are ignored). Hits
  * and images do not depend on them. */
 int pt_bvh_set_trace_tuning(pt_bvh* bvh, const pt_tuning* tuning);
 /* Renders n jobs to completion, admitting them in order as slots free up;

@@ -74,7 +74,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [("iterations_per_sync", ctypes.c_int32), ("trace_blocks_per_cu", ctypes.c_int32),
                 ("trace_chunk", ctypes.c_int32), ("shade_blocks_per_cu", ctypes.c_int32),
                 ("admission", ctypes.c_int32), ("solo_log2_samples", ctypes.c_int32),
-                ("threads_per_ray", ctypes.c_int32)]
+                ("threads_per_ray", ctypes.c_int32), ("leaf_batch", ctypes.c_int32)]
 
 
 ADMIT_FIT = 0    # pt_render_batch admits any pending job whose slots are free
@@ -478,9 +478,9 @@ class Bvh:
                                                          _ptr(rng, _i), _ptr(leaf, _i)), "pt_bvh_export_nodes")
         return {"boxes": boxes, "child": child, "range": rng, "leaf": leaf}
 
-    def set_trace_tuning(self, trace_blocks_per_cu=0, trace_chunk=64, threads_per_ray=1):
+    def set_trace_tuning(self, trace_blocks_per_cu=0, trace_chunk=64, threads_per_ray=1, leaf_batch=0):
         """pt_bvh_set_trace_tuning: launch shapes of this BVH's traces and renders."""
-        t = Tuning(8, trace_blocks_per_cu, trace_chunk, 8, 0, 0, threads_per_ray)
+        t = Tuning(8, trace_blocks_per_cu, trace_chunk, 8, 0, 0, threads_per_ray, leaf_batch)
         self.ctx.check(self.ctx._lib.pt_bvh_set_trace_tuning(self.handle, ctypes.byref(t)), "pt_bvh_set_trace_tuning")
 
     def trace_closest(self, rays, t_out=None, prim_out=None, stream=None):
@@ -656,12 +656,14 @@ class Engine:
         self.handle = h
 
     def set_tuning(self, iterations_per_sync=8, trace_blocks_per_cu=0, trace_chunk=64, shade_blocks_per_cu=8,
-                   admission=ADMIT_FIT, solo_log2_samples=0, threads_per_ray=1):
+                   admission=ADMIT_FIT, solo_log2_samples=0, threads_per_ray=1, leaf_batch=0):
         """pt_engine_set_tuning. solo_log2_samples = k > 0: render_batch renders each job of
         at least 2^k samples alone on the context's single-job engine, beside the others.
-        threads_per_ray = 2: two lanes walk each ray of a binary BVH (the same hits)."""
+        threads_per_ray = 2: two lanes walk each ray of a binary BVH (the same hits).
+        leaf_batch = k in 1..4 (0: the default, 3): the binary walk runs in wave-wide node
+        and triangle phases; -1: every step runs both (the same hits either way)."""
         t = Tuning(iterations_per_sync, trace_blocks_per_cu, trace_chunk, shade_blocks_per_cu, admission,
-                   solo_log2_samples, threads_per_ray)
+                   solo_log2_samples, threads_per_ray, leaf_batch)
         self.ctx.check(self.ctx._lib.pt_engine_set_tuning(self.handle, ctypes.byref(t)), "pt_engine_set_tuning")
 
     def get_tuning(self):

@@ -2,7 +2,7 @@
 
 Chooses, for a given set of render jobs, the BVH node width (when the jobs are given over
 BVHs of several widths: 2, 4, 8), the path-pool size (the capacity of every ray queue), the graph
-replays per host synchronisation, the lanes per ray of the binary walk and the launch
+replays per host synchronisation, the lanes per ray and the phases of the binary walk and the launch
 shapes of the trace and shading kernels (pt_tuning). It times pt_render_batch on a calibration batch - the same scenes and
 resolutions at a few samples per pixel - first for each width and then for each pool size
 with the default launch shapes, then with coordinate descent over the space below on the
@@ -21,7 +21,7 @@ when the caller accepts that (exact_only=False).
 import time
 
 DEFAULT = {"iterations_per_sync": 8, "trace_blocks_per_cu": 0, "trace_chunk": 64, "shade_blocks_per_cu": 8,
-           "threads_per_ray": 1}
+           "threads_per_ray": 1, "leaf_batch": 3}
 # Thresholds tried for rendering the largest jobs alone on the single-job engine, in log2
 # samples (0: every job in the shared engine). They are timed on the full batch, since the
 # calibration batch's fewer samples per pixel would move every job across them.
@@ -30,6 +30,8 @@ SOLO = (0, 23, 25, 27)
 SPACE = {
     # lanes per ray of the binary walk (the same walk and hits either way)
     "threads_per_ray": [1, 2],
+    # phased binary walk: triangle steps once 1/2^(k-1) of a wave's rays are in a leaf (-1: off)
+    "leaf_batch": [-1, 2, 3, 4],
     "iterations_per_sync": [2, 4, 8, 16, 32],
     "trace_blocks_per_cu": [2, 3, 4, 6],  # 0 (default): as many as the LDS stacks allow
     "trace_chunk": [64, 128, 256, 512],

@@ -842,11 +842,12 @@ int pt_bvh_set_trace_tuning(pt_bvh* b, const pt_tuning* t)
     pt_context* ctx = b->scene->ctx;
     const int tpr = t->threads_per_ray == 0 ? 1 : t->threads_per_ray;
     if (t->trace_blocks_per_cu < 0 || t->trace_blocks_per_cu > 8 || t->trace_chunk < 64 || t->trace_chunk > 4096 ||
-        (tpr != 1 && tpr != 2))
+        (tpr != 1 && tpr != 2) || t->leaf_batch < -1 || t->leaf_batch > 4)
         return fail(ctx, PT_ERR_ARG, "trace tuning out of range");
     b->trace_tune.trace_blocks_per_cu = t->trace_blocks_per_cu;
     b->trace_tune.trace_chunk = t->trace_chunk;
     b->trace_tune.threads_per_ray = tpr;
+    b->trace_tune.leaf_batch = t->leaf_batch == 0 ? pt::LaunchTuning{}.leaf_batch : t->leaf_batch;
     return PT_OK;
 }
 

@@ -670,7 +670,7 @@ int pt_engine_set_tuning(pt_engine* E, const pt_tuning* t)
         t->trace_blocks_per_cu > 8 || t->trace_chunk < 64 || t->trace_chunk > 4096 || t->shade_blocks_per_cu < 1 ||
         t->shade_blocks_per_cu > 32 || (t->admission != PT_ADMIT_FIT && t->admission != PT_ADMIT_TAILS) ||
         (t->solo_log2_samples != 0 && (t->solo_log2_samples < 10 || t->solo_log2_samples > 40)) ||
-        t->threads_per_ray < 0 || t->threads_per_ray > 2)
+        t->threads_per_ray < 0 || t->threads_per_ray > 2 || t->leaf_batch < -1 || t->leaf_batch > 4)
         return pt_fail(E->ctx, PT_ERR_ARG, "tuning out of range");
     E->iters_per_sync = t->iterations_per_sync;
     E->tune.trace_blocks_per_cu = t->trace_blocks_per_cu;
@@ -679,6 +679,7 @@ int pt_engine_set_tuning(pt_engine* E, const pt_tuning* t)
     E->tune.admission = t->admission;
     E->tune.solo_log2_samples = t->solo_log2_samples;
     E->tune.threads_per_ray = t->threads_per_ray == 0 ? 1 : t->threads_per_ray;
+    E->tune.leaf_batch = t->leaf_batch == 0 ? pt::LaunchTuning{}.leaf_batch : t->leaf_batch;
     E->drop_graphs(); // the launch shapes are captured in the graphs
     return PT_OK;
 }
@@ -693,6 +694,7 @@ int pt_engine_get_tuning(pt_engine* E, pt_tuning* t)
     t->admission = E->tune.admission;
     t->solo_log2_samples = E->tune.solo_log2_samples;
     t->threads_per_ray = E->tune.threads_per_ray;
+    t->leaf_batch = E->tune.leaf_batch;
     return PT_OK;
 }
 

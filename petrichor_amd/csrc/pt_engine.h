@@ -169,16 +169,17 @@ struct pt_engine {
     static constexpr int kFinishLogCap = 64;
 
     // Turns the launch log on (clearing it) or off; the graphs are captured again.
-    // The trace launch shapes of t (blocks per CU, chunk, lanes per ray); the graphs are
-    // captured again when they change.
+    // The trace launch shapes of t (blocks per CU, chunk, lanes per ray, phased steps); the
+    // graphs are captured again when they change.
     void use_trace_tuning(const pt::LaunchTuning& t)
     {
         if (tune.trace_blocks_per_cu == t.trace_blocks_per_cu && tune.trace_chunk == t.trace_chunk &&
-            tune.threads_per_ray == t.threads_per_ray)
+            tune.threads_per_ray == t.threads_per_ray && tune.leaf_batch == t.leaf_batch)
             return;
         tune.trace_blocks_per_cu = t.trace_blocks_per_cu;
         tune.trace_chunk = t.trace_chunk;
         tune.threads_per_ray = t.threads_per_ray;
+        tune.leaf_batch = t.leaf_batch;
         drop_graphs();
     }
 

@@ -140,6 +140,7 @@ struct TraceJob {
     int chunk = 64;              // rays a wave takes from the cursor at a time (>= 64; set by the launcher)
     int width = 2;               // node width of the walk (every BVH of a multi-job launch has the same)
     int ext = 0;                 // multi-job launches: some BVH has a deep tree or spheres (the extended walk)
+    int leaf_shift = 0;          // phased binary walk: a triangle step runs once (leaf rays << leaf_shift) >= rays
     // optional launch log: the launch writes its span (and with log_counts its node visits
     // and triangle tests) into launch_log[*launch_index]; launch_log_tick then records the
     // ray counts and advances the index
@@ -156,6 +157,8 @@ struct LaunchTuning {
     int admission = 0; // PT_ADMIT_* (host-side scheduling of pt_render_batch)
     int solo_log2_samples = 0; // jobs of at least 2^k samples render alone on the single-job engine (0: none)
     int threads_per_ray = 1;   // 1, or 2 lanes per ray for the binary walk of plain trees (trace_step_pair)
+    int leaf_batch = 3;        // -1 (off), or k in 1..4: phased binary walk (trace_step_phase), triangle
+                               // steps once 1/2^(k-1) of a wave's rays are in a leaf
 };
 // The rays of a trace launch are handed out in PT_TRACE_PARTS parts, one per XCD: part x
 // holds the x-th eighth of every segment, and the blocks that the hardware places on XCD

@@ -52,10 +52,11 @@ __device__ __forceinline__ void part_item(const int* seg_n, int num_seg, int x, 
 
 // TPR: lanes per ray, 1, or 2 for the binary walk of plain trees (trace_step_pair): the two
 // lanes of a pair hold the same ray and take it from the pool together.
-template <bool STATS, bool MULTI, int W, bool EXT, int TPR>
+template <bool STATS, bool MULTI, int W, bool EXT, int TPR, bool PH>
 __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int* cursor, unsigned long long* stats)
 {
     static_assert(TPR == 1 || (TPR == 2 && W == 2 && !EXT), "two lanes per ray: binary walk of plain trees");
+    static_assert(!PH || (TPR == 1 && W == 2 && !EXT), "phased steps: binary walk of plain trees");
     // segment sizes come from device counters (written by the previous stage)
     int seg_n[3] = {0, 0, 0};
     int total = 0;
@@ -153,13 +154,21 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
                 pool_end = end2;
             }
         }
-        if (__ballot(item >= 0) == 0ull) {
+        const unsigned long long busy = __ballot(item >= 0);
+        if (busy == 0ull) {
             if (exhausted) break; // no ray left anywhere for this wave
             continue;
+        }
+        // phased walk: a triangle step once enough of the wave's rays are in a leaf
+        bool tri_phase = false;
+        if constexpr (PH) {
+            const unsigned long long in_leaf = __ballot(item >= 0 && s.leaf_k < s.leaf_end);
+            tri_phase = (__popcll(in_leaf) << job.leaf_shift) >= __popcll(busy);
         }
         if (item >= 0) {
             bool done;
             if constexpr (TPR == 2) done = trace_step_pair<STATS>(b, r, s, lds, spill, sub);
+            else if constexpr (PH) done = trace_step_phase<STATS>(b, r, s, lds, spill, tri_phase);
             else done = trace_any_width_step<STATS, EXT, W>(b, r, s, lds, spill);
             if (done) {
                 if (STATS && sub == 0) {
@@ -185,12 +194,12 @@ __device__ __forceinline__ void trace_body(BvhView b1, const TraceJob& job, int*
 // are all resident (8 waves per SIMD; unbounded it takes 67 and fits 7, 2.8 % slower on
 // the 64-job batch). The single-job binary walk needs 61 by itself, and the 4-wide walk
 // is held to the same 8 blocks; the 8-wide walk is LDS-bound at 5 blocks per CU.
-template <bool STATS, bool MULTI, int W, bool EXT, int TPR>
+template <bool STATS, bool MULTI, int W, bool EXT, int TPR, bool PH>
 __global__ void __launch_bounds__(PT_TRACE_BLOCK, W == 8 ? 1 : (W == 4 ? 6 : 8)) k_trace_persistent(BvhView b1, TraceJob job,
                                                                                     int* cursor,
                                                                                     unsigned long long* stats)
 {
-    trace_body<STATS, MULTI, W, EXT, TPR>(b1, job, cursor, stats);
+    trace_body<STATS, MULTI, W, EXT, TPR, PH>(b1, job, cursor, stats);
 }
 
 // Ray records (o, dist_min), (d, dist_max) from structure-of-arrays inputs; a null
@@ -286,8 +295,11 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
     const bool counted = stats || (job.launch_log && job.log_counts);
     // two lanes per ray on request, for the binary walk of plain trees
     const bool pair = tune.threads_per_ray == 2 && width == 2 && !ext;
+    // phased steps on request, for the one-lane binary walk of plain trees
+    const bool phased = !pair && tune.leaf_batch >= 1 && width == 2 && !ext;
+    job.leaf_shift = phased ? (tune.leaf_batch > 4 ? 3 : tune.leaf_batch - 1) : 0;
 #define PT_LAUNCH_TRACE(S, M, W, E) \
-    hipLaunchKernelGGL((k_trace_persistent<S, M, W, E, 1>), grid, block, 0, st, b, job, cursor, stats)
+    hipLaunchKernelGGL((k_trace_persistent<S, M, W, E, 1, false>), grid, block, 0, st, b, job, cursor, stats)
 #define PT_LAUNCH_TRACE_E(S, M, W) \
     do {                          \
         if (ext) PT_LAUNCH_TRACE(S, M, W, true); \
@@ -300,7 +312,9 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
         else PT_LAUNCH_TRACE_E(S, M, 2);                 \
     } while (0)
 #define PT_LAUNCH_PAIR(S, M) \
-    hipLaunchKernelGGL((k_trace_persistent<S, M, 2, false, 2>), grid, block, 0, st, b, job, cursor, stats)
+    hipLaunchKernelGGL((k_trace_persistent<S, M, 2, false, 2, false>), grid, block, 0, st, b, job, cursor, stats)
+#define PT_LAUNCH_PHASED(S, M) \
+    hipLaunchKernelGGL((k_trace_persistent<S, M, 2, false, 1, true>), grid, block, 0, st, b, job, cursor, stats)
     if (pair) {
         if (counted) {
             if (multi) PT_LAUNCH_PAIR(true, true);
@@ -309,6 +323,14 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
             if (multi) PT_LAUNCH_PAIR(false, true);
             else PT_LAUNCH_PAIR(false, false);
         }
+    } else if (phased) {
+        if (counted) {
+            if (multi) PT_LAUNCH_PHASED(true, true);
+            else PT_LAUNCH_PHASED(true, false);
+        } else {
+            if (multi) PT_LAUNCH_PHASED(false, true);
+            else PT_LAUNCH_PHASED(false, false);
+        }
     } else if (counted) {
         if (multi) PT_LAUNCH_TRACE_W(true, true);
         else PT_LAUNCH_TRACE_W(true, false);
@@ -316,6 +338,7 @@ hipError_t launch_trace_job(const BvhView& b, const TraceJob& job_in, int* curso
         if (multi) PT_LAUNCH_TRACE_W(false, true);
         else PT_LAUNCH_TRACE_W(false, false);
     }
+#undef PT_LAUNCH_PHASED
 #undef PT_LAUNCH_PAIR
 #undef PT_LAUNCH_TRACE_W
 #undef PT_LAUNCH_TRACE_E

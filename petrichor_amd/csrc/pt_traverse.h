@@ -287,6 +287,60 @@ __device__ __forceinline__ bool leaf_or_pop(const BvhView& b, const TraceRay& r,
     return false;
 }
 
+// The inner node s.cur of the binary walk: its two child boxes are tested and the ray
+// moves to the child the reference visits first (the other one, when hit, is pushed).
+// Returns true when it moved to a child; false when it hit neither (the entry is done).
+template <bool STATS, bool EXT>
+__device__ __forceinline__ bool node_step(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
+                                          StackEntry* spill)
+{
+    const float4* nd = b.nodes + 4 * (size_t)pt_ref_index(s.cur);
+    const float4 a = nd[0], bb = nd[1], c = nd[2], mf = nd[3];
+    const uint32_t ml = __float_as_uint(mf.x), mr = __float_as_uint(mf.y);
+    const V3 llo = v3(a.x, a.y, a.z), lhi = v3(a.w, bb.x, bb.y);
+    const V3 rlo = v3(bb.z, bb.w, c.x), rhi = v3(c.y, c.z, c.w);
+    if (STATS) s.nv += 2;
+    const float dl = sq_length(0.5f * (llo + lhi) - r.o);
+    const float dr = sq_length(0.5f * (rlo + rhi) - r.o);
+    float tl, tr_;
+    const bool hl = slab(r.o, r.inv, r.sx, r.sy, r.sz, llo, lhi, &tl);
+    const bool hr = slab(r.o, r.inv, r.sx, r.sy, r.sz, rlo, rhi, &tr_);
+    const uint32_t el = ml | (box_contains(llo, lhi, r.o) ? PT_REF_INSIDE : 0u);
+    const uint32_t er = mr | (box_contains(rlo, rhi, r.o) ? PT_REF_INSIDE : 0u);
+    // the child the reference pops first: left if dl < dr, else right
+    const bool left_first = dl < dr;
+    const bool h1 = left_first ? hl : hr, h2 = left_first ? hr : hl;
+    const uint32_t e1 = left_first ? el : er, e2 = left_first ? er : el;
+    const float d1 = left_first ? tl : tr_, d2 = left_first ? tr_ : tl;
+    if (EXT && b.parents) {
+        if (h1 || h2) {
+            lds[0].x = pt_ref_index(s.cur); // deep tree: the parent of the child entered
+            s.cur = h1 ? e1 : e2;
+            s.cur_d = h1 ? d1 : d2;
+            return true;
+        }
+    } else if (h1) {
+        if (h2) {
+            stack_push<2>(lds, spill, s.sp, e2, d2);
+            ++s.sp;
+        }
+        s.cur = e1;
+        s.cur_d = d1;
+        return true;
+    } else if (h2) {
+        s.cur = e2;
+        s.cur_d = d2;
+        return true;
+    }
+    return false;
+}
+
+// Whether the entry s.cur is culled: a nearer hit exists and the origin is outside its box.
+__device__ __forceinline__ bool entry_culled(const TraceState& s)
+{
+    return s.best >= 0 && !(s.cur & PT_REF_INSIDE) && s.best_t < s.cur_d;
+}
+
 // One traversal step: an inner node or triangles of a leaf, then the next entry from the
 // stack when the current one is done. Returns true when the ray is finished. EXT: the walk
 // handles trees deeper than the stack and sphere leaves (BvhView::parents, ::kinds); the
@@ -298,51 +352,49 @@ __device__ __forceinline__ bool trace_step(const BvhView& b, const TraceRay& r, 
     bool next = false;
     if (s.leaf_k < s.leaf_end) {
         // in the middle of a leaf
-    } else if (s.best >= 0 && !(s.cur & PT_REF_INSIDE) && s.best_t < s.cur_d) {
-        // culled: a nearer hit exists and the origin is outside this box
+    } else if (entry_culled(s)) {
+        // culled
     } else if (pt_ref_cnt(s.cur) != 0) {
         enter_leaf(b, s);
     } else {
-        const float4* nd = b.nodes + 4 * (size_t)pt_ref_index(s.cur);
-        const float4 a = nd[0], bb = nd[1], c = nd[2], mf = nd[3];
-        const uint32_t ml = __float_as_uint(mf.x), mr = __float_as_uint(mf.y);
-        const V3 llo = v3(a.x, a.y, a.z), lhi = v3(a.w, bb.x, bb.y);
-        const V3 rlo = v3(bb.z, bb.w, c.x), rhi = v3(c.y, c.z, c.w);
-        if (STATS) s.nv += 2;
-        const float dl = sq_length(0.5f * (llo + lhi) - r.o);
-        const float dr = sq_length(0.5f * (rlo + rhi) - r.o);
-        float tl, tr_;
-        const bool hl = slab(r.o, r.inv, r.sx, r.sy, r.sz, llo, lhi, &tl);
-        const bool hr = slab(r.o, r.inv, r.sx, r.sy, r.sz, rlo, rhi, &tr_);
-        const uint32_t el = ml | (box_contains(llo, lhi, r.o) ? PT_REF_INSIDE : 0u);
-        const uint32_t er = mr | (box_contains(rlo, rhi, r.o) ? PT_REF_INSIDE : 0u);
-        // the child the reference pops first: left if dl < dr, else right
-        const bool left_first = dl < dr;
-        const bool h1 = left_first ? hl : hr, h2 = left_first ? hr : hl;
-        const uint32_t e1 = left_first ? el : er, e2 = left_first ? er : el;
-        const float d1 = left_first ? tl : tr_, d2 = left_first ? tr_ : tl;
-        if (EXT && b.parents) {
-            if (h1 || h2) {
-                lds[0].x = pt_ref_index(s.cur); // deep tree: the parent of the child entered
-                s.cur = h1 ? e1 : e2;
-                s.cur_d = h1 ? d1 : d2;
-                next = true;
-            }
-        } else if (h1) {
-            if (h2) {
-                stack_push<2>(lds, spill, s.sp, e2, d2);
-                ++s.sp;
-            }
-            s.cur = e1;
-            s.cur_d = d1;
-            next = true;
-        } else if (h2) {
-            s.cur = e2;
-            s.cur_d = d2;
-            next = true;
-        }
+        next = node_step<STATS, EXT>(b, r, s, lds, spill);
     }
     return leaf_or_pop<STATS, 2, EXT>(b, r, s, lds, spill, next);
+}
+
+// One step of the binary walk of a plain tree in one of two wave-wide phases. In a node
+// phase (tri_phase false) the rays that are not in a leaf take their next entry: an inner
+// node is tested, a culled entry is dropped, a leaf is entered (its triangles wait for a
+// triangle phase); rays in a leaf wait. In a triangle phase the rays in a leaf test their
+// next triangle and the others wait. Each ray still makes the moves of trace_step in the
+// same order, so its hits are the same; a wave only runs the code of one kind of move per
+// step, where trace_step runs both whenever its rays are at both. Returns true when the
+// ray is finished.
+template <bool STATS>
+__device__ __forceinline__ bool trace_step_phase(const BvhView& b, const TraceRay& r, TraceState& s, uint2* lds,
+                                                 StackEntry* spill, bool tri_phase)
+{
+    const bool in_leaf = s.leaf_k < s.leaf_end;
+    if (tri_phase) {
+        if (!in_leaf) return false;
+        if (tri_test<STATS, 2, false>(b, r, s, s.leaf_k++)) return true;
+        if (s.leaf_k < s.leaf_end) return false;
+    } else {
+        if (in_leaf) return false;
+        if (entry_culled(s)) {
+            // culled
+        } else if (pt_ref_cnt(s.cur) != 0) {
+            enter_leaf(b, s);
+            if (s.leaf_k < s.leaf_end) return false;
+        } else if (node_step<STATS, false>(b, r, s, lds, spill)) {
+            return false;
+        }
+    }
+    // the entry is done: the next one from the stack
+    if (s.sp == 0) return true;
+    --s.sp;
+    stack_pop<2>(lds, spill, s.sp, &s.cur, &s.cur_d);
+    return false;
 }
 
 __device__ __forceinline__ void sort_pair(float& da, uint32_t& ra, float& db, uint32_t& rb)

@@ -186,16 +186,26 @@ def torch_equal(a, b):
     return bool(torch.equal(a, b))
 
 
+# Walk variants selected by pt_bvh_set_trace_tuning / pt_engine_set_tuning that must
+# give the same hits as the one-lane walk: two lanes per ray, and the phased walk at
+# each threshold.
+WALKS = {"pair": {"threads_per_ray": 2}, "phased1": {"leaf_batch": 1}, "phased2": {"leaf_batch": 2},
+         "phased3": {"leaf_batch": 3}, "phased4": {"leaf_batch": 4}}
+ONE_STEP = {"leaf_batch": -1}  # the one-lane walk whose steps run both kinds of move
+
+
+@pytest.mark.parametrize("walk", list(WALKS))
 @pytest.mark.parametrize("name", list(SCENES))
-def test_two_lanes_per_ray_trace_equals_oracle(gpu, name):
-    """threads_per_ray = 2 (pt_bvh_set_trace_tuning): the lanes of a pair split each step's
-    two child boxes or two leaf triangles and take the reference's decision together, so
+def test_walk_variants_trace_equals_oracle(gpu, name, walk):
+    """threads_per_ray = 2: the lanes of a pair split each step's two child boxes or two leaf
+    triangles and take the reference's decision together. leaf_batch = k: a wave runs node
+    steps and triangle steps in turn, each ray making its own moves in order. Either way
     hits, distances, any-hit flags and the node-visit and triangle-test counts equal the
     oracle's walk; also through the coherence-ordered path of a launch of 2^20 rays."""
-    log("two lanes per ray", name)
+    log("walk variant", walk, name)
     sc = SCENES[name]()
     b = ctx().scene(sc).build_bvh(16)
-    b.set_trace_tuning(threads_per_ray=2)
+    b.set_trace_tuning(**WALKS[walk])
     o = O.OracleScene(sc).use_tree(b.export_nodes() | {"prim_ids": b.export_order()})
     r = rays_into(sc, 20000, 5)
     r[::3, 3] = 0.1
@@ -206,32 +216,34 @@ def test_two_lanes_per_ray_trace_equals_oracle(gpu, name):
     assert np.array_equal(b.trace_any(to_dev(r)).cpu().numpy(), (po >= 0).astype(np.int32))
     big = np.concatenate([r] * 53)[: (1 << 20) + 7]
     t2, p2 = b.trace_closest(to_dev(big))
-    b.set_trace_tuning(threads_per_ray=1)
+    b.set_trace_tuning(**ONE_STEP)
     t1, p1 = b.trace_closest(to_dev(big))
     assert torch_equal(p2, p1) and bits_equal(t2.cpu().numpy(), t1.cpu().numpy())
 
 
+@pytest.mark.parametrize("walk", ["pair", "phased1", "phased3", "phased4"])
 @pytest.mark.parametrize("integrator", [0, 1])
 @pytest.mark.parametrize("name", ["cornell", "spheres"])
-def test_two_lanes_per_ray_render_equals_one(gpu, name, integrator):
-    """Renders with two lanes per ray (single job, the frame's two halves, a batch engine)
+def test_walk_variants_render_equal_one_lane(gpu, name, integrator, walk):
+    """Renders with a walk variant (single job, the frame's two halves, a batch engine)
     equal the one-lane renders bit for bit."""
     sc = SCENES[name]()
     c = ctx()
     b = c.scene(sc).build_bvh(16)
     w, h, spp, depth = (96, 64, 8, 8) if name == "cornell" else (128, 72, 6, 12)
+    b.set_trace_tuning(**ONE_STEP)
     img1, s1 = b.render(w, h, spp, depth, integrator, 3)
-    b.set_trace_tuning(threads_per_ray=2)
+    b.set_trace_tuning(**WALKS[walk])
     img2, s2 = b.render(w, h, spp, depth, integrator, 3)
     assert bits_equal(img1, img2) and s1["queries"] == s2["queries"]
     big1, _ = b.render(1024, 512, 2, depth, integrator, 4, two_streams=True)
-    b.set_trace_tuning(threads_per_ray=1)
+    b.set_trace_tuning(**ONE_STEP)
     big0, _ = b.render(1024, 512, 2, depth, integrator, 4, two_streams=True)
     assert bits_equal(big1, big0)
     import petrichor_amd as P
     jobs = [P.Job(b, w, h, spp, depth, integrator, 5), P.Job(b, 40, 30, 3, depth, integrator, 6)]
     eng = P.Engine(c, 1 << 14, max_jobs=4)
-    eng.set_tuning(threads_per_ray=2)
+    eng.set_tuning(**WALKS[walk])
     eng.render_batch(jobs)
     for j in jobs:
         single, _ = b.render(j.width, j.height, j.spp, j.max_bounces, j.integrator, j.seed)
