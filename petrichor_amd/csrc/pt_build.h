@@ -60,5 +60,8 @@ hipError_t gpu_pack_bvh(const GpuBvhResult& r, int num_tris, const float* tri_po
                         int num_spheres, const int32_t* tri_mat_d, const int32_t* mat_shading_of_d,
                         const int32_t* smooth_d, const float* nrm_d, hipStream_t st, GpuPacked* out);
 void gpu_packed_free(GpuPacked* p);
+// Whether every child box of the num_inner binary nodes has lo <= hi on every axis (false
+// when a coordinate is NaN): the trace kernels' branch-free slab test relies on it.
+hipError_t gpu_boxes_ordered(const float4* nodes_d, int num_inner, hipStream_t st, bool* ordered);
 
 } // namespace pt

@@ -815,6 +815,16 @@ __global__ void k_pack_nodes(const BuildNode* nodes, int num_nodes, const int* p
     o[3] = make_float4(__uint_as_float(rl), __uint_as_float(rr), 0.0f, 0.0f);
 }
 
+__global__ void k_boxes_ordered(const float4* nodes, int num_inner, int* bad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num_inner) return;
+    // left box (a.xyz, (a.w, b.x, b.y)), right box ((b.z, b.w, c.x), c.yzw)
+    const float4 a = nodes[4 * (size_t)i], b = nodes[4 * (size_t)i + 1], c = nodes[4 * (size_t)i + 2];
+    const bool ok = a.x <= a.w && a.y <= b.x && a.z <= b.y && b.z <= c.y && b.w <= c.z && c.x <= c.w;
+    if (!ok) bad[0] = 1;
+}
+
 struct GatherArgs {
     const float* tri_pos;
     const float4* spheres; // primitives ntri.. are spheres (centre, radius)
@@ -1193,6 +1203,26 @@ void gpu_packed_free(GpuPacked* p)
                     (void*)p->mat_d, (void*)p->mat_sh_d, (void*)p->smooth_d, (void*)p->nrm_d, (void*)p->kinds_d})
         if (q) (void)hipFree(q);
     *p = GpuPacked();
+}
+
+hipError_t gpu_boxes_ordered(const float4* nodes_d, int num_inner, hipStream_t st, bool* ordered)
+{
+    *ordered = true;
+    if (num_inner <= 0) return hipSuccess;
+    int* bad_d = nullptr;
+    int bad = 0;
+    hipError_t e = hipMalloc((void**)&bad_d, sizeof(int));
+    if (e == hipSuccess) e = hipMemsetAsync(bad_d, 0, sizeof(int), st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_boxes_ordered, dim3((unsigned)((num_inner + 255) / 256)), dim3(256), 0, st, nodes_d,
+                           num_inner, bad_d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&bad, bad_d, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (bad_d) (void)hipFree(bad_d);
+    *ordered = e == hipSuccess && bad == 0;
+    return e;
 }
 
 } // namespace pt

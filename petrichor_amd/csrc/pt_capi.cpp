@@ -120,6 +120,16 @@ int add_parents(pt_bvh* b)
     return PT_OK;
 }
 
+// Sets BvhView::boxes_ordered from the packed binary nodes.
+static int check_boxes(pt_bvh* b)
+{
+    pt_context* ctx = b->scene->ctx;
+    bool ordered = false;
+    PT_HIP(ctx, gpu_boxes_ordered(b->nodes_d, b->num_inner, ctx->stream, &ordered));
+    b->view.boxes_ordered = ordered ? 1 : 0;
+    return PT_OK;
+}
+
 int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
 {
     pt_context* ctx = s->ctx;
@@ -175,7 +185,9 @@ int build_device(pt_scene* s, int bins, int leaf_max, pt_bvh** out)
     const int64_t nt = s->num_prims();
     b->device_bytes = (int64_t)pk.num_inner * 72 + (int64_t)pk.num_big * 8 + nt * 60 +
                       (pk.smooth_d ? nt * 4 : 0) + (pk.nrm_d ? nt * 36 : 0) + (pk.kinds_d ? nt : 0);
-    if (const int rc = add_parents(b)) {
+    int rc = check_boxes(b);
+    if (rc == PT_OK) rc = add_parents(b);
+    if (rc != PT_OK) {
         pt_bvh_destroy(b);
         return rc;
     }
@@ -640,7 +652,9 @@ static int bvh_build_impl(pt_scene* s, const pt_bvh_desc* desc, pt_bvh** out)
                       (int64_t)ranges.size() * 8 +
                       (int64_t)nt * 12 + (int64_t)(smooth.size() * 4 + nrm.size() * 4 + kinds.size());
     b->num_inner = (int)(nodes.size() / 4);
-    if (const int rc = add_parents(b)) {
+    int rc = check_boxes(b);
+    if (rc == PT_OK) rc = add_parents(b);
+    if (rc != PT_OK) {
         pt_bvh_destroy(b);
         return rc;
     }

@@ -786,6 +786,28 @@ PT_HD bool slab(V3 o, V3 inv, int sx, int sy, int sz, V3 lo, V3 hi, float* dist)
     return true;
 }
 
+// The same test without NaN: when lo <= hi on every axis, inv is finite and nonzero and o
+// is finite, no product is NaN; (lo - o) * inv and (hi - o) * inv are then ordered by the
+// sign of inv (rounding is monotone), so the near plane's distance is their minimum and
+// the far plane's their maximum, and the reference's sequence of axis checks and updates
+// reduces to max3 of the near distances against min3 of the far ones. Returns the same
+// hit flag and distance as slab().
+PT_HD bool slab_ordered(V3 o, V3 inv, V3 lo, V3 hi, float* dist)
+{
+    const float lx = (lo.x - o.x) * inv.x, hx = (hi.x - o.x) * inv.x;
+    const float ly = (lo.y - o.y) * inv.y, hy = (hi.y - o.y) * inv.y;
+    const float lz = (lo.z - o.z) * inv.z, hz = (hi.z - o.z) * inv.z;
+    const float tmin = fmaxf(fmaxf(fminf(lx, hx), fminf(ly, hy)), fminf(lz, hz));
+    const float tmax = fminf(fminf(fmaxf(lx, hx), fmaxf(ly, hy)), fmaxf(lz, hz));
+    float d = FLT_MAX;
+    const bool near_ok = (0.0f <= tmin && tmin < d);
+    if (near_ok) d = tmin;
+    const bool far_ok = (0.0f <= tmax && tmax < d);
+    if (far_ok) d = tmax;
+    *dist = d;
+    return tmin <= tmax && (near_ok || far_ok);
+}
+
 // Child / stack references of the traversal: bits 0-26 node index or first leaf slot,
 // bits 27-30 leaf size (0 = inner node, 15 = big leaf: bits 0-26 index the big-leaf
 // table), bit 31 = the ray origin is strictly inside the box (stack entries only).

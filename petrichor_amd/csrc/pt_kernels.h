@@ -35,6 +35,9 @@ struct BvhView {
     const int32_t* parents = nullptr;
     // per leaf slot: 1 for a sphere (its record holds centre and radius); null without spheres
     const int8_t* kinds = nullptr;
+    // 1 when every child box of the binary nodes has lo <= hi on every axis (no NaN): rays
+    // with a finite origin and finite, nonzero 1/dir then take the branch-free slab test
+    int boxes_ordered = 0;
 };
 
 struct SceneView {

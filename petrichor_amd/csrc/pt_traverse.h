@@ -68,6 +68,7 @@ struct TraceRay {
     float tnear; // max(dist_min, 0): where child boxes start to matter (8-wide traversal)
     int sx, sy, sz;
     bool any;
+    bool ordered; // slab_ordered applies: finite origin, finite nonzero inv, ordered boxes
 };
 
 struct TraceState {
@@ -122,6 +123,10 @@ __device__ __forceinline__ bool trace_begin(const BvhView& b, const float4 r0, c
     r.sx = r.inv.x < 0.0f;
     r.sy = r.inv.y < 0.0f;
     r.sz = r.inv.z < 0.0f;
+    const float big = __builtin_huge_valf();
+    r.ordered = b.boxes_ordered != 0 && fabsf(r.o.x) < big && fabsf(r.o.y) < big && fabsf(r.o.z) < big &&
+                fabsf(r.inv.x) < big && fabsf(r.inv.y) < big && fabsf(r.inv.z) < big && r.inv.x != 0.0f &&
+                r.inv.y != 0.0f && r.inv.z != 0.0f;
     s.best = -1;
     s.best_t = 0.0f;
     s.sp = 0;
@@ -303,8 +308,14 @@ __device__ __forceinline__ bool node_step(const BvhView& b, const TraceRay& r, T
     const float dl = sq_length(0.5f * (llo + lhi) - r.o);
     const float dr = sq_length(0.5f * (rlo + rhi) - r.o);
     float tl, tr_;
-    const bool hl = slab(r.o, r.inv, r.sx, r.sy, r.sz, llo, lhi, &tl);
-    const bool hr = slab(r.o, r.inv, r.sx, r.sy, r.sz, rlo, rhi, &tr_);
+    bool hl, hr;
+    if (__ballot(!r.ordered) == 0ull) { // every ray stepping here: the branch-free test
+        hl = slab_ordered(r.o, r.inv, llo, lhi, &tl);
+        hr = slab_ordered(r.o, r.inv, rlo, rhi, &tr_);
+    } else {
+        hl = slab(r.o, r.inv, r.sx, r.sy, r.sz, llo, lhi, &tl);
+        hr = slab(r.o, r.inv, r.sx, r.sy, r.sz, rlo, rhi, &tr_);
+    }
     const uint32_t el = ml | (box_contains(llo, lhi, r.o) ? PT_REF_INSIDE : 0u);
     const uint32_t er = mr | (box_contains(rlo, rhi, r.o) ? PT_REF_INSIDE : 0u);
     // the child the reference pops first: left if dl < dr, else right

@@ -67,6 +67,46 @@ def test_trace_matches_oracle(gpu, name):
     assert np.array_equal(p2.cpu().numpy(), p2o)
 
 
+def zero_component_rays(sc, n, seed):
+    """Rays whose directions have zero (and negative-zero) components, with origins on the
+    planes of the scene's box: 1/dir is infinite there, so (plane - origin) * 1/dir can be
+    NaN, which only the reference's sequence of slab checks handles (the branch-free test
+    is for rays with a finite, nonzero 1/dir). Ordinary rays are mixed in, every wave
+    of 64 rays holds both kinds, and the first 4096 rays are zero-component rays only."""
+    r = rays_into(sc, n, seed)
+    t = sc["tri_pos"].reshape(-1, 3)
+    lo = t.min(0)
+    i = np.arange(n)
+    r[i % 5 == 0, 4] = 0.0                       # no x component
+    r[i % 7 == 0, 5] = -0.0                      # 1/dir.y = -inf
+    axis = i % 11 == 0                           # along one axis
+    r[axis, 4:7] = 0.0
+    r[axis, 4 + (i[axis] // 11) % 3] = np.where(i[axis] % 2 == 0, 1.0, -1.0)
+    on_plane = i % 13 == 0                       # origin on the box's lower planes
+    r[on_plane, 0:3] = np.where(r[on_plane, 4:7] == 0.0, lo, r[on_plane, 0:3])
+    r[:4096, 4] = 0.0
+    return r
+
+
+@pytest.mark.parametrize("walk", ["one_step", "phased"])
+@pytest.mark.parametrize("name", list(SCENES))
+def test_zero_direction_components_match_oracle(gpu, name, walk):
+    """Rays with zero direction components, alone in a wave and mixed with ordinary rays:
+    hits, distances and the node and triangle counts equal the oracle's walk."""
+    log("zero direction components", name, walk)
+    sc = SCENES[name]()
+    b = ctx().scene(sc).build_bvh(16)
+    b.set_trace_tuning(leaf_batch=-1 if walk == "one_step" else 3)
+    o = O.OracleScene(sc).use_tree(b.export_nodes() | {"prim_ids": b.export_order()})
+    r = zero_component_rays(sc, 20000, 9)
+    tg, pg, st = b.trace_stats(to_dev(r))
+    to, po, so = o.trace_closest(r)
+    assert (po >= 0).sum() > 1000
+    assert np.array_equal(pg.cpu().numpy(), po) and bits_equal(tg.cpu().numpy(), to)
+    assert np.array_equal(st, so)
+    assert np.array_equal(b.trace_any(to_dev(r)).cpu().numpy(), (po >= 0).astype(np.int32))
+
+
 def render_pair(sc, w, h, spp, depth, integrator, seed=1):
     c = ctx()
     s = c.scene(sc)
