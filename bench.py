@@ -642,6 +642,7 @@ def main():
                     help="frame mode: also render the frame through the 4- and 8-wide trees")
     ap.add_argument("--separate-launches", action="store_true",
                     help="frame mode: also a frame with every launch on its own (per-kernel split)")
+    ap.add_argument("--no-three", action="store_true", help="frame mode: skip the three-stream frames")
     ap.add_argument("--no-batch", action="store_true", help="frame mode: skip the batched form")
     ap.add_argument("--no-serve", action="store_true", help="frame mode: skip the served form")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
@@ -767,6 +768,25 @@ def main():
                         trace_stats=True)
     torch.cuda.synchronize()
     assert torch.equal(logged, out), "the counted frame must match the production frames"
+    # The same frame in three parts on three streams (PT_RENDER_THREE_STREAMS): reported
+    # beside the headline. Its trace launches overlap each other, so their spans no
+    # longer time the kernel, and the roofline stays on the two-stream schedule.
+    three = None
+    if not args.no_three:
+        bvh.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=logged, three_streams=True)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        q3 = 0
+        for i in range(2):
+            _, st3 = bvh.render(W, H, SPP, D, P.INTEGRATOR_MIS, 1, out=logged, three_streams=True)
+            q3 += st3["queries"]
+        torch.cuda.synchronize()
+        w3 = time.perf_counter() - t3
+        three = {"mrays_s": round(q3 / w3 / 1e6, 2), "ms_per_frame": round(w3 / 2 * 1e3, 2), "frames": 2,
+                 "image_bit_identical": bool(torch.equal(logged, out)),
+                 "render": "pt_render with PT_RENDER_TWO_STREAMS | PT_RENDER_THREE_STREAMS (three parts of the "
+                           "pixels on three streams)"}
+        log("three streams:", three)
     L, C = lst["log"], cst["log"]
     log("logged frame:", L)
     assert C["trace_rays"] == L["trace_rays"] and C["trace_launches"] == L["trace_launches"]
@@ -899,6 +919,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
+        "three_streams": three,
         "batched": batched,
         "served": served,
     }

@@ -150,6 +150,9 @@ typedef struct pt_render_desc {
  * of pt_render_stats (same image). Its only extra work is one single-thread kernel after
  * each trace launch. */
 #define PT_RENDER_LAUNCH_LOG 16
+/* With PT_RENDER_TWO_STREAMS: three parts of the pixels on three streams instead of two
+ * (same image). The process then needs three hardware queues for them. */
+#define PT_RENDER_THREE_STREAMS 32
 
 /* Per-kernel figures: index 0 extension (closest hit), 1 environment shadow rays (any
  * hit), 2 area-light shadow rays (closest hit), 3 shading. Times are filled only with

@@ -86,6 +86,7 @@ RENDER_PLAIN_LAUNCHES = 2
 RENDER_TRACE_STATS = 4
 RENDER_TWO_STREAMS = 8
 RENDER_LAUNCH_LOG = 16
+RENDER_THREE_STREAMS = 32
 MAX_SAMPLER_LENGTH = 65536  # PT_MAX_SAMPLER_LENGTH
 AOV_DENOISING_ALBEDO, AOV_DENOISING_NORMAL, AOV_UV = 0, 1, 2
 
@@ -574,17 +575,19 @@ class Bvh:
 
     def render(self, width, height, spp, max_bounces, integrator=INTEGRATOR_MIS, seed=1, out=None,
                time_kernels=False, plain=False, trace_stats=False, two_streams=False, launch_log=False,
-               sampler_length=0):
+               sampler_length=0, three_streams=False):
         """Renders one image; returns (rgb [h, w, 3] float32 numpy array or the given device
         tensor, stats dict). time_kernels: every launch timed on its own; plain: plain
         launches with one synchronisation per iteration; two_streams: a large image in two
-        halves on two streams; launch_log: the production schedule with every trace launch
+        halves on two streams (three_streams: three parts on three streams); launch_log: the
+        production schedule with every trace launch
         logged (with trace_stats: and its work counted) (all give the same image).
         sampler_length > 0: the 2D sampler is MicroJitteredSampler over Halton sequences of
         that length instead of RandomSampler2D."""
         flags = (RENDER_TIME_KERNELS if (time_kernels or trace_stats) and not launch_log else 0) | \
             (RENDER_PLAIN_LAUNCHES if plain else 0) | (RENDER_TRACE_STATS if trace_stats else 0) | \
-            (RENDER_TWO_STREAMS if two_streams else 0) | (RENDER_LAUNCH_LOG if launch_log else 0)
+            (RENDER_TWO_STREAMS if two_streams or three_streams else 0) | \
+            (RENDER_THREE_STREAMS if three_streams else 0) | (RENDER_LAUNCH_LOG if launch_log else 0)
         d = RenderDesc(width, height, spp, max_bounces, integrator, seed, flags, sampler_length)
         st = RenderStats()
         if out is None:

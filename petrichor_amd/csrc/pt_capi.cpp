@@ -333,7 +333,8 @@ int pt_ctx_destroy(pt_context* ctx)
 {
     if (!ctx) return PT_ERR_ARG;
     if (ctx->engine) pt_engine_destroy(ctx->engine);
-    if (ctx->engine2) pt_engine_destroy(ctx->engine2);
+    for (pt_engine* e : ctx->engine2)
+        if (e) pt_engine_destroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PT_OK;

@@ -42,6 +42,9 @@ static constexpr int64_t kHalvesMinPixels = (int64_t)1 << 19;
 #define PT_SINGLE_JOB_SLOTS ((int64_t)1 << 23)
 #endif
 static constexpr int64_t kSingleJobSlots = PT_SINGLE_JOB_SLOTS;
+// Most parts (engines and streams) of an image rendered with PT_RENDER_TWO_STREAMS: two,
+// or three with PT_RENDER_THREE_STREAMS.
+static constexpr int kRenderParts = 3;
 
 namespace pt {
 
@@ -324,22 +327,24 @@ int run_jobs_solo(pt_engine& E, pt_engine& S, std::vector<EngineJob>& jobs, int6
 // launches the GPU overlaps: one half's kernels fill the CUs that the other's leave idle
 // at the end of each launch (the trace launches wait for their slowest rays). Each half is
 // the same job restricted to a pixel range, so every pixel is computed as in one piece.
-int run_halves(pt_context* ctx, pt_engine* E[2], const pt_job_desc& jd, int iters_per_sync, pt_batch_stats* stats)
+int run_halves(pt_context* ctx, pt_engine* const* E, int n, const pt_job_desc& jd, int iters_per_sync,
+               pt_batch_stats* stats)
 {
     EngineJob full = make_job(jd, 0);
     int rc = prepare_output(ctx, full);
     if (rc != PT_OK) return rc;
-    EngineJob h[2] = {full, full};
-    const int64_t half = full.npix / 2;
-    h[0].npix = half;
-    h[1].npix = full.npix - half;
-    h[1].pixel_base = half;
-    h[0].own_out = h[1].own_out = false; // the image belongs to `full`
+    EngineJob h[kRenderParts];
+    for (int i = 0; i < n; ++i) {
+        h[i] = full;
+        h[i].pixel_base = full.npix * i / n;
+        h[i].npix = full.npix * (i + 1) / n - h[i].pixel_base;
+        h[i].own_out = false; // the image belongs to `full`
+    }
     // the engines' streams must not start before the output exists (ctx stream work done)
     PT_HIP(ctx, hipDeviceSynchronize());
     const auto t0 = std::chrono::steady_clock::now();
-    bool done[2] = {false, false};
-    for (int i = 0; i < 2; ++i) {
+    bool done[kRenderParts] = {};
+    for (int i = 0; i < n; ++i) {
         E[i]->iterations = 0;
         PT_HIP(ctx, hipMemsetAsync(E[i]->queries_d, 0, sizeof(unsigned long long), E[i]->st));
         h[i].items = h[i].npix * h[i].desc.spp;
@@ -348,19 +353,29 @@ int run_halves(pt_context* ctx, pt_engine* E[2], const pt_job_desc& jd, int iter
         if (rc != PT_OK) return rc;
         E[i]->live_h = (int)h[i].slots;
     }
-    while (!done[0] || !done[1]) {
-        // both halves end together: their completion launches start in the same round, once
-        // every sample of both is handed out and the live paths of both fit one completion
-        // launch
-        const int live = (done[0] ? 0 : E[0]->live_h) + (done[1] ? 0 : E[1]->live_h);
-        const bool exhausted = (done[0] || E[0]->job_exhausted(h[0])) && (done[1] || E[1]->job_exhausted(h[1]));
+    auto all_done = [&] {
+        for (int i = 0; i < n; ++i)
+            if (!done[i]) return false;
+        return true;
+    };
+    while (!all_done()) {
+        // the parts end together: their completion launches start in the same round, once
+        // every sample of every part is handed out and the live paths of all fit one
+        // completion launch
+        int live = 0;
+        bool exhausted = true;
+        for (int i = 0; i < n; ++i) {
+            if (done[i]) continue;
+            live += E[i]->live_h;
+            exhausted = exhausted && E[i]->job_exhausted(h[i]);
+        }
         const bool last = exhausted && live > 0 && live <= E[0]->finish_threshold;
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < n; ++i) {
             if (done[i]) continue;
             rc = last ? E[i]->launch_finish() : E[i]->launch(iters_per_sync);
             if (rc != PT_OK) return rc;
         }
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < n; ++i) {
             if (done[i]) continue;
             rc = E[i]->wait();
             if (rc != PT_OK) return rc;
@@ -373,15 +388,19 @@ int run_halves(pt_context* ctx, pt_engine* E[2], const pt_job_desc& jd, int iter
             }
         }
     }
-    for (int i = 0; i < 2; ++i) PT_HIP(ctx, hipStreamSynchronize(E[i]->st));
+    for (int i = 0; i < n; ++i) PT_HIP(ctx, hipStreamSynchronize(E[i]->st));
     const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    for (int i = 0; i < 2; ++i) E[i]->retire(h[i]);
+    for (int i = 0; i < n; ++i) E[i]->retire(h[i]);
     rc = finish_output(ctx, full, E[0]->st);
     if (rc != PT_OK) return rc;
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
-        stats->queries = E[0]->queries() + E[1]->queries();
-        stats->iterations = std::max(E[0]->iterations, E[1]->iterations);
+        stats->queries = 0;
+        stats->iterations = 0;
+        for (int i = 0; i < n; ++i) {
+            stats->queries += E[i]->queries();
+            stats->iterations = std::max(stats->iterations, E[i]->iterations);
+        }
         stats->elapsed_ms = wall;
         stats->wall_ms = wall;
         stats->syncs = 0;
@@ -483,10 +502,13 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
     // launches overlap.
     if ((d->flags & PT_RENDER_TWO_STREAMS) && npix >= kHalvesMinPixels &&
         !(d->flags & (PT_RENDER_PLAIN_LAUNCHES | PT_RENDER_TIME_KERNELS))) {
-        const int64_t cap = slots_for(npix - npix / 2);
-        pt_engine** slots[2] = {&ctx->engine, &ctx->engine2};
-        pt_engine* E2[2];
-        for (int i = 0; i < 2; ++i) {
+        const int n = (d->flags & PT_RENDER_THREE_STREAMS) ? 3 : 2;
+        const int64_t cap = slots_for(npix - npix * (n - 1) / n);
+        pt_engine** slots[kRenderParts];
+        pt_engine* E2[kRenderParts];
+        slots[0] = &ctx->engine;
+        for (int i = 1; i < n; ++i) slots[i] = &ctx->engine2[i - 1];
+        for (int i = 0; i < n; ++i) {
             if (!*slots[i] || (*slots[i])->capacity < cap) {
                 delete *slots[i];
                 *slots[i] = nullptr;
@@ -498,13 +520,13 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
             E2[i] = *slots[i];
         }
         const bool log = (d->flags & PT_RENDER_LAUNCH_LOG) != 0;
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < n; ++i) {
             E2[i]->use_trace_tuning(b->trace_tune);
             rc = E2[i]->set_logging(log, (d->flags & PT_RENDER_TRACE_STATS) != 0);
             if (rc != PT_OK) return rc;
         }
         pt_batch_stats bs;
-        rc = run_halves(ctx, E2, jd, E2[0]->iters_per_sync, &bs);
+        rc = run_halves(ctx, E2, n, jd, E2[0]->iters_per_sync, &bs);
         if (rc != PT_OK) return rc;
         if (stats) {
             std::memset(stats, 0, sizeof *stats);
@@ -513,7 +535,7 @@ int pt_render(pt_scene* s, pt_bvh* b, const pt_render_desc* d, float* rgb_out, i
             stats->elapsed_ms = bs.elapsed_ms;
         }
         if (log) {
-            for (int i = 0; i < 2; ++i) {
+            for (int i = 0; i < n; ++i) {
                 if (stats) rc = E2[i]->read_log(stats);
                 if (rc == PT_OK) rc = E2[i]->set_logging(false, false);
                 if (rc != PT_OK) return rc;

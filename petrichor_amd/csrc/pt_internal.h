@@ -15,7 +15,7 @@ struct pt_context {
     hipStream_t stream = nullptr;
     std::string err;
     pt_engine* engine = nullptr; // single-job engine of pt_render, grown on demand
-    pt_engine* engine2 = nullptr; // the second one, for a large image rendered in two halves
+    pt_engine* engine2[2] = {nullptr, nullptr}; // the other parts' engines of a large image rendered in parts
 };
 
 struct pt_scene {

@@ -107,6 +107,20 @@ def test_zero_direction_components_match_oracle(gpu, name, walk):
     assert np.array_equal(b.trace_any(to_dev(r)).cpu().numpy(), (po >= 0).astype(np.int32))
 
 
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_three_streams_equal_two_and_one_piece(gpu, integrator):
+    """PT_RENDER_THREE_STREAMS (three parts of the pixels on three streams) renders the
+    image of the two-stream and the one-piece schedules bit for bit, twice in a row."""
+    sc = SCENES["spheres"]()
+    b = ctx().scene(sc).build_bvh(16)
+    one, s1 = b.render(1024, 512, 3, 12, integrator, 4)
+    two, s2 = b.render(1024, 512, 3, 12, integrator, 4, two_streams=True)
+    for _ in range(2):
+        three, s3 = b.render(1024, 512, 3, 12, integrator, 4, three_streams=True)
+        assert bits_equal(three, two) and bits_equal(three, one)
+        assert s3["queries"] == s2["queries"] == s1["queries"]
+
+
 def render_pair(sc, w, h, spp, depth, integrator, seed=1):
     c = ctx()
     s = c.scene(sc)
