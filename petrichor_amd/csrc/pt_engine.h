@@ -510,7 +510,10 @@ struct pt_engine {
             drop_graphs();
         }
         if (clear_samples) PT_HIP(ctx, pt::launch_samples_clear(v, st));
-        PT_HIP(ctx, pt::launch_paths_init(jd.sc, v, ps, queue(set, 0), act[set], counters + 4 * set + 3, st));
+        // counters[8..9]: the reservation of the job's first entries (free words of the
+        // counters' first 256-byte line)
+        PT_HIP(ctx, pt::launch_paths_init(jd.sc, v, ps, queue(set, 0), act[set], counters + 4 * set + 3, counters + 8,
+                                          st));
         j.started = std::chrono::steady_clock::now();
         return PT_OK;
     }

@@ -216,7 +216,7 @@ hipError_t launch_preview(const JobView& job, const PathView& ps, float* out, un
 // sets the job's item cursor and sample count, and appends the paths to the live list and
 // their camera rays to the extension queue.
 hipError_t launch_paths_init(const SceneView& sc, const JobView& job, const PathView& ps, const Queue& ext,
-                             int* active, int* active_count, hipStream_t st);
+                             int* active, int* active_count, int* base, hipStream_t st);
 // The image of a job whose samples are all done: each pixel is the sum of its samples in
 // sample order, divided by spp (PathTracing.cpp:194-198).
 hipError_t launch_resolve(const JobView& job, hipStream_t st);

@@ -311,11 +311,24 @@ __device__ __forceinline__ Ray start_sample(const SceneView& sc, const JobView& 
 __device__ __forceinline__ float camera_dmin(int integrator) { return integrator == 0 ? 0.0f : PT_EPS; }
 __device__ __forceinline__ float camera_dmax(int integrator) { return integrator == 0 ? FLT_MAX : PT_INFINITY; }
 
+// Reserves a job's first n paths in the live list and its camera rays in the extension
+// queue: base[0] and base[1] receive the two counts before they grow by n.
+__global__ void k_reserve_init(int* active_count, int* ext_count, int n, int* base)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    base[0] = *active_count;
+    *active_count += n;
+    base[1] = *ext_count;
+    *ext_count += n;
+}
+
 // One-time set-up of every path slot of a job with the job's first items; the path joins
-// the live list and its camera ray the extension queue. Thread 0 sets the job's cursor
-// past the items handed out here and zeroes its sample count.
+// the live list and its camera ray the extension queue, at the entries k_reserve_init
+// reserved (the live slots are a prefix, so slot i takes entry base + i, without atomics
+// on the two counts). Thread 0 sets the job's cursor past the items handed out here and
+// zeroes its sample count.
 __global__ void __launch_bounds__(256) k_paths_init(SceneView sc, JobView job, PathView ps, Queue ext, int* active,
-                                                    int* active_count)
+                                                    const int* base)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < job.num_slots && (unsigned long long)i < job.items;
@@ -324,12 +337,10 @@ __global__ void __launch_bounds__(256) k_paths_init(SceneView sc, JobView job, P
         *job.cursor = (unsigned long long)job.num_slots < job.items ? (unsigned long long)job.num_slots : job.items;
         *job.done = 0ull;
     }
-    Ray r;
-    if (live) r = start_sample(sc, job, ps, p, (unsigned long long)i);
-    const int a = queue_slot(active_count, live);
-    if (live) active[a] = p;
-    const int slot = queue_slot(ext.count, live);
-    if (live) push_ray(ext, slot, p, r.o, camera_dmin(job.integrator), r.dir, camera_dmax(job.integrator));
+    if (!live) return;
+    const Ray r = start_sample(sc, job, ps, p, (unsigned long long)i);
+    active[base[0] + i] = p;
+    push_ray(ext, base[1] + i, p, r.o, camera_dmin(job.integrator), r.dir, camera_dmax(job.integrator));
 }
 
 // The shading stage of the MIS integrator (PathTracing::Render, PathTracing.cpp:20-200,
@@ -961,12 +972,14 @@ hipError_t launch_copy_u64(const unsigned long long* src, unsigned long long* ds
 }
 
 hipError_t launch_paths_init(const SceneView& sc, const JobView& job, const PathView& ps, const Queue& ext,
-                             int* active, int* active_count, hipStream_t st)
+                             int* active, int* active_count, int* base, hipStream_t st)
 {
     const int64_t n = job.num_slots;
     if (n <= 0) return hipSuccess;
+    const int live = (int)((unsigned long long)n < job.items ? (unsigned long long)n : job.items);
+    hipLaunchKernelGGL(k_reserve_init, dim3(1), dim3(64), 0, st, active_count, ext.count, live, base);
     hipLaunchKernelGGL(k_paths_init, dim3(grid_for(n, 256)), dim3(256), 0, st, sc, job, ps, ext, active,
-                       active_count);
+                       (const int*)base);
     return hipGetLastError();
 }
 
