@@ -668,6 +668,8 @@ def main():
                          "engine beside the shared pool (0: none)")
     ap.add_argument("--threads-per-ray", type=int, default=1, choices=[1, 2],
                     help="frame and trace modes: lanes per ray of the binary walk (pt_bvh_set_trace_tuning)")
+    ap.add_argument("--trace-chunk", type=int, default=64,
+                    help="frame mode: rays a trace wave takes from its part's cursor at a time (64..4096)")
     ap.add_argument("--trace-blocks-per-cu", type=int, default=0, choices=range(0, 9),
                     help="frame mode: persistent trace blocks per CU (0: as many as the LDS stacks allow)")
     ap.add_argument("--leaf-batch", type=int, default=0, choices=[-1, 0, 1, 2, 3, 4],
@@ -731,9 +733,9 @@ def main():
     s = ctx.scene(sc)
     width = args.bvh_width or 2
     bvh = s.build_bvh(16, width=width)
-    if args.threads_per_ray != 1 or args.leaf_batch or args.trace_blocks_per_cu:
-        bvh.set_trace_tuning(trace_blocks_per_cu=args.trace_blocks_per_cu, threads_per_ray=args.threads_per_ray,
-                             leaf_batch=args.leaf_batch)
+    if args.threads_per_ray != 1 or args.leaf_batch or args.trace_blocks_per_cu or args.trace_chunk != 64:
+        bvh.set_trace_tuning(trace_blocks_per_cu=args.trace_blocks_per_cu, trace_chunk=args.trace_chunk,
+                             threads_per_ray=args.threads_per_ray, leaf_batch=args.leaf_batch)
     info = bvh.info()
     log("bvh:", info)
     W, H, SPP, D = args.width, args.height, args.spp, args.depth
@@ -911,7 +913,7 @@ def main():
                           ("collapsed to %d-wide nodes" % width if width != 2
                            else "binary (reference topology and walk order)"),
                    "threads_per_ray": args.threads_per_ray, "leaf_batch": args.leaf_batch or 3,
-                   "trace_blocks_per_cu": args.trace_blocks_per_cu or 8,
+                   "trace_blocks_per_cu": args.trace_blocks_per_cu or 8, "trace_chunk": args.trace_chunk,
                    "render": "pt_render with PT_RENDER_TWO_STREAMS (two halves of the pixels on two streams)"},
         "queries_per_step": int(queries / args.steps),
         "iterations_per_step": int(stats[-1]["iterations"]),
