@@ -570,10 +570,11 @@ struct pt_engine {
         tj.job_of_path = multi ? ps.job : nullptr;
         tj.queries = queries_d;
         log_job(tj);
-        PT_HIP(ctx, pt::launch_zero(counters + PT_CURSOR_STRIDE, PT_TRACE_PARTS, st, PT_CURSOR_STRIDE));
+        // set w's counts are read by nothing before shading appends to them
+        PT_HIP(ctx, pt::launch_zero_iteration(counters + PT_CURSOR_STRIDE, PT_TRACE_PARTS, PT_CURSOR_STRIDE,
+                                              counters + 4 * w, st));
         PT_HIP(ctx, pt::launch_trace_job(sc1.bvh, tj, counters + PT_CURSOR_STRIDE, nullptr, st, tune));
         if (logging) PT_HIP(ctx, pt::launch_log_tick(counters + 4 * r, log_d, log_n_d, st));
-        PT_HIP(ctx, pt::launch_zero(counters + 4 * w, 4, st));
         pt::ShadeArgs a;
         a.ps = ps;
         a.active_in = act[r];

@@ -194,6 +194,8 @@ hipError_t launch_pack_rays_soa(int64_t n, const float* ox, const float* oy, con
 hipError_t launch_hits_soa(int64_t n, const BvhView& b, const float4* rays, float* t, int32_t* prim, float* u, float* v,
                            hipStream_t st);
 // *p = value, by a kernel (no host copy).
+// Zeroes n trace cursors (at the given stride) and the four queue counts at `counts`.
+hipError_t launch_zero_iteration(int* cursors, int n, int stride, int* counts, hipStream_t st);
 hipError_t launch_set(int* p, int value, hipStream_t st);
 // After a logged trace launch: log[*index].items = the three segment counts, ++*index
 // (while *index < PT_LAUNCH_LOG_CAP - 1).

@@ -993,6 +993,15 @@ __global__ void k_zero_counters(int* c, int n, int stride)
     if (i < n) c[i * stride] = 0;
 }
 
+// One launch for an iteration's two resets: the trace cursors (n at the given stride) and
+// the four counts of the queue set that shading writes (the trace only reads the other set).
+__global__ void k_zero_iteration(int* cursors, int n, int stride, int* counts)
+{
+    const int i = threadIdx.x;
+    if (i < n) cursors[i * stride] = 0;
+    if (i >= 32 && i < 36) counts[i - 32] = 0;
+}
+
 __global__ void k_set_int(int* p, int v)
 {
     if (threadIdx.x == 0) *p = v;
@@ -1020,6 +1029,13 @@ static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + b
 hipError_t launch_zero(int* counters, int n, hipStream_t st, int stride)
 {
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, counters, n, stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_iteration(int* cursors, int n, int stride, int* counts, hipStream_t st)
+{
+    if (n < 0 || n > 32) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zero_iteration, dim3(1), dim3(64), 0, st, cursors, n, stride, counts);
     return hipGetLastError();
 }
 
